@@ -1,0 +1,509 @@
+// assemble.hip -- GenMatRhs on gfx950: heat (Q2) and coupled Navier-Stokes
+// (Q2-Q1 Taylor-Hood) element matrices scattered into CSR.
+//
+// Volume kernels: one workgroup per element (grid-stride over the elements of
+// one colour).  Reference tables (phi, dphi/dxi, weights) are staged in LDS
+// once per workgroup; per element the 27 (9) Gauss-point threads build the
+// Jacobian, its inverse, JxW and the global gradients (MGFE::Jac3D/Jac2D,
+// get_dphi_gl_g: src/MGFE.C:1066-1166, 1496-1519) into LDS, then all threads
+// accumulate element-matrix entries over the Gauss points and add them to A
+// through the precomputed element->CSR position map.  Elements of one colour
+// share no node, so no atomics are needed and results are deterministic.
+// Boundary faces (penalty Dirichlet, Robin/Neumann) run in a second kernel
+// over the boundary elements of the same colour, after the volume kernel.
+//
+// Heat:  template_appl/nse/SRC/MGSolverT.C:228-343 (volume), :366-507 (faces)
+// NS:    template_appl/nse/SRC/MGSolverNS.C:285-495 (volume), :503-690 (faces)
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "fm_kernels.h"
+
+__constant__ int c_face2d[4][3] = {{0,1,4},{1,2,5},{2,3,6},{3,0,7}};
+__constant__ int c_face3d[6][9] = {
+  {0,1,2,3,8,9,10,11,20},{0,1,5,4,8,13,16,12,21},{1,2,6,5,9,14,17,13,22},
+  {2,3,7,6,10,15,18,14,23},{3,0,4,7,11,12,19,15,24},{4,5,6,7,16,17,18,19,25}};
+
+template <int DIM> __device__ inline int face_node(int s, int k) { return DIM == 2 ? c_face2d[s][k] : c_face3d[s][k]; }
+
+// Jacobian at Gauss point qp: returns det, fills IJ (MGFE::Jac2D / Jac3D)
+template <int DIM, int N2, int NG>
+__device__ inline double jac_qp(int qp, const double *xx, const double *dref, double *IJ) {
+  const int off = N2 * NG;
+  if (DIM == 2) {
+    double x_xi = 0., x_eta = 0., y_xi = 0., y_eta = 0.;
+    for (int s = 0; s < N2; s++) {
+      int sng = s * NG + qp;
+      double a = dref[sng], b = dref[sng + off];
+      x_xi += xx[s] * a; x_eta += xx[s] * b; y_xi += xx[s + N2] * a; y_eta += xx[s + N2] * b;
+    }
+    double det = x_xi * y_eta - y_xi * x_eta, idet = 1. / det;
+    IJ[0] = y_eta * idet; IJ[1] = -y_xi * idet; IJ[2] = -x_eta * idet; IJ[3] = x_xi * idet;
+    return det;
+  } else {
+    double x_xi = 0., x_eta = 0., x_zeta = 0., y_xi = 0., y_eta = 0., y_zeta = 0., z_xi = 0., z_eta = 0., z_zeta = 0.;
+    for (int s = 0; s < N2; s++) {
+      int sng = s * NG + qp;
+      double a = dref[sng], b = dref[sng + off], c = dref[sng + 2 * off];
+      x_xi += xx[s] * a; x_eta += xx[s] * b; x_zeta += xx[s] * c;
+      y_xi += xx[s + N2] * a; y_eta += xx[s + N2] * b; y_zeta += xx[s + N2] * c;
+      z_xi += xx[s + 2 * N2] * a; z_eta += xx[s + 2 * N2] * b; z_zeta += xx[s + 2 * N2] * c;
+    }
+    double det = x_xi * (y_eta * z_zeta - y_zeta * z_eta) - x_eta * (y_xi * z_zeta - y_zeta * z_xi) +
+                 x_zeta * (y_xi * z_eta - y_eta * z_xi);
+    double idet = 1. / det;
+    IJ[0] = (y_eta * z_zeta - y_zeta * z_eta) * idet;
+    IJ[3] = -(x_eta * z_zeta - x_zeta * z_eta) * idet;
+    IJ[6] = (y_zeta * x_eta - x_zeta * y_eta) * idet;
+    IJ[1] = -(y_xi * z_zeta - y_zeta * z_xi) * idet;
+    IJ[4] = (x_xi * z_zeta - x_zeta * z_xi) * idet;
+    IJ[7] = -(y_zeta * x_xi - y_xi * x_zeta) * idet;
+    IJ[2] = (z_eta * y_xi - y_eta * z_xi) * idet;
+    IJ[5] = -(x_xi * z_eta - x_eta * z_xi) * idet;
+    IJ[8] = (x_xi * y_eta - y_xi * x_eta) * idet;
+    return det;
+  }
+}
+
+// surface Jacobian determinant on a face (MGFE::JacSur2D / JacSur3D); xb[k + d*NB]
+template <int DIM, int NB, int NGB>
+__device__ inline double jac_sur(int qp, const double *xb, const FeTab *tab) {
+  if (DIM == 2) {
+    double dx = 0., dy = 0.;
+    for (int s = 0; s < NB; s++) { double d = tab->bdphi2[s * NGB + qp]; dx += xb[s] * d; dy += xb[s + NB] * d; }
+    return sqrt(dx * dx + dy * dy);
+  } else {
+    const int off = NB * NGB;
+    double dxdxi = 0., dxdeta = 0., dydxi = 0., dydeta = 0., dzdxi = 0., dzdeta = 0.;
+    for (int s = 0; s < NB; s++) {
+      int sng = s * NGB + qp;
+      double a = tab->bdphi2[sng], b = tab->bdphi2[sng + off];
+      dxdxi += xb[s] * a; dxdeta += xb[s] * b; dydxi += xb[s + NB] * a; dydeta += xb[s + NB] * b;
+      dzdxi += xb[s + 2 * NB] * a; dzdeta += xb[s + 2 * NB] * b;
+    }
+    double c1 = dxdxi * dydeta - dxdeta * dydxi, c2 = dydxi * dzdeta - dydeta * dzdxi, c3 = dzdxi * dxdeta - dzdeta * dxdxi;
+    return sqrt(c1 * c1 + c2 * c2 + c3 * c3);
+  }
+}
+
+// ===========================================================================
+// Heat volume kernel
+// ===========================================================================
+template <int DIM>
+__global__ __launch_bounds__(256) void k_heat_vol(int nel, const int *__restrict__ elems, const int *__restrict__ conn,
+                                                  const int *__restrict__ eldof, const int *__restrict__ elpos,
+                                                  const double *__restrict__ xyz, int Ntop, const int *__restrict__ dof_top,
+                                                  const int *__restrict__ bc0, const double *__restrict__ x_old,
+                                                  const double *__restrict__ coef, const double *__restrict__ vel,
+                                                  const FeTab *__restrict__ tab, HeatArgs a, int mode,
+                                                  double *__restrict__ val, double *__restrict__ rhs) {
+  constexpr int N2 = DIM == 2 ? 9 : 27, NG = N2;
+  __shared__ double s_phi[N2 * NG];
+  __shared__ double s_dref[DIM * N2 * NG];
+  __shared__ double s_dg[NG * N2 * DIM];   // [qp][s][d]
+  __shared__ double s_xx[DIM * N2], s_Told[N2], s_veln[DIM * N2];
+  __shared__ double s_JxW[NG], s_Tg[NG], s_vel[NG * DIM], s_cl[NG * DIM];
+  __shared__ int s_bcv[N2];
+  const int tid = threadIdx.x;
+  for (int t = tid; t < N2 * NG; t += blockDim.x) s_phi[t] = tab->phi2[t];
+  for (int t = tid; t < DIM * N2 * NG; t += blockDim.x) s_dref[t] = tab->dphi2[t];
+  for (int ei = blockIdx.x; ei < nel; ei += gridDim.x) {
+    const int e = elems[ei];
+    __syncthreads();
+    if (tid < N2) {
+      int node = conn[(int64_t)e * N2 + tid];
+      int kd = dof_top[node];
+      for (int d = 0; d < DIM; d++) {
+        s_xx[tid + d * N2] = xyz[node + (int64_t)d * Ntop];
+        s_veln[tid + d * N2] = vel ? vel[node + (int64_t)d * Ntop] : a.vel_const[d];
+      }
+      s_Told[tid] = x_old[kd];
+      s_bcv[tid] = bc0[kd];
+    }
+    __syncthreads();
+    const double ke = coef ? coef[e] : 1.0;
+    if (tid < NG) {
+      const int qp = tid;
+      double IJ[DIM * DIM];
+      double det = jac_qp<DIM, N2, NG>(qp, s_xx, s_dref, IJ);
+      s_JxW[qp] = det * tab->w[qp];
+      double Tg = 0., vg[DIM];
+      for (int d = 0; d < DIM; d++) vg[d] = 0.;
+      for (int s = 0; s < N2; s++) {
+        double ph = s_phi[s * NG + qp];
+        Tg += ph * s_Told[s];
+        for (int d = 0; d < DIM; d++) vg[d] += ph * s_veln[s + d * N2];
+        double g[DIM];
+        for (int d = 0; d < DIM; d++) g[d] = s_dref[s * NG + qp + d * N2 * NG];
+        for (int i = 0; i < DIM; i++) {
+          double sum = 0.;
+          for (int j = 0; j < DIM; j++) sum += IJ[j + i * DIM] * g[j];
+          s_dg[(qp * N2 + s) * DIM + i] = sum;
+        }
+      }
+      double h_eff[DIM];
+      for (int d = 0; d < DIM; d++) {
+        double h = fabs(s_xx[d * N2 + 2 + 4 * (DIM - 2)] - s_xx[d * N2 + 0]);
+        double h1 = fabs(s_xx[d * N2 + 3 + 4 * (DIM - 2)] - s_xx[d * N2 + 1]);
+        h_eff[d] = h < h1 ? h1 : h;
+      }
+      double mod2 = 1.e-20;
+      for (int d = 0; d < DIM; d++) mod2 += vg[d] * vg[d];
+      mod2 = sqrt(mod2);
+      s_Tg[qp] = Tg;
+      for (int d = 0; d < DIM; d++) {
+        double Pe = 0.5 * mod2 * h_eff[d] / a.alpha0;
+        double fup = a.upwind * a.rhocp * 0.5 * (1. / tanh(Pe) - 1. / Pe) * h_eff[d] / mod2;
+        s_vel[qp * DIM + d] = vg[d];
+        s_cl[qp * DIM + d] = a.alpha0 * ke + fup * vg[d] * vg[d];
+      }
+    }
+    __syncthreads();
+    const double tcdt = a.tc * a.rhocp / a.dt;
+    const int64_t pbase = (int64_t)e * N2 * N2;
+    for (int t = tid; t < N2 * N2; t += blockDim.x) {
+      const int i = t / N2, j = t % N2;
+      if (s_bcv[i] == 0) continue;
+      double K = 0.;
+      for (int qp = 0; qp < NG; qp++) {
+        const double phii = s_phi[i * NG + qp], phij = s_phi[j * NG + qp];
+        double Adv = 0., Lap = 0.;
+        for (int d = 0; d < DIM; d++) {
+          const double dj = s_dg[(qp * N2 + j) * DIM + d];
+          Adv += s_vel[qp * DIM + d] * dj;
+          Lap += s_cl[qp * DIM + d] * dj * s_dg[(qp * N2 + i) * DIM + d];
+        }
+        K += s_JxW[qp] * (tcdt * phii * phij + a.rhocp * Adv * phii + Lap);
+      }
+      const int p = elpos[pbase + t];
+      if (p >= 0) val[p] += K;
+    }
+    if (mode == 1 && tid < N2) {
+      const int i = tid;
+      if (s_bcv[i] != 0) {
+        double F = 0.;
+        for (int qp = 0; qp < NG; qp++) {
+          const double phii = s_phi[i * NG + qp];
+          F += s_JxW[qp] * (tcdt * s_Tg[qp] * phii + a.source * phii);
+        }
+        rhs[eldof[(int64_t)e * N2 + i]] += F;
+      }
+    }
+  }
+}
+
+// Heat boundary faces: one thread per element of the colour (only elements
+// with boundary sides do work).
+template <int DIM>
+__global__ void k_heat_bd(int nel, const int *__restrict__ elems, const int *__restrict__ conn, const int *__restrict__ eldof,
+                          const int *__restrict__ elpos, const unsigned char *__restrict__ bmask,
+                          const double *__restrict__ xyz, int Ntop, const int *__restrict__ dof_top,
+                          const int *__restrict__ bc0, const int *__restrict__ bc1, const double *__restrict__ x_old,
+                          const FeTab *__restrict__ tab, HeatArgs a, int mode, double *__restrict__ val,
+                          double *__restrict__ rhs) {
+  constexpr int N2 = DIM == 2 ? 9 : 27, NB = DIM == 2 ? 3 : 9, NGB = NB, NS = 2 * DIM;
+  int ei = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ei >= nel) return;
+  int e = elems[ei];
+  unsigned char bm = bmask[e];
+  if (!bm) return;
+  const int64_t pbase = (int64_t)e * N2 * N2;
+  for (int s = 0; s < NS; s++) {
+    if (!((bm >> s) & 1)) continue;
+    int sur[NB];
+    double xb[DIM * NB];
+    for (int k = 0; k < NB; k++) {
+      sur[k] = face_node<DIM>(s, k);
+      int node = conn[(int64_t)e * N2 + sur[k]];
+      for (int d = 0; d < DIM; d++) xb[k + d * NB] = xyz[node + (int64_t)d * Ntop];
+    }
+    int cnode = conn[(int64_t)e * N2 + sur[NB - 1]];
+    int ckd = dof_top[cnode];
+    if (bc0[ckd] == 0) {
+      int bc_s = bc1[ckd];
+      double Ipen = jac_sur<DIM, NB, NGB>(NGB - 1, xb, tab) / a.dt;
+      for (int k = 0; k < NB; k++) {
+        int lv = sur[k];
+        int p = elpos[pbase + lv * N2 + lv];
+        if (p >= 0) val[p] += Ipen;
+        if (mode == 1) {
+          int node = conn[(int64_t)e * N2 + lv];
+          rhs[eldof[(int64_t)e * N2 + lv]] += (bc_s % 2) * Ipen * x_old[dof_top[node]];
+        }
+      }
+    } else if (bc1[ckd] != 0) {
+      for (int qp = 0; qp < NGB; qp++) {
+        double JxW = jac_sur<DIM, NB, NGB>(qp, xb, tab) * tab->bw[qp];
+        for (int i = 0; i < NB; i++) {
+          int lei = sur[i];
+          int kd = dof_top[conn[(int64_t)e * N2 + lei]];
+          int bcs = bc1[kd], bcv = bc0[kd];
+          double dtxJxW = JxW * bcv;
+          int bc_alpha = (bcs & 2) >> 1;
+          if (bc_alpha == 0) continue;
+          double phii = tab->bphi2[i * NGB + qp];
+          for (int j = 0; j < NB; j++) {
+            int p = elpos[pbase + lei * N2 + sur[j]];
+            if (p >= 0) val[p] += dtxJxW * bc_alpha * a.alpha0 * phii * tab->bphi2[j * NGB + qp];
+          }
+        }
+      }
+    }
+  }
+}
+
+// ===========================================================================
+// Navier-Stokes (Q2-Q1 coupled) volume kernel
+// ===========================================================================
+template <int DIM>
+__global__ __launch_bounds__(256) void k_ns_vol(int nel, const int *__restrict__ elems, const int *__restrict__ conn,
+                                                const int *__restrict__ eldof, const int *__restrict__ elpos,
+                                                const double *__restrict__ xyz, int Ntop, const int *__restrict__ dof_top,
+                                                const int *__restrict__ bc0, const double *__restrict__ x_old,
+                                                const double *__restrict__ coef, const FeTab *__restrict__ tab, NsArgs a,
+                                                int mode, double *__restrict__ val, double *__restrict__ rhs) {
+  constexpr int N2 = DIM == 2 ? 9 : 27, N1 = DIM == 2 ? 4 : 8, NG = N2, NL = DIM * N2 + N1;
+  __shared__ double s_phi[N2 * NG], s_psi[N1 * NG];
+  __shared__ double s_dref[DIM * N2 * NG];
+  __shared__ double s_dg[NG * N2 * DIM];
+  __shared__ double s_xx[DIM * N2], s_uold[DIM * N2];
+  __shared__ double s_JxW[NG], s_vel[NG * DIM], s_fup[NG * DIM], s_cl[NG * DIM];
+  __shared__ int s_bcv[DIM * N2];
+  const int tid = threadIdx.x;
+  for (int t = tid; t < N2 * NG; t += blockDim.x) s_phi[t] = tab->phi2[t];
+  for (int t = tid; t < N1 * NG; t += blockDim.x) s_psi[t] = tab->phi1[t];
+  for (int t = tid; t < DIM * N2 * NG; t += blockDim.x) s_dref[t] = tab->dphi2[t];
+  for (int ei = blockIdx.x; ei < nel; ei += gridDim.x) {
+    const int e = elems[ei];
+    __syncthreads();
+    if (tid < N2) {
+      int node = conn[(int64_t)e * N2 + tid];
+      for (int d = 0; d < DIM; d++) {
+        s_xx[tid + d * N2] = xyz[node + (int64_t)d * Ntop];
+        int kd = dof_top[node + d * Ntop];
+        s_uold[tid + d * N2] = x_old[kd];
+        s_bcv[tid + d * N2] = bc0[kd];
+      }
+    }
+    __syncthreads();
+    const double nu = coef ? coef[e] : a.IRe;
+    if (tid < NG) {
+      const int qp = tid;
+      double IJ[DIM * DIM];
+      double det = jac_qp<DIM, N2, NG>(qp, s_xx, s_dref, IJ);
+      s_JxW[qp] = det * tab->w[qp];
+      double vg[DIM];
+      for (int d = 0; d < DIM; d++) vg[d] = 0.;
+      for (int s = 0; s < N2; s++) {
+        double ph = s_phi[s * NG + qp];
+        for (int d = 0; d < DIM; d++) vg[d] += ph * s_uold[s + d * N2];
+        double g[DIM];
+        for (int d = 0; d < DIM; d++) g[d] = s_dref[s * NG + qp + d * N2 * NG];
+        for (int i = 0; i < DIM; i++) {
+          double sum = 0.;
+          for (int j = 0; j < DIM; j++) sum += IJ[j + i * DIM] * g[j];
+          s_dg[(qp * N2 + s) * DIM + i] = sum;
+        }
+      }
+      double h_eff[DIM];
+      for (int d = 0; d < DIM; d++) {
+        double h = fabs(s_xx[d * N2 + 2 + 4 * (DIM - 2)] - s_xx[d * N2 + 0]);
+        double h1 = fabs(s_xx[d * N2 + 3 + 4 * (DIM - 2)] - s_xx[d * N2 + 1]);
+        h_eff[d] = h < h1 ? h1 : h;
+      }
+      double mod2 = 1.e-20;
+      for (int d = 0; d < DIM; d++) mod2 += vg[d] * vg[d];
+      mod2 = sqrt(mod2);
+      for (int d = 0; d < DIM; d++) {
+        double Pe = 0.5 * mod2 * h_eff[d] / a.IRe;
+        double fup = a.upwind * 0.5 * (1. / tanh(Pe) - 1. / Pe) * h_eff[d] / mod2;
+        s_vel[qp * DIM + d] = vg[d];
+        s_fup[qp * DIM + d] = fup;
+        s_cl[qp * DIM + d] = nu + fup * vg[d] * vg[d];
+      }
+    }
+    __syncthreads();
+    const double tcdt = a.tc / a.dt;
+    const int64_t pbase = (int64_t)e * NL * NL;
+    for (int t = tid; t < NL * NL; t += blockDim.x) {
+      const int ra = t / NL, cb = t % NL;
+      double K = 0.;
+      if (ra < DIM * N2) {
+        const int iv = ra / N2, i = ra % N2;
+        if (s_bcv[i + iv * N2] == 0) continue;
+        if (cb < DIM * N2) {
+          const int jv = cb / N2, j = cb % N2;
+          if (jv == iv) {
+            for (int qp = 0; qp < NG; qp++) {
+              const double phii = s_phi[i * NG + qp], phij = s_phi[j * NG + qp];
+              double Adv = 0., Lap = 0.;
+              for (int k = 0; k < DIM; k++) {
+                const double dj = s_dg[(qp * N2 + j) * DIM + k];
+                Adv += s_vel[qp * DIM + k] * dj * phii;
+                Lap += s_cl[qp * DIM + k] * (dj * s_dg[(qp * N2 + i) * DIM + k]);
+              }
+              K += s_JxW[qp] * (tcdt * phij * phii + Adv + Lap +
+                                s_cl[qp * DIM + iv] * (s_dg[(qp * N2 + j) * DIM + iv] * s_dg[(qp * N2 + i) * DIM + iv]));
+            }
+          } else {
+            for (int qp = 0; qp < NG; qp++)
+              K += s_JxW[qp] * (nu + s_fup[qp * DIM + iv] * s_vel[qp * DIM + iv] * s_vel[qp * DIM + jv]) *
+                   s_dg[(qp * N2 + j) * DIM + iv] * s_dg[(qp * N2 + i) * DIM + jv];
+          }
+        } else {
+          const int j = cb - DIM * N2;
+          for (int qp = 0; qp < NG; qp++) K += s_JxW[qp] * (-s_psi[j * NG + qp] * s_dg[(qp * N2 + i) * DIM + iv]);
+        }
+      } else {
+        if (cb >= DIM * N2) continue;
+        const int i = ra - DIM * N2, jv = cb / N2, j = cb % N2;
+        for (int qp = 0; qp < NG; qp++) K += s_JxW[qp] * (s_psi[i * NG + qp] * s_dg[(qp * N2 + j) * DIM + jv]);
+      }
+      const int p = elpos[pbase + t];
+      if (p >= 0) val[p] += K;
+    }
+    if (mode == 1 && tid < DIM * N2) {
+      const int iv = tid / N2, i = tid % N2;
+      if (s_bcv[i + iv * N2] != 0) {
+        double F = 0.;
+        for (int qp = 0; qp < NG; qp++) {
+          const double phii = s_phi[i * NG + qp];
+          F += s_JxW[qp] * (tcdt * s_vel[qp * DIM + iv] * phii + a.IFr * a.dirg[iv] * phii + a.src[iv] * phii);
+        }
+        rhs[eldof[(int64_t)e * NL + tid]] += F;
+      }
+    }
+  }
+}
+
+// NS boundary faces (penalty Dirichlet with normal/tangential bits; Neumann tangential/normal terms)
+template <int DIM>
+__global__ void k_ns_bd(int nel, const int *__restrict__ elems, const int *__restrict__ conn, const int *__restrict__ eldof,
+                        const int *__restrict__ elpos, const unsigned char *__restrict__ bmask, const double *__restrict__ xyz,
+                        int Ntop, const int *__restrict__ dof_top, const int *__restrict__ bc0, const int *__restrict__ bc1,
+                        const double *__restrict__ x_old, const FeTab *__restrict__ tab, NsArgs a, int mode,
+                        double *__restrict__ val, double *__restrict__ rhs) {
+  constexpr int N2 = DIM == 2 ? 9 : 27, N1 = DIM == 2 ? 4 : 8, NB = DIM == 2 ? 3 : 9, NGB = NB, NS = 2 * DIM;
+  constexpr int NL = DIM * N2 + N1;
+  int ei = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ei >= nel) return;
+  int e = elems[ei];
+  unsigned char bm = bmask[e];
+  if (!bm) return;
+  const int64_t pbase = (int64_t)e * NL * NL;
+  for (int s = 0; s < NS; s++) {
+    if (!((bm >> s) & 1)) continue;
+    int sur[NB], gn[NB];
+    double xb[DIM * NB], nrm[DIM];
+    for (int k = 0; k < NB; k++) {
+      sur[k] = face_node<DIM>(s, k);
+      gn[k] = conn[(int64_t)e * N2 + sur[k]];
+      for (int d = 0; d < DIM; d++) xb[k + d * NB] = xyz[gn[k] + (int64_t)d * Ntop];
+    }
+    // MGFE::normal_g
+    if (DIM == 2) {
+      nrm[0] = xb[1 + NB] - xb[NB]; nrm[1] = -(xb[1] - xb[0]);
+    } else {
+      double t1[3], t3[3];
+      for (int i = 0; i < 3; i++) { t1[i] = xb[1 + i * NB] - xb[i * NB]; t3[i] = xb[3 + i * NB] - xb[i * NB]; }
+      nrm[0] = t1[1] * t3[2] - t1[2] * t3[1];
+      nrm[DIM > 1 ? 1 : 0] = t1[2] * t3[0] - t1[0] * t3[2];
+      nrm[DIM > 2 ? 2 : 0] = t1[0] * t3[1] - t1[1] * t3[0];
+    }
+    double mm = 0.;
+    for (int d = 0; d < DIM; d++) mm += nrm[d] * nrm[d];
+    mm = sqrt(mm);
+    for (int d = 0; d < DIM; d++) nrm[d] /= mm;
+    for (int iv = 0; iv < DIM; iv++) {
+      int ckd = dof_top[gn[NB - 1] + iv * Ntop];
+      if (bc0[ckd] == 0) {
+        double Ipen = jac_sur<DIM, NB, NGB>(NGB - 1, xb, tab) / a.dt;
+        for (int i = 0; i < NB; i++) {
+          int row = sur[i] + iv * N2;
+          int kd = dof_top[gn[i] + iv * Ntop];
+          int bcs = bc1[kd];
+          int bc_rhs = (bcs & 4) >> 2, bc_tg = (bcs & 2) >> 1, bc_n = bcs % 2;
+          double vo[DIM], vdotn = 0.;
+          for (int k = 0; k < DIM; k++) { vo[k] = x_old[dof_top[gn[i] + k * Ntop]]; vdotn += nrm[k] * vo[k]; }
+          if (mode == 1)
+            rhs[eldof[(int64_t)e * NL + row]] += bc_rhs * Ipen * ((1 - bc_n) * (1 - bc_tg) * vo[iv] + bc_n * vdotn * nrm[iv] +
+                                                                    bc_tg * (vo[iv] - vdotn * nrm[iv]));
+          int p = elpos[pbase + row * NL + row];
+          if (p >= 0) val[p] += ((1 - bc_n) * (1 - bc_tg) + bc_tg) * Ipen + Ipen * nrm[iv] * nrm[iv] * (bc_n - bc_tg);
+          for (int jv = iv + 1; jv < DIM; jv++) {
+            int pc = elpos[pbase + row * NL + sur[i] + jv * N2];
+            if (pc >= 0) val[pc] += Ipen * nrm[jv] * nrm[iv] * (bc_n - bc_tg);
+          }
+        }
+      } else if (bc1[ckd] != 0) {
+        for (int qp = 0; qp < NGB; qp++) {
+          double JxW = jac_sur<DIM, NB, NGB>(qp, xb, tab) * tab->bw[qp];
+          for (int i = 0; i < NB; i++) {
+            int row = sur[i] + iv * N2;
+            int kd = dof_top[gn[i] + iv * Ntop];
+            int bcs = bc1[kd], bcv = bc0[kd];
+            double dtJxW = JxW * (bcv % 2);
+            int bc_tg = (bcs & 2) >> 1, bc_n = bcs % 2;
+            if (bc_tg == 0 && bc_n == 0) continue;
+            double phii = tab->bphi2[i * NGB + qp];
+            for (int j = 0; j < NB; j++) {
+              double phij = tab->bphi2[j * NGB + qp];
+              int p = elpos[pbase + row * NL + row];
+              if (p >= 0) val[p] += dtJxW * bc_tg * phij * phii + dtJxW * nrm[iv] * nrm[iv] * phij * phii * (bc_n - bc_tg);
+              for (int jv = iv + 1; jv < DIM; jv++) {
+                int pc = elpos[pbase + row * NL + sur[j] + jv * N2];
+                if (pc >= 0) val[pc] += dtJxW * nrm[jv] * nrm[iv] * phij * phii * (bc_n - bc_tg);
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+namespace fmk {
+
+static inline unsigned nblk(int nel) {
+  int b = nel < 4096 ? nel : 4096;
+  return (unsigned)(b < 1 ? 1 : b);
+}
+
+int heat_assemble(hipStream_t s, int dim, int nel, const int *elems, const int *conn, const int *eldof,
+                  const int *elpos, const unsigned char *bmask, const double *xyz, int Ntop, const int *dof_top,
+                  const int *bc0, const int *bc1, const double *x_old, const double *coef, const double *vel,
+                  const FeTab *tab, HeatArgs a, int mode, double *val, double *rhs) {
+  if (nel <= 0) return 0;
+  if (dim == 2) {
+    hipLaunchKernelGGL(k_heat_vol<2>, dim3(nblk(nel)), dim3(128), 0, s, nel, elems, conn, eldof, elpos, xyz, Ntop,
+                       dof_top, bc0, x_old, coef, vel, tab, a, mode, val, rhs);
+    hipLaunchKernelGGL(k_heat_bd<2>, dim3((nel + 127) / 128), dim3(128), 0, s, nel, elems, conn, eldof, elpos, bmask,
+                       xyz, Ntop, dof_top, bc0, bc1, x_old, tab, a, mode, val, rhs);
+  } else {
+    hipLaunchKernelGGL(k_heat_vol<3>, dim3(nblk(nel)), dim3(256), 0, s, nel, elems, conn, eldof, elpos, xyz, Ntop,
+                       dof_top, bc0, x_old, coef, vel, tab, a, mode, val, rhs);
+    hipLaunchKernelGGL(k_heat_bd<3>, dim3((nel + 127) / 128), dim3(128), 0, s, nel, elems, conn, eldof, elpos, bmask,
+                       xyz, Ntop, dof_top, bc0, bc1, x_old, tab, a, mode, val, rhs);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int ns_assemble(hipStream_t s, int dim, int nel, const int *elems, const int *conn, const int *eldof,
+                const int *elpos, const unsigned char *bmask, const double *xyz, int Ntop, const int *dof_top,
+                const int *bc0, const int *bc1, const double *x_old, const double *coef,
+                const FeTab *tab, NsArgs a, int mode, double *val, double *rhs) {
+  if (nel <= 0) return 0;
+  if (dim == 2) {
+    hipLaunchKernelGGL(k_ns_vol<2>, dim3(nblk(nel)), dim3(256), 0, s, nel, elems, conn, eldof, elpos, xyz, Ntop,
+                       dof_top, bc0, x_old, coef, tab, a, mode, val, rhs);
+    hipLaunchKernelGGL(k_ns_bd<2>, dim3((nel + 127) / 128), dim3(128), 0, s, nel, elems, conn, eldof, elpos, bmask,
+                       xyz, Ntop, dof_top, bc0, bc1, x_old, tab, a, mode, val, rhs);
+  } else {
+    hipLaunchKernelGGL(k_ns_vol<3>, dim3(nblk(nel)), dim3(256), 0, s, nel, elems, conn, eldof, elpos, xyz, Ntop,
+                       dof_top, bc0, x_old, coef, tab, a, mode, val, rhs);
+    hipLaunchKernelGGL(k_ns_bd<3>, dim3((nel + 127) / 128), dim3(128), 0, s, nel, elems, conn, eldof, elpos, bmask,
+                       xyz, Ntop, dof_top, bc0, bc1, x_old, tab, a, mode, val, rhs);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+}  // namespace fmk

@@ -1,0 +1,591 @@
+// femus.cpp -- host side of the MI355X FEMuS hot path: context, system setup
+// (MGSolDA::init_dof / GenBc / init), GenMatRhs drivers, the multigrid
+// V-cycle (MGSolBase::MGStep / MGSolve) in FP64 and FP32, the FP64 outer
+// loops (iterative refinement, flexible GMRES) and the C ABI.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "fm_internal.h"
+#include "fm_kernels.h"
+
+static thread_local std::string g_err;
+void fm_set_error(const std::string &msg) { g_err = msg; }
+extern "C" const char *fm_last_error(void) { return g_err.c_str(); }
+
+#define FM_CHECK(x)                                 \
+  do {                                              \
+    int rc_ = (x);                                  \
+    if (rc_ != 0) {                                 \
+      if (g_err.empty()) fm_set_error(#x);          \
+      return rc_ < 0 ? rc_ : FM_EHIP;               \
+    }                                               \
+  } while (0)
+
+template <typename T> static int dalloc(T **p, size_t n) {
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+  if (e != hipSuccess) { fm_set_error(std::string("hipMalloc: ") + hipGetErrorString(e)); return FM_ENOMEM; }
+  return 0;
+}
+template <typename T> static void dfree(T *&p) { if (p) { (void)hipFree(p); p = nullptr; } }
+
+// ===========================================================================
+// context
+// ===========================================================================
+extern "C" int fm_ctx_create(int device, fm_ctx **out) {
+  if (!out) return FM_EINVAL;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) { fm_set_error("no HIP device"); return FM_EHIP; }
+  if (device < 0 || device >= ndev) return FM_EINVAL;
+  fm_ctx *c = new fm_ctx();
+  c->device = device;
+  FM_HIP(hipSetDevice(device));
+  FM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  FM_HIP(hipEventCreate(&c->ev0));
+  FM_HIP(hipEventCreate(&c->ev1));
+  FM_CHECK(dalloc(&c->scratch, 4096));
+  FM_HIP(hipHostMalloc((void **)&c->h_scratch, 4096 * sizeof(double), hipHostMallocDefault));
+  *out = c;
+  return FM_OK;
+}
+
+extern "C" int fm_ctx_destroy(fm_ctx *c) {
+  if (!c) return FM_EINVAL;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  dfree(c->scratch);
+  if (c->h_scratch) (void)hipHostFree(c->h_scratch);
+  (void)hipEventDestroy(c->ev0);
+  (void)hipEventDestroy(c->ev1);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return FM_OK;
+}
+
+extern "C" int fm_sync(fm_ctx *c) {
+  if (!c) return FM_EINVAL;
+  FM_HIP(hipStreamSynchronize(c->stream));
+  return FM_OK;
+}
+
+// ===========================================================================
+// mesh
+// ===========================================================================
+extern "C" int fm_mesh_create_box(int dim, const int nx0[3], const double lo[3], const double hi[3], int nlev,
+                                  fm_mesh **out) {
+  if (!out || !nx0 || !lo || !hi) return FM_EINVAL;
+  fm_mesh *m = new fm_mesh();
+  int rc = fm_build_box(dim, nx0, lo, hi, nlev, m);
+  if (rc) { delete m; fm_set_error("fm_mesh_create_box: invalid sizes"); return rc; }
+  *out = m;
+  return FM_OK;
+}
+extern "C" int fm_mesh_destroy(fm_mesh *m) { delete m; return FM_OK; }
+extern "C" int fm_mesh_info(const fm_mesh *m, int *info) {
+  if (!m || !info) return FM_EINVAL;
+  info[0] = m->dim; info[1] = m->nlev; info[2] = m->n2; info[3] = m->n1; info[4] = m->nsides; info[5] = m->Ntop;
+  return FM_OK;
+}
+extern "C" int fm_mesh_level(const fm_mesh *m, int L, int *n_el, int *n_q2, int *n_q1) {
+  if (!m || L < 0 || L >= m->nlev) return FM_EINVAL;
+  if (n_el) *n_el = m->nel[L];
+  if (n_q2) *n_q2 = m->nq2[L];
+  if (n_q1) *n_q1 = m->nq1[L];
+  return FM_OK;
+}
+extern "C" int fm_mesh_coords(const fm_mesh *m, double *xyz) {
+  if (!m || !xyz) return FM_EINVAL;
+  std::memcpy(xyz, m->xyz.data(), m->xyz.size() * sizeof(double));
+  return FM_OK;
+}
+extern "C" int fm_mesh_conn(const fm_mesh *m, int L, int *conn, int *neigh, int *parent) {
+  if (!m || L < 0 || L >= m->nlev) return FM_EINVAL;
+  if (conn) std::memcpy(conn, m->conn[L].data(), m->conn[L].size() * sizeof(int));
+  if (neigh) std::memcpy(neigh, m->neigh[L].data(), m->neigh[L].size() * sizeof(int));
+  if (parent) std::memcpy(parent, m->parent[L].data(), m->parent[L].size() * sizeof(int));
+  return FM_OK;
+}
+
+// ===========================================================================
+// FE tables (closed-form Q2/Q1 Lagrange at 3-point Gauss-Legendre points)
+// ===========================================================================
+static double l2f(int c, double x) { return c < 0 ? .5 * x * (x - 1.) : (c > 0 ? .5 * x * (x + 1.) : 1. - x * x); }
+static double dl2f(int c, double x) { return c < 0 ? x - .5 : (c > 0 ? x + .5 : -2. * x); }
+static double l1f(int c, double x) { return c < 0 ? .5 * (1. - x) : .5 * (1. + x); }
+static int ref_c(int dim, int k, int d) {
+  if (dim == 1) return k == 0 ? -1 : (k == 1 ? 1 : 0);
+  return dim == 2 ? fm_quad9_ref[k][d] : fm_hex27_ref[k][d];
+}
+static void gauss_pt(int dim, int qp, double *xi, double *w) {
+  const double a = -std::sqrt(3. / 5.), x1[3] = {a, 0., -a}, w1[3] = {5. / 9., 8. / 9., 5. / 9.};
+  int idx[3] = {0, 0, 0};
+  if (dim == 1) idx[0] = qp;
+  else if (dim == 2) { idx[0] = qp / 3; idx[1] = qp % 3; }
+  else { idx[0] = qp / 9; idx[1] = (qp / 3) % 3; idx[2] = qp % 3; }
+  *w = 1.;
+  for (int d = 0; d < dim; d++) { xi[d] = x1[idx[d]]; *w *= w1[idx[d]]; }
+}
+static void build_tables(int dim, FeTab &t) {
+  std::memset(&t, 0, sizeof(t));
+  int n2 = dim == 2 ? 9 : 27, n1 = dim == 2 ? 4 : 8, ng = n2;
+  for (int qp = 0; qp < ng; qp++) {
+    double xi[3], w;
+    gauss_pt(dim, qp, xi, &w);
+    t.w[qp] = w;
+    for (int s = 0; s < n2; s++) {
+      double v = 1.;
+      for (int d = 0; d < dim; d++) v *= l2f(ref_c(dim, s, d), xi[d]);
+      t.phi2[s * ng + qp] = v;
+      for (int dir = 0; dir < dim; dir++) {
+        double g = 1.;
+        for (int d = 0; d < dim; d++) g *= d == dir ? dl2f(ref_c(dim, s, d), xi[d]) : l2f(ref_c(dim, s, d), xi[d]);
+        t.dphi2[(s * ng + qp) + dir * n2 * ng] = g;
+      }
+    }
+    for (int s = 0; s < n1; s++) {
+      double v = 1.;
+      for (int d = 0; d < dim; d++) v *= l1f(ref_c(dim, s, d), xi[d]);
+      t.phi1[s * ng + qp] = v;
+    }
+  }
+  int bd = dim - 1, nb = dim == 2 ? 3 : 9, ngb = nb;
+  for (int qp = 0; qp < ngb; qp++) {
+    double xi[3], w;
+    gauss_pt(bd, qp, xi, &w);
+    t.bw[qp] = w;
+    for (int s = 0; s < nb; s++) {
+      double v = 1.;
+      for (int d = 0; d < bd; d++) v *= l2f(ref_c(bd, s, d), xi[d]);
+      t.bphi2[s * ngb + qp] = v;
+      for (int dir = 0; dir < bd; dir++) {
+        double g = 1.;
+        for (int d = 0; d < bd; d++) g *= d == dir ? dl2f(ref_c(bd, s, d), xi[d]) : l2f(ref_c(bd, s, d), xi[d]);
+        t.bdphi2[(s * ngb + qp) + dir * nb * ngb] = g;
+      }
+    }
+  }
+}
+
+// ===========================================================================
+// system
+// ===========================================================================
+struct SysExtra {
+  FeTab *d_tab = nullptr;
+  int *d_elems_tmp = nullptr;
+};
+static std::vector<std::pair<fm_system *, SysExtra>> g_extra;  // tables per system
+static SysExtra &extra(fm_system *s) {
+  for (auto &p : g_extra) if (p.first == s) return p.second;
+  g_extra.push_back({s, SysExtra()});
+  return g_extra.back().second;
+}
+
+extern "C" int fm_system_create(fm_ctx *ctx, const fm_mesh *m, int nq2, int nq1, fm_system **out) {
+  if (!ctx || !m || !out || nq2 < 0 || nq1 < 0 || nq2 + nq1 == 0 || nq2 + nq1 > 8) return FM_EINVAL;
+  if (nq1 > 0 && nq2 == 0) return FM_EINVAL;
+  fm_system *s = new fm_system();
+  s->ctx = ctx; s->mesh = m; s->nq2 = nq2; s->nq1 = nq1;
+  s->nloc = nq2 * m->n2 + nq1 * m->n1;
+  s->node_dof.resize(m->nlev);
+  s->Dim.resize(m->nlev);
+  for (int L = 0; L < m->nlev; L++) fm_init_dof(*m, L, nq2, nq1, s->node_dof[L], s->Dim[L]);
+  int64_t nt = s->Dim[m->nlev - 1];
+  s->bc0.assign(nt, 1); s->bc1.assign(nt, 1);
+  s->lev.resize(m->nlev);
+  *out = s;
+  return FM_OK;
+}
+
+static void free_csr(DevCSR &c) { dfree(c.rp); dfree(c.ci); dfree(c.v); dfree(c.vf); }
+
+extern "C" int fm_system_destroy(fm_system *s) {
+  if (!s) return FM_EINVAL;
+  (void)hipSetDevice(s->ctx->device);
+  (void)hipStreamSynchronize(s->ctx->stream);
+  for (auto &l : s->lev) {
+    free_csr(l.A); free_csr(l.R); free_csr(l.P);
+    dfree(l.x); dfree(l.b); dfree(l.r); dfree(l.t); dfree(l.d); dfree(l.d2); dfree(l.dinv);
+    dfree(l.xf); dfree(l.bf); dfree(l.rf); dfree(l.tf); dfree(l.df); dfree(l.d2f); dfree(l.dinvf);
+    dfree(l.conn); dfree(l.eldof); dfree(l.elpos); dfree(l.bmask); dfree(l.color_elems); dfree(l.coef);
+    dfree(l.Ainv); dfree(l.Ainvf); dfree(l.vinv); dfree(l.vwt); dfree(l.vwtf); dfree(l.nrm); dfree(l.res);
+  }
+  dfree(s->d_bc0); dfree(s->d_bc1); dfree(s->d_dof_top); dfree(s->xyz); dfree(s->x_old); dfree(s->vel);
+  SysExtra &ex = extra(s);
+  dfree(ex.d_tab);
+  for (size_t i = 0; i < g_extra.size(); i++) if (g_extra[i].first == s) { g_extra.erase(g_extra.begin() + i); break; }
+  delete s;
+  return FM_OK;
+}
+
+extern "C" int64_t fm_system_ndof(const fm_system *s, int L) {
+  if (!s || L < 0 || L >= (int)s->Dim.size()) return -1;
+  return s->Dim[L];
+}
+extern "C" int fm_system_node_dof(const fm_system *s, int L, int *nd) {
+  if (!s || !nd || L < 0 || L >= (int)s->Dim.size()) return FM_EINVAL;
+  std::memcpy(nd, s->node_dof[L].data(), s->node_dof[L].size() * sizeof(int));
+  return FM_OK;
+}
+
+// MGSolDA::GenBc_loop (src/MGSolverDA.C:786-862): interior nodes keep the
+// default (bc_intern_read: all 1); nodes of top-level boundary faces take
+// bc_read(xp).  Q1 variables are set on face vertices only.
+extern "C" int fm_system_genbc(fm_system *s, fm_bc_fn fn, void *user) {
+  if (!s || !fn) return FM_EINVAL;
+  const fm_mesh &m = *s->mesh;
+  int top = m.nlev - 1, N = m.Ntop, dim = m.dim, nv = s->nq2 + s->nq1;
+  int nb1 = dim == 2 ? 2 : 4;
+  const std::vector<int> &dof = s->node_dof[top];
+  std::fill(s->bc0.begin(), s->bc0.end(), 1);
+  std::fill(s->bc1.begin(), s->bc1.end(), 1);
+  int neum[8], flag[8];
+  for (int e = 0; e < m.nel[top]; e++)
+    for (int sd = 0; sd < m.nsides; sd++) {
+      if (m.neigh[top][(size_t)e * m.nsides + sd] != -1) continue;
+      for (int i = 0; i < m.nb2; i++) {
+        int ln = dim == 2 ? fm_face2d[sd][i] : fm_face3d[sd][i];
+        int k = m.conn[top][(size_t)e * m.n2 + ln];
+        double xp[3] = {0, 0, 0};
+        for (int d = 0; d < dim; d++) xp[d] = m.xyz[k + (size_t)d * N];
+        for (int iv = 0; iv < nv; iv++) { neum[iv] = 1; flag[iv] = 1; }
+        fn(xp, neum, flag, user);
+        for (int iv = 0; iv < nv; iv++) {
+          if (iv >= s->nq2 && i >= nb1) continue;
+          int kd = dof[k + (size_t)iv * N];
+          s->bc0[kd] = neum[iv];
+          s->bc1[kd] = flag[iv];
+        }
+      }
+    }
+  s->initialized = false;
+  return FM_OK;
+}
+
+extern "C" int fm_system_get_bc(const fm_system *s, int *bc0, int *bc1) {
+  if (!s) return FM_EINVAL;
+  if (bc0) std::memcpy(bc0, s->bc0.data(), s->bc0.size() * sizeof(int));
+  if (bc1) std::memcpy(bc1, s->bc1.data(), s->bc1.size() * sizeof(int));
+  return FM_OK;
+}
+
+static int upload_csr(hipStream_t st, DevCSR &c, int64_t nr, int64_t nc, std::vector<int64_t> &rp, std::vector<int> &ci,
+                      std::vector<double> *v) {
+  free_csr(c);
+  c.nr = nr; c.nc = nc; c.nnz = rp[nr];
+  FM_CHECK(dalloc(&c.rp, nr + 1));
+  FM_CHECK(dalloc(&c.ci, c.nnz));
+  FM_CHECK(dalloc(&c.v, c.nnz));
+  FM_CHECK(dalloc(&c.vf, c.nnz));
+  FM_HIP(hipMemcpyAsync(c.rp, rp.data(), (nr + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  FM_HIP(hipMemcpyAsync(c.ci, ci.data(), c.nnz * sizeof(int), hipMemcpyHostToDevice, st));
+  if (v) {
+    FM_HIP(hipMemcpyAsync(c.v, v->data(), c.nnz * sizeof(double), hipMemcpyHostToDevice, st));
+    FM_CHECK(fmk::cast_vals(st, c.nnz, c.v, c.vf));
+  } else {
+    FM_HIP(hipMemsetAsync(c.v, 0, c.nnz * sizeof(double), st));
+    FM_HIP(hipMemsetAsync(c.vf, 0, c.nnz * sizeof(float), st));
+  }
+  FM_HIP(hipStreamSynchronize(st));
+  c.h_rp.swap(rp);
+  c.h_ci.swap(ci);
+  if (v) c.h_v.swap(*v);
+  return 0;
+}
+
+template <typename T> static CsrView<T> view(const DevCSR &c);
+template <> CsrView<double> view<double>(const DevCSR &c) {
+  CsrView<double> v; v.nr = c.nr; v.rp = c.rp; v.ci = c.ci; v.v = c.v; v.lpr = fmk::lanes_for(c.nnz, c.nr); return v;
+}
+template <> CsrView<float> view<float>(const DevCSR &c) {
+  CsrView<float> v; v.nr = c.nr; v.rp = c.rp; v.ci = c.ci; v.v = c.vf; v.lpr = fmk::lanes_for(c.nnz, c.nr); return v;
+}
+
+extern "C" int fm_system_init(fm_system *s) {
+  if (!s) return FM_EINVAL;
+  fm_ctx *c = s->ctx;
+  FM_HIP(hipSetDevice(c->device));
+  hipStream_t st = c->stream;
+  const fm_mesh &m = *s->mesh;
+  int nlev = m.nlev, top = nlev - 1, N = m.Ntop, nv = s->nq2 + s->nq1, nloc = s->nloc;
+  // top-level maps, coordinates and BC on the device
+  dfree(s->d_bc0); dfree(s->d_bc1); dfree(s->d_dof_top); dfree(s->xyz); dfree(s->x_old);
+  int64_t nt = s->Dim[top];
+  FM_CHECK(dalloc(&s->d_bc0, nt)); FM_CHECK(dalloc(&s->d_bc1, nt));
+  FM_CHECK(dalloc(&s->d_dof_top, (size_t)nv * N));
+  FM_CHECK(dalloc(&s->xyz, (size_t)m.dim * N));
+  FM_CHECK(dalloc(&s->x_old, nt));
+  FM_HIP(hipMemcpyAsync(s->d_bc0, s->bc0.data(), nt * sizeof(int), hipMemcpyHostToDevice, st));
+  FM_HIP(hipMemcpyAsync(s->d_bc1, s->bc1.data(), nt * sizeof(int), hipMemcpyHostToDevice, st));
+  FM_HIP(hipMemcpyAsync(s->d_dof_top, s->node_dof[top].data(), (size_t)nv * N * sizeof(int), hipMemcpyHostToDevice, st));
+  FM_HIP(hipMemcpyAsync(s->xyz, m.xyz.data(), m.xyz.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  FM_HIP(hipMemsetAsync(s->x_old, 0, nt * sizeof(double), st));
+  SysExtra &ex = extra(s);
+  if (!ex.d_tab) {
+    FeTab t;
+    build_tables(m.dim, t);
+    FM_CHECK(dalloc(&ex.d_tab, 1));
+    FM_HIP(hipMemcpy(ex.d_tab, &t, sizeof(FeTab), hipMemcpyHostToDevice));
+  }
+  for (int L = 0; L < nlev; L++) {
+    Level &l = s->lev[L];
+    l.n = s->Dim[L];
+    l.nel = m.nel[L];
+    // pattern
+    std::vector<int64_t> rp;
+    std::vector<int> ci;
+    fm_pattern(m, L, s->nq2, s->nq1, s->node_dof[L], rp, ci);
+    if (rp[l.n] >= ((int64_t)1 << 31)) { fm_set_error("nnz exceeds int32 element map range"); return FM_EINVAL; }
+    FM_CHECK(upload_csr(st, l.A, l.n, l.n, rp, ci, nullptr));
+    // transfers
+    if (L < top) {
+      std::vector<int64_t> trp; std::vector<int> tci; std::vector<double> tv;
+      fm_transfer(m, L, s->nq2, s->nq1, true, s->node_dof[L], s->node_dof[L + 1], &s->node_dof[top], &s->bc0, trp, tci, tv);
+      FM_CHECK(upload_csr(st, l.R, l.n, s->Dim[L + 1], trp, tci, &tv));
+    }
+    if (L > 0) {
+      std::vector<int64_t> trp; std::vector<int> tci; std::vector<double> tv;
+      fm_transfer(m, L, s->nq2, s->nq1, false, s->node_dof[L], s->node_dof[L - 1], nullptr, nullptr, trp, tci, tv);
+      FM_CHECK(upload_csr(st, l.P, l.n, s->Dim[L - 1], trp, tci, &tv));
+    }
+    // vectors
+    dfree(l.x); dfree(l.b); dfree(l.r); dfree(l.t); dfree(l.d); dfree(l.d2); dfree(l.dinv); dfree(l.res);
+    dfree(l.xf); dfree(l.bf); dfree(l.rf); dfree(l.tf); dfree(l.df); dfree(l.d2f); dfree(l.dinvf); dfree(l.nrm);
+    FM_CHECK(dalloc(&l.x, l.n)); FM_CHECK(dalloc(&l.b, l.n)); FM_CHECK(dalloc(&l.r, l.n)); FM_CHECK(dalloc(&l.t, l.n));
+    FM_CHECK(dalloc(&l.d, l.n)); FM_CHECK(dalloc(&l.d2, l.n)); FM_CHECK(dalloc(&l.dinv, l.n)); FM_CHECK(dalloc(&l.res, l.n));
+    FM_CHECK(dalloc(&l.xf, l.n)); FM_CHECK(dalloc(&l.bf, l.n)); FM_CHECK(dalloc(&l.rf, l.n)); FM_CHECK(dalloc(&l.tf, l.n));
+    FM_CHECK(dalloc(&l.df, l.n)); FM_CHECK(dalloc(&l.d2f, l.n)); FM_CHECK(dalloc(&l.dinvf, l.n));
+    FM_CHECK(dalloc(&l.nrm, 8));
+    FM_HIP(hipMemsetAsync(l.x, 0, l.n * sizeof(double), st));
+    FM_HIP(hipMemsetAsync(l.b, 0, l.n * sizeof(double), st));
+    FM_HIP(hipMemsetAsync(l.res, 0, l.n * sizeof(double), st));
+    // elements: connectivity, DoFs, colours, boundary masks, CSR positions
+    std::vector<int> eld((size_t)l.nel * nloc);
+    std::vector<unsigned char> bm(l.nel, 0);
+    std::vector<int> col(l.nel);
+    for (int e = 0; e < l.nel; e++) {
+      fm_elem_dofs(m, L, e, s->nq2, s->nq1, s->node_dof[L], &eld[(size_t)e * nloc]);
+      for (int sd = 0; sd < m.nsides; sd++) if (m.neigh[L][(size_t)e * m.nsides + sd] == -1) bm[e] |= (unsigned char)(1 << sd);
+      const int *g = &m.egrid[L][3 * (size_t)e];
+      col[e] = (g[0] & 1) + 2 * (g[1] & 1) + (m.dim == 3 ? 4 * (g[2] & 1) : 0);
+    }
+    l.ncolor = 1 << m.dim;
+    l.color_off.assign(l.ncolor + 1, 0);
+    for (int e = 0; e < l.nel; e++) l.color_off[col[e] + 1]++;
+    for (int k = 0; k < l.ncolor; k++) l.color_off[k + 1] += l.color_off[k];
+    std::vector<int> ce(l.nel), fillp(l.color_off.begin(), l.color_off.end() - 1);
+    for (int e = 0; e < l.nel; e++) ce[fillp[col[e]]++] = e;
+    dfree(l.conn); dfree(l.eldof); dfree(l.elpos); dfree(l.bmask); dfree(l.color_elems);
+    FM_CHECK(dalloc(&l.conn, (size_t)l.nel * m.n2));
+    FM_CHECK(dalloc(&l.eldof, (size_t)l.nel * nloc));
+    FM_CHECK(dalloc(&l.elpos, (size_t)l.nel * nloc * nloc));
+    FM_CHECK(dalloc(&l.bmask, l.nel));
+    FM_CHECK(dalloc(&l.color_elems, l.nel));
+    FM_HIP(hipMemcpyAsync(l.conn, m.conn[L].data(), m.conn[L].size() * sizeof(int), hipMemcpyHostToDevice, st));
+    FM_HIP(hipMemcpyAsync(l.eldof, eld.data(), eld.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    FM_HIP(hipMemcpyAsync(l.bmask, bm.data(), bm.size(), hipMemcpyHostToDevice, st));
+    FM_HIP(hipMemcpyAsync(l.color_elems, ce.data(), ce.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    FM_CHECK(fmk::elpos(st, l.nel, nloc, l.eldof, l.A.rp, l.A.ci, l.elpos));
+    FM_HIP(hipStreamSynchronize(st));
+  }
+  s->initialized = true;
+  s->mg_ready = false;
+  return FM_OK;
+}
+
+extern "C" int64_t fm_system_csr_nnz(const fm_system *s, int which, int L) {
+  if (!s || L < 0 || L >= (int)s->lev.size()) return -1;
+  const Level &l = s->lev[L];
+  const DevCSR &c = which == 0 ? l.A : (which == 1 ? l.R : l.P);
+  return c.nnz;
+}
+
+extern "C" int fm_system_csr_get(const fm_system *s, int which, int L, int64_t *rowptr, int *colind, double *val) {
+  if (!s || L < 0 || L >= (int)s->lev.size() || !s->initialized) return FM_EINVAL;
+  const Level &l = s->lev[L];
+  const DevCSR &c = which == 0 ? l.A : (which == 1 ? l.R : l.P);
+  if (c.nr == 0) return FM_EINVAL;
+  FM_HIP(hipSetDevice(s->ctx->device));
+  FM_HIP(hipStreamSynchronize(s->ctx->stream));
+  if (rowptr) FM_HIP(hipMemcpy(rowptr, c.rp, (c.nr + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
+  if (colind) FM_HIP(hipMemcpy(colind, c.ci, c.nnz * sizeof(int), hipMemcpyDeviceToHost));
+  if (val) FM_HIP(hipMemcpy(val, c.v, c.nnz * sizeof(double), hipMemcpyDeviceToHost));
+  return FM_OK;
+}
+
+static int check_level(const fm_system *s, int L) {
+  if (!s || !s->initialized || L < 0 || L >= (int)s->lev.size()) { fm_set_error("system not initialised or bad level"); return FM_ESTATE; }
+  return 0;
+}
+
+extern "C" int fm_system_set_xold(fm_system *s, const double *x) {
+  FM_CHECK(check_level(s, 0));
+  FM_HIP(hipMemcpyAsync(s->x_old, x, s->Dim.back() * sizeof(double), hipMemcpyHostToDevice, s->ctx->stream));
+  FM_HIP(hipStreamSynchronize(s->ctx->stream));
+  return FM_OK;
+}
+static int copy_vec(fm_system *s, int L, double *dst_dev_or_host, const double *src, bool to_host) {
+  FM_HIP(hipMemcpyAsync(dst_dev_or_host, src, s->lev[L].n * sizeof(double), to_host ? hipMemcpyDeviceToHost : hipMemcpyHostToDevice, s->ctx->stream));
+  FM_HIP(hipStreamSynchronize(s->ctx->stream));
+  return 0;
+}
+extern "C" int fm_system_get_x(const fm_system *s, int L, double *x) {
+  FM_CHECK(check_level(s, L));
+  return copy_vec(const_cast<fm_system *>(s), L, x, s->lev[L].x, true);
+}
+extern "C" int fm_system_set_x(fm_system *s, int L, const double *x) {
+  FM_CHECK(check_level(s, L));
+  return copy_vec(s, L, s->lev[L].x, x, false);
+}
+extern "C" int fm_system_get_rhs(const fm_system *s, int L, double *b) {
+  FM_CHECK(check_level(s, L));
+  return copy_vec(const_cast<fm_system *>(s), L, b, s->lev[L].b, true);
+}
+extern "C" int fm_system_set_rhs(fm_system *s, int L, const double *b) {
+  FM_CHECK(check_level(s, L));
+  return copy_vec(s, L, s->lev[L].b, b, false);
+}
+extern "C" int fm_get_res(const fm_system *s, int L, double *r) {
+  FM_CHECK(check_level(s, L));
+  return copy_vec(const_cast<fm_system *>(s), L, r, s->lev[L].res, true);
+}
+
+extern "C" int fm_system_set_elem_coef(fm_system *s, int L, const double *coef) {
+  FM_CHECK(check_level(s, L));
+  Level &l = s->lev[L];
+  if (!coef) { dfree(l.coef); return FM_OK; }
+  if (!l.coef) FM_CHECK(dalloc(&l.coef, l.nel));
+  FM_HIP(hipMemcpyAsync(l.coef, coef, l.nel * sizeof(double), hipMemcpyHostToDevice, s->ctx->stream));
+  FM_HIP(hipStreamSynchronize(s->ctx->stream));
+  return FM_OK;
+}
+extern "C" int fm_system_set_velocity(fm_system *s, const double *vel) {
+  FM_CHECK(check_level(s, 0));
+  if (!vel) { dfree(s->vel); return FM_OK; }
+  size_t n = (size_t)s->mesh->dim * s->mesh->Ntop;
+  if (!s->vel) FM_CHECK(dalloc(&s->vel, n));
+  FM_HIP(hipMemcpyAsync(s->vel, vel, n * sizeof(double), hipMemcpyHostToDevice, s->ctx->stream));
+  FM_HIP(hipStreamSynchronize(s->ctx->stream));
+  return FM_OK;
+}
+
+// refresh FP32 values and diagonal inverses after an assembly
+static int after_assembly(fm_system *s, int L) {
+  Level &l = s->lev[L];
+  hipStream_t st = s->ctx->stream;
+  FM_CHECK(fmk::cast_vals(st, l.A.nnz, l.A.v, l.A.vf));
+  FM_CHECK(fmk::diag_inv(st, l.n, l.A.rp, l.A.ci, l.A.v, l.dinv, l.dinvf));
+  l.lam = 0.;
+  s->mg_ready = false;
+  return 0;
+}
+
+extern "C" int fm_heat_genmatrhs(fm_system *s, int L, int mode, const fm_heat_params *p) {
+  FM_CHECK(check_level(s, L));
+  if (!p || s->nq2 != 1 || s->nq1 != 0 || p->dt == 0. || p->alpha0 <= 0.) return FM_EINVAL;
+  FM_HIP(hipSetDevice(s->ctx->device));
+  Level &l = s->lev[L];
+  hipStream_t st = s->ctx->stream;
+  const fm_mesh &m = *s->mesh;
+  HeatArgs a;
+  a.alpha0 = p->alpha0; a.rhocp = p->rhocp; a.dt = p->dt; a.tc = p->time_coeff; a.source = p->source; a.upwind = p->upwind;
+  for (int d = 0; d < 3; d++) a.vel_const[d] = p->vel_const[d];
+  FM_HIP(hipMemsetAsync(l.A.v, 0, l.A.nnz * sizeof(double), st));
+  if (mode == 1) FM_HIP(hipMemsetAsync(l.b, 0, l.n * sizeof(double), st));
+  SysExtra &ex = extra(s);
+  for (int k = 0; k < l.ncolor; k++) {
+    int n = l.color_off[k + 1] - l.color_off[k];
+    FM_CHECK(fmk::heat_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldof, l.elpos, l.bmask, s->xyz,
+                                m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, l.coef, s->vel, ex.d_tab, a, mode,
+                                l.A.v, l.b));
+  }
+  FM_CHECK(after_assembly(s, L));
+  FM_HIP(hipStreamSynchronize(st));
+  return FM_OK;
+}
+
+extern "C" int fm_ns_genmatrhs(fm_system *s, int L, int mode, const fm_ns_params *p) {
+  FM_CHECK(check_level(s, L));
+  const fm_mesh &m = *s->mesh;
+  if (!p || s->nq2 != m.dim || s->nq1 != 1 || p->dt == 0. || p->IRe <= 0.) return FM_EINVAL;
+  FM_HIP(hipSetDevice(s->ctx->device));
+  Level &l = s->lev[L];
+  hipStream_t st = s->ctx->stream;
+  NsArgs a;
+  a.IRe = p->IRe; a.dt = p->dt; a.tc = p->time_coeff; a.upwind = p->upwind; a.IFr = p->IFr;
+  for (int d = 0; d < 3; d++) { a.dirg[d] = p->dirg[d]; a.src[d] = p->src[d]; }
+  FM_HIP(hipMemsetAsync(l.A.v, 0, l.A.nnz * sizeof(double), st));
+  if (mode == 1) FM_HIP(hipMemsetAsync(l.b, 0, l.n * sizeof(double), st));
+  SysExtra &ex = extra(s);
+  for (int k = 0; k < l.ncolor; k++) {
+    int n = l.color_off[k + 1] - l.color_off[k];
+    FM_CHECK(fmk::ns_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldof, l.elpos, l.bmask, s->xyz,
+                              m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, l.coef, ex.d_tab, a, mode, l.A.v, l.b));
+  }
+  FM_CHECK(after_assembly(s, L));
+  FM_HIP(hipStreamSynchronize(st));
+  return FM_OK;
+}
+
+// ===========================================================================
+// level operators
+// ===========================================================================
+extern "C" int fm_spmv(fm_system *s, int L, const double *xh, double *yh) {
+  FM_CHECK(check_level(s, L));
+  Level &l = s->lev[L];
+  hipStream_t st = s->ctx->stream;
+  FM_HIP(hipMemcpyAsync(l.t, xh, l.n * sizeof(double), hipMemcpyHostToDevice, st));
+  FM_CHECK(fmk::spmv(st, view<double>(l.A), l.t, l.d));
+  FM_HIP(hipMemcpyAsync(yh, l.d, l.n * sizeof(double), hipMemcpyDeviceToHost, st));
+  FM_HIP(hipStreamSynchronize(st));
+  return FM_OK;
+}
+extern "C" int fm_residual(fm_system *s, int L) {
+  FM_CHECK(check_level(s, L));
+  Level &l = s->lev[L];
+  FM_CHECK(fmk::resid(s->ctx->stream, view<double>(l.A), l.b, l.x, l.res));
+  FM_HIP(hipStreamSynchronize(s->ctx->stream));
+  return FM_OK;
+}
+extern "C" int fm_restrict(fm_system *s, int L) {
+  FM_CHECK(check_level(s, L));
+  if (L < 1) return FM_EINVAL;
+  Level &lc = s->lev[L - 1], &lf = s->lev[L];
+  FM_CHECK(fmk::spmv(s->ctx->stream, view<double>(lc.R), lf.res, lc.b));
+  FM_HIP(hipStreamSynchronize(s->ctx->stream));
+  return FM_OK;
+}
+extern "C" int fm_prolong(fm_system *s, int L) {
+  FM_CHECK(check_level(s, L));
+  if (L < 1) return FM_EINVAL;
+  Level &lc = s->lev[L - 1], &lf = s->lev[L];
+  FM_CHECK(fmk::spmv(s->ctx->stream, view<double>(lf.P), lc.x, lf.res));
+  FM_HIP(hipStreamSynchronize(s->ctx->stream));
+  return FM_OK;
+}
+
+extern "C" int fm_bench_spmv(fm_system *s, int L, int prec, int reps, double *sec) {
+  FM_CHECK(check_level(s, L));
+  if (reps < 1 || !sec) return FM_EINVAL;
+  Level &l = s->lev[L];
+  hipStream_t st = s->ctx->stream;
+  FM_CHECK(fmk::fill<double>(st, l.n, l.t, 1.0));
+  FM_CHECK(fmk::fill<float>(st, l.n, l.tf, 1.0f));
+  // warm-up
+  if (prec == 0) FM_CHECK(fmk::spmv(st, view<double>(l.A), l.t, l.d));
+  else FM_CHECK(fmk::spmv(st, view<float>(l.A), l.tf, l.df));
+  FM_HIP(hipEventRecord(s->ctx->ev0, st));
+  for (int k = 0; k < reps; k++) {
+    if (prec == 0) FM_CHECK(fmk::spmv(st, view<double>(l.A), l.t, l.d));
+    else FM_CHECK(fmk::spmv(st, view<float>(l.A), l.tf, l.df));
+  }
+  FM_HIP(hipEventRecord(s->ctx->ev1, st));
+  FM_HIP(hipEventSynchronize(s->ctx->ev1));
+  float ms = 0.f;
+  FM_HIP(hipEventElapsedTime(&ms, s->ctx->ev0, s->ctx->ev1));
+  *sec = ms * 1e-3 / reps;
+  return FM_OK;
+}

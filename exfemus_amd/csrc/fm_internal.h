@@ -1,0 +1,121 @@
+// fm_internal.h -- private structures of the MI355X FEMuS hot-path library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+#include "../../include/femus_hip.h"
+
+void fm_set_error(const std::string &msg);
+
+#define FM_HIP(call)                                                           \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess) {                                                    \
+      fm_set_error(std::string(#call) + ": " + hipGetErrorString(e_));         \
+      return FM_EHIP;                                                          \
+    }                                                                          \
+  } while (0)
+
+// Host-side mesh hierarchy (gencase result; src/MGMesh.C layout).
+struct fm_mesh {
+  int dim = 0, nlev = 0, n2 = 0, n1 = 0, nsides = 0, nb2 = 0, nch = 0, Ntop = 0;
+  int nx0[3] = {1, 1, 1};
+  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  std::vector<int> nq2, nq1, nel;
+  std::vector<double> xyz;                    // xyz[node + d*Ntop]
+  std::vector<std::vector<int>> conn, neigh, parent, egrid;  // egrid: element grid index (3 ints)
+};
+
+// Device CSR (int64 row pointers, int32 columns); values in FP64 and FP32.
+struct DevCSR {
+  int64_t nr = 0, nc = 0, nnz = 0;
+  int64_t *rp = nullptr;
+  int *ci = nullptr;
+  double *v = nullptr;
+  float *vf = nullptr;
+  // host copies (pattern + values as built)
+  std::vector<int64_t> h_rp;
+  std::vector<int> h_ci;
+  std::vector<double> h_v;
+};
+
+struct Level {
+  int64_t n = 0;           // _Dim[L]
+  int nel = 0;
+  DevCSR A, R, P;          // A[L]; R: coarse L <- fine L+1; P: fine L <- coarse L-1
+  // FP64 vectors
+  double *x = nullptr, *b = nullptr, *r = nullptr, *t = nullptr, *d = nullptr, *d2 = nullptr, *dinv = nullptr;
+  // FP32 vectors
+  float *xf = nullptr, *bf = nullptr, *rf = nullptr, *tf = nullptr, *df = nullptr, *d2f = nullptr, *dinvf = nullptr;
+  double lam = 0.0;        // lambda_max(D^-1 A) estimate
+  // assembly support
+  int *conn = nullptr;     // nel * n2 node ids
+  int *eldof = nullptr;    // nel * nloc dofs
+  int *elpos = nullptr;    // nel * nloc * nloc CSR positions in A
+  unsigned char *bmask = nullptr;  // nel: bit s set if side s on the boundary
+  int ncolor = 0;
+  std::vector<int> color_off; // element ranges per colour (elements reordered by colour)
+  int *color_elems = nullptr; // element ids grouped by colour
+  double *coef = nullptr;  // per-element k or nu (nullptr -> default)
+  // coarse direct solve (level 0)
+  double *Ainv = nullptr;
+  float *Ainvf = nullptr;
+  // Vanka local inverses (nel * nloc * nloc, FP64)
+  double *vinv = nullptr;
+  double *vwt = nullptr;   // 1 / multiplicity per dof
+  float *vwtf = nullptr;
+  // norms for the reference correction rule
+  double *nrm = nullptr;   // device scalars
+  // residual for transfers
+  double *res = nullptr;
+};
+
+struct fm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double *scratch = nullptr;    // reduction scratch (device)
+  double *h_scratch = nullptr;  // pinned host mirror
+  int nblocks_red = 1024;
+};
+
+struct fm_system {
+  fm_ctx *ctx = nullptr;
+  const fm_mesh *mesh = nullptr;
+  int nq2 = 0, nq1 = 0, nloc = 0;
+  std::vector<std::vector<int>> node_dof;   // per level, (nq2+nq1)*Ntop
+  std::vector<int64_t> Dim;
+  std::vector<int> bc0, bc1;                // top-level DoFs
+  int *d_bc0 = nullptr, *d_bc1 = nullptr;   // device copies (top DoFs)
+  int *d_dof_top = nullptr;                 // node_dof[top] on device
+  double *xyz = nullptr;                    // device coords
+  double *x_old = nullptr;                  // top-level x_old (device)
+  double *vel = nullptr;                    // heat advection field [dim*Ntop] or nullptr
+  std::vector<Level> lev;
+  bool initialized = false;
+  bool mg_ready = false;
+  int mg_smoother = -1;
+};
+
+// gencase.cpp
+int fm_build_box(int dim, const int nx0[3], const double lo[3], const double hi[3], int nlev, fm_mesh *m);
+void fm_init_dof(const fm_mesh &m, int L, int nq2, int nq1, std::vector<int> &node_dof, int64_t &ndof);
+void fm_pattern(const fm_mesh &m, int L, int nq2, int nq1, const std::vector<int> &dofL,
+                std::vector<int64_t> &rp, std::vector<int> &ci);
+void fm_transfer(const fm_mesh &m, int L, int nq2, int nq1, bool rest, const std::vector<int> &dof_row,
+                 const std::vector<int> &dof_col, const std::vector<int> *dof_top, const std::vector<int> *bc0,
+                 std::vector<int64_t> &rp, std::vector<int> &ci, std::vector<double> &v);
+void fm_elem_dofs(const fm_mesh &m, int L, int e, int nq2, int nq1, const std::vector<int> &dof, int *out);
+extern const int fm_hex27_ref[27][3];
+extern const int fm_quad9_ref[9][2];
+extern const int fm_face2d[4][3];
+extern const int fm_face3d[6][9];
+
+// kernels.hip launchers
+namespace fmk {
+int spmv(hipStream_t s, const DevCSR &A, const double *x, double *y);
+int spmvf(hipStream_t s, const DevCSR &A, const float *x, float *y);
+int resid(hipStream_t s, const DevCSR &A, const double *b, const double *x, double *r);
+int residf(hipStream_t s, const DevCSR &A, const float *b, const float *x, float *r);
+}

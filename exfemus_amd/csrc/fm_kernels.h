@@ -1,0 +1,73 @@
+// fm_kernels.h -- launchers of the gfx950 kernels (kernels.hip, assemble.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+template <typename T>
+struct CsrView {
+  int64_t nr = 0;
+  const int64_t *rp = nullptr;
+  const int *ci = nullptr;
+  const T *v = nullptr;
+  int lpr = 16;  // lanes per row
+};
+
+// Reference-element tables (src/MGFE.C:195-924 layout: phi[s*ng+qp],
+// dphi[(s*ng+qp) + d*ns*ng]); face tables are the (dim-1) Q2 tables.
+struct FeTab {
+  double phi2[27 * 27];
+  double dphi2[3 * 27 * 27];
+  double w[27];
+  double phi1[8 * 27];
+  double bphi2[9 * 9];
+  double bdphi2[2 * 9 * 9];
+  double bw[9];
+};
+
+struct HeatArgs {
+  double alpha0, rhocp, dt, tc, source, upwind;
+  double vel_const[3];
+};
+struct NsArgs {
+  double IRe, dt, tc, upwind, IFr;
+  double dirg[3], src[3];
+};
+
+namespace fmk {
+int lanes_for(int64_t nnz, int64_t n);
+int spmv(hipStream_t s, const CsrView<double> &A, const double *x, double *y);
+int spmv(hipStream_t s, const CsrView<float> &A, const float *x, float *y);
+int resid(hipStream_t s, const CsrView<double> &A, const double *b, const double *x, double *r);
+int resid(hipStream_t s, const CsrView<float> &A, const float *b, const float *x, float *r);
+int cheb_step(hipStream_t s, const CsrView<double> &A, double *r, double *x, const double *dold, double *dnew,
+              const double *dinv, double c1, double c2);
+int cheb_step(hipStream_t s, const CsrView<float> &A, float *r, float *x, const float *dold, float *dnew,
+              const float *dinv, float c1, float c2);
+int addmul(hipStream_t s, const CsrView<double> &P, const double *xc, double *x, double alpha, const double *cond);
+int addmul(hipStream_t s, const CsrView<float> &P, const float *xc, float *x, float alpha, const double *cond);
+int jacobi_step(hipStream_t s, const CsrView<double> &A, const double *b, const double *x, double *xn,
+                const double *dinv, double omega);
+int jacobi_step(hipStream_t s, const CsrView<float> &A, const float *b, const float *x, float *xn,
+                const float *dinv, float omega);
+template <typename T> int fill(hipStream_t s, int64_t n, T *x, T a);
+template <typename T> int axpby(hipStream_t s, int64_t n, T a, T *x, T b, const T *y);
+template <typename T> int cheb_first(hipStream_t s, int64_t n, const T *dinv, const T *r, T *d, T *x, T a);
+int to_f32(hipStream_t s, int64_t n, const double *a, float *b, const double *scale);
+int add_f32(hipStream_t s, int64_t n, double *x, const float *e, const double *scale);
+template <typename T> int dot_dev(hipStream_t s, int64_t n, const T *x, const T *y, double *part, double *out, int do_sqrt);
+int ratio(hipStream_t s, const double *a, const double *b, double *out);
+template <typename T> int dense_mv(hipStream_t s, int n, const T *M, const T *b, T *y);
+int diag_inv(hipStream_t s, int64_t n, const int64_t *rp, const int *ci, const double *v, double *d64, float *d32);
+int cast_vals(hipStream_t s, int64_t n, const double *v, float *vf);
+int elpos(hipStream_t s, int nel, int nloc, const int *eldof, const int64_t *rp, const int *ci, int *pos);
+
+// assemble.hip
+int heat_assemble(hipStream_t s, int dim, int nel_color, const int *elems, const int *conn, const int *eldof,
+                  const int *elpos, const unsigned char *bmask, const double *xyz, int Ntop, const int *dof_top,
+                  const int *bc0, const int *bc1, const double *x_old, const double *coef, const double *vel,
+                  const FeTab *tab, HeatArgs a, int mode, double *val, double *rhs);
+int ns_assemble(hipStream_t s, int dim, int nel_color, const int *elems, const int *conn, const int *eldof,
+                const int *elpos, const unsigned char *bmask, const double *xyz, int Ntop, const int *dof_top,
+                const int *bc0, const int *bc1, const double *x_old, const double *coef,
+                const FeTab *tab, NsArgs a, int mode, double *val, double *rhs);
+}  // namespace fmk

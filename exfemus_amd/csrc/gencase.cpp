@@ -1,0 +1,348 @@
+// gencase.cpp -- host construction of the multigrid mesh hierarchy and of the
+// per-level operators' patterns for the MI355X FEMuS hot path.
+//
+// Mirrors the reference preprocessor and DoF setup:
+//   USER_APPL/gencase/src/MGGenCase.C:98-181   build_square/build_cube + uniformly_refine
+//   USER_APPL/gencase/src/MGGenCase.C:534-620  element order (lev+nlev*subdom, libMesh id),
+//                                              node order (var, levp, lev, subdom, libMesh id)
+//   USER_APPL/gencase/src/MGGenCase.C:1781-1853 sparsity (element couplings)
+//   USER_APPL/gencase/src/MGGenCase.C:2308-2390 prolongation (last fine element wins)
+//   USER_APPL/gencase/src/MGGenCase.C:2452-2566 restriction (first found, |E|>1e-6)
+//   src/MGSolverDA.C:880-948 init_dof; :2400-2430 ReadRest Dirichlet mask
+// libMesh ids are emulated: level-0 elements k,j,i (i fastest), children
+// appended parent by parent, node ids by first encounter over that element
+// list.  Rows of A are stored with sorted columns (the order PETSc's AIJ and
+// LASPACK's Q_SortEl both impose before any product).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include "fm_internal.h"
+
+const int fm_quad9_ref[9][2] = {{-1,-1},{1,-1},{1,1},{-1,1},{0,-1},{1,0},{0,1},{-1,0},{0,0}};
+const int fm_hex27_ref[27][3] = {
+  {-1,-1,-1},{1,-1,-1},{1,1,-1},{-1,1,-1},{-1,-1,1},{1,-1,1},{1,1,1},{-1,1,1},
+  {0,-1,-1},{1,0,-1},{0,1,-1},{-1,0,-1},{-1,-1,0},{1,-1,0},{1,1,0},{-1,1,0},
+  {0,-1,1},{1,0,1},{0,1,1},{-1,0,1},{0,0,-1},{0,-1,0},{1,0,0},{0,1,0},{-1,0,0},{0,0,1},{0,0,0}};
+// side -> local face nodes (src/MGGeomEl.C:172-177, 310-317)
+const int fm_face2d[4][3] = {{0,1,4},{1,2,5},{2,3,6},{3,0,7}};
+const int fm_face3d[6][9] = {
+  {0,1,2,3,8,9,10,11,20},{0,1,5,4,8,13,16,12,21},{1,2,6,5,9,14,17,13,22},
+  {2,3,7,6,10,15,18,14,23},{3,0,4,7,11,12,19,15,24},{4,5,6,7,16,17,18,19,25}};
+
+static inline int refc(int dim, int k, int d) { return dim == 2 ? fm_quad9_ref[k][d] : fm_hex27_ref[k][d]; }
+
+int fm_build_box(int dim, const int nx0[3], const double lo[3], const double hi[3], int nlev, fm_mesh *m) {
+  if ((dim != 2 && dim != 3) || nlev < 1 || nlev > 12) return FM_EINVAL;
+  m->dim = dim; m->nlev = nlev;
+  m->n2 = dim == 2 ? 9 : 27; m->n1 = dim == 2 ? 4 : 8; m->nsides = 2 * dim;
+  m->nb2 = dim == 2 ? 3 : 9; m->nch = 1 << dim;
+  int64_t G[3] = {1, 1, 1}, gtot = 1;
+  for (int d = 0; d < 3; d++) {
+    m->nx0[d] = d < dim ? nx0[d] : 1;
+    if (m->nx0[d] < 1) return FM_EINVAL;
+    m->lo[d] = d < dim ? lo[d] : 0.; m->hi[d] = d < dim ? hi[d] : 0.;
+    if (d < dim) G[d] = (int64_t)2 * m->nx0[d] * (1LL << (nlev - 1)) + 1;
+    gtot *= G[d];
+  }
+  if (gtot > (int64_t)1 << 31) return FM_EINVAL;
+  m->nel.assign(nlev, 0);
+  m->egrid.assign(nlev, {});
+  m->parent.assign(nlev, {});
+  // element lists: level 0 lexicographic, then children parent by parent
+  {
+    int n0 = m->nx0[0] * m->nx0[1] * m->nx0[2];
+    m->nel[0] = n0;
+    m->egrid[0].resize(3 * (size_t)n0);
+    m->parent[0].assign(n0, -1);
+    int e = 0;
+    for (int k = 0; k < m->nx0[2]; k++)
+      for (int j = 0; j < m->nx0[1]; j++)
+        for (int i = 0; i < m->nx0[0]; i++, e++) {
+          m->egrid[0][3 * e] = i; m->egrid[0][3 * e + 1] = j; m->egrid[0][3 * e + 2] = k;
+        }
+  }
+  for (int L = 1; L < nlev; L++) {
+    int np = m->nel[L - 1], nc = np * m->nch;
+    m->nel[L] = nc;
+    m->egrid[L].resize(3 * (size_t)nc);
+    m->parent[L].resize(nc);
+    for (int p = 0; p < np; p++)
+      for (int c = 0; c < m->nch; c++) {
+        int ce = p * m->nch + c;
+        for (int d = 0; d < 3; d++)
+          m->egrid[L][3 * ce + d] = d < dim ? 2 * m->egrid[L - 1][3 * p + d] + ((c >> d) & 1) : 0;
+        m->parent[L][ce] = p;
+      }
+  }
+  // node of element e (level L), local k -> top-grid linear index
+  auto gidx = [&](int L, int e, int k) -> int64_t {
+    int S = 1 << (nlev - L);
+    int64_t g = 0, stride = 1;
+    for (int d = 0; d < dim; d++) {
+      int64_t I = (int64_t)m->egrid[L][3 * e + d] * S + (refc(dim, k, d) + 1) * (S / 2);
+      g += I * stride; stride *= G[d];
+    }
+    return g;
+  };
+  // first-encounter ids, level of first appearance, first level as a vertex
+  std::vector<int> lid(gtot, -1);
+  std::vector<unsigned char> glev(gtot, (unsigned char)nlev), glevp(gtot, (unsigned char)nlev);
+  std::vector<int64_t> g_of_lid;
+  g_of_lid.reserve(gtot);
+  for (int L = 0; L < nlev; L++)
+    for (int e = 0; e < m->nel[L]; e++)
+      for (int k = 0; k < m->n2; k++) {
+        int64_t g = gidx(L, e, k);
+        if (lid[g] < 0) { lid[g] = (int)g_of_lid.size(); g_of_lid.push_back(g); glev[g] = (unsigned char)L; }
+        if (k < m->n1 && glevp[g] > L) glevp[g] = (unsigned char)L;
+      }
+  int N = (int)g_of_lid.size();
+  m->Ntop = N;
+  // stable bucket sort by key = levp + (nlev+1)*lev over nodes in id order
+  int nk = (nlev + 1) * (nlev + 1);
+  std::vector<int> cnt(nk + 1, 0);
+  for (int i = 0; i < N; i++) { int64_t g = g_of_lid[i]; cnt[glevp[g] + (nlev + 1) * glev[g] + 1]++; }
+  for (int k = 0; k < nk; k++) cnt[k + 1] += cnt[k];
+  std::vector<int> gnew(gtot, -1);
+  m->xyz.assign((size_t)dim * N, 0.);
+  m->nq2.assign(nlev, 0); m->nq1.assign(nlev, 0);
+  for (int i = 0; i < N; i++) {
+    int64_t g = g_of_lid[i];
+    int nn = cnt[glevp[g] + (nlev + 1) * glev[g]]++;
+    gnew[g] = nn;
+    int64_t I[3] = {g % G[0], (g / G[0]) % G[1], g / (G[0] * G[1])};
+    for (int d = 0; d < dim; d++)
+      m->xyz[nn + (size_t)d * N] = m->lo[d] + (m->hi[d] - m->lo[d]) * (double)I[d] / (double)(G[d] - 1);
+    for (int L = glev[g]; L < nlev; L++) m->nq2[L]++;
+    for (int L = glevp[g]; L < nlev; L++) m->nq1[L]++;
+  }
+  // connectivity and face neighbours per level
+  m->conn.assign(nlev, {});
+  m->neigh.assign(nlev, {});
+  static const int s2[4][2] = {{1, -1}, {0, 1}, {1, 1}, {0, -1}};
+  static const int s3[6][2] = {{2, -1}, {1, -1}, {0, 1}, {1, 1}, {0, -1}, {2, 1}};
+  for (int L = 0; L < nlev; L++) {
+    int ne = m->nel[L];
+    m->conn[L].resize((size_t)ne * m->n2);
+    m->neigh[L].resize((size_t)ne * m->nsides);
+    int nE[3] = {1, 1, 1};
+    for (int d = 0; d < dim; d++) nE[d] = m->nx0[d] << L;
+    std::vector<int> at((size_t)nE[0] * nE[1] * nE[2]);
+    for (int e = 0; e < ne; e++) {
+      const int *c = &m->egrid[L][3 * e];
+      at[c[0] + (size_t)nE[0] * (c[1] + (size_t)nE[1] * c[2])] = e;
+      for (int k = 0; k < m->n2; k++) m->conn[L][(size_t)e * m->n2 + k] = gnew[gidx(L, e, k)];
+    }
+    for (int e = 0; e < ne; e++) {
+      const int *c = &m->egrid[L][3 * e];
+      for (int s = 0; s < m->nsides; s++) {
+        int ax = dim == 2 ? s2[s][0] : s3[s][0], sg = dim == 2 ? s2[s][1] : s3[s][1];
+        int cc[3] = {c[0], c[1], c[2]};
+        cc[ax] += sg;
+        int nb = -1;
+        if (cc[ax] >= 0 && cc[ax] < nE[ax]) nb = at[cc[0] + (size_t)nE[0] * (cc[1] + (size_t)nE[1] * cc[2])];
+        m->neigh[L][(size_t)e * m->nsides + s] = nb;
+      }
+    }
+  }
+  return FM_OK;
+}
+
+void fm_init_dof(const fm_mesh &m, int L, int nq2, int nq1, std::vector<int> &node_dof, int64_t &ndof) {
+  int N = m.Ntop;
+  node_dof.assign((size_t)N * (nq2 + nq1), -1);
+  int64_t c = 0;
+  for (int iv = 0; iv < nq2; iv++)
+    for (int k = 0; k < m.nq2[L]; k++) node_dof[k + (size_t)iv * N] = (int)c++;
+  for (int iv = 0; iv < nq1; iv++)
+    for (int k = 0; k < m.nq1[L]; k++) node_dof[k + (size_t)(nq2 + iv) * N] = (int)c++;
+  ndof = c;
+}
+
+void fm_elem_dofs(const fm_mesh &m, int L, int e, int nq2, int nq1, const std::vector<int> &dof, int *out) {
+  int N = m.Ntop, c = 0;
+  const int *cn = &m.conn[L][(size_t)e * m.n2];
+  for (int iv = 0; iv < nq2; iv++) for (int k = 0; k < m.n2; k++) out[c++] = dof[cn[k] + (size_t)iv * N];
+  for (int iv = 0; iv < nq1; iv++) for (int k = 0; k < m.n1; k++) out[c++] = dof[cn[k] + (size_t)(nq2 + iv) * N];
+}
+
+// node -> incident elements (in element order) for level L
+static void node_elems(const fm_mesh &m, int L, int nodes, std::vector<int64_t> &off, std::vector<int> &els) {
+  off.assign(nodes + 1, 0);
+  int ne = m.nel[L];
+  for (int e = 0; e < ne; e++)
+    for (int k = 0; k < m.n2; k++) off[m.conn[L][(size_t)e * m.n2 + k] + 1]++;
+  for (int i = 0; i < nodes; i++) off[i + 1] += off[i];
+  els.resize(off[nodes]);
+  std::vector<int64_t> f(off.begin(), off.end() - 1);
+  for (int e = 0; e < ne; e++)
+    for (int k = 0; k < m.n2; k++) els[f[m.conn[L][(size_t)e * m.n2 + k]]++] = e;
+}
+
+void fm_pattern(const fm_mesh &m, int L, int nq2, int nq1, const std::vector<int> &dofL,
+                std::vector<int64_t> &rp, std::vector<int> &ci) {
+  int N = m.Ntop, nv = nq2 + nq1, nloc = nq2 * m.n2 + nq1 * m.n1;
+  int64_t n = (int64_t)nq2 * m.nq2[L] + (int64_t)nq1 * m.nq1[L];
+  std::vector<int64_t> off;
+  std::vector<int> els;
+  node_elems(m, L, m.nq2[L], off, els);
+  // dof -> node
+  std::vector<int> dnode(n);
+  for (int iv = 0; iv < nv; iv++) {
+    int nn = iv < nq2 ? m.nq2[L] : m.nq1[L];
+    for (int k = 0; k < nn; k++) dnode[dofL[k + (size_t)iv * N]] = k;
+  }
+  // all rows of one node share the same column set
+  std::vector<int64_t> len(n + 1, 0);
+  int maxlen = 0;
+  std::vector<std::vector<int>> cols_of_node_tmp;  // unused when streaming
+  // first pass: count (per node) by building the sorted column list
+  int nnodes = m.nq2[L];
+  std::vector<int> node_len(nnodes, 0);
+#pragma omp parallel
+  {
+    std::vector<int> buf;
+    std::vector<int> ed(nloc);
+#pragma omp for schedule(dynamic, 1024)
+    for (int k = 0; k < nnodes; k++) {
+      buf.clear();
+      for (int64_t q = off[k]; q < off[k + 1]; q++) {
+        fm_elem_dofs(m, L, els[q], nq2, nq1, dofL, ed.data());
+        buf.insert(buf.end(), ed.begin(), ed.end());
+      }
+      std::sort(buf.begin(), buf.end());
+      node_len[k] = (int)(std::unique(buf.begin(), buf.end()) - buf.begin());
+    }
+  }
+  for (int64_t i = 0; i < n; i++) {
+    int k = dnode[i];
+    len[i + 1] = node_len[k];
+    if (node_len[k] > maxlen) maxlen = node_len[k];
+  }
+  for (int64_t i = 0; i < n; i++) len[i + 1] += len[i];
+  rp = len;
+  ci.resize(rp[n]);
+  // second pass: fill each dof row of node k with the node's column set
+#pragma omp parallel
+  {
+    std::vector<int> buf;
+    std::vector<int> ed(nloc);
+#pragma omp for schedule(dynamic, 1024)
+    for (int k = 0; k < nnodes; k++) {
+      buf.clear();
+      for (int64_t q = off[k]; q < off[k + 1]; q++) {
+        fm_elem_dofs(m, L, els[q], nq2, nq1, dofL, ed.data());
+        buf.insert(buf.end(), ed.begin(), ed.end());
+      }
+      std::sort(buf.begin(), buf.end());
+      buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
+      for (int iv = 0; iv < nv; iv++) {
+        int nn = iv < nq2 ? m.nq2[L] : m.nq1[L];
+        if (k >= nn) continue;
+        int row = dofL[k + (size_t)iv * N];
+        std::copy(buf.begin(), buf.end(), ci.begin() + rp[row]);
+      }
+    }
+  }
+}
+
+// Child embedding value E[c][i][j] = phi_j(child node i) for Q2 (kind 2) / Q1 (kind 1)
+static double lag2(int node, double x) { return node < 0 ? .5 * x * (x - 1.) : (node > 0 ? .5 * x * (x + 1.) : 1. - x * x); }
+static double lag1(int node, double x) { return node < 0 ? .5 * (1. - x) : .5 * (1. + x); }
+static void embed_tables(int dim, int kind, std::vector<double> &E) {
+  int ns = kind == 2 ? (dim == 2 ? 9 : 27) : (dim == 2 ? 4 : 8), nch = 1 << dim;
+  E.assign((size_t)nch * ns * ns, 0.);
+  for (int c = 0; c < nch; c++)
+    for (int i = 0; i < ns; i++) {
+      double xp[3];
+      for (int d = 0; d < dim; d++) xp[d] = (((c >> d) & 1) ? 0. : -1.) + (refc(dim, i, d) + 1) * 0.5;
+      for (int j = 0; j < ns; j++) {
+        double v = 1.;
+        for (int d = 0; d < dim; d++) v *= kind == 2 ? lag2(refc(dim, j, d), xp[d]) : lag1(refc(dim, j, d), xp[d]);
+        E[((size_t)c * ns + i) * ns + j] = v;
+      }
+    }
+}
+
+// Transfer operators in DoF space.  rest=true: R (coarse L <- fine L+1), rows
+// masked by bc0 % 2 of the coarse node's top DoF; rest=false: P (fine L <- coarse L-1).
+void fm_transfer(const fm_mesh &m, int L, int nq2, int nq1, bool rest, const std::vector<int> &dof_row,
+                 const std::vector<int> &dof_col, const std::vector<int> *dof_top, const std::vector<int> *bc0,
+                 std::vector<int64_t> &rp, std::vector<int> &ci, std::vector<double> &v) {
+  int N = m.Ntop, dim = m.dim, nch = m.nch, nv = nq2 + nq1;
+  int64_t nrows = (int64_t)nq2 * m.nq2[L] + (int64_t)nq1 * m.nq1[L];
+  // per node-kind block rows: list of (col node, value)
+  std::vector<std::vector<std::pair<int, double>>> blk[2];
+  for (int t = 0; t < 2; t++) {
+    if ((t == 0 && nq2 == 0) || (t == 1 && nq1 == 0)) continue;
+    int kind = t == 0 ? 2 : 1, ns = kind == 2 ? m.n2 : m.n1;
+    int nr = kind == 2 ? m.nq2[L] : m.nq1[L];
+    std::vector<double> E;
+    embed_tables(dim, kind, E);
+    blk[t].assign(nr, {});
+    if (!rest) {
+      // last fine element (in level order) containing the node defines its row
+      std::vector<int> last(nr, -1), lastk(nr, -1);
+      for (int e = 0; e < m.nel[L]; e++)
+        for (int kk = 0; kk < ns; kk++) { int r = m.conn[L][(size_t)e * m.n2 + kk]; last[r] = e; lastk[r] = kk; }
+      for (int r = 0; r < nr; r++) {
+        int e = last[r], kk = lastk[r], par = m.parent[L][e], ch = e - par * nch;
+        for (int kj = 0; kj < ns; kj++) {
+          double val = E[((size_t)ch * ns + kk) * ns + kj];
+          if (std::fabs(val) > 1.e-8) blk[t][r].push_back({m.conn[L - 1][(size_t)par * m.n2 + kj], val});
+        }
+      }
+    } else {
+      // row r collects, over the coarse elements containing r (in element
+      // order), their children's nodes in order: first occurrence kept.
+      std::vector<int64_t> off;
+      std::vector<int> els;
+      node_elems(m, L, m.nq2[L], off, els);
+      int nfine = m.nq2[L + 1];
+#pragma omp parallel
+      {
+        std::vector<int> mark(nfine, -1);
+#pragma omp for schedule(dynamic, 256)
+        for (int r = 0; r < nr; r++) {
+          auto &row = blk[t][r];
+          for (int64_t q = off[r]; q < off[r + 1]; q++) {
+            int e = els[q], kk = -1;
+            for (int k = 0; k < ns; k++) if (m.conn[L][(size_t)e * m.n2 + k] == r) kk = k;
+            if (kk < 0) continue;  // r is not among this element's first ns nodes (Q1 block)
+            for (int ich = 0; ich < nch; ich++) {
+              int ch = e * nch + ich;
+              for (int kj = 0; kj < ns; kj++) {
+                double val = E[((size_t)ich * ns + kj) * ns + kk];
+                if (std::fabs(val) <= 1.e-6) continue;
+                int col = m.conn[L + 1][(size_t)ch * m.n2 + kj];
+                if (mark[col] == r) continue;
+                mark[col] = r;
+                row.push_back({col, val});
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+  rp.assign(nrows + 1, 0);
+  for (int iv = 0; iv < nv; iv++) {
+    int t = iv < nq2 ? 0 : 1;
+    int nr = t == 0 ? m.nq2[L] : m.nq1[L];
+    for (int i = 0; i < nr; i++) rp[dof_row[i + (size_t)iv * N] + 1] = (int64_t)blk[t][i].size();
+  }
+  for (int64_t i = 0; i < nrows; i++) rp[i + 1] += rp[i];
+  ci.resize(rp[nrows]);
+  v.resize(rp[nrows]);
+  for (int iv = 0; iv < nv; iv++) {
+    int t = iv < nq2 ? 0 : 1;
+    int nr = t == 0 ? m.nq2[L] : m.nq1[L];
+    for (int i = 0; i < nr; i++) {
+      int row = dof_row[i + (size_t)iv * N];
+      double mask = 1.;
+      if (rest) mask = (double)((*bc0)[(*dof_top)[i + (size_t)iv * N]] % 2);
+      int64_t q = rp[row];
+      for (auto &pr : blk[t][i]) { ci[q] = dof_col[pr.first + (size_t)iv * N]; v[q] = mask * pr.second; q++; }
+    }
+  }
+}

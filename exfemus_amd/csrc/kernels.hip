@@ -1,0 +1,319 @@
+// kernels.hip -- gfx950 kernels of the FEMuS hot path (wave64, CDNA4).
+//
+//  * CSR SpMV family (y = Ax, r = b - Ax, fused Chebyshev step, fused
+//    prolongation-correction): LPR lanes of a 64-wide wavefront own one row,
+//    partial sums are reduced with cross-lane shuffles.  Rows are
+//    independent; values/columns stream once from HBM (the roofline bound).
+//    Replaces LaspackVectorM::resid / Mul_QV (contrib/matrix/laspack_vectorM.C:147-189,
+//    contrib/laspack/operats.c:782-1047).
+//  * Element assembly of the heat (Q2) and Navier-Stokes (Q2-Q1) systems:
+//    one workgroup per element, shape tables and global gradients in LDS,
+//    elements processed colour by colour (2^dim colours: no two elements of
+//    a colour share a node) so the scatter into CSR needs no atomics.
+//    Restates MGSolT::GenMatRhs / MGSolNS::GenMatRhs
+//    (template_appl/nse/SRC/MGSolverT.C:93-525, MGSolverNS.C:108-717).
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "fm_kernels.h"
+
+#define WAVE 64
+
+// ---------------------------------------------------------------------------
+// SpMV core with epilogues
+// ---------------------------------------------------------------------------
+enum { EP_Y = 0, EP_RES = 1, EP_CHEB = 2, EP_ADD = 3, EP_JAC = 4 };
+
+template <typename T>
+struct EpArgs {
+  T *y;          // EP_Y: y; EP_RES: r; EP_CHEB: r (in/out); EP_ADD: x (in/out); EP_JAC: x (in/out)
+  const T *b;    // EP_RES / EP_JAC: b
+  T *x;          // EP_CHEB: x (in/out)
+  T *dnew;       // EP_CHEB: new direction
+  const T *dold; // EP_CHEB: old direction (also the gathered vector)
+  const T *dinv; // EP_CHEB / EP_JAC
+  T c1, c2;      // EP_CHEB: dnew = c1*dold + c2*dinv*r ; EP_ADD: x += c1*(Px) ; EP_JAC: omega
+  const double *cond;  // EP_ADD: if non-null, apply only when cond[0] < cond[1] (reference rule)
+};
+
+template <int LPR, typename T, int EP>
+__global__ __launch_bounds__(256) void k_spmv(int64_t nrows, const int64_t *__restrict__ rp,
+                                              const int *__restrict__ ci, const T *__restrict__ v,
+                                              const T *__restrict__ xg, EpArgs<T> ep) {
+  const int lane = threadIdx.x % LPR;
+  const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / LPR;
+  if (row >= nrows) return;
+  const int64_t q0 = rp[row], q1 = rp[row + 1];
+  T sum = 0;
+  for (int64_t q = q0 + lane; q < q1; q += LPR) sum += v[q] * xg[ci[q]];
+#pragma unroll
+  for (int off = LPR / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, LPR);
+  if (lane != 0) return;
+  if (EP == EP_Y) {
+    ep.y[row] = sum;
+  } else if (EP == EP_RES) {
+    ep.y[row] = ep.b[row] - sum;
+  } else if (EP == EP_CHEB) {
+    T r = ep.y[row] - sum;
+    ep.y[row] = r;
+    T dn = ep.c1 * ep.dold[row] + ep.c2 * ep.dinv[row] * r;
+    ep.dnew[row] = dn;
+    ep.x[row] += dn;
+  } else if (EP == EP_ADD) {
+    if (ep.cond == nullptr || ep.cond[0] < ep.cond[1]) ep.y[row] += ep.c1 * sum;
+  } else if (EP == EP_JAC) {
+    // x_new = x + omega * dinv * (b - A x); written to dnew (x read via xg)
+    ep.dnew[row] = xg[row] + ep.c1 * ep.dinv[row] * (ep.b[row] - sum);
+  }
+}
+
+template <typename T, int EP>
+static int launch_spmv(hipStream_t s, int lpr, int64_t n, const int64_t *rp, const int *ci, const T *v,
+                       const T *x, const EpArgs<T> &ep) {
+  if (n <= 0) return 0;
+  const int block = 256;
+  int64_t threads = n * lpr;
+  dim3 grid((unsigned)((threads + block - 1) / block));
+  switch (lpr) {
+    case 4: hipLaunchKernelGGL((k_spmv<4, T, EP>), grid, dim3(block), 0, s, n, rp, ci, v, x, ep); break;
+    case 8: hipLaunchKernelGGL((k_spmv<8, T, EP>), grid, dim3(block), 0, s, n, rp, ci, v, x, ep); break;
+    case 16: hipLaunchKernelGGL((k_spmv<16, T, EP>), grid, dim3(block), 0, s, n, rp, ci, v, x, ep); break;
+    case 32: hipLaunchKernelGGL((k_spmv<32, T, EP>), grid, dim3(block), 0, s, n, rp, ci, v, x, ep); break;
+    default: hipLaunchKernelGGL((k_spmv<64, T, EP>), grid, dim3(block), 0, s, n, rp, ci, v, x, ep); break;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+namespace fmk {
+
+int lanes_for(int64_t nnz, int64_t n) {
+  double avg = n > 0 ? (double)nnz / (double)n : 1.0;
+  if (avg <= 6) return 4;
+  if (avg <= 14) return 8;
+  if (avg <= 40) return 16;
+  if (avg <= 100) return 32;
+  return 64;
+}
+
+template <typename T> static int spmv_t(hipStream_t s, const CsrView<T> &A, const T *x, T *y) {
+  EpArgs<T> ep{}; ep.y = y;
+  return launch_spmv<T, EP_Y>(s, A.lpr, A.nr, A.rp, A.ci, A.v, x, ep);
+}
+template <typename T> static int resid_t(hipStream_t s, const CsrView<T> &A, const T *b, const T *x, T *r) {
+  EpArgs<T> ep{}; ep.y = r; ep.b = b;
+  return launch_spmv<T, EP_RES>(s, A.lpr, A.nr, A.rp, A.ci, A.v, x, ep);
+}
+template <typename T> static int cheb_t(hipStream_t s, const CsrView<T> &A, T *r, T *x, const T *dold, T *dnew,
+                                        const T *dinv, T c1, T c2) {
+  EpArgs<T> ep{}; ep.y = r; ep.x = x; ep.dold = dold; ep.dnew = dnew; ep.dinv = dinv; ep.c1 = c1; ep.c2 = c2;
+  return launch_spmv<T, EP_CHEB>(s, A.lpr, A.nr, A.rp, A.ci, A.v, dold, ep);
+}
+template <typename T> static int addmul_t(hipStream_t s, const CsrView<T> &P, const T *xc, T *x, T alpha,
+                                          const double *cond) {
+  EpArgs<T> ep{}; ep.y = x; ep.c1 = alpha; ep.cond = cond;
+  return launch_spmv<T, EP_ADD>(s, P.lpr, P.nr, P.rp, P.ci, P.v, xc, ep);
+}
+template <typename T> static int jacobi_t(hipStream_t s, const CsrView<T> &A, const T *b, const T *x, T *xn,
+                                          const T *dinv, T omega) {
+  EpArgs<T> ep{}; ep.b = b; ep.dnew = xn; ep.dinv = dinv; ep.c1 = omega;
+  return launch_spmv<T, EP_JAC>(s, A.lpr, A.nr, A.rp, A.ci, A.v, x, ep);
+}
+
+int spmv(hipStream_t s, const CsrView<double> &A, const double *x, double *y) { return spmv_t(s, A, x, y); }
+int spmv(hipStream_t s, const CsrView<float> &A, const float *x, float *y) { return spmv_t(s, A, x, y); }
+int resid(hipStream_t s, const CsrView<double> &A, const double *b, const double *x, double *r) { return resid_t(s, A, b, x, r); }
+int resid(hipStream_t s, const CsrView<float> &A, const float *b, const float *x, float *r) { return resid_t(s, A, b, x, r); }
+int cheb_step(hipStream_t s, const CsrView<double> &A, double *r, double *x, const double *dold, double *dnew,
+              const double *dinv, double c1, double c2) { return cheb_t(s, A, r, x, dold, dnew, dinv, c1, c2); }
+int cheb_step(hipStream_t s, const CsrView<float> &A, float *r, float *x, const float *dold, float *dnew,
+              const float *dinv, float c1, float c2) { return cheb_t(s, A, r, x, dold, dnew, dinv, c1, c2); }
+int addmul(hipStream_t s, const CsrView<double> &P, const double *xc, double *x, double alpha, const double *cond) {
+  return addmul_t(s, P, xc, x, alpha, cond);
+}
+int addmul(hipStream_t s, const CsrView<float> &P, const float *xc, float *x, float alpha, const double *cond) {
+  return addmul_t(s, P, xc, x, alpha, cond);
+}
+int jacobi_step(hipStream_t s, const CsrView<double> &A, const double *b, const double *x, double *xn,
+                const double *dinv, double omega) { return jacobi_t(s, A, b, x, xn, dinv, omega); }
+int jacobi_step(hipStream_t s, const CsrView<float> &A, const float *b, const float *x, float *xn,
+                const float *dinv, float omega) { return jacobi_t(s, A, b, x, xn, dinv, omega); }
+
+// ---------------------------------------------------------------------------
+// vector kernels
+// ---------------------------------------------------------------------------
+template <typename T> __global__ void k_fill(int64_t n, T *x, T a) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) x[i] = a;
+}
+// x = a*x + b*y  (both same type)
+template <typename T> __global__ void k_axpby(int64_t n, T a, T *x, T b, const T *y) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = a * x[i] + b * y[i];
+}
+// y = a * dinv .* x / theta   (first Chebyshev direction; also x += y)
+template <typename T> __global__ void k_cheb_first(int64_t n, const T *dinv, const T *r, T *d, T *x, T a) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    T di = a * dinv[i] * r[i];
+    d[i] = di;
+    x[i] += di;
+  }
+}
+__global__ void k_f64_to_f32(int64_t n, const double *a, float *b) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) b[i] = (float)a[i];
+}
+// x64 += (double) e32
+__global__ void k_add_f32(int64_t n, double *x, const float *e) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) x[i] += (double)e[i];
+}
+// scaled conversion: b32 = (float)(s * r64) with s = 1 / scale[0]
+__global__ void k_f64_to_f32_scaled(int64_t n, const double *a, float *b, const double *scale) {
+  double sinv = scale[0] > 0. ? 1.0 / scale[0] : 1.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) b[i] = (float)(a[i] * sinv);
+}
+// x64 += scale[0] * e32
+__global__ void k_add_f32_scaled(int64_t n, double *x, const float *e, const double *scale) {
+  double s = scale[0] > 0. ? scale[0] : 1.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) x[i] += s * (double)e[i];
+}
+
+static inline dim3 vgrid(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  if (b > 8192) b = 8192;
+  if (b < 1) b = 1;
+  return dim3((unsigned)b);
+}
+template <typename T> int fill(hipStream_t s, int64_t n, T *x, T a) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_fill<T>, vgrid(n), dim3(256), 0, s, n, x, a);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+template int fill<double>(hipStream_t, int64_t, double *, double);
+template int fill<float>(hipStream_t, int64_t, float *, float);
+template <typename T> int axpby(hipStream_t s, int64_t n, T a, T *x, T b, const T *y) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_axpby<T>, vgrid(n), dim3(256), 0, s, n, a, x, b, y);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+template int axpby<double>(hipStream_t, int64_t, double, double *, double, const double *);
+template int axpby<float>(hipStream_t, int64_t, float, float *, float, const float *);
+template <typename T> int cheb_first(hipStream_t s, int64_t n, const T *dinv, const T *r, T *d, T *x, T a) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_cheb_first<T>, vgrid(n), dim3(256), 0, s, n, dinv, r, d, x, a);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+template int cheb_first<double>(hipStream_t, int64_t, const double *, const double *, double *, double *, double);
+template int cheb_first<float>(hipStream_t, int64_t, const float *, const float *, float *, float *, float);
+int to_f32(hipStream_t s, int64_t n, const double *a, float *b, const double *scale) {
+  if (n <= 0) return 0;
+  if (scale) hipLaunchKernelGGL(k_f64_to_f32_scaled, vgrid(n), dim3(256), 0, s, n, a, b, scale);
+  else hipLaunchKernelGGL(k_f64_to_f32, vgrid(n), dim3(256), 0, s, n, a, b);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+int add_f32(hipStream_t s, int64_t n, double *x, const float *e, const double *scale) {
+  if (n <= 0) return 0;
+  if (scale) hipLaunchKernelGGL(k_add_f32_scaled, vgrid(n), dim3(256), 0, s, n, x, e, scale);
+  else hipLaunchKernelGGL(k_add_f32, vgrid(n), dim3(256), 0, s, n, x, e);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+// ---- reductions: partial sums per block into scratch, finished by one block
+template <typename T> __global__ __launch_bounds__(256) void k_dot_partial(int64_t n, const T *x, const T *y, double *part) {
+  double s = 0.;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    s += (double)x[i] * (double)y[i];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  __shared__ double ws[4];
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
+}
+__global__ __launch_bounds__(256) void k_sum_final(int nb, const double *part, double *out, int do_sqrt) {
+  double s = 0.;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) s += part[i];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  __shared__ double ws[4];
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) { double t = ws[0] + ws[1] + ws[2] + ws[3]; out[0] = do_sqrt ? sqrt(t) : t; }
+}
+template <typename T> int dot_dev(hipStream_t s, int64_t n, const T *x, const T *y, double *part, double *out, int do_sqrt) {
+  int nb = 1024;
+  hipLaunchKernelGGL(k_dot_partial<T>, dim3(nb), dim3(256), 0, s, n, x, y, part);
+  hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, s, nb, part, out, do_sqrt);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+template int dot_dev<double>(hipStream_t, int64_t, const double *, const double *, double *, double *, int);
+template int dot_dev<float>(hipStream_t, int64_t, const float *, const float *, double *, double *, int);
+
+// scalar helpers on device: out = a[0] / b[0] (relative norms for the reference rule)
+__global__ void k_ratio(const double *a, const double *b, double *out) {
+  if (threadIdx.x == 0) out[0] = b[0] != 0. ? a[0] / b[0] : 1.0;
+}
+int ratio(hipStream_t s, const double *a, const double *b, double *out) {
+  hipLaunchKernelGGL(k_ratio, dim3(1), dim3(64), 0, s, a, b, out);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+// ---- dense coarse solve: y = Ainv * b (row-major n x n); one wave per row
+template <typename T> __global__ __launch_bounds__(256) void k_dense_mv(int n, const T *__restrict__ M, const T *__restrict__ b, T *__restrict__ y) {
+  int row = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x % WAVE;
+  if (row >= n) return;
+  T s = 0;
+  for (int j = lane; j < n; j += WAVE) s += M[(int64_t)row * n + j] * b[j];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) y[row] = s;
+}
+template <typename T> int dense_mv(hipStream_t s, int n, const T *M, const T *b, T *y) {
+  int64_t threads = (int64_t)n * WAVE;
+  hipLaunchKernelGGL(k_dense_mv<T>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, n, M, b, y);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+template int dense_mv<double>(hipStream_t, int, const double *, const double *, double *);
+template int dense_mv<float>(hipStream_t, int, const float *, const float *, float *);
+
+// ---- diagonal inverse
+template <typename T> __global__ void k_diag_inv(int64_t n, const int64_t *rp, const int *ci, const double *v, T *dinv) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double d = 0.;
+    for (int64_t q = rp[i]; q < rp[i + 1]; q++) if (ci[q] == i) d = v[q];
+    dinv[i] = (T)(d != 0. ? 1.0 / d : 0.0);
+  }
+}
+int diag_inv(hipStream_t s, int64_t n, const int64_t *rp, const int *ci, const double *v, double *d64, float *d32) {
+  if (d64) hipLaunchKernelGGL(k_diag_inv<double>, vgrid(n), dim3(256), 0, s, n, rp, ci, v, d64);
+  if (d32) hipLaunchKernelGGL(k_diag_inv<float>, vgrid(n), dim3(256), 0, s, n, rp, ci, v, d32);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+__global__ void k_cast_vals(int64_t n, const double *v, float *vf) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) vf[i] = (float)v[i];
+}
+int cast_vals(hipStream_t s, int64_t n, const double *v, float *vf) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_cast_vals, vgrid(n), dim3(256), 0, s, n, v, vf);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+// ---- element -> CSR position map (binary search in sorted rows)
+__global__ void k_elpos(int nel, int nloc, const int *__restrict__ eldof, const int64_t *__restrict__ rp,
+                        const int *__restrict__ ci, int *__restrict__ pos) {
+  int64_t total = (int64_t)nel * nloc * nloc;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    int64_t e = t / ((int64_t)nloc * nloc);
+    int ab = (int)(t - e * nloc * nloc), a = ab / nloc, b = ab % nloc;
+    int row = eldof[e * nloc + a], col = eldof[e * nloc + b];
+    int64_t lo = rp[row], hi = rp[row + 1] - 1, found = -1;
+    while (lo <= hi) {
+      int64_t mid = (lo + hi) >> 1;
+      int c = ci[mid];
+      if (c == col) { found = mid; break; }
+      if (c < col) lo = mid + 1; else hi = mid - 1;
+    }
+    pos[t] = (int)found;
+  }
+}
+int elpos(hipStream_t s, int nel, int nloc, const int *eldof, const int64_t *rp, const int *ci, int *pos) {
+  int64_t total = (int64_t)nel * nloc * nloc;
+  int64_t b = (total + 255) / 256; if (b > 65536) b = 65536;
+  hipLaunchKernelGGL(k_elpos, dim3((unsigned)b), dim3(256), 0, s, nel, nloc, eldof, rp, ci, pos);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+}  // namespace fmk

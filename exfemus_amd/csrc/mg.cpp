@@ -1,0 +1,533 @@
+// mg.cpp -- multigrid for the MI355X FEMuS hot path.
+//
+// MGSolBase::MGStep / MGSolve (src/MGSolverBase.C:163-289) on the device:
+// recursive V-cycle (Gamma = 1) with nu_pre / nu_post sweeps doubled below
+// the top level, restriction b[L-1] = Rst res[L], x[L-1] = 0, coarse direct
+// solve, prolongation x[L] += Prl x[L-1] (reference rule: 0.95 and only when
+// the coarse residual is below the fine one), post-smoothing.
+// Precision modes: all-FP64 cycles; FP32 V-cycle (FP32 values, FP32
+// accumulation) inside FP64 iterative refinement or FP64 flexible GMRES,
+// where the outer residual b - A x is always formed in FP64.
+// The coarse-level inverse and the Vanka local inverses are built in FP64.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+#include "fm_internal.h"
+#include "fm_kernels.h"
+
+#define FM_CHECK(x)                                 \
+  do {                                              \
+    int rc_ = (x);                                  \
+    if (rc_ != 0) return rc_ < 0 ? rc_ : FM_EHIP;   \
+  } while (0)
+
+int fm_vanka_setup(fm_system *s, int L, std::vector<struct LOps> *ops);
+int fm_vanka_sweep64(fm_system *s, int L, const struct LOps &o, double *x, const double *b, int its, double omega);
+int fm_vanka_sweep32(fm_system *s, int L, const struct LOps &o, float *x, const float *b, int its, double omega);
+
+// Per-level operator set used by the V-cycle (forward or transposed).
+struct LOps {
+  int64_t n = 0;
+  const DevCSR *A = nullptr, *R = nullptr, *P = nullptr;
+  double *dinv = nullptr;
+  float *dinvf = nullptr;
+  double lam = 0.;
+  double *Ainv = nullptr;
+  float *Ainvf = nullptr;
+  Level *v = nullptr;  // work vectors
+  double *vinv = nullptr;  // Vanka local inverses
+};
+
+template <typename T> static CsrView<T> cview(const DevCSR &c);
+template <> CsrView<double> cview<double>(const DevCSR &c) {
+  CsrView<double> v; v.nr = c.nr; v.rp = c.rp; v.ci = c.ci; v.v = c.v; v.lpr = fmk::lanes_for(c.nnz, c.nr); return v;
+}
+template <> CsrView<float> cview<float>(const DevCSR &c) {
+  CsrView<float> v; v.nr = c.nr; v.rp = c.rp; v.ci = c.ci; v.v = c.vf; v.lpr = fmk::lanes_for(c.nnz, c.nr); return v;
+}
+
+// typed access to a level's work vectors
+template <typename T> struct Vec;
+template <> struct Vec<double> {
+  static double *x(Level *l) { return l->x; }  static double *b(Level *l) { return l->b; }
+  static double *r(Level *l) { return l->r; }  static double *t(Level *l) { return l->t; }
+  static double *d(Level *l) { return l->d; }  static double *d2(Level *l) { return l->d2; }
+  static double *dinv(const LOps &o) { return o.dinv; }
+  static double *Ainv(const LOps &o) { return o.Ainv; }
+};
+template <> struct Vec<float> {
+  static float *x(Level *l) { return l->xf; }  static float *b(Level *l) { return l->bf; }
+  static float *r(Level *l) { return l->rf; }  static float *t(Level *l) { return l->tf; }
+  static float *d(Level *l) { return l->df; }  static float *d2(Level *l) { return l->d2f; }
+  static float *dinv(const LOps &o) { return o.dinvf; }
+  static float *Ainv(const LOps &o) { return o.Ainvf; }
+};
+
+// ---- device reductions read back on the host -----------------------------
+template <typename T> static double dev_dot(fm_ctx *c, int64_t n, const T *x, const T *y, bool do_sqrt) {
+  if (fmk::dot_dev<T>(c->stream, n, x, y, c->scratch, c->scratch + 2048, do_sqrt ? 1 : 0) != 0) return NAN;
+  if (hipMemcpyAsync(c->h_scratch, c->scratch + 2048, sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess) return NAN;
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return NAN;
+  return c->h_scratch[0];
+}
+
+// ---- lambda_max(D^-1 A) by power iteration (same start vector as the oracle)
+__global__ void k_start_vec(int64_t n, double *x) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = 1.0 + (double)(i % 7) / 7.0;
+}
+// y = dinv .* x
+__global__ void k_scale_mul(int64_t n, double *y, const double *__restrict__ dinv, const double *__restrict__ x) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = dinv[i] * x[i];
+}
+static int lambda_max(fm_ctx *c, const LOps &o, int iters, double *lam) {
+  Level *l = o.v;
+  hipStream_t st = c->stream;
+  int64_t n = o.n;
+  unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_start_vec, dim3(g), dim3(256), 0, st, n, l->t);
+  double lm = 0.;
+  for (int it = 0; it < iters; it++) {
+    double xn = dev_dot<double>(c, n, l->t, l->t, true);
+    if (!(xn > 0.)) return FM_ESINGULAR;
+    // t = t / ||t||  (scale folded into the product below)
+    FM_CHECK(fmk::axpby<double>(st, n, 1.0 / xn, l->t, 0.0, l->t));
+    FM_CHECK(fmk::spmv(st, cview<double>(*o.A), l->t, l->d));
+    hipLaunchKernelGGL(k_scale_mul, dim3(g), dim3(256), 0, st, n, l->t, o.dinv, l->d);
+    lm = dev_dot<double>(c, n, l->t, l->t, true);
+  }
+  *lam = lm;
+  return 0;
+}
+
+// ---- coarse inverse (host FP64 Gauss-Jordan with partial pivoting) --------
+static int coarse_inverse(fm_system *s, const DevCSR &A, double **dAinv, float **dAinvf) {
+  int n = (int)A.nr;
+  std::vector<double> vals(A.nnz);
+  std::vector<int64_t> rp(n + 1);
+  std::vector<int> ci(A.nnz);
+  FM_HIP(hipStreamSynchronize(s->ctx->stream));
+  FM_HIP(hipMemcpy(vals.data(), A.v, A.nnz * sizeof(double), hipMemcpyDeviceToHost));
+  FM_HIP(hipMemcpy(rp.data(), A.rp, (n + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
+  FM_HIP(hipMemcpy(ci.data(), A.ci, A.nnz * sizeof(int), hipMemcpyDeviceToHost));
+  std::vector<double> M((size_t)n * n, 0.), I((size_t)n * n, 0.);
+  double amax = 0.;
+  for (int i = 0; i < n; i++)
+    for (int64_t q = rp[i]; q < rp[i + 1]; q++) { M[(size_t)i * n + ci[q]] = vals[q]; amax = std::max(amax, std::fabs(vals[q])); }
+  std::vector<int> pinned;
+  for (int attempt = 0; attempt < 2; attempt++) {
+    std::vector<double> W = M;
+    for (int i : pinned) { for (int j = 0; j < n; j++) W[(size_t)i * n + j] = 0.; W[(size_t)i * n + i] = 1.; }
+    std::fill(I.begin(), I.end(), 0.);
+    for (int i = 0; i < n; i++) I[(size_t)i * n + i] = 1.;
+    std::vector<int> perm(n);
+    bool singular = false;
+    int bad = -1;
+    for (int k = 0; k < n && !singular; k++) {
+      int p = k;
+      double mx = std::fabs(W[(size_t)k * n + k]);
+      for (int i = k + 1; i < n; i++) if (std::fabs(W[(size_t)i * n + k]) > mx) { mx = std::fabs(W[(size_t)i * n + k]); p = i; }
+      if (mx <= 1e-13 * amax) { singular = true; bad = k; break; }
+      if (p != k) {
+        for (int j = 0; j < n; j++) { std::swap(W[(size_t)k * n + j], W[(size_t)p * n + j]); std::swap(I[(size_t)k * n + j], I[(size_t)p * n + j]); }
+      }
+      double piv = 1.0 / W[(size_t)k * n + k];
+      for (int j = 0; j < n; j++) { W[(size_t)k * n + j] *= piv; I[(size_t)k * n + j] *= piv; }
+#pragma omp parallel for schedule(static)
+      for (int i = 0; i < n; i++) {
+        if (i == k) continue;
+        double f = W[(size_t)i * n + k];
+        if (f == 0.) continue;
+        for (int j = 0; j < n; j++) { W[(size_t)i * n + j] -= f * W[(size_t)k * n + j]; I[(size_t)i * n + j] -= f * I[(size_t)k * n + j]; }
+      }
+    }
+    if (!singular) {
+      for (int i : pinned) for (int j = 0; j < n; j++) I[(size_t)j * n + i] = 0.;  // ignore rhs of pinned rows
+      break;
+    }
+    if (attempt == 1 || s->nq1 == 0) { fm_set_error("coarse matrix singular"); return FM_ESINGULAR; }
+    (void)bad;
+    // enclosed flow: pressure defined up to a constant -> pin the last Q1 DoF
+    pinned.push_back(n - 1);
+  }
+  if (!*dAinv) FM_CHECK(hipMalloc((void **)dAinv, (size_t)n * n * sizeof(double)) == hipSuccess ? 0 : FM_ENOMEM);
+  if (!*dAinvf) FM_CHECK(hipMalloc((void **)dAinvf, (size_t)n * n * sizeof(float)) == hipSuccess ? 0 : FM_ENOMEM);
+  std::vector<float> If(I.begin(), I.end());
+  FM_HIP(hipMemcpy(*dAinv, I.data(), (size_t)n * n * sizeof(double), hipMemcpyHostToDevice));
+  FM_HIP(hipMemcpy(*dAinvf, If.data(), (size_t)n * n * sizeof(float), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// ---- forward operator set
+static std::vector<LOps> forward_ops(fm_system *s) {
+  std::vector<LOps> ops(s->lev.size());
+  for (size_t L = 0; L < s->lev.size(); L++) {
+    Level &l = s->lev[L];
+    LOps &o = ops[L];
+    o.n = l.n; o.A = &l.A; o.R = &l.R; o.P = &l.P;
+    o.dinv = l.dinv; o.dinvf = l.dinvf; o.lam = l.lam; o.Ainv = l.Ainv; o.Ainvf = l.Ainvf; o.v = &l; o.vinv = l.vinv;
+  }
+  return ops;
+}
+
+// ---- smoothers ----------------------------------------------------------
+template <typename T>
+static int smooth(fm_system *s, const fm_mg_opts *opt, int L, const LOps &o, int its) {
+  hipStream_t st = s->ctx->stream;
+  Level *l = o.v;
+  T *x = Vec<T>::x(l), *b = Vec<T>::b(l), *r = Vec<T>::r(l), *d = Vec<T>::d(l), *d2 = Vec<T>::d2(l);
+  T *dinv = Vec<T>::dinv(o);
+  CsrView<T> A = cview<T>(*o.A);
+  if (its <= 0) return 0;
+  if (opt->smoother == FM_SMOOTH_CHEBYSHEV) {
+    double lo = opt->cheb_lo * o.lam, hi = opt->cheb_hi * o.lam;
+    double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta, rho = 1. / sigma;
+    FM_CHECK(fmk::resid(st, A, b, x, r));
+    FM_CHECK(fmk::cheb_first<T>(st, o.n, dinv, r, d, x, (T)(1.0 / theta)));
+    for (int k = 2; k <= its; k++) {
+      double rho_new = 1. / (2. * sigma - rho);
+      FM_CHECK(fmk::cheb_step(st, A, r, x, d, d2, dinv, (T)(rho_new * rho), (T)(2. * rho_new / delta)));
+      std::swap(d, d2);
+      rho = rho_new;
+    }
+    return 0;
+  }
+  if (opt->smoother == FM_SMOOTH_JACOBI) {
+    // ping-pong through t: x_{k+1} = x_k + omega D^-1 (b - A x_k)
+    T *t = Vec<T>::t(l);
+    for (int k = 0; k < its; k++) {
+      FM_CHECK(fmk::jacobi_step(st, A, b, x, t, dinv, (T)opt->omega));
+      hipError_t e = hipMemcpyAsync(x, t, o.n * sizeof(T), hipMemcpyDeviceToDevice, st);
+      if (e != hipSuccess) return FM_EHIP;
+    }
+    return 0;
+  }
+  if (opt->smoother == FM_SMOOTH_VANKA) {
+    if (sizeof(T) == 8) return fm_vanka_sweep64(s, L, o, (double *)x, (const double *)b, its, opt->omega);
+    return fm_vanka_sweep32(s, L, o, (float *)x, (const float *)b, its, opt->omega);
+  }
+  return FM_EINVAL;
+}
+
+// ---- V-cycle (MGSolBase::MGStep) -----------------------------------------
+template <typename T>
+static int vcycle(fm_system *s, const fm_mg_opts *opt, std::vector<LOps> &ops, int L) {
+  hipStream_t st = s->ctx->stream;
+  const LOps &o = ops[L];
+  Level *l = o.v;
+  if (L == 0) {
+    FM_CHECK(fmk::dense_mv<T>(st, (int)o.n, Vec<T>::Ainv(o), Vec<T>::b(l), Vec<T>::x(l)));
+    return 0;
+  }
+  int top = (int)ops.size() - 1;
+  int mult = L < top ? 2 : 1;
+  FM_CHECK(smooth<T>(s, opt, L, o, opt->nu_pre * mult));
+  CsrView<T> A = cview<T>(*o.A);
+  T *x = Vec<T>::x(l), *b = Vec<T>::b(l), *r = Vec<T>::r(l);
+  FM_CHECK(fmk::resid(st, A, b, x, r));
+  const LOps &oc = ops[L - 1];
+  Level *lc = oc.v;
+  double *cond = nullptr;
+  if (opt->correction == 0) {
+    // rest = ||r||/||b|| on this level; coarse_rest measured after the coarse cycle
+    FM_CHECK(fmk::dot_dev<T>(st, o.n, r, r, s->ctx->scratch, l->nrm + 0, 1));
+    FM_CHECK(fmk::dot_dev<T>(st, o.n, b, b, s->ctx->scratch, l->nrm + 1, 1));
+    FM_CHECK(fmk::ratio(st, l->nrm + 0, l->nrm + 1, l->nrm + 3));   // fine rest
+  }
+  FM_CHECK(fmk::spmv(st, cview<T>(*oc.R), r, Vec<T>::b(lc)));
+  FM_CHECK(fmk::fill<T>(st, oc.n, Vec<T>::x(lc), (T)0));
+  FM_CHECK(vcycle<T>(s, opt, ops, L - 1));
+  double alpha = 1.0;
+  if (opt->correction == 0) {
+    // coarse_rest: relative residual of the coarse level after its cycle
+    CsrView<T> Ac = cview<T>(*oc.A);
+    FM_CHECK(fmk::resid(st, Ac, Vec<T>::b(lc), Vec<T>::x(lc), Vec<T>::r(lc)));
+    FM_CHECK(fmk::dot_dev<T>(st, oc.n, Vec<T>::r(lc), Vec<T>::r(lc), s->ctx->scratch, l->nrm + 4, 1));
+    FM_CHECK(fmk::dot_dev<T>(st, oc.n, Vec<T>::b(lc), Vec<T>::b(lc), s->ctx->scratch, l->nrm + 5, 1));
+    FM_CHECK(fmk::ratio(st, l->nrm + 4, l->nrm + 5, l->nrm + 2));   // nrm[2] = coarse rest, nrm[3] = fine rest
+    cond = l->nrm + 2;
+    alpha = 0.95;
+  }
+  FM_CHECK(fmk::addmul(st, cview<T>(*o.P), Vec<T>::x(lc), x, (T)alpha, cond));
+  FM_CHECK(smooth<T>(s, opt, L, o, opt->nu_post * mult));
+  return 0;
+}
+
+// ---- setup: eigen bounds, coarse inverse, Vanka --------------------------
+static int mg_setup_ops(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &ops, bool adjoint);
+
+extern "C" int fm_mg_setup(fm_system *s, const fm_mg_opts *o) {
+  if (!s || !o || !s->initialized) return FM_ESTATE;
+  FM_HIP(hipSetDevice(s->ctx->device));
+  std::vector<LOps> ops = forward_ops(s);
+  FM_CHECK(mg_setup_ops(s, o, ops, false));
+  for (size_t L = 0; L < s->lev.size(); L++) {
+    s->lev[L].lam = ops[L].lam;
+    s->lev[L].Ainv = ops[L].Ainv; s->lev[L].Ainvf = ops[L].Ainvf; s->lev[L].vinv = ops[L].vinv;
+  }
+  s->mg_ready = true;
+  s->mg_smoother = o->smoother;
+  return FM_OK;
+}
+
+static int mg_setup_ops(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &ops, bool adjoint) {
+  (void)adjoint;
+  for (size_t L = 1; L < ops.size(); L++) {
+    if (o->smoother == FM_SMOOTH_CHEBYSHEV && ops[L].lam == 0.) FM_CHECK(lambda_max(s->ctx, ops[L], 20, &ops[L].lam));
+    if (o->smoother == FM_SMOOTH_VANKA) FM_CHECK(fm_vanka_setup(s, (int)L, &ops));
+  }
+  FM_CHECK(coarse_inverse(s, *ops[0].A, &ops[0].Ainv, &ops[0].Ainvf));
+  return 0;
+}
+
+// ---- outer solvers ---------------------------------------------------------
+struct Timer {
+  fm_ctx *c;
+  explicit Timer(fm_ctx *cc) : c(cc) { (void)hipEventRecord(c->ev0, c->stream); }
+  double stop() {
+    (void)hipEventRecord(c->ev1, c->stream);
+    (void)hipEventSynchronize(c->ev1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    return ms * 1e-3;
+  }
+};
+
+// x[top] (FP64) := solution; b[top] FP64 rhs.  Stationary cycles / IR.
+static int solve_stationary(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &ops, fm_mg_report *rep) {
+  fm_ctx *c = s->ctx;
+  hipStream_t st = c->stream;
+  int top = (int)ops.size() - 1;
+  LOps &ot = ops[top];
+  Level *l = ot.v;
+  double bn = dev_dot<double>(c, ot.n, l->b, l->b, true);
+  if (!(bn > 0.)) { rep->iters = 0; rep->converged = 1; rep->rel_resid = 0.; return 0; }
+  CsrView<double> A = cview<double>(*ot.A);
+  int it = 0;
+  double rel = 1.;
+  FM_CHECK(fmk::resid(st, A, l->b, l->x, l->res));
+  rel = dev_dot<double>(c, ot.n, l->res, l->res, true) / bn;
+  while (it < o->max_iter && rel > o->tol) {
+    if (o->precision == FM_PREC_FP64) {
+      FM_CHECK(vcycle<double>(s, o, ops, top));
+    } else {
+      // FP32 correction: A e = r (FP32 V-cycle from e = 0), x += e
+      FM_CHECK(fmk::to_f32(st, ot.n, l->res, l->bf, nullptr));
+      FM_CHECK(fmk::fill<float>(st, ot.n, l->xf, 0.f));
+      FM_CHECK(vcycle<float>(s, o, ops, top));
+      FM_CHECK(fmk::add_f32(st, ot.n, l->x, l->xf, nullptr));
+    }
+    FM_CHECK(fmk::resid(st, A, l->b, l->x, l->res));
+    rel = dev_dot<double>(c, ot.n, l->res, l->res, true) / bn;
+    if (it < 256) rep->hist[it] = rel;
+    it++;
+    if (!std::isfinite(rel)) break;
+  }
+  rep->iters = it;
+  rep->rel_resid = rel;
+  rep->converged = rel <= o->tol;
+  return 0;
+}
+
+// Flexible GMRES(m), right preconditioned by one V-cycle (FP32 or FP64).
+__global__ void k_multi_dot(int64_t n, int k, const double *__restrict__ V, const double *__restrict__ w, double *part) {
+  // part[j * gridDim.x + block] = sum_i V[j][i] * w[i]  for j < k
+  extern __shared__ double sh[];
+  for (int j = 0; j < k; j++) {
+    double s = 0.;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+      s += V[(int64_t)j * n + i] * w[i];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.;
+      for (int q = 0; q < (int)(blockDim.x >> 6); q++) t += sh[q];
+      part[(int64_t)j * gridDim.x + blockIdx.x] = t;
+    }
+    __syncthreads();
+  }
+}
+__global__ void k_multi_sum(int k, int nb, const double *part, double *out) {
+  int j = blockIdx.x;
+  double s = 0.;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) s += part[(int64_t)j * nb + i];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  __shared__ double sh[4];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[j] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+// w -= sum_j h[j] V[j]
+__global__ void k_multi_axpy(int64_t n, int k, const double *__restrict__ V, const double *__restrict__ h, double *w) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double s = w[i];
+    for (int j = 0; j < k; j++) s -= h[j] * V[(int64_t)j * n + i];
+    w[i] = s;
+  }
+}
+// x += sum_j y[j] Z[j]
+__global__ void k_multi_add(int64_t n, int k, const double *__restrict__ Z, const double *__restrict__ y, double *x) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double s = x[i];
+    for (int j = 0; j < k; j++) s += y[j] * Z[(int64_t)j * n + i];
+    x[i] = s;
+  }
+}
+
+static int solve_fgmres(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &ops, fm_mg_report *rep) {
+  fm_ctx *c = s->ctx;
+  hipStream_t st = c->stream;
+  int top = (int)ops.size() - 1;
+  LOps &ot = ops[top];
+  Level *l = ot.v;
+  int64_t n = ot.n;
+  int m = std::max(1, std::min(o->fgmres_restart > 0 ? o->fgmres_restart : 30, 60));
+  double *V = nullptr, *Z = nullptr, *hd = nullptr;
+  if (hipMalloc((void **)&V, (size_t)(m + 1) * n * sizeof(double)) != hipSuccess) return FM_ENOMEM;
+  if (hipMalloc((void **)&Z, (size_t)m * n * sizeof(double)) != hipSuccess) { (void)hipFree(V); return FM_ENOMEM; }
+  if (hipMalloc((void **)&hd, 128 * sizeof(double)) != hipSuccess) { (void)hipFree(V); (void)hipFree(Z); return FM_ENOMEM; }
+  const int nb = 512;
+  double *part = c->scratch;  // (m+1) * nb <= 4096 doubles: m <= 6 uses scratch; else allocate
+  double *part_alloc = nullptr;
+  if ((int64_t)(m + 1) * nb > 2048) {
+    if (hipMalloc((void **)&part_alloc, (size_t)(m + 1) * nb * sizeof(double)) != hipSuccess) { (void)hipFree(V); (void)hipFree(Z); (void)hipFree(hd); return FM_ENOMEM; }
+    part = part_alloc;
+  }
+  std::vector<double> H((size_t)(m + 1) * m, 0.), cs(m), sn(m), g(m + 1), y(m), hh(m + 1);
+  CsrView<double> A = cview<double>(*ot.A);
+  int rc = 0;
+  double bn = dev_dot<double>(c, n, l->b, l->b, true);
+  int total = 0;
+  double rel = 1.;
+  unsigned gv = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
+  if (!(bn > 0.)) { rel = 0.; goto done; }
+  while (total < o->max_iter) {
+    if ((rc = fmk::resid(st, A, l->b, l->x, V)) != 0) goto done;
+    {
+      double beta = dev_dot<double>(c, n, V, V, true);
+      rel = beta / bn;
+      if (total == 0) rep->hist[0] = rel;
+      if (rel <= o->tol || !std::isfinite(rel)) break;
+      if ((rc = fmk::axpby<double>(st, n, 1.0 / beta, V, 0.0, V)) != 0) goto done;
+      std::fill(g.begin(), g.end(), 0.);
+      g[0] = beta;
+      int j = 0;
+      for (; j < m && total < o->max_iter; j++) {
+        double *vj = V + (size_t)j * n, *zj = Z + (size_t)j * n, *w = V + (size_t)(j + 1) * n;
+        // z_j = M v_j
+        if (o->precision == FM_PREC_FP64_FGMRES) {
+          // use level-top b/x of the FP64 cycle: save b in t
+          if (hipMemcpyAsync(l->t, l->b, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
+          if (hipMemcpyAsync(l->b, vj, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
+          if (hipMemcpyAsync(l->d2, l->x, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
+          if ((rc = fmk::fill<double>(st, n, l->x, 0.0)) != 0) goto done;
+          if ((rc = vcycle<double>(s, o, ops, top)) != 0) goto done;
+          if (hipMemcpyAsync(zj, l->x, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
+          if (hipMemcpyAsync(l->x, l->d2, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
+          if (hipMemcpyAsync(l->b, l->t, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
+        } else {
+          if ((rc = fmk::to_f32(st, n, vj, l->bf, nullptr)) != 0) goto done;
+          if ((rc = fmk::fill<float>(st, n, l->xf, 0.f)) != 0) goto done;
+          if ((rc = vcycle<float>(s, o, ops, top)) != 0) goto done;
+          if ((rc = fmk::fill<double>(st, n, zj, 0.0)) != 0) goto done;
+          if ((rc = fmk::add_f32(st, n, zj, l->xf, nullptr)) != 0) goto done;
+        }
+        // w = A z_j ; classical Gram-Schmidt twice
+        if ((rc = fmk::spmv(st, A, zj, w)) != 0) goto done;
+        for (int k = 0; k <= j; k++) H[(size_t)k * m + j] = 0.;
+        for (int pass = 0; pass < 2; pass++) {
+          hipLaunchKernelGGL(k_multi_dot, dim3(nb), dim3(256), 4 * sizeof(double), st, n, j + 1, V, w, part);
+          hipLaunchKernelGGL(k_multi_sum, dim3(j + 1), dim3(256), 0, st, j + 1, nb, part, hd);
+          hipLaunchKernelGGL(k_multi_axpy, dim3(gv), dim3(256), 0, st, n, j + 1, V, hd, w);
+          if (hipMemcpyAsync(hh.data(), hd, (j + 1) * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess) { rc = FM_EHIP; goto done; }
+          if (hipStreamSynchronize(st) != hipSuccess) { rc = FM_EHIP; goto done; }
+          for (int k = 0; k <= j; k++) H[(size_t)k * m + j] += hh[k];
+        }
+        double hn = dev_dot<double>(c, n, w, w, true);
+        H[(size_t)(j + 1) * m + j] = hn;
+        if (hn > 0.) { if ((rc = fmk::axpby<double>(st, n, 1.0 / hn, w, 0.0, w)) != 0) goto done; }
+        // Givens
+        for (int k = 0; k < j; k++) {
+          double a = H[(size_t)k * m + j], b = H[(size_t)(k + 1) * m + j];
+          H[(size_t)k * m + j] = cs[k] * a + sn[k] * b;
+          H[(size_t)(k + 1) * m + j] = -sn[k] * a + cs[k] * b;
+        }
+        double a = H[(size_t)j * m + j], b = H[(size_t)(j + 1) * m + j];
+        double r = std::sqrt(a * a + b * b);
+        cs[j] = r > 0 ? a / r : 1.; sn[j] = r > 0 ? b / r : 0.;
+        H[(size_t)j * m + j] = r; H[(size_t)(j + 1) * m + j] = 0.;
+        g[j + 1] = -sn[j] * g[j];
+        g[j] = cs[j] * g[j];
+        total++;
+        rel = std::fabs(g[j + 1]) / bn;
+        if (total < 256) rep->hist[total] = rel;
+        if (rel <= o->tol * 0.5 || hn == 0.) { j++; break; }
+      }
+      // solve H y = g, x += Z y
+      for (int i = j - 1; i >= 0; i--) {
+        double sacc = g[i];
+        for (int k = i + 1; k < j; k++) sacc -= H[(size_t)i * m + k] * y[k];
+        y[i] = sacc / H[(size_t)i * m + i];
+      }
+      if (hipMemcpyAsync(hd, y.data(), j * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
+      hipLaunchKernelGGL(k_multi_add, dim3(gv), dim3(256), 0, st, n, j, Z, hd, l->x);
+    }
+  }
+  // true FP64 residual at exit
+  if ((rc = fmk::resid(st, A, l->b, l->x, l->res)) != 0) goto done;
+  rel = dev_dot<double>(c, n, l->res, l->res, true) / bn;
+done:
+  (void)hipStreamSynchronize(st);
+  (void)hipFree(V); (void)hipFree(Z); (void)hipFree(hd);
+  if (part_alloc) (void)hipFree(part_alloc);
+  rep->iters = total;
+  rep->rel_resid = rel;
+  rep->converged = rel <= o->tol;
+  return rc;
+}
+
+extern "C" int fm_mg_solve(fm_system *s, const fm_mg_opts *o, fm_mg_report *rep) {
+  if (!s || !o || !rep) return FM_EINVAL;
+  if (!s->initialized) return FM_ESTATE;
+  FM_HIP(hipSetDevice(s->ctx->device));
+  std::memset(rep, 0, sizeof(*rep));
+  if (!s->mg_ready || s->mg_smoother != o->smoother) FM_CHECK(fm_mg_setup(s, o));
+  std::vector<LOps> ops = forward_ops(s);
+  Timer tm(s->ctx);
+  int rc;
+  if (o->precision == FM_PREC_FP64 || o->precision == FM_PREC_FP32_IR) rc = solve_stationary(s, o, ops, rep);
+  else rc = solve_fgmres(s, o, ops, rep);
+  rep->seconds = tm.stop();
+  if (rc) return rc;
+  return rep->converged ? FM_OK : FM_ENOCONV;
+}
+
+// ---- Vanka (element patches, additive or coloured), FP64 local inverses ---
+int fm_vanka_setup(fm_system *s, int L, std::vector<LOps> *ops) {
+  (void)s; (void)L; (void)ops;
+  fm_set_error("Vanka smoother not available in this build");
+  return FM_EINVAL;
+}
+int fm_vanka_sweep64(fm_system *s, int L, const LOps &o, double *x, const double *b, int its, double omega) {
+  (void)s; (void)L; (void)o; (void)x; (void)b; (void)its; (void)omega;
+  return FM_EINVAL;
+}
+int fm_vanka_sweep32(fm_system *s, int L, const LOps &o, float *x, const float *b, int its, double omega) {
+  (void)s; (void)L; (void)o; (void)x; (void)b; (void)its; (void)omega;
+  return FM_EINVAL;
+}
+
+extern "C" int fm_adjoint_solve(fm_system *s, const fm_mg_opts *o, const double *g, double *lam, fm_mg_report *rep) {
+  (void)s; (void)o; (void)g; (void)lam; (void)rep;
+  fm_set_error("adjoint solve not available in this build");
+  return FM_EINVAL;
+}
+extern "C" int fm_heat_grad_coef(fm_system *s, const fm_heat_params *p, const double *lam, const double *x, double *grad) {
+  (void)s; (void)p; (void)lam; (void)x; (void)grad;
+  fm_set_error("gradient not available in this build");
+  return FM_EINVAL;
+}

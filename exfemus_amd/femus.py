@@ -1,0 +1,376 @@
+"""Python mirror of the FEMuS operator interface over the HIP C ABI.
+
+The classes follow the reference's MGSolBase / MGSolDA / MGSolT / MGSolNS
+interface (include/MGSolverBase.h:24-389, include/MGSolverDA.h:55-376,
+template_appl/nse/SRC/MGSolverT.h, MGSolverNS.h): same method names
+(GenBc, GenIc, init, GenMatRhs, MGSolve, MGStep-driven cycles, MGTimeStep),
+same argument meaning, and -- instead of abort() -- a FemusError carrying the
+library's status code and message.  All compute goes through
+exfemus_amd/libfemus_hip.so on the GPU; there is no CPU fallback: loading
+fails loudly when the library is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libfemus_hip.so")
+
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+BCFN = C.CFUNCTYPE(None, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_void_p)
+
+FM_PREC_FP64, FM_PREC_FP32_IR, FM_PREC_FP32_FGMRES, FM_PREC_FP64_FGMRES = 0, 1, 2, 3
+FM_SMOOTH_CHEBYSHEV, FM_SMOOTH_JACOBI, FM_SMOOTH_VANKA = 0, 1, 2
+FM_ENOCONV = -3
+
+# Exported entry points (the C ABI of include/femus_hip.h)
+EXPORTS = [
+    "fm_ctx_create", "fm_ctx_destroy", "fm_last_error", "fm_mesh_create_box", "fm_mesh_destroy",
+    "fm_mesh_info", "fm_mesh_level", "fm_mesh_coords", "fm_mesh_conn", "fm_system_create",
+    "fm_system_destroy", "fm_system_ndof", "fm_system_node_dof", "fm_system_genbc", "fm_system_get_bc",
+    "fm_system_init", "fm_system_csr_nnz", "fm_system_csr_get", "fm_system_set_xold", "fm_system_get_x",
+    "fm_system_set_x", "fm_system_get_rhs", "fm_system_set_rhs", "fm_system_set_elem_coef",
+    "fm_system_set_velocity", "fm_heat_genmatrhs", "fm_ns_genmatrhs", "fm_spmv", "fm_residual",
+    "fm_restrict", "fm_prolong", "fm_get_res", "fm_mg_setup", "fm_mg_solve", "fm_adjoint_solve",
+    "fm_heat_grad_coef", "fm_bench_spmv", "fm_sync",
+]
+
+
+class FemusError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"femus_hip error {code}: {msg}")
+        self.code = code
+
+
+class HeatParams(C.Structure):
+    _fields_ = [("alpha0", C.c_double), ("rhocp", C.c_double), ("dt", C.c_double),
+                ("time_coeff", C.c_double), ("source", C.c_double), ("upwind", C.c_double),
+                ("vel_const", C.c_double * 3)]
+
+
+class NSParams(C.Structure):
+    _fields_ = [("IRe", C.c_double), ("dt", C.c_double), ("time_coeff", C.c_double),
+                ("upwind", C.c_double), ("IFr", C.c_double), ("dirg", C.c_double * 3),
+                ("src", C.c_double * 3)]
+
+
+class MGOpts(C.Structure):
+    _fields_ = [("precision", C.c_int), ("smoother", C.c_int), ("nu_pre", C.c_int), ("nu_post", C.c_int),
+                ("correction", C.c_int), ("max_iter", C.c_int), ("fgmres_restart", C.c_int),
+                ("tol", C.c_double), ("cheb_lo", C.c_double), ("cheb_hi", C.c_double),
+                ("omega", C.c_double), ("vanka_colored", C.c_int)]
+
+
+class MGReport(C.Structure):
+    _fields_ = [("iters", C.c_int), ("converged", C.c_int), ("rel_resid", C.c_double),
+                ("hist", C.c_double * 256), ("seconds", C.c_double)]
+
+
+_LIB = None
+
+
+def lib():
+    """Load libfemus_hip.so (raises if it is not built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIBPATH):
+        raise ImportError(f"{LIBPATH} missing: run __graft_entry__.build() (hipcc, gfx950)")
+    L = C.CDLL(LIBPATH)
+    vp = C.c_void_p
+    pp = C.POINTER(C.c_void_p)
+    L.fm_last_error.restype = C.c_char_p
+    L.fm_ctx_create.argtypes = [C.c_int, pp]
+    L.fm_ctx_destroy.argtypes = [vp]
+    L.fm_sync.argtypes = [vp]
+    L.fm_mesh_create_box.argtypes = [C.c_int, i32p, f64p, f64p, C.c_int, pp]
+    L.fm_mesh_destroy.argtypes = [vp]
+    L.fm_mesh_info.argtypes = [vp, i32p]
+    L.fm_mesh_level.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.fm_mesh_coords.argtypes = [vp, f64p]
+    L.fm_mesh_conn.argtypes = [vp, C.c_int, vp, vp, vp]
+    L.fm_system_create.argtypes = [vp, vp, C.c_int, C.c_int, pp]
+    L.fm_system_destroy.argtypes = [vp]
+    L.fm_system_ndof.restype = C.c_int64
+    L.fm_system_ndof.argtypes = [vp, C.c_int]
+    L.fm_system_node_dof.argtypes = [vp, C.c_int, i32p]
+    L.fm_system_genbc.argtypes = [vp, BCFN, vp]
+    L.fm_system_get_bc.argtypes = [vp, vp, vp]
+    L.fm_system_init.argtypes = [vp]
+    L.fm_system_csr_nnz.restype = C.c_int64
+    L.fm_system_csr_nnz.argtypes = [vp, C.c_int, C.c_int]
+    L.fm_system_csr_get.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
+    for fn in ("fm_system_set_xold",):
+        getattr(L, fn).argtypes = [vp, f64p]
+    for fn in ("fm_system_get_x", "fm_system_set_x", "fm_system_get_rhs", "fm_system_set_rhs", "fm_get_res"):
+        getattr(L, fn).argtypes = [vp, C.c_int, f64p]
+    L.fm_system_set_elem_coef.argtypes = [vp, C.c_int, vp]
+    L.fm_system_set_velocity.argtypes = [vp, vp]
+    L.fm_heat_genmatrhs.argtypes = [vp, C.c_int, C.c_int, C.POINTER(HeatParams)]
+    L.fm_ns_genmatrhs.argtypes = [vp, C.c_int, C.c_int, C.POINTER(NSParams)]
+    L.fm_spmv.argtypes = [vp, C.c_int, f64p, f64p]
+    for fn in ("fm_residual", "fm_restrict", "fm_prolong"):
+        getattr(L, fn).argtypes = [vp, C.c_int]
+    L.fm_mg_setup.argtypes = [vp, C.POINTER(MGOpts)]
+    L.fm_mg_solve.argtypes = [vp, C.POINTER(MGOpts), C.POINTER(MGReport)]
+    L.fm_adjoint_solve.argtypes = [vp, C.POINTER(MGOpts), f64p, f64p, C.POINTER(MGReport)]
+    L.fm_heat_grad_coef.argtypes = [vp, C.POINTER(HeatParams), f64p, f64p, f64p]
+    L.fm_bench_spmv.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]
+    _LIB = L
+    return L
+
+
+def _ck(rc, ok=(0,)):
+    if rc not in ok:
+        raise FemusError(rc, lib().fm_last_error().decode(errors="replace"))
+    return rc
+
+
+class Context:
+    """One HIP device + one stream (fm_ctx)."""
+
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        _ck(lib().fm_ctx_create(device, C.byref(self.h)))
+
+    def sync(self):
+        _ck(lib().fm_sync(self.h))
+
+    def close(self):
+        if self.h:
+            lib().fm_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Mesh:
+    """gencase hierarchy of a box: build_square/build_cube + uniformly_refine + printMesh ordering."""
+
+    def __init__(self, dim, nx0, lo, hi, nlev):
+        nx = np.ones(3, np.int32); nx[:dim] = nx0
+        l = np.zeros(3); l[:dim] = lo
+        h = np.zeros(3); h[:dim] = hi
+        self.h = C.c_void_p()
+        _ck(lib().fm_mesh_create_box(dim, nx, l, h, nlev, C.byref(self.h)))
+        info = np.zeros(6, np.int32)
+        _ck(lib().fm_mesh_info(self.h, info))
+        self.dim, self.nlev, self.n2, self.n1, self.nsides, self.N = (int(v) for v in info)
+
+    def level(self, L):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        _ck(lib().fm_mesh_level(self.h, L, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def coords(self):
+        x = np.zeros(self.dim * self.N)
+        _ck(lib().fm_mesh_coords(self.h, x))
+        return x.reshape(self.dim, self.N)
+
+    def conn(self, L):
+        ne = self.level(L)[0]
+        c = np.zeros(ne * self.n2, np.int32); nb = np.zeros(ne * self.nsides, np.int32); p = np.zeros(ne, np.int32)
+        _ck(lib().fm_mesh_conn(self.h, L, c.ctypes.data_as(C.c_void_p), nb.ctypes.data_as(C.c_void_p),
+                               p.ctypes.data_as(C.c_void_p)))
+        return c.reshape(ne, self.n2), nb.reshape(ne, self.nsides), p
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().fm_mesh_destroy(self.h)
+                self.h = C.c_void_p()
+        except Exception:
+            pass
+
+
+class MGSolDA:
+    """One multigrid equation system (MGSolBase + MGSolDA state and operators)."""
+
+    def __init__(self, ctx, mesh, nq2, nq1):
+        self.ctx, self.mesh, self.nq2, self.nq1 = ctx, mesh, nq2, nq1
+        self.h = C.c_void_p()
+        _ck(lib().fm_system_create(ctx.h, mesh.h, nq2, nq1, C.byref(self.h)))
+        self._NoLevels = mesh.nlev
+        self._Dim = [int(lib().fm_system_ndof(self.h, L)) for L in range(mesh.nlev)]
+        self._cb = None
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().fm_system_destroy(self.h)
+                self.h = C.c_void_p()
+        except Exception:
+            pass
+
+    # -- DoF / BC / operators -------------------------------------------------
+    def node_dof(self, L):
+        nd = np.zeros((self.nq2 + self.nq1) * self.mesh.N, np.int32)
+        _ck(lib().fm_system_node_dof(self.h, L, nd))
+        return nd
+
+    def GenBc(self, bc_read):
+        """bc_read(xp, bc_Neum, bc_flag) fills lists over the variables (MGSolDA::GenBc)."""
+        nv = self.nq2 + self.nq1
+        dim = self.mesh.dim
+
+        def cb(xp, neum, flag, user):
+            x = [xp[i] for i in range(dim)]
+            nn = [neum[i] for i in range(nv)]
+            ff = [flag[i] for i in range(nv)]
+            bc_read(x, nn, ff)
+            for i in range(nv):
+                neum[i] = int(nn[i]); flag[i] = int(ff[i])
+
+        self._cb = BCFN(cb)
+        _ck(lib().fm_system_genbc(self.h, self._cb, None))
+
+    def bc(self):
+        n = self._Dim[-1]
+        b0 = np.zeros(n, np.int32); b1 = np.zeros(n, np.int32)
+        _ck(lib().fm_system_get_bc(self.h, b0.ctypes.data_as(C.c_void_p), b1.ctypes.data_as(C.c_void_p)))
+        return b0, b1
+
+    def init(self):
+        """MGSolDA::init for every level: A pattern, Rst (masked), Prl on the device."""
+        _ck(lib().fm_system_init(self.h))
+
+    def GenIc(self, ic_read):
+        """ic_read(xp) -> list of values per variable; sets x_old[top] (MGSolDA::GenIc)."""
+        xyz = self.mesh.coords()
+        nd = self.node_dof(self._NoLevels - 1)
+        N = self.mesh.N
+        x = np.zeros(self._Dim[-1])
+        for k in range(N):
+            vals = ic_read(xyz[:, k])
+            for iv in range(self.nq2 + self.nq1):
+                d = nd[k + iv * N]
+                if d >= 0:
+                    x[d] = vals[iv]
+        self.set_xold(x)
+        return x
+
+    def csr(self, which, L):
+        nnz = int(lib().fm_system_csr_nnz(self.h, which, L))
+        nr = self._Dim[L]
+        rp = np.zeros(nr + 1, np.int64); ci = np.zeros(nnz, np.int32); v = np.zeros(nnz)
+        _ck(lib().fm_system_csr_get(self.h, which, L, rp.ctypes.data_as(C.c_void_p), ci.ctypes.data_as(C.c_void_p),
+                                    v.ctypes.data_as(C.c_void_p)))
+        return rp, ci, v
+
+    def set_xold(self, x):
+        _ck(lib().fm_system_set_xold(self.h, np.ascontiguousarray(x, np.float64)))
+
+    def get_x(self, L=None):
+        L = self._NoLevels - 1 if L is None else L
+        x = np.zeros(self._Dim[L]); _ck(lib().fm_system_get_x(self.h, L, x)); return x
+
+    def set_x(self, x, L=None):
+        L = self._NoLevels - 1 if L is None else L
+        _ck(lib().fm_system_set_x(self.h, L, np.ascontiguousarray(x, np.float64)))
+
+    def get_rhs(self, L=None):
+        L = self._NoLevels - 1 if L is None else L
+        b = np.zeros(self._Dim[L]); _ck(lib().fm_system_get_rhs(self.h, L, b)); return b
+
+    def set_rhs(self, b, L=None):
+        L = self._NoLevels - 1 if L is None else L
+        _ck(lib().fm_system_set_rhs(self.h, L, np.ascontiguousarray(b, np.float64)))
+
+    def get_res(self, L):
+        r = np.zeros(self._Dim[L]); _ck(lib().fm_get_res(self.h, L, r)); return r
+
+    def set_elem_coef(self, L, coef):
+        c = None if coef is None else np.ascontiguousarray(coef, np.float64)
+        _ck(lib().fm_system_set_elem_coef(self.h, L, None if c is None else c.ctypes.data_as(C.c_void_p)))
+        self._keep_coef = c
+
+    # -- LA operators (NumericVectorM::resid / matrix_mult) -----------------------
+    def spmv(self, L, x):
+        y = np.zeros(self._Dim[L]); _ck(lib().fm_spmv(self.h, L, np.ascontiguousarray(x, np.float64), y)); return y
+
+    def resid(self, L):
+        _ck(lib().fm_residual(self.h, L))
+
+    def restrict(self, L):
+        _ck(lib().fm_restrict(self.h, L))
+
+    def prolong(self, L):
+        _ck(lib().fm_prolong(self.h, L))
+
+    # -- MGSolve (src/MGSolverBase.C:163-208) --------------------------------------
+    @staticmethod
+    def opts(precision=FM_PREC_FP64, smoother=FM_SMOOTH_CHEBYSHEV, nu_pre=2, nu_post=2, correction=1,
+             max_iter=100, fgmres_restart=30, tol=1e-8, cheb_lo=0.2, cheb_hi=1.05, omega=0.6, vanka_colored=0):
+        return MGOpts(precision, smoother, nu_pre, nu_post, correction, max_iter, fgmres_restart, tol,
+                      cheb_lo, cheb_hi, omega, vanka_colored)
+
+    def MGSetup(self, opts):
+        _ck(lib().fm_mg_setup(self.h, C.byref(opts)))
+
+    def MGSolve(self, Eps=1e-8, MaxIter=100, Gamma=1, Nc_pre=2, Nc_coarse=40, Nc_post=2, opts=None,
+                raise_on_noconv=False):
+        """V-cycles until ||b-Ax||/||b|| <= Eps (FP64 residual) or MaxIter; returns MGReport."""
+        o = opts if opts is not None else self.opts(nu_pre=Nc_pre, nu_post=Nc_post, max_iter=MaxIter, tol=Eps)
+        rep = MGReport()
+        rc = lib().fm_mg_solve(self.h, C.byref(o), C.byref(rep))
+        _ck(rc, ok=(0,) if raise_on_noconv else (0, FM_ENOCONV))
+        return rep
+
+    def bench_spmv(self, L, prec, reps):
+        s = C.c_double()
+        _ck(lib().fm_bench_spmv(self.h, L, prec, reps, C.byref(s)))
+        return s.value
+
+
+class MGSolT(MGSolDA):
+    """Heat system (template_appl/nse/SRC/MGSolverT.C); one Q2 variable T."""
+
+    def __init__(self, ctx, mesh, alpha0=1.0, dt=0.1, rhocp=1.0, time_coeff=1.0, source=0.0, upwind=1.0,
+                 vel=(0.0, 0.0, 0.0)):
+        super().__init__(ctx, mesh, 1, 0)
+        self.params = HeatParams(alpha0, rhocp, dt, time_coeff, source, upwind, (C.c_double * 3)(*vel))
+
+    def GenMatRhs(self, time, Level, mode):
+        _ck(lib().fm_heat_genmatrhs(self.h, Level, mode, C.byref(self.params)))
+
+    def set_velocity(self, vel_top):
+        v = None if vel_top is None else np.ascontiguousarray(vel_top, np.float64)
+        self._keep_vel = v
+        _ck(lib().fm_system_set_velocity(self.h, None if v is None else v.ctypes.data_as(C.c_void_p)))
+
+    def MGTimeStep(self, time, opts=None):
+        """Assemble all levels (top with rhs), MGSolve, x_old <- x (MGSolT::MGTimeStep)."""
+        top = self._NoLevels - 1
+        self.GenMatRhs(time, top, 1)
+        for L in range(top):
+            self.GenMatRhs(time, L, 0)
+        rep = self.MGSolve(opts=opts)
+        self.set_xold(self.get_x())
+        return rep
+
+
+class MGSolNS(MGSolDA):
+    """Coupled Q2-Q1 Navier-Stokes (template_appl/nse/SRC/MGSolverNS.C, NS_EQUATIONS==1)."""
+
+    def __init__(self, ctx, mesh, IRe=0.01, dt=0.1, time_coeff=1.0, upwind=0.1, IFr=0.0, dirg=(0, 0, 0), src=(0, 0, 0)):
+        super().__init__(ctx, mesh, mesh.dim, 1)
+        self.params = NSParams(IRe, dt, time_coeff, upwind, IFr, (C.c_double * 3)(*dirg), (C.c_double * 3)(*src))
+
+    def GenMatRhs(self, time, Level, mode):
+        _ck(lib().fm_ns_genmatrhs(self.h, Level, mode, C.byref(self.params)))
+
+    def MGTimeStep(self, time, opts=None):
+        top = self._NoLevels - 1
+        self.GenMatRhs(time, top, 1)
+        for L in range(top):
+            self.GenMatRhs(time, L, 0)
+        rep = self.MGSolve(opts=opts)
+        self.set_xold(self.get_x())
+        return rep

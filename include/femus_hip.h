@@ -1,0 +1,180 @@
+/*
+ * femus_hip.h -- C ABI of the MI355X (gfx950) FEMuS hot path.
+ *
+ * One context = one HIP device + one HIP stream; calls on a context are not
+ * thread-safe.  Status codes: 0 = OK, <0 = error (see FM_E*); no exceptions
+ * cross this boundary.  Host arrays are copied in; device memory is owned by
+ * the library.  Vectors exchanged with the caller are in the reference DoF
+ * order (MGSolDA::_node_dof, src/MGSolverDA.C:880-948).
+ *
+ * The entry points replace the reference's operator interface:
+ *   MGSolBase / MGSolDA   include/MGSolverBase.h:24-389, include/MGSolverDA.h:55-376
+ *   SparseMatrixM / SparseMMatrixM / NumericVectorM / LinearSolverM
+ *                         contrib/matrix/sparse_matrixM.h:49-186,
+ *                         contrib/matrix/numeric_vectorM.h:284-293,
+ *                         contrib/matrix/linear_solverM.h:150-162
+ * Each declaration names the reference function it stands in for.
+ */
+#ifndef FEMUS_HIP_H
+#define FEMUS_HIP_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FM_OK 0
+#define FM_EINVAL -1      /* invalid argument                                 */
+#define FM_EHIP -2        /* HIP runtime error                                */
+#define FM_ENOCONV -3     /* solver did not reach the tolerance               */
+#define FM_ESINGULAR -4   /* singular coarse or local (Vanka) matrix          */
+#define FM_ESTATE -5      /* call out of order (e.g. solve before assembly)   */
+#define FM_ENOMEM -6      /* device allocation failed                         */
+
+typedef struct fm_ctx fm_ctx;
+typedef struct fm_mesh fm_mesh;
+typedef struct fm_system fm_system;
+
+/* ---- context ---------------------------------------------------------- */
+int fm_ctx_create(int device, fm_ctx **out);
+int fm_ctx_destroy(fm_ctx *ctx);
+const char *fm_last_error(void);
+
+/* ---- mesh hierarchy: gencase GenCase + printMesh -----------------------
+ * USER_APPL/gencase/src/MGGenCase.C:63-229 (build_square/build_cube +
+ * uniformly_refine), :234-758 (element/node ordering, offsets, node maps).
+ * dim = 2 (Quad9) or 3 (Hex27); nx0 = coarse intervals; nlev levels.      */
+int fm_mesh_create_box(int dim, const int nx0[3], const double lo[3], const double hi[3],
+                       int nlev, fm_mesh **out);
+int fm_mesh_destroy(fm_mesh *m);
+/* info[0..5] = dim, nlev, n2, n1, nsides, n_nodes_top   (MGMesh::read_c) */
+int fm_mesh_info(const fm_mesh *m, int *info);
+/* per level: elements, Q2 nodes (_NoNodes[L]), Q1 nodes                    */
+int fm_mesh_level(const fm_mesh *m, int L, int *n_el, int *n_q2, int *n_q1);
+/* coordinates SoA xyz[node + d*Ntop]  (MGMesh::_xyz, include/MGMesh.h:65-80) */
+int fm_mesh_coords(const fm_mesh *m, double *xyz);
+/* connectivity / neighbours / parents of level L (MGMesh::get_el_nod_conn
+ * src/MGMesh.C:216-236, get_el_neighbor :261-275); any pointer may be NULL */
+int fm_mesh_conn(const fm_mesh *m, int L, int *conn, int *neigh, int *parent);
+
+/* ---- system (one MGSolDA equation) --------------------------------------
+ * nq2 Q2 variables (u,v,w or T) followed by nq1 Q1 variables (p):
+ * MGSolDA ctor + init_dof (src/MGSolverDA.C:880-948).                      */
+int fm_system_create(fm_ctx *ctx, const fm_mesh *m, int nq2, int nq1, fm_system **out);
+int fm_system_destroy(fm_system *s);
+/* _Dim[L] */
+int64_t fm_system_ndof(const fm_system *s, int L);
+/* _node_dof[L]: copies (nq2+nq1)*Ntop ints (-1 where absent)               */
+int fm_system_node_dof(const fm_system *s, int L, int *node_dof);
+
+/* bc_read callback (MGSolDA::bc_read, template_appl/nse/SRC/User_NS.C):
+ * xp[dim] -> bc_neum[nvars] (0 = Dirichlet), bc_flag[nvars]; both preset 1. */
+typedef void (*fm_bc_fn)(const double *xp, int *bc_neum, int *bc_flag, void *user);
+/* MGSolDA::GenBc / GenBc_loop (src/MGSolverDA.C:696-862)                   */
+int fm_system_genbc(fm_system *s, fm_bc_fn fn, void *user);
+/* read back bc[0], bc[1] over the top-level DoFs                           */
+int fm_system_get_bc(const fm_system *s, int *bc0, int *bc1);
+
+/* MGSolDA::init(L) for all L: sparsity (ReadMatrix, MGGenCase.C:1370-1551),
+ * restriction with Dirichlet mask (ReadRest, src/MGSolverDA.C:2236-2444),
+ * prolongation (ReadProl, :2011-2230).  Builds device CSR + element maps.  */
+int fm_system_init(fm_system *s);
+/* Pattern / transfer export for parity checks (host arrays).
+ * which: 0 = A[L] (sorted CSR), 1 = Rst[L] (coarse L <- fine L+1), 2 = Prl[L] */
+int64_t fm_system_csr_nnz(const fm_system *s, int which, int L);
+int fm_system_csr_get(const fm_system *s, int which, int L, int64_t *rowptr, int *colind,
+                      double *val);
+
+/* x_old (top level, DoF order): SparseMatrix fields read by GenMatRhs      */
+int fm_system_set_xold(fm_system *s, const double *x_old_top);
+int fm_system_get_x(const fm_system *s, int L, double *x);
+int fm_system_set_x(fm_system *s, int L, const double *x);
+int fm_system_get_rhs(const fm_system *s, int L, double *b);
+int fm_system_set_rhs(fm_system *s, int L, const double *b);
+
+/* ---- assembly (GenMatRhs) ---------------------------------------------- */
+typedef struct {
+  double alpha0;      /* _IPrdl*_IRe                                        */
+  double rhocp;       /* 1 (CONST)                                          */
+  double dt;          /* penalty det/dt and time terms                      */
+  double time_coeff;  /* 1 = reference transient terms, 0 = steady          */
+  double source;      /* volume source                                      */
+  double upwind;      /* UP_WIND_T                                          */
+  double vel_const[3];/* advection velocity when no field is set            */
+} fm_heat_params;
+
+typedef struct {
+  double IRe, dt, time_coeff, upwind, IFr;
+  double dirg[3], src[3];
+} fm_ns_params;
+
+/* per-element conductivity / viscosity of level L (NULL = 1 / IRe)        */
+int fm_system_set_elem_coef(fm_system *s, int L, const double *coef);
+/* nodal Q2 advection velocity for heat [dim*Ntop] (NULL = vel_const)       */
+int fm_system_set_velocity(fm_system *s, const double *vel_top);
+/* MGSolT::GenMatRhs (template_appl/nse/SRC/MGSolverT.C:93-525)             */
+int fm_heat_genmatrhs(fm_system *s, int L, int mode, const fm_heat_params *p);
+/* MGSolNS::GenMatRhs (template_appl/nse/SRC/MGSolverNS.C:108-717)          */
+int fm_ns_genmatrhs(fm_system *s, int L, int mode, const fm_ns_params *p);
+
+/* ---- operators on level vectors ----------------------------------------
+ * LaspackVectorM::resid (contrib/matrix/laspack_vectorM.C:147-168),
+ * Mul_QV (contrib/laspack/operats.c:782-1047),
+ * NumericVectorM::matrix_mult with Rst / Prl (laspack_vectorM.C:170-189).   */
+int fm_spmv(fm_system *s, int L, const double *x_host, double *y_host);
+int fm_residual(fm_system *s, int L);            /* res[L] = b[L] - A[L] x[L] */
+int fm_restrict(fm_system *s, int L);            /* b[L-1] = Rst[L-1] res[L]   */
+int fm_prolong(fm_system *s, int L);             /* res[L] = Prl[L] x[L-1]     */
+int fm_get_res(const fm_system *s, int L, double *r);
+
+/* ---- multigrid solve (MGSolBase::MGSolve / MGStep, src/MGSolverBase.C:163-289) */
+#define FM_PREC_FP64 0        /* all-FP64 V-cycle                               */
+#define FM_PREC_FP32_IR 1     /* FP32 V-cycle inside FP64 iterative refinement  */
+#define FM_PREC_FP32_FGMRES 2 /* FP32 V-cycle preconditioning FP64 FGMRES       */
+#define FM_PREC_FP64_FGMRES 3 /* FP64 V-cycle preconditioning FP64 FGMRES       */
+#define FM_SMOOTH_CHEBYSHEV 0
+#define FM_SMOOTH_JACOBI 1
+#define FM_SMOOTH_VANKA 2
+typedef struct {
+  int precision;      /* FM_PREC_*                                          */
+  int smoother;       /* FM_SMOOTH_*                                        */
+  int nu_pre, nu_post;/* sweeps; doubled below the top level (MGStep)       */
+  int correction;     /* 0 = reference (x += 0.95 P xc if coarse_rest < rest), 1 = plain */
+  int max_iter;       /* cycles (or outer iterations)                        */
+  int fgmres_restart;
+  double tol;         /* ||b - A x||_2 / ||b||_2 target (FP64 residual)      */
+  double cheb_lo, cheb_hi;   /* Chebyshev interval, fractions of lambda_max  */
+  double omega;       /* Jacobi / Vanka damping                              */
+  int vanka_colored;  /* 1 = multiplicative over 2^dim element colours       */
+} fm_mg_opts;
+
+typedef struct {
+  int iters;          /* cycles / outer iterations done                      */
+  int converged;
+  double rel_resid;   /* final ||b-Ax||/||b|| (FP64)                         */
+  double hist[256];   /* relative residual after each iteration              */
+  double seconds;     /* device time of the solve (HIP events)               */
+} fm_mg_report;
+
+int fm_mg_setup(fm_system *s, const fm_mg_opts *o);   /* eigen bounds, coarse LU, Vanka inverses */
+int fm_mg_solve(fm_system *s, const fm_mg_opts *o, fm_mg_report *rep);   /* x[top] in/out, b[top] */
+
+/* ---- discrete adjoint -------------------------------------------------- */
+/* Transposed multigrid solve A^T lambda = g on the assembled hierarchy:
+ * restriction and prolongation swap roles (Prl^T, Rst^T), smoothers act on A^T. */
+int fm_adjoint_solve(fm_system *s, const fm_mg_opts *o, const double *g_host,
+                     double *lambda_host, fm_mg_report *rep);
+/* dJ/dk_e = -lambda^T (dA/dk_e) x  for the heat system (per top element)    */
+int fm_heat_grad_coef(fm_system *s, const fm_heat_params *p, const double *lambda_host,
+                      const double *x_host, double *grad_host);
+
+/* ---- timing helpers for benchmarks ------------------------------------- */
+/* Run `reps` SpMVs y = A[L] x on device buffers (prec 0 = FP64, 1 = FP32);
+ * returns the average kernel time in seconds via *sec (HIP events on the
+ * library stream). */
+int fm_bench_spmv(fm_system *s, int L, int prec, int reps, double *sec);
+int fm_sync(fm_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,165 @@
+"""GPU parity of the heat (Q2) path through the C ABI against the CPU oracle.
+
+Mesh hierarchy, DoF map, BC map, sparsity, restriction (with Dirichlet mask)
+and prolongation must match bit-exactly; assembled FP64 matrices/RHS to
+rtol 1e-12; multigrid solutions to the FP64 direct solve within the stated
+tolerances.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from exfemus_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    (2, [2, 2], 3),
+    (3, [2, 2, 2], 3),
+    (3, [1, 2, 3], 2),
+]
+
+
+def _build(femus, oracle, ctx, dim, nx, nlev, k_sigma=1.0, source=1.0, time_coeff=0.0):
+    lo, hi = [0.0] * dim, [1.0] * dim
+    pm = femus.Mesh(dim, nx, lo, hi, nlev)
+    om = oracle.Mesh(dim, nx, lo, hi, nlev)
+    sysT = femus.MGSolT(ctx, pm, alpha0=1.0, dt=0.1, time_coeff=time_coeff, source=source, upwind=1.0)
+    sysT.GenBc(configs.heat_dirichlet_all)
+    sysT.init()
+    ne_top = pm.level(nlev - 1)[0]
+    k_top = configs.lognormal_k(ne_top, sigma=k_sigma)
+    ks = configs.coarsen_coef(k_top, nlev, 2 ** dim)
+    for L in range(nlev):
+        sysT.set_elem_coef(L, ks[L])
+    return pm, om, sysT, ks
+
+
+@pytest.mark.parametrize("dim,nx,nlev", CASES)
+def test_mesh_and_maps_bitexact(femus, oracle, ctx, dim, nx, nlev):
+    pm, om, sysT, _ = _build(femus, oracle, ctx, dim, nx, nlev)
+    assert pm.N == om.N
+    np.testing.assert_array_equal(pm.coords(), om.coords())
+    for L in range(nlev):
+        assert pm.level(L) == om.level(L)
+        pc, pn, pp = pm.conn(L)
+        oc, on, op = om.conn(L)
+        np.testing.assert_array_equal(pc, oc)
+        np.testing.assert_array_equal(pn, on)
+        np.testing.assert_array_equal(pp, op)
+        od, ndof = oracle.init_dof(om, L, 1, 0)
+        np.testing.assert_array_equal(sysT.node_dof(L), od)
+        assert sysT._Dim[L] == ndof
+    od_top, _ = oracle.init_dof(om, nlev - 1, 1, 0)
+    ob0, ob1 = oracle.genbc(om, 1, 0, od_top, configs.heat_dirichlet_all)
+    pb0, pb1 = sysT.bc()
+    np.testing.assert_array_equal(pb0, ob0)
+    np.testing.assert_array_equal(pb1, ob1)
+
+
+@pytest.mark.parametrize("dim,nx,nlev", CASES)
+def test_operators_bitexact(femus, oracle, ctx, dim, nx, nlev):
+    pm, om, sysT, _ = _build(femus, oracle, ctx, dim, nx, nlev)
+    dofs = [oracle.init_dof(om, L, 1, 0)[0] for L in range(nlev)]
+    top = nlev - 1
+    ob0, _ = oracle.genbc(om, 1, 0, dofs[top], configs.heat_dirichlet_all)
+    for L in range(nlev):
+        rp, ci = oracle.system_pattern(om, L, 1, 0, dofs[L])
+        prp, pci, _ = sysT.csr(0, L)
+        np.testing.assert_array_equal(prp, rp)
+        np.testing.assert_array_equal(pci, ci)
+        if L < top:
+            rrp, rci, rv = oracle.system_rest(om, L, 1, 0, dofs[L], dofs[L + 1], dofs[top], ob0)
+            q = sysT.csr(1, L)
+            np.testing.assert_array_equal(q[0], rrp)
+            np.testing.assert_array_equal(q[1], rci)
+            np.testing.assert_array_equal(q[2], rv)
+        if L > 0:
+            prp2, pci2, pv2 = oracle.system_prol(om, L, 1, 0, dofs[L], dofs[L - 1])
+            q = sysT.csr(2, L)
+            np.testing.assert_array_equal(q[0], prp2)
+            np.testing.assert_array_equal(q[1], pci2)
+            np.testing.assert_array_equal(q[2], pv2)
+
+
+@pytest.mark.parametrize("dim,nx,nlev", CASES)
+@pytest.mark.parametrize("time_coeff", [0.0, 1.0])
+def test_assembly_parity(femus, oracle, ctx, dim, nx, nlev, time_coeff):
+    pm, om, sysT, ks = _build(femus, oracle, ctx, dim, nx, nlev, time_coeff=time_coeff)
+    dofs = [oracle.init_dof(om, L, 1, 0)[0] for L in range(nlev)]
+    top = nlev - 1
+    ob0, ob1 = oracle.genbc(om, 1, 0, dofs[top], configs.heat_dirichlet_all)
+    rng = np.random.default_rng(7)
+    xold = rng.standard_normal(sysT._Dim[top])
+    sysT.set_xold(xold)
+    p = oracle.HeatParams(1.0, 1.0, 0.1, time_coeff, 1.0, 1.0, (0.0, 0.0, 0.0))
+    for L in range(nlev):
+        mode = 1 if L == top else 0
+        sysT.GenMatRhs(0.0, L, mode)
+        rp, ci = oracle.system_pattern(om, L, 1, 0, dofs[L])
+        ov, orhs = oracle.heat_genmatrhs(om, L, mode, p, dofs[L], dofs[top], ob0, ob1, xold, rp, ci, k_elem=ks[L])
+        _, _, pv = sysT.csr(0, L)
+        scale = np.abs(ov).max()
+        np.testing.assert_allclose(pv, ov, rtol=1e-12, atol=1e-13 * scale)
+        if mode == 1:
+            np.testing.assert_allclose(sysT.get_rhs(L), orhs, rtol=1e-12, atol=1e-13 * np.abs(orhs).max())
+
+
+def _direct(sysT, L):
+    rp, ci, v = sysT.csr(0, L)
+    n = len(rp) - 1
+    A = sp.csr_matrix((v, ci, rp), shape=(n, n))
+    return spla.spsolve(A.tocsc(), sysT.get_rhs(L)), A
+
+
+@pytest.mark.parametrize("dim,nx,nlev", [(2, [2, 2], 4), (3, [2, 2, 2], 3)])
+@pytest.mark.parametrize("prec", [0, 1, 2])
+def test_mg_solution(femus, oracle, ctx, dim, nx, nlev, prec):
+    pm, om, sysT, ks = _build(femus, oracle, ctx, dim, nx, nlev)
+    top = nlev - 1
+    sysT.set_xold(np.zeros(sysT._Dim[top]))
+    for L in range(nlev):
+        sysT.GenMatRhs(0.0, L, 1 if L == top else 0)
+    xref, A = _direct(sysT, top)
+    sysT.set_x(np.zeros(sysT._Dim[top]))
+    rep = sysT.MGSolve(opts=sysT.opts(precision=prec, tol=1e-10, max_iter=60))
+    x = sysT.get_x()
+    b = sysT.get_rhs()
+    rel = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+    assert rep.converged == 1, (rep.iters, rep.rel_resid)
+    assert rel <= 1.5e-10
+    err = np.linalg.norm(x - xref) / np.linalg.norm(xref)
+    assert err <= 1e-8
+
+
+@pytest.mark.parametrize("dim,nx,nlev", [(2, [2, 2], 3), (3, [2, 2, 2], 3)])
+def test_mg_matches_oracle_cycles(femus, oracle, ctx, dim, nx, nlev):
+    """Same V-cycle (Chebyshev smoother, reference correction rule, LU coarse) on GPU and oracle:
+    the FP64 residual histories agree to fp tolerance."""
+    pm, om, sysT, ks = _build(femus, oracle, ctx, dim, nx, nlev)
+    top = nlev - 1
+    sysT.set_xold(np.zeros(sysT._Dim[top]))
+    for L in range(nlev):
+        sysT.GenMatRhs(0.0, L, 1 if L == top else 0)
+    H = oracle.Hier(nlev)
+    for L in range(nlev):
+        rp, ci, v = sysT.csr(0, L)
+        H.set_A(L, rp, ci, v)
+        if L < top:
+            r = sysT.csr(1, L)
+            H.set_R(L, r[0], r[1], r[2], sysT._Dim[L + 1])
+        if L > 0:
+            q = sysT.csr(2, L)
+            H.set_P(L, q[0], q[1], q[2], sysT._Dim[L - 1])
+    b = sysT.get_rhs()
+    for corr in (0, 1):
+        xo, hist_o = H.solve(b, smoother=1, coarse=1, correction=corr, nu_pre=2, nu_post=2, max_cycles=6,
+                             eps=1e-30, cheb_lo=0.2, cheb_hi=1.05, ref_exit=0)
+        sysT.set_x(np.zeros(sysT._Dim[top]))
+        rep = sysT.MGSolve(opts=sysT.opts(precision=0, correction=corr, tol=1e-30, max_iter=6, cheb_lo=0.2,
+                                          cheb_hi=1.05))
+        hist_g = np.array(rep.hist[:rep.iters])
+        assert rep.iters == len(hist_o)
+        np.testing.assert_allclose(hist_g, hist_o, rtol=1e-6)
+        np.testing.assert_allclose(sysT.get_x(), xo, rtol=1e-7, atol=1e-12 * np.abs(xo).max())
