@@ -55,6 +55,7 @@ extern "C" int fm_ctx_create(int device, fm_ctx **out) {
 
 extern "C" int fm_ctx_destroy(fm_ctx *c) {
   if (!c) return FM_EINVAL;
+  if (c->nsys > 0) { fm_set_error("fm_ctx_destroy: systems still alive on this context"); return FM_ESTATE; }
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   dfree(c->scratch);
@@ -196,11 +197,83 @@ extern "C" int fm_system_create(fm_ctx *ctx, const fm_mesh *m, int nq2, int nq1,
   int64_t nt = s->Dim[m->nlev - 1];
   s->bc0.assign(nt, 1); s->bc1.assign(nt, 1);
   s->lev.resize(m->nlev);
+  ctx->nsys++;
   *out = s;
   return FM_OK;
 }
 
-static void free_csr(DevCSR &c) { dfree(c.rp); dfree(c.ci); dfree(c.v); dfree(c.vf); }
+void fm_free_csr(DevCSR &c) {
+  dfree(c.rp); dfree(c.ci); dfree(c.v); dfree(c.vf);
+  DevSELL &q = c.sell;
+  dfree(q.sl_off); dfree(q.sl_w); dfree(q.perm); dfree(q.cols); dfree(q.map); dfree(q.v); dfree(q.vf);
+  q.nsl = 0; q.nslot = 0;
+}
+static void free_csr(DevCSR &c) { fm_free_csr(c); }
+
+// SELL-64 copy of a CSR pattern (sigma window 4096 rows, rows sorted by
+// decreasing length inside each window; padding repeats the row's last column).
+int fm_build_sell(hipStream_t st, DevCSR &c) {
+  const int C = 64, W = 4096;
+  int64_t n = c.nr;
+  const std::vector<int64_t> &rp = c.h_rp;
+  const std::vector<int> &ci = c.h_ci;
+  if ((int64_t)rp.size() != n + 1) { fm_set_error("SELL build without host pattern"); return FM_ESTATE; }
+  std::vector<int> order(n);
+  for (int64_t i = 0; i < n; i++) order[i] = (int)i;
+  for (int64_t w0 = 0; w0 < n; w0 += W) {
+    int64_t w1 = std::min<int64_t>(n, w0 + W);
+    std::stable_sort(order.begin() + w0, order.begin() + w1, [&](int a, int b) {
+      return rp[a + 1] - rp[a] > rp[b + 1] - rp[b];
+    });
+  }
+  int64_t nsl = (n + C - 1) / C;
+  std::vector<int64_t> off(nsl + 1, 0);
+  std::vector<int> wd(nsl), perm((size_t)nsl * C, -1);
+  for (int64_t s = 0; s < nsl; s++) {
+    int w = 0;
+    for (int l = 0; l < C; l++) {
+      int64_t p = s * C + l;
+      if (p >= n) break;
+      int r = order[p];
+      perm[p] = r;
+      w = std::max<int>(w, (int)(rp[r + 1] - rp[r]));
+    }
+    wd[s] = w;
+    off[s + 1] = off[s] + (int64_t)w * C;
+  }
+  int64_t nslot = off[nsl];
+  std::vector<int> cols(nslot, 0), map(nslot, -1);
+#pragma omp parallel for schedule(static)
+  for (int64_t s = 0; s < nsl; s++)
+    for (int l = 0; l < C; l++) {
+      int r = perm[s * C + l];
+      int len = r >= 0 ? (int)(rp[r + 1] - rp[r]) : 0;
+      int lastc = len > 0 ? ci[rp[r] + len - 1] : 0;
+      for (int k = 0; k < wd[s]; k++) {
+        int64_t slot = off[s] + (int64_t)k * C + l;
+        if (k < len) { cols[slot] = ci[rp[r] + k]; map[slot] = (int)(rp[r] + k); }
+        else cols[slot] = lastc;
+      }
+    }
+  DevSELL &q = c.sell;
+  dfree(q.sl_off); dfree(q.sl_w); dfree(q.perm); dfree(q.cols); dfree(q.map); dfree(q.v); dfree(q.vf);
+  q.nsl = nsl; q.nslot = nslot;
+  FM_CHECK(dalloc(&q.sl_off, nsl + 1)); FM_CHECK(dalloc(&q.sl_w, nsl)); FM_CHECK(dalloc(&q.perm, nsl * C));
+  FM_CHECK(dalloc(&q.cols, nslot)); FM_CHECK(dalloc(&q.map, nslot)); FM_CHECK(dalloc(&q.v, nslot)); FM_CHECK(dalloc(&q.vf, nslot));
+  FM_HIP(hipMemcpyAsync(q.sl_off, off.data(), (nsl + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  FM_HIP(hipMemcpyAsync(q.sl_w, wd.data(), nsl * sizeof(int), hipMemcpyHostToDevice, st));
+  FM_HIP(hipMemcpyAsync(q.perm, perm.data(), nsl * C * sizeof(int), hipMemcpyHostToDevice, st));
+  FM_HIP(hipMemcpyAsync(q.cols, cols.data(), nslot * sizeof(int), hipMemcpyHostToDevice, st));
+  FM_HIP(hipMemcpyAsync(q.map, map.data(), nslot * sizeof(int), hipMemcpyHostToDevice, st));
+  FM_HIP(hipStreamSynchronize(st));
+  return fm_sync_sell(st, c);
+}
+
+// refresh SELL values (FP64 and FP32) from the CSR values
+int fm_sync_sell(hipStream_t st, DevCSR &c) {
+  if (c.sell.nsl == 0) return 0;
+  return fmk::csr2sell(st, c.sell.nslot, c.sell.map, c.v, c.sell.v, c.sell.vf);
+}
 
 extern "C" int fm_system_destroy(fm_system *s) {
   if (!s) return FM_EINVAL;
@@ -212,11 +285,13 @@ extern "C" int fm_system_destroy(fm_system *s) {
     dfree(l.xf); dfree(l.bf); dfree(l.rf); dfree(l.tf); dfree(l.df); dfree(l.d2f); dfree(l.dinvf);
     dfree(l.conn); dfree(l.eldof); dfree(l.elpos); dfree(l.bmask); dfree(l.color_elems); dfree(l.coef);
     dfree(l.Ainv); dfree(l.Ainvf); dfree(l.vinv); dfree(l.vwt); dfree(l.vwtf); dfree(l.nrm); dfree(l.res);
+    dfree(l.sb); dfree(l.sx); dfree(l.vone);
   }
   dfree(s->d_bc0); dfree(s->d_bc1); dfree(s->d_dof_top); dfree(s->xyz); dfree(s->x_old); dfree(s->vel);
   SysExtra &ex = extra(s);
   dfree(ex.d_tab);
   for (size_t i = 0; i < g_extra.size(); i++) if (g_extra[i].first == s) { g_extra.erase(g_extra.begin() + i); break; }
+  s->ctx->nsys--;
   delete s;
   return FM_OK;
 }
@@ -293,16 +368,10 @@ static int upload_csr(hipStream_t st, DevCSR &c, int64_t nr, int64_t nc, std::ve
   c.h_rp.swap(rp);
   c.h_ci.swap(ci);
   if (v) c.h_v.swap(*v);
-  return 0;
+  return fm_build_sell(st, c);
 }
 
-template <typename T> static CsrView<T> view(const DevCSR &c);
-template <> CsrView<double> view<double>(const DevCSR &c) {
-  CsrView<double> v; v.nr = c.nr; v.rp = c.rp; v.ci = c.ci; v.v = c.v; v.lpr = fmk::lanes_for(c.nnz, c.nr); return v;
-}
-template <> CsrView<float> view<float>(const DevCSR &c) {
-  CsrView<float> v; v.nr = c.nr; v.rp = c.rp; v.ci = c.ci; v.v = c.vf; v.lpr = fmk::lanes_for(c.nnz, c.nr); return v;
-}
+template <typename T> static inline CsrView<T> view(const DevCSR &c) { return fm_view<T>(c); }
 
 extern "C" int fm_system_init(fm_system *s) {
   if (!s) return FM_EINVAL;
@@ -359,6 +428,8 @@ extern "C" int fm_system_init(fm_system *s) {
     FM_CHECK(dalloc(&l.xf, l.n)); FM_CHECK(dalloc(&l.bf, l.n)); FM_CHECK(dalloc(&l.rf, l.n)); FM_CHECK(dalloc(&l.tf, l.n));
     FM_CHECK(dalloc(&l.df, l.n)); FM_CHECK(dalloc(&l.d2f, l.n)); FM_CHECK(dalloc(&l.dinvf, l.n));
     FM_CHECK(dalloc(&l.nrm, 8));
+    dfree(l.sb); dfree(l.sx);
+    if (L == top) { FM_CHECK(dalloc(&l.sb, l.n)); FM_CHECK(dalloc(&l.sx, l.n)); }
     FM_HIP(hipMemsetAsync(l.x, 0, l.n * sizeof(double), st));
     FM_HIP(hipMemsetAsync(l.b, 0, l.n * sizeof(double), st));
     FM_HIP(hipMemsetAsync(l.res, 0, l.n * sizeof(double), st));
@@ -477,6 +548,7 @@ static int after_assembly(fm_system *s, int L) {
   Level &l = s->lev[L];
   hipStream_t st = s->ctx->stream;
   FM_CHECK(fmk::cast_vals(st, l.A.nnz, l.A.v, l.A.vf));
+  FM_CHECK(fm_sync_sell(st, l.A));
   FM_CHECK(fmk::diag_inv(st, l.n, l.A.rp, l.A.ci, l.A.v, l.dinv, l.dinvf));
   l.lam = 0.;
   s->mg_ready = false;
@@ -569,18 +641,22 @@ extern "C" int fm_prolong(fm_system *s, int L) {
 
 extern "C" int fm_bench_spmv(fm_system *s, int L, int prec, int reps, double *sec) {
   FM_CHECK(check_level(s, L));
-  if (reps < 1 || !sec) return FM_EINVAL;
+  if (reps < 1 || !sec || prec < 0 || prec > 3) return FM_EINVAL;
   Level &l = s->lev[L];
   hipStream_t st = s->ctx->stream;
+  // prec 0 / 1: SELL-64 FP64 / FP32 (the solver's format); 2 / 3: CSR FP64 / FP32
+  CsrView<double> v64 = view<double>(l.A);
+  CsrView<float> v32 = view<float>(l.A);
+  if (prec >= 2) { v64.nsl = 0; v32.nsl = 0; }
+  bool f32 = (prec & 1) != 0;
   FM_CHECK(fmk::fill<double>(st, l.n, l.t, 1.0));
   FM_CHECK(fmk::fill<float>(st, l.n, l.tf, 1.0f));
-  // warm-up
-  if (prec == 0) FM_CHECK(fmk::spmv(st, view<double>(l.A), l.t, l.d));
-  else FM_CHECK(fmk::spmv(st, view<float>(l.A), l.tf, l.df));
+  if (!f32) FM_CHECK(fmk::spmv(st, v64, l.t, l.d));
+  else FM_CHECK(fmk::spmv(st, v32, l.tf, l.df));
   FM_HIP(hipEventRecord(s->ctx->ev0, st));
   for (int k = 0; k < reps; k++) {
-    if (prec == 0) FM_CHECK(fmk::spmv(st, view<double>(l.A), l.t, l.d));
-    else FM_CHECK(fmk::spmv(st, view<float>(l.A), l.tf, l.df));
+    if (!f32) FM_CHECK(fmk::spmv(st, v64, l.t, l.d));
+    else FM_CHECK(fmk::spmv(st, v32, l.tf, l.df));
   }
   FM_HIP(hipEventRecord(s->ctx->ev1, st));
   FM_HIP(hipEventSynchronize(s->ctx->ev1));

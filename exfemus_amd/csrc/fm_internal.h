@@ -27,6 +27,19 @@ struct fm_mesh {
   std::vector<std::vector<int>> conn, neigh, parent, egrid;  // egrid: element grid index (3 ints)
 };
 
+// Sliced ELLPACK copy (SELL-C-sigma, C = 64 = one wavefront): rows sorted by
+// length inside windows of 4096, slice s holds rows perm[64s .. 64s+63] with
+// width sl_w[s]; entry k of lane l sits at sl_off[s] + 64k + l (column-major
+// inside the slice, so a wavefront's loads are contiguous).  map[slot] is
+// the CSR position of the entry (-1 for padding).
+struct DevSELL {
+  int64_t nsl = 0, nslot = 0;
+  int64_t *sl_off = nullptr;
+  int *sl_w = nullptr, *perm = nullptr, *cols = nullptr, *map = nullptr;
+  double *v = nullptr;
+  float *vf = nullptr;
+};
+
 // Device CSR (int64 row pointers, int32 columns); values in FP64 and FP32.
 struct DevCSR {
   int64_t nr = 0, nc = 0, nnz = 0;
@@ -38,6 +51,7 @@ struct DevCSR {
   std::vector<int64_t> h_rp;
   std::vector<int> h_ci;
   std::vector<double> h_v;
+  DevSELL sell;
 };
 
 struct Level {
@@ -64,11 +78,14 @@ struct Level {
   // Vanka local inverses (nel * nloc * nloc, FP64)
   double *vinv = nullptr;
   double *vwt = nullptr;   // 1 / multiplicity per dof
+  double *vone = nullptr;  // all ones (coloured Vanka weights)
   float *vwtf = nullptr;
   // norms for the reference correction rule
   double *nrm = nullptr;   // device scalars
   // residual for transfers
   double *res = nullptr;
+  // saved rhs / iterate while an outer Krylov method borrows the level's cycle vectors
+  double *sb = nullptr, *sx = nullptr;
 };
 
 struct fm_ctx {
@@ -78,6 +95,7 @@ struct fm_ctx {
   double *scratch = nullptr;    // reduction scratch (device)
   double *h_scratch = nullptr;  // pinned host mirror
   int nblocks_red = 1024;
+  int nsys = 0;                 // live systems created on this context
 };
 
 struct fm_system {
@@ -112,10 +130,25 @@ extern const int fm_quad9_ref[9][2];
 extern const int fm_face2d[4][3];
 extern const int fm_face3d[6][9];
 
-// kernels.hip launchers
-namespace fmk {
-int spmv(hipStream_t s, const DevCSR &A, const double *x, double *y);
-int spmvf(hipStream_t s, const DevCSR &A, const float *x, float *y);
-int resid(hipStream_t s, const DevCSR &A, const double *b, const double *x, double *r);
-int residf(hipStream_t s, const DevCSR &A, const float *b, const float *x, float *r);
+// views of a device matrix for the kernel launchers (SELL copy when present)
+#include "fm_kernels.h"
+template <typename T> CsrView<T> fm_view(const DevCSR &c);
+template <> inline CsrView<double> fm_view<double>(const DevCSR &c) {
+  CsrView<double> v;
+  v.nr = c.nr; v.rp = c.rp; v.ci = c.ci; v.v = c.v;
+  v.lpr = fmk::lanes_for(c.nnz, c.nr);
+  v.nsl = c.sell.nsl; v.sl_off = c.sell.sl_off; v.sl_w = c.sell.sl_w; v.perm = c.sell.perm; v.scols = c.sell.cols;
+  v.sv = c.sell.v;
+  return v;
 }
+template <> inline CsrView<float> fm_view<float>(const DevCSR &c) {
+  CsrView<float> v;
+  v.nr = c.nr; v.rp = c.rp; v.ci = c.ci; v.v = c.vf;
+  v.lpr = fmk::lanes_for(c.nnz, c.nr);
+  v.nsl = c.sell.nsl; v.sl_off = c.sell.sl_off; v.sl_w = c.sell.sl_w; v.perm = c.sell.perm; v.scols = c.sell.cols;
+  v.sv = c.sell.vf;
+  return v;
+}
+int fm_build_sell(hipStream_t st, DevCSR &c);
+int fm_sync_sell(hipStream_t st, DevCSR &c);
+void fm_free_csr(DevCSR &c);

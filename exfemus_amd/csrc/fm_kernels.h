@@ -10,6 +10,11 @@ struct CsrView {
   const int *ci = nullptr;
   const T *v = nullptr;
   int lpr = 16;  // lanes per row
+  // SELL-64 copy (used when nsl > 0)
+  int64_t nsl = 0;
+  const int64_t *sl_off = nullptr;
+  const int *sl_w = nullptr, *perm = nullptr, *scols = nullptr;
+  const T *sv = nullptr;
 };
 
 // Reference-element tables (src/MGFE.C:195-924 layout: phi[s*ng+qp],
@@ -60,6 +65,7 @@ template <typename T> int dense_mv(hipStream_t s, int n, const T *M, const T *b,
 int diag_inv(hipStream_t s, int64_t n, const int64_t *rp, const int *ci, const double *v, double *d64, float *d32);
 int cast_vals(hipStream_t s, int64_t n, const double *v, float *vf);
 int elpos(hipStream_t s, int nel, int nloc, const int *eldof, const int64_t *rp, const int *ci, int *pos);
+int csr2sell(hipStream_t s, int64_t nslot, const int *map, const double *csr_v, double *sv, float *svf);
 
 // assemble.hip
 int heat_assemble(hipStream_t s, int dim, int nel_color, const int *elems, const int *conn, const int *eldof,

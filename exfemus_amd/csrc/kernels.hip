@@ -35,19 +35,8 @@ struct EpArgs {
   const double *cond;  // EP_ADD: if non-null, apply only when cond[0] < cond[1] (reference rule)
 };
 
-template <int LPR, typename T, int EP>
-__global__ __launch_bounds__(256) void k_spmv(int64_t nrows, const int64_t *__restrict__ rp,
-                                              const int *__restrict__ ci, const T *__restrict__ v,
-                                              const T *__restrict__ xg, EpArgs<T> ep) {
-  const int lane = threadIdx.x % LPR;
-  const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / LPR;
-  if (row >= nrows) return;
-  const int64_t q0 = rp[row], q1 = rp[row + 1];
-  T sum = 0;
-  for (int64_t q = q0 + lane; q < q1; q += LPR) sum += v[q] * xg[ci[q]];
-#pragma unroll
-  for (int off = LPR / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, LPR);
-  if (lane != 0) return;
+template <typename T, int EP>
+__device__ inline void apply_ep(int64_t row, T sum, const T *__restrict__ xg, const EpArgs<T> &ep) {
   if (EP == EP_Y) {
     ep.y[row] = sum;
   } else if (EP == EP_RES) {
@@ -66,14 +55,76 @@ __global__ __launch_bounds__(256) void k_spmv(int64_t nrows, const int64_t *__re
   }
 }
 
+// CSR: LPR lanes per row, shuffle reduction
+template <int LPR, typename T, int EP>
+__global__ __launch_bounds__(256) void k_spmv(int64_t nrows, const int64_t *__restrict__ rp,
+                                              const int *__restrict__ ci, const T *__restrict__ v,
+                                              const T *__restrict__ xg, EpArgs<T> ep) {
+  const int lane = threadIdx.x % LPR;
+  const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / LPR;
+  if (row >= nrows) return;
+  const int64_t q0 = rp[row], q1 = rp[row + 1];
+  T sum = 0;
+  for (int64_t q = q0 + lane; q < q1; q += LPR) sum += v[q] * xg[ci[q]];
+#pragma unroll
+  for (int off = LPR / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, LPR);
+  if (lane != 0) return;
+  apply_ep<T, EP>(row, sum, xg, ep);
+}
+
+// SELL-64: one wavefront per slice, one lane per row, entries of a row summed
+// in order; matrix values and columns are streamed with non-temporal loads so
+// that the gathered vector stays resident in L2 / Infinity Cache.
 template <typename T, int EP>
-static int launch_spmv(hipStream_t s, int lpr, int64_t n, const int64_t *rp, const int *ci, const T *v,
-                       const T *x, const EpArgs<T> &ep) {
-  if (n <= 0) return 0;
+__global__ __launch_bounds__(256) void k_sell(int64_t nsl, const int64_t *__restrict__ sl_off,
+                                              const int *__restrict__ sl_w, const int *__restrict__ perm,
+                                              const int *__restrict__ cols, const T *__restrict__ vals,
+                                              const T *__restrict__ xg, EpArgs<T> ep) {
+  const int64_t s = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (s >= nsl) return;
+  const int lane = threadIdx.x & 63;
+  const int row = perm[s * 64 + lane];
+  const int w = sl_w[s];
+  const T *vp = vals + sl_off[s] + lane;
+  const int *cp = cols + sl_off[s] + lane;
+  T sum = 0;
+  int k = 0;
+  for (; k + 4 <= w; k += 4) {
+    const T v0 = __builtin_nontemporal_load(vp + (k + 0) * 64);
+    const T v1 = __builtin_nontemporal_load(vp + (k + 1) * 64);
+    const T v2 = __builtin_nontemporal_load(vp + (k + 2) * 64);
+    const T v3 = __builtin_nontemporal_load(vp + (k + 3) * 64);
+    const int c0 = __builtin_nontemporal_load(cp + (k + 0) * 64);
+    const int c1 = __builtin_nontemporal_load(cp + (k + 1) * 64);
+    const int c2 = __builtin_nontemporal_load(cp + (k + 2) * 64);
+    const int c3 = __builtin_nontemporal_load(cp + (k + 3) * 64);
+    const T x0 = xg[c0], x1 = xg[c1], x2 = xg[c2], x3 = xg[c3];
+    sum += v0 * x0;
+    sum += v1 * x1;
+    sum += v2 * x2;
+    sum += v3 * x3;
+  }
+  for (; k < w; k++) sum += __builtin_nontemporal_load(vp + k * 64) * xg[__builtin_nontemporal_load(cp + k * 64)];
+  if (row < 0) return;
+  apply_ep<T, EP>(row, sum, xg, ep);
+}
+
+template <typename T, int EP>
+static int launch_spmv(hipStream_t s, const CsrView<T> &A, const T *x, const EpArgs<T> &ep) {
+  if (A.nr <= 0) return 0;
   const int block = 256;
-  int64_t threads = n * lpr;
+  if (A.nsl > 0) {
+    dim3 grid((unsigned)((A.nsl + 3) / 4));
+    hipLaunchKernelGGL((k_sell<T, EP>), grid, dim3(block), 0, s, A.nsl, A.sl_off, A.sl_w, A.perm, A.scols, A.sv, x, ep);
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+  }
+  int64_t threads = A.nr * A.lpr;
   dim3 grid((unsigned)((threads + block - 1) / block));
-  switch (lpr) {
+  int64_t n = A.nr;
+  const int64_t *rp = A.rp;
+  const int *ci = A.ci;
+  const T *v = A.v;
+  switch (A.lpr) {
     case 4: hipLaunchKernelGGL((k_spmv<4, T, EP>), grid, dim3(block), 0, s, n, rp, ci, v, x, ep); break;
     case 8: hipLaunchKernelGGL((k_spmv<8, T, EP>), grid, dim3(block), 0, s, n, rp, ci, v, x, ep); break;
     case 16: hipLaunchKernelGGL((k_spmv<16, T, EP>), grid, dim3(block), 0, s, n, rp, ci, v, x, ep); break;
@@ -96,26 +147,26 @@ int lanes_for(int64_t nnz, int64_t n) {
 
 template <typename T> static int spmv_t(hipStream_t s, const CsrView<T> &A, const T *x, T *y) {
   EpArgs<T> ep{}; ep.y = y;
-  return launch_spmv<T, EP_Y>(s, A.lpr, A.nr, A.rp, A.ci, A.v, x, ep);
+  return launch_spmv<T, EP_Y>(s, A, x, ep);
 }
 template <typename T> static int resid_t(hipStream_t s, const CsrView<T> &A, const T *b, const T *x, T *r) {
   EpArgs<T> ep{}; ep.y = r; ep.b = b;
-  return launch_spmv<T, EP_RES>(s, A.lpr, A.nr, A.rp, A.ci, A.v, x, ep);
+  return launch_spmv<T, EP_RES>(s, A, x, ep);
 }
 template <typename T> static int cheb_t(hipStream_t s, const CsrView<T> &A, T *r, T *x, const T *dold, T *dnew,
                                         const T *dinv, T c1, T c2) {
   EpArgs<T> ep{}; ep.y = r; ep.x = x; ep.dold = dold; ep.dnew = dnew; ep.dinv = dinv; ep.c1 = c1; ep.c2 = c2;
-  return launch_spmv<T, EP_CHEB>(s, A.lpr, A.nr, A.rp, A.ci, A.v, dold, ep);
+  return launch_spmv<T, EP_CHEB>(s, A, dold, ep);
 }
 template <typename T> static int addmul_t(hipStream_t s, const CsrView<T> &P, const T *xc, T *x, T alpha,
                                           const double *cond) {
   EpArgs<T> ep{}; ep.y = x; ep.c1 = alpha; ep.cond = cond;
-  return launch_spmv<T, EP_ADD>(s, P.lpr, P.nr, P.rp, P.ci, P.v, xc, ep);
+  return launch_spmv<T, EP_ADD>(s, P, xc, ep);
 }
 template <typename T> static int jacobi_t(hipStream_t s, const CsrView<T> &A, const T *b, const T *x, T *xn,
                                           const T *dinv, T omega) {
   EpArgs<T> ep{}; ep.b = b; ep.dnew = xn; ep.dinv = dinv; ep.c1 = omega;
-  return launch_spmv<T, EP_JAC>(s, A.lpr, A.nr, A.rp, A.ci, A.v, x, ep);
+  return launch_spmv<T, EP_JAC>(s, A, x, ep);
 }
 
 int spmv(hipStream_t s, const CsrView<double> &A, const double *x, double *y) { return spmv_t(s, A, x, y); }
@@ -313,6 +364,21 @@ int elpos(hipStream_t s, int nel, int nloc, const int *eldof, const int64_t *rp,
   int64_t total = (int64_t)nel * nloc * nloc;
   int64_t b = (total + 255) / 256; if (b > 65536) b = 65536;
   hipLaunchKernelGGL(k_elpos, dim3((unsigned)b), dim3(256), 0, s, nel, nloc, eldof, rp, ci, pos);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+__global__ void k_csr2sell(int64_t nslot, const int *__restrict__ map, const double *__restrict__ cv,
+                           double *__restrict__ sv, float *__restrict__ svf) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nslot; i += (int64_t)gridDim.x * blockDim.x) {
+    int q = map[i];
+    double v = q >= 0 ? cv[q] : 0.0;
+    sv[i] = v;
+    svf[i] = (float)v;
+  }
+}
+int csr2sell(hipStream_t s, int64_t nslot, const int *map, const double *csr_v, double *sv, float *svf) {
+  if (nslot <= 0) return 0;
+  hipLaunchKernelGGL(k_csr2sell, vgrid(nslot), dim3(256), 0, s, nslot, map, csr_v, sv, svf);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -24,8 +24,8 @@
   } while (0)
 
 int fm_vanka_setup(fm_system *s, int L, std::vector<struct LOps> *ops);
-int fm_vanka_sweep64(fm_system *s, int L, const struct LOps &o, double *x, const double *b, int its, double omega);
-int fm_vanka_sweep32(fm_system *s, int L, const struct LOps &o, float *x, const float *b, int its, double omega);
+int fm_vanka_sweep64(fm_system *s, int L, const struct LOps &o, double *x, const double *b, int its, double omega, int colored);
+int fm_vanka_sweep32(fm_system *s, int L, const struct LOps &o, float *x, const float *b, int its, double omega, int colored);
 
 // Per-level operator set used by the V-cycle (forward or transposed).
 struct LOps {
@@ -40,13 +40,7 @@ struct LOps {
   double *vinv = nullptr;  // Vanka local inverses
 };
 
-template <typename T> static CsrView<T> cview(const DevCSR &c);
-template <> CsrView<double> cview<double>(const DevCSR &c) {
-  CsrView<double> v; v.nr = c.nr; v.rp = c.rp; v.ci = c.ci; v.v = c.v; v.lpr = fmk::lanes_for(c.nnz, c.nr); return v;
-}
-template <> CsrView<float> cview<float>(const DevCSR &c) {
-  CsrView<float> v; v.nr = c.nr; v.rp = c.rp; v.ci = c.ci; v.v = c.vf; v.lpr = fmk::lanes_for(c.nnz, c.nr); return v;
-}
+template <typename T> static inline CsrView<T> cview(const DevCSR &c) { return fm_view<T>(c); }
 
 // typed access to a level's work vectors
 template <typename T> struct Vec;
@@ -206,8 +200,8 @@ static int smooth(fm_system *s, const fm_mg_opts *opt, int L, const LOps &o, int
     return 0;
   }
   if (opt->smoother == FM_SMOOTH_VANKA) {
-    if (sizeof(T) == 8) return fm_vanka_sweep64(s, L, o, (double *)x, (const double *)b, its, opt->omega);
-    return fm_vanka_sweep32(s, L, o, (float *)x, (const float *)b, its, opt->omega);
+    if (sizeof(T) == 8) return fm_vanka_sweep64(s, L, o, (double *)x, (const double *)b, its, opt->omega, opt->vanka_colored);
+    return fm_vanka_sweep32(s, L, o, (float *)x, (const float *)b, its, opt->omega, opt->vanka_colored);
   }
   return FM_EINVAL;
 }
@@ -239,7 +233,8 @@ static int vcycle(fm_system *s, const fm_mg_opts *opt, std::vector<LOps> &ops, i
   }
   FM_CHECK(fmk::spmv(st, cview<T>(*oc.R), r, Vec<T>::b(lc)));
   FM_CHECK(fmk::fill<T>(st, oc.n, Vec<T>::x(lc), (T)0));
-  FM_CHECK(vcycle<T>(s, opt, ops, L - 1));
+  const int gamma = (L - 1 > 0 && opt->gamma > 1) ? opt->gamma : 1;
+  for (int g = 0; g < gamma; g++) FM_CHECK(vcycle<T>(s, opt, ops, L - 1));
   double alpha = 1.0;
   if (opt->correction == 0) {
     // coarse_rest: relative residual of the coarse level after its cycle
@@ -421,14 +416,15 @@ static int solve_fgmres(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &op
         // z_j = M v_j
         if (o->precision == FM_PREC_FP64_FGMRES) {
           // use level-top b/x of the FP64 cycle: save b in t
-          if (hipMemcpyAsync(l->t, l->b, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
+          // the FP64 cycle works on the level's own b / x: park them in sb / sx
+          if (hipMemcpyAsync(l->sb, l->b, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
           if (hipMemcpyAsync(l->b, vj, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
-          if (hipMemcpyAsync(l->d2, l->x, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
+          if (hipMemcpyAsync(l->sx, l->x, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
           if ((rc = fmk::fill<double>(st, n, l->x, 0.0)) != 0) goto done;
           if ((rc = vcycle<double>(s, o, ops, top)) != 0) goto done;
           if (hipMemcpyAsync(zj, l->x, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
-          if (hipMemcpyAsync(l->x, l->d2, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
-          if (hipMemcpyAsync(l->b, l->t, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
+          if (hipMemcpyAsync(l->x, l->sx, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
+          if (hipMemcpyAsync(l->b, l->sb, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
         } else {
           if ((rc = fmk::to_f32(st, n, vj, l->bf, nullptr)) != 0) goto done;
           if ((rc = fmk::fill<float>(st, n, l->xf, 0.f)) != 0) goto done;
@@ -506,19 +502,23 @@ extern "C" int fm_mg_solve(fm_system *s, const fm_mg_opts *o, fm_mg_report *rep)
   return rep->converged ? FM_OK : FM_ENOCONV;
 }
 
-// ---- Vanka (element patches, additive or coloured), FP64 local inverses ---
+// ---- Vanka (vanka.hip) -----------------------------------------------------
+int fm_vanka_invert(fm_system *s, int L);
+int fm_vanka_apply64(fm_system *s, int L, const CsrView<double> &A, const double *vinv, double *x, const double *b,
+                     double *r, int its, double omega, int colored);
+int fm_vanka_apply32(fm_system *s, int L, const CsrView<float> &A, const double *vinv, float *x, const float *b,
+                     float *r, int its, double omega, int colored);
 int fm_vanka_setup(fm_system *s, int L, std::vector<LOps> *ops) {
-  (void)s; (void)L; (void)ops;
-  fm_set_error("Vanka smoother not available in this build");
-  return FM_EINVAL;
+  int rc = fm_vanka_invert(s, L);
+  if (rc) return rc;
+  (*ops)[L].vinv = s->lev[L].vinv;
+  return 0;
 }
-int fm_vanka_sweep64(fm_system *s, int L, const LOps &o, double *x, const double *b, int its, double omega) {
-  (void)s; (void)L; (void)o; (void)x; (void)b; (void)its; (void)omega;
-  return FM_EINVAL;
+int fm_vanka_sweep64(fm_system *s, int L, const LOps &o, double *x, const double *b, int its, double omega, int colored) {
+  return fm_vanka_apply64(s, L, cview<double>(*o.A), o.vinv, x, b, o.v->r, its, omega, colored);
 }
-int fm_vanka_sweep32(fm_system *s, int L, const LOps &o, float *x, const float *b, int its, double omega) {
-  (void)s; (void)L; (void)o; (void)x; (void)b; (void)its; (void)omega;
-  return FM_EINVAL;
+int fm_vanka_sweep32(fm_system *s, int L, const LOps &o, float *x, const float *b, int its, double omega, int colored) {
+  return fm_vanka_apply32(s, L, cview<float>(*o.A), o.vinv, x, b, o.v->rf, its, omega, colored);
 }
 
 extern "C" int fm_adjoint_solve(fm_system *s, const fm_mg_opts *o, const double *g, double *lam, fm_mg_report *rep) {

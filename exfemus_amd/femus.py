@@ -11,6 +11,7 @@ fails loudly when the library is missing.
 """
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -61,7 +62,7 @@ class MGOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("smoother", C.c_int), ("nu_pre", C.c_int), ("nu_post", C.c_int),
                 ("correction", C.c_int), ("max_iter", C.c_int), ("fgmres_restart", C.c_int),
                 ("tol", C.c_double), ("cheb_lo", C.c_double), ("cheb_hi", C.c_double),
-                ("omega", C.c_double), ("vanka_colored", C.c_int)]
+                ("omega", C.c_double), ("vanka_colored", C.c_int), ("gamma", C.c_int)]
 
 
 class MGReport(C.Structure):
@@ -130,18 +131,21 @@ def _ck(rc, ok=(0,)):
 
 
 class Context:
-    """One HIP device + one stream (fm_ctx)."""
+    """One HIP device + one stream (fm_ctx).  Systems created on it are closed first when it closes."""
 
     def __init__(self, device=0):
         self.h = C.c_void_p()
+        self._systems = weakref.WeakSet()
         _ck(lib().fm_ctx_create(device, C.byref(self.h)))
 
     def sync(self):
         _ck(lib().fm_sync(self.h))
 
     def close(self):
+        for s in list(self._systems):
+            s.close()
         if self.h:
-            lib().fm_ctx_destroy(self.h)
+            _ck(lib().fm_ctx_destroy(self.h))
             self.h = C.c_void_p()
 
     def __del__(self):
@@ -197,15 +201,19 @@ class MGSolDA:
         self.ctx, self.mesh, self.nq2, self.nq1 = ctx, mesh, nq2, nq1
         self.h = C.c_void_p()
         _ck(lib().fm_system_create(ctx.h, mesh.h, nq2, nq1, C.byref(self.h)))
+        ctx._systems.add(self)
         self._NoLevels = mesh.nlev
         self._Dim = [int(lib().fm_system_ndof(self.h, L)) for L in range(mesh.nlev)]
         self._cb = None
 
+    def close(self):
+        if self.h:
+            lib().fm_system_destroy(self.h)
+            self.h = C.c_void_p()
+
     def __del__(self):
         try:
-            if self.h:
-                lib().fm_system_destroy(self.h)
-                self.h = C.c_void_p()
+            self.close()
         except Exception:
             pass
 
@@ -307,9 +315,12 @@ class MGSolDA:
     # -- MGSolve (src/MGSolverBase.C:163-208) --------------------------------------
     @staticmethod
     def opts(precision=FM_PREC_FP64, smoother=FM_SMOOTH_CHEBYSHEV, nu_pre=2, nu_post=2, correction=1,
-             max_iter=100, fgmres_restart=30, tol=1e-8, cheb_lo=0.2, cheb_hi=1.05, omega=0.6, vanka_colored=0):
+             max_iter=100, fgmres_restart=30, tol=1e-8, cheb_lo=0.2, cheb_hi=1.05, omega=None, vanka_colored=0,
+             gamma=1):
+        if omega is None:
+            omega = 0.5 if smoother == FM_SMOOTH_VANKA else 0.6
         return MGOpts(precision, smoother, nu_pre, nu_post, correction, max_iter, fgmres_restart, tol,
-                      cheb_lo, cheb_hi, omega, vanka_colored)
+                      cheb_lo, cheb_hi, omega, vanka_colored, gamma)
 
     def MGSetup(self, opts):
         _ck(lib().fm_mg_setup(self.h, C.byref(opts)))

@@ -145,6 +145,7 @@ typedef struct {
   double cheb_lo, cheb_hi;   /* Chebyshev interval, fractions of lambda_max  */
   double omega;       /* Jacobi / Vanka damping                              */
   int vanka_colored;  /* 1 = multiplicative over 2^dim element colours       */
+  int gamma;          /* coarse-grid visits per level: 1 = V-cycle, 2 = W (MGStep Gamma) */
 } fm_mg_opts;
 
 typedef struct {

@@ -1,0 +1,359 @@
+"""CPU tests pinning the oracle (FP64 restatement of the reference path).
+
+Each oracle function is checked against something that does not restate its
+own formula: the reference's own numeric tables (embedding matrices parsed
+from src/MGGeomEl.C into tests/golden), LASPACK compiled from the reference
+sources (golden vectors in tests/golden, live when oracle/_ref is built),
+closed forms (1D Q2 stiffness/mass, Gauss-Legendre exactness, Lagrange
+interpolation of quadratics), brute force (element-sharing sparsity) and
+invariants (nested node prefixes, R = P^T, symmetry of the Stokes part).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from exfemus_amd import configs
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# ----------------------------------------------------------------------------- FE tables
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_shapes_partition_of_unity_and_weights(oracle, dim):
+    phi, dphi, w = oracle.fe_tables(2, dim)
+    np.testing.assert_allclose(phi.sum(0), 1.0, atol=1e-14)
+    np.testing.assert_allclose(dphi.sum(1), 0.0, atol=1e-13)
+    assert abs(w.sum() - 2.0 ** dim) < 1e-14
+    phi1, _, _ = oracle.fe_tables(1, dim)
+    np.testing.assert_allclose(phi1.sum(0), 1.0, atol=1e-14)
+    # reference-held constants (src/MGFE.C:19, 40): a = -sqrt(3/5), w = 5/9, 8/9
+    _, _, w1 = oracle.fe_tables(2, 1)
+    np.testing.assert_allclose(w1, [5 / 9, 8 / 9, 5 / 9], rtol=0, atol=1e-16)
+
+
+def test_gauss_rule_exact_for_degree5(oracle):
+    # 3-point Gauss-Legendre integrates x^4 exactly: int_{-1}^{1} x^4 = 2/5; the table's phi for the
+    # +1 node is x(x+1)/2 -> integral 2/3.
+    phi, _, w = oracle.fe_tables(2, 1)
+    assert abs((phi[1] * w).sum() - 1.0 / 3.0) < 1e-15      # int phi_(+1) = 1/3
+    assert abs((phi[2] * w).sum() - 4.0 / 3.0) < 1e-15      # int (1 - x^2) = 4/3
+
+
+@pytest.mark.parametrize("kind,dim,key", [(2, 2, "q2_2d"), (1, 2, "q1_2d"), (2, 3, "q2_3d"), (1, 3, "q1_3d")])
+def test_embedding_matches_reference_tables(oracle, kind, dim, key):
+    ref = np.load(os.path.join(GOLD, f"embedding_{key}.npy"))
+    E = oracle.embedding(kind, dim)
+    # the reference stores float: all entries are dyadic, hence exact
+    np.testing.assert_array_equal(E.astype(np.float32).astype(np.float64), ref)
+    np.testing.assert_array_equal(E, ref)
+
+
+# ----------------------------------------------------------------------------- mesh / DoF / BC
+@pytest.mark.parametrize("dim,nx,nlev", [(2, [2, 3], 3), (3, [2, 1, 2], 3)])
+def test_mesh_counts_and_nesting(oracle, dim, nx, nlev):
+    m = oracle.Mesh(dim, nx, [0] * dim, [1] * dim, nlev)
+    for L in range(nlev):
+        ne, nq2, nq1 = m.level(L)
+        f = 2 ** L
+        assert ne == np.prod([n * f for n in nx])
+        assert nq2 == np.prod([2 * n * f + 1 for n in nx])
+        assert nq1 == np.prod([n * f + 1 for n in nx])
+        if L > 0:
+            assert nq1 == m.level(L - 1)[1]      # Q1 nodes of L = Q2 nodes of L-1
+        conn, neigh, par = m.conn(L)
+        assert conn.max() < nq2                  # level nodes are a prefix of the top numbering
+        # vertices of level-L elements are exactly the Q1 nodes of level L
+        assert conn[:, :m.n1].max() < nq1
+        assert len(np.unique(conn[:, :m.n1])) == nq1
+        # neighbour relation is symmetric
+        for e in range(ne):
+            for s in range(m.nsides):
+                nb = neigh[e, s]
+                if nb >= 0:
+                    assert e in neigh[nb]
+    lid, lev, levp = m.node_keys()
+    key = levp + (nlev + 1) * lev
+    assert np.all(np.diff(key) >= 0)            # gencase sort key is non-decreasing
+    same = np.diff(key) == 0
+    assert np.all(np.diff(lid)[same] > 0)       # ties ordered by the libMesh-style id
+
+
+def test_node_coordinates_match_element_geometry(oracle):
+    m = oracle.Mesh(3, [2, 2, 2], [0, 0, 0], [1, 2, 3], 2)
+    xyz = m.coords()
+    conn, _, _ = m.conn(1)
+    h = np.array([1 / 4, 2 / 4, 3 / 4])
+    for e in range(conn.shape[0]):
+        c = xyz[:, conn[e]]
+        lo = c[:, 0]
+        # hex27 local node k sits at lo + (ref+1)/2 * h
+        ref = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1], [-1, -1, 1], [1, -1, 1], [1, 1, 1],
+                        [-1, 1, 1]]).T
+        np.testing.assert_allclose(c[:, :8], lo[:, None] + (ref + 1) / 2 * h[:, None], atol=1e-14)
+
+
+def test_init_dof_and_genbc(oracle):
+    m = oracle.Mesh(2, [2, 2], [0, 0], [1, 1], 2)
+    d, n = oracle.init_dof(m, 1, 2, 1)
+    nq2, nq1 = m.level(1)[1], m.level(1)[2]
+    assert n == 2 * nq2 + nq1
+    N = m.N
+    np.testing.assert_array_equal(d[:nq2], np.arange(nq2))
+    np.testing.assert_array_equal(d[N:N + nq2], nq2 + np.arange(nq2))
+    np.testing.assert_array_equal(d[2 * N:2 * N + nq1], 2 * nq2 + np.arange(nq1))
+    assert np.all(d[2 * N + nq1:3 * N] == -1)
+    b0, b1 = oracle.genbc(m, 2, 1, d, lambda xp, nn, ff: configs.cavity_bc(xp, nn, ff, 2))
+    xyz = m.coords()
+    bnd = (xyz[0] < 1e-12) | (xyz[0] > 1 - 1e-12) | (xyz[1] < 1e-12) | (xyz[1] > 1 - 1e-12)
+    for k in range(N):
+        for iv in range(2):
+            assert b0[d[k + iv * N]] == (0 if bnd[k] else 1)
+    lid = (xyz[1] > 1 - 1e-12) & (xyz[0] > 1e-12) & (xyz[0] < 1 - 1e-12)
+    assert np.all(b1[d[np.where(lid)[0]]] == 4)
+    # pressure rows are never Dirichlet in the cavity
+    assert np.all(b0[2 * nq2:] == 1)
+
+
+# ----------------------------------------------------------------------------- gencase operators
+def test_sparsity_blocks_bruteforce(oracle):
+    m = oracle.Mesh(2, [2, 2], [0, 0], [1, 1], 2)
+    conn, _, _ = m.conn(1)
+    nq2, nq1 = m.level(1)[1], m.level(1)[2]
+    for rk, ck in ((2, 2), (2, 1), (1, 2), (1, 1)):
+        ln, pos = oracle.mat_block(m, 1, rk, ck)
+        nr = m.n2 if rk == 2 else m.n1
+        nc = m.n2 if ck == 2 else m.n1
+        rows = nq2 if rk == 2 else nq1
+        # brute force first-occurrence lists
+        lists = [[] for _ in range(rows)]
+        for e in range(conn.shape[0]):
+            for kk in range(nr):
+                for kj in range(nc):
+                    c = conn[e, kj]
+                    if c not in lists[conn[e, kk]]:
+                        lists[conn[e, kk]].append(c)
+        flat = [c for l in lists for c in l]
+        np.testing.assert_array_equal(pos, flat)
+        np.testing.assert_array_equal(np.diff(ln), [len(l) for l in lists])
+
+
+def test_system_pattern_counts(oracle):
+    # C1 (2D cavity 32^2, 3 levels) top rows 9,539 and nnz 377,289
+    m = oracle.Mesh(2, [8, 8], [0, 0], [1, 1], 3)
+    d, n = oracle.init_dof(m, 2, 2, 1)
+    rp, ci = oracle.system_pattern(m, 2, 2, 1, d)
+    assert n == 9539 and len(ci) == 377289
+    # 3D Q2 scalar: (8n+1)^3 non-zeros
+    m3 = oracle.Mesh(3, [2, 2, 2], [0, 0, 0], [1, 1, 1], 2)
+    d3, n3 = oracle.init_dof(m3, 1, 1, 0)
+    rp3, ci3 = oracle.system_pattern(m3, 1, 1, 0, d3)
+    assert len(ci3) == (8 * 4 + 1) ** 3
+
+
+@pytest.mark.parametrize("dim,nx", [(2, [2, 3]), (3, [2, 1, 2])])
+def test_prolongation_interpolates_quadratics(oracle, dim, nx):
+    m = oracle.Mesh(dim, nx, [0] * dim, [1] * dim, 3)
+    xyz = m.coords()
+    rng = np.random.default_rng(0)
+    cq = rng.standard_normal((dim, dim))
+    cl = rng.standard_normal(dim)
+
+    def quad(x):  # a full quadratic, reproduced exactly by Q2 interpolation
+        return 0.3 + cl @ x + np.einsum("in,ij,jn->n", x, cq, x)
+
+    def lin(x):  # multilinear, reproduced exactly by Q1 interpolation
+        return 1.0 + cl @ x + (x[0] * x[1] if dim >= 2 else 0)
+
+    for L in (1, 2):
+        for kind, f in ((2, quad), (1, lin)):
+            ln, pos, val = oracle.prol_block(m, L, kind)
+            nr = m.level(L)[1] if kind == 2 else m.level(L)[2]
+            ncol = m.level(L - 1)[1] if kind == 2 else m.level(L - 1)[2]
+            P = sp.csr_matrix((val, pos, ln), shape=(nr, ncol))
+            np.testing.assert_allclose(P @ f(xyz[:, :ncol]), f(xyz[:, :nr]), atol=1e-12)
+
+
+def test_restriction_is_prolongation_transpose(oracle):
+    m = oracle.Mesh(2, [2, 2], [0, 0], [1, 1], 3)
+    for L in (0, 1):
+        for kind in (2, 1):
+            rl, rpos, rv = oracle.rest_block(m, L, kind)
+            pl, ppos, pv = oracle.prol_block(m, L + 1, kind)
+            nc = m.level(L)[1] if kind == 2 else m.level(L)[2]
+            nf = m.level(L + 1)[1] if kind == 2 else m.level(L + 1)[2]
+            R = sp.csr_matrix((rv, rpos, rl), shape=(nc, nf)).toarray()
+            P = sp.csr_matrix((pv, ppos, pl), shape=(nf, nc)).toarray()
+            np.testing.assert_array_equal(R, P.T)
+
+
+def test_restriction_dirichlet_mask(oracle):
+    m = oracle.Mesh(2, [2, 2], [0, 0], [1, 1], 2)
+    dofs = [oracle.init_dof(m, L, 1, 0)[0] for L in range(2)]
+    b0, _ = oracle.genbc(m, 1, 0, dofs[1], configs.heat_dirichlet_all)
+    rp, ci, v = oracle.system_rest(m, 0, 1, 0, dofs[0], dofs[1], dofs[1], b0)
+    xyz = m.coords()
+    nc = m.level(0)[1]
+    for k in range(nc):
+        row = dofs[0][k]
+        on_bnd = min(xyz[0, k], xyz[1, k], 1 - xyz[0, k], 1 - xyz[1, k]) < 1e-12
+        if on_bnd:
+            assert np.all(v[rp[row]:rp[row + 1]] == 0.0)
+        else:
+            assert np.any(v[rp[row]:rp[row + 1]] != 0.0)
+
+
+# ----------------------------------------------------------------------------- element matrices
+def _1d():
+    S = np.array([[7, -8, 1], [-8, 16, -8], [1, -8, 7]]) / 3.0      # h = 1 stiffness, nodes (-,0,+) -> scaled
+    M = np.array([[4, 2, -1], [2, 16, 2], [-1, 2, 4]]) / 30.0
+    return S, M
+
+
+def _perm_1d_to_hex(dim):
+    # local node order of Quad9/Hex27 in terms of (i,j,k) with 0 = -1, 1 = 0, 2 = +1
+    from exfemus_amd.configs import TOL  # noqa: F401 (import keeps configs loaded)
+    q9 = [(-1, -1), (1, -1), (1, 1), (-1, 1), (0, -1), (1, 0), (0, 1), (-1, 0), (0, 0)]
+    h27 = [(-1, -1, -1), (1, -1, -1), (1, 1, -1), (-1, 1, -1), (-1, -1, 1), (1, -1, 1), (1, 1, 1), (-1, 1, 1),
+           (0, -1, -1), (1, 0, -1), (0, 1, -1), (-1, 0, -1), (-1, -1, 0), (1, -1, 0), (1, 1, 0), (-1, 1, 0),
+           (0, -1, 1), (1, 0, 1), (0, 1, 1), (-1, 0, 1), (0, 0, -1), (0, -1, 0), (1, 0, 0), (0, 1, 0), (-1, 0, 0),
+           (0, 0, 1), (0, 0, 0)]
+    return [tuple(c + 1 for c in t) for t in (q9 if dim == 2 else h27)]
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_heat_element_closed_form(oracle, dim):
+    """Interior element, k = 1, no advection: K = sum_d kron(S_d, M_others) + M/dt (1D Q2 closed forms)."""
+    nx = [4] * dim
+    m = oracle.Mesh(dim, nx, [0] * dim, [1] * dim, 1)
+    d, n = oracle.init_dof(m, 0, 1, 0)
+    b0 = np.ones(n, np.int32); b1 = np.ones(n, np.int32)
+    dt = 0.37
+    p = oracle.HeatParams(1.0, 1.0, dt, 1.0, 0.0, 1.0, (0.0, 0.0, 0.0))
+    KeM = np.zeros(27 * 27); FeM = np.zeros(27); dofs = np.zeros(27, np.int32)
+    conn, neigh, _ = m.conn(0)
+    e = [i for i in range(conn.shape[0]) if np.all(neigh[i] >= 0)][0]
+    oracle.lib().om_heat_element(m.h, 0, e, 0, C.byref(p), d, b0, b1, np.zeros(n), 1.0, None, KeM, FeM, dofs, None)
+    n2 = 3 ** dim
+    K = KeM[:n2 * n2].reshape(n2, n2)
+    h = 0.25
+    S, M = _1d()
+    S1, M1 = S / h, M * h
+    idx = _perm_1d_to_hex(dim)
+    Kref = np.zeros((n2, n2))
+    for a in range(n2):
+        for b in range(n2):
+            ia, ib = idx[a], idx[b]
+            mass = np.prod([M1[ia[t], ib[t]] for t in range(dim)])
+            lap = 0.0
+            for dd in range(dim):
+                lap += np.prod([(S1 if t == dd else M1)[ia[t], ib[t]] for t in range(dim)])
+            Kref[a, b] = lap + mass / dt
+    np.testing.assert_allclose(K, Kref, rtol=1e-12, atol=1e-14)
+
+
+def test_ns_element_stokes_structure(oracle):
+    """Zero advection field: the velocity block is symmetric and B = -(B^T)^T (MGSolverNS.C:438-489)."""
+    m = oracle.Mesh(2, [4, 4], [0, 0], [1, 1], 1)
+    d, n = oracle.init_dof(m, 0, 2, 1)
+    b0 = np.ones(n, np.int32); b1 = np.ones(n, np.int32)
+    p = oracle.NSParams(0.01, 0.1, 1.0, 0.1, 0.0, (0, 0, 0), (0, 0, 0))
+    nl = 2 * 9 + 4
+    KeM = np.zeros(89 * 89); FeM = np.zeros(89); dofs = np.zeros(89, np.int32)
+    conn, neigh, _ = m.conn(0)
+    e = [i for i in range(conn.shape[0]) if np.all(neigh[i] >= 0)][0]
+    oracle.lib().om_ns_element(m.h, 0, e, 0, C.byref(p), d, b0, b1, np.zeros(n), 0.01, KeM, FeM, dofs, None)
+    K = KeM[:nl * nl].reshape(nl, nl)
+    V = K[:18, :18]
+    np.testing.assert_allclose(V, V.T, rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(K[18:, :18], -K[:18, 18:].T, rtol=1e-12, atol=1e-15)
+    assert np.all(K[18:, 18:] == 0.0)
+    # discrete divergence of a constant velocity vanishes
+    u = np.concatenate([np.ones(9), 2 * np.ones(9)])
+    np.testing.assert_allclose(K[18:, :18] @ u, 0.0, atol=1e-14)
+
+
+def test_heat_assembly_rows_and_dirichlet(oracle):
+    m = oracle.Mesh(2, [2, 2], [0, 0], [1, 1], 2)
+    d, n = oracle.init_dof(m, 1, 1, 0)
+    b0, b1 = oracle.genbc(m, 1, 0, d, configs.heat_dirichlet_all)
+    rp, ci = oracle.system_pattern(m, 1, 1, 0, d)
+    p = oracle.HeatParams(1.0, 1.0, 0.1, 0.0, 1.0, 1.0, (0.0, 0.0, 0.0))
+    v, rhs = oracle.heat_genmatrhs(m, 1, 1, p, d, d, b0, b1, np.zeros(n), rp, ci)
+    A = sp.csr_matrix((v, ci, rp), shape=(n, n))
+    for i in range(n):
+        row = A.getrow(i)
+        if b0[i] == 0:      # Dirichlet rows hold only the penalty diagonal
+            assert row.nnz == 1 or np.count_nonzero(row.data) == 1
+            assert rhs[i] == 0.0
+        else:               # steady Laplacian rows sum to zero (constants in the kernel)
+            assert abs(row.sum()) < 1e-12
+    # solution is symmetric under x <-> y for the symmetric square
+    x = spla.spsolve(A.tocsc(), rhs)
+    xyz = m.coords()
+    for k in range(m.N):
+        j = np.where((np.abs(xyz[0] - xyz[1, k]) < 1e-12) & (np.abs(xyz[1] - xyz[0, k]) < 1e-12))[0][0]
+        assert abs(x[d[k]] - x[d[j]]) < 1e-12
+
+
+# ----------------------------------------------------------------------------- LASPACK golden vectors
+def _golden(name):
+    return np.load(os.path.join(GOLD, f"laspack_{name}.npz"))
+
+
+def test_spmv_matches_laspack(oracle):
+    g = _golden("heat2d")
+    y = oracle.spmv(g["rp"], g["ci"], g["v"], g["x"])
+    np.testing.assert_allclose(y, g["spmv"], rtol=1e-14, atol=1e-14 * np.abs(g["spmv"]).max())
+
+
+def test_gmres_ilu_matches_laspack(oracle):
+    g = _golden("heat2d")
+    for its, key in ((8, "gmres8"), (45, "gmres45")):
+        x, it, acc = oracle.gmres_ilu(g["rp"], g["ci"], g["v"], g["b"], np.zeros(len(g["b"])), its)
+        assert it == int(g[key + "_its"])
+        np.testing.assert_allclose(x, g[key], rtol=1e-9, atol=1e-12 * np.abs(g[key]).max())
+        assert abs(acc - float(g[key + "_acc"])) <= 1e-6 * float(g[key + "_acc"]) + 1e-14
+
+
+def test_ilu_matches_live_laspack(oracle):
+    R = oracle.ref_lib()
+    if R is None:
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    g = _golden("heat2d")
+    n = len(g["b"])
+    yi = np.zeros(n)
+    R.lr_ilu_apply(n, g["rp"], g["ci"], g["v"], g["x"], yi)
+    np.testing.assert_array_equal(yi, g["ilu"])
+
+
+# ----------------------------------------------------------------------------- multigrid
+def test_oracle_mg_reference_path_converges(oracle):
+    """MGSolve with GMRES(30)+ILU(0) smoothing (the reference CPU path) reaches the direct solution."""
+    nlev = 3
+    m = oracle.Mesh(2, [2, 2], [0, 0], [1, 1], nlev)
+    top = nlev - 1
+    dofs = [oracle.init_dof(m, L, 1, 0)[0] for L in range(nlev)]
+    b0, b1 = oracle.genbc(m, 1, 0, dofs[top], configs.heat_dirichlet_all)
+    p = oracle.HeatParams(1.0, 1.0, 0.1, 0.0, 1.0, 1.0, (0.0, 0.0, 0.0))
+    H = oracle.Hier(nlev)
+    for L in range(nlev):
+        rp, ci = oracle.system_pattern(m, L, 1, 0, dofs[L])
+        v, rhs = oracle.heat_genmatrhs(m, L, 1, p, dofs[L], dofs[top], b0, b1, np.zeros(len(b0)), rp, ci)
+        H.set_A(L, rp, ci, v)
+        if L == top:
+            A, b = sp.csr_matrix((v, ci, rp)), rhs
+        if L < top:
+            r = oracle.system_rest(m, L, 1, 0, dofs[L], dofs[L + 1], dofs[top], b0)
+            H.set_R(L, *r, m.level(L + 1)[1])
+        if L > 0:
+            q = oracle.system_prol(m, L, 1, 0, dofs[L], dofs[L - 1])
+            H.set_P(L, *q, m.level(L - 1)[1])
+    x, hist = H.solve(b, smoother=0, coarse=0, correction=0, eps=1e-10, ref_exit=0, max_cycles=40)
+    xd = spla.spsolve(A.tocsc(), b)
+    assert hist[-1] <= 1e-10
+    np.testing.assert_allclose(x, xd, rtol=1e-8, atol=1e-12)
+    assert len(hist) < 15
