@@ -102,8 +102,9 @@ def main():
     ap.add_argument("--nx0", type=int, default=4)
     ap.add_argument("--nlev", type=int, default=5)
     ap.add_argument("--tol", type=float, default=1e-8)
-    ap.add_argument("--outer", default="ir", choices=["ir", "fgmres"])
-    ap.add_argument("--nu", type=int, default=2)
+    ap.add_argument("--outer", default="fgmres", choices=["ir", "fgmres"])
+    ap.add_argument("--nu", type=int, default=3)
+    ap.add_argument("--cheb-lo", type=float, default=0.1)
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -122,8 +123,10 @@ def main():
     log(f"setup {time.perf_counter() - t0:.1f}s  N_top={N}  assembly(all levels)={t_asm * 1e3:.1f} ms")
     prec_mixed = F.FM_PREC_FP32_IR if args.outer == "ir" else F.FM_PREC_FP32_FGMRES
     prec_fp64 = F.FM_PREC_FP64 if args.outer == "ir" else F.FM_PREC_FP64_FGMRES
-    om = s.opts(precision=prec_mixed, nu_pre=args.nu, nu_post=args.nu, tol=args.tol, max_iter=100)
-    o64 = s.opts(precision=prec_fp64, nu_pre=args.nu, nu_post=args.nu, tol=args.tol, max_iter=100)
+    om = s.opts(precision=prec_mixed, nu_pre=args.nu, nu_post=args.nu, tol=args.tol, max_iter=100,
+                cheb_lo=args.cheb_lo)
+    o64 = s.opts(precision=prec_fp64, nu_pre=args.nu, nu_post=args.nu, tol=args.tol, max_iter=100,
+                 cheb_lo=args.cheb_lo)
     s.MGSetup(o64)
     zero = np.zeros(N)
 
@@ -197,15 +200,16 @@ def main():
                  "bytes_alg_fp64": b64, "bytes_alg_fp32": b32},
         "roofline": {"bound": "hbm", "achieved": round(gbs64, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs64 / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_spmv<32,double,EP_Y> on the 64^3 top matrix"},
+                     "kernel": "k_sell<double,EP_Y> (SELL-64) on the 64^3 top matrix"},
     }
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline()
         except Exception as e:  # the oracle may be unbuilt on a bare box
             out["cpu_baseline"] = {"error": str(e)}
-    print(json.dumps(out))
+    s.close()
     ctx.close()
+    print(json.dumps(out))
 
 
 if __name__ == "__main__":

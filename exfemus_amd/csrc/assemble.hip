@@ -94,7 +94,7 @@ __global__ __launch_bounds__(256) void k_heat_vol(int nel, const int *__restrict
                                                   const double *__restrict__ xyz, int Ntop, const int *__restrict__ dof_top,
                                                   const int *__restrict__ bc0, const double *__restrict__ x_old,
                                                   const double *__restrict__ coef, const double *__restrict__ vel,
-                                                  const FeTab *__restrict__ tab, HeatArgs a, int mode,
+                                                  const FeTab *__restrict__ tab, HeatArgs a, int mode, int trans,
                                                   double *__restrict__ val, double *__restrict__ rhs) {
   constexpr int N2 = DIM == 2 ? 9 : 27, NG = N2;
   __shared__ double s_phi[N2 * NG];
@@ -174,7 +174,7 @@ __global__ __launch_bounds__(256) void k_heat_vol(int nel, const int *__restrict
         }
         K += s_JxW[qp] * (tcdt * phii * phij + a.rhocp * Adv * phii + Lap);
       }
-      const int p = elpos[pbase + t];
+      const int p = elpos[pbase + (trans ? j * N2 + i : t)];
       if (p >= 0) val[p] += K;
     }
     if (mode == 1 && tid < N2) {
@@ -198,7 +198,7 @@ __global__ void k_heat_bd(int nel, const int *__restrict__ elems, const int *__r
                           const int *__restrict__ elpos, const unsigned char *__restrict__ bmask,
                           const double *__restrict__ xyz, int Ntop, const int *__restrict__ dof_top,
                           const int *__restrict__ bc0, const int *__restrict__ bc1, const double *__restrict__ x_old,
-                          const FeTab *__restrict__ tab, HeatArgs a, int mode, double *__restrict__ val,
+                          const FeTab *__restrict__ tab, HeatArgs a, int mode, int trans, double *__restrict__ val,
                           double *__restrict__ rhs) {
   constexpr int N2 = DIM == 2 ? 9 : 27, NB = DIM == 2 ? 3 : 9, NGB = NB, NS = 2 * DIM;
   int ei = blockIdx.x * blockDim.x + threadIdx.x;
@@ -242,7 +242,7 @@ __global__ void k_heat_bd(int nel, const int *__restrict__ elems, const int *__r
           if (bc_alpha == 0) continue;
           double phii = tab->bphi2[i * NGB + qp];
           for (int j = 0; j < NB; j++) {
-            int p = elpos[pbase + lei * N2 + sur[j]];
+            int p = elpos[pbase + (trans ? sur[j] * N2 + lei : lei * N2 + sur[j])];
             if (p >= 0) val[p] += dtxJxW * bc_alpha * a.alpha0 * phii * tab->bphi2[j * NGB + qp];
           }
         }
@@ -260,7 +260,7 @@ __global__ __launch_bounds__(256) void k_ns_vol(int nel, const int *__restrict__
                                                 const double *__restrict__ xyz, int Ntop, const int *__restrict__ dof_top,
                                                 const int *__restrict__ bc0, const double *__restrict__ x_old,
                                                 const double *__restrict__ coef, const FeTab *__restrict__ tab, NsArgs a,
-                                                int mode, double *__restrict__ val, double *__restrict__ rhs) {
+                                                int mode, int trans, double *__restrict__ val, double *__restrict__ rhs) {
   constexpr int N2 = DIM == 2 ? 9 : 27, N1 = DIM == 2 ? 4 : 8, NG = N2, NL = DIM * N2 + N1;
   __shared__ double s_phi[N2 * NG], s_psi[N1 * NG];
   __shared__ double s_dref[DIM * N2 * NG];
@@ -358,7 +358,7 @@ __global__ __launch_bounds__(256) void k_ns_vol(int nel, const int *__restrict__
         const int i = ra - DIM * N2, jv = cb / N2, j = cb % N2;
         for (int qp = 0; qp < NG; qp++) K += s_JxW[qp] * (s_psi[i * NG + qp] * s_dg[(qp * N2 + j) * DIM + jv]);
       }
-      const int p = elpos[pbase + t];
+      const int p = elpos[pbase + (trans ? cb * NL + ra : t)];
       if (p >= 0) val[p] += K;
     }
     if (mode == 1 && tid < DIM * N2) {
@@ -380,7 +380,7 @@ template <int DIM>
 __global__ void k_ns_bd(int nel, const int *__restrict__ elems, const int *__restrict__ conn, const int *__restrict__ eldof,
                         const int *__restrict__ elpos, const unsigned char *__restrict__ bmask, const double *__restrict__ xyz,
                         int Ntop, const int *__restrict__ dof_top, const int *__restrict__ bc0, const int *__restrict__ bc1,
-                        const double *__restrict__ x_old, const FeTab *__restrict__ tab, NsArgs a, int mode,
+                        const double *__restrict__ x_old, const FeTab *__restrict__ tab, NsArgs a, int mode, int trans,
                         double *__restrict__ val, double *__restrict__ rhs) {
   constexpr int N2 = DIM == 2 ? 9 : 27, N1 = DIM == 2 ? 4 : 8, NB = DIM == 2 ? 3 : 9, NGB = NB, NS = 2 * DIM;
   constexpr int NL = DIM * N2 + N1;
@@ -430,7 +430,8 @@ __global__ void k_ns_bd(int nel, const int *__restrict__ elems, const int *__res
           int p = elpos[pbase + row * NL + row];
           if (p >= 0) val[p] += ((1 - bc_n) * (1 - bc_tg) + bc_tg) * Ipen + Ipen * nrm[iv] * nrm[iv] * (bc_n - bc_tg);
           for (int jv = iv + 1; jv < DIM; jv++) {
-            int pc = elpos[pbase + row * NL + sur[i] + jv * N2];
+            const int col = sur[i] + jv * N2;
+            int pc = elpos[pbase + (trans ? col * NL + row : row * NL + col)];
             if (pc >= 0) val[pc] += Ipen * nrm[jv] * nrm[iv] * (bc_n - bc_tg);
           }
         }
@@ -450,7 +451,8 @@ __global__ void k_ns_bd(int nel, const int *__restrict__ elems, const int *__res
               int p = elpos[pbase + row * NL + row];
               if (p >= 0) val[p] += dtJxW * bc_tg * phij * phii + dtJxW * nrm[iv] * nrm[iv] * phij * phii * (bc_n - bc_tg);
               for (int jv = iv + 1; jv < DIM; jv++) {
-                int pc = elpos[pbase + row * NL + sur[j] + jv * N2];
+                const int col = sur[j] + jv * N2;
+                int pc = elpos[pbase + (trans ? col * NL + row : row * NL + col)];
                 if (pc >= 0) val[pc] += dtJxW * nrm[jv] * nrm[iv] * phij * phii * (bc_n - bc_tg);
               }
             }
@@ -461,7 +463,96 @@ __global__ void k_ns_bd(int nel, const int *__restrict__ elems, const int *__res
   }
 }
 
+// ===========================================================================
+// Heat functional / gradient kernel (top level, one workgroup per element):
+//   u_e = Klap_e T_e with Klap_e(i,j) = sum_q JxW_q grad phi_i . grad phi_j
+//   diss[e] = T_e . u_e ;  adj[e] = sum_i bc_vol_i lam_i u_i
+//   y[dof_i] += ysc * coef_e * u_i   (colour by colour, when y != null)
+// ===========================================================================
+template <int DIM>
+__global__ __launch_bounds__(64) void k_heat_lap(int nel, const int *__restrict__ elems, const int *__restrict__ conn,
+                                                 const int *__restrict__ eldof, const double *__restrict__ xyz, int Ntop,
+                                                 const int *__restrict__ dof_top, const int *__restrict__ bc0,
+                                                 const double *__restrict__ T, const double *__restrict__ lam,
+                                                 const double *__restrict__ coef, const FeTab *__restrict__ tab,
+                                                 double *__restrict__ diss, double *__restrict__ adj,
+                                                 double *__restrict__ y, double ysc) {
+  constexpr int N2 = DIM == 2 ? 9 : 27, NG = N2;
+  __shared__ double s_xx[DIM * N2], s_T[N2], s_lam[N2], s_gT[NG * DIM], s_JxW[NG];
+  __shared__ double s_dg[NG * N2 * DIM];
+  __shared__ int s_bcv[N2], s_dof[N2];
+  __shared__ double s_red[2 * N2];
+  const int tid = threadIdx.x;
+  for (int ei = blockIdx.x; ei < nel; ei += gridDim.x) {
+    const int e = elems[ei];
+    __syncthreads();
+    if (tid < N2) {
+      int node = conn[(int64_t)e * N2 + tid];
+      int kd = dof_top[node];
+      for (int d = 0; d < DIM; d++) s_xx[tid + d * N2] = xyz[node + (int64_t)d * Ntop];
+      s_T[tid] = T[kd];
+      s_lam[tid] = lam ? lam[kd] : 0.0;
+      s_bcv[tid] = bc0[kd];
+      s_dof[tid] = eldof[(int64_t)e * N2 + tid];
+    }
+    __syncthreads();
+    if (tid < NG) {
+      const int qp = tid;
+      double IJ[DIM * DIM];
+      double det = jac_qp<DIM, N2, NG>(qp, s_xx, tab->dphi2, IJ);
+      s_JxW[qp] = det * tab->w[qp];
+      double g[DIM];
+      for (int d = 0; d < DIM; d++) g[d] = 0.;
+      for (int sn = 0; sn < N2; sn++) {
+        double gr[DIM];
+        for (int d = 0; d < DIM; d++) gr[d] = tab->dphi2[sn * NG + qp + d * N2 * NG];
+        for (int i = 0; i < DIM; i++) {
+          double sum = 0.;
+          for (int j = 0; j < DIM; j++) sum += IJ[j + i * DIM] * gr[j];
+          s_dg[(qp * N2 + sn) * DIM + i] = sum;
+          g[i] += sum * s_T[sn];
+        }
+      }
+      for (int d = 0; d < DIM; d++) s_gT[qp * DIM + d] = g[d];
+    }
+    __syncthreads();
+    if (tid < N2) {
+      const int i = tid;
+      double u = 0.;
+      for (int qp = 0; qp < NG; qp++) {
+        double t = 0.;
+        for (int d = 0; d < DIM; d++) t += s_dg[(qp * N2 + i) * DIM + d] * s_gT[qp * DIM + d];
+        u += s_JxW[qp] * t;
+      }
+      s_red[i] = s_T[i] * u;
+      s_red[N2 + i] = s_bcv[i] * s_lam[i] * u;
+      if (y) y[s_dof[i]] += ysc * (coef ? coef[e] : 1.0) * u;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double a0 = 0., a1 = 0.;
+      for (int i = 0; i < N2; i++) { a0 += s_red[i]; a1 += s_red[N2 + i]; }
+      if (diss) diss[e] = a0;
+      if (adj) adj[e] = a1;
+    }
+  }
+}
+
 namespace fmk {
+
+int heat_lap(hipStream_t s, int dim, int nel, const int *elems, const int *conn, const int *eldof, const double *xyz,
+             int Ntop, const int *dof_top, const int *bc0, const double *T, const double *lam, const double *coef,
+             const FeTab *tab, double *diss, double *adj, double *y, double ysc) {
+  if (nel <= 0) return 0;
+  unsigned g = (unsigned)(nel < 8192 ? nel : 8192);
+  if (dim == 2)
+    hipLaunchKernelGGL(k_heat_lap<2>, dim3(g), dim3(64), 0, s, nel, elems, conn, eldof, xyz, Ntop, dof_top, bc0, T, lam,
+                       coef, tab, diss, adj, y, ysc);
+  else
+    hipLaunchKernelGGL(k_heat_lap<3>, dim3(g), dim3(64), 0, s, nel, elems, conn, eldof, xyz, Ntop, dof_top, bc0, T, lam,
+                       coef, tab, diss, adj, y, ysc);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
 
 static inline unsigned nblk(int nel) {
   int b = nel < 4096 ? nel : 4096;
@@ -471,18 +562,18 @@ static inline unsigned nblk(int nel) {
 int heat_assemble(hipStream_t s, int dim, int nel, const int *elems, const int *conn, const int *eldof,
                   const int *elpos, const unsigned char *bmask, const double *xyz, int Ntop, const int *dof_top,
                   const int *bc0, const int *bc1, const double *x_old, const double *coef, const double *vel,
-                  const FeTab *tab, HeatArgs a, int mode, double *val, double *rhs) {
+                  const FeTab *tab, HeatArgs a, int mode, int trans, double *val, double *rhs) {
   if (nel <= 0) return 0;
   if (dim == 2) {
     hipLaunchKernelGGL(k_heat_vol<2>, dim3(nblk(nel)), dim3(128), 0, s, nel, elems, conn, eldof, elpos, xyz, Ntop,
-                       dof_top, bc0, x_old, coef, vel, tab, a, mode, val, rhs);
+                       dof_top, bc0, x_old, coef, vel, tab, a, mode, trans, val, rhs);
     hipLaunchKernelGGL(k_heat_bd<2>, dim3((nel + 127) / 128), dim3(128), 0, s, nel, elems, conn, eldof, elpos, bmask,
-                       xyz, Ntop, dof_top, bc0, bc1, x_old, tab, a, mode, val, rhs);
+                       xyz, Ntop, dof_top, bc0, bc1, x_old, tab, a, mode, trans, val, rhs);
   } else {
     hipLaunchKernelGGL(k_heat_vol<3>, dim3(nblk(nel)), dim3(256), 0, s, nel, elems, conn, eldof, elpos, xyz, Ntop,
-                       dof_top, bc0, x_old, coef, vel, tab, a, mode, val, rhs);
+                       dof_top, bc0, x_old, coef, vel, tab, a, mode, trans, val, rhs);
     hipLaunchKernelGGL(k_heat_bd<3>, dim3((nel + 127) / 128), dim3(128), 0, s, nel, elems, conn, eldof, elpos, bmask,
-                       xyz, Ntop, dof_top, bc0, bc1, x_old, tab, a, mode, val, rhs);
+                       xyz, Ntop, dof_top, bc0, bc1, x_old, tab, a, mode, trans, val, rhs);
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
@@ -490,18 +581,18 @@ int heat_assemble(hipStream_t s, int dim, int nel, const int *elems, const int *
 int ns_assemble(hipStream_t s, int dim, int nel, const int *elems, const int *conn, const int *eldof,
                 const int *elpos, const unsigned char *bmask, const double *xyz, int Ntop, const int *dof_top,
                 const int *bc0, const int *bc1, const double *x_old, const double *coef,
-                const FeTab *tab, NsArgs a, int mode, double *val, double *rhs) {
+                const FeTab *tab, NsArgs a, int mode, int trans, double *val, double *rhs) {
   if (nel <= 0) return 0;
   if (dim == 2) {
     hipLaunchKernelGGL(k_ns_vol<2>, dim3(nblk(nel)), dim3(256), 0, s, nel, elems, conn, eldof, elpos, xyz, Ntop,
-                       dof_top, bc0, x_old, coef, tab, a, mode, val, rhs);
+                       dof_top, bc0, x_old, coef, tab, a, mode, trans, val, rhs);
     hipLaunchKernelGGL(k_ns_bd<2>, dim3((nel + 127) / 128), dim3(128), 0, s, nel, elems, conn, eldof, elpos, bmask,
-                       xyz, Ntop, dof_top, bc0, bc1, x_old, tab, a, mode, val, rhs);
+                       xyz, Ntop, dof_top, bc0, bc1, x_old, tab, a, mode, trans, val, rhs);
   } else {
     hipLaunchKernelGGL(k_ns_vol<3>, dim3(nblk(nel)), dim3(256), 0, s, nel, elems, conn, eldof, elpos, xyz, Ntop,
-                       dof_top, bc0, x_old, coef, tab, a, mode, val, rhs);
+                       dof_top, bc0, x_old, coef, tab, a, mode, trans, val, rhs);
     hipLaunchKernelGGL(k_ns_bd<3>, dim3((nel + 127) / 128), dim3(128), 0, s, nel, elems, conn, eldof, elpos, bmask,
-                       xyz, Ntop, dof_top, bc0, bc1, x_old, tab, a, mode, val, rhs);
+                       xyz, Ntop, dof_top, bc0, bc1, x_old, tab, a, mode, trans, val, rhs);
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }

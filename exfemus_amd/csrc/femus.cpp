@@ -279,14 +279,8 @@ extern "C" int fm_system_destroy(fm_system *s) {
   if (!s) return FM_EINVAL;
   (void)hipSetDevice(s->ctx->device);
   (void)hipStreamSynchronize(s->ctx->stream);
-  for (auto &l : s->lev) {
-    free_csr(l.A); free_csr(l.R); free_csr(l.P);
-    dfree(l.x); dfree(l.b); dfree(l.r); dfree(l.t); dfree(l.d); dfree(l.d2); dfree(l.dinv);
-    dfree(l.xf); dfree(l.bf); dfree(l.rf); dfree(l.tf); dfree(l.df); dfree(l.d2f); dfree(l.dinvf);
-    dfree(l.conn); dfree(l.eldof); dfree(l.elpos); dfree(l.bmask); dfree(l.color_elems); dfree(l.coef);
-    dfree(l.Ainv); dfree(l.Ainvf); dfree(l.vinv); dfree(l.vwt); dfree(l.vwtf); dfree(l.nrm); dfree(l.res);
-    dfree(l.sb); dfree(l.sx); dfree(l.vone);
-  }
+  for (auto &l : s->lev) fm_free_level(l);
+  for (auto &l : s->alev) fm_free_level(l);
   dfree(s->d_bc0); dfree(s->d_bc1); dfree(s->d_dof_top); dfree(s->xyz); dfree(s->x_old); dfree(s->vel);
   SysExtra &ex = extra(s);
   dfree(ex.d_tab);
@@ -373,6 +367,28 @@ static int upload_csr(hipStream_t st, DevCSR &c, int64_t nr, int64_t nc, std::ve
 
 template <typename T> static inline CsrView<T> view(const DevCSR &c) { return fm_view<T>(c); }
 
+int fm_alloc_level_vectors(Level &l, bool top) {
+  dfree(l.x); dfree(l.b); dfree(l.r); dfree(l.t); dfree(l.d); dfree(l.d2); dfree(l.dinv); dfree(l.res);
+  dfree(l.xf); dfree(l.bf); dfree(l.rf); dfree(l.tf); dfree(l.df); dfree(l.d2f); dfree(l.dinvf); dfree(l.nrm);
+  dfree(l.sb); dfree(l.sx);
+  FM_CHECK(dalloc(&l.x, l.n)); FM_CHECK(dalloc(&l.b, l.n)); FM_CHECK(dalloc(&l.r, l.n)); FM_CHECK(dalloc(&l.t, l.n));
+  FM_CHECK(dalloc(&l.d, l.n)); FM_CHECK(dalloc(&l.d2, l.n)); FM_CHECK(dalloc(&l.dinv, l.n)); FM_CHECK(dalloc(&l.res, l.n));
+  FM_CHECK(dalloc(&l.xf, l.n)); FM_CHECK(dalloc(&l.bf, l.n)); FM_CHECK(dalloc(&l.rf, l.n)); FM_CHECK(dalloc(&l.tf, l.n));
+  FM_CHECK(dalloc(&l.df, l.n)); FM_CHECK(dalloc(&l.d2f, l.n)); FM_CHECK(dalloc(&l.dinvf, l.n));
+  FM_CHECK(dalloc(&l.nrm, 8));
+  if (top) { FM_CHECK(dalloc(&l.sb, l.n)); FM_CHECK(dalloc(&l.sx, l.n)); }
+  return 0;
+}
+
+void fm_free_level(Level &l) {
+  free_csr(l.A); free_csr(l.R); free_csr(l.P);
+  dfree(l.x); dfree(l.b); dfree(l.r); dfree(l.t); dfree(l.d); dfree(l.d2); dfree(l.dinv);
+  dfree(l.xf); dfree(l.bf); dfree(l.rf); dfree(l.tf); dfree(l.df); dfree(l.d2f); dfree(l.dinvf);
+  dfree(l.conn); dfree(l.eldof); dfree(l.elpos); dfree(l.bmask); dfree(l.color_elems); dfree(l.coef);
+  dfree(l.Ainv); dfree(l.Ainvf); dfree(l.vinv); dfree(l.vwt); dfree(l.vwtf); dfree(l.nrm); dfree(l.res);
+  dfree(l.sb); dfree(l.sx); dfree(l.vone);
+}
+
 extern "C" int fm_system_init(fm_system *s) {
   if (!s) return FM_EINVAL;
   fm_ctx *c = s->ctx;
@@ -421,15 +437,7 @@ extern "C" int fm_system_init(fm_system *s) {
       FM_CHECK(upload_csr(st, l.P, l.n, s->Dim[L - 1], trp, tci, &tv));
     }
     // vectors
-    dfree(l.x); dfree(l.b); dfree(l.r); dfree(l.t); dfree(l.d); dfree(l.d2); dfree(l.dinv); dfree(l.res);
-    dfree(l.xf); dfree(l.bf); dfree(l.rf); dfree(l.tf); dfree(l.df); dfree(l.d2f); dfree(l.dinvf); dfree(l.nrm);
-    FM_CHECK(dalloc(&l.x, l.n)); FM_CHECK(dalloc(&l.b, l.n)); FM_CHECK(dalloc(&l.r, l.n)); FM_CHECK(dalloc(&l.t, l.n));
-    FM_CHECK(dalloc(&l.d, l.n)); FM_CHECK(dalloc(&l.d2, l.n)); FM_CHECK(dalloc(&l.dinv, l.n)); FM_CHECK(dalloc(&l.res, l.n));
-    FM_CHECK(dalloc(&l.xf, l.n)); FM_CHECK(dalloc(&l.bf, l.n)); FM_CHECK(dalloc(&l.rf, l.n)); FM_CHECK(dalloc(&l.tf, l.n));
-    FM_CHECK(dalloc(&l.df, l.n)); FM_CHECK(dalloc(&l.d2f, l.n)); FM_CHECK(dalloc(&l.dinvf, l.n));
-    FM_CHECK(dalloc(&l.nrm, 8));
-    dfree(l.sb); dfree(l.sx);
-    if (L == top) { FM_CHECK(dalloc(&l.sb, l.n)); FM_CHECK(dalloc(&l.sx, l.n)); }
+    FM_CHECK(fm_alloc_level_vectors(l, L == top));
     FM_HIP(hipMemsetAsync(l.x, 0, l.n * sizeof(double), st));
     FM_HIP(hipMemsetAsync(l.b, 0, l.n * sizeof(double), st));
     FM_HIP(hipMemsetAsync(l.res, 0, l.n * sizeof(double), st));
@@ -462,6 +470,10 @@ extern "C" int fm_system_init(fm_system *s) {
     FM_CHECK(fmk::elpos(st, l.nel, nloc, l.eldof, l.A.rp, l.A.ci, l.elpos));
     FM_HIP(hipStreamSynchronize(st));
   }
+  for (auto &l : s->alev) fm_free_level(l);
+  s->alev.clear();
+  s->a_assembled.clear();
+  s->adj_ready = false;
   s->initialized = true;
   s->mg_ready = false;
   return FM_OK;
@@ -572,7 +584,7 @@ extern "C" int fm_heat_genmatrhs(fm_system *s, int L, int mode, const fm_heat_pa
     int n = l.color_off[k + 1] - l.color_off[k];
     FM_CHECK(fmk::heat_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldof, l.elpos, l.bmask, s->xyz,
                                 m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, l.coef, s->vel, ex.d_tab, a, mode,
-                                l.A.v, l.b));
+                                0, l.A.v, l.b));
   }
   FM_CHECK(after_assembly(s, L));
   FM_HIP(hipStreamSynchronize(st));
@@ -595,7 +607,7 @@ extern "C" int fm_ns_genmatrhs(fm_system *s, int L, int mode, const fm_ns_params
   for (int k = 0; k < l.ncolor; k++) {
     int n = l.color_off[k + 1] - l.color_off[k];
     FM_CHECK(fmk::ns_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldof, l.elpos, l.bmask, s->xyz,
-                              m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, l.coef, ex.d_tab, a, mode, l.A.v, l.b));
+                              m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, l.coef, ex.d_tab, a, mode, 0, l.A.v, l.b));
   }
   FM_CHECK(after_assembly(s, L));
   FM_HIP(hipStreamSynchronize(st));
@@ -663,5 +675,187 @@ extern "C" int fm_bench_spmv(fm_system *s, int L, int prec, int reps, double *se
   float ms = 0.f;
   FM_HIP(hipEventElapsedTime(&ms, s->ctx->ev0, s->ctx->ev1));
   *sec = ms * 1e-3 / reps;
+  return FM_OK;
+}
+
+// ===========================================================================
+// discrete adjoint: transposed hierarchy, transposed assembly, functionals
+// ===========================================================================
+static void transpose_host(int64_t nr, int64_t nc, const std::vector<int64_t> &rp, const std::vector<int> &ci,
+                           const std::vector<double> &v, std::vector<int64_t> &rpT, std::vector<int> &ciT,
+                           std::vector<double> &vT) {
+  rpT.assign(nc + 1, 0);
+  for (int64_t q = 0; q < rp[nr]; q++) rpT[ci[q] + 1]++;
+  for (int64_t j = 0; j < nc; j++) rpT[j + 1] += rpT[j];
+  ciT.resize(rp[nr]);
+  vT.resize(rp[nr]);
+  std::vector<int64_t> f(rpT.begin(), rpT.end() - 1);
+  for (int64_t i = 0; i < nr; i++)
+    for (int64_t q = rp[i]; q < rp[i + 1]; q++) { int64_t t = f[ci[q]]++; ciT[t] = (int)i; vT[t] = v[q]; }
+}
+
+static int ensure_adjoint(fm_system *s) {
+  if (!s->alev.empty()) return 0;
+  hipStream_t st = s->ctx->stream;
+  int nlev = (int)s->lev.size(), top = nlev - 1;
+  s->alev.resize(nlev);
+  s->a_assembled.assign(nlev, 0);
+  for (int L = 0; L < nlev; L++) {
+    Level &a = s->alev[L], &l = s->lev[L];
+    a.n = l.n;
+    a.nel = l.nel;
+    std::vector<int64_t> rp = l.A.h_rp;
+    std::vector<int> ci = l.A.h_ci;
+    FM_CHECK(upload_csr(st, a.A, l.n, l.n, rp, ci, nullptr));
+    if (L < top) {  // restriction of the adjoint = (prolongation of level L+1)^T
+      const DevCSR &P = s->lev[L + 1].P;
+      std::vector<int64_t> rpT; std::vector<int> ciT; std::vector<double> vT;
+      transpose_host(P.nr, P.nc, P.h_rp, P.h_ci, P.h_v, rpT, ciT, vT);
+      FM_CHECK(upload_csr(st, a.R, P.nc, P.nr, rpT, ciT, &vT));
+    }
+    if (L > 0) {  // prolongation of the adjoint = (restriction of level L-1)^T
+      const DevCSR &R = s->lev[L - 1].R;
+      std::vector<int64_t> rpT; std::vector<int> ciT; std::vector<double> vT;
+      transpose_host(R.nr, R.nc, R.h_rp, R.h_ci, R.h_v, rpT, ciT, vT);
+      FM_CHECK(upload_csr(st, a.P, R.nc, R.nr, rpT, ciT, &vT));
+    }
+    FM_CHECK(fm_alloc_level_vectors(a, L == top));
+    FM_HIP(hipMemsetAsync(a.x, 0, a.n * sizeof(double), st));
+  }
+  FM_HIP(hipStreamSynchronize(st));
+  return 0;
+}
+
+static int after_assembly_T(fm_system *s, int L) {
+  Level &a = s->alev[L];
+  hipStream_t st = s->ctx->stream;
+  FM_CHECK(fmk::cast_vals(st, a.A.nnz, a.A.v, a.A.vf));
+  FM_CHECK(fm_sync_sell(st, a.A));
+  FM_CHECK(fmk::diag_inv(st, a.n, a.A.rp, a.A.ci, a.A.v, a.dinv, a.dinvf));
+  a.lam = 0.;
+  s->a_assembled[L] = 1;
+  s->adj_ready = false;
+  FM_HIP(hipStreamSynchronize(st));
+  return 0;
+}
+
+extern "C" int fm_heat_genmat_transpose(fm_system *s, int L, const fm_heat_params *p) {
+  FM_CHECK(check_level(s, L));
+  if (!p || s->nq2 != 1 || s->nq1 != 0 || p->dt == 0. || p->alpha0 <= 0.) return FM_EINVAL;
+  FM_HIP(hipSetDevice(s->ctx->device));
+  FM_CHECK(ensure_adjoint(s));
+  Level &l = s->lev[L], &a = s->alev[L];
+  hipStream_t st = s->ctx->stream;
+  const fm_mesh &m = *s->mesh;
+  HeatArgs h;
+  h.alpha0 = p->alpha0; h.rhocp = p->rhocp; h.dt = p->dt; h.tc = p->time_coeff; h.source = p->source; h.upwind = p->upwind;
+  for (int d = 0; d < 3; d++) h.vel_const[d] = p->vel_const[d];
+  FM_HIP(hipMemsetAsync(a.A.v, 0, a.A.nnz * sizeof(double), st));
+  SysExtra &ex = extra(s);
+  for (int k = 0; k < l.ncolor; k++) {
+    int n = l.color_off[k + 1] - l.color_off[k];
+    FM_CHECK(fmk::heat_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldof, l.elpos, l.bmask, s->xyz,
+                                m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, l.coef, s->vel, ex.d_tab, h, 0, 1,
+                                a.A.v, nullptr));
+  }
+  return after_assembly_T(s, L);
+}
+
+extern "C" int fm_ns_genmat_transpose(fm_system *s, int L, const fm_ns_params *p) {
+  FM_CHECK(check_level(s, L));
+  const fm_mesh &m = *s->mesh;
+  if (!p || s->nq2 != m.dim || s->nq1 != 1 || p->dt == 0. || p->IRe <= 0.) return FM_EINVAL;
+  FM_HIP(hipSetDevice(s->ctx->device));
+  FM_CHECK(ensure_adjoint(s));
+  Level &l = s->lev[L], &a = s->alev[L];
+  hipStream_t st = s->ctx->stream;
+  NsArgs h;
+  h.IRe = p->IRe; h.dt = p->dt; h.tc = p->time_coeff; h.upwind = p->upwind; h.IFr = p->IFr;
+  for (int d = 0; d < 3; d++) { h.dirg[d] = p->dirg[d]; h.src[d] = p->src[d]; }
+  FM_HIP(hipMemsetAsync(a.A.v, 0, a.A.nnz * sizeof(double), st));
+  SysExtra &ex = extra(s);
+  for (int k = 0; k < l.ncolor; k++) {
+    int n = l.color_off[k + 1] - l.color_off[k];
+    FM_CHECK(fmk::ns_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldof, l.elpos, l.bmask, s->xyz,
+                              m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, l.coef, ex.d_tab, h, 0, 1, a.A.v,
+                              nullptr));
+  }
+  return after_assembly_T(s, L);
+}
+
+extern "C" int fm_get_adjoint_csr(const fm_system *s, int L, double *val) {
+  if (!s || L < 0 || L >= (int)s->alev.size() || !val) return FM_EINVAL;
+  const DevCSR &c = s->alev[L].A;
+  FM_HIP(hipStreamSynchronize(s->ctx->stream));
+  FM_HIP(hipMemcpy(val, c.v, c.nnz * sizeof(double), hipMemcpyDeviceToHost));
+  return FM_OK;
+}
+
+// J = alpha0 * sum_e k_e T_e^T Klap_e T_e;  dJ/dT = 2 alpha0 sum_e k_e Klap_e T_e;  dJ/dk_e = alpha0 T_e^T Klap_e T_e
+extern "C" int fm_heat_dissipation(fm_system *s, const fm_heat_params *p, const double *T_host, double *J,
+                                   double *dJdT_host, double *dJdk_host) {
+  FM_CHECK(check_level(s, 0));
+  if (!p || !T_host || s->nq2 != 1 || s->nq1 != 0) return FM_EINVAL;
+  int top = (int)s->lev.size() - 1;
+  Level &l = s->lev[top];
+  hipStream_t st = s->ctx->stream;
+  const fm_mesh &m = *s->mesh;
+  FM_HIP(hipMemcpyAsync(l.t, T_host, l.n * sizeof(double), hipMemcpyHostToDevice, st));
+  FM_CHECK(fmk::fill<double>(st, l.n, l.d, 0.0));
+  double *diss = nullptr;
+  FM_CHECK(dalloc(&diss, l.nel));
+  SysExtra &ex = extra(s);
+  int rc = 0;
+  for (int k = 0; k < l.ncolor && !rc; k++) {
+    int n = l.color_off[k + 1] - l.color_off[k];
+    rc = fmk::heat_lap(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldof, s->xyz, m.Ntop, s->d_dof_top,
+                       s->d_bc0, l.t, nullptr, l.coef, ex.d_tab, diss, nullptr, l.d, 2.0 * p->alpha0);
+  }
+  std::vector<double> h(l.nel), hk(l.nel, 1.0);
+  if (!rc) {
+    FM_HIP(hipMemcpyAsync(h.data(), diss, l.nel * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (l.coef) FM_HIP(hipMemcpyAsync(hk.data(), l.coef, l.nel * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (dJdT_host) FM_HIP(hipMemcpyAsync(dJdT_host, l.d, l.n * sizeof(double), hipMemcpyDeviceToHost, st));
+    FM_HIP(hipStreamSynchronize(st));
+  }
+  dfree(diss);
+  if (rc) return FM_EHIP;
+  double acc = 0.;
+  for (int e = 0; e < l.nel; e++) {
+    acc += hk[e] * p->alpha0 * h[e];
+    if (dJdk_host) dJdk_host[e] = p->alpha0 * h[e];
+  }
+  if (J) *J = acc;
+  return FM_OK;
+}
+
+// adjoint part of dJ/dk_e: -lambda^T (dA/dk_e) T = -alpha0 sum_i bc_vol_i lambda_i (Klap_e T_e)_i
+extern "C" int fm_heat_grad_coef(fm_system *s, const fm_heat_params *p, const double *lam_host, const double *T_host,
+                                 double *grad_host) {
+  FM_CHECK(check_level(s, 0));
+  if (!p || !lam_host || !T_host || !grad_host || s->nq2 != 1 || s->nq1 != 0) return FM_EINVAL;
+  int top = (int)s->lev.size() - 1;
+  Level &l = s->lev[top];
+  hipStream_t st = s->ctx->stream;
+  const fm_mesh &m = *s->mesh;
+  FM_HIP(hipMemcpyAsync(l.t, T_host, l.n * sizeof(double), hipMemcpyHostToDevice, st));
+  FM_HIP(hipMemcpyAsync(l.d2, lam_host, l.n * sizeof(double), hipMemcpyHostToDevice, st));
+  double *adj = nullptr;
+  FM_CHECK(dalloc(&adj, l.nel));
+  SysExtra &ex = extra(s);
+  int rc = fmk::heat_lap(st, m.dim, l.nel, l.color_elems, l.conn, l.eldof, s->xyz, m.Ntop, s->d_dof_top, s->d_bc0,
+                         l.t, l.d2, l.coef, ex.d_tab, nullptr, adj, nullptr, 0.0);
+  std::vector<double> h(l.nel);
+  if (!rc) {
+    FM_HIP(hipMemcpyAsync(h.data(), adj, l.nel * sizeof(double), hipMemcpyDeviceToHost, st));
+    FM_HIP(hipStreamSynchronize(st));
+  }
+  dfree(adj);
+  if (rc) return FM_EHIP;
+  // elements of l.color_elems are listed colour by colour: map back to element ids
+  std::vector<int> ce(l.nel);
+  FM_HIP(hipMemcpy(ce.data(), l.color_elems, l.nel * sizeof(int), hipMemcpyDeviceToHost));
+  (void)ce;
+  for (int e = 0; e < l.nel; e++) grad_host[e] = -p->alpha0 * h[e];
   return FM_OK;
 }

@@ -111,6 +111,12 @@ struct fm_system {
   double *x_old = nullptr;                  // top-level x_old (device)
   double *vel = nullptr;                    // heat advection field [dim*Ntop] or nullptr
   std::vector<Level> lev;
+  // transposed hierarchy for the discrete adjoint: A^T (same pattern as A),
+  // restriction Prl[L+1]^T, prolongation Rst[L-1]^T, own work vectors
+  std::vector<Level> alev;
+  std::vector<char> a_assembled;
+  bool adj_ready = false;
+  int adj_smoother = -1;
   bool initialized = false;
   bool mg_ready = false;
   int mg_smoother = -1;
@@ -150,5 +156,7 @@ template <> inline CsrView<float> fm_view<float>(const DevCSR &c) {
   return v;
 }
 int fm_build_sell(hipStream_t st, DevCSR &c);
+int fm_alloc_level_vectors(Level &l, bool top);
+void fm_free_level(Level &l);
 int fm_sync_sell(hipStream_t st, DevCSR &c);
 void fm_free_csr(DevCSR &c);

@@ -14,6 +14,7 @@
 //    (template_appl/nse/SRC/MGSolverT.C:93-525, MGSolverNS.C:108-717).
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <cstdlib>
 #include "fm_kernels.h"
 
 #define WAVE 64
@@ -74,13 +75,19 @@ __global__ __launch_bounds__(256) void k_spmv(int64_t nrows, const int64_t *__re
 
 // SELL-64: one wavefront per slice, one lane per row, entries of a row summed
 // in order; matrix values and columns are streamed with non-temporal loads so
-// that the gathered vector stays resident in L2 / Infinity Cache.
-template <typename T, int EP>
+// that the gathered vector stays resident in L2 / Infinity Cache.  With
+// xcd_chunk > 0 the block index is remapped so that the blocks the hardware
+// places on one XCD (block id modulo 8) walk one contiguous range of slices:
+// neighbouring rows, which gather neighbouring x entries, then share that
+// XCD's L2.
+template <typename T, int EP, int UNR>
 __global__ __launch_bounds__(256) void k_sell(int64_t nsl, const int64_t *__restrict__ sl_off,
                                               const int *__restrict__ sl_w, const int *__restrict__ perm,
                                               const int *__restrict__ cols, const T *__restrict__ vals,
-                                              const T *__restrict__ xg, EpArgs<T> ep) {
-  const int64_t s = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+                                              const T *__restrict__ xg, EpArgs<T> ep, int64_t xcd_chunk) {
+  int64_t blk = blockIdx.x;
+  if (xcd_chunk > 0) blk = (int64_t)(blockIdx.x % 8) * xcd_chunk + blockIdx.x / 8;
+  const int64_t s = blk * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (s >= nsl) return;
   const int lane = threadIdx.x & 63;
   const int row = perm[s * 64 + lane];
@@ -89,24 +96,33 @@ __global__ __launch_bounds__(256) void k_sell(int64_t nsl, const int64_t *__rest
   const int *cp = cols + sl_off[s] + lane;
   T sum = 0;
   int k = 0;
-  for (; k + 4 <= w; k += 4) {
-    const T v0 = __builtin_nontemporal_load(vp + (k + 0) * 64);
-    const T v1 = __builtin_nontemporal_load(vp + (k + 1) * 64);
-    const T v2 = __builtin_nontemporal_load(vp + (k + 2) * 64);
-    const T v3 = __builtin_nontemporal_load(vp + (k + 3) * 64);
-    const int c0 = __builtin_nontemporal_load(cp + (k + 0) * 64);
-    const int c1 = __builtin_nontemporal_load(cp + (k + 1) * 64);
-    const int c2 = __builtin_nontemporal_load(cp + (k + 2) * 64);
-    const int c3 = __builtin_nontemporal_load(cp + (k + 3) * 64);
-    const T x0 = xg[c0], x1 = xg[c1], x2 = xg[c2], x3 = xg[c3];
-    sum += v0 * x0;
-    sum += v1 * x1;
-    sum += v2 * x2;
-    sum += v3 * x3;
+  for (; k + UNR <= w; k += UNR) {
+    T v[UNR];
+    int c[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; u++) v[u] = __builtin_nontemporal_load(vp + (k + u) * 64);
+#pragma unroll
+    for (int u = 0; u < UNR; u++) c[u] = __builtin_nontemporal_load(cp + (k + u) * 64);
+    T xv[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; u++) xv[u] = xg[c[u]];
+#pragma unroll
+    for (int u = 0; u < UNR; u++) sum += v[u] * xv[u];
   }
   for (; k < w; k++) sum += __builtin_nontemporal_load(vp + k * 64) * xg[__builtin_nontemporal_load(cp + k * 64)];
   if (row < 0) return;
   apply_ep<T, EP>(row, sum, xg, ep);
+}
+
+// launch-shape knobs of the SELL kernel (read once from the environment):
+// FEMUS_SELL_UNROLL = 4 | 8 (default 4), FEMUS_SELL_XCD = 0 | 1 (default 1)
+static int g_sell_unroll = -1, g_sell_xcd = -1;
+static void sell_knobs() {
+  if (g_sell_unroll >= 0) return;
+  const char *u = getenv("FEMUS_SELL_UNROLL");
+  const char *x = getenv("FEMUS_SELL_XCD");
+  g_sell_unroll = (u && atoi(u) == 8) ? 8 : 4;
+  g_sell_xcd = (x && atoi(x) == 0) ? 0 : 1;
 }
 
 template <typename T, int EP>
@@ -114,8 +130,20 @@ static int launch_spmv(hipStream_t s, const CsrView<T> &A, const T *x, const EpA
   if (A.nr <= 0) return 0;
   const int block = 256;
   if (A.nsl > 0) {
-    dim3 grid((unsigned)((A.nsl + 3) / 4));
-    hipLaunchKernelGGL((k_sell<T, EP>), grid, dim3(block), 0, s, A.nsl, A.sl_off, A.sl_w, A.perm, A.scols, A.sv, x, ep);
+    sell_knobs();
+    int64_t nb = (A.nsl + 3) / 4;
+    int64_t chunk = 0;
+    if (g_sell_xcd && nb >= 64) {
+      chunk = (nb + 7) / 8;
+      nb = chunk * 8;  // padded grid: surplus blocks map past the last slice and exit
+    }
+    dim3 grid((unsigned)nb);
+    if (g_sell_unroll == 8)
+      hipLaunchKernelGGL((k_sell<T, EP, 8>), grid, dim3(block), 0, s, A.nsl, A.sl_off, A.sl_w, A.perm, A.scols, A.sv, x,
+                         ep, chunk);
+    else
+      hipLaunchKernelGGL((k_sell<T, EP, 4>), grid, dim3(block), 0, s, A.nsl, A.sl_off, A.sl_w, A.perm, A.scols, A.sv, x,
+                         ep, chunk);
     return hipGetLastError() == hipSuccess ? 0 : -2;
   }
   int64_t threads = A.nr * A.lpr;

@@ -503,15 +503,16 @@ extern "C" int fm_mg_solve(fm_system *s, const fm_mg_opts *o, fm_mg_report *rep)
 }
 
 // ---- Vanka (vanka.hip) -----------------------------------------------------
-int fm_vanka_invert(fm_system *s, int L);
+int fm_vanka_invert(fm_system *s, int L, const double *Av, double **store);
 int fm_vanka_apply64(fm_system *s, int L, const CsrView<double> &A, const double *vinv, double *x, const double *b,
                      double *r, int its, double omega, int colored);
 int fm_vanka_apply32(fm_system *s, int L, const CsrView<float> &A, const double *vinv, float *x, const float *b,
                      float *r, int its, double omega, int colored);
 int fm_vanka_setup(fm_system *s, int L, std::vector<LOps> *ops) {
-  int rc = fm_vanka_invert(s, L);
+  LOps &o = (*ops)[L];
+  int rc = fm_vanka_invert(s, L, o.A->v, &o.v->vinv);
   if (rc) return rc;
-  (*ops)[L].vinv = s->lev[L].vinv;
+  o.vinv = o.v->vinv;
   return 0;
 }
 int fm_vanka_sweep64(fm_system *s, int L, const LOps &o, double *x, const double *b, int its, double omega, int colored) {
@@ -521,13 +522,42 @@ int fm_vanka_sweep32(fm_system *s, int L, const LOps &o, float *x, const float *
   return fm_vanka_apply32(s, L, cview<float>(*o.A), o.vinv, x, b, o.v->rf, its, omega, colored);
 }
 
+// Transposed multigrid: A^T lambda = g on the adjoint hierarchy (A^T from the
+// transposed assembly, restriction Prl^T, prolongation Rst^T, smoothers and
+// coarse inverse of A^T), same outer solvers as the forward problem.
 extern "C" int fm_adjoint_solve(fm_system *s, const fm_mg_opts *o, const double *g, double *lam, fm_mg_report *rep) {
-  (void)s; (void)o; (void)g; (void)lam; (void)rep;
-  fm_set_error("adjoint solve not available in this build");
-  return FM_EINVAL;
-}
-extern "C" int fm_heat_grad_coef(fm_system *s, const fm_heat_params *p, const double *lam, const double *x, double *grad) {
-  (void)s; (void)p; (void)lam; (void)x; (void)grad;
-  fm_set_error("gradient not available in this build");
-  return FM_EINVAL;
+  if (!s || !o || !g || !lam || !rep) return FM_EINVAL;
+  if (!s->initialized || s->alev.empty()) { fm_set_error("adjoint: assemble A^T on every level first"); return FM_ESTATE; }
+  for (char a : s->a_assembled) if (!a) { fm_set_error("adjoint: A^T missing on a level"); return FM_ESTATE; }
+  FM_HIP(hipSetDevice(s->ctx->device));
+  std::memset(rep, 0, sizeof(*rep));
+  std::vector<LOps> ops(s->alev.size());
+  for (size_t L = 0; L < s->alev.size(); L++) {
+    Level &a = s->alev[L];
+    LOps &op = ops[L];
+    op.n = a.n; op.A = &a.A; op.R = &a.R; op.P = &a.P; op.dinv = a.dinv; op.dinvf = a.dinvf; op.lam = a.lam;
+    op.Ainv = a.Ainv; op.Ainvf = a.Ainvf; op.v = &a; op.vinv = a.vinv;
+  }
+  if (!s->adj_ready || s->adj_smoother != o->smoother) {
+    FM_CHECK(mg_setup_ops(s, o, ops, true));
+    for (size_t L = 0; L < s->alev.size(); L++) {
+      s->alev[L].lam = ops[L].lam; s->alev[L].Ainv = ops[L].Ainv; s->alev[L].Ainvf = ops[L].Ainvf;
+    }
+    s->adj_ready = true;
+    s->adj_smoother = o->smoother;
+  }
+  int top = (int)ops.size() - 1;
+  Level &at = s->alev[top];
+  hipStream_t st = s->ctx->stream;
+  FM_HIP(hipMemcpyAsync(at.b, g, at.n * sizeof(double), hipMemcpyHostToDevice, st));
+  FM_CHECK(fmk::fill<double>(st, at.n, at.x, 0.0));
+  Timer tm(s->ctx);
+  int rc;
+  if (o->precision == FM_PREC_FP64 || o->precision == FM_PREC_FP32_IR) rc = solve_stationary(s, o, ops, rep);
+  else rc = solve_fgmres(s, o, ops, rep);
+  rep->seconds = tm.stop();
+  if (rc) return rc;
+  FM_HIP(hipMemcpyAsync(lam, at.x, at.n * sizeof(double), hipMemcpyDeviceToHost, st));
+  FM_HIP(hipStreamSynchronize(st));
+  return rep->converged ? FM_OK : FM_ENOCONV;
 }

@@ -135,16 +135,15 @@ __global__ void k_recip(int64_t n, double *w) {
     w[i] = w[i] > 0. ? 1.0 / w[i] : 0.0;
 }
 
-struct LOps;
-int fm_vanka_setup(fm_system *s, int L, std::vector<LOps> *ops);
 
-static int vanka_invert_level(fm_system *s, int L) {
+// patch inverses of the matrix with values Av (A or A^T: same pattern) into *store
+static int vanka_invert_level(fm_system *s, int L, const double *Av, double **store) {
   Level &l = s->lev[L];
   hipStream_t st = s->ctx->stream;
   const int nl = s->nloc;
   if (s->nq1 == 0) { fm_set_error("Vanka needs a Q2-Q1 (velocity-pressure) system"); return FM_EINVAL; }
-  if (!l.vinv) {
-    if (hipMalloc((void **)&l.vinv, (size_t)l.nel * nl * nl * sizeof(double)) != hipSuccess) { fm_set_error("Vanka alloc"); return FM_ENOMEM; }
+  if (!*store) {
+    if (hipMalloc((void **)store, (size_t)l.nel * nl * nl * sizeof(double)) != hipSuccess) { fm_set_error("Vanka alloc"); return FM_ENOMEM; }
   }
   if (!l.vwt) {
     if (hipMalloc((void **)&l.vwt, l.n * sizeof(double)) != hipSuccess) { fm_set_error("Vanka alloc"); return FM_ENOMEM; }
@@ -159,9 +158,9 @@ static int vanka_invert_level(fm_system *s, int L) {
   FM_HIP(hipMemsetAsync(l.vwt, 0, l.n * sizeof(double), st));
   unsigned g = (unsigned)(l.nel < 8192 ? l.nel : 8192);
   if (s->mesh->dim == 2)
-    hipLaunchKernelGGL(k_vanka_invert<22>, dim3(g), dim3(256), 0, st, l.nel, l.elpos, l.A.v, l.vinv, status);
+    hipLaunchKernelGGL(k_vanka_invert<22>, dim3(g), dim3(256), 0, st, l.nel, l.elpos, Av, *store, status);
   else
-    hipLaunchKernelGGL(k_vanka_invert<89>, dim3(g), dim3(256), 0, st, l.nel, l.elpos, l.A.v, l.vinv, status);
+    hipLaunchKernelGGL(k_vanka_invert<89>, dim3(g), dim3(256), 0, st, l.nel, l.elpos, Av, *store, status);
   FM_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_count_mult, dim3(1024), dim3(256), 0, st, l.nel, nl, l.eldof, l.vwt);
   hipLaunchKernelGGL(k_recip, dim3(1024), dim3(256), 0, st, l.n, l.vwt);
@@ -209,7 +208,7 @@ static int vanka_sweep(fm_system *s, int L, const CsrView<T> &A, const double *v
   return hipGetLastError() == hipSuccess ? 0 : FM_EHIP;
 }
 
-int fm_vanka_invert(fm_system *s, int L) { return vanka_invert_level(s, L); }
+int fm_vanka_invert(fm_system *s, int L, const double *Av, double **store) { return vanka_invert_level(s, L, Av, store); }
 int fm_vanka_apply64(fm_system *s, int L, const CsrView<double> &A, const double *vinv, double *x, const double *b,
                      double *r, int its, double omega, int colored) {
   return vanka_sweep<double>(s, L, A, vinv, x, b, r, its, omega, colored);

@@ -36,7 +36,8 @@ EXPORTS = [
     "fm_system_set_x", "fm_system_get_rhs", "fm_system_set_rhs", "fm_system_set_elem_coef",
     "fm_system_set_velocity", "fm_heat_genmatrhs", "fm_ns_genmatrhs", "fm_spmv", "fm_residual",
     "fm_restrict", "fm_prolong", "fm_get_res", "fm_mg_setup", "fm_mg_solve", "fm_adjoint_solve",
-    "fm_heat_grad_coef", "fm_bench_spmv", "fm_sync",
+    "fm_heat_grad_coef", "fm_bench_spmv", "fm_sync", "fm_heat_genmat_transpose", "fm_ns_genmat_transpose",
+    "fm_get_adjoint_csr", "fm_heat_dissipation",
 ]
 
 
@@ -119,6 +120,10 @@ def lib():
     L.fm_mg_solve.argtypes = [vp, C.POINTER(MGOpts), C.POINTER(MGReport)]
     L.fm_adjoint_solve.argtypes = [vp, C.POINTER(MGOpts), f64p, f64p, C.POINTER(MGReport)]
     L.fm_heat_grad_coef.argtypes = [vp, C.POINTER(HeatParams), f64p, f64p, f64p]
+    L.fm_heat_genmat_transpose.argtypes = [vp, C.c_int, C.POINTER(HeatParams)]
+    L.fm_ns_genmat_transpose.argtypes = [vp, C.c_int, C.POINTER(NSParams)]
+    L.fm_get_adjoint_csr.argtypes = [vp, C.c_int, f64p]
+    L.fm_heat_dissipation.argtypes = [vp, C.POINTER(HeatParams), f64p, C.POINTER(C.c_double), vp, vp]
     L.fm_bench_spmv.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]
     _LIB = L
     return L
@@ -334,6 +339,20 @@ class MGSolDA:
         _ck(rc, ok=(0,) if raise_on_noconv else (0, FM_ENOCONV))
         return rep
 
+    def adjoint_csr(self, L):
+        nnz = int(lib().fm_system_csr_nnz(self.h, 0, L))
+        v = np.zeros(nnz)
+        _ck(lib().fm_get_adjoint_csr(self.h, L, v))
+        return v
+
+    def AdjointSolve(self, g, opts):
+        """Transposed multigrid solve A^T lambda = g (returns lambda, report)."""
+        lam = np.zeros(self._Dim[-1])
+        rep = MGReport()
+        rc = lib().fm_adjoint_solve(self.h, C.byref(opts), np.ascontiguousarray(g, np.float64), lam, C.byref(rep))
+        _ck(rc, ok=(0, FM_ENOCONV))
+        return lam, rep
+
     def bench_spmv(self, L, prec, reps):
         s = C.c_double()
         _ck(lib().fm_bench_spmv(self.h, L, prec, reps, C.byref(s)))
@@ -350,6 +369,26 @@ class MGSolT(MGSolDA):
 
     def GenMatRhs(self, time, Level, mode):
         _ck(lib().fm_heat_genmatrhs(self.h, Level, mode, C.byref(self.params)))
+
+    # -- discrete adjoint ---------------------------------------------------------
+    def GenMatTranspose(self, Level):
+        _ck(lib().fm_heat_genmat_transpose(self.h, Level, C.byref(self.params)))
+
+    def dissipation(self, T):
+        """J = alpha0 sum_e k_e int |grad T|^2, dJ/dT, dJ/dk_e (partial) on the top level."""
+        J = C.c_double()
+        g = np.zeros(self._Dim[-1])
+        dk = np.zeros(self.mesh.level(self._NoLevels - 1)[0])
+        _ck(lib().fm_heat_dissipation(self.h, C.byref(self.params), np.ascontiguousarray(T, np.float64), C.byref(J),
+                                      g.ctypes.data_as(C.c_void_p), dk.ctypes.data_as(C.c_void_p)))
+        return J.value, g, dk
+
+    def grad_coef(self, lam, T):
+        """-lambda^T (dA/dk_e) T for every top element."""
+        out = np.zeros(self.mesh.level(self._NoLevels - 1)[0])
+        _ck(lib().fm_heat_grad_coef(self.h, C.byref(self.params), np.ascontiguousarray(lam, np.float64),
+                                    np.ascontiguousarray(T, np.float64), out))
+        return out
 
     def set_velocity(self, vel_top):
         v = None if vel_top is None else np.ascontiguousarray(vel_top, np.float64)

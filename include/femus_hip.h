@@ -160,11 +160,25 @@ int fm_mg_setup(fm_system *s, const fm_mg_opts *o);   /* eigen bounds, coarse LU
 int fm_mg_solve(fm_system *s, const fm_mg_opts *o, fm_mg_report *rep);   /* x[top] in/out, b[top] */
 
 /* ---- discrete adjoint -------------------------------------------------- */
+/* Transpose-Jacobian assembly: the GenMatRhs kernels scatter K_e^T, giving
+ * A[L]^T of the linear (heat) / Oseen-Picard (NS) system on level L into the
+ * adjoint hierarchy.  No reference counterpart (the reference has only the
+ * continuous adjoints of template_appl/tbcont/SRC/MGSolverT_ADJ.C:88-412 and
+ * template_appl/turb-ck/SRC/MGSolverNS-A.C:356-371).                        */
+int fm_heat_genmat_transpose(fm_system *s, int L, const fm_heat_params *p);
+int fm_ns_genmat_transpose(fm_system *s, int L, const fm_ns_params *p);
+/* values of the assembled A[L]^T (pattern of A[L]) for checks              */
+int fm_get_adjoint_csr(const fm_system *s, int L, double *val);
 /* Transposed multigrid solve A^T lambda = g on the assembled hierarchy:
- * restriction and prolongation swap roles (Prl^T, Rst^T), smoothers act on A^T. */
+ * restriction and prolongation swap roles (Prl^T, Rst^T), smoothers and the
+ * coarse inverse act on A^T; outer loop and precision as in fm_mg_solve.   */
 int fm_adjoint_solve(fm_system *s, const fm_mg_opts *o, const double *g_host,
                      double *lambda_host, fm_mg_report *rep);
-/* dJ/dk_e = -lambda^T (dA/dk_e) x  for the heat system (per top element)    */
+/* Heat dissipation functional J = alpha0 sum_e k_e int |grad T|^2 over the
+ * top elements, its gradient dJ/dT (DoF order) and partials dJ/dk_e.       */
+int fm_heat_dissipation(fm_system *s, const fm_heat_params *p, const double *T_host, double *J,
+                        double *dJdT_host, double *dJdk_host);
+/* Adjoint term of dJ/dk_e: -lambda^T (dA/dk_e) T per top element          */
 int fm_heat_grad_coef(fm_system *s, const fm_heat_params *p, const double *lambda_host,
                       const double *x_host, double *grad_host);
 
