@@ -49,6 +49,20 @@ def build_heat(F, ctx, nx0, nlev, k_sigma=1.0):
     return m, s, t_asm
 
 
+def pmc_traffic(n, nnz, key):
+    """HBM bytes per top-level SpMV launch from the committed rocprofv3 PMC summary, when it was measured on
+    this matrix and this storage format; otherwise None."""
+    from exfemus_amd import femus as F
+    path = os.path.join(ROOT, "profiles", "sell_heat64_pmc_traffic.json")
+    try:
+        d = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    if d.get("n_rows") != n or d.get("nnz") != nnz or d.get("format") != F.SPMV_FORMAT:
+        return None
+    return d.get(key, {}).get("traffic_bytes")
+
+
 def spmv_bytes(n, nnz, v):
     p = 8 if nnz > 2 ** 31 else 4
     return nnz * (v + 4) + (n + 1) * p + n * v + n * v
@@ -171,6 +185,11 @@ def main():
     gbs32 = b32 / sec32 / 1e9
     log(f"spmv fp64 {sec64 * 1e6:.1f} us {gbs64:.0f} GB/s ({gbs64 / HBM_PEAK_GBS:.3f}); "
         f"fp32 {sec32 * 1e6:.1f} us {gbs32:.0f} GB/s ({gbs32 / HBM_PEAK_GBS:.3f})")
+    # STREAM copy yardstick: 1 GiB read + 1 GiB written per launch
+    copy_bytes = 1 << 30
+    sec_copy = ctx.stream_copy(copy_bytes, reps=10)
+    gbs_copy = 2 * copy_bytes / sec_copy / 1e9
+    log(f"stream copy {gbs_copy:.0f} GB/s")
     out = {
         "metric": "MDoF/s to 1e-8 resid: FP32-MG+FP64 refine vs FP64 MG, 1 GPU; SpMV %HBM peak",
         "value": round(value, 3),
@@ -197,10 +216,12 @@ def main():
         "assembly_ms_all_levels": round(t_asm * 1e3, 2),
         "spmv": {"fp64_us": round(sec64 * 1e6, 2), "fp64_GBs": round(gbs64, 1), "fp64_frac": round(gbs64 / HBM_PEAK_GBS, 4),
                  "fp32_us": round(sec32 * 1e6, 2), "fp32_GBs": round(gbs32, 1), "fp32_frac": round(gbs32 / HBM_PEAK_GBS, 4),
-                 "bytes_alg_fp64": b64, "bytes_alg_fp32": b32},
-        "roofline": {"bound": "hbm", "achieved": round(gbs64, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs64 / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_sell<double,EP_Y> (SELL-64) on the 64^3 top matrix"},
+                 "bytes_alg_fp64": b64, "bytes_alg_fp32": b32,
+                 "stream_copy_GBs": round(gbs_copy, 1), "fp64_frac_of_stream": round(gbs64 / gbs_copy, 4)},
+        # the mixed solve's dominant kernel is the FP32 SELL SpMV of the V-cycle
+        "roofline": {"bound": "hbm", "achieved": round(gbs32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs32 / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(N, nnz, "fp32"),
+                     "bytes_alg": b32, "kernel": f"k_sell<float> y = A x ({F.SPMV_FORMAT}) on the top matrix"},
     }
     if not args.no_cpu:
         try:

@@ -538,7 +538,110 @@ __global__ __launch_bounds__(64) void k_heat_lap(int nel, const int *__restrict_
   }
 }
 
+// ===========================================================================
+// Navier-Stokes viscous functional / gradient kernel (top level):
+//   w_(iv,i) = sum_q JxW_q sum_k d_k phi_i (d_k u_iv + d_iv u_k)   ( = dA/dnu_e u )
+//   diss[e]  = sum_q JxW_q sum_{iv,k} (d_k u_iv)^2                 ( int_e |grad u|^2 )
+//   adj[e]   = sum_(iv,i) bc_vol lam_(iv,i) w_(iv,i)
+//   y[dof]  += ysc * nu_e * sum_q JxW_q d_k phi_i d_k u_iv          ( 2 nu K_lap u with ysc = 2 )
+// ===========================================================================
+template <int DIM>
+__global__ __launch_bounds__(128) void k_ns_visc(int nel, const int *__restrict__ elems, const int *__restrict__ conn,
+                                                const int *__restrict__ eldof, const double *__restrict__ xyz, int Ntop,
+                                                const int *__restrict__ dof_top, const int *__restrict__ bc0,
+                                                const double *__restrict__ U, const double *__restrict__ lam,
+                                                const double *__restrict__ coef, double nu0, const FeTab *__restrict__ tab,
+                                                double *__restrict__ diss, double *__restrict__ adj,
+                                                double *__restrict__ y, double ysc) {
+  constexpr int N2 = DIM == 2 ? 9 : 27, N1 = DIM == 2 ? 4 : 8, NG = N2, NL = DIM * N2 + N1;
+  // one thread per velocity row (DIM * N2 <= 81) and per quadrature point: launched with 128 threads
+  static_assert(DIM * N2 <= 128, "k_ns_visc needs a thread per velocity row");
+  __shared__ double s_xx[DIM * N2], s_u[DIM * N2], s_lam[DIM * N2], s_JxW[NG];
+  __shared__ double s_gu[NG * DIM * DIM];  // [qp][iv][k] = d_k u_iv
+  __shared__ double s_dg[NG * N2 * DIM];
+  __shared__ int s_bcv[DIM * N2], s_dof[DIM * N2];
+  __shared__ double s_red[2 * DIM * N2 + NG];
+  const int tid = threadIdx.x;
+  for (int ei = blockIdx.x; ei < nel; ei += gridDim.x) {
+    const int e = elems[ei];
+    __syncthreads();
+    if (tid < N2) {
+      int node = conn[(int64_t)e * N2 + tid];
+      for (int d = 0; d < DIM; d++) {
+        s_xx[tid + d * N2] = xyz[node + (int64_t)d * Ntop];
+        int kd = dof_top[node + (int64_t)d * Ntop];
+        s_u[tid + d * N2] = U[kd];
+        s_lam[tid + d * N2] = lam ? lam[kd] : 0.0;
+        s_bcv[tid + d * N2] = bc0[kd];
+        s_dof[tid + d * N2] = eldof[(int64_t)e * NL + tid + d * N2];
+      }
+    }
+    __syncthreads();
+    if (tid < NG) {
+      const int qp = tid;
+      double IJ[DIM * DIM];
+      double det = jac_qp<DIM, N2, NG>(qp, s_xx, tab->dphi2, IJ);
+      s_JxW[qp] = det * tab->w[qp];
+      double g[DIM * DIM];
+      for (int t = 0; t < DIM * DIM; t++) g[t] = 0.;
+      for (int sn = 0; sn < N2; sn++) {
+        double gr[DIM];
+        for (int d = 0; d < DIM; d++) gr[d] = tab->dphi2[sn * NG + qp + d * N2 * NG];
+        for (int k = 0; k < DIM; k++) {
+          double sum = 0.;
+          for (int j = 0; j < DIM; j++) sum += IJ[j + k * DIM] * gr[j];
+          s_dg[(qp * N2 + sn) * DIM + k] = sum;
+          for (int iv = 0; iv < DIM; iv++) g[iv * DIM + k] += sum * s_u[sn + iv * N2];
+        }
+      }
+      double q2 = 0.;
+      for (int t = 0; t < DIM * DIM; t++) { s_gu[qp * DIM * DIM + t] = g[t]; q2 += g[t] * g[t]; }
+      s_red[2 * DIM * N2 + qp] = s_JxW[qp] * q2;
+    }
+    __syncthreads();
+    const double nu = coef ? coef[e] : nu0;
+    if (tid < DIM * N2) {
+      const int iv = tid / N2, i = tid % N2;
+      double w = 0., lapu = 0.;
+      for (int qp = 0; qp < NG; qp++) {
+        double t = 0., l = 0.;
+        for (int k = 0; k < DIM; k++) {
+          const double dk = s_dg[(qp * N2 + i) * DIM + k];
+          t += dk * (s_gu[qp * DIM * DIM + iv * DIM + k] + s_gu[qp * DIM * DIM + k * DIM + iv]);
+          l += dk * s_gu[qp * DIM * DIM + iv * DIM + k];
+        }
+        w += s_JxW[qp] * t;
+        lapu += s_JxW[qp] * l;
+      }
+      s_red[tid] = s_bcv[tid] * s_lam[tid] * w;
+      if (y) y[s_dof[tid]] += ysc * nu * lapu;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double a0 = 0., a1 = 0.;
+      for (int t = 0; t < DIM * N2; t++) a1 += s_red[t];
+      for (int q = 0; q < NG; q++) a0 += s_red[2 * DIM * N2 + q];
+      if (diss) diss[e] = a0;
+      if (adj) adj[e] = a1;
+    }
+  }
+}
+
 namespace fmk {
+
+int ns_visc(hipStream_t s, int dim, int nel, const int *elems, const int *conn, const int *eldof, const double *xyz,
+            int Ntop, const int *dof_top, const int *bc0, const double *U, const double *lam, const double *coef,
+            double nu0, const FeTab *tab, double *diss, double *adj, double *y, double ysc) {
+  if (nel <= 0) return 0;
+  unsigned g = (unsigned)(nel < 8192 ? nel : 8192);
+  if (dim == 2)
+    hipLaunchKernelGGL(k_ns_visc<2>, dim3(g), dim3(128), 0, s, nel, elems, conn, eldof, xyz, Ntop, dof_top, bc0, U, lam,
+                       coef, nu0, tab, diss, adj, y, ysc);
+  else
+    hipLaunchKernelGGL(k_ns_visc<3>, dim3(g), dim3(128), 0, s, nel, elems, conn, eldof, xyz, Ntop, dof_top, bc0, U, lam,
+                       coef, nu0, tab, diss, adj, y, ysc);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
 
 int heat_lap(hipStream_t s, int dim, int nel, const int *elems, const int *conn, const int *eldof, const double *xyz,
              int Ntop, const int *dof_top, const int *bc0, const double *T, const double *lam, const double *coef,

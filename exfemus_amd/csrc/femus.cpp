@@ -275,8 +275,14 @@ int fm_sync_sell(hipStream_t st, DevCSR &c) {
   return fmk::csr2sell(st, c.sell.nslot, c.sell.map, c.v, c.sell.v, c.sell.vf);
 }
 
+void fm_drop_graphs(fm_system *s) {
+  for (auto &g : s->graphs) (void)hipGraphExecDestroy(g.exec);
+  s->graphs.clear();
+}
+
 extern "C" int fm_system_destroy(fm_system *s) {
   if (!s) return FM_EINVAL;
+  fm_drop_graphs(s);
   (void)hipSetDevice(s->ctx->device);
   (void)hipStreamSynchronize(s->ctx->stream);
   for (auto &l : s->lev) fm_free_level(l);
@@ -470,6 +476,7 @@ extern "C" int fm_system_init(fm_system *s) {
     FM_CHECK(fmk::elpos(st, l.nel, nloc, l.eldof, l.A.rp, l.A.ci, l.elpos));
     FM_HIP(hipStreamSynchronize(st));
   }
+  fm_drop_graphs(s);
   for (auto &l : s->alev) fm_free_level(l);
   s->alev.clear();
   s->a_assembled.clear();
@@ -852,10 +859,97 @@ extern "C" int fm_heat_grad_coef(fm_system *s, const fm_heat_params *p, const do
   }
   dfree(adj);
   if (rc) return FM_EHIP;
-  // elements of l.color_elems are listed colour by colour: map back to element ids
-  std::vector<int> ce(l.nel);
-  FM_HIP(hipMemcpy(ce.data(), l.color_elems, l.nel * sizeof(int), hipMemcpyDeviceToHost));
-  (void)ce;
   for (int e = 0; e < l.nel; e++) grad_host[e] = -p->alpha0 * h[e];
+  return FM_OK;
+}
+
+// STREAM copy of `bytes` (rounded down to 16 B) between two device buffers;
+// *sec = average kernel time over reps (HIP events on the context stream).
+extern "C" int fm_bench_stream(fm_ctx *c, int64_t bytes, int reps, double *sec) {
+  if (!c || bytes < 16 || reps < 1 || !sec) return FM_EINVAL;
+  FM_HIP(hipSetDevice(c->device));
+  int64_t n16 = bytes / 16;
+  void *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, n16 * 16) != hipSuccess) return FM_ENOMEM;
+  if (hipMalloc(&b, n16 * 16) != hipSuccess) { (void)hipFree(a); return FM_ENOMEM; }
+  int rc = 0;
+  if (hipMemsetAsync(a, 0, n16 * 16, c->stream) != hipSuccess) rc = FM_EHIP;
+  if (!rc) rc = fmk::copy16(c->stream, n16, a, b);
+  if (!rc && hipEventRecord(c->ev0, c->stream) != hipSuccess) rc = FM_EHIP;
+  for (int k = 0; k < reps && !rc; k++) rc = fmk::copy16(c->stream, n16, a, b);
+  if (!rc && hipEventRecord(c->ev1, c->stream) != hipSuccess) rc = FM_EHIP;
+  if (!rc && hipEventSynchronize(c->ev1) != hipSuccess) rc = FM_EHIP;
+  float ms = 0.f;
+  if (!rc && hipEventElapsedTime(&ms, c->ev0, c->ev1) != hipSuccess) rc = FM_EHIP;
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (rc) return rc < 0 ? rc : FM_EHIP;
+  *sec = ms * 1e-3 / reps;
+  return FM_OK;
+}
+
+// NS viscous dissipation J = sum_e nu_e int_e |grad u|^2 (top level), dJ/dU (DoF order; pressure rows 0),
+// partials dJ/dnu_e.
+extern "C" int fm_ns_dissipation(fm_system *s, const fm_ns_params *p, const double *U_host, double *J,
+                                 double *dJdU_host, double *dJdnu_host) {
+  FM_CHECK(check_level(s, 0));
+  if (!p || !U_host || s->nq2 != s->mesh->dim || s->nq1 != 1) return FM_EINVAL;
+  int top = (int)s->lev.size() - 1;
+  Level &l = s->lev[top];
+  hipStream_t st = s->ctx->stream;
+  const fm_mesh &m = *s->mesh;
+  FM_HIP(hipMemcpyAsync(l.t, U_host, l.n * sizeof(double), hipMemcpyHostToDevice, st));
+  FM_CHECK(fmk::fill<double>(st, l.n, l.d, 0.0));
+  double *diss = nullptr;
+  FM_CHECK(dalloc(&diss, l.nel));
+  SysExtra &ex = extra(s);
+  int rc = 0;
+  for (int k = 0; k < l.ncolor && !rc; k++) {
+    int n = l.color_off[k + 1] - l.color_off[k];
+    rc = fmk::ns_visc(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldof, s->xyz, m.Ntop, s->d_dof_top,
+                      s->d_bc0, l.t, nullptr, l.coef, p->IRe, ex.d_tab, diss, nullptr, l.d, 2.0);
+  }
+  std::vector<double> h(l.nel), hk(l.nel, p->IRe);
+  if (!rc) {
+    FM_HIP(hipMemcpyAsync(h.data(), diss, l.nel * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (l.coef) FM_HIP(hipMemcpyAsync(hk.data(), l.coef, l.nel * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (dJdU_host) FM_HIP(hipMemcpyAsync(dJdU_host, l.d, l.n * sizeof(double), hipMemcpyDeviceToHost, st));
+    FM_HIP(hipStreamSynchronize(st));
+  }
+  dfree(diss);
+  if (rc) return FM_EHIP;
+  double acc = 0.;
+  for (int e = 0; e < l.nel; e++) {
+    acc += hk[e] * h[e];
+    if (dJdnu_host) dJdnu_host[e] = h[e];
+  }
+  if (J) *J = acc;
+  return FM_OK;
+}
+
+// adjoint term of dJ/dnu_e for the Oseen step: -lambda^T (dA/dnu_e) U
+extern "C" int fm_ns_grad_coef(fm_system *s, const fm_ns_params *p, const double *lam_host, const double *U_host,
+                               double *grad_host) {
+  FM_CHECK(check_level(s, 0));
+  if (!p || !lam_host || !U_host || !grad_host || s->nq2 != s->mesh->dim || s->nq1 != 1) return FM_EINVAL;
+  int top = (int)s->lev.size() - 1;
+  Level &l = s->lev[top];
+  hipStream_t st = s->ctx->stream;
+  const fm_mesh &m = *s->mesh;
+  FM_HIP(hipMemcpyAsync(l.t, U_host, l.n * sizeof(double), hipMemcpyHostToDevice, st));
+  FM_HIP(hipMemcpyAsync(l.d2, lam_host, l.n * sizeof(double), hipMemcpyHostToDevice, st));
+  double *adj = nullptr;
+  FM_CHECK(dalloc(&adj, l.nel));
+  SysExtra &ex = extra(s);
+  int rc = fmk::ns_visc(st, m.dim, l.nel, l.color_elems, l.conn, l.eldof, s->xyz, m.Ntop, s->d_dof_top, s->d_bc0,
+                        l.t, l.d2, l.coef, p->IRe, ex.d_tab, nullptr, adj, nullptr, 0.0);
+  std::vector<double> h(l.nel);
+  if (!rc) {
+    FM_HIP(hipMemcpyAsync(h.data(), adj, l.nel * sizeof(double), hipMemcpyDeviceToHost, st));
+    FM_HIP(hipStreamSynchronize(st));
+  }
+  dfree(adj);
+  if (rc) return FM_EHIP;
+  for (int e = 0; e < l.nel; e++) grad_host[e] = -h[e];
   return FM_OK;
 }

@@ -98,6 +98,13 @@ struct fm_ctx {
   int nsys = 0;                 // live systems created on this context
 };
 
+// a captured V-cycle (one per precision / option set / forward-or-adjoint)
+struct GraphEntry {
+  int prec, adj, smoother, nu_pre, nu_post, correction, gamma, colored;
+  double lo, hi, omega;
+  hipGraphExec_t exec;
+};
+
 struct fm_system {
   fm_ctx *ctx = nullptr;
   const fm_mesh *mesh = nullptr;
@@ -116,6 +123,7 @@ struct fm_system {
   std::vector<Level> alev;
   std::vector<char> a_assembled;
   bool adj_ready = false;
+  std::vector<GraphEntry> graphs;
   int adj_smoother = -1;
   bool initialized = false;
   bool mg_ready = false;
@@ -157,6 +165,7 @@ template <> inline CsrView<float> fm_view<float>(const DevCSR &c) {
 }
 int fm_build_sell(hipStream_t st, DevCSR &c);
 int fm_alloc_level_vectors(Level &l, bool top);
+void fm_drop_graphs(fm_system *s);
 void fm_free_level(Level &l);
 int fm_sync_sell(hipStream_t st, DevCSR &c);
 void fm_free_csr(DevCSR &c);

@@ -65,6 +65,7 @@ template <typename T> int dense_mv(hipStream_t s, int n, const T *M, const T *b,
 int diag_inv(hipStream_t s, int64_t n, const int64_t *rp, const int *ci, const double *v, double *d64, float *d32);
 int cast_vals(hipStream_t s, int64_t n, const double *v, float *vf);
 int elpos(hipStream_t s, int nel, int nloc, const int *eldof, const int64_t *rp, const int *ci, int *pos);
+int copy16(hipStream_t s, int64_t n16, const void *a, void *b);
 int csr2sell(hipStream_t s, int64_t nslot, const int *map, const double *csr_v, double *sv, float *svf);
 
 // assemble.hip
@@ -76,6 +77,9 @@ int ns_assemble(hipStream_t s, int dim, int nel_color, const int *elems, const i
                 const int *elpos, const unsigned char *bmask, const double *xyz, int Ntop, const int *dof_top,
                 const int *bc0, const int *bc1, const double *x_old, const double *coef,
                 const FeTab *tab, NsArgs a, int mode, int trans, double *val, double *rhs);
+int ns_visc(hipStream_t s, int dim, int nel, const int *elems, const int *conn, const int *eldof, const double *xyz,
+            int Ntop, const int *dof_top, const int *bc0, const double *U, const double *lam, const double *coef,
+            double nu0, const FeTab *tab, double *diss, double *adj, double *y, double ysc);
 int heat_lap(hipStream_t s, int dim, int nel, const int *elems, const int *conn, const int *eldof, const double *xyz,
              int Ntop, const int *dof_top, const int *bc0, const double *T, const double *lam, const double *coef,
              const FeTab *tab, double *diss, double *adj, double *y, double ysc);

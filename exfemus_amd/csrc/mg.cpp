@@ -251,12 +251,45 @@ static int vcycle(fm_system *s, const fm_mg_opts *opt, std::vector<LOps> &ops, i
   return 0;
 }
 
+// ---- top-level cycle, replayed from a captured HIP graph -------------------
+template <typename T>
+static int cycle_top(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &ops, int adj) {
+  const int top = (int)ops.size() - 1;
+  if (!o->use_graph) return vcycle<T>(s, o, ops, top);
+  const int prec = sizeof(T) == 8 ? 64 : 32;
+  hipStream_t st = s->ctx->stream;
+  for (auto &g : s->graphs)
+    if (g.prec == prec && g.adj == adj && g.smoother == o->smoother && g.nu_pre == o->nu_pre && g.nu_post == o->nu_post &&
+        g.correction == o->correction && g.gamma == o->gamma && g.colored == o->vanka_colored && g.lo == o->cheb_lo &&
+        g.hi == o->cheb_hi && g.omega == o->omega)
+      return hipGraphLaunch(g.exec, st) == hipSuccess ? 0 : FM_EHIP;
+  hipGraph_t graph = nullptr;
+  if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return FM_EHIP;
+  int rc = vcycle<T>(s, o, ops, top);
+  hipError_t e = hipStreamEndCapture(st, &graph);
+  if (rc || e != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    fm_set_error("V-cycle capture failed");
+    return rc ? rc : FM_EHIP;
+  }
+  GraphEntry g;
+  g.prec = prec; g.adj = adj; g.smoother = o->smoother; g.nu_pre = o->nu_pre; g.nu_post = o->nu_post;
+  g.correction = o->correction; g.gamma = o->gamma; g.colored = o->vanka_colored; g.lo = o->cheb_lo; g.hi = o->cheb_hi;
+  g.omega = o->omega;
+  e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) { fm_set_error("V-cycle graph instantiate failed"); return FM_EHIP; }
+  s->graphs.push_back(g);
+  return hipGraphLaunch(g.exec, st) == hipSuccess ? 0 : FM_EHIP;
+}
+
 // ---- setup: eigen bounds, coarse inverse, Vanka --------------------------
 static int mg_setup_ops(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &ops, bool adjoint);
 
 extern "C" int fm_mg_setup(fm_system *s, const fm_mg_opts *o) {
   if (!s || !o || !s->initialized) return FM_ESTATE;
   FM_HIP(hipSetDevice(s->ctx->device));
+  fm_drop_graphs(s);
   std::vector<LOps> ops = forward_ops(s);
   FM_CHECK(mg_setup_ops(s, o, ops, false));
   for (size_t L = 0; L < s->lev.size(); L++) {
@@ -292,7 +325,7 @@ struct Timer {
 };
 
 // x[top] (FP64) := solution; b[top] FP64 rhs.  Stationary cycles / IR.
-static int solve_stationary(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &ops, fm_mg_report *rep) {
+static int solve_stationary(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &ops, fm_mg_report *rep, int adj) {
   fm_ctx *c = s->ctx;
   hipStream_t st = c->stream;
   int top = (int)ops.size() - 1;
@@ -307,12 +340,12 @@ static int solve_stationary(fm_system *s, const fm_mg_opts *o, std::vector<LOps>
   rel = dev_dot<double>(c, ot.n, l->res, l->res, true) / bn;
   while (it < o->max_iter && rel > o->tol) {
     if (o->precision == FM_PREC_FP64) {
-      FM_CHECK(vcycle<double>(s, o, ops, top));
+      FM_CHECK(cycle_top<double>(s, o, ops, adj));
     } else {
       // FP32 correction: A e = r (FP32 V-cycle from e = 0), x += e
       FM_CHECK(fmk::to_f32(st, ot.n, l->res, l->bf, nullptr));
       FM_CHECK(fmk::fill<float>(st, ot.n, l->xf, 0.f));
-      FM_CHECK(vcycle<float>(s, o, ops, top));
+      FM_CHECK(cycle_top<float>(s, o, ops, adj));
       FM_CHECK(fmk::add_f32(st, ot.n, l->x, l->xf, nullptr));
     }
     FM_CHECK(fmk::resid(st, A, l->b, l->x, l->res));
@@ -373,7 +406,7 @@ __global__ void k_multi_add(int64_t n, int k, const double *__restrict__ Z, cons
   }
 }
 
-static int solve_fgmres(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &ops, fm_mg_report *rep) {
+static int solve_fgmres(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &ops, fm_mg_report *rep, int adj) {
   fm_ctx *c = s->ctx;
   hipStream_t st = c->stream;
   int top = (int)ops.size() - 1;
@@ -421,14 +454,14 @@ static int solve_fgmres(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &op
           if (hipMemcpyAsync(l->b, vj, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
           if (hipMemcpyAsync(l->sx, l->x, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
           if ((rc = fmk::fill<double>(st, n, l->x, 0.0)) != 0) goto done;
-          if ((rc = vcycle<double>(s, o, ops, top)) != 0) goto done;
+          if ((rc = cycle_top<double>(s, o, ops, adj)) != 0) goto done;
           if (hipMemcpyAsync(zj, l->x, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
           if (hipMemcpyAsync(l->x, l->sx, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
           if (hipMemcpyAsync(l->b, l->sb, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
         } else {
           if ((rc = fmk::to_f32(st, n, vj, l->bf, nullptr)) != 0) goto done;
           if ((rc = fmk::fill<float>(st, n, l->xf, 0.f)) != 0) goto done;
-          if ((rc = vcycle<float>(s, o, ops, top)) != 0) goto done;
+          if ((rc = cycle_top<float>(s, o, ops, adj)) != 0) goto done;
           if ((rc = fmk::fill<double>(st, n, zj, 0.0)) != 0) goto done;
           if ((rc = fmk::add_f32(st, n, zj, l->xf, nullptr)) != 0) goto done;
         }
@@ -495,8 +528,8 @@ extern "C" int fm_mg_solve(fm_system *s, const fm_mg_opts *o, fm_mg_report *rep)
   std::vector<LOps> ops = forward_ops(s);
   Timer tm(s->ctx);
   int rc;
-  if (o->precision == FM_PREC_FP64 || o->precision == FM_PREC_FP32_IR) rc = solve_stationary(s, o, ops, rep);
-  else rc = solve_fgmres(s, o, ops, rep);
+  if (o->precision == FM_PREC_FP64 || o->precision == FM_PREC_FP32_IR) rc = solve_stationary(s, o, ops, rep, 0);
+  else rc = solve_fgmres(s, o, ops, rep, 0);
   rep->seconds = tm.stop();
   if (rc) return rc;
   return rep->converged ? FM_OK : FM_ENOCONV;
@@ -539,6 +572,7 @@ extern "C" int fm_adjoint_solve(fm_system *s, const fm_mg_opts *o, const double 
     op.Ainv = a.Ainv; op.Ainvf = a.Ainvf; op.v = &a; op.vinv = a.vinv;
   }
   if (!s->adj_ready || s->adj_smoother != o->smoother) {
+    fm_drop_graphs(s);
     FM_CHECK(mg_setup_ops(s, o, ops, true));
     for (size_t L = 0; L < s->alev.size(); L++) {
       s->alev[L].lam = ops[L].lam; s->alev[L].Ainv = ops[L].Ainv; s->alev[L].Ainvf = ops[L].Ainvf;
@@ -553,8 +587,8 @@ extern "C" int fm_adjoint_solve(fm_system *s, const fm_mg_opts *o, const double 
   FM_CHECK(fmk::fill<double>(st, at.n, at.x, 0.0));
   Timer tm(s->ctx);
   int rc;
-  if (o->precision == FM_PREC_FP64 || o->precision == FM_PREC_FP32_IR) rc = solve_stationary(s, o, ops, rep);
-  else rc = solve_fgmres(s, o, ops, rep);
+  if (o->precision == FM_PREC_FP64 || o->precision == FM_PREC_FP32_IR) rc = solve_stationary(s, o, ops, rep, 1);
+  else rc = solve_fgmres(s, o, ops, rep, 1);
   rep->seconds = tm.stop();
   if (rc) return rc;
   FM_HIP(hipMemcpyAsync(lam, at.x, at.n * sizeof(double), hipMemcpyDeviceToHost, st));

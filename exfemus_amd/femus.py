@@ -27,6 +27,9 @@ FM_PREC_FP64, FM_PREC_FP32_IR, FM_PREC_FP32_FGMRES, FM_PREC_FP64_FGMRES = 0, 1, 
 FM_SMOOTH_CHEBYSHEV, FM_SMOOTH_JACOBI, FM_SMOOTH_VANKA = 0, 1, 2
 FM_ENOCONV = -3
 
+# storage format of the SELL SpMV (names the layout the PMC traffic figures were measured on)
+SPMV_FORMAT = "SELL-64 sigma4096 int32 cols"
+
 # Exported entry points (the C ABI of include/femus_hip.h)
 EXPORTS = [
     "fm_ctx_create", "fm_ctx_destroy", "fm_last_error", "fm_mesh_create_box", "fm_mesh_destroy",
@@ -37,7 +40,7 @@ EXPORTS = [
     "fm_system_set_velocity", "fm_heat_genmatrhs", "fm_ns_genmatrhs", "fm_spmv", "fm_residual",
     "fm_restrict", "fm_prolong", "fm_get_res", "fm_mg_setup", "fm_mg_solve", "fm_adjoint_solve",
     "fm_heat_grad_coef", "fm_bench_spmv", "fm_sync", "fm_heat_genmat_transpose", "fm_ns_genmat_transpose",
-    "fm_get_adjoint_csr", "fm_heat_dissipation",
+    "fm_get_adjoint_csr", "fm_heat_dissipation", "fm_bench_stream", "fm_ns_dissipation", "fm_ns_grad_coef",
 ]
 
 
@@ -63,7 +66,7 @@ class MGOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("smoother", C.c_int), ("nu_pre", C.c_int), ("nu_post", C.c_int),
                 ("correction", C.c_int), ("max_iter", C.c_int), ("fgmres_restart", C.c_int),
                 ("tol", C.c_double), ("cheb_lo", C.c_double), ("cheb_hi", C.c_double),
-                ("omega", C.c_double), ("vanka_colored", C.c_int), ("gamma", C.c_int)]
+                ("omega", C.c_double), ("vanka_colored", C.c_int), ("gamma", C.c_int), ("use_graph", C.c_int)]
 
 
 class MGReport(C.Structure):
@@ -125,6 +128,9 @@ def lib():
     L.fm_get_adjoint_csr.argtypes = [vp, C.c_int, f64p]
     L.fm_heat_dissipation.argtypes = [vp, C.POINTER(HeatParams), f64p, C.POINTER(C.c_double), vp, vp]
     L.fm_bench_spmv.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]
+    L.fm_bench_stream.argtypes = [vp, C.c_int64, C.c_int, C.POINTER(C.c_double)]
+    L.fm_ns_dissipation.argtypes = [vp, C.POINTER(NSParams), f64p, C.POINTER(C.c_double), vp, vp]
+    L.fm_ns_grad_coef.argtypes = [vp, C.POINTER(NSParams), f64p, f64p, f64p]
     _LIB = L
     return L
 
@@ -145,6 +151,12 @@ class Context:
 
     def sync(self):
         _ck(lib().fm_sync(self.h))
+
+    def stream_copy(self, nbytes, reps=20):
+        """Average seconds of a device copy of nbytes (reads nbytes, writes nbytes)."""
+        sec = C.c_double()
+        _ck(lib().fm_bench_stream(self.h, int(nbytes), reps, C.byref(sec)))
+        return sec.value
 
     def close(self):
         for s in list(self._systems):
@@ -321,11 +333,11 @@ class MGSolDA:
     @staticmethod
     def opts(precision=FM_PREC_FP64, smoother=FM_SMOOTH_CHEBYSHEV, nu_pre=2, nu_post=2, correction=1,
              max_iter=100, fgmres_restart=30, tol=1e-8, cheb_lo=0.2, cheb_hi=1.05, omega=None, vanka_colored=0,
-             gamma=1):
+             gamma=1, use_graph=1):
         if omega is None:
             omega = 0.5 if smoother == FM_SMOOTH_VANKA else 0.6
         return MGOpts(precision, smoother, nu_pre, nu_post, correction, max_iter, fgmres_restart, tol,
-                      cheb_lo, cheb_hi, omega, vanka_colored, gamma)
+                      cheb_lo, cheb_hi, omega, vanka_colored, gamma, use_graph)
 
     def MGSetup(self, opts):
         _ck(lib().fm_mg_setup(self.h, C.byref(opts)))
@@ -415,6 +427,26 @@ class MGSolNS(MGSolDA):
 
     def GenMatRhs(self, time, Level, mode):
         _ck(lib().fm_ns_genmatrhs(self.h, Level, mode, C.byref(self.params)))
+
+    # -- discrete adjoint of the Oseen (Picard) system ----------------------------
+    def GenMatTranspose(self, Level):
+        _ck(lib().fm_ns_genmat_transpose(self.h, Level, C.byref(self.params)))
+
+    def dissipation(self, U):
+        """J = sum_e nu_e int_e |grad u|^2, dJ/dU, dJ/dnu_e (partial) on the top level."""
+        J = C.c_double()
+        g = np.zeros(self._Dim[-1])
+        dn = np.zeros(self.mesh.level(self._NoLevels - 1)[0])
+        _ck(lib().fm_ns_dissipation(self.h, C.byref(self.params), np.ascontiguousarray(U, np.float64), C.byref(J),
+                                    g.ctypes.data_as(C.c_void_p), dn.ctypes.data_as(C.c_void_p)))
+        return J.value, g, dn
+
+    def grad_coef(self, lam, U):
+        """-lambda^T (dA/dnu_e) U for every top element."""
+        out = np.zeros(self.mesh.level(self._NoLevels - 1)[0])
+        _ck(lib().fm_ns_grad_coef(self.h, C.byref(self.params), np.ascontiguousarray(lam, np.float64),
+                                  np.ascontiguousarray(U, np.float64), out))
+        return out
 
     def MGTimeStep(self, time, opts=None):
         top = self._NoLevels - 1

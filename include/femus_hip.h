@@ -146,6 +146,7 @@ typedef struct {
   double omega;       /* Jacobi / Vanka damping                              */
   int vanka_colored;  /* 1 = multiplicative over 2^dim element colours       */
   int gamma;          /* coarse-grid visits per level: 1 = V-cycle, 2 = W (MGStep Gamma) */
+  int use_graph;      /* 1 = capture each cycle once into a HIP graph and replay it */
 } fm_mg_opts;
 
 typedef struct {
@@ -181,6 +182,12 @@ int fm_heat_dissipation(fm_system *s, const fm_heat_params *p, const double *T_h
 /* Adjoint term of dJ/dk_e: -lambda^T (dA/dk_e) T per top element          */
 int fm_heat_grad_coef(fm_system *s, const fm_heat_params *p, const double *lambda_host,
                       const double *x_host, double *grad_host);
+/* NS viscous dissipation J = sum_e nu_e int_e |grad u|^2, dJ/dU, dJ/dnu_e  */
+int fm_ns_dissipation(fm_system *s, const fm_ns_params *p, const double *U_host, double *J,
+                      double *dJdU_host, double *dJdnu_host);
+/* Adjoint term of dJ/dnu_e for the Oseen (Picard) system: -lambda^T (dA/dnu_e) U */
+int fm_ns_grad_coef(fm_system *s, const fm_ns_params *p, const double *lambda_host,
+                    const double *U_host, double *grad_host);
 
 /* ---- timing helpers for benchmarks ------------------------------------- */
 /* Run `reps` SpMVs y = A[L] x on device buffers (prec 0 = FP64, 1 = FP32);
@@ -188,6 +195,9 @@ int fm_heat_grad_coef(fm_system *s, const fm_heat_params *p, const double *lambd
  * library stream). */
 int fm_bench_spmv(fm_system *s, int L, int prec, int reps, double *sec);
 int fm_sync(fm_ctx *ctx);
+/* STREAM copy of `bytes` between two fresh device buffers (read + write),
+ * average kernel seconds over reps: the achievable-bandwidth yardstick.    */
+int fm_bench_stream(fm_ctx *ctx, int64_t bytes, int reps, double *sec);
 
 #ifdef __cplusplus
 }

@@ -156,3 +156,72 @@ def test_ns_transposed_assembly_and_solve(femus, ctx):
     lam, rep = ns.AdjointSolve(g, ns.opts(precision=3, smoother=femus.FM_SMOOTH_VANKA, tol=1e-9, max_iter=80))
     assert rep.converged == 1, (rep.iters, rep.rel_resid)
     assert np.linalg.norm(A.T @ lam - g) / np.linalg.norm(g) < 2e-9
+
+
+def _channel(femus, ctx, nu_sigma=0.5, IRe=0.05):
+    m = femus.Mesh(3, [4, 2, 2], [0, 0, 0], [4, 1, 1], 2)
+    ns = femus.MGSolNS(ctx, m, IRe=IRe, dt=0.1, time_coeff=1.0, upwind=0.1)
+    ns.GenBc(configs.channel_bc)
+    ns.init()
+    x0 = ns.GenIc(configs.channel_ic)
+    top = 1
+    nu = configs.coarsen_coef(IRe * configs.lognormal_k(m.level(top)[0], seed=11, sigma=nu_sigma), 2, 8)
+    for L in range(2):
+        ns.set_elem_coef(L, nu[L])
+    rng = np.random.default_rng(4)
+    ns.set_xold(x0 + 0.3 * rng.standard_normal(len(x0)) * (ns.bc()[0] == 1))
+    return m, ns, nu
+
+
+def test_ns_dissipation_closed_form(femus, ctx):
+    """J(U) for a linear velocity field u = G x is sum_e nu_e vol_e |G|_F^2; dJ/dU . U = 2 J (J is quadratic)."""
+    m, ns, nu = _channel(femus, ctx)
+    G = np.array([[0.3, -1.2, 0.7], [2.0, 0.1, -0.4], [-0.6, 0.9, 1.5]])
+    U = ns.GenIc(lambda xp: list(G @ np.asarray(xp[:3])) + [0.0])
+    J, dJdU, dJdnu = ns.dissipation(U)
+    ne = m.level(1)[0]
+    vol = 4.0 / ne
+    np.testing.assert_allclose(dJdnu, vol * (G ** 2).sum(), rtol=1e-12)
+    np.testing.assert_allclose(J, (nu[1] * vol).sum() * (G ** 2).sum(), rtol=1e-12)
+    np.testing.assert_allclose(dJdU @ U, 2 * J, rtol=1e-11)
+    # dJ/dU against central differences of J (exact for a quadratic, up to rounding)
+    rng = np.random.default_rng(3)
+    W = rng.standard_normal(len(U))
+    h = 1e-3
+    fd = (ns.dissipation(U + h * W)[0] - ns.dissipation(U - h * W)[0]) / (2 * h)
+    np.testing.assert_allclose(fd, dJdU @ W, rtol=1e-9)
+
+
+def test_ns_viscosity_gradient_fd(femus, ctx):
+    """Gradient of J = sum_e nu_e int_e |grad u|^2 w.r.t. per-element viscosity through one Oseen step
+    (x_old fixed): multigrid forward + transposed Vanka multigrid adjoint vs central finite differences
+    (h = 1e-6 relative, direct solves) within 1e-5 relative."""
+    m, ns, nu = _channel(femus, ctx)
+    top = 1
+    for L in range(2):
+        ns.GenMatRhs(0.0, L, 1 if L == top else 0)
+        ns.GenMatTranspose(L)
+    opts = ns.opts(precision=3, smoother=femus.FM_SMOOTH_VANKA, tol=1e-12, max_iter=120)
+    ns.set_x(np.zeros(ns._Dim[top]))
+    rep = ns.MGSolve(opts=opts)
+    assert rep.converged == 1, (rep.iters, rep.rel_resid)
+    U = ns.get_x()
+    J, dJdU, dJdnu = ns.dissipation(U)
+    lam, rep = ns.AdjointSolve(dJdU, opts)
+    assert rep.converged == 1, (rep.iters, rep.rel_resid)
+    grad = dJdnu + ns.grad_coef(lam, U)
+
+    def J_of(nut):
+        ns.set_elem_coef(top, nut)
+        ns.GenMatRhs(0.0, top, 1)
+        Uk = spla.spsolve(_csr(ns, top).tocsc(), ns.get_rhs())
+        return ns.dissipation(Uk)[0]
+
+    ne = m.level(top)[0]
+    h = 1e-6
+    for e in [0, 5, 33, ne // 2, ne - 1]:
+        up = nu[top].copy(); up[e] *= 1 + h
+        um = nu[top].copy(); um[e] *= 1 - h
+        fd = (J_of(up) - J_of(um)) / (2 * h * nu[top][e])
+        assert abs(fd - grad[e]) <= 1e-5 * abs(grad[e]), (e, fd, grad[e])
+    ns.set_elem_coef(top, nu[top])

@@ -163,3 +163,26 @@ def test_mg_matches_oracle_cycles(femus, oracle, ctx, dim, nx, nlev):
         assert rep.iters == len(hist_o)
         np.testing.assert_allclose(hist_g, hist_o, rtol=1e-6)
         np.testing.assert_allclose(sysT.get_x(), xo, rtol=1e-7, atol=1e-12 * np.abs(xo).max())
+
+
+@pytest.mark.parametrize("prec", [0, 1, 2, 3])
+def test_graph_replay_matches_eager(femus, oracle, ctx, prec):
+    """V-cycles replayed from the captured HIP graph give bitwise the same iterates as eager launches,
+    over repeated solves with eager outer-loop work and host synchronisation between the replays."""
+    pm, om, sysT, ks = _build(femus, oracle, ctx, 3, [2, 2, 2], 3)
+    top = 2
+    sysT.set_xold(np.zeros(sysT._Dim[top]))
+    for L in range(3):
+        sysT.GenMatRhs(0.0, L, 1 if L == top else 0)
+    res = {}
+    for g in (0, 1, 1, 0, 1):
+        sysT.set_x(np.zeros(sysT._Dim[top]))
+        rep = sysT.MGSolve(opts=sysT.opts(precision=prec, tol=1e-10, max_iter=60, use_graph=g))
+        x = sysT.get_x()
+        key = (rep.iters, tuple(rep.hist[:rep.iters]))
+        res.setdefault(g, []).append((key, x))
+    ref_key, ref_x = res[0][0]
+    for g in (0, 1):
+        for key, x in res[g]:
+            assert key == ref_key
+            np.testing.assert_array_equal(x, ref_x)

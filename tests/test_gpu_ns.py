@@ -173,3 +173,17 @@ def test_ns_picard_steps_match_oracle(femus, oracle, ctx):
         assert ev < 1e-8 and ep < 1e-6, (step, ev, ep)
         # continue both from the oracle's iterate (pressure constant of the GPU one is arbitrary)
         xo = xo_new
+
+
+def test_ns_graph_replay_matches_eager(femus, oracle, ctx):
+    pm, om, ns, bc, x0 = _setup(femus, oracle, ctx, "cavity2d", 2, [2, 2], 3)
+    for L in range(3):
+        ns.GenMatRhs(0.0, L, 1 if L == 2 else 0)
+    out = []
+    for g in (0, 1, 1):
+        ns.set_x(x0)
+        rep = ns.MGSolve(opts=ns.opts(precision=2, smoother=femus.FM_SMOOTH_VANKA, tol=1e-9, max_iter=60, use_graph=g))
+        out.append((rep.iters, tuple(rep.hist[:rep.iters]), ns.get_x()))
+    for it, h, x in out[1:]:
+        assert (it, h) == out[0][:2]
+        np.testing.assert_array_equal(x, out[0][2])
