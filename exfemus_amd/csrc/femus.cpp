@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -202,31 +203,39 @@ extern "C" int fm_system_create(fm_ctx *ctx, const fm_mesh *m, int nq2, int nq1,
   return FM_OK;
 }
 
+static void free_sell(DevSELL &q) {
+  dfree(q.sl_off); dfree(q.sl_c32); dfree(q.sl_w); dfree(q.perm); dfree(q.c16); dfree(q.base); dfree(q.c32);
+  dfree(q.map); dfree(q.v); dfree(q.vf);
+  q.nsl = 0; q.nslot = 0; q.n32 = 0;
+}
+
 void fm_free_csr(DevCSR &c) {
   dfree(c.rp); dfree(c.ci); dfree(c.v); dfree(c.vf);
-  DevSELL &q = c.sell;
-  dfree(q.sl_off); dfree(q.sl_w); dfree(q.perm); dfree(q.cols); dfree(q.map); dfree(q.v); dfree(q.vf);
-  q.nsl = 0; q.nslot = 0;
+  free_sell(c.sell);
 }
 static void free_csr(DevCSR &c) { fm_free_csr(c); }
 
-// SELL-64 copy of a CSR pattern (sigma window 4096 rows, rows sorted by
-// decreasing length inside each window; padding repeats the row's last column).
-int fm_build_sell(hipStream_t st, DevCSR &c) {
+// SELL-64 copy of a CSR pattern in internal DoF order: internal row i holds CSR
+// row ro[i], CSR column j becomes internal column cr[j]; sigma window 4096 rows
+// (rows sorted by decreasing length inside each window), entries of a row
+// sorted by internal column, 16-bit column offsets from a per-position base
+// with a 32-bit fallback per slice.
+int fm_build_sell(hipStream_t st, DevCSR &c, const int *ro, const int *cr) {
   const int C = 64, W = 4096;
-  int64_t n = c.nr;
+  const int64_t n = c.nr;
   const std::vector<int64_t> &rp = c.h_rp;
   const std::vector<int> &ci = c.h_ci;
   if ((int64_t)rp.size() != n + 1) { fm_set_error("SELL build without host pattern"); return FM_ESTATE; }
+  auto crow = [&](int64_t i) -> int64_t { return ro ? ro[i] : i; };
+  auto rlen = [&](int64_t i) -> int { int64_t r = crow(i); return (int)(rp[r + 1] - rp[r]); };
   std::vector<int> order(n);
   for (int64_t i = 0; i < n; i++) order[i] = (int)i;
   for (int64_t w0 = 0; w0 < n; w0 += W) {
     int64_t w1 = std::min<int64_t>(n, w0 + W);
-    std::stable_sort(order.begin() + w0, order.begin() + w1, [&](int a, int b) {
-      return rp[a + 1] - rp[a] > rp[b + 1] - rp[b];
-    });
+    std::stable_sort(order.begin() + w0, order.begin() + w1, [&](int a, int b) { return rlen(a) > rlen(b); });
   }
-  int64_t nsl = (n + C - 1) / C;
+  const int64_t nsl = (n + C - 1) / C;
+  if (nsl >= ((int64_t)1 << 31) / C) { fm_set_error("SELL: too many rows"); return FM_EINVAL; }
   std::vector<int64_t> off(nsl + 1, 0);
   std::vector<int> wd(nsl), perm((size_t)nsl * C, -1);
   for (int64_t s = 0; s < nsl; s++) {
@@ -234,36 +243,77 @@ int fm_build_sell(hipStream_t st, DevCSR &c) {
     for (int l = 0; l < C; l++) {
       int64_t p = s * C + l;
       if (p >= n) break;
-      int r = order[p];
-      perm[p] = r;
-      w = std::max<int>(w, (int)(rp[r + 1] - rp[r]));
+      perm[p] = order[p];
+      w = std::max(w, rlen(order[p]));
     }
     wd[s] = w;
     off[s + 1] = off[s] + (int64_t)w * C;
   }
-  int64_t nslot = off[nsl];
-  std::vector<int> cols(nslot, 0), map(nslot, -1);
+  const int64_t nslot = off[nsl];
+  std::vector<int> absc(nslot, -1), map(nslot, -1), base(nslot / C, 0);
+  std::vector<char> wide(nsl, 0);
+#pragma omp parallel
+  {
+    std::vector<std::pair<int, int>> ent;
+#pragma omp for schedule(dynamic, 64)
+    for (int64_t s = 0; s < nsl; s++) {
+      for (int l = 0; l < C; l++) {
+        int i = perm[s * C + l];
+        if (i < 0) continue;
+        int64_t r = crow(i);
+        ent.clear();
+        for (int64_t q = rp[r]; q < rp[r + 1]; q++) ent.emplace_back(cr ? cr[ci[q]] : ci[q], (int)q);
+        std::sort(ent.begin(), ent.end());
+        for (size_t k = 0; k < ent.size(); k++) {
+          int64_t slot = off[s] + (int64_t)k * C + l;
+          absc[slot] = ent[k].first;
+          map[slot] = ent[k].second;
+        }
+      }
+      for (int k = 0; k < wd[s]; k++) {
+        int mn = INT32_MAX, mx = -1;
+        for (int l = 0; l < C; l++) {
+          int cc = absc[off[s] + (int64_t)k * C + l];
+          if (cc < 0) continue;
+          mn = std::min(mn, cc);
+          mx = std::max(mx, cc);
+        }
+        base[off[s] / C + k] = mn;
+        if (mx - mn > 65535) wide[s] = 1;
+      }
+    }
+  }
+  std::vector<int64_t> o32(nsl, -1);
+  int64_t n32 = 0;
+  for (int64_t s = 0; s < nsl; s++)
+    if (wide[s]) { o32[s] = n32; n32 += (int64_t)wd[s] * C; }
+  std::vector<unsigned short> c16(nslot, 0);
+  std::vector<int> c32(std::max<int64_t>(n32, 1), 0);
 #pragma omp parallel for schedule(static)
   for (int64_t s = 0; s < nsl; s++)
-    for (int l = 0; l < C; l++) {
-      int r = perm[s * C + l];
-      int len = r >= 0 ? (int)(rp[r + 1] - rp[r]) : 0;
-      int lastc = len > 0 ? ci[rp[r] + len - 1] : 0;
-      for (int k = 0; k < wd[s]; k++) {
+    for (int k = 0; k < wd[s]; k++) {
+      const int b = base[off[s] / C + k];
+      for (int l = 0; l < C; l++) {
         int64_t slot = off[s] + (int64_t)k * C + l;
-        if (k < len) { cols[slot] = ci[rp[r] + k]; map[slot] = (int)(rp[r] + k); }
-        else cols[slot] = lastc;
+        const int cc = absc[slot] >= 0 ? absc[slot] : b;  // padding: the position's base column, value 0
+        if (wide[s]) c32[o32[s] + (int64_t)k * C + l] = cc;
+        else c16[slot] = (unsigned short)(cc - b);
       }
     }
   DevSELL &q = c.sell;
-  dfree(q.sl_off); dfree(q.sl_w); dfree(q.perm); dfree(q.cols); dfree(q.map); dfree(q.v); dfree(q.vf);
-  q.nsl = nsl; q.nslot = nslot;
-  FM_CHECK(dalloc(&q.sl_off, nsl + 1)); FM_CHECK(dalloc(&q.sl_w, nsl)); FM_CHECK(dalloc(&q.perm, nsl * C));
-  FM_CHECK(dalloc(&q.cols, nslot)); FM_CHECK(dalloc(&q.map, nslot)); FM_CHECK(dalloc(&q.v, nslot)); FM_CHECK(dalloc(&q.vf, nslot));
+  free_sell(q);
+  q.nsl = nsl; q.nslot = nslot; q.n32 = n32;
+  FM_CHECK(dalloc(&q.sl_off, nsl + 1)); FM_CHECK(dalloc(&q.sl_c32, nsl)); FM_CHECK(dalloc(&q.sl_w, nsl));
+  FM_CHECK(dalloc(&q.perm, nsl * C)); FM_CHECK(dalloc(&q.c16, nslot)); FM_CHECK(dalloc(&q.base, nslot / C));
+  FM_CHECK(dalloc(&q.c32, c32.size())); FM_CHECK(dalloc(&q.map, nslot)); FM_CHECK(dalloc(&q.v, nslot));
+  FM_CHECK(dalloc(&q.vf, nslot));
   FM_HIP(hipMemcpyAsync(q.sl_off, off.data(), (nsl + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  FM_HIP(hipMemcpyAsync(q.sl_c32, o32.data(), nsl * sizeof(int64_t), hipMemcpyHostToDevice, st));
   FM_HIP(hipMemcpyAsync(q.sl_w, wd.data(), nsl * sizeof(int), hipMemcpyHostToDevice, st));
   FM_HIP(hipMemcpyAsync(q.perm, perm.data(), nsl * C * sizeof(int), hipMemcpyHostToDevice, st));
-  FM_HIP(hipMemcpyAsync(q.cols, cols.data(), nslot * sizeof(int), hipMemcpyHostToDevice, st));
+  FM_HIP(hipMemcpyAsync(q.c16, c16.data(), nslot * sizeof(unsigned short), hipMemcpyHostToDevice, st));
+  FM_HIP(hipMemcpyAsync(q.base, base.data(), (nslot / C) * sizeof(int), hipMemcpyHostToDevice, st));
+  FM_HIP(hipMemcpyAsync(q.c32, c32.data(), c32.size() * sizeof(int), hipMemcpyHostToDevice, st));
   FM_HIP(hipMemcpyAsync(q.map, map.data(), nslot * sizeof(int), hipMemcpyHostToDevice, st));
   FM_HIP(hipStreamSynchronize(st));
   return fm_sync_sell(st, c);
@@ -348,7 +398,7 @@ extern "C" int fm_system_get_bc(const fm_system *s, int *bc0, int *bc1) {
 }
 
 static int upload_csr(hipStream_t st, DevCSR &c, int64_t nr, int64_t nc, std::vector<int64_t> &rp, std::vector<int> &ci,
-                      std::vector<double> *v) {
+                      std::vector<double> *v, const std::vector<int> &ro, const std::vector<int> &cr) {
   free_csr(c);
   c.nr = nr; c.nc = nc; c.nnz = rp[nr];
   FM_CHECK(dalloc(&c.rp, nr + 1));
@@ -368,7 +418,7 @@ static int upload_csr(hipStream_t st, DevCSR &c, int64_t nr, int64_t nc, std::ve
   c.h_rp.swap(rp);
   c.h_ci.swap(ci);
   if (v) c.h_v.swap(*v);
-  return fm_build_sell(st, c);
+  return fm_build_sell(st, c, ro.data(), cr.data());
 }
 
 template <typename T> static inline CsrView<T> view(const DevCSR &c) { return fm_view<T>(c); }
@@ -390,9 +440,58 @@ void fm_free_level(Level &l) {
   free_csr(l.A); free_csr(l.R); free_csr(l.P);
   dfree(l.x); dfree(l.b); dfree(l.r); dfree(l.t); dfree(l.d); dfree(l.d2); dfree(l.dinv);
   dfree(l.xf); dfree(l.bf); dfree(l.rf); dfree(l.tf); dfree(l.df); dfree(l.d2f); dfree(l.dinvf);
-  dfree(l.conn); dfree(l.eldof); dfree(l.elpos); dfree(l.bmask); dfree(l.color_elems); dfree(l.coef);
+  dfree(l.conn); dfree(l.eldof); dfree(l.eldofi); dfree(l.elpos); dfree(l.bmask); dfree(l.color_elems); dfree(l.coef);
+  dfree(l.ord); dfree(l.rank);
   dfree(l.Ainv); dfree(l.Ainvf); dfree(l.vinv); dfree(l.vwt); dfree(l.vwtf); dfree(l.nrm); dfree(l.res);
   dfree(l.sb); dfree(l.sx); dfree(l.vone);
+}
+
+// Internal DoF order of level L: the level's nodes in lexicographic order of
+// their grid coordinates (the axis with the most nodes slowest, so a row's
+// neighbours lie within ~2 node planes of the smallest cross-section), all
+// variables of a node adjacent (Q2 variables, then the Q1 variable where the
+// node carries one).  This keeps the columns of a SELL slice inside few
+// narrow windows, so the gather of x hits cache and the column offsets fit
+// 16 bits.
+static int level_order(const fm_mesh &m, int L, int nv, const std::vector<int> &dofL, int64_t ndof,
+                       std::vector<int> &ord, std::vector<int> &rank) {
+  const int N = m.Ntop, dim = m.dim;
+  std::vector<int> nodes;
+  nodes.reserve(m.nq2[L]);
+  for (int k = 0; k < N; k++) if (dofL[k] >= 0) nodes.push_back(k);
+  int axes[3] = {0, 1, 2};
+  int64_t cnt[3];
+  double h[3];
+  for (int d = 0; d < dim; d++) {
+    cnt[d] = (int64_t)m.nx0[d] << (m.nlev - 1);
+    h[d] = (m.hi[d] - m.lo[d]) / (2.0 * (double)cnt[d]);
+  }
+  std::stable_sort(axes, axes + dim, [&](int a, int b) { return cnt[a] > cnt[b] || (cnt[a] == cnt[b] && a > b); });
+  std::vector<int64_t> key(N, 0);
+  for (int k : nodes) {
+    int64_t kk = 0;
+    for (int t = 0; t < dim; t++) {
+      int d = axes[t];
+      int64_t q = std::llround((m.xyz[k + (size_t)d * N] - m.lo[d]) / h[d]);
+      kk = kk * (2 * cnt[d] + 1) + q;
+    }
+    key[k] = kk;
+  }
+  std::stable_sort(nodes.begin(), nodes.end(), [&](int a, int b) { return key[a] < key[b]; });
+  ord.clear();
+  ord.reserve(ndof);
+  for (int k : nodes)
+    for (int iv = 0; iv < nv; iv++) {
+      int d = dofL[k + (size_t)iv * N];
+      if (d >= 0) ord.push_back(d);
+    }
+  if ((int64_t)ord.size() != ndof) { fm_set_error("level order does not cover the DoFs"); return FM_ESTATE; }
+  rank.assign(ndof, -1);
+  for (int64_t i = 0; i < ndof; i++) {
+    if (ord[i] < 0 || ord[i] >= ndof || rank[ord[i]] >= 0) { fm_set_error("level order is not a permutation"); return FM_ESTATE; }
+    rank[ord[i]] = (int)i;
+  }
+  return 0;
 }
 
 extern "C" int fm_system_init(fm_system *s) {
@@ -425,22 +524,30 @@ extern "C" int fm_system_init(fm_system *s) {
     Level &l = s->lev[L];
     l.n = s->Dim[L];
     l.nel = m.nel[L];
+    FM_CHECK(level_order(m, L, nv, s->node_dof[L], l.n, l.h_ord, l.h_rank));
+    dfree(l.ord); dfree(l.rank);
+    FM_CHECK(dalloc(&l.ord, l.n)); FM_CHECK(dalloc(&l.rank, l.n));
+    FM_HIP(hipMemcpyAsync(l.ord, l.h_ord.data(), l.n * sizeof(int), hipMemcpyHostToDevice, st));
+    FM_HIP(hipMemcpyAsync(l.rank, l.h_rank.data(), l.n * sizeof(int), hipMemcpyHostToDevice, st));
+  }
+  for (int L = 0; L < nlev; L++) {
+    Level &l = s->lev[L];
     // pattern
     std::vector<int64_t> rp;
     std::vector<int> ci;
     fm_pattern(m, L, s->nq2, s->nq1, s->node_dof[L], rp, ci);
     if (rp[l.n] >= ((int64_t)1 << 31)) { fm_set_error("nnz exceeds int32 element map range"); return FM_EINVAL; }
-    FM_CHECK(upload_csr(st, l.A, l.n, l.n, rp, ci, nullptr));
+    FM_CHECK(upload_csr(st, l.A, l.n, l.n, rp, ci, nullptr, l.h_ord, l.h_rank));
     // transfers
     if (L < top) {
       std::vector<int64_t> trp; std::vector<int> tci; std::vector<double> tv;
       fm_transfer(m, L, s->nq2, s->nq1, true, s->node_dof[L], s->node_dof[L + 1], &s->node_dof[top], &s->bc0, trp, tci, tv);
-      FM_CHECK(upload_csr(st, l.R, l.n, s->Dim[L + 1], trp, tci, &tv));
+      FM_CHECK(upload_csr(st, l.R, l.n, s->Dim[L + 1], trp, tci, &tv, l.h_ord, s->lev[L + 1].h_rank));
     }
     if (L > 0) {
       std::vector<int64_t> trp; std::vector<int> tci; std::vector<double> tv;
       fm_transfer(m, L, s->nq2, s->nq1, false, s->node_dof[L], s->node_dof[L - 1], nullptr, nullptr, trp, tci, tv);
-      FM_CHECK(upload_csr(st, l.P, l.n, s->Dim[L - 1], trp, tci, &tv));
+      FM_CHECK(upload_csr(st, l.P, l.n, s->Dim[L - 1], trp, tci, &tv, l.h_ord, s->lev[L - 1].h_rank));
     }
     // vectors
     FM_CHECK(fm_alloc_level_vectors(l, L == top));
@@ -463,14 +570,18 @@ extern "C" int fm_system_init(fm_system *s) {
     for (int k = 0; k < l.ncolor; k++) l.color_off[k + 1] += l.color_off[k];
     std::vector<int> ce(l.nel), fillp(l.color_off.begin(), l.color_off.end() - 1);
     for (int e = 0; e < l.nel; e++) ce[fillp[col[e]]++] = e;
-    dfree(l.conn); dfree(l.eldof); dfree(l.elpos); dfree(l.bmask); dfree(l.color_elems);
+    std::vector<int> eldi(eld.size());
+    for (size_t q = 0; q < eld.size(); q++) eldi[q] = eld[q] >= 0 ? l.h_rank[eld[q]] : -1;
+    dfree(l.conn); dfree(l.eldof); dfree(l.eldofi); dfree(l.elpos); dfree(l.bmask); dfree(l.color_elems);
     FM_CHECK(dalloc(&l.conn, (size_t)l.nel * m.n2));
     FM_CHECK(dalloc(&l.eldof, (size_t)l.nel * nloc));
+    FM_CHECK(dalloc(&l.eldofi, (size_t)l.nel * nloc));
     FM_CHECK(dalloc(&l.elpos, (size_t)l.nel * nloc * nloc));
     FM_CHECK(dalloc(&l.bmask, l.nel));
     FM_CHECK(dalloc(&l.color_elems, l.nel));
     FM_HIP(hipMemcpyAsync(l.conn, m.conn[L].data(), m.conn[L].size() * sizeof(int), hipMemcpyHostToDevice, st));
     FM_HIP(hipMemcpyAsync(l.eldof, eld.data(), eld.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    FM_HIP(hipMemcpyAsync(l.eldofi, eldi.data(), eldi.size() * sizeof(int), hipMemcpyHostToDevice, st));
     FM_HIP(hipMemcpyAsync(l.bmask, bm.data(), bm.size(), hipMemcpyHostToDevice, st));
     FM_HIP(hipMemcpyAsync(l.color_elems, ce.data(), ce.size() * sizeof(int), hipMemcpyHostToDevice, st));
     FM_CHECK(fmk::elpos(st, l.nel, nloc, l.eldof, l.A.rp, l.A.ci, l.elpos));
@@ -517,30 +628,48 @@ extern "C" int fm_system_set_xold(fm_system *s, const double *x) {
   FM_HIP(hipStreamSynchronize(s->ctx->stream));
   return FM_OK;
 }
-static int copy_vec(fm_system *s, int L, double *dst_dev_or_host, const double *src, bool to_host) {
-  FM_HIP(hipMemcpyAsync(dst_dev_or_host, src, s->lev[L].n * sizeof(double), to_host ? hipMemcpyDeviceToHost : hipMemcpyHostToDevice, s->ctx->stream));
-  FM_HIP(hipStreamSynchronize(s->ctx->stream));
+// Level vectors live in internal DoF order; the ABI exchanges them in the
+// reference numbering: out[dof] = v[rank[dof]], v[i] = in[ord[i]].
+static int vec_to_host(const fm_system *s, int L, const double *v, double *host) {
+  const Level &l = s->lev[L];
+  hipStream_t st = s->ctx->stream;
+  FM_CHECK(fmk::gather(st, l.n, l.rank, v, l.t));
+  FM_HIP(hipMemcpyAsync(host, l.t, l.n * sizeof(double), hipMemcpyDeviceToHost, st));
+  FM_HIP(hipStreamSynchronize(st));
+  return 0;
+}
+static int vec_from_host(fm_system *s, int L, const double *host, double *v) {
+  Level &l = s->lev[L];
+  hipStream_t st = s->ctx->stream;
+  FM_HIP(hipMemcpyAsync(l.t, host, l.n * sizeof(double), hipMemcpyHostToDevice, st));
+  FM_CHECK(fmk::gather(st, l.n, l.ord, l.t, v));
+  FM_HIP(hipStreamSynchronize(st));
   return 0;
 }
 extern "C" int fm_system_get_x(const fm_system *s, int L, double *x) {
   FM_CHECK(check_level(s, L));
-  return copy_vec(const_cast<fm_system *>(s), L, x, s->lev[L].x, true);
+  if (!x) return FM_EINVAL;
+  return vec_to_host(s, L, s->lev[L].x, x);
 }
 extern "C" int fm_system_set_x(fm_system *s, int L, const double *x) {
   FM_CHECK(check_level(s, L));
-  return copy_vec(s, L, s->lev[L].x, x, false);
+  if (!x) return FM_EINVAL;
+  return vec_from_host(s, L, x, s->lev[L].x);
 }
 extern "C" int fm_system_get_rhs(const fm_system *s, int L, double *b) {
   FM_CHECK(check_level(s, L));
-  return copy_vec(const_cast<fm_system *>(s), L, b, s->lev[L].b, true);
+  if (!b) return FM_EINVAL;
+  return vec_to_host(s, L, s->lev[L].b, b);
 }
 extern "C" int fm_system_set_rhs(fm_system *s, int L, const double *b) {
   FM_CHECK(check_level(s, L));
-  return copy_vec(s, L, s->lev[L].b, b, false);
+  if (!b) return FM_EINVAL;
+  return vec_from_host(s, L, b, s->lev[L].b);
 }
 extern "C" int fm_get_res(const fm_system *s, int L, double *r) {
   FM_CHECK(check_level(s, L));
-  return copy_vec(const_cast<fm_system *>(s), L, r, s->lev[L].res, true);
+  if (!r) return FM_EINVAL;
+  return vec_to_host(s, L, s->lev[L].res, r);
 }
 
 extern "C" int fm_system_set_elem_coef(fm_system *s, int L, const double *coef) {
@@ -568,7 +697,7 @@ static int after_assembly(fm_system *s, int L) {
   hipStream_t st = s->ctx->stream;
   FM_CHECK(fmk::cast_vals(st, l.A.nnz, l.A.v, l.A.vf));
   FM_CHECK(fm_sync_sell(st, l.A));
-  FM_CHECK(fmk::diag_inv(st, l.n, l.A.rp, l.A.ci, l.A.v, l.dinv, l.dinvf));
+  FM_CHECK(fmk::diag_inv(st, l.n, l.A.rp, l.A.ci, l.A.v, l.ord, l.dinv, l.dinvf));
   l.lam = 0.;
   s->mg_ready = false;
   return 0;
@@ -589,7 +718,7 @@ extern "C" int fm_heat_genmatrhs(fm_system *s, int L, int mode, const fm_heat_pa
   SysExtra &ex = extra(s);
   for (int k = 0; k < l.ncolor; k++) {
     int n = l.color_off[k + 1] - l.color_off[k];
-    FM_CHECK(fmk::heat_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldof, l.elpos, l.bmask, s->xyz,
+    FM_CHECK(fmk::heat_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldofi, l.elpos, l.bmask, s->xyz,
                                 m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, l.coef, s->vel, ex.d_tab, a, mode,
                                 0, l.A.v, l.b));
   }
@@ -613,7 +742,7 @@ extern "C" int fm_ns_genmatrhs(fm_system *s, int L, int mode, const fm_ns_params
   SysExtra &ex = extra(s);
   for (int k = 0; k < l.ncolor; k++) {
     int n = l.color_off[k + 1] - l.color_off[k];
-    FM_CHECK(fmk::ns_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldof, l.elpos, l.bmask, s->xyz,
+    FM_CHECK(fmk::ns_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldofi, l.elpos, l.bmask, s->xyz,
                               m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, l.coef, ex.d_tab, a, mode, 0, l.A.v, l.b));
   }
   FM_CHECK(after_assembly(s, L));
@@ -629,7 +758,9 @@ extern "C" int fm_spmv(fm_system *s, int L, const double *xh, double *yh) {
   Level &l = s->lev[L];
   hipStream_t st = s->ctx->stream;
   FM_HIP(hipMemcpyAsync(l.t, xh, l.n * sizeof(double), hipMemcpyHostToDevice, st));
-  FM_CHECK(fmk::spmv(st, view<double>(l.A), l.t, l.d));
+  CsrView<double> A = view<double>(l.A);
+  A.nsl = 0;  // CSR copy: reference numbering on both sides
+  FM_CHECK(fmk::spmv(st, A, l.t, l.d));
   FM_HIP(hipMemcpyAsync(yh, l.d, l.n * sizeof(double), hipMemcpyDeviceToHost, st));
   FM_HIP(hipStreamSynchronize(st));
   return FM_OK;
@@ -711,20 +842,25 @@ static int ensure_adjoint(fm_system *s) {
     Level &a = s->alev[L], &l = s->lev[L];
     a.n = l.n;
     a.nel = l.nel;
+    a.h_ord = l.h_ord;
+    a.h_rank = l.h_rank;
+    FM_CHECK(dalloc(&a.ord, a.n)); FM_CHECK(dalloc(&a.rank, a.n));
+    FM_HIP(hipMemcpyAsync(a.ord, l.h_ord.data(), a.n * sizeof(int), hipMemcpyHostToDevice, st));
+    FM_HIP(hipMemcpyAsync(a.rank, l.h_rank.data(), a.n * sizeof(int), hipMemcpyHostToDevice, st));
     std::vector<int64_t> rp = l.A.h_rp;
     std::vector<int> ci = l.A.h_ci;
-    FM_CHECK(upload_csr(st, a.A, l.n, l.n, rp, ci, nullptr));
+    FM_CHECK(upload_csr(st, a.A, l.n, l.n, rp, ci, nullptr, l.h_ord, l.h_rank));
     if (L < top) {  // restriction of the adjoint = (prolongation of level L+1)^T
       const DevCSR &P = s->lev[L + 1].P;
       std::vector<int64_t> rpT; std::vector<int> ciT; std::vector<double> vT;
       transpose_host(P.nr, P.nc, P.h_rp, P.h_ci, P.h_v, rpT, ciT, vT);
-      FM_CHECK(upload_csr(st, a.R, P.nc, P.nr, rpT, ciT, &vT));
+      FM_CHECK(upload_csr(st, a.R, P.nc, P.nr, rpT, ciT, &vT, l.h_ord, s->lev[L + 1].h_rank));
     }
     if (L > 0) {  // prolongation of the adjoint = (restriction of level L-1)^T
       const DevCSR &R = s->lev[L - 1].R;
       std::vector<int64_t> rpT; std::vector<int> ciT; std::vector<double> vT;
       transpose_host(R.nr, R.nc, R.h_rp, R.h_ci, R.h_v, rpT, ciT, vT);
-      FM_CHECK(upload_csr(st, a.P, R.nc, R.nr, rpT, ciT, &vT));
+      FM_CHECK(upload_csr(st, a.P, R.nc, R.nr, rpT, ciT, &vT, l.h_ord, s->lev[L - 1].h_rank));
     }
     FM_CHECK(fm_alloc_level_vectors(a, L == top));
     FM_HIP(hipMemsetAsync(a.x, 0, a.n * sizeof(double), st));
@@ -738,7 +874,7 @@ static int after_assembly_T(fm_system *s, int L) {
   hipStream_t st = s->ctx->stream;
   FM_CHECK(fmk::cast_vals(st, a.A.nnz, a.A.v, a.A.vf));
   FM_CHECK(fm_sync_sell(st, a.A));
-  FM_CHECK(fmk::diag_inv(st, a.n, a.A.rp, a.A.ci, a.A.v, a.dinv, a.dinvf));
+  FM_CHECK(fmk::diag_inv(st, a.n, a.A.rp, a.A.ci, a.A.v, a.ord, a.dinv, a.dinvf));
   a.lam = 0.;
   s->a_assembled[L] = 1;
   s->adj_ready = false;

@@ -27,15 +27,22 @@ struct fm_mesh {
   std::vector<std::vector<int>> conn, neigh, parent, egrid;  // egrid: element grid index (3 ints)
 };
 
-// Sliced ELLPACK copy (SELL-C-sigma, C = 64 = one wavefront): rows sorted by
-// length inside windows of 4096, slice s holds rows perm[64s .. 64s+63] with
-// width sl_w[s]; entry k of lane l sits at sl_off[s] + 64k + l (column-major
-// inside the slice, so a wavefront's loads are contiguous).  map[slot] is
-// the CSR position of the entry (-1 for padding).
+// Sliced ELLPACK copy (SELL-C-sigma, C = 64 = one wavefront) in the level's
+// internal DoF order (see Level::ord): rows sorted by length inside windows of
+// 4096, slice s holds internal rows perm[64s .. 64s+63] with width sl_w[s];
+// entry k of lane l sits at slot sl_off[s] + 64k + l (column-major inside the
+// slice, so a wavefront's loads are contiguous).  Inside each row the entries
+// are sorted by internal column.  Columns are stored as 16-bit offsets from a
+// per-slot-column base, base[sl_off[s]/64 + k] (the smallest column among the
+// 64 lanes at position k); a slice in which some position spans 65536 columns
+// or more keeps absolute 32-bit columns instead, at c32[sl_c32[s] + 64k + l]
+// (sl_c32[s] = -1 for 16-bit slices).  map[slot] is the CSR position of the
+// entry (-1 for padding; padding repeats the slot's base column with value 0).
 struct DevSELL {
-  int64_t nsl = 0, nslot = 0;
-  int64_t *sl_off = nullptr;
-  int *sl_w = nullptr, *perm = nullptr, *cols = nullptr, *map = nullptr;
+  int64_t nsl = 0, nslot = 0, n32 = 0;
+  int64_t *sl_off = nullptr, *sl_c32 = nullptr;
+  int *sl_w = nullptr, *perm = nullptr, *base = nullptr, *c32 = nullptr, *map = nullptr;
+  unsigned short *c16 = nullptr;
   double *v = nullptr;
   float *vf = nullptr;
 };
@@ -57,6 +64,11 @@ struct DevCSR {
 struct Level {
   int64_t n = 0;           // _Dim[L]
   int nel = 0;
+  // internal DoF order of the level's multigrid vectors and SELL operators:
+  // ord[i] = reference DoF (MGSolDA::_node_dof numbering) at internal position i,
+  // rank[dof] = internal position of a reference DoF
+  std::vector<int> h_ord, h_rank;
+  int *ord = nullptr, *rank = nullptr;
   DevCSR A, R, P;          // A[L]; R: coarse L <- fine L+1; P: fine L <- coarse L-1
   // FP64 vectors
   double *x = nullptr, *b = nullptr, *r = nullptr, *t = nullptr, *d = nullptr, *d2 = nullptr, *dinv = nullptr;
@@ -65,7 +77,8 @@ struct Level {
   double lam = 0.0;        // lambda_max(D^-1 A) estimate
   // assembly support
   int *conn = nullptr;     // nel * n2 node ids
-  int *eldof = nullptr;    // nel * nloc dofs
+  int *eldof = nullptr;    // nel * nloc dofs (reference numbering)
+  int *eldofi = nullptr;   // nel * nloc dofs (internal numbering)
   int *elpos = nullptr;    // nel * nloc * nloc CSR positions in A
   unsigned char *bmask = nullptr;  // nel: bit s set if side s on the boundary
   int ncolor = 0;
@@ -146,12 +159,16 @@ extern const int fm_face3d[6][9];
 
 // views of a device matrix for the kernel launchers (SELL copy when present)
 #include "fm_kernels.h"
+template <typename T> inline void fm_sell_fields(CsrView<T> &v, const DevSELL &q) {
+  v.nsl = q.nsl; v.sl_off = q.sl_off; v.sl_w = q.sl_w; v.perm = q.perm; v.c16 = q.c16; v.base = q.base;
+  v.sl_c32 = q.sl_c32; v.c32 = q.c32;
+}
 template <typename T> CsrView<T> fm_view(const DevCSR &c);
 template <> inline CsrView<double> fm_view<double>(const DevCSR &c) {
   CsrView<double> v;
   v.nr = c.nr; v.rp = c.rp; v.ci = c.ci; v.v = c.v;
   v.lpr = fmk::lanes_for(c.nnz, c.nr);
-  v.nsl = c.sell.nsl; v.sl_off = c.sell.sl_off; v.sl_w = c.sell.sl_w; v.perm = c.sell.perm; v.scols = c.sell.cols;
+  fm_sell_fields(v, c.sell);
   v.sv = c.sell.v;
   return v;
 }
@@ -159,11 +176,12 @@ template <> inline CsrView<float> fm_view<float>(const DevCSR &c) {
   CsrView<float> v;
   v.nr = c.nr; v.rp = c.rp; v.ci = c.ci; v.v = c.vf;
   v.lpr = fmk::lanes_for(c.nnz, c.nr);
-  v.nsl = c.sell.nsl; v.sl_off = c.sell.sl_off; v.sl_w = c.sell.sl_w; v.perm = c.sell.perm; v.scols = c.sell.cols;
+  fm_sell_fields(v, c.sell);
   v.sv = c.sell.vf;
   return v;
 }
-int fm_build_sell(hipStream_t st, DevCSR &c);
+// ro[i] = CSR row stored at internal row i; cr[j] = internal column of CSR column j (nullptr = identity)
+int fm_build_sell(hipStream_t st, DevCSR &c, const int *ro, const int *cr);
 int fm_alloc_level_vectors(Level &l, bool top);
 void fm_drop_graphs(fm_system *s);
 void fm_free_level(Level &l);

@@ -10,10 +10,11 @@ struct CsrView {
   const int *ci = nullptr;
   const T *v = nullptr;
   int lpr = 16;  // lanes per row
-  // SELL-64 copy (used when nsl > 0)
+  // SELL-64 copy in internal DoF order (used when nsl > 0)
   int64_t nsl = 0;
-  const int64_t *sl_off = nullptr;
-  const int *sl_w = nullptr, *perm = nullptr, *scols = nullptr;
+  const int64_t *sl_off = nullptr, *sl_c32 = nullptr;
+  const int *sl_w = nullptr, *perm = nullptr, *base = nullptr, *c32 = nullptr;
+  const unsigned short *c16 = nullptr;
   const T *sv = nullptr;
 };
 
@@ -62,7 +63,9 @@ int add_f32(hipStream_t s, int64_t n, double *x, const float *e, const double *s
 template <typename T> int dot_dev(hipStream_t s, int64_t n, const T *x, const T *y, double *part, double *out, int do_sqrt);
 int ratio(hipStream_t s, const double *a, const double *b, double *out);
 template <typename T> int dense_mv(hipStream_t s, int n, const T *M, const T *b, T *y);
-int diag_inv(hipStream_t s, int64_t n, const int64_t *rp, const int *ci, const double *v, double *d64, float *d32);
+int diag_inv(hipStream_t s, int64_t n, const int64_t *rp, const int *ci, const double *v, const int *ord, double *d64,
+             float *d32);
+int gather(hipStream_t s, int64_t n, const int *idx, const double *src, double *dst);
 int cast_vals(hipStream_t s, int64_t n, const double *v, float *vf);
 int elpos(hipStream_t s, int nel, int nloc, const int *eldof, const int64_t *rp, const int *ci, int *pos);
 int copy16(hipStream_t s, int64_t n16, const void *a, void *b);

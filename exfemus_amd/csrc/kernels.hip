@@ -73,43 +73,71 @@ __global__ __launch_bounds__(256) void k_spmv(int64_t nrows, const int64_t *__re
   apply_ep<T, EP>(row, sum, xg, ep);
 }
 
-// SELL-64: one wavefront per slice, one lane per row, entries of a row summed
-// in order; matrix values and columns are streamed with non-temporal loads so
-// that the gathered vector stays resident in L2 / Infinity Cache.  With
-// xcd_chunk > 0 the block index is remapped so that the blocks the hardware
-// places on one XCD (block id modulo 8) walk one contiguous range of slices:
-// neighbouring rows, which gather neighbouring x entries, then share that
-// XCD's L2.
+// SELL-64: one wavefront per slice, one lane per row, the entries of a row
+// summed in increasing internal column order; matrix values and column
+// offsets are streamed with non-temporal loads so that the gathered vector
+// stays resident in L2 / Infinity Cache.  The slice index is made wave-uniform
+// (readfirstlane) so that the slice header and the per-position column bases
+// are scalar loads shared by the 64 lanes.  A 16-bit slice decodes column
+// base[k] + c16[slot]; a 32-bit slice reads c32.  With xcd_chunk > 0 the block
+// index is remapped so that the blocks the hardware places on one XCD (block
+// id modulo 8) walk one contiguous range of slices.
 template <typename T, int EP, int UNR>
-__global__ __launch_bounds__(256) void k_sell(int64_t nsl, const int64_t *__restrict__ sl_off,
-                                              const int *__restrict__ sl_w, const int *__restrict__ perm,
-                                              const int *__restrict__ cols, const T *__restrict__ vals,
-                                              const T *__restrict__ xg, EpArgs<T> ep, int64_t xcd_chunk) {
-  int64_t blk = blockIdx.x;
-  if (xcd_chunk > 0) blk = (int64_t)(blockIdx.x % 8) * xcd_chunk + blockIdx.x / 8;
-  const int64_t s = blk * (blockDim.x >> 6) + (threadIdx.x >> 6);
+__global__ __launch_bounds__(256) void k_sell(int nsl, const int64_t *__restrict__ sl_off,
+                                              const int64_t *__restrict__ sl_c32, const int *__restrict__ sl_w,
+                                              const int *__restrict__ perm, const unsigned short *__restrict__ c16,
+                                              const int *__restrict__ base, const int *__restrict__ c32,
+                                              const T *__restrict__ vals, const T *__restrict__ xg, EpArgs<T> ep,
+                                              int xcd_chunk) {
+  int blk = blockIdx.x;
+  if (xcd_chunk > 0) blk = (int)(blockIdx.x % 8) * xcd_chunk + (int)(blockIdx.x / 8);
+  const int s = __builtin_amdgcn_readfirstlane(blk * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6));
   if (s >= nsl) return;
   const int lane = threadIdx.x & 63;
-  const int row = perm[s * 64 + lane];
+  const int row = perm[(int64_t)s * 64 + lane];
   const int w = sl_w[s];
-  const T *vp = vals + sl_off[s] + lane;
-  const int *cp = cols + sl_off[s] + lane;
+  const int64_t off = sl_off[s];
+  const int64_t o32 = sl_c32[s];
+  const T *vp = vals + off + lane;
   T sum = 0;
   int k = 0;
-  for (; k + UNR <= w; k += UNR) {
-    T v[UNR];
-    int c[UNR];
+  if (o32 < 0) {
+    const unsigned short *cp = c16 + off + lane;
+    const int *bp = base + (off >> 6);
+    for (; k + UNR <= w; k += UNR) {
+      T v[UNR];
+      int c[UNR];
 #pragma unroll
-    for (int u = 0; u < UNR; u++) v[u] = __builtin_nontemporal_load(vp + (k + u) * 64);
+      for (int u = 0; u < UNR; u++) v[u] = __builtin_nontemporal_load(vp + (k + u) * 64);
 #pragma unroll
-    for (int u = 0; u < UNR; u++) c[u] = __builtin_nontemporal_load(cp + (k + u) * 64);
-    T xv[UNR];
+      for (int u = 0; u < UNR; u++) c[u] = (int)__builtin_nontemporal_load(cp + (k + u) * 64);
 #pragma unroll
-    for (int u = 0; u < UNR; u++) xv[u] = xg[c[u]];
+      for (int u = 0; u < UNR; u++) c[u] += bp[k + u];
+      T xv[UNR];
 #pragma unroll
-    for (int u = 0; u < UNR; u++) sum += v[u] * xv[u];
+      for (int u = 0; u < UNR; u++) xv[u] = xg[c[u]];
+#pragma unroll
+      for (int u = 0; u < UNR; u++) sum += v[u] * xv[u];
+    }
+    for (; k < w; k++)
+      sum += __builtin_nontemporal_load(vp + k * 64) * xg[bp[k] + (int)__builtin_nontemporal_load(cp + k * 64)];
+  } else {
+    const int *cp = c32 + o32 + lane;
+    for (; k + UNR <= w; k += UNR) {
+      T v[UNR];
+      int c[UNR];
+#pragma unroll
+      for (int u = 0; u < UNR; u++) v[u] = __builtin_nontemporal_load(vp + (k + u) * 64);
+#pragma unroll
+      for (int u = 0; u < UNR; u++) c[u] = __builtin_nontemporal_load(cp + (k + u) * 64);
+      T xv[UNR];
+#pragma unroll
+      for (int u = 0; u < UNR; u++) xv[u] = xg[c[u]];
+#pragma unroll
+      for (int u = 0; u < UNR; u++) sum += v[u] * xv[u];
+    }
+    for (; k < w; k++) sum += __builtin_nontemporal_load(vp + k * 64) * xg[__builtin_nontemporal_load(cp + k * 64)];
   }
-  for (; k < w; k++) sum += __builtin_nontemporal_load(vp + k * 64) * xg[__builtin_nontemporal_load(cp + k * 64)];
   if (row < 0) return;
   apply_ep<T, EP>(row, sum, xg, ep);
 }
@@ -140,12 +168,13 @@ static int launch_spmv(hipStream_t s, const CsrView<T> &A, const T *x, const EpA
       nb = chunk * 8;  // padded grid: surplus blocks map past the last slice and exit
     }
     dim3 grid((unsigned)nb);
+    const int nsl = (int)A.nsl;
     if (g_sell_unroll == 8)
-      hipLaunchKernelGGL((k_sell<T, EP, 8>), grid, dim3(block), 0, s, A.nsl, A.sl_off, A.sl_w, A.perm, A.scols, A.sv, x,
-                         ep, chunk);
+      hipLaunchKernelGGL((k_sell<T, EP, 8>), grid, dim3(block), 0, s, nsl, A.sl_off, A.sl_c32, A.sl_w, A.perm, A.c16,
+                         A.base, A.c32, A.sv, x, ep, (int)chunk);
     else
-      hipLaunchKernelGGL((k_sell<T, EP, 4>), grid, dim3(block), 0, s, A.nsl, A.sl_off, A.sl_w, A.perm, A.scols, A.sv, x,
-                         ep, chunk);
+      hipLaunchKernelGGL((k_sell<T, EP, 4>), grid, dim3(block), 0, s, nsl, A.sl_off, A.sl_c32, A.sl_w, A.perm, A.c16,
+                         A.base, A.c32, A.sv, x, ep, (int)chunk);
     return hipGetLastError() == hipSuccess ? 0 : -2;
   }
   int64_t threads = A.nr * A.lpr;
@@ -351,16 +380,30 @@ template int dense_mv<double>(hipStream_t, int, const double *, const double *, 
 template int dense_mv<float>(hipStream_t, int, const float *, const float *, float *);
 
 // ---- diagonal inverse
-template <typename T> __global__ void k_diag_inv(int64_t n, const int64_t *rp, const int *ci, const double *v, T *dinv) {
+// dinv[i] = 1 / A(ord[i], ord[i]) from the CSR copy (ord = nullptr: identity)
+template <typename T>
+__global__ void k_diag_inv(int64_t n, const int64_t *rp, const int *ci, const double *v, const int *ord, T *dinv) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = ord ? ord[i] : i;
     double d = 0.;
-    for (int64_t q = rp[i]; q < rp[i + 1]; q++) if (ci[q] == i) d = v[q];
+    for (int64_t q = rp[r]; q < rp[r + 1]; q++) if (ci[q] == r) d = v[q];
     dinv[i] = (T)(d != 0. ? 1.0 / d : 0.0);
   }
 }
-int diag_inv(hipStream_t s, int64_t n, const int64_t *rp, const int *ci, const double *v, double *d64, float *d32) {
-  if (d64) hipLaunchKernelGGL(k_diag_inv<double>, vgrid(n), dim3(256), 0, s, n, rp, ci, v, d64);
-  if (d32) hipLaunchKernelGGL(k_diag_inv<float>, vgrid(n), dim3(256), 0, s, n, rp, ci, v, d32);
+int diag_inv(hipStream_t s, int64_t n, const int64_t *rp, const int *ci, const double *v, const int *ord, double *d64,
+             float *d32) {
+  if (d64) hipLaunchKernelGGL(k_diag_inv<double>, vgrid(n), dim3(256), 0, s, n, rp, ci, v, ord, d64);
+  if (d32) hipLaunchKernelGGL(k_diag_inv<float>, vgrid(n), dim3(256), 0, s, n, rp, ci, v, ord, d32);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+// dst[i] = src[idx[i]]  (reference <-> internal DoF order)
+__global__ void k_gather(int64_t n, const int *__restrict__ idx, const double *__restrict__ src, double *__restrict__ dst) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[idx[i]];
+}
+int gather(hipStream_t s, int64_t n, const int *idx, const double *src, double *dst) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_gather, vgrid(n), dim3(256), 0, s, n, idx, src, dst);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 __global__ void k_cast_vals(int64_t n, const double *v, float *vf) {

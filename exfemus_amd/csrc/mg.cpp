@@ -67,10 +67,11 @@ template <typename T> static double dev_dot(fm_ctx *c, int64_t n, const T *x, co
   return c->h_scratch[0];
 }
 
-// ---- lambda_max(D^-1 A) by power iteration (same start vector as the oracle)
-__global__ void k_start_vec(int64_t n, double *x) {
+// ---- lambda_max(D^-1 A) by power iteration (same start vector as the oracle,
+// 1 + (dof mod 7) / 7 in the reference numbering: ord[i] is the reference DoF of internal row i)
+__global__ void k_start_vec(int64_t n, const int *__restrict__ ord, double *x) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    x[i] = 1.0 + (double)(i % 7) / 7.0;
+    x[i] = 1.0 + (double)(ord[i] % 7) / 7.0;
 }
 // y = dinv .* x
 __global__ void k_scale_mul(int64_t n, double *y, const double *__restrict__ dinv, const double *__restrict__ x) {
@@ -82,7 +83,7 @@ static int lambda_max(fm_ctx *c, const LOps &o, int iters, double *lam) {
   hipStream_t st = c->stream;
   int64_t n = o.n;
   unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
-  hipLaunchKernelGGL(k_start_vec, dim3(g), dim3(256), 0, st, n, l->t);
+  hipLaunchKernelGGL(k_start_vec, dim3(g), dim3(256), 0, st, n, l->ord, l->t);
   double lm = 0.;
   for (int it = 0; it < iters; it++) {
     double xn = dev_dot<double>(c, n, l->t, l->t, true);
@@ -98,7 +99,8 @@ static int lambda_max(fm_ctx *c, const LOps &o, int iters, double *lam) {
 }
 
 // ---- coarse inverse (host FP64 Gauss-Jordan with partial pivoting) --------
-static int coarse_inverse(fm_system *s, const DevCSR &A, double **dAinv, float **dAinvf) {
+// The dense inverse is laid out in the level's internal DoF order (rank[dof]).
+static int coarse_inverse(fm_system *s, const DevCSR &A, const std::vector<int> &rank, double **dAinv, float **dAinvf) {
   int n = (int)A.nr;
   std::vector<double> vals(A.nnz);
   std::vector<int64_t> rp(n + 1);
@@ -109,8 +111,12 @@ static int coarse_inverse(fm_system *s, const DevCSR &A, double **dAinv, float *
   FM_HIP(hipMemcpy(ci.data(), A.ci, A.nnz * sizeof(int), hipMemcpyDeviceToHost));
   std::vector<double> M((size_t)n * n, 0.), I((size_t)n * n, 0.);
   double amax = 0.;
+  if ((int)rank.size() != n) { fm_set_error("coarse level without internal order"); return FM_ESTATE; }
   for (int i = 0; i < n; i++)
-    for (int64_t q = rp[i]; q < rp[i + 1]; q++) { M[(size_t)i * n + ci[q]] = vals[q]; amax = std::max(amax, std::fabs(vals[q])); }
+    for (int64_t q = rp[i]; q < rp[i + 1]; q++) {
+      M[(size_t)rank[i] * n + rank[ci[q]]] = vals[q];
+      amax = std::max(amax, std::fabs(vals[q]));
+    }
   std::vector<int> pinned;
   for (int attempt = 0; attempt < 2; attempt++) {
     std::vector<double> W = M;
@@ -145,7 +151,7 @@ static int coarse_inverse(fm_system *s, const DevCSR &A, double **dAinv, float *
     if (attempt == 1 || s->nq1 == 0) { fm_set_error("coarse matrix singular"); return FM_ESINGULAR; }
     (void)bad;
     // enclosed flow: pressure defined up to a constant -> pin the last Q1 DoF
-    pinned.push_back(n - 1);
+    pinned.push_back(rank[n - 1]);
   }
   if (!*dAinv) FM_CHECK(hipMalloc((void **)dAinv, (size_t)n * n * sizeof(double)) == hipSuccess ? 0 : FM_ENOMEM);
   if (!*dAinvf) FM_CHECK(hipMalloc((void **)dAinvf, (size_t)n * n * sizeof(float)) == hipSuccess ? 0 : FM_ENOMEM);
@@ -307,7 +313,7 @@ static int mg_setup_ops(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &op
     if (o->smoother == FM_SMOOTH_CHEBYSHEV && ops[L].lam == 0.) FM_CHECK(lambda_max(s->ctx, ops[L], 20, &ops[L].lam));
     if (o->smoother == FM_SMOOTH_VANKA) FM_CHECK(fm_vanka_setup(s, (int)L, &ops));
   }
-  FM_CHECK(coarse_inverse(s, *ops[0].A, &ops[0].Ainv, &ops[0].Ainvf));
+  FM_CHECK(coarse_inverse(s, *ops[0].A, ops[0].v->h_rank, &ops[0].Ainv, &ops[0].Ainvf));
   return 0;
 }
 
@@ -583,7 +589,9 @@ extern "C" int fm_adjoint_solve(fm_system *s, const fm_mg_opts *o, const double 
   int top = (int)ops.size() - 1;
   Level &at = s->alev[top];
   hipStream_t st = s->ctx->stream;
-  FM_HIP(hipMemcpyAsync(at.b, g, at.n * sizeof(double), hipMemcpyHostToDevice, st));
+  // g arrives in the reference numbering; the hierarchy works in internal order
+  FM_HIP(hipMemcpyAsync(at.t, g, at.n * sizeof(double), hipMemcpyHostToDevice, st));
+  FM_CHECK(fmk::gather(st, at.n, at.ord, at.t, at.b));
   FM_CHECK(fmk::fill<double>(st, at.n, at.x, 0.0));
   Timer tm(s->ctx);
   int rc;
@@ -591,7 +599,8 @@ extern "C" int fm_adjoint_solve(fm_system *s, const fm_mg_opts *o, const double 
   else rc = solve_fgmres(s, o, ops, rep, 1);
   rep->seconds = tm.stop();
   if (rc) return rc;
-  FM_HIP(hipMemcpyAsync(lam, at.x, at.n * sizeof(double), hipMemcpyDeviceToHost, st));
+  FM_CHECK(fmk::gather(st, at.n, at.rank, at.x, at.t));
+  FM_HIP(hipMemcpyAsync(lam, at.t, at.n * sizeof(double), hipMemcpyDeviceToHost, st));
   FM_HIP(hipStreamSynchronize(st));
   return rep->converged ? FM_OK : FM_ENOCONV;
 }

@@ -162,7 +162,7 @@ static int vanka_invert_level(fm_system *s, int L, const double *Av, double **st
   else
     hipLaunchKernelGGL(k_vanka_invert<89>, dim3(g), dim3(256), 0, st, l.nel, l.elpos, Av, *store, status);
   FM_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_count_mult, dim3(1024), dim3(256), 0, st, l.nel, nl, l.eldof, l.vwt);
+  hipLaunchKernelGGL(k_count_mult, dim3(1024), dim3(256), 0, st, l.nel, nl, l.eldofi, l.vwt);
   hipLaunchKernelGGL(k_recip, dim3(1024), dim3(256), 0, st, l.n, l.vwt);
   int h_status = 0;
   FM_HIP(hipMemcpyAsync(&h_status, status, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -198,10 +198,10 @@ static int vanka_sweep(fm_system *s, int L, const CsrView<T> &A, const double *v
       unsigned g = (unsigned)(n < 8192 ? n : 8192);
       const double *w = colored ? l.vone : wt;
       if (s->mesh->dim == 2)
-        hipLaunchKernelGGL((k_vanka_apply<22, T>), dim3(g), dim3(128), 0, st, n, l.color_elems + l.color_off[c], l.eldof,
+        hipLaunchKernelGGL((k_vanka_apply<22, T>), dim3(g), dim3(128), 0, st, n, l.color_elems + l.color_off[c], l.eldofi,
                            vinv, r, x, w, omega);
       else
-        hipLaunchKernelGGL((k_vanka_apply<89, T>), dim3(g), dim3(128), 0, st, n, l.color_elems + l.color_off[c], l.eldof,
+        hipLaunchKernelGGL((k_vanka_apply<89, T>), dim3(g), dim3(128), 0, st, n, l.color_elems + l.color_off[c], l.eldofi,
                            vinv, r, x, w, omega);
     }
   }

@@ -186,3 +186,43 @@ def test_graph_replay_matches_eager(femus, oracle, ctx, prec):
         for key, x in res[g]:
             assert key == ref_key
             np.testing.assert_array_equal(x, ref_x)
+
+
+@pytest.mark.parametrize("dim,nx,nlev", CASES)
+def test_level_operators_vs_csr(femus, oracle, ctx, dim, nx, nlev):
+    """NumericVectorM::resid / matrix_mult(R), matrix_mult(P) (contrib/matrix/laspack_vectorM.C:147-189) through
+    the C ABI: the device runs them on SELL copies in its internal DoF order; vectors cross the ABI in the
+    reference numbering and must equal the same products formed with the reference-order CSR matrices."""
+    pm, om, sysT, ks = _build(femus, oracle, ctx, dim, nx, nlev)
+    top = nlev - 1
+    sysT.set_xold(np.zeros(sysT._Dim[top]))
+    for L in range(nlev):
+        sysT.GenMatRhs(0.0, L, 1 if L == top else 0)
+    rng = np.random.default_rng(11)
+    for L in range(nlev):
+        n = sysT._Dim[L]
+        rp, ci, v = sysT.csr(0, L)
+        A = sp.csr_matrix((v, ci, rp), shape=(n, n))
+        x = rng.standard_normal(n)
+        b = rng.standard_normal(n)
+        # y = A x through the CSR kernel (reference order) against the oracle's LASPACK-order product
+        np.testing.assert_allclose(sysT.spmv(L, x), oracle.spmv(rp, ci, v, x), rtol=1e-13, atol=1e-13 * np.abs(v).sum())
+        sysT.set_x(x, L)
+        sysT.set_rhs(b, L)
+        np.testing.assert_array_equal(sysT.get_x(L), x)
+        np.testing.assert_array_equal(sysT.get_rhs(L), b)
+        sysT.resid(L)
+        res = sysT.get_res(L)
+        ref = b - A @ x
+        np.testing.assert_allclose(res, ref, rtol=1e-12, atol=1e-13 * (np.abs(b).max() + np.abs(A).sum(axis=1).max()))
+        if L > 0:
+            sysT.restrict(L)
+            rr, rc, rv = sysT.csr(1, L - 1)
+            R = sp.csr_matrix((rv, rc, rr), shape=(sysT._Dim[L - 1], n))
+            np.testing.assert_allclose(sysT.get_rhs(L - 1), R @ res, rtol=1e-12, atol=1e-12 * np.abs(res).max())
+            xc = rng.standard_normal(sysT._Dim[L - 1])
+            sysT.set_x(xc, L - 1)
+            sysT.prolong(L)
+            pr, pc, pv = sysT.csr(2, L)
+            P = sp.csr_matrix((pv, pc, pr), shape=(n, sysT._Dim[L - 1]))
+            np.testing.assert_allclose(sysT.get_res(L), P @ xc, rtol=1e-12, atol=1e-12 * np.abs(xc).max())

@@ -187,3 +187,37 @@ def test_ns_graph_replay_matches_eager(femus, oracle, ctx):
     for it, h, x in out[1:]:
         assert (it, h) == out[0][:2]
         np.testing.assert_array_equal(x, out[0][2])
+
+
+@pytest.mark.parametrize("case,dim,nx,nlev", CASES)
+def test_ns_level_operators_vs_csr(femus, oracle, ctx, case, dim, nx, nlev):
+    """Residual, restriction and prolongation of the coupled system through the C ABI (SELL copies in the
+    device's internal node-interleaved order) against the reference-order CSR products."""
+    pm, om, ns, bc, x0 = _setup(femus, oracle, ctx, case, dim, nx, nlev)
+    top = nlev - 1
+    for L in range(nlev):
+        ns.GenMatRhs(0.0, L, 1 if L == top else 0)
+    rng = np.random.default_rng(12)
+    for L in range(nlev):
+        n = ns._Dim[L]
+        rp, ci, v = ns.csr(0, L)
+        A = sp.csr_matrix((v, ci, rp), shape=(n, n))
+        x = rng.standard_normal(n)
+        b = rng.standard_normal(n)
+        ns.set_x(x, L)
+        ns.set_rhs(b, L)
+        ns.resid(L)
+        res = ns.get_res(L)
+        np.testing.assert_allclose(res, b - A @ x, rtol=1e-12,
+                                   atol=1e-13 * (np.abs(b).max() + np.abs(A).sum(axis=1).max()))
+        if L > 0:
+            ns.restrict(L)
+            rr, rc, rv = ns.csr(1, L - 1)
+            R = sp.csr_matrix((rv, rc, rr), shape=(ns._Dim[L - 1], n))
+            np.testing.assert_allclose(ns.get_rhs(L - 1), R @ res, rtol=1e-12, atol=1e-12 * np.abs(res).max())
+            xc = rng.standard_normal(ns._Dim[L - 1])
+            ns.set_x(xc, L - 1)
+            ns.prolong(L)
+            pr, pc, pv = ns.csr(2, L)
+            P = sp.csr_matrix((pv, pc, pr), shape=(n, ns._Dim[L - 1]))
+            np.testing.assert_allclose(ns.get_res(L), P @ xc, rtol=1e-12, atol=1e-12 * np.abs(xc).max())
