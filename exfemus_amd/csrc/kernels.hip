@@ -143,16 +143,18 @@ __global__ __launch_bounds__(256) void k_sell(int nsl, const int64_t *__restrict
 }
 
 // launch-shape knobs of the SELL kernel (read once from the environment):
-// FEMUS_SELL_UNROLL = 4 | 8 (default 8), FEMUS_SELL_XCD = 0 | 1 (default 0).
-// Measured on the 64^3 heat matrix: unroll 8 beats 4 by ~6%; the contiguous
-// per-XCD slice ranges lose ~40% against the hardware's round-robin spread.
+// FEMUS_SELL_UNROLL = 4 | 8 (default 8), FEMUS_SELL_XCD = 0 | 1 (default 1).
+// Measured on MI355X, 3D heat hex27 64^3 top matrix (135M nnz) in the
+// node-lexicographic internal order: contiguous per-XCD slice ranges take the
+// FP64 SpMV from 389 to 301 us and the FP32 one from 249 to 187 us (each XCD's
+// L2 then holds the x window of its own range); unroll 8 beats 4 by ~2%.
 static int g_sell_unroll = -1, g_sell_xcd = -1;
 static void sell_knobs() {
   if (g_sell_unroll >= 0) return;
   const char *u = getenv("FEMUS_SELL_UNROLL");
   const char *x = getenv("FEMUS_SELL_XCD");
   g_sell_unroll = (u && atoi(u) == 4) ? 4 : 8;
-  g_sell_xcd = (x && atoi(x) == 1) ? 1 : 0;
+  g_sell_xcd = (x && atoi(x) == 0) ? 0 : 1;
 }
 
 template <typename T, int EP>
