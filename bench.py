@@ -108,16 +108,55 @@ def cpu_baseline(budget_s=25.0):
                       f"{hist[-1]:.2e} in {dt:.2f} s"}
 
 
+def build_ns(F, ctx, workload):
+    """C3: channel [0,4]x[0,1]^2, 96x24x24 hex27 top (12x3x3 coarse, 4 levels), Re = 400, parabolic inflow,
+    zero traction at x = 4; C1: lid-driven cavity 32^2 quad9 (8^2 coarse, 3 levels), Re = 100.  One Oseen
+    (Picard) system with x_old = the initial condition, assembled on every level."""
+    import functools
+    from exfemus_amd import configs
+    if workload == "channel":
+        m = F.Mesh(3, [12, 3, 3], [0, 0, 0], [4, 1, 1], 4)
+        s = F.MGSolNS(ctx, m, IRe=1.0 / 400, dt=0.1, time_coeff=0.0, upwind=0.1)
+        s.GenBc(configs.channel_bc)
+        s.init()
+        s.GenIc(configs.channel_ic)
+    else:
+        m = F.Mesh(2, [8, 8], [0, 0], [1, 1], 3)
+        s = F.MGSolNS(ctx, m, IRe=0.01, dt=0.1, time_coeff=0.0, upwind=0.1)
+        s.GenBc(functools.partial(configs.cavity_bc, dim=2))
+        s.init()
+        s.GenIc(functools.partial(configs.cavity_ic, dim=2))
+    top = m.nlev - 1
+    t0 = time.perf_counter()
+    for L in range(m.nlev):
+        s.GenMatRhs(0.0, L, 1 if L == top else 0)
+    ctx.sync()
+    return m, s, time.perf_counter() - t0
+
+
+def field_ids(s):
+    """Variable index of every top-level DoF (0..nq2-1 velocity components / T, nq2 = pressure)."""
+    top = s._NoLevels - 1
+    nd = s.node_dof(top)
+    N = s.mesh.N
+    var = np.full(s._Dim[top], -1)
+    for iv in range(s.nq2 + s.nq1):
+        d = nd[iv * N:(iv + 1) * N]
+        var[d[d >= 0]] = iv
+    return var
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="heat64", choices=["heat64", "channel", "cavity"])
     ap.add_argument("--nx0", type=int, default=4)
     ap.add_argument("--nlev", type=int, default=5)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--outer", default="fgmres", choices=["ir", "fgmres"])
-    ap.add_argument("--nu", type=int, default=3)
+    ap.add_argument("--nu", type=int, default=None)
     ap.add_argument("--cheb-lo", type=float, default=0.1)
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
@@ -129,18 +168,33 @@ def main():
     from exfemus_amd import femus as F
     ctx = F.Context(0)
     t0 = time.perf_counter()
-    nx0 = [args.nx0] * 3
-    m, s, t_asm = build_heat(F, ctx, nx0, args.nlev)
-    top = args.nlev - 1
+    heat = args.workload == "heat64"
+    if heat:
+        nu = args.nu or 3
+        m, s, t_asm = build_heat(F, ctx, [args.nx0] * 3, args.nlev)
+        kw = dict(nu_pre=nu, nu_post=nu, cheb_lo=args.cheb_lo)
+        smoother = f"Chebyshev-Jacobi {nu}/{nu}"
+        workload = (f"3D heat hex27 {args.nx0 * 16}^3, {args.nlev} levels, lognormal k, "
+                    f"FP32-MG+FP64-{args.outer.upper()} to {args.tol:g}")
+        data = "synthetic (seeded log-normal k, f = 1)"
+    else:
+        nu = args.nu or 4
+        m, s, t_asm = build_ns(F, ctx, args.workload)
+        kw = dict(nu_pre=nu, nu_post=nu, smoother=F.FM_SMOOTH_VANKA, vanka_colored=1)
+        smoother = f"coloured element Vanka {nu}/{nu}"
+        workload = ("3D NS channel hex27 Q2-Q1 96x24x24, 4 levels, Re 400, Oseen step from the inflow profile"
+                    if args.workload == "channel" else
+                    "2D NS lid-driven cavity quad9 Q2-Q1 32^2, 3 levels, Re 100, Oseen step from the lid data")
+        workload += f", FP32-MG+FP64-{args.outer.upper()} to {args.tol:g}"
+        data = "synthetic (analytic boundary data)"
+    top = m.nlev - 1
     N = s._Dim[top]
     nel_top = m.level(top)[0]
     log(f"setup {time.perf_counter() - t0:.1f}s  N_top={N}  assembly(all levels)={t_asm * 1e3:.1f} ms")
     prec_mixed = F.FM_PREC_FP32_IR if args.outer == "ir" else F.FM_PREC_FP32_FGMRES
     prec_fp64 = F.FM_PREC_FP64 if args.outer == "ir" else F.FM_PREC_FP64_FGMRES
-    om = s.opts(precision=prec_mixed, nu_pre=args.nu, nu_post=args.nu, tol=args.tol, max_iter=100,
-                cheb_lo=args.cheb_lo)
-    o64 = s.opts(precision=prec_fp64, nu_pre=args.nu, nu_post=args.nu, tol=args.tol, max_iter=100,
-                 cheb_lo=args.cheb_lo)
+    om = s.opts(precision=prec_mixed, tol=args.tol, max_iter=200, **kw)
+    o64 = s.opts(precision=prec_fp64, tol=args.tol, max_iter=200, **kw)
     s.MGSetup(o64)
     zero = np.zeros(N)
 
@@ -169,12 +223,17 @@ def main():
     ctx.sync()
     wall = time.perf_counter() - t_wall0
     xm = s.get_x()
-    err_T = float(np.linalg.norm(xm - x64) / np.linalg.norm(x64))
+    var = field_ids(s)
+    names = ["T"] if heat else (["u", "v", "w"][:m.dim] + ["p"])
+    errs = {}
+    for iv, nm in enumerate(names):
+        sel = var == iv
+        errs[nm] = float(np.linalg.norm(xm[sel] - x64[sel]) / max(np.linalg.norm(x64[sel]), 1e-300))
     t_mixed = float(np.sum(tm))
     value = N * args.steps / t_mixed / 1e6
     mdofs_fp64 = N * args.steps / float(np.sum(t64)) / 1e6
     log(f"mixed: iters={reps[-1].iters} rel={reps[-1].rel_resid:.3e} t={np.median(tm) * 1e3:.2f} ms; "
-        f"fp64: iters={rep64.iters} t={np.median(t64) * 1e3:.2f} ms; err_T={err_T:.2e}")
+        f"fp64: iters={rep64.iters} t={np.median(t64) * 1e3:.2f} ms; err={errs}")
     # SpMV roofline on the top matrix (FP64 and FP32 values)
     nnz = int(F.lib().fm_system_csr_nnz(s.h, 0, top))
     sec64 = s.bench_spmv(top, 0, args.spmv_reps)
@@ -202,16 +261,14 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32 V-cycle + fp64 residual",
-        "data": "synthetic (seeded log-normal k, f = 1)",
-        "config": {"workload": f"3D heat hex27 {args.nx0 * 16}^3, {args.nlev} levels, lognormal k, "
-                               f"FP32-MG+FP64-{args.outer.upper()} to {args.tol:g}",
-                   "n_dof_top": N, "nnz_top": nnz, "elements_top": nel_top, "outer": args.outer,
-                   "smoother": f"Chebyshev-Jacobi {args.nu}/{args.nu}", "parallelism": "single GPU"},
+        "data": data,
+        "config": {"workload": workload, "n_dof_top": N, "nnz_top": nnz, "elements_top": nel_top,
+                   "outer": args.outer, "smoother": smoother, "parallelism": "single GPU"},
         "fp64_mg": {"value": round(mdofs_fp64, 3), "unit": "MDoF/s", "ms_per_solve": round(float(np.median(t64)) * 1e3, 3),
                     "iters": rep64.iters},
         "mixed_iters": reps[-1].iters,
         "speedup_vs_fp64": round(value / mdofs_fp64, 3),
-        "rel_l2_err_T_vs_fp64": err_T,
+        "rel_l2_err_vs_fp64": errs,
         "final_rel_resid": reps[-1].rel_resid,
         "assembly_ms_all_levels": round(t_asm * 1e3, 2),
         "spmv": {"fp64_us": round(sec64 * 1e6, 2), "fp64_GBs": round(gbs64, 1), "fp64_frac": round(gbs64 / HBM_PEAK_GBS, 4),
@@ -223,7 +280,7 @@ def main():
                      "frac": round(gbs32 / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(N, nnz, "fp32"),
                      "bytes_alg": b32, "kernel": f"k_sell<float> y = A x ({F.SPMV_FORMAT}) on the top matrix"},
     }
-    if not args.no_cpu:
+    if not args.no_cpu and heat:
         try:
             out["cpu_baseline"] = cpu_baseline()
         except Exception as e:  # the oracle may be unbuilt on a bare box
