@@ -224,7 +224,7 @@ __global__ void k_heat_bd(int nel, const int *__restrict__ elems, const int *__r
       for (int k = 0; k < NB; k++) {
         int lv = sur[k];
         int p = elpos[pbase + lv * N2 + lv];
-        if (p >= 0) val[p] += Ipen;
+        if (val && p >= 0) val[p] += Ipen;
         if (mode == 1) {
           int node = conn[(int64_t)e * N2 + lv];
           rhs[eldof[(int64_t)e * N2 + lv]] += (bc_s % 2) * Ipen * x_old[dof_top[node]];
@@ -243,7 +243,7 @@ __global__ void k_heat_bd(int nel, const int *__restrict__ elems, const int *__r
           double phii = tab->bphi2[i * NGB + qp];
           for (int j = 0; j < NB; j++) {
             int p = elpos[pbase + (trans ? sur[j] * N2 + lei : lei * N2 + sur[j])];
-            if (p >= 0) val[p] += dtxJxW * bc_alpha * a.alpha0 * phii * tab->bphi2[j * NGB + qp];
+            if (val && p >= 0) val[p] += dtxJxW * bc_alpha * a.alpha0 * phii * tab->bphi2[j * NGB + qp];
           }
         }
       }
@@ -427,6 +427,7 @@ __global__ void k_ns_bd(int nel, const int *__restrict__ elems, const int *__res
           if (mode == 1)
             rhs[eldof[(int64_t)e * NL + row]] += bc_rhs * Ipen * ((1 - bc_n) * (1 - bc_tg) * vo[iv] + bc_n * vdotn * nrm[iv] +
                                                                     bc_tg * (vo[iv] - vdotn * nrm[iv]));
+          if (!val) continue;
           int p = elpos[pbase + row * NL + row];
           if (p >= 0) val[p] += ((1 - bc_n) * (1 - bc_tg) + bc_tg) * Ipen + Ipen * nrm[iv] * nrm[iv] * (bc_n - bc_tg);
           for (int jv = iv + 1; jv < DIM; jv++) {
@@ -435,7 +436,7 @@ __global__ void k_ns_bd(int nel, const int *__restrict__ elems, const int *__res
             if (pc >= 0) val[pc] += Ipen * nrm[jv] * nrm[iv] * (bc_n - bc_tg);
           }
         }
-      } else if (bc1[ckd] != 0) {
+      } else if (bc1[ckd] != 0 && val) {
         for (int qp = 0; qp < NGB; qp++) {
           double JxW = jac_sur<DIM, NB, NGB>(qp, xb, tab) * tab->bw[qp];
           for (int i = 0; i < NB; i++) {
@@ -662,40 +663,49 @@ static inline unsigned nblk(int nel) {
   return (unsigned)(b < 1 ? 1 : b);
 }
 
+// xbc: Dirichlet data for the boundary-face penalty RHS (nullptr: x_old, as in the reference);
+// bd_only: boundary kernel alone (with val == nullptr: the boundary RHS alone)
 int heat_assemble(hipStream_t s, int dim, int nel, const int *elems, const int *conn, const int *eldof,
                   const int *elpos, const unsigned char *bmask, const double *xyz, int Ntop, const int *dof_top,
-                  const int *bc0, const int *bc1, const double *x_old, const double *coef, const double *vel,
-                  const FeTab *tab, HeatArgs a, int mode, int trans, double *val, double *rhs) {
+                  const int *bc0, const int *bc1, const double *x_old, const double *xbc, const double *coef,
+                  const double *vel, const FeTab *tab, HeatArgs a, int mode, int trans, int bd_only, double *val,
+                  double *rhs) {
   if (nel <= 0) return 0;
+  const double *xb = xbc ? xbc : x_old;
   if (dim == 2) {
-    hipLaunchKernelGGL(k_heat_vol<2>, dim3(nblk(nel)), dim3(128), 0, s, nel, elems, conn, eldof, elpos, xyz, Ntop,
-                       dof_top, bc0, x_old, coef, vel, tab, a, mode, trans, val, rhs);
+    if (!bd_only)
+      hipLaunchKernelGGL(k_heat_vol<2>, dim3(nblk(nel)), dim3(128), 0, s, nel, elems, conn, eldof, elpos, xyz, Ntop,
+                         dof_top, bc0, x_old, coef, vel, tab, a, mode, trans, val, rhs);
     hipLaunchKernelGGL(k_heat_bd<2>, dim3((nel + 127) / 128), dim3(128), 0, s, nel, elems, conn, eldof, elpos, bmask,
-                       xyz, Ntop, dof_top, bc0, bc1, x_old, tab, a, mode, trans, val, rhs);
+                       xyz, Ntop, dof_top, bc0, bc1, xb, tab, a, mode, trans, val, rhs);
   } else {
-    hipLaunchKernelGGL(k_heat_vol<3>, dim3(nblk(nel)), dim3(256), 0, s, nel, elems, conn, eldof, elpos, xyz, Ntop,
-                       dof_top, bc0, x_old, coef, vel, tab, a, mode, trans, val, rhs);
+    if (!bd_only)
+      hipLaunchKernelGGL(k_heat_vol<3>, dim3(nblk(nel)), dim3(256), 0, s, nel, elems, conn, eldof, elpos, xyz, Ntop,
+                         dof_top, bc0, x_old, coef, vel, tab, a, mode, trans, val, rhs);
     hipLaunchKernelGGL(k_heat_bd<3>, dim3((nel + 127) / 128), dim3(128), 0, s, nel, elems, conn, eldof, elpos, bmask,
-                       xyz, Ntop, dof_top, bc0, bc1, x_old, tab, a, mode, trans, val, rhs);
+                       xyz, Ntop, dof_top, bc0, bc1, xb, tab, a, mode, trans, val, rhs);
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 int ns_assemble(hipStream_t s, int dim, int nel, const int *elems, const int *conn, const int *eldof,
                 const int *elpos, const unsigned char *bmask, const double *xyz, int Ntop, const int *dof_top,
-                const int *bc0, const int *bc1, const double *x_old, const double *coef,
-                const FeTab *tab, NsArgs a, int mode, int trans, double *val, double *rhs) {
+                const int *bc0, const int *bc1, const double *x_old, const double *xbc, const double *coef,
+                const FeTab *tab, NsArgs a, int mode, int trans, int bd_only, double *val, double *rhs) {
   if (nel <= 0) return 0;
+  const double *xb = xbc ? xbc : x_old;
   if (dim == 2) {
-    hipLaunchKernelGGL(k_ns_vol<2>, dim3(nblk(nel)), dim3(256), 0, s, nel, elems, conn, eldof, elpos, xyz, Ntop,
-                       dof_top, bc0, x_old, coef, tab, a, mode, trans, val, rhs);
+    if (!bd_only)
+      hipLaunchKernelGGL(k_ns_vol<2>, dim3(nblk(nel)), dim3(256), 0, s, nel, elems, conn, eldof, elpos, xyz, Ntop,
+                         dof_top, bc0, x_old, coef, tab, a, mode, trans, val, rhs);
     hipLaunchKernelGGL(k_ns_bd<2>, dim3((nel + 127) / 128), dim3(128), 0, s, nel, elems, conn, eldof, elpos, bmask,
-                       xyz, Ntop, dof_top, bc0, bc1, x_old, tab, a, mode, trans, val, rhs);
+                       xyz, Ntop, dof_top, bc0, bc1, xb, tab, a, mode, trans, val, rhs);
   } else {
-    hipLaunchKernelGGL(k_ns_vol<3>, dim3(nblk(nel)), dim3(256), 0, s, nel, elems, conn, eldof, elpos, xyz, Ntop,
-                       dof_top, bc0, x_old, coef, tab, a, mode, trans, val, rhs);
+    if (!bd_only)
+      hipLaunchKernelGGL(k_ns_vol<3>, dim3(nblk(nel)), dim3(256), 0, s, nel, elems, conn, eldof, elpos, xyz, Ntop,
+                         dof_top, bc0, x_old, coef, tab, a, mode, trans, val, rhs);
     hipLaunchKernelGGL(k_ns_bd<3>, dim3((nel + 127) / 128), dim3(128), 0, s, nel, elems, conn, eldof, elpos, bmask,
-                       xyz, Ntop, dof_top, bc0, bc1, x_old, tab, a, mode, trans, val, rhs);
+                       xyz, Ntop, dof_top, bc0, bc1, xb, tab, a, mode, trans, val, rhs);
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }

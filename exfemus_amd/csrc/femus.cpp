@@ -337,7 +337,7 @@ extern "C" int fm_system_destroy(fm_system *s) {
   (void)hipStreamSynchronize(s->ctx->stream);
   for (auto &l : s->lev) fm_free_level(l);
   for (auto &l : s->alev) fm_free_level(l);
-  dfree(s->d_bc0); dfree(s->d_bc1); dfree(s->d_dof_top); dfree(s->xyz); dfree(s->x_old); dfree(s->vel);
+  dfree(s->d_bc0); dfree(s->d_bc1); dfree(s->d_dof_top); dfree(s->xyz); dfree(s->x_old); dfree(s->vel); dfree(s->xbc);
   SysExtra &ex = extra(s);
   dfree(ex.d_tab);
   for (size_t i = 0; i < g_extra.size(); i++) if (g_extra[i].first == s) { g_extra.erase(g_extra.begin() + i); break; }
@@ -681,6 +681,69 @@ extern "C" int fm_system_set_elem_coef(fm_system *s, int L, const double *coef) 
   FM_HIP(hipStreamSynchronize(s->ctx->stream));
   return FM_OK;
 }
+// Dirichlet data of the boundary-face penalty RHS (top-level DoFs, reference
+// numbering).  The reference takes it from x_old (which also carries the
+// advection field); a separate vector lets a parameter act on the boundary
+// data alone.  nullptr restores x_old.
+extern "C" int fm_system_set_bcdata(fm_system *s, const double *g) {
+  FM_CHECK(check_level(s, 0));
+  if (!g) { dfree(s->xbc); return FM_OK; }
+  if (!s->xbc) FM_CHECK(dalloc(&s->xbc, s->Dim.back()));
+  FM_HIP(hipMemcpyAsync(s->xbc, g, s->Dim.back() * sizeof(double), hipMemcpyHostToDevice, s->ctx->stream));
+  FM_HIP(hipStreamSynchronize(s->ctx->stream));
+  return FM_OK;
+}
+
+// Boundary-face penalty RHS of the top level for Dirichlet data g (reference
+// numbering in and out), matrix untouched: the linear map g -> b_bd(g).
+static int bc_rhs(fm_system *s, const void *params, const double *g, double *rhs) {
+  FM_CHECK(check_level(s, 0));
+  if (!params || !g || !rhs) return FM_EINVAL;
+  const fm_mesh &m = *s->mesh;
+  int top = (int)s->lev.size() - 1;
+  Level &l = s->lev[top];
+  hipStream_t st = s->ctx->stream;
+  SysExtra &ex = extra(s);
+  FM_HIP(hipMemcpyAsync(l.t, g, l.n * sizeof(double), hipMemcpyHostToDevice, st));
+  FM_CHECK(fmk::fill<double>(st, l.n, l.d, 0.0));
+  const bool heat = s->nq2 == 1 && s->nq1 == 0;
+  if (!heat && !(s->nq2 == m.dim && s->nq1 == 1)) return FM_EINVAL;
+  for (int k = 0; k < l.ncolor; k++) {
+    int n = l.color_off[k + 1] - l.color_off[k];
+    const int *el = l.color_elems + l.color_off[k];
+    if (heat) {
+      const fm_heat_params *p = (const fm_heat_params *)params;
+      if (p->dt == 0.) return FM_EINVAL;
+      HeatArgs a;
+      a.alpha0 = p->alpha0; a.rhocp = p->rhocp; a.dt = p->dt; a.tc = p->time_coeff; a.source = p->source;
+      a.upwind = p->upwind;
+      for (int d = 0; d < 3; d++) a.vel_const[d] = p->vel_const[d];
+      FM_CHECK(fmk::heat_assemble(st, m.dim, n, el, l.conn, l.eldof, l.elpos, l.bmask, s->xyz, m.Ntop, s->d_dof_top,
+                                  s->d_bc0, s->d_bc1, s->x_old, l.t, l.coef, s->vel, ex.d_tab, a, 1, 0, 1, nullptr, l.d));
+    } else {
+      const fm_ns_params *p = (const fm_ns_params *)params;
+      if (p->dt == 0.) return FM_EINVAL;
+      NsArgs a;
+      a.IRe = p->IRe; a.dt = p->dt; a.tc = p->time_coeff; a.upwind = p->upwind; a.IFr = p->IFr;
+      for (int d = 0; d < 3; d++) { a.dirg[d] = p->dirg[d]; a.src[d] = p->src[d]; }
+      FM_CHECK(fmk::ns_assemble(st, m.dim, n, el, l.conn, l.eldof, l.elpos, l.bmask, s->xyz, m.Ntop, s->d_dof_top,
+                                s->d_bc0, s->d_bc1, s->x_old, l.t, l.coef, ex.d_tab, a, 1, 0, 1, nullptr, l.d));
+    }
+  }
+  FM_HIP(hipMemcpyAsync(rhs, l.d, l.n * sizeof(double), hipMemcpyDeviceToHost, st));
+  FM_HIP(hipStreamSynchronize(st));
+  return FM_OK;
+}
+
+extern "C" int fm_heat_bc_rhs(fm_system *s, const fm_heat_params *p, const double *g, double *rhs) {
+  if (!s || s->nq2 != 1 || s->nq1 != 0) return FM_EINVAL;
+  return bc_rhs(s, p, g, rhs);
+}
+extern "C" int fm_ns_bc_rhs(fm_system *s, const fm_ns_params *p, const double *g, double *rhs) {
+  if (!s || !s->mesh || s->nq2 != s->mesh->dim || s->nq1 != 1) return FM_EINVAL;
+  return bc_rhs(s, p, g, rhs);
+}
+
 extern "C" int fm_system_set_velocity(fm_system *s, const double *vel) {
   FM_CHECK(check_level(s, 0));
   if (!vel) { dfree(s->vel); return FM_OK; }
@@ -719,8 +782,8 @@ extern "C" int fm_heat_genmatrhs(fm_system *s, int L, int mode, const fm_heat_pa
   for (int k = 0; k < l.ncolor; k++) {
     int n = l.color_off[k + 1] - l.color_off[k];
     FM_CHECK(fmk::heat_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldofi, l.elpos, l.bmask, s->xyz,
-                                m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, l.coef, s->vel, ex.d_tab, a, mode,
-                                0, l.A.v, l.b));
+                                m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, s->xbc, l.coef, s->vel, ex.d_tab, a,
+                                mode, 0, 0, l.A.v, l.b));
   }
   FM_CHECK(after_assembly(s, L));
   FM_HIP(hipStreamSynchronize(st));
@@ -743,7 +806,8 @@ extern "C" int fm_ns_genmatrhs(fm_system *s, int L, int mode, const fm_ns_params
   for (int k = 0; k < l.ncolor; k++) {
     int n = l.color_off[k + 1] - l.color_off[k];
     FM_CHECK(fmk::ns_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldofi, l.elpos, l.bmask, s->xyz,
-                              m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, l.coef, ex.d_tab, a, mode, 0, l.A.v, l.b));
+                              m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, s->xbc, l.coef, ex.d_tab, a, mode, 0, 0,
+                              l.A.v, l.b));
   }
   FM_CHECK(after_assembly(s, L));
   FM_HIP(hipStreamSynchronize(st));
@@ -898,8 +962,8 @@ extern "C" int fm_heat_genmat_transpose(fm_system *s, int L, const fm_heat_param
   for (int k = 0; k < l.ncolor; k++) {
     int n = l.color_off[k + 1] - l.color_off[k];
     FM_CHECK(fmk::heat_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldof, l.elpos, l.bmask, s->xyz,
-                                m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, l.coef, s->vel, ex.d_tab, h, 0, 1,
-                                a.A.v, nullptr));
+                                m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, s->xbc, l.coef, s->vel, ex.d_tab, h,
+                                0, 1, 0, a.A.v, nullptr));
   }
   return after_assembly_T(s, L);
 }
@@ -920,8 +984,8 @@ extern "C" int fm_ns_genmat_transpose(fm_system *s, int L, const fm_ns_params *p
   for (int k = 0; k < l.ncolor; k++) {
     int n = l.color_off[k + 1] - l.color_off[k];
     FM_CHECK(fmk::ns_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldof, l.elpos, l.bmask, s->xyz,
-                              m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, l.coef, ex.d_tab, h, 0, 1, a.A.v,
-                              nullptr));
+                              m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, s->xbc, l.coef, ex.d_tab, h, 0, 1, 0,
+                              a.A.v, nullptr));
   }
   return after_assembly_T(s, L);
 }
@@ -1087,5 +1151,58 @@ extern "C" int fm_ns_grad_coef(fm_system *s, const fm_ns_params *p, const double
   dfree(adj);
   if (rc) return FM_EHIP;
   for (int e = 0; e < l.nel; e++) grad_host[e] = -h[e];
+  return FM_OK;
+}
+
+// Pressure drop between the faces x = lo[0] (inlet) and x = hi[0] (outlet) of
+// the top level: J = mean_inlet(p) - mean_outlet(p), face means of the Q1
+// pressure (bilinear on each face: weight area/4 per vertex in 3D, length/2
+// per vertex in 2D on the box's axis-aligned faces).  J is linear in U;
+// dJdU (reference numbering, zero on velocity DoFs) is returned with J.
+extern "C" int fm_ns_pressure_drop(fm_system *s, const double *U, double *J, double *dJdU) {
+  FM_CHECK(check_level(s, 0));
+  const fm_mesh &m = *s->mesh;
+  if (!U || s->nq2 != m.dim || s->nq1 != 1) return FM_EINVAL;
+  const int top = m.nlev - 1, N = m.Ntop, dim = m.dim, nvert = dim == 2 ? 2 : 4;
+  const std::vector<int> &dof = s->node_dof[top];
+  const int64_t nt = s->Dim[top];
+  std::vector<double> w(nt, 0.);
+  double area[2] = {0., 0.};
+  const double tol = 1e-9 * (m.hi[0] - m.lo[0]);
+  for (int e = 0; e < m.nel[top]; e++)
+    for (int sd = 0; sd < m.nsides; sd++) {
+      if (m.neigh[top][(size_t)e * m.nsides + sd] != -1) continue;
+      int nodes[4];
+      bool in = true, out = true;
+      for (int i = 0; i < nvert; i++) {
+        int ln = dim == 2 ? fm_face2d[sd][i] : fm_face3d[sd][i];
+        nodes[i] = m.conn[top][(size_t)e * m.n2 + ln];
+        double x = m.xyz[nodes[i]];
+        in = in && std::fabs(x - m.lo[0]) < tol;
+        out = out && std::fabs(x - m.hi[0]) < tol;
+      }
+      if (!in && !out) continue;
+      double a = 1.;
+      for (int d = 1; d < dim; d++) {
+        double lo = 1e300, hi = -1e300;
+        for (int i = 0; i < nvert; i++) { double x = m.xyz[nodes[i] + (size_t)d * N]; lo = std::min(lo, x); hi = std::max(hi, x); }
+        a *= hi - lo;
+      }
+      const int side = in ? 0 : 1;
+      area[side] += a;
+      for (int i = 0; i < nvert; i++) {
+        int pd = dof[nodes[i] + (size_t)dim * N];
+        if (pd < 0) { fm_set_error("pressure drop: face vertex without a pressure DoF"); return FM_ESTATE; }
+        w[pd] += (side == 0 ? 1. : -1.) * a / nvert;
+      }
+    }
+  if (area[0] <= 0. || area[1] <= 0.) { fm_set_error("pressure drop: no inlet or outlet face"); return FM_EINVAL; }
+  double acc = 0.;
+  for (int64_t i = 0; i < nt; i++) {
+    if (w[i] != 0.) w[i] /= (w[i] > 0. ? area[0] : area[1]);
+    acc += w[i] * U[i];
+  }
+  if (J) *J = acc;
+  if (dJdU) std::memcpy(dJdU, w.data(), nt * sizeof(double));
   return FM_OK;
 }

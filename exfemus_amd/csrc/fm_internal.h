@@ -130,6 +130,7 @@ struct fm_system {
   double *xyz = nullptr;                    // device coords
   double *x_old = nullptr;                  // top-level x_old (device)
   double *vel = nullptr;                    // heat advection field [dim*Ntop] or nullptr
+  double *xbc = nullptr;                    // Dirichlet data of the penalty RHS (nullptr: x_old)
   std::vector<Level> lev;
   // transposed hierarchy for the discrete adjoint: A^T (same pattern as A),
   // restriction Prl[L+1]^T, prolongation Rst[L-1]^T, own work vectors

@@ -74,12 +74,13 @@ int csr2sell(hipStream_t s, int64_t nslot, const int *map, const double *csr_v, 
 // assemble.hip
 int heat_assemble(hipStream_t s, int dim, int nel_color, const int *elems, const int *conn, const int *eldof,
                   const int *elpos, const unsigned char *bmask, const double *xyz, int Ntop, const int *dof_top,
-                  const int *bc0, const int *bc1, const double *x_old, const double *coef, const double *vel,
-                  const FeTab *tab, HeatArgs a, int mode, int trans, double *val, double *rhs);
+                  const int *bc0, const int *bc1, const double *x_old, const double *xbc, const double *coef,
+                  const double *vel, const FeTab *tab, HeatArgs a, int mode, int trans, int bd_only, double *val,
+                  double *rhs);
 int ns_assemble(hipStream_t s, int dim, int nel_color, const int *elems, const int *conn, const int *eldof,
                 const int *elpos, const unsigned char *bmask, const double *xyz, int Ntop, const int *dof_top,
-                const int *bc0, const int *bc1, const double *x_old, const double *coef,
-                const FeTab *tab, NsArgs a, int mode, int trans, double *val, double *rhs);
+                const int *bc0, const int *bc1, const double *x_old, const double *xbc, const double *coef,
+                const FeTab *tab, NsArgs a, int mode, int trans, int bd_only, double *val, double *rhs);
 int ns_visc(hipStream_t s, int dim, int nel, const int *elems, const int *conn, const int *eldof, const double *xyz,
             int Ntop, const int *dof_top, const int *bc0, const double *U, const double *lam, const double *coef,
             double nu0, const FeTab *tab, double *diss, double *adj, double *y, double ysc);

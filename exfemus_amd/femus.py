@@ -41,6 +41,7 @@ EXPORTS = [
     "fm_restrict", "fm_prolong", "fm_get_res", "fm_mg_setup", "fm_mg_solve", "fm_adjoint_solve",
     "fm_heat_grad_coef", "fm_bench_spmv", "fm_sync", "fm_heat_genmat_transpose", "fm_ns_genmat_transpose",
     "fm_get_adjoint_csr", "fm_heat_dissipation", "fm_bench_stream", "fm_ns_dissipation", "fm_ns_grad_coef",
+    "fm_system_set_bcdata", "fm_heat_bc_rhs", "fm_ns_bc_rhs", "fm_ns_pressure_drop",
 ]
 
 
@@ -131,6 +132,10 @@ def lib():
     L.fm_bench_stream.argtypes = [vp, C.c_int64, C.c_int, C.POINTER(C.c_double)]
     L.fm_ns_dissipation.argtypes = [vp, C.POINTER(NSParams), f64p, C.POINTER(C.c_double), vp, vp]
     L.fm_ns_grad_coef.argtypes = [vp, C.POINTER(NSParams), f64p, f64p, f64p]
+    L.fm_system_set_bcdata.argtypes = [vp, vp]
+    L.fm_heat_bc_rhs.argtypes = [vp, C.POINTER(HeatParams), f64p, f64p]
+    L.fm_ns_bc_rhs.argtypes = [vp, C.POINTER(NSParams), f64p, f64p]
+    L.fm_ns_pressure_drop.argtypes = [vp, f64p, C.POINTER(C.c_double), vp]
     _LIB = L
     return L
 
@@ -311,6 +316,21 @@ class MGSolDA:
     def get_res(self, L):
         r = np.zeros(self._Dim[L]); _ck(lib().fm_get_res(self.h, L, r)); return r
 
+    def set_bcdata(self, g):
+        """Dirichlet data of the penalty RHS (top-level DoFs); None = x_old (the reference's source)."""
+        if g is None:
+            _ck(lib().fm_system_set_bcdata(self.h, None))
+        else:
+            c = np.ascontiguousarray(g, np.float64)
+            _ck(lib().fm_system_set_bcdata(self.h, c.ctypes.data_as(C.c_void_p)))
+
+    def bc_rhs(self, g):
+        """Boundary-face penalty RHS of the top level for Dirichlet data g (linear in g)."""
+        out = np.zeros(self._Dim[-1])
+        fn = lib().fm_heat_bc_rhs if self.nq1 == 0 else lib().fm_ns_bc_rhs
+        _ck(fn(self.h, C.byref(self.params), np.ascontiguousarray(g, np.float64), out))
+        return out
+
     def set_elem_coef(self, L, coef):
         c = None if coef is None else np.ascontiguousarray(coef, np.float64)
         _ck(lib().fm_system_set_elem_coef(self.h, L, None if c is None else c.ctypes.data_as(C.c_void_p)))
@@ -440,6 +460,14 @@ class MGSolNS(MGSolDA):
         _ck(lib().fm_ns_dissipation(self.h, C.byref(self.params), np.ascontiguousarray(U, np.float64), C.byref(J),
                                     g.ctypes.data_as(C.c_void_p), dn.ctypes.data_as(C.c_void_p)))
         return J.value, g, dn
+
+    def pressure_drop(self, U):
+        """J = mean p(inlet x = lo) - mean p(outlet x = hi) and dJ/dU."""
+        J = C.c_double()
+        g = np.zeros(self._Dim[-1])
+        _ck(lib().fm_ns_pressure_drop(self.h, np.ascontiguousarray(U, np.float64), C.byref(J),
+                                      g.ctypes.data_as(C.c_void_p)))
+        return J.value, g
 
     def grad_coef(self, lam, U):
         """-lambda^T (dA/dnu_e) U for every top element."""

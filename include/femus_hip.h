@@ -111,6 +111,16 @@ typedef struct {
 int fm_system_set_elem_coef(fm_system *s, int L, const double *coef);
 /* nodal Q2 advection velocity for heat [dim*Ntop] (NULL = vel_const)       */
 int fm_system_set_velocity(fm_system *s, const double *vel_top);
+/* Dirichlet data of the boundary-face penalty RHS, top-level DoFs in the
+ * reference numbering.  The reference reads it from x_old
+ * (template_appl/nse/SRC/MGSolverNS.C:532-601, MGSolverT.C:401-438); a
+ * separate vector lets a parameter act on the boundary data alone.
+ * NULL = x_old again.                                                      */
+int fm_system_set_bcdata(fm_system *s, const double *g);
+/* The linear map g -> boundary-face penalty RHS of the top level (the RHS
+ * part of the same face loops), matrix untouched; reference numbering.     */
+int fm_heat_bc_rhs(fm_system *s, const fm_heat_params *p, const double *g, double *rhs);
+int fm_ns_bc_rhs(fm_system *s, const fm_ns_params *p, const double *g, double *rhs);
 /* MGSolT::GenMatRhs (template_appl/nse/SRC/MGSolverT.C:93-525)             */
 int fm_heat_genmatrhs(fm_system *s, int L, int mode, const fm_heat_params *p);
 /* MGSolNS::GenMatRhs (template_appl/nse/SRC/MGSolverNS.C:108-717)          */
@@ -188,6 +198,9 @@ int fm_ns_dissipation(fm_system *s, const fm_ns_params *p, const double *U_host,
 /* Adjoint term of dJ/dnu_e for the Oseen (Picard) system: -lambda^T (dA/dnu_e) U */
 int fm_ns_grad_coef(fm_system *s, const fm_ns_params *p, const double *lambda_host,
                     const double *U_host, double *grad_host);
+/* Pressure drop J = mean p(x = lo_x) - mean p(x = hi_x) on the top level
+ * (face means of the Q1 pressure) and dJ/dU (linear functional)            */
+int fm_ns_pressure_drop(fm_system *s, const double *U_host, double *J, double *dJdU_host);
 
 /* ---- timing helpers for benchmarks ------------------------------------- */
 /* Run `reps` SpMVs y = A[L] x on device buffers (prec 0 = FP64, 1 = FP32);

@@ -225,3 +225,172 @@ def test_ns_viscosity_gradient_fd(femus, ctx):
         fd = (J_of(up) - J_of(um)) / (2 * h * nu[top][e])
         assert abs(fd - grad[e]) <= 1e-5 * abs(grad[e]), (e, fd, grad[e])
     ns.set_elem_coef(top, nu[top])
+
+
+# ---------------------------------------------------------------------------
+# Cavity at Re ~ 400: per-element viscosity (log-normal around 0.0025) and 16 Fourier coefficients of the
+# lid velocity u(x) = sum_k a_k sin(k pi x); J = viscous dissipation of one Oseen step (advection frozen at
+# the lid profile).  Gradients from the GPU adjoint against central differences (h = 1e-6 relative, direct
+# solves), against a gradient built on the CPU from the oracle's assembly, and through torch.autograd.gradcheck.
+# ---------------------------------------------------------------------------
+def _cavity_adjoint(femus, ctx, nx=(4, 4), nlev=3, IRe=0.0025, K=16):
+    from exfemus_amd import adjoint
+    m = femus.Mesh(2, list(nx), [0, 0], [1, 1], nlev)
+    ns = femus.MGSolNS(ctx, m, IRe=IRe, dt=0.1, time_coeff=0.0, upwind=0.1)
+    ns.GenBc(functools.partial(configs.cavity_bc, dim=2))
+    ns.init()
+    G = adjoint.lid_fourier_basis(ns, K)
+    rng = np.random.default_rng(2026)
+    a0 = rng.standard_normal(K) / np.arange(1, K + 1)
+    ns.set_xold(a0 @ G)                       # advection field: the lid profile
+    ne = m.level(nlev - 1)[0]
+    nu = IRe * configs.lognormal_k(ne, seed=7, sigma=1.0)
+    opts = ns.opts(precision=3, smoother=femus.FM_SMOOTH_VANKA, vanka_colored=1, nu_pre=4, nu_post=4, tol=1e-13,
+                   max_iter=300)
+    prob = adjoint.OseenProblem(ns, G, functional="dissipation", opts=opts)
+    return m, ns, G, a0, nu, prob
+
+
+def test_cavity_lid_and_viscosity_gradient_fd(femus, ctx):
+    m, ns, G, a0, nu, prob = _cavity_adjoint(femus, ctx)
+    top = m.nlev - 1
+    prob.solve(nu, a0)
+    g_nu, g_a = prob.gradient()
+
+    def J_of(nu_t, a):
+        ns.set_elem_coef(top, nu_t)
+        ns.set_bcdata(a @ G)
+        ns.GenMatRhs(0.0, top, 1)
+        U = spla.spsolve(_csr(ns, top).tocsc(), ns.get_rhs())
+        return ns.dissipation(U)[0]
+
+    h = 1e-6
+    J0 = J_of(nu, a0)
+    # central differences carry the direct solve's rounding, ~eps |J| / (2 h nu_e |dJ/dnu_e|) relative: the
+    # checked elements are those with the largest relative sensitivity nu_e |dJ/dnu_e| / |J|
+    for e in np.argsort(-np.abs(g_nu * nu))[:4]:
+        up = nu.copy(); up[e] *= 1 + h
+        um = nu.copy(); um[e] *= 1 - h
+        fd = (J_of(up, a0) - J_of(um, a0)) / (2 * h * nu[e])
+        assert abs(fd - g_nu[e]) <= 1e-5 * abs(g_nu[e]), (e, fd, g_nu[e], J0)
+    for k in [0, 1, 3, 7]:
+        hk = h * max(abs(a0[k]), 1e-3)
+        ap = a0.copy(); ap[k] += hk
+        am = a0.copy(); am[k] -= hk
+        fd = (J_of(nu, ap) - J_of(nu, am)) / (2 * hk)
+        assert abs(fd - g_a[k]) <= 1e-5 * abs(g_a[k]), (k, fd, g_a[k])
+
+
+def test_cavity_gradient_vs_cpu_reference(femus, oracle, ctx):
+    """The same gradient assembled and solved on the CPU: oracle Oseen matrix and RHS, sparse direct solves,
+    element Laplacians and viscous blocks from the oracle's element routines."""
+    m, ns, G, a0, nu, prob = _cavity_adjoint(femus, ctx)
+    prob.solve(nu, a0)
+    g_nu, g_a = prob.gradient()
+    nx, nlev, dim = (4, 4), m.nlev, 2
+    top = nlev - 1
+    om = oracle.Mesh(2, list(nx), [0, 0], [1, 1], nlev)
+    d, n = oracle.init_dof(om, top, 2, 1)
+    bc = functools.partial(configs.cavity_bc, dim=2)
+    b0, b1 = oracle.genbc(om, 2, 1, d, bc)
+    rp, ci = oracle.system_pattern(om, top, 2, 1, d)
+    p = oracle.NSParams(0.0025, 0.1, 0.0, 0.1, 0.0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0))
+    xold = a0 @ G
+    v, rhs = oracle.ns_genmatrhs(om, top, 1, p, d, d, b0, b1, xold, rp, ci, nu_elem=nu)
+    A = sp.csr_matrix((v, ci, rp), shape=(n, n)).tocsc()
+    U = spla.spsolve(A, rhs)
+    # element Laplacians (scalar Q2) from the heat element routine with a pure-diffusion parameter set
+    dh, nh = oracle.init_dof(om, top, 1, 0)
+    node_of_h = np.empty(nh, np.int64)
+    Nn = om.N
+    node_of_h[dh[:Nn][dh[:Nn] >= 0]] = np.flatnonzero(dh[:Nn] >= 0)
+    ph = oracle.HeatParams(1.0, 1.0, 0.1, 0.0, 0.0, 0.0, (0.0, 0.0, 0.0))
+    ones = np.ones(nh, np.int32)
+    ne = len(nu)
+    nl = dim * 9 + 4
+    zero = np.zeros(n)
+    Kl, Dv, vd = [], [], []
+    for e in range(ne):
+        KeM = np.zeros(27 * 27); FeM = np.zeros(27); hd = np.zeros(27, np.int32)
+        oracle.lib().om_heat_element(om.h, top, e, 0, C.byref(ph), dh, ones, ones, np.zeros(nh), 1.0, None, KeM, FeM,
+                                     hd, dh.ctypes.data_as(C.c_void_p))
+        Kl.append(KeM[:81].reshape(9, 9))
+        nodes = node_of_h[hd[:9]]
+        vd.append(np.array([[d[nd + iv * Nn] for nd in nodes] for iv in range(dim)]))
+        M1 = np.zeros(89 * 89); M0 = np.zeros(89 * 89); F = np.zeros(89); dd = np.zeros(89, np.int32)
+        oracle.lib().om_ns_element(om.h, top, e, 0, C.byref(p), d, b0, b1, zero, 1.0, M1, F, dd,
+                                   d.ctypes.data_as(C.c_void_p))
+        oracle.lib().om_ns_element(om.h, top, e, 0, C.byref(p), d, b0, b1, zero, 0.0, M0, F, dd,
+                                   d.ctypes.data_as(C.c_void_p))
+        Dv.append(((M1 - M0)[:nl * nl].reshape(nl, nl)[:dim * 9, :dim * 9], dd[:dim * 9].copy()))
+    dJdU = np.zeros(n)
+    J = 0.0
+    for e in range(ne):
+        for iv in range(dim):
+            ue = U[vd[e][iv]]
+            J += nu[e] * ue @ Kl[e] @ ue
+            dJdU[vd[e][iv]] += 2 * nu[e] * Kl[e] @ ue
+    lam = spla.spsolve(A.T.tocsc(), dJdU)
+    gc_nu = np.zeros(ne)
+    for e in range(ne):
+        part = sum(U[vd[e][iv]] @ Kl[e] @ U[vd[e][iv]] for iv in range(dim))
+        Dm, dd = Dv[e]
+        gc_nu[e] = part - lam[dd] @ (Dm @ U[dd])
+    gc_a = np.zeros(len(a0))
+    for k in range(len(a0)):
+        _, rk = oracle.ns_genmatrhs(om, top, 1, p, d, d, b0, b1, G[k], rp, ci, nu_elem=nu)
+        gc_a[k] = lam @ rk
+    assert abs(ns.dissipation(U)[0] - J) <= 1e-10 * abs(J)
+    np.testing.assert_allclose(g_nu, gc_nu, rtol=1e-7, atol=1e-9 * np.abs(gc_nu).max())
+    np.testing.assert_allclose(g_a, gc_a, rtol=1e-7, atol=1e-9 * np.abs(gc_a).max())
+
+
+def test_cavity_gradcheck_torch(femus, ctx):
+    """torch.autograd.gradcheck of the autograd wrapper around the GPU adjoint (central differences,
+    eps = 1e-6) over 16 lid coefficients and three element viscosities."""
+    import torch
+    from exfemus_amd import adjoint
+    m, ns, G, a0, nu, prob = _cavity_adjoint(femus, ctx)
+    idx = torch.tensor([3, 40, 200])
+    nu0 = torch.from_numpy(nu.copy())
+
+    def f(nu_sub, a):
+        full = nu0.index_put((idx,), nu_sub)
+        return adjoint.OseenFunctional.apply(full, a, prob)
+
+    nu_sub = nu0[idx].clone().requires_grad_(True)
+    a = torch.from_numpy(a0.copy()).requires_grad_(True)
+    assert torch.autograd.gradcheck(f, (nu_sub, a), eps=1e-6, atol=1e-8, rtol=1e-5, fast_mode=False)
+
+
+def test_channel_pressure_drop_gradient_fd(femus, ctx):
+    """Pressure drop inlet-outlet of the 3D channel: gradient w.r.t. element viscosities and the inflow
+    amplitude from the GPU adjoint against central differences."""
+    from exfemus_amd import adjoint
+    m, ns, nu = _channel(femus, ctx)
+    top = 1
+    x0 = ns.GenIc(configs.channel_ic)          # inflow profile on the inflow u-DoFs, zero elsewhere
+    ns.set_xold(x0)
+    G = x0[None, :].copy()                     # one parameter: the inflow amplitude
+    opts = ns.opts(precision=3, smoother=femus.FM_SMOOTH_VANKA, vanka_colored=1, nu_pre=4, nu_post=4, tol=1e-13,
+                   max_iter=300)
+    prob = adjoint.OseenProblem(ns, G, functional="pressure_drop", opts=opts)
+    a0 = np.array([1.0])
+    J0 = prob.solve(nu[top], a0)
+    g_nu, g_a = prob.gradient()
+
+    def J_of(nu_t, a):
+        ns.set_elem_coef(top, nu_t)
+        ns.set_bcdata(a @ G)
+        ns.GenMatRhs(0.0, top, 1)
+        U = spla.spsolve(_csr(ns, top).tocsc(), ns.get_rhs())
+        return ns.pressure_drop(U)[0]
+
+    h = 1e-6
+    for e in [0, 9, 64, len(nu[top]) - 1]:
+        up = nu[top].copy(); up[e] *= 1 + h
+        um = nu[top].copy(); um[e] *= 1 - h
+        fd = (J_of(up, a0) - J_of(um, a0)) / (2 * h * nu[top][e])
+        assert abs(fd - g_nu[e]) <= 1e-5 * abs(g_nu[e]) + 1e-9 * abs(J0), (e, fd, g_nu[e])
+    fd = (J_of(nu[top], a0 + h) - J_of(nu[top], a0 - h)) / (2 * h)
+    assert abs(fd - g_a[0]) <= 1e-5 * abs(g_a[0])
