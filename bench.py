@@ -134,6 +134,84 @@ def build_ns(F, ctx, workload):
     return m, s, time.perf_counter() - t0
 
 
+def adjoint_workload(F, ctx, args):
+    """C4: 2D lid-driven cavity 128^2 quad9 Q2-Q1 (16^2 coarse, 4 levels), Re 400: per-element viscosity
+    log-normal around 0.0025 (sigma 1, seeded) and 16 Fourier coefficients of the lid profile; J = viscous
+    dissipation of one Oseen step.  Times the forward multigrid solve, the transposed multigrid solve and the
+    gradient assembly; checks components against central differences (h = 1e-6, sparse direct solves)."""
+    import functools
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    from exfemus_amd import adjoint, configs
+    m = F.Mesh(2, [16, 16], [0, 0], [1, 1], 4)
+    ns = F.MGSolNS(ctx, m, IRe=0.0025, dt=0.1, time_coeff=0.0, upwind=0.1)
+    ns.GenBc(functools.partial(configs.cavity_bc, dim=2))
+    ns.init()
+    K = 16
+    G = adjoint.lid_fourier_basis(ns, K)
+    rng = np.random.default_rng(2026)
+    a0 = rng.standard_normal(K) / np.arange(1, K + 1)
+    ns.set_xold(a0 @ G)
+    top = 3
+    ne = m.level(top)[0]
+    nu = 0.0025 * configs.lognormal_k(ne, seed=7, sigma=1.0)
+    o = ns.opts(precision=F.FM_PREC_FP64_FGMRES, smoother=F.FM_SMOOTH_VANKA, vanka_colored=1, nu_pre=4, nu_post=4,
+                tol=1e-12, max_iter=400)
+    prob = adjoint.OseenProblem(ns, G, functional="dissipation", opts=o)
+    J = prob.solve(nu, a0)
+    g_nu, g_a = prob.gradient()
+    t_fwd, t_adj, t_grad = [], [], []
+    for _ in range(args.steps):
+        ns.set_x(np.zeros(ns._Dim[top]))
+        r = ns.MGSolve(opts=o)
+        t_fwd.append(r.seconds)
+        U = ns.get_x()
+        _, dJdU, dJdnu = ns.dissipation(U)
+        lam, ra = ns.AdjointSolve(dJdU, o)
+        t_adj.append(ra.seconds)
+        t0 = time.perf_counter()
+        gn = dJdnu + ns.grad_coef(lam, U)
+        ga = np.array([lam @ ns.bc_rhs(Gk) for Gk in G])
+        t_grad.append(time.perf_counter() - t0)
+
+    def J_of(nu_t, a):
+        ns.set_elem_coef(top, nu_t)
+        ns.set_bcdata(a @ G)
+        ns.GenMatRhs(0.0, top, 1)
+        rp, ci, v = ns.csr(0, top)
+        n = len(rp) - 1
+        Uk = spla.spsolve(sp.csr_matrix((v, ci, rp), shape=(n, n)).tocsc(), ns.get_rhs())
+        return ns.dissipation(Uk)[0]
+
+    h = 1e-6
+    errs_nu, errs_a = [], []
+    for e in np.argsort(-np.abs(g_nu * nu))[:4]:
+        up = nu.copy(); up[e] *= 1 + h
+        um = nu.copy(); um[e] *= 1 - h
+        fd = (J_of(up, a0) - J_of(um, a0)) / (2 * h * nu[e])
+        errs_nu.append(abs(fd - g_nu[e]) / abs(g_nu[e]))
+    for k in (0, 1, 3, 7, 15):
+        hk = h * max(abs(a0[k]), 1e-3)
+        ap = a0.copy(); ap[k] += hk
+        am = a0.copy(); am[k] -= hk
+        fd = (J_of(nu, ap) - J_of(nu, am)) / (2 * hk)
+        errs_a.append(abs(fd - g_a[k]) / abs(g_a[k]))
+    t_total = float(np.median(t_fwd) + np.median(t_adj) + np.median(t_grad))
+    N = ns._Dim[top]
+    out = {"metric": "adjoint gradient of the dissipation w.r.t. 16384 element viscosities + 16 lid coefficients",
+           "value": round(t_total * 1e3, 3), "unit": "ms per gradient (forward MG + adjoint MG + gradient)",
+           "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": round(t_total * 1e3, 3),
+           "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (seeded log-normal viscosity, seeded lid coefficients)",
+           "config": {"workload": "2D NS cavity quad9 Q2-Q1 128^2, 4 levels, Re 400 (nu 0.0025 log-normal), "
+                                  "16 lid Fourier coefficients, J = viscous dissipation of one Oseen step",
+                      "n_dof_top": N, "elements_top": ne, "smoother": "coloured element Vanka 4/4, FP64 FGMRES to 1e-12"},
+           "J": J, "forward_ms": round(float(np.median(t_fwd)) * 1e3, 3), "adjoint_ms": round(float(np.median(t_adj)) * 1e3, 3),
+           "gradient_ms": round(float(np.median(t_grad)) * 1e3, 3),
+           "fd_rel_err_nu_max": float(max(errs_nu)), "fd_rel_err_lid_max": float(max(errs_a)), "fd_h": h}
+    return out
+
+
 def field_ids(s):
     """Variable index of every top-level DoF (0..nq2-1 velocity components / T, nq2 = pressure)."""
     top = s._NoLevels - 1
@@ -151,7 +229,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="heat64", choices=["heat64", "channel", "cavity"])
+    ap.add_argument("--workload", default="heat64", choices=["heat64", "channel", "cavity", "cavity-adjoint"])
     ap.add_argument("--nx0", type=int, default=4)
     ap.add_argument("--nlev", type=int, default=5)
     ap.add_argument("--tol", type=float, default=1e-8)
@@ -167,6 +245,13 @@ def main():
         sys.exit(2)
     from exfemus_amd import femus as F
     ctx = F.Context(0)
+    if args.workload == "cavity-adjoint":
+        out = adjoint_workload(F, ctx, args)
+        for sysm in list(ctx._systems):
+            sysm.close()
+        ctx.close()
+        print(json.dumps(out))
+        return
     t0 = time.perf_counter()
     heat = args.workload == "heat64"
     if heat:
