@@ -113,7 +113,7 @@ struct fm_ctx {
 
 // a captured V-cycle (one per precision / option set / forward-or-adjoint)
 struct GraphEntry {
-  int prec, adj, smoother, nu_pre, nu_post, correction, gamma, colored;
+  int prec, adj, xz, smoother, nu_pre, nu_post, correction, gamma, colored;
   double lo, hi, omega;
   hipGraphExec_t exec;
 };

@@ -24,8 +24,10 @@
   } while (0)
 
 int fm_vanka_setup(fm_system *s, int L, std::vector<struct LOps> *ops);
-int fm_vanka_sweep64(fm_system *s, int L, const struct LOps &o, double *x, const double *b, int its, double omega, int colored);
-int fm_vanka_sweep32(fm_system *s, int L, const struct LOps &o, float *x, const float *b, int its, double omega, int colored);
+int fm_vanka_sweep64(fm_system *s, int L, const struct LOps &o, double *x, const double *b, int its, double omega,
+                     int colored, bool xz);
+int fm_vanka_sweep32(fm_system *s, int L, const struct LOps &o, float *x, const float *b, int its, double omega,
+                     int colored, bool xz);
 
 // Per-level operator set used by the V-cycle (forward or transposed).
 struct LOps {
@@ -174,8 +176,9 @@ static std::vector<LOps> forward_ops(fm_system *s) {
 }
 
 // ---- smoothers ----------------------------------------------------------
+// xz: x is zero on entry, so the first residual is b itself (a copy instead of a product)
 template <typename T>
-static int smooth(fm_system *s, const fm_mg_opts *opt, int L, const LOps &o, int its) {
+static int smooth(fm_system *s, const fm_mg_opts *opt, int L, const LOps &o, int its, bool xz) {
   hipStream_t st = s->ctx->stream;
   Level *l = o.v;
   T *x = Vec<T>::x(l), *b = Vec<T>::b(l), *r = Vec<T>::r(l), *d = Vec<T>::d(l), *d2 = Vec<T>::d2(l);
@@ -185,7 +188,11 @@ static int smooth(fm_system *s, const fm_mg_opts *opt, int L, const LOps &o, int
   if (opt->smoother == FM_SMOOTH_CHEBYSHEV) {
     double lo = opt->cheb_lo * o.lam, hi = opt->cheb_hi * o.lam;
     double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta, rho = 1. / sigma;
-    FM_CHECK(fmk::resid(st, A, b, x, r));
+    if (xz) {
+      if (hipMemcpyAsync(r, b, o.n * sizeof(T), hipMemcpyDeviceToDevice, st) != hipSuccess) return FM_EHIP;
+    } else {
+      FM_CHECK(fmk::resid(st, A, b, x, r));
+    }
     FM_CHECK(fmk::cheb_first<T>(st, o.n, dinv, r, d, x, (T)(1.0 / theta)));
     for (int k = 2; k <= its; k++) {
       double rho_new = 1. / (2. * sigma - rho);
@@ -206,15 +213,17 @@ static int smooth(fm_system *s, const fm_mg_opts *opt, int L, const LOps &o, int
     return 0;
   }
   if (opt->smoother == FM_SMOOTH_VANKA) {
-    if (sizeof(T) == 8) return fm_vanka_sweep64(s, L, o, (double *)x, (const double *)b, its, opt->omega, opt->vanka_colored);
-    return fm_vanka_sweep32(s, L, o, (float *)x, (const float *)b, its, opt->omega, opt->vanka_colored);
+    if (sizeof(T) == 8)
+      return fm_vanka_sweep64(s, L, o, (double *)x, (const double *)b, its, opt->omega, opt->vanka_colored, xz);
+    return fm_vanka_sweep32(s, L, o, (float *)x, (const float *)b, its, opt->omega, opt->vanka_colored, xz);
   }
   return FM_EINVAL;
 }
 
 // ---- V-cycle (MGSolBase::MGStep) -----------------------------------------
+// xz: x[L] is zero on entry (every coarse level; the top level of a correction cycle)
 template <typename T>
-static int vcycle(fm_system *s, const fm_mg_opts *opt, std::vector<LOps> &ops, int L) {
+static int vcycle(fm_system *s, const fm_mg_opts *opt, std::vector<LOps> &ops, int L, bool xz) {
   hipStream_t st = s->ctx->stream;
   const LOps &o = ops[L];
   Level *l = o.v;
@@ -224,7 +233,7 @@ static int vcycle(fm_system *s, const fm_mg_opts *opt, std::vector<LOps> &ops, i
   }
   int top = (int)ops.size() - 1;
   int mult = L < top ? 2 : 1;
-  FM_CHECK(smooth<T>(s, opt, L, o, opt->nu_pre * mult));
+  FM_CHECK(smooth<T>(s, opt, L, o, opt->nu_pre * mult, xz));
   CsrView<T> A = cview<T>(*o.A);
   T *x = Vec<T>::x(l), *b = Vec<T>::b(l), *r = Vec<T>::r(l);
   FM_CHECK(fmk::resid(st, A, b, x, r));
@@ -240,7 +249,7 @@ static int vcycle(fm_system *s, const fm_mg_opts *opt, std::vector<LOps> &ops, i
   FM_CHECK(fmk::spmv(st, cview<T>(*oc.R), r, Vec<T>::b(lc)));
   FM_CHECK(fmk::fill<T>(st, oc.n, Vec<T>::x(lc), (T)0));
   const int gamma = (L - 1 > 0 && opt->gamma > 1) ? opt->gamma : 1;
-  for (int g = 0; g < gamma; g++) FM_CHECK(vcycle<T>(s, opt, ops, L - 1));
+  for (int g = 0; g < gamma; g++) FM_CHECK(vcycle<T>(s, opt, ops, L - 1, g == 0));
   double alpha = 1.0;
   if (opt->correction == 0) {
     // coarse_rest: relative residual of the coarse level after its cycle
@@ -253,25 +262,25 @@ static int vcycle(fm_system *s, const fm_mg_opts *opt, std::vector<LOps> &ops, i
     alpha = 0.95;
   }
   FM_CHECK(fmk::addmul(st, cview<T>(*o.P), Vec<T>::x(lc), x, (T)alpha, cond));
-  FM_CHECK(smooth<T>(s, opt, L, o, opt->nu_post * mult));
+  FM_CHECK(smooth<T>(s, opt, L, o, opt->nu_post * mult, false));
   return 0;
 }
 
 // ---- top-level cycle, replayed from a captured HIP graph -------------------
 template <typename T>
-static int cycle_top(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &ops, int adj) {
+static int cycle_top(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &ops, int adj, bool xz) {
   const int top = (int)ops.size() - 1;
-  if (!o->use_graph) return vcycle<T>(s, o, ops, top);
+  if (!o->use_graph) return vcycle<T>(s, o, ops, top, xz);
   const int prec = sizeof(T) == 8 ? 64 : 32;
   hipStream_t st = s->ctx->stream;
   for (auto &g : s->graphs)
-    if (g.prec == prec && g.adj == adj && g.smoother == o->smoother && g.nu_pre == o->nu_pre && g.nu_post == o->nu_post &&
+    if (g.prec == prec && g.adj == adj && g.xz == (int)xz && g.smoother == o->smoother && g.nu_pre == o->nu_pre && g.nu_post == o->nu_post &&
         g.correction == o->correction && g.gamma == o->gamma && g.colored == o->vanka_colored && g.lo == o->cheb_lo &&
         g.hi == o->cheb_hi && g.omega == o->omega)
       return hipGraphLaunch(g.exec, st) == hipSuccess ? 0 : FM_EHIP;
   hipGraph_t graph = nullptr;
   if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return FM_EHIP;
-  int rc = vcycle<T>(s, o, ops, top);
+  int rc = vcycle<T>(s, o, ops, top, xz);
   hipError_t e = hipStreamEndCapture(st, &graph);
   if (rc || e != hipSuccess) {
     if (graph) (void)hipGraphDestroy(graph);
@@ -279,7 +288,7 @@ static int cycle_top(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &ops, 
     return rc ? rc : FM_EHIP;
   }
   GraphEntry g;
-  g.prec = prec; g.adj = adj; g.smoother = o->smoother; g.nu_pre = o->nu_pre; g.nu_post = o->nu_post;
+  g.prec = prec; g.adj = adj; g.xz = (int)xz; g.smoother = o->smoother; g.nu_pre = o->nu_pre; g.nu_post = o->nu_post;
   g.correction = o->correction; g.gamma = o->gamma; g.colored = o->vanka_colored; g.lo = o->cheb_lo; g.hi = o->cheb_hi;
   g.omega = o->omega;
   e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
@@ -346,12 +355,12 @@ static int solve_stationary(fm_system *s, const fm_mg_opts *o, std::vector<LOps>
   rel = dev_dot<double>(c, ot.n, l->res, l->res, true) / bn;
   while (it < o->max_iter && rel > o->tol) {
     if (o->precision == FM_PREC_FP64) {
-      FM_CHECK(cycle_top<double>(s, o, ops, adj));
+      FM_CHECK(cycle_top<double>(s, o, ops, adj, false));
     } else {
       // FP32 correction: A e = r (FP32 V-cycle from e = 0), x += e
       FM_CHECK(fmk::to_f32(st, ot.n, l->res, l->bf, nullptr));
       FM_CHECK(fmk::fill<float>(st, ot.n, l->xf, 0.f));
-      FM_CHECK(cycle_top<float>(s, o, ops, adj));
+      FM_CHECK(cycle_top<float>(s, o, ops, adj, true));
       FM_CHECK(fmk::add_f32(st, ot.n, l->x, l->xf, nullptr));
     }
     FM_CHECK(fmk::resid(st, A, l->b, l->x, l->res));
@@ -460,14 +469,14 @@ static int solve_fgmres(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &op
           if (hipMemcpyAsync(l->b, vj, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
           if (hipMemcpyAsync(l->sx, l->x, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
           if ((rc = fmk::fill<double>(st, n, l->x, 0.0)) != 0) goto done;
-          if ((rc = cycle_top<double>(s, o, ops, adj)) != 0) goto done;
+          if ((rc = cycle_top<double>(s, o, ops, adj, true)) != 0) goto done;
           if (hipMemcpyAsync(zj, l->x, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
           if (hipMemcpyAsync(l->x, l->sx, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
           if (hipMemcpyAsync(l->b, l->sb, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
         } else {
           if ((rc = fmk::to_f32(st, n, vj, l->bf, nullptr)) != 0) goto done;
           if ((rc = fmk::fill<float>(st, n, l->xf, 0.f)) != 0) goto done;
-          if ((rc = cycle_top<float>(s, o, ops, adj)) != 0) goto done;
+          if ((rc = cycle_top<float>(s, o, ops, adj, true)) != 0) goto done;
           if ((rc = fmk::fill<double>(st, n, zj, 0.0)) != 0) goto done;
           if ((rc = fmk::add_f32(st, n, zj, l->xf, nullptr)) != 0) goto done;
         }
@@ -544,9 +553,9 @@ extern "C" int fm_mg_solve(fm_system *s, const fm_mg_opts *o, fm_mg_report *rep)
 // ---- Vanka (vanka.hip) -----------------------------------------------------
 int fm_vanka_invert(fm_system *s, int L, const double *Av, double **store);
 int fm_vanka_apply64(fm_system *s, int L, const CsrView<double> &A, const double *vinv, double *x, const double *b,
-                     double *r, int its, double omega, int colored);
+                     double *r, int its, double omega, int colored, bool xz);
 int fm_vanka_apply32(fm_system *s, int L, const CsrView<float> &A, const double *vinv, float *x, const float *b,
-                     float *r, int its, double omega, int colored);
+                     float *r, int its, double omega, int colored, bool xz);
 int fm_vanka_setup(fm_system *s, int L, std::vector<LOps> *ops) {
   LOps &o = (*ops)[L];
   int rc = fm_vanka_invert(s, L, o.A->v, &o.v->vinv);
@@ -554,11 +563,13 @@ int fm_vanka_setup(fm_system *s, int L, std::vector<LOps> *ops) {
   o.vinv = o.v->vinv;
   return 0;
 }
-int fm_vanka_sweep64(fm_system *s, int L, const LOps &o, double *x, const double *b, int its, double omega, int colored) {
-  return fm_vanka_apply64(s, L, cview<double>(*o.A), o.vinv, x, b, o.v->r, its, omega, colored);
+int fm_vanka_sweep64(fm_system *s, int L, const LOps &o, double *x, const double *b, int its, double omega, int colored,
+                     bool xz) {
+  return fm_vanka_apply64(s, L, cview<double>(*o.A), o.vinv, x, b, o.v->r, its, omega, colored, xz);
 }
-int fm_vanka_sweep32(fm_system *s, int L, const LOps &o, float *x, const float *b, int its, double omega, int colored) {
-  return fm_vanka_apply32(s, L, cview<float>(*o.A), o.vinv, x, b, o.v->rf, its, omega, colored);
+int fm_vanka_sweep32(fm_system *s, int L, const LOps &o, float *x, const float *b, int its, double omega, int colored,
+                     bool xz) {
+  return fm_vanka_apply32(s, L, cview<float>(*o.A), o.vinv, x, b, o.v->rf, its, omega, colored, xz);
 }
 
 // Transposed multigrid: A^T lambda = g on the adjoint hierarchy (A^T from the

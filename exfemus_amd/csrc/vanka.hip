@@ -175,24 +175,33 @@ static int vanka_invert_level(fm_system *s, int L, const double *Av, double **st
   return 0;
 }
 
+// xz: x is zero on entry, so the first residual is b (copied, no product)
 template <typename T>
 static int vanka_sweep(fm_system *s, int L, const CsrView<T> &A, const double *vinv, T *x, const T *b, T *r, int its,
-                       double omega, int colored) {
+                       double omega, int colored, bool xz) {
   Level &l = s->lev[L];
   hipStream_t st = s->ctx->stream;
   // additive: one residual per sweep, damped and weighted by 1/multiplicity;
   // coloured: residual refreshed before every colour, undamped weights 1
   const double *wt = l.vwt;
+  bool fresh = xz;  // x still zero: the residual is b
+  auto residual = [&]() -> int {
+    if (fresh) {
+      fresh = false;
+      return hipMemcpyAsync(r, b, l.n * sizeof(T), hipMemcpyDeviceToDevice, st) == hipSuccess ? 0 : FM_EHIP;
+    }
+    return fmk::resid(st, A, b, x, r);
+  };
   for (int it = 0; it < its; it++) {
     if (!colored) {
-      int rc = fmk::resid(st, A, b, x, r);
+      int rc = residual();
       if (rc) return rc;
     }
     for (int c = 0; c < l.ncolor; c++) {
       int n = l.color_off[c + 1] - l.color_off[c];
       if (n <= 0) continue;
       if (colored) {
-        int rc = fmk::resid(st, A, b, x, r);
+        int rc = residual();
         if (rc) return rc;
       }
       unsigned g = (unsigned)(n < 8192 ? n : 8192);
@@ -210,10 +219,10 @@ static int vanka_sweep(fm_system *s, int L, const CsrView<T> &A, const double *v
 
 int fm_vanka_invert(fm_system *s, int L, const double *Av, double **store) { return vanka_invert_level(s, L, Av, store); }
 int fm_vanka_apply64(fm_system *s, int L, const CsrView<double> &A, const double *vinv, double *x, const double *b,
-                     double *r, int its, double omega, int colored) {
-  return vanka_sweep<double>(s, L, A, vinv, x, b, r, its, omega, colored);
+                     double *r, int its, double omega, int colored, bool xz) {
+  return vanka_sweep<double>(s, L, A, vinv, x, b, r, its, omega, colored, xz);
 }
 int fm_vanka_apply32(fm_system *s, int L, const CsrView<float> &A, const double *vinv, float *x, const float *b,
-                     float *r, int its, double omega, int colored) {
-  return vanka_sweep<float>(s, L, A, vinv, x, b, r, its, omega, colored);
+                     float *r, int its, double omega, int colored, bool xz) {
+  return vanka_sweep<float>(s, L, A, vinv, x, b, r, its, omega, colored, xz);
 }
