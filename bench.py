@@ -38,7 +38,9 @@ def build_heat(F, ctx, nx0, nlev, k_sigma=1.0):
     s.init()
     top = nlev - 1
     k_top = configs.lognormal_k(m.level(top)[0], sigma=k_sigma)
-    ks = configs.coarsen_coef(k_top, nlev, 8)
+    # coarse levels: geometric mean of the children (log-normal field); it shapes only the preconditioner
+    # (11 instead of 16 FGMRES iterations against the arithmetic mean at 64^3)
+    ks = configs.coarsen_coef(k_top, nlev, 8, mean="geometric")
     for L in range(nlev):
         s.set_elem_coef(L, ks[L])
     s.set_xold(np.zeros(s._Dim[top]))
@@ -69,7 +71,7 @@ def spmv_bytes(n, nnz, v):
 
 
 def cpu_baseline(budget_s=25.0):
-    """Oracle (restated reference CPU path) on a bounded heat sample, 1 core."""
+    """Oracle (restated reference CPU path) on a bounded heat sample, 1 core (coarse k: arithmetic mean)."""
     from oracle import oracle as O
     from exfemus_amd import configs
     nx0, nlev = [4, 4, 4], 4
@@ -255,13 +257,13 @@ def main():
     t0 = time.perf_counter()
     heat = args.workload == "heat64"
     if heat:
-        nu = args.nu or 3
+        nu = args.nu or 4
         m, s, t_asm = build_heat(F, ctx, [args.nx0] * 3, args.nlev)
         kw = dict(nu_pre=nu, nu_post=nu, cheb_lo=args.cheb_lo)
         smoother = f"Chebyshev-Jacobi {nu}/{nu}"
         workload = (f"3D heat hex27 {args.nx0 * 16}^3, {args.nlev} levels, lognormal k, "
                     f"FP32-MG+FP64-{args.outer.upper()} to {args.tol:g}")
-        data = "synthetic (seeded log-normal k, f = 1)"
+        data = "synthetic (seeded log-normal k, f = 1; coarse-level k = geometric mean of the children)"
     else:
         nu = args.nu or 4
         m, s, t_asm = build_ns(F, ctx, args.workload)

@@ -72,11 +72,18 @@ def lognormal_k(n_top_elem, seed=20261203, mu=0.0, sigma=1.0):
     return np.exp(mu + sigma * rng.standard_normal(n_top_elem))
 
 
-def coarsen_coef(k_top, nlev, nch):
-    """Per-level coefficients: each coarse element takes the mean of its children (children are stored
-    parent by parent, so level L+1 element e has parent e // nch)."""
+def coarsen_coef(k_top, nlev, nch, mean="arithmetic"):
+    """Per-level coefficients: each coarse element takes the mean (arithmetic, geometric or harmonic) of its
+    children (children are stored parent by parent, so level L+1 element e has parent e // nch).  Coarse
+    coefficients shape only the multigrid preconditioner, not the solution."""
     ks = [None] * nlev
     ks[nlev - 1] = np.asarray(k_top, np.float64)
     for L in range(nlev - 2, -1, -1):
-        ks[L] = ks[L + 1].reshape(-1, nch).mean(axis=1)
+        c = ks[L + 1].reshape(-1, nch)
+        if mean == "geometric":
+            ks[L] = np.exp(np.log(c).mean(axis=1))
+        elif mean == "harmonic":
+            ks[L] = 1.0 / (1.0 / c).mean(axis=1)
+        else:
+            ks[L] = c.mean(axis=1)
     return ks

@@ -202,6 +202,23 @@ static int smooth(fm_system *s, const fm_mg_opts *opt, int L, const LOps &o, int
     }
     return 0;
   }
+  if (opt->smoother == FM_SMOOTH_CHEBYSHEV4) {
+    // d_0 = 4/(3 rho) D^-1 r_0; r_k = r_{k-1} - A d_{k-1};
+    // d_k = (2k-1)/(2k+3) d_{k-1} + (8k+4)/((2k+3) rho) D^-1 r_k; x += d_k   (rho = cheb_hi * lambda_max)
+    const double rho = opt->cheb_hi * o.lam;
+    if (xz) {
+      if (hipMemcpyAsync(r, b, o.n * sizeof(T), hipMemcpyDeviceToDevice, st) != hipSuccess) return FM_EHIP;
+    } else {
+      FM_CHECK(fmk::resid(st, A, b, x, r));
+    }
+    FM_CHECK(fmk::cheb_first<T>(st, o.n, dinv, r, d, x, (T)(4.0 / (3.0 * rho))));
+    for (int k = 1; k < its; k++) {
+      const double c1 = (2.0 * k - 1.0) / (2.0 * k + 3.0), c2 = (8.0 * k + 4.0) / ((2.0 * k + 3.0) * rho);
+      FM_CHECK(fmk::cheb_step(st, A, r, x, d, d2, dinv, (T)c1, (T)c2));
+      std::swap(d, d2);
+    }
+    return 0;
+  }
   if (opt->smoother == FM_SMOOTH_JACOBI) {
     // ping-pong through t: x_{k+1} = x_k + omega D^-1 (b - A x_k)
     T *t = Vec<T>::t(l);
@@ -319,7 +336,8 @@ extern "C" int fm_mg_setup(fm_system *s, const fm_mg_opts *o) {
 static int mg_setup_ops(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &ops, bool adjoint) {
   (void)adjoint;
   for (size_t L = 1; L < ops.size(); L++) {
-    if (o->smoother == FM_SMOOTH_CHEBYSHEV && ops[L].lam == 0.) FM_CHECK(lambda_max(s->ctx, ops[L], 20, &ops[L].lam));
+    if ((o->smoother == FM_SMOOTH_CHEBYSHEV || o->smoother == FM_SMOOTH_CHEBYSHEV4) && ops[L].lam == 0.)
+      FM_CHECK(lambda_max(s->ctx, ops[L], 20, &ops[L].lam));
     if (o->smoother == FM_SMOOTH_VANKA) FM_CHECK(fm_vanka_setup(s, (int)L, &ops));
   }
   FM_CHECK(coarse_inverse(s, *ops[0].A, ops[0].v->h_rank, &ops[0].Ainv, &ops[0].Ainvf));

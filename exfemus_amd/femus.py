@@ -24,7 +24,7 @@ f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 BCFN = C.CFUNCTYPE(None, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_void_p)
 
 FM_PREC_FP64, FM_PREC_FP32_IR, FM_PREC_FP32_FGMRES, FM_PREC_FP64_FGMRES = 0, 1, 2, 3
-FM_SMOOTH_CHEBYSHEV, FM_SMOOTH_JACOBI, FM_SMOOTH_VANKA = 0, 1, 2
+FM_SMOOTH_CHEBYSHEV, FM_SMOOTH_JACOBI, FM_SMOOTH_VANKA, FM_SMOOTH_CHEBYSHEV4 = 0, 1, 2, 3
 FM_ENOCONV = -3
 
 # storage format of the SELL SpMV (names the layout the PMC traffic figures were measured on)

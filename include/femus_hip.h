@@ -144,6 +144,9 @@ int fm_get_res(const fm_system *s, int L, double *r);
 #define FM_SMOOTH_CHEBYSHEV 0
 #define FM_SMOOTH_JACOBI 1
 #define FM_SMOOTH_VANKA 2
+/* fourth-kind Chebyshev polynomial in D^-1 A (Lottes, "Optimal polynomial
+ * smoothers for multigrid V-cycles", 2023): needs only lambda_max            */
+#define FM_SMOOTH_CHEBYSHEV4 3
 typedef struct {
   int precision;      /* FM_PREC_*                                          */
   int smoother;       /* FM_SMOOTH_*                                        */
