@@ -31,6 +31,7 @@ __device__ inline double jac_qp(int qp, const double *xx, const double *dref, do
   const int off = N2 * NG;
   if (DIM == 2) {
     double x_xi = 0., x_eta = 0., y_xi = 0., y_eta = 0.;
+#pragma unroll 3
     for (int s = 0; s < N2; s++) {
       int sng = s * NG + qp;
       double a = dref[sng], b = dref[sng + off];
@@ -41,6 +42,7 @@ __device__ inline double jac_qp(int qp, const double *xx, const double *dref, do
     return det;
   } else {
     double x_xi = 0., x_eta = 0., x_zeta = 0., y_xi = 0., y_eta = 0., y_zeta = 0., z_xi = 0., z_eta = 0., z_zeta = 0.;
+#pragma unroll 3
     for (int s = 0; s < N2; s++) {
       int sng = s * NG + qp;
       double a = dref[sng], b = dref[sng + off], c = dref[sng + 2 * off];
@@ -87,25 +89,39 @@ __device__ inline double jac_sur(int qp, const double *xb, const FeTab *tab) {
 
 // ===========================================================================
 // Heat volume kernel
+//
+// One 128-thread workgroup per element (grid-stride over one colour).
+//  1. 27 (9) Gauss-point threads: Jacobian, its inverse, JxW, T_old and the
+//     advection velocity at the point, the diffusion weights
+//     w_qd = JxW (alpha0 k_e + f_up v_d^2) (MGSolverT.C:228-330).
+//  2. All threads: global gradients G[q][d][i], a[q][i] = JxW phi_i and
+//     b[q][j] = tcdt phi_j + rhocp v.grad phi_j into LDS.
+//  3. The element matrix is the rank-(DIM+1)*NG product
+//        K_ij = sum_q ( sum_d w_qd G_qdi G_qdj + a_qi b_qj )
+//     computed by 81 threads, each owning a 3x3 tile (i, i+9, i+18) x
+//     (j, j+9, j+18) of the 27x27 hex27 matrix (1x1 of the 9x9 quad9 one):
+//     6 LDS reads feed 9 FMAs per (q, d), against 15 reads per FMA pair when
+//     each thread owns one entry.
+//  4. Entries of Dirichlet rows are dropped (bc_vol row mask) and the rest
+//     are added to A through the element->CSR position map.
 // ===========================================================================
 template <int DIM>
-__global__ __launch_bounds__(256) void k_heat_vol(int nel, const int *__restrict__ elems, const int *__restrict__ conn,
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) void k_heat_vol(int nel, const int *__restrict__ elems, const int *__restrict__ conn,
                                                   const int *__restrict__ eldof, const int *__restrict__ elpos,
                                                   const double *__restrict__ xyz, int Ntop, const int *__restrict__ dof_top,
                                                   const int *__restrict__ bc0, const double *__restrict__ x_old,
                                                   const double *__restrict__ coef, const double *__restrict__ vel,
                                                   const FeTab *__restrict__ tab, HeatArgs a, int mode, int trans,
                                                   double *__restrict__ val, double *__restrict__ rhs) {
-  constexpr int N2 = DIM == 2 ? 9 : 27, NG = N2;
-  __shared__ double s_phi[N2 * NG];
-  __shared__ double s_dref[DIM * N2 * NG];
-  __shared__ double s_dg[NG * N2 * DIM];   // [qp][s][d]
+  constexpr int N2 = DIM == 2 ? 9 : 27, NG = N2, TI = DIM == 2 ? 1 : 3;
+  __shared__ double s_G[NG * DIM * N2];  // [q][d][i]
+  __shared__ double s_a[NG * N2], s_b[NG * N2];
+  __shared__ double s_w[NG * DIM], s_IJ[NG * DIM * DIM], s_vq[NG * DIM];
+  __shared__ double s_JxW[NG], s_Tg[NG];
   __shared__ double s_xx[DIM * N2], s_Told[N2], s_veln[DIM * N2];
-  __shared__ double s_JxW[NG], s_Tg[NG], s_vel[NG * DIM], s_cl[NG * DIM];
   __shared__ int s_bcv[N2];
   const int tid = threadIdx.x;
-  for (int t = tid; t < N2 * NG; t += blockDim.x) s_phi[t] = tab->phi2[t];
-  for (int t = tid; t < DIM * N2 * NG; t += blockDim.x) s_dref[t] = tab->dphi2[t];
+  const double tcdt = a.tc * a.rhocp / a.dt;
   for (int ei = blockIdx.x; ei < nel; ei += gridDim.x) {
     const int e = elems[ei];
     __syncthreads();
@@ -124,21 +140,16 @@ __global__ __launch_bounds__(256) void k_heat_vol(int nel, const int *__restrict
     if (tid < NG) {
       const int qp = tid;
       double IJ[DIM * DIM];
-      double det = jac_qp<DIM, N2, NG>(qp, s_xx, s_dref, IJ);
-      s_JxW[qp] = det * tab->w[qp];
+      const double det = jac_qp<DIM, N2, NG>(qp, s_xx, tab->dphi2, IJ);
+      const double JxW = det * tab->w[qp];
+      s_JxW[qp] = JxW;
       double Tg = 0., vg[DIM];
       for (int d = 0; d < DIM; d++) vg[d] = 0.;
-      for (int s = 0; s < N2; s++) {
-        double ph = s_phi[s * NG + qp];
-        Tg += ph * s_Told[s];
-        for (int d = 0; d < DIM; d++) vg[d] += ph * s_veln[s + d * N2];
-        double g[DIM];
-        for (int d = 0; d < DIM; d++) g[d] = s_dref[s * NG + qp + d * N2 * NG];
-        for (int i = 0; i < DIM; i++) {
-          double sum = 0.;
-          for (int j = 0; j < DIM; j++) sum += IJ[j + i * DIM] * g[j];
-          s_dg[(qp * N2 + s) * DIM + i] = sum;
-        }
+#pragma unroll 3
+      for (int sn = 0; sn < N2; sn++) {
+        const double ph = tab->phi2[sn * NG + qp];
+        Tg += ph * s_Told[sn];
+        for (int d = 0; d < DIM; d++) vg[d] += ph * s_veln[sn + d * N2];
       }
       double h_eff[DIM];
       for (int d = 0; d < DIM; d++) {
@@ -151,40 +162,78 @@ __global__ __launch_bounds__(256) void k_heat_vol(int nel, const int *__restrict
       mod2 = sqrt(mod2);
       s_Tg[qp] = Tg;
       for (int d = 0; d < DIM; d++) {
-        double Pe = 0.5 * mod2 * h_eff[d] / a.alpha0;
-        double fup = a.upwind * a.rhocp * 0.5 * (1. / tanh(Pe) - 1. / Pe) * h_eff[d] / mod2;
-        s_vel[qp * DIM + d] = vg[d];
-        s_cl[qp * DIM + d] = a.alpha0 * ke + fup * vg[d] * vg[d];
+        const double Pe = 0.5 * mod2 * h_eff[d] / a.alpha0;
+        const double fup = a.upwind * a.rhocp * 0.5 * (1. / tanh(Pe) - 1. / Pe) * h_eff[d] / mod2;
+        s_vq[qp * DIM + d] = vg[d];
+        s_w[qp * DIM + d] = JxW * (a.alpha0 * ke + fup * vg[d] * vg[d]);
       }
+      for (int t = 0; t < DIM * DIM; t++) s_IJ[qp * DIM * DIM + t] = IJ[t];
     }
     __syncthreads();
-    const double tcdt = a.tc * a.rhocp / a.dt;
-    const int64_t pbase = (int64_t)e * N2 * N2;
-    for (int t = tid; t < N2 * N2; t += blockDim.x) {
-      const int i = t / N2, j = t % N2;
-      if (s_bcv[i] == 0) continue;
-      double K = 0.;
-      for (int qp = 0; qp < NG; qp++) {
-        const double phii = s_phi[i * NG + qp], phij = s_phi[j * NG + qp];
-        double Adv = 0., Lap = 0.;
-        for (int d = 0; d < DIM; d++) {
-          const double dj = s_dg[(qp * N2 + j) * DIM + d];
-          Adv += s_vel[qp * DIM + d] * dj;
-          Lap += s_cl[qp * DIM + d] * dj * s_dg[(qp * N2 + i) * DIM + d];
-        }
-        K += s_JxW[qp] * (tcdt * phii * phij + a.rhocp * Adv * phii + Lap);
+    for (int pq = tid; pq < NG * N2; pq += blockDim.x) {
+      const int q = pq / N2, i = pq % N2;
+      double gr[DIM];
+      for (int d = 0; d < DIM; d++) gr[d] = tab->dphi2[i * NG + q + d * N2 * NG];
+      double adv = 0.;
+      for (int d = 0; d < DIM; d++) {
+        double g = 0.;
+        for (int j = 0; j < DIM; j++) g += s_IJ[q * DIM * DIM + j + d * DIM] * gr[j];
+        s_G[(q * DIM + d) * N2 + i] = g;
+        adv += s_vq[q * DIM + d] * g;
       }
-      const int p = elpos[pbase + (trans ? j * N2 + i : t)];
-      if (p >= 0) val[p] += K;
+      const double ph = tab->phi2[i * NG + q];
+      s_a[q * N2 + i] = s_JxW[q] * ph;
+      s_b[q * N2 + i] = tcdt * ph + a.rhocp * adv;
+    }
+    __syncthreads();
+    if (tid < 81) {
+      const int ti = tid / 9, tj = tid % 9;
+      double K[TI][TI];
+#pragma unroll
+      for (int u = 0; u < TI; u++)
+#pragma unroll
+        for (int v = 0; v < TI; v++) K[u][v] = 0.;
+#pragma unroll 1
+      for (int q = 0; q < NG; q++) {
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+          const double *G = s_G + (q * DIM + d) * N2;
+          const double w = s_w[q * DIM + d];
+          double gi[TI], gj[TI];
+#pragma unroll
+          for (int u = 0; u < TI; u++) { gi[u] = w * G[ti + 9 * u]; gj[u] = G[tj + 9 * u]; }
+#pragma unroll
+          for (int u = 0; u < TI; u++)
+#pragma unroll
+            for (int v = 0; v < TI; v++) K[u][v] += gi[u] * gj[v];
+        }
+        double ai[TI], bj[TI];
+#pragma unroll
+        for (int u = 0; u < TI; u++) { ai[u] = s_a[q * N2 + ti + 9 * u]; bj[u] = s_b[q * N2 + tj + 9 * u]; }
+#pragma unroll
+        for (int u = 0; u < TI; u++)
+#pragma unroll
+          for (int v = 0; v < TI; v++) K[u][v] += ai[u] * bj[v];
+      }
+      const int64_t pbase = (int64_t)e * N2 * N2;
+#pragma unroll
+      for (int u = 0; u < TI; u++) {
+        const int i = ti + 9 * u;
+        if (s_bcv[i] == 0) continue;
+#pragma unroll
+        for (int v = 0; v < TI; v++) {
+          const int j = tj + 9 * v;
+          const int p = elpos[pbase + (trans ? j * N2 + i : i * N2 + j)];
+          if (p >= 0) val[p] += K[u][v];
+        }
+      }
     }
     if (mode == 1 && tid < N2) {
       const int i = tid;
       if (s_bcv[i] != 0) {
         double F = 0.;
-        for (int qp = 0; qp < NG; qp++) {
-          const double phii = s_phi[i * NG + qp];
-          F += s_JxW[qp] * (tcdt * s_Tg[qp] * phii + a.source * phii);
-        }
+#pragma unroll 3
+        for (int qp = 0; qp < NG; qp++) F += s_a[qp * N2 + i] * (tcdt * s_Tg[qp] + a.source);
         rhs[eldof[(int64_t)e * N2 + i]] += F;
       }
     }
@@ -680,7 +729,7 @@ int heat_assemble(hipStream_t s, int dim, int nel, const int *elems, const int *
                        xyz, Ntop, dof_top, bc0, bc1, xb, tab, a, mode, trans, val, rhs);
   } else {
     if (!bd_only)
-      hipLaunchKernelGGL(k_heat_vol<3>, dim3(nblk(nel)), dim3(256), 0, s, nel, elems, conn, eldof, elpos, xyz, Ntop,
+      hipLaunchKernelGGL(k_heat_vol<3>, dim3(nblk(nel)), dim3(128), 0, s, nel, elems, conn, eldof, elpos, xyz, Ntop,
                          dof_top, bc0, x_old, coef, vel, tab, a, mode, trans, val, rhs);
     hipLaunchKernelGGL(k_heat_bd<3>, dim3((nel + 127) / 128), dim3(128), 0, s, nel, elems, conn, eldof, elpos, bmask,
                        xyz, Ntop, dof_top, bc0, bc1, xb, tab, a, mode, trans, val, rhs);

@@ -319,6 +319,27 @@ int fm_build_sell(hipStream_t st, DevCSR &c, const int *ro, const int *cr) {
   return fm_sync_sell(st, c);
 }
 
+int fm_csr_from_sell(hipStream_t st, DevCSR &c) {
+  if (c.sell.nsl == 0) return 0;
+  FM_CHECK(fmk::sell2csr(st, c.sell.nslot, c.sell.map, c.sell.v, c.v));
+  FM_CHECK(fmk::cast_vals(st, c.nnz, c.v, c.vf));
+  return 0;
+}
+
+// SELL slot of every element-matrix entry and of every diagonal: elpos holds
+// CSR positions on entry and SELL slots on exit
+static int slots_from_csr(hipStream_t st, DevCSR &A, const int *ord, int64_t nelpos, int *elpos, int *dslot) {
+  int *slot_of = nullptr;
+  FM_CHECK(dalloc(&slot_of, A.nnz));
+  int rc = hipMemsetAsync(slot_of, 0xff, A.nnz * sizeof(int), st) == hipSuccess ? 0 : FM_EHIP;
+  if (!rc) rc = fmk::inv_map(st, A.sell.nslot, A.sell.map, slot_of);
+  if (!rc && elpos) rc = fmk::remap(st, nelpos, elpos, slot_of);
+  if (!rc) rc = fmk::diag_slot(st, A.nr, A.rp, A.ci, ord, slot_of, dslot);
+  if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = FM_EHIP;
+  dfree(slot_of);
+  return rc;
+}
+
 // refresh SELL values (FP64 and FP32) from the CSR values
 int fm_sync_sell(hipStream_t st, DevCSR &c) {
   if (c.sell.nsl == 0) return 0;
@@ -585,6 +606,9 @@ extern "C" int fm_system_init(fm_system *s) {
     FM_HIP(hipMemcpyAsync(l.bmask, bm.data(), bm.size(), hipMemcpyHostToDevice, st));
     FM_HIP(hipMemcpyAsync(l.color_elems, ce.data(), ce.size() * sizeof(int), hipMemcpyHostToDevice, st));
     FM_CHECK(fmk::elpos(st, l.nel, nloc, l.eldof, l.A.rp, l.A.ci, l.elpos));
+    dfree(l.dslot);
+    FM_CHECK(dalloc(&l.dslot, l.n));
+    FM_CHECK(slots_from_csr(st, l.A, l.ord, (int64_t)l.nel * nloc * nloc, l.elpos, l.dslot));
     FM_HIP(hipStreamSynchronize(st));
   }
   fm_drop_graphs(s);
@@ -610,6 +634,7 @@ extern "C" int fm_system_csr_get(const fm_system *s, int which, int L, int64_t *
   const DevCSR &c = which == 0 ? l.A : (which == 1 ? l.R : l.P);
   if (c.nr == 0) return FM_EINVAL;
   FM_HIP(hipSetDevice(s->ctx->device));
+  if (which == 0 && val) FM_CHECK(fm_csr_from_sell(s->ctx->stream, const_cast<DevCSR &>(c)));
   FM_HIP(hipStreamSynchronize(s->ctx->stream));
   if (rowptr) FM_HIP(hipMemcpy(rowptr, c.rp, (c.nr + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
   if (colind) FM_HIP(hipMemcpy(colind, c.ci, c.nnz * sizeof(int), hipMemcpyDeviceToHost));
@@ -758,9 +783,8 @@ extern "C" int fm_system_set_velocity(fm_system *s, const double *vel) {
 static int after_assembly(fm_system *s, int L) {
   Level &l = s->lev[L];
   hipStream_t st = s->ctx->stream;
-  FM_CHECK(fmk::cast_vals(st, l.A.nnz, l.A.v, l.A.vf));
-  FM_CHECK(fm_sync_sell(st, l.A));
-  FM_CHECK(fmk::diag_inv(st, l.n, l.A.rp, l.A.ci, l.A.v, l.ord, l.dinv, l.dinvf));
+  FM_CHECK(fmk::cast_vals(st, l.A.sell.nslot, l.A.sell.v, l.A.sell.vf));
+  FM_CHECK(fmk::dinv_slot(st, l.n, l.dslot, l.A.sell.v, l.dinv, l.dinvf));
   l.lam = 0.;
   s->mg_ready = false;
   return 0;
@@ -776,14 +800,14 @@ extern "C" int fm_heat_genmatrhs(fm_system *s, int L, int mode, const fm_heat_pa
   HeatArgs a;
   a.alpha0 = p->alpha0; a.rhocp = p->rhocp; a.dt = p->dt; a.tc = p->time_coeff; a.source = p->source; a.upwind = p->upwind;
   for (int d = 0; d < 3; d++) a.vel_const[d] = p->vel_const[d];
-  FM_HIP(hipMemsetAsync(l.A.v, 0, l.A.nnz * sizeof(double), st));
+  FM_HIP(hipMemsetAsync(l.A.sell.v, 0, l.A.sell.nslot * sizeof(double), st));
   if (mode == 1) FM_HIP(hipMemsetAsync(l.b, 0, l.n * sizeof(double), st));
   SysExtra &ex = extra(s);
   for (int k = 0; k < l.ncolor; k++) {
     int n = l.color_off[k + 1] - l.color_off[k];
     FM_CHECK(fmk::heat_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldofi, l.elpos, l.bmask, s->xyz,
                                 m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, s->xbc, l.coef, s->vel, ex.d_tab, a,
-                                mode, 0, 0, l.A.v, l.b));
+                                mode, 0, 0, l.A.sell.v, l.b));
   }
   FM_CHECK(after_assembly(s, L));
   FM_HIP(hipStreamSynchronize(st));
@@ -800,14 +824,14 @@ extern "C" int fm_ns_genmatrhs(fm_system *s, int L, int mode, const fm_ns_params
   NsArgs a;
   a.IRe = p->IRe; a.dt = p->dt; a.tc = p->time_coeff; a.upwind = p->upwind; a.IFr = p->IFr;
   for (int d = 0; d < 3; d++) { a.dirg[d] = p->dirg[d]; a.src[d] = p->src[d]; }
-  FM_HIP(hipMemsetAsync(l.A.v, 0, l.A.nnz * sizeof(double), st));
+  FM_HIP(hipMemsetAsync(l.A.sell.v, 0, l.A.sell.nslot * sizeof(double), st));
   if (mode == 1) FM_HIP(hipMemsetAsync(l.b, 0, l.n * sizeof(double), st));
   SysExtra &ex = extra(s);
   for (int k = 0; k < l.ncolor; k++) {
     int n = l.color_off[k + 1] - l.color_off[k];
     FM_CHECK(fmk::ns_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldofi, l.elpos, l.bmask, s->xyz,
                               m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, s->xbc, l.coef, ex.d_tab, a, mode, 0, 0,
-                              l.A.v, l.b));
+                              l.A.sell.v, l.b));
   }
   FM_CHECK(after_assembly(s, L));
   FM_HIP(hipStreamSynchronize(st));
@@ -822,6 +846,7 @@ extern "C" int fm_spmv(fm_system *s, int L, const double *xh, double *yh) {
   Level &l = s->lev[L];
   hipStream_t st = s->ctx->stream;
   FM_HIP(hipMemcpyAsync(l.t, xh, l.n * sizeof(double), hipMemcpyHostToDevice, st));
+  FM_CHECK(fm_csr_from_sell(st, l.A));
   CsrView<double> A = view<double>(l.A);
   A.nsl = 0;  // CSR copy: reference numbering on both sides
   FM_CHECK(fmk::spmv(st, A, l.t, l.d));
@@ -861,7 +886,10 @@ extern "C" int fm_bench_spmv(fm_system *s, int L, int prec, int reps, double *se
   // prec 0 / 1: SELL-64 FP64 / FP32 (the solver's format); 2 / 3: CSR FP64 / FP32
   CsrView<double> v64 = view<double>(l.A);
   CsrView<float> v32 = view<float>(l.A);
-  if (prec >= 2) { v64.nsl = 0; v32.nsl = 0; }
+  if (prec >= 2) {
+    FM_CHECK(fm_csr_from_sell(st, l.A));
+    v64.nsl = 0; v32.nsl = 0;
+  }
   bool f32 = (prec & 1) != 0;
   FM_CHECK(fmk::fill<double>(st, l.n, l.t, 1.0));
   FM_CHECK(fmk::fill<float>(st, l.n, l.tf, 1.0f));
@@ -914,6 +942,9 @@ static int ensure_adjoint(fm_system *s) {
     std::vector<int64_t> rp = l.A.h_rp;
     std::vector<int> ci = l.A.h_ci;
     FM_CHECK(upload_csr(st, a.A, l.n, l.n, rp, ci, nullptr, l.h_ord, l.h_rank));
+    if (a.A.sell.nslot != l.A.sell.nslot) { fm_set_error("adjoint SELL layout differs"); return FM_ESTATE; }
+    FM_CHECK(dalloc(&a.dslot, a.n));
+    FM_CHECK(slots_from_csr(st, a.A, a.ord, 0, nullptr, a.dslot));
     if (L < top) {  // restriction of the adjoint = (prolongation of level L+1)^T
       const DevCSR &P = s->lev[L + 1].P;
       std::vector<int64_t> rpT; std::vector<int> ciT; std::vector<double> vT;
@@ -936,9 +967,8 @@ static int ensure_adjoint(fm_system *s) {
 static int after_assembly_T(fm_system *s, int L) {
   Level &a = s->alev[L];
   hipStream_t st = s->ctx->stream;
-  FM_CHECK(fmk::cast_vals(st, a.A.nnz, a.A.v, a.A.vf));
-  FM_CHECK(fm_sync_sell(st, a.A));
-  FM_CHECK(fmk::diag_inv(st, a.n, a.A.rp, a.A.ci, a.A.v, a.ord, a.dinv, a.dinvf));
+  FM_CHECK(fmk::cast_vals(st, a.A.sell.nslot, a.A.sell.v, a.A.sell.vf));
+  FM_CHECK(fmk::dinv_slot(st, a.n, a.dslot, a.A.sell.v, a.dinv, a.dinvf));
   a.lam = 0.;
   s->a_assembled[L] = 1;
   s->adj_ready = false;
@@ -957,13 +987,13 @@ extern "C" int fm_heat_genmat_transpose(fm_system *s, int L, const fm_heat_param
   HeatArgs h;
   h.alpha0 = p->alpha0; h.rhocp = p->rhocp; h.dt = p->dt; h.tc = p->time_coeff; h.source = p->source; h.upwind = p->upwind;
   for (int d = 0; d < 3; d++) h.vel_const[d] = p->vel_const[d];
-  FM_HIP(hipMemsetAsync(a.A.v, 0, a.A.nnz * sizeof(double), st));
+  FM_HIP(hipMemsetAsync(a.A.sell.v, 0, a.A.sell.nslot * sizeof(double), st));
   SysExtra &ex = extra(s);
   for (int k = 0; k < l.ncolor; k++) {
     int n = l.color_off[k + 1] - l.color_off[k];
     FM_CHECK(fmk::heat_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldof, l.elpos, l.bmask, s->xyz,
                                 m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, s->xbc, l.coef, s->vel, ex.d_tab, h,
-                                0, 1, 0, a.A.v, nullptr));
+                                0, 1, 0, a.A.sell.v, nullptr));
   }
   return after_assembly_T(s, L);
 }
@@ -979,20 +1009,21 @@ extern "C" int fm_ns_genmat_transpose(fm_system *s, int L, const fm_ns_params *p
   NsArgs h;
   h.IRe = p->IRe; h.dt = p->dt; h.tc = p->time_coeff; h.upwind = p->upwind; h.IFr = p->IFr;
   for (int d = 0; d < 3; d++) { h.dirg[d] = p->dirg[d]; h.src[d] = p->src[d]; }
-  FM_HIP(hipMemsetAsync(a.A.v, 0, a.A.nnz * sizeof(double), st));
+  FM_HIP(hipMemsetAsync(a.A.sell.v, 0, a.A.sell.nslot * sizeof(double), st));
   SysExtra &ex = extra(s);
   for (int k = 0; k < l.ncolor; k++) {
     int n = l.color_off[k + 1] - l.color_off[k];
     FM_CHECK(fmk::ns_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldof, l.elpos, l.bmask, s->xyz,
                               m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, s->xbc, l.coef, ex.d_tab, h, 0, 1, 0,
-                              a.A.v, nullptr));
+                              a.A.sell.v, nullptr));
   }
   return after_assembly_T(s, L);
 }
 
 extern "C" int fm_get_adjoint_csr(const fm_system *s, int L, double *val) {
   if (!s || L < 0 || L >= (int)s->alev.size() || !val) return FM_EINVAL;
-  const DevCSR &c = s->alev[L].A;
+  DevCSR &c = const_cast<DevCSR &>(s->alev[L].A);
+  FM_CHECK(fm_csr_from_sell(s->ctx->stream, c));
   FM_HIP(hipStreamSynchronize(s->ctx->stream));
   FM_HIP(hipMemcpy(val, c.v, c.nnz * sizeof(double), hipMemcpyDeviceToHost));
   return FM_OK;

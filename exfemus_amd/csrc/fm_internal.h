@@ -79,7 +79,8 @@ struct Level {
   int *conn = nullptr;     // nel * n2 node ids
   int *eldof = nullptr;    // nel * nloc dofs (reference numbering)
   int *eldofi = nullptr;   // nel * nloc dofs (internal numbering)
-  int *elpos = nullptr;    // nel * nloc * nloc CSR positions in A
+  int *elpos = nullptr;    // nel * nloc * nloc SELL slots of A (the assembly writes the SELL values directly)
+  int *dslot = nullptr;    // n: SELL slot of the diagonal of internal row i
   unsigned char *bmask = nullptr;  // nel: bit s set if side s on the boundary
   int ncolor = 0;
   std::vector<int> color_off; // element ranges per colour (elements reordered by colour)
@@ -187,4 +188,6 @@ int fm_alloc_level_vectors(Level &l, bool top);
 void fm_drop_graphs(fm_system *s);
 void fm_free_level(Level &l);
 int fm_sync_sell(hipStream_t st, DevCSR &c);
+// A's values live in its SELL copy; this refreshes the CSR copy's FP64/FP32 values from it
+int fm_csr_from_sell(hipStream_t st, DevCSR &c);
 void fm_free_csr(DevCSR &c);

@@ -63,9 +63,12 @@ int add_f32(hipStream_t s, int64_t n, double *x, const float *e, const double *s
 template <typename T> int dot_dev(hipStream_t s, int64_t n, const T *x, const T *y, double *part, double *out, int do_sqrt);
 int ratio(hipStream_t s, const double *a, const double *b, double *out);
 template <typename T> int dense_mv(hipStream_t s, int n, const T *M, const T *b, T *y);
-int diag_inv(hipStream_t s, int64_t n, const int64_t *rp, const int *ci, const double *v, const int *ord, double *d64,
-             float *d32);
 int gather(hipStream_t s, int64_t n, const int *idx, const double *src, double *dst);
+int dinv_slot(hipStream_t s, int64_t n, const int *dslot, const double *sv, double *d64, float *d32);
+int sell2csr(hipStream_t s, int64_t nslot, const int *map, const double *sv, double *cv);
+int inv_map(hipStream_t s, int64_t nslot, const int *map, int *inv);
+int remap(hipStream_t s, int64_t n, int *io, const int *table);
+int diag_slot(hipStream_t s, int64_t n, const int64_t *rp, const int *ci, const int *ord, const int *slot_of, int *dslot);
 int cast_vals(hipStream_t s, int64_t n, const double *v, float *vf);
 int elpos(hipStream_t s, int nel, int nloc, const int *eldof, const int64_t *rp, const int *ci, int *pos);
 int copy16(hipStream_t s, int64_t n16, const void *a, void *b);

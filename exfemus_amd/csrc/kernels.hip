@@ -382,20 +382,72 @@ template int dense_mv<double>(hipStream_t, int, const double *, const double *, 
 template int dense_mv<float>(hipStream_t, int, const float *, const float *, float *);
 
 // ---- diagonal inverse
-// dinv[i] = 1 / A(ord[i], ord[i]) from the CSR copy (ord = nullptr: identity)
-template <typename T>
-__global__ void k_diag_inv(int64_t n, const int64_t *rp, const int *ci, const double *v, const int *ord, T *dinv) {
+// dinv[i] = 1 / A_ii read at the diagonal's SELL slot dslot[i] (internal row i), FP64 and FP32
+__global__ void k_dinv_slot(int64_t n, const int *__restrict__ dslot, const double *__restrict__ sv, double *d64,
+                            float *d32) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t r = ord ? ord[i] : i;
-    double d = 0.;
-    for (int64_t q = rp[r]; q < rp[r + 1]; q++) if (ci[q] == r) d = v[q];
-    dinv[i] = (T)(d != 0. ? 1.0 / d : 0.0);
+    const int q = dslot[i];
+    const double d = q >= 0 ? sv[q] : 0.0;
+    const double r = d != 0. ? 1.0 / d : 0.0;
+    d64[i] = r;
+    d32[i] = (float)r;
   }
 }
-int diag_inv(hipStream_t s, int64_t n, const int64_t *rp, const int *ci, const double *v, const int *ord, double *d64,
-             float *d32) {
-  if (d64) hipLaunchKernelGGL(k_diag_inv<double>, vgrid(n), dim3(256), 0, s, n, rp, ci, v, ord, d64);
-  if (d32) hipLaunchKernelGGL(k_diag_inv<float>, vgrid(n), dim3(256), 0, s, n, rp, ci, v, ord, d32);
+int dinv_slot(hipStream_t s, int64_t n, const int *dslot, const double *sv, double *d64, float *d32) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_dinv_slot, vgrid(n), dim3(256), 0, s, n, dslot, sv, d64, d32);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+// CSR values from the SELL copy: cv[map[slot]] = sv[slot]
+__global__ void k_sell2csr(int64_t nslot, const int *__restrict__ map, const double *__restrict__ sv, double *cv) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nslot; i += (int64_t)gridDim.x * blockDim.x) {
+    const int q = map[i];
+    if (q >= 0) cv[q] = sv[i];
+  }
+}
+int sell2csr(hipStream_t s, int64_t nslot, const int *map, const double *sv, double *cv) {
+  if (nslot <= 0) return 0;
+  hipLaunchKernelGGL(k_sell2csr, vgrid(nslot), dim3(256), 0, s, nslot, map, sv, cv);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+// inv[map[slot]] = slot (inv pre-filled with -1)
+__global__ void k_inv_map(int64_t nslot, const int *__restrict__ map, int *inv) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nslot; i += (int64_t)gridDim.x * blockDim.x) {
+    const int q = map[i];
+    if (q >= 0) inv[q] = (int)i;
+  }
+}
+int inv_map(hipStream_t s, int64_t nslot, const int *map, int *inv) {
+  if (nslot <= 0) return 0;
+  hipLaunchKernelGGL(k_inv_map, vgrid(nslot), dim3(256), 0, s, nslot, map, inv);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+// io[t] = table[io[t]] for io[t] >= 0
+__global__ void k_remap(int64_t n, int *io, const int *__restrict__ table) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int p = io[i];
+    io[i] = p >= 0 ? table[p] : -1;
+  }
+}
+int remap(hipStream_t s, int64_t n, int *io, const int *table) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_remap, vgrid(n), dim3(256), 0, s, n, io, table);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+// dslot[i] = SELL slot of the diagonal entry of CSR row ord[i] (-1 if absent)
+__global__ void k_diag_slot(int64_t n, const int64_t *__restrict__ rp, const int *__restrict__ ci,
+                            const int *__restrict__ ord, const int *__restrict__ slot_of, int *dslot) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = ord[i];
+    int p = -1;
+    for (int64_t q = rp[r]; q < rp[r + 1]; q++)
+      if (ci[q] == r) { p = slot_of[q]; break; }
+    dslot[i] = p;
+  }
+}
+int diag_slot(hipStream_t s, int64_t n, const int64_t *rp, const int *ci, const int *ord, const int *slot_of, int *dslot) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_diag_slot, vgrid(n), dim3(256), 0, s, n, rp, ci, ord, slot_of, dslot);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 // dst[i] = src[idx[i]]  (reference <-> internal DoF order)

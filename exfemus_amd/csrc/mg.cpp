@@ -107,6 +107,7 @@ static int coarse_inverse(fm_system *s, const DevCSR &A, const std::vector<int> 
   std::vector<double> vals(A.nnz);
   std::vector<int64_t> rp(n + 1);
   std::vector<int> ci(A.nnz);
+  FM_CHECK(fm_csr_from_sell(s->ctx->stream, const_cast<DevCSR &>(A)));
   FM_HIP(hipStreamSynchronize(s->ctx->stream));
   FM_HIP(hipMemcpy(vals.data(), A.v, A.nnz * sizeof(double), hipMemcpyDeviceToHost));
   FM_HIP(hipMemcpy(rp.data(), A.rp, (n + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
@@ -576,7 +577,7 @@ int fm_vanka_apply32(fm_system *s, int L, const CsrView<float> &A, const double 
                      float *r, int its, double omega, int colored, bool xz);
 int fm_vanka_setup(fm_system *s, int L, std::vector<LOps> *ops) {
   LOps &o = (*ops)[L];
-  int rc = fm_vanka_invert(s, L, o.A->v, &o.v->vinv);
+  int rc = fm_vanka_invert(s, L, o.A->sell.v, &o.v->vinv);
   if (rc) return rc;
   o.vinv = o.v->vinv;
   return 0;
