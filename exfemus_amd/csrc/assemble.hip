@@ -94,14 +94,14 @@ __device__ inline double jac_sur(int qp, const double *xb, const FeTab *tab) {
 //  1. 27 (9) Gauss-point threads: Jacobian, its inverse, JxW, T_old and the
 //     advection velocity at the point, the diffusion weights
 //     w_qd = JxW (alpha0 k_e + f_up v_d^2) (MGSolverT.C:228-330).
-//  2. All threads: global gradients G[q][d][i], a[q][i] = JxW phi_i and
-//     b[q][j] = tcdt phi_j + rhocp v.grad phi_j into LDS.
+//  2. All threads: global gradients G[q][d][i] into LDS (the reference
+//     shape values phi[i][q] are staged in LDS once per workgroup).
 //  3. The element matrix is the rank-(DIM+1)*NG product
-//        K_ij = sum_q ( sum_d w_qd G_qdi G_qdj + a_qi b_qj )
-//     computed by 81 threads, each owning a 3x3 tile (i, i+9, i+18) x
-//     (j, j+9, j+18) of the 27x27 hex27 matrix (1x1 of the 9x9 quad9 one):
-//     6 LDS reads feed 9 FMAs per (q, d), against 15 reads per FMA pair when
-//     each thread owns one entry.
+//        K_ij = sum_q ( sum_d w_qd G_qdi G_qdj + a_qi b_qj ),
+//     a_qi = JxW phi_i, b_qj = tcdt phi_j + rhocp v.grad phi_j (formed in
+//     registers from G), computed by 81 threads, each owning a 3x3 tile
+//     (i, i+9, i+18) x (j, j+9, j+18) of the 27x27 hex27 matrix (1x1 of the
+//     9x9 quad9 one): 6 LDS reads feed 9 FMAs per (q, d).
 //  4. Entries of Dirichlet rows are dropped (bc_vol row mask) and the rest
 //     are added to A through the element->CSR position map.
 // ===========================================================================
@@ -115,13 +115,14 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) void k
                                                   double *__restrict__ val, double *__restrict__ rhs) {
   constexpr int N2 = DIM == 2 ? 9 : 27, NG = N2, TI = DIM == 2 ? 1 : 3;
   __shared__ double s_G[NG * DIM * N2];  // [q][d][i]
-  __shared__ double s_a[NG * N2], s_b[NG * N2];
+  __shared__ double s_phi[N2 * NG];      // [i][q]
   __shared__ double s_w[NG * DIM], s_IJ[NG * DIM * DIM], s_vq[NG * DIM];
   __shared__ double s_JxW[NG], s_Tg[NG];
   __shared__ double s_xx[DIM * N2], s_Told[N2], s_veln[DIM * N2];
   __shared__ int s_bcv[N2];
   const int tid = threadIdx.x;
   const double tcdt = a.tc * a.rhocp / a.dt;
+  for (int t = tid; t < N2 * NG; t += blockDim.x) s_phi[t] = tab->phi2[t];
   for (int ei = blockIdx.x; ei < nel; ei += gridDim.x) {
     const int e = elems[ei];
     __syncthreads();
@@ -147,7 +148,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) void k
       for (int d = 0; d < DIM; d++) vg[d] = 0.;
 #pragma unroll 3
       for (int sn = 0; sn < N2; sn++) {
-        const double ph = tab->phi2[sn * NG + qp];
+        const double ph = s_phi[sn * NG + qp];
         Tg += ph * s_Told[sn];
         for (int d = 0; d < DIM; d++) vg[d] += ph * s_veln[sn + d * N2];
       }
@@ -174,16 +175,11 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) void k
       const int q = pq / N2, i = pq % N2;
       double gr[DIM];
       for (int d = 0; d < DIM; d++) gr[d] = tab->dphi2[i * NG + q + d * N2 * NG];
-      double adv = 0.;
       for (int d = 0; d < DIM; d++) {
         double g = 0.;
         for (int j = 0; j < DIM; j++) g += s_IJ[q * DIM * DIM + j + d * DIM] * gr[j];
         s_G[(q * DIM + d) * N2 + i] = g;
-        adv += s_vq[q * DIM + d] * g;
       }
-      const double ph = tab->phi2[i * NG + q];
-      s_a[q * N2 + i] = s_JxW[q] * ph;
-      s_b[q * N2 + i] = tcdt * ph + a.rhocp * adv;
     }
     __syncthreads();
     if (tid < 81) {
@@ -195,25 +191,30 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) void k
         for (int v = 0; v < TI; v++) K[u][v] = 0.;
 #pragma unroll 1
       for (int q = 0; q < NG; q++) {
+        double bj[TI];
+#pragma unroll
+        for (int v = 0; v < TI; v++) bj[v] = tcdt * s_phi[(tj + 9 * v) * NG + q];
 #pragma unroll
         for (int d = 0; d < DIM; d++) {
           const double *G = s_G + (q * DIM + d) * N2;
-          const double w = s_w[q * DIM + d];
+          const double w = s_w[q * DIM + d], rv = a.rhocp * s_vq[q * DIM + d];
           double gi[TI], gj[TI];
 #pragma unroll
           for (int u = 0; u < TI; u++) { gi[u] = w * G[ti + 9 * u]; gj[u] = G[tj + 9 * u]; }
+#pragma unroll
+          for (int v = 0; v < TI; v++) bj[v] += rv * gj[v];
 #pragma unroll
           for (int u = 0; u < TI; u++)
 #pragma unroll
             for (int v = 0; v < TI; v++) K[u][v] += gi[u] * gj[v];
         }
-        double ai[TI], bj[TI];
+        const double jw = s_JxW[q];
 #pragma unroll
-        for (int u = 0; u < TI; u++) { ai[u] = s_a[q * N2 + ti + 9 * u]; bj[u] = s_b[q * N2 + tj + 9 * u]; }
+        for (int u = 0; u < TI; u++) {
+          const double ai = jw * s_phi[(ti + 9 * u) * NG + q];
 #pragma unroll
-        for (int u = 0; u < TI; u++)
-#pragma unroll
-          for (int v = 0; v < TI; v++) K[u][v] += ai[u] * bj[v];
+          for (int v = 0; v < TI; v++) K[u][v] += ai * bj[v];
+        }
       }
       const int64_t pbase = (int64_t)e * N2 * N2;
 #pragma unroll
@@ -233,7 +234,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) void k
       if (s_bcv[i] != 0) {
         double F = 0.;
 #pragma unroll 3
-        for (int qp = 0; qp < NG; qp++) F += s_a[qp * N2 + i] * (tcdt * s_Tg[qp] + a.source);
+        for (int qp = 0; qp < NG; qp++) F += s_JxW[qp] * s_phi[i * NG + qp] * (tcdt * s_Tg[qp] + a.source);
         rhs[eldof[(int64_t)e * N2 + i]] += F;
       }
     }

@@ -45,6 +45,7 @@ struct DevSELL {
   unsigned short *c16 = nullptr;
   double *v = nullptr;
   float *vf = nullptr;
+  std::vector<int64_t> h_gslice;  // first slice of each row group (+ end)
 };
 
 // Device CSR (int64 row pointers, int32 columns); values in FP64 and FP32.
@@ -69,6 +70,12 @@ struct Level {
   // rank[dof] = internal position of a reference DoF
   std::vector<int> h_ord, h_rank;
   int *ord = nullptr, *rank = nullptr;
+  // row groups of equal colour signature (coupled systems): internal row ranges and signatures;
+  // per colour c the SELL slices of the groups it touches: cslist[cs_off[c] .. cs_off[c+1])
+  std::vector<int64_t> h_goff;
+  std::vector<int> h_gmask;
+  int *cslist = nullptr;
+  std::vector<int64_t> cs_off;
   DevCSR A, R, P;          // A[L]; R: coarse L <- fine L+1; P: fine L <- coarse L-1
   // FP64 vectors
   double *x = nullptr, *b = nullptr, *r = nullptr, *t = nullptr, *d = nullptr, *d2 = nullptr, *dinv = nullptr;
@@ -183,7 +190,7 @@ template <> inline CsrView<float> fm_view<float>(const DevCSR &c) {
   return v;
 }
 // ro[i] = CSR row stored at internal row i; cr[j] = internal column of CSR column j (nullptr = identity)
-int fm_build_sell(hipStream_t st, DevCSR &c, const int *ro, const int *cr);
+int fm_build_sell(hipStream_t st, DevCSR &c, const int *ro, const int *cr, const std::vector<int64_t> *groups);
 int fm_alloc_level_vectors(Level &l, bool top);
 void fm_drop_graphs(fm_system *s);
 void fm_free_level(Level &l);

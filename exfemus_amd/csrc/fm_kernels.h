@@ -16,6 +16,9 @@ struct CsrView {
   const int *sl_w = nullptr, *perm = nullptr, *base = nullptr, *c32 = nullptr;
   const unsigned short *c16 = nullptr;
   const T *sv = nullptr;
+  // optional restriction of the SELL product to the slices slist[0 .. nlist-1]
+  const int *slist = nullptr;
+  int64_t nlist = 0;
 };
 
 // Reference-element tables (src/MGFE.C:195-924 layout: phi[s*ng+qp],

@@ -88,11 +88,12 @@ __global__ __launch_bounds__(256) void k_sell(int nsl, const int64_t *__restrict
                                               const int *__restrict__ perm, const unsigned short *__restrict__ c16,
                                               const int *__restrict__ base, const int *__restrict__ c32,
                                               const T *__restrict__ vals, const T *__restrict__ xg, EpArgs<T> ep,
-                                              int xcd_chunk) {
+                                              int xcd_chunk, const int *__restrict__ slist) {
   int blk = blockIdx.x;
   if (xcd_chunk > 0) blk = (int)(blockIdx.x % 8) * xcd_chunk + (int)(blockIdx.x / 8);
-  const int s = __builtin_amdgcn_readfirstlane(blk * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6));
-  if (s >= nsl) return;
+  const int w_idx = __builtin_amdgcn_readfirstlane(blk * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6));
+  if (w_idx >= nsl) return;  // nsl: number of slices, or of list entries when slist is given
+  const int s = slist ? __builtin_amdgcn_readfirstlane(slist[w_idx]) : w_idx;
   const int lane = threadIdx.x & 63;
   const int row = perm[(int64_t)s * 64 + lane];
   const int w = sl_w[s];
@@ -163,20 +164,22 @@ static int launch_spmv(hipStream_t s, const CsrView<T> &A, const T *x, const EpA
   const int block = 256;
   if (A.nsl > 0) {
     sell_knobs();
-    int64_t nb = (A.nsl + 3) / 4;
+    const int64_t nwork = A.slist ? A.nlist : A.nsl;
+    if (nwork <= 0) return 0;
+    int64_t nb = (nwork + 3) / 4;
     int64_t chunk = 0;
     if (g_sell_xcd && nb >= 64) {
       chunk = (nb + 7) / 8;
       nb = chunk * 8;  // padded grid: surplus blocks map past the last slice and exit
     }
     dim3 grid((unsigned)nb);
-    const int nsl = (int)A.nsl;
+    const int nsl = (int)nwork;
     if (g_sell_unroll == 8)
       hipLaunchKernelGGL((k_sell<T, EP, 8>), grid, dim3(block), 0, s, nsl, A.sl_off, A.sl_c32, A.sl_w, A.perm, A.c16,
-                         A.base, A.c32, A.sv, x, ep, (int)chunk);
+                         A.base, A.c32, A.sv, x, ep, (int)chunk, A.slist);
     else
       hipLaunchKernelGGL((k_sell<T, EP, 4>), grid, dim3(block), 0, s, nsl, A.sl_off, A.sl_c32, A.sl_w, A.perm, A.c16,
-                         A.base, A.c32, A.sv, x, ep, (int)chunk);
+                         A.base, A.c32, A.sv, x, ep, (int)chunk, A.slist);
     return hipGetLastError() == hipSuccess ? 0 : -2;
   }
   int64_t threads = A.nr * A.lpr;

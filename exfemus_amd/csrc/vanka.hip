@@ -185,23 +185,30 @@ static int vanka_sweep(fm_system *s, int L, const CsrView<T> &A, const double *v
   // coloured: residual refreshed before every colour, undamped weights 1
   const double *wt = l.vwt;
   bool fresh = xz;  // x still zero: the residual is b
-  auto residual = [&]() -> int {
+  // colour c > = 0: only the rows that colour's elements touch (whole SELL slices, see Level::cslist)
+  auto residual = [&](int c) -> int {
     if (fresh) {
       fresh = false;
       return hipMemcpyAsync(r, b, l.n * sizeof(T), hipMemcpyDeviceToDevice, st) == hipSuccess ? 0 : FM_EHIP;
+    }
+    if (c >= 0 && l.cslist && c + 1 < (int)l.cs_off.size()) {
+      CsrView<T> Ac = A;
+      Ac.slist = l.cslist + l.cs_off[c];
+      Ac.nlist = l.cs_off[c + 1] - l.cs_off[c];
+      return fmk::resid(st, Ac, b, x, r);
     }
     return fmk::resid(st, A, b, x, r);
   };
   for (int it = 0; it < its; it++) {
     if (!colored) {
-      int rc = residual();
+      int rc = residual(-1);
       if (rc) return rc;
     }
     for (int c = 0; c < l.ncolor; c++) {
       int n = l.color_off[c + 1] - l.color_off[c];
       if (n <= 0) continue;
       if (colored) {
-        int rc = residual();
+        int rc = residual(c);
         if (rc) return rc;
       }
       unsigned g = (unsigned)(n < 8192 ? n : 8192);
