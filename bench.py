@@ -240,6 +240,7 @@ def main():
     ap.add_argument("--cheb-lo", type=float, default=0.1)
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="launch V-cycles eagerly instead of replaying HIP graphs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; multi-GPU sharding is described in "
@@ -280,6 +281,7 @@ def main():
     log(f"setup {time.perf_counter() - t0:.1f}s  N_top={N}  assembly(all levels)={t_asm * 1e3:.1f} ms")
     prec_mixed = F.FM_PREC_FP32_IR if args.outer == "ir" else F.FM_PREC_FP32_FGMRES
     prec_fp64 = F.FM_PREC_FP64 if args.outer == "ir" else F.FM_PREC_FP64_FGMRES
+    kw["use_graph"] = 0 if args.no_graph else 1
     om = s.opts(precision=prec_mixed, tol=args.tol, max_iter=200, **kw)
     o64 = s.opts(precision=prec_fp64, tol=args.tol, max_iter=200, **kw)
     s.MGSetup(o64)

@@ -477,7 +477,7 @@ void fm_free_level(Level &l) {
   dfree(l.xf); dfree(l.bf); dfree(l.rf); dfree(l.tf); dfree(l.df); dfree(l.d2f); dfree(l.dinvf);
   dfree(l.conn); dfree(l.eldof); dfree(l.eldofi); dfree(l.elpos); dfree(l.bmask); dfree(l.color_elems); dfree(l.coef);
   dfree(l.ord); dfree(l.rank);
-  dfree(l.Ainv); dfree(l.Ainvf); dfree(l.vinv); dfree(l.vwt); dfree(l.vwtf); dfree(l.nrm); dfree(l.res);
+  dfree(l.Ainv); dfree(l.Ainvf); dfree(l.vinv); dfree(l.vinvf); dfree(l.vwt); dfree(l.vwtf); dfree(l.nrm); dfree(l.res);
   dfree(l.sb); dfree(l.sx); dfree(l.vone);
 }
 

@@ -96,8 +96,9 @@ struct Level {
   // coarse direct solve (level 0)
   double *Ainv = nullptr;
   float *Ainvf = nullptr;
-  // Vanka local inverses (nel * nloc * nloc, FP64)
+  // Vanka local inverses (nel * nloc * nloc, transposed; FP64 and its FP32 copy)
   double *vinv = nullptr;
+  float *vinvf = nullptr;
   double *vwt = nullptr;   // 1 / multiplicity per dof
   double *vone = nullptr;  // all ones (coloured Vanka weights)
   float *vwtf = nullptr;

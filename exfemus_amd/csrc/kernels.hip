@@ -144,7 +144,7 @@ __global__ __launch_bounds__(256) void k_sell(int nsl, const int64_t *__restrict
 }
 
 // launch-shape knobs of the SELL kernel (read once from the environment):
-// FEMUS_SELL_UNROLL = 4 | 8 (default 8), FEMUS_SELL_XCD = 0 | 1 (default 1).
+// FEMUS_SELL_UNROLL = 4 | 8 | 16 (default 8), FEMUS_SELL_XCD = 0 | 1 (default 1).
 // Measured on MI355X, 3D heat hex27 64^3 top matrix (135M nnz) in the
 // node-lexicographic internal order: contiguous per-XCD slice ranges take the
 // FP64 SpMV from 389 to 301 us and the FP32 one from 249 to 187 us (each XCD's
@@ -154,7 +154,7 @@ static void sell_knobs() {
   if (g_sell_unroll >= 0) return;
   const char *u = getenv("FEMUS_SELL_UNROLL");
   const char *x = getenv("FEMUS_SELL_XCD");
-  g_sell_unroll = (u && atoi(u) == 4) ? 4 : 8;
+  g_sell_unroll = (u && atoi(u) == 4) ? 4 : ((u && atoi(u) == 16) ? 16 : 8);
   g_sell_xcd = (x && atoi(x) == 0) ? 0 : 1;
 }
 
@@ -176,6 +176,9 @@ static int launch_spmv(hipStream_t s, const CsrView<T> &A, const T *x, const EpA
     const int nsl = (int)nwork;
     if (g_sell_unroll == 8)
       hipLaunchKernelGGL((k_sell<T, EP, 8>), grid, dim3(block), 0, s, nsl, A.sl_off, A.sl_c32, A.sl_w, A.perm, A.c16,
+                         A.base, A.c32, A.sv, x, ep, (int)chunk, A.slist);
+    else if (g_sell_unroll == 16)
+      hipLaunchKernelGGL((k_sell<T, EP, 16>), grid, dim3(block), 0, s, nsl, A.sl_off, A.sl_c32, A.sl_w, A.perm, A.c16,
                          A.base, A.c32, A.sv, x, ep, (int)chunk, A.slist);
     else
       hipLaunchKernelGGL((k_sell<T, EP, 4>), grid, dim3(block), 0, s, nsl, A.sl_off, A.sl_c32, A.sl_w, A.perm, A.c16,

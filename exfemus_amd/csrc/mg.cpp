@@ -39,7 +39,8 @@ struct LOps {
   double *Ainv = nullptr;
   float *Ainvf = nullptr;
   Level *v = nullptr;  // work vectors
-  double *vinv = nullptr;  // Vanka local inverses
+  double *vinv = nullptr;  // Vanka local inverses (transposed), FP64 and FP32 copies
+  float *vinvf = nullptr;
 };
 
 template <typename T> static inline CsrView<T> cview(const DevCSR &c) { return fm_view<T>(c); }
@@ -172,6 +173,7 @@ static std::vector<LOps> forward_ops(fm_system *s) {
     LOps &o = ops[L];
     o.n = l.n; o.A = &l.A; o.R = &l.R; o.P = &l.P;
     o.dinv = l.dinv; o.dinvf = l.dinvf; o.lam = l.lam; o.Ainv = l.Ainv; o.Ainvf = l.Ainvf; o.v = &l; o.vinv = l.vinv;
+    o.vinvf = l.vinvf;
   }
   return ops;
 }
@@ -328,6 +330,7 @@ extern "C" int fm_mg_setup(fm_system *s, const fm_mg_opts *o) {
   for (size_t L = 0; L < s->lev.size(); L++) {
     s->lev[L].lam = ops[L].lam;
     s->lev[L].Ainv = ops[L].Ainv; s->lev[L].Ainvf = ops[L].Ainvf; s->lev[L].vinv = ops[L].vinv;
+    s->lev[L].vinvf = ops[L].vinvf;
   }
   s->mg_ready = true;
   s->mg_smoother = o->smoother;
@@ -570,16 +573,17 @@ extern "C" int fm_mg_solve(fm_system *s, const fm_mg_opts *o, fm_mg_report *rep)
 }
 
 // ---- Vanka (vanka.hip) -----------------------------------------------------
-int fm_vanka_invert(fm_system *s, int L, const double *Av, double **store);
+int fm_vanka_invert(fm_system *s, int L, const double *Av, double **store, float **storef);
 int fm_vanka_apply64(fm_system *s, int L, const CsrView<double> &A, const double *vinv, double *x, const double *b,
                      double *r, int its, double omega, int colored, bool xz);
-int fm_vanka_apply32(fm_system *s, int L, const CsrView<float> &A, const double *vinv, float *x, const float *b,
+int fm_vanka_apply32(fm_system *s, int L, const CsrView<float> &A, const float *vinv, float *x, const float *b,
                      float *r, int its, double omega, int colored, bool xz);
 int fm_vanka_setup(fm_system *s, int L, std::vector<LOps> *ops) {
   LOps &o = (*ops)[L];
-  int rc = fm_vanka_invert(s, L, o.A->sell.v, &o.v->vinv);
+  int rc = fm_vanka_invert(s, L, o.A->sell.v, &o.v->vinv, &o.v->vinvf);
   if (rc) return rc;
   o.vinv = o.v->vinv;
+  o.vinvf = o.v->vinvf;
   return 0;
 }
 int fm_vanka_sweep64(fm_system *s, int L, const LOps &o, double *x, const double *b, int its, double omega, int colored,
@@ -588,7 +592,7 @@ int fm_vanka_sweep64(fm_system *s, int L, const LOps &o, double *x, const double
 }
 int fm_vanka_sweep32(fm_system *s, int L, const LOps &o, float *x, const float *b, int its, double omega, int colored,
                      bool xz) {
-  return fm_vanka_apply32(s, L, cview<float>(*o.A), o.vinv, x, b, o.v->rf, its, omega, colored, xz);
+  return fm_vanka_apply32(s, L, cview<float>(*o.A), o.vinvf, x, b, o.v->rf, its, omega, colored, xz);
 }
 
 // Transposed multigrid: A^T lambda = g on the adjoint hierarchy (A^T from the
@@ -605,7 +609,7 @@ extern "C" int fm_adjoint_solve(fm_system *s, const fm_mg_opts *o, const double 
     Level &a = s->alev[L];
     LOps &op = ops[L];
     op.n = a.n; op.A = &a.A; op.R = &a.R; op.P = &a.P; op.dinv = a.dinv; op.dinvf = a.dinvf; op.lam = a.lam;
-    op.Ainv = a.Ainv; op.Ainvf = a.Ainvf; op.v = &a; op.vinv = a.vinv;
+    op.Ainv = a.Ainv; op.Ainvf = a.Ainvf; op.v = &a; op.vinv = a.vinv; op.vinvf = a.vinvf;
   }
   if (!s->adj_ready || s->adj_smoother != o->smoother) {
     fm_drop_graphs(s);

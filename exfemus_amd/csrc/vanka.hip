@@ -10,8 +10,11 @@
 // FP64 in LDS (Gauss-Jordan, partial pivoting).  A sweep forms r = b - A x
 // once and adds omega * w_i * (K_e^-1 r_e)_i to x, colour by colour (no two
 // elements of a colour share a DoF, so the scatter needs no atomics);
-// w_i = 1 / (number of patches holding DoF i).  The local inverses stay FP64
-// also inside the FP32 V-cycle.
+// w_i = 1 / (number of patches holding DoF i).  The local inverses are
+// computed in FP64 and stored transposed (so the threads of a patch read
+// them coalesced) in FP64 and as an FP32 copy: the FP64 cycle applies the
+// FP64 inverse, the FP32 cycle the FP32 copy with FP32 accumulation (as it
+// applies the FP32 copy of the FP64 coarse-level inverse).
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <vector>
@@ -20,7 +23,8 @@
 
 template <int NL>
 __global__ __launch_bounds__(256) void k_vanka_invert(int nel, const int *__restrict__ elpos, const double *__restrict__ Av,
-                                                      double *__restrict__ vinv, int *__restrict__ status) {
+                                                      double *__restrict__ vinv, float *__restrict__ vinvf,
+                                                      int *__restrict__ status) {
   __shared__ double M[NL * NL];
   __shared__ int piv[NL];
   __shared__ double s_best;
@@ -82,7 +86,7 @@ __global__ __launch_bounds__(256) void k_vanka_invert(int nel, const int *__rest
     }
     if (singular) {
       if (tid == 0) atomicAdd(status, 1);
-      for (int t = tid; t < NL * NL; t += blockDim.x) vinv[base + t] = 0.0;
+      for (int t = tid; t < NL * NL; t += blockDim.x) { vinv[base + t] = 0.0; vinvf[base + t] = 0.f; }
       continue;
     }
     // undo the row interchanges as column interchanges, last to first
@@ -94,16 +98,22 @@ __global__ __launch_bounds__(256) void k_vanka_invert(int nel, const int *__rest
         }
       __syncthreads();
     }
-    for (int t = tid; t < NL * NL; t += blockDim.x) vinv[base + t] = M[t];
+    // transposed: entry (i, j) of the inverse at j * NL + i
+    for (int t = tid; t < NL * NL; t += blockDim.x) {
+      const int i = t / NL, j = t % NL;
+      vinv[base + j * NL + i] = M[t];
+      vinvf[base + j * NL + i] = (float)M[t];
+    }
   }
 }
 
-// x[dof] += omega * w[dof] * (K_e^-1 r_e) for the elements of one colour
+// x[dof] += omega * w[dof] * (K_e^-1 r_e) for the elements of one colour; the
+// inverse is stored transposed (vinv[e][b][a] = (K_e^-1)_ab) in the cycle's precision T
 template <int NL, typename T>
 __global__ __launch_bounds__(128) void k_vanka_apply(int nel, const int *__restrict__ elems, const int *__restrict__ eldof,
-                                                     const double *__restrict__ vinv, const T *__restrict__ r,
+                                                     const T *__restrict__ vinv, const T *__restrict__ r,
                                                      T *__restrict__ x, const double *__restrict__ wt, double omega) {
-  __shared__ double re[NL];
+  __shared__ T re[NL];
   __shared__ int dof[NL];
   const int tid = threadIdx.x;
   for (int ei = blockIdx.x; ei < nel; ei += gridDim.x) {
@@ -112,15 +122,15 @@ __global__ __launch_bounds__(128) void k_vanka_apply(int nel, const int *__restr
     for (int a = tid; a < NL; a += blockDim.x) {
       int d = eldof[(int64_t)e * NL + a];
       dof[a] = d;
-      re[a] = (double)r[d];
+      re[a] = r[d];
     }
     __syncthreads();
-    const double *Ki = vinv + (int64_t)e * NL * NL;
+    const T *Ki = vinv + (int64_t)e * NL * NL;
     for (int a = tid; a < NL; a += blockDim.x) {
-      double s = 0.;
-      for (int b = 0; b < NL; b++) s += Ki[a * NL + b] * re[b];
+      T s = 0;
+      for (int b = 0; b < NL; b++) s += Ki[b * NL + a] * re[b];
       const int d = dof[a];
-      x[d] = (T)((double)x[d] + omega * wt[d] * s);
+      x[d] = x[d] + (T)(omega * wt[d]) * s;
     }
   }
 }
@@ -137,13 +147,16 @@ __global__ void k_recip(int64_t n, double *w) {
 
 
 // patch inverses of the matrix with values Av (A or A^T: same pattern) into *store
-static int vanka_invert_level(fm_system *s, int L, const double *Av, double **store) {
+static int vanka_invert_level(fm_system *s, int L, const double *Av, double **store, float **storef) {
   Level &l = s->lev[L];
   hipStream_t st = s->ctx->stream;
   const int nl = s->nloc;
   if (s->nq1 == 0) { fm_set_error("Vanka needs a Q2-Q1 (velocity-pressure) system"); return FM_EINVAL; }
   if (!*store) {
     if (hipMalloc((void **)store, (size_t)l.nel * nl * nl * sizeof(double)) != hipSuccess) { fm_set_error("Vanka alloc"); return FM_ENOMEM; }
+  }
+  if (!*storef) {
+    if (hipMalloc((void **)storef, (size_t)l.nel * nl * nl * sizeof(float)) != hipSuccess) { fm_set_error("Vanka alloc"); return FM_ENOMEM; }
   }
   if (!l.vwt) {
     if (hipMalloc((void **)&l.vwt, l.n * sizeof(double)) != hipSuccess) { fm_set_error("Vanka alloc"); return FM_ENOMEM; }
@@ -158,9 +171,9 @@ static int vanka_invert_level(fm_system *s, int L, const double *Av, double **st
   FM_HIP(hipMemsetAsync(l.vwt, 0, l.n * sizeof(double), st));
   unsigned g = (unsigned)(l.nel < 8192 ? l.nel : 8192);
   if (s->mesh->dim == 2)
-    hipLaunchKernelGGL(k_vanka_invert<22>, dim3(g), dim3(256), 0, st, l.nel, l.elpos, Av, *store, status);
+    hipLaunchKernelGGL(k_vanka_invert<22>, dim3(g), dim3(256), 0, st, l.nel, l.elpos, Av, *store, *storef, status);
   else
-    hipLaunchKernelGGL(k_vanka_invert<89>, dim3(g), dim3(256), 0, st, l.nel, l.elpos, Av, *store, status);
+    hipLaunchKernelGGL(k_vanka_invert<89>, dim3(g), dim3(256), 0, st, l.nel, l.elpos, Av, *store, *storef, status);
   FM_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_count_mult, dim3(1024), dim3(256), 0, st, l.nel, nl, l.eldofi, l.vwt);
   hipLaunchKernelGGL(k_recip, dim3(1024), dim3(256), 0, st, l.n, l.vwt);
@@ -177,7 +190,7 @@ static int vanka_invert_level(fm_system *s, int L, const double *Av, double **st
 
 // xz: x is zero on entry, so the first residual is b (copied, no product)
 template <typename T>
-static int vanka_sweep(fm_system *s, int L, const CsrView<T> &A, const double *vinv, T *x, const T *b, T *r, int its,
+static int vanka_sweep(fm_system *s, int L, const CsrView<T> &A, const T *vinv, T *x, const T *b, T *r, int its,
                        double omega, int colored, bool xz) {
   Level &l = s->lev[L];
   hipStream_t st = s->ctx->stream;
@@ -224,12 +237,14 @@ static int vanka_sweep(fm_system *s, int L, const CsrView<T> &A, const double *v
   return hipGetLastError() == hipSuccess ? 0 : FM_EHIP;
 }
 
-int fm_vanka_invert(fm_system *s, int L, const double *Av, double **store) { return vanka_invert_level(s, L, Av, store); }
+int fm_vanka_invert(fm_system *s, int L, const double *Av, double **store, float **storef) {
+  return vanka_invert_level(s, L, Av, store, storef);
+}
 int fm_vanka_apply64(fm_system *s, int L, const CsrView<double> &A, const double *vinv, double *x, const double *b,
                      double *r, int its, double omega, int colored, bool xz) {
   return vanka_sweep<double>(s, L, A, vinv, x, b, r, its, omega, colored, xz);
 }
-int fm_vanka_apply32(fm_system *s, int L, const CsrView<float> &A, const double *vinv, float *x, const float *b,
+int fm_vanka_apply32(fm_system *s, int L, const CsrView<float> &A, const float *vinv, float *x, const float *b,
                      float *r, int its, double omega, int colored, bool xz) {
   return vanka_sweep<float>(s, L, A, vinv, x, b, r, its, omega, colored, xz);
 }
