@@ -304,24 +304,35 @@ __global__ void k_heat_bd(int nel, const int *__restrict__ elems, const int *__r
 // ===========================================================================
 // Navier-Stokes (Q2-Q1 coupled) volume kernel
 // ===========================================================================
+// One 256-thread workgroup per element.  After the Gauss-point phase (as in
+// the heat kernel) the global gradients G[q][d][i] sit in LDS and every block
+// of the element matrix is a short sum over the Gauss points of products of
+// G rows:
+//   (iv,iv): sum_q [ sum_k JxW cl_k G_ki G_kj + JxW cl_iv G_iv,i G_iv,j
+//                    + JxW phi_i (tcdt phi_j + u_old.grad phi_j) ]
+//   (iv,jv): sum_q JxW (nu + f_up,iv u_iv u_jv) G_jv,i G_iv,j      (iv != jv)
+//   B^T:     -sum_q JxW psi_j G_iv,i        B: sum_q JxW psi_i G_jv,j
+// (MGSolverNS.C:364-489).  The DIM^2 velocity blocks are split into 3x3
+// register tiles (1x1 in 2D), 81 per block; B^T / B entries are one per
+// thread.  Velocity rows of Dirichlet DoFs are dropped (bc_vol), pressure
+// rows are not.
 template <int DIM>
-__global__ __launch_bounds__(256) void k_ns_vol(int nel, const int *__restrict__ elems, const int *__restrict__ conn,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_ns_vol(int nel, const int *__restrict__ elems, const int *__restrict__ conn,
                                                 const int *__restrict__ eldof, const int *__restrict__ elpos,
                                                 const double *__restrict__ xyz, int Ntop, const int *__restrict__ dof_top,
                                                 const int *__restrict__ bc0, const double *__restrict__ x_old,
                                                 const double *__restrict__ coef, const FeTab *__restrict__ tab, NsArgs a,
                                                 int mode, int trans, double *__restrict__ val, double *__restrict__ rhs) {
-  constexpr int N2 = DIM == 2 ? 9 : 27, N1 = DIM == 2 ? 4 : 8, NG = N2, NL = DIM * N2 + N1;
+  constexpr int N2 = DIM == 2 ? 9 : 27, N1 = DIM == 2 ? 4 : 8, NG = N2, NL = DIM * N2 + N1, TI = DIM == 2 ? 1 : 3;
+  __shared__ double s_G[NG * DIM * N2];  // [q][d][i]
   __shared__ double s_phi[N2 * NG], s_psi[N1 * NG];
-  __shared__ double s_dref[DIM * N2 * NG];
-  __shared__ double s_dg[NG * N2 * DIM];
   __shared__ double s_xx[DIM * N2], s_uold[DIM * N2];
-  __shared__ double s_JxW[NG], s_vel[NG * DIM], s_fup[NG * DIM], s_cl[NG * DIM];
+  __shared__ double s_JxW[NG], s_vel[NG * DIM], s_fup[NG * DIM], s_cl[NG * DIM], s_IJ[NG * DIM * DIM];
   __shared__ int s_bcv[DIM * N2];
   const int tid = threadIdx.x;
+  const double tcdt = a.tc / a.dt;
   for (int t = tid; t < N2 * NG; t += blockDim.x) s_phi[t] = tab->phi2[t];
   for (int t = tid; t < N1 * NG; t += blockDim.x) s_psi[t] = tab->phi1[t];
-  for (int t = tid; t < DIM * N2 * NG; t += blockDim.x) s_dref[t] = tab->dphi2[t];
   for (int ei = blockIdx.x; ei < nel; ei += gridDim.x) {
     const int e = elems[ei];
     __syncthreads();
@@ -339,20 +350,14 @@ __global__ __launch_bounds__(256) void k_ns_vol(int nel, const int *__restrict__
     if (tid < NG) {
       const int qp = tid;
       double IJ[DIM * DIM];
-      double det = jac_qp<DIM, N2, NG>(qp, s_xx, s_dref, IJ);
+      const double det = jac_qp<DIM, N2, NG>(qp, s_xx, tab->dphi2, IJ);
       s_JxW[qp] = det * tab->w[qp];
       double vg[DIM];
       for (int d = 0; d < DIM; d++) vg[d] = 0.;
-      for (int s = 0; s < N2; s++) {
-        double ph = s_phi[s * NG + qp];
-        for (int d = 0; d < DIM; d++) vg[d] += ph * s_uold[s + d * N2];
-        double g[DIM];
-        for (int d = 0; d < DIM; d++) g[d] = s_dref[s * NG + qp + d * N2 * NG];
-        for (int i = 0; i < DIM; i++) {
-          double sum = 0.;
-          for (int j = 0; j < DIM; j++) sum += IJ[j + i * DIM] * g[j];
-          s_dg[(qp * N2 + s) * DIM + i] = sum;
-        }
+#pragma unroll 3
+      for (int sn = 0; sn < N2; sn++) {
+        const double ph = s_phi[sn * NG + qp];
+        for (int d = 0; d < DIM; d++) vg[d] += ph * s_uold[sn + d * N2];
       }
       double h_eff[DIM];
       for (int d = 0; d < DIM; d++) {
@@ -364,51 +369,112 @@ __global__ __launch_bounds__(256) void k_ns_vol(int nel, const int *__restrict__
       for (int d = 0; d < DIM; d++) mod2 += vg[d] * vg[d];
       mod2 = sqrt(mod2);
       for (int d = 0; d < DIM; d++) {
-        double Pe = 0.5 * mod2 * h_eff[d] / a.IRe;
-        double fup = a.upwind * 0.5 * (1. / tanh(Pe) - 1. / Pe) * h_eff[d] / mod2;
+        const double Pe = 0.5 * mod2 * h_eff[d] / a.IRe;
+        const double fup = a.upwind * 0.5 * (1. / tanh(Pe) - 1. / Pe) * h_eff[d] / mod2;
         s_vel[qp * DIM + d] = vg[d];
         s_fup[qp * DIM + d] = fup;
         s_cl[qp * DIM + d] = nu + fup * vg[d] * vg[d];
       }
+      for (int t = 0; t < DIM * DIM; t++) s_IJ[qp * DIM * DIM + t] = IJ[t];
     }
     __syncthreads();
-    const double tcdt = a.tc / a.dt;
+    for (int pq = tid; pq < NG * N2; pq += blockDim.x) {
+      const int q = pq / N2, i = pq % N2;
+      double gr[DIM];
+      for (int d = 0; d < DIM; d++) gr[d] = tab->dphi2[i * NG + q + d * N2 * NG];
+      for (int d = 0; d < DIM; d++) {
+        double g = 0.;
+        for (int j = 0; j < DIM; j++) g += s_IJ[q * DIM * DIM + j + d * DIM] * gr[j];
+        s_G[(q * DIM + d) * N2 + i] = g;
+      }
+    }
+    __syncthreads();
     const int64_t pbase = (int64_t)e * NL * NL;
-    for (int t = tid; t < NL * NL; t += blockDim.x) {
-      const int ra = t / NL, cb = t % NL;
-      double K = 0.;
-      if (ra < DIM * N2) {
-        const int iv = ra / N2, i = ra % N2;
-        if (s_bcv[i + iv * N2] == 0) continue;
-        if (cb < DIM * N2) {
-          const int jv = cb / N2, j = cb % N2;
-          if (jv == iv) {
-            for (int qp = 0; qp < NG; qp++) {
-              const double phii = s_phi[i * NG + qp], phij = s_phi[j * NG + qp];
-              double Adv = 0., Lap = 0.;
-              for (int k = 0; k < DIM; k++) {
-                const double dj = s_dg[(qp * N2 + j) * DIM + k];
-                Adv += s_vel[qp * DIM + k] * dj * phii;
-                Lap += s_cl[qp * DIM + k] * (dj * s_dg[(qp * N2 + i) * DIM + k]);
-              }
-              K += s_JxW[qp] * (tcdt * phij * phii + Adv + Lap +
-                                s_cl[qp * DIM + iv] * (s_dg[(qp * N2 + j) * DIM + iv] * s_dg[(qp * N2 + i) * DIM + iv]));
-            }
-          } else {
-            for (int qp = 0; qp < NG; qp++)
-              K += s_JxW[qp] * (nu + s_fup[qp * DIM + iv] * s_vel[qp * DIM + iv] * s_vel[qp * DIM + jv]) *
-                   s_dg[(qp * N2 + j) * DIM + iv] * s_dg[(qp * N2 + i) * DIM + jv];
+    // velocity blocks: DIM^2 blocks x 81 tiles
+    for (int u = tid; u < DIM * DIM * 81; u += blockDim.x) {
+      const int blk = u / 81, t = u % 81;
+      const int iv = blk / DIM, jv = blk % DIM, ti = t / 9, tj = t % 9;
+      double K[TI][TI];
+#pragma unroll
+      for (int x = 0; x < TI; x++)
+#pragma unroll
+        for (int y = 0; y < TI; y++) K[x][y] = 0.;
+      if (iv == jv) {
+#pragma unroll 1
+        for (int q = 0; q < NG; q++) {
+          const double jw = s_JxW[q];
+          double bj[TI];
+#pragma unroll
+          for (int y = 0; y < TI; y++) bj[y] = tcdt * s_phi[(tj + 9 * y) * NG + q];
+#pragma unroll
+          for (int k = 0; k < DIM; k++) {
+            const double *G = s_G + (q * DIM + k) * N2;
+            const double w = jw * (s_cl[q * DIM + k] + (k == iv ? s_cl[q * DIM + iv] : 0.));
+            const double vk = s_vel[q * DIM + k];
+            double gi[TI], gj[TI];
+#pragma unroll
+            for (int x = 0; x < TI; x++) { gi[x] = w * G[ti + 9 * x]; gj[x] = G[tj + 9 * x]; }
+#pragma unroll
+            for (int y = 0; y < TI; y++) bj[y] += vk * gj[y];
+#pragma unroll
+            for (int x = 0; x < TI; x++)
+#pragma unroll
+              for (int y = 0; y < TI; y++) K[x][y] += gi[x] * gj[y];
           }
-        } else {
-          const int j = cb - DIM * N2;
-          for (int qp = 0; qp < NG; qp++) K += s_JxW[qp] * (-s_psi[j * NG + qp] * s_dg[(qp * N2 + i) * DIM + iv]);
+#pragma unroll
+          for (int x = 0; x < TI; x++) {
+            const double ai = jw * s_phi[(ti + 9 * x) * NG + q];
+#pragma unroll
+            for (int y = 0; y < TI; y++) K[x][y] += ai * bj[y];
+          }
         }
       } else {
-        if (cb >= DIM * N2) continue;
-        const int i = ra - DIM * N2, jv = cb / N2, j = cb % N2;
-        for (int qp = 0; qp < NG; qp++) K += s_JxW[qp] * (s_psi[i * NG + qp] * s_dg[(qp * N2 + j) * DIM + jv]);
+#pragma unroll 1
+        for (int q = 0; q < NG; q++) {
+          const double c = s_JxW[q] * (nu + s_fup[q * DIM + iv] * s_vel[q * DIM + iv] * s_vel[q * DIM + jv]);
+          const double *Gi = s_G + (q * DIM + jv) * N2, *Gj = s_G + (q * DIM + iv) * N2;
+          double gi[TI], gj[TI];
+#pragma unroll
+          for (int x = 0; x < TI; x++) { gi[x] = c * Gi[ti + 9 * x]; gj[x] = Gj[tj + 9 * x]; }
+#pragma unroll
+          for (int x = 0; x < TI; x++)
+#pragma unroll
+            for (int y = 0; y < TI; y++) K[x][y] += gi[x] * gj[y];
+        }
       }
-      const int p = elpos[pbase + (trans ? cb * NL + ra : t)];
+#pragma unroll
+      for (int x = 0; x < TI; x++) {
+        const int row = iv * N2 + ti + 9 * x;
+        if (s_bcv[row] == 0) continue;
+#pragma unroll
+        for (int y = 0; y < TI; y++) {
+          const int col = jv * N2 + tj + 9 * y;
+          const int p = elpos[pbase + (trans ? col * NL + row : row * NL + col)];
+          if (p >= 0) val[p] += K[x][y];
+        }
+      }
+    }
+    // B^T (velocity rows, pressure columns) and B (pressure rows, velocity columns)
+    constexpr int NBT = DIM * N2 * N1;
+    for (int u = tid; u < 2 * NBT; u += blockDim.x) {
+      int row, col;
+      double K = 0.;
+      if (u < NBT) {
+        row = u / N1;
+        const int j = u % N1, iv = row / N2, i = row % N2;
+        if (s_bcv[row] == 0) continue;
+        col = DIM * N2 + j;
+#pragma unroll 3
+        for (int q = 0; q < NG; q++) K -= s_JxW[q] * s_psi[j * NG + q] * s_G[(q * DIM + iv) * N2 + i];
+      } else {
+        const int u2 = u - NBT, ip = u2 / (DIM * N2);
+        col = u2 % (DIM * N2);
+        const int jv = col / N2, j = col % N2;
+        row = DIM * N2 + ip;
+#pragma unroll 3
+        for (int q = 0; q < NG; q++) K += s_JxW[q] * s_psi[ip * NG + q] * s_G[(q * DIM + jv) * N2 + j];
+      }
+      const int p = elpos[pbase + (trans ? col * NL + row : row * NL + col)];
       if (p >= 0) val[p] += K;
     }
     if (mode == 1 && tid < DIM * N2) {
@@ -434,7 +500,11 @@ __global__ void k_ns_bd(int nel, const int *__restrict__ elems, const int *__res
                         double *__restrict__ val, double *__restrict__ rhs) {
   constexpr int N2 = DIM == 2 ? 9 : 27, N1 = DIM == 2 ? 4 : 8, NB = DIM == 2 ? 3 : 9, NGB = NB, NS = 2 * DIM;
   constexpr int NL = DIM * N2 + N1;
-  int ei = blockIdx.x * blockDim.x + threadIdx.x;
+  // one thread per (element, velocity component): a thread writes only the rows of its own component
+  // (entries (row_iv, col_jv) with jv >= iv, or their transposed positions), so the threads of one
+  // element never touch the same entry
+  const int tix = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ei = tix / DIM, iv = tix % DIM;
   if (ei >= nel) return;
   int e = elems[ei];
   unsigned char bm = bmask[e];
@@ -463,7 +533,7 @@ __global__ void k_ns_bd(int nel, const int *__restrict__ elems, const int *__res
     for (int d = 0; d < DIM; d++) mm += nrm[d] * nrm[d];
     mm = sqrt(mm);
     for (int d = 0; d < DIM; d++) nrm[d] /= mm;
-    for (int iv = 0; iv < DIM; iv++) {
+    {
       int ckd = dof_top[gn[NB - 1] + iv * Ntop];
       if (bc0[ckd] == 0) {
         double Ipen = jac_sur<DIM, NB, NGB>(NGB - 1, xb, tab) / a.dt;
@@ -748,13 +818,13 @@ int ns_assemble(hipStream_t s, int dim, int nel, const int *elems, const int *co
     if (!bd_only)
       hipLaunchKernelGGL(k_ns_vol<2>, dim3(nblk(nel)), dim3(256), 0, s, nel, elems, conn, eldof, elpos, xyz, Ntop,
                          dof_top, bc0, x_old, coef, tab, a, mode, trans, val, rhs);
-    hipLaunchKernelGGL(k_ns_bd<2>, dim3((nel + 127) / 128), dim3(128), 0, s, nel, elems, conn, eldof, elpos, bmask,
+    hipLaunchKernelGGL(k_ns_bd<2>, dim3((2 * nel + 127) / 128), dim3(128), 0, s, nel, elems, conn, eldof, elpos, bmask,
                        xyz, Ntop, dof_top, bc0, bc1, xb, tab, a, mode, trans, val, rhs);
   } else {
     if (!bd_only)
       hipLaunchKernelGGL(k_ns_vol<3>, dim3(nblk(nel)), dim3(256), 0, s, nel, elems, conn, eldof, elpos, xyz, Ntop,
                          dof_top, bc0, x_old, coef, tab, a, mode, trans, val, rhs);
-    hipLaunchKernelGGL(k_ns_bd<3>, dim3((nel + 127) / 128), dim3(128), 0, s, nel, elems, conn, eldof, elpos, bmask,
+    hipLaunchKernelGGL(k_ns_bd<3>, dim3((3 * nel + 127) / 128), dim3(128), 0, s, nel, elems, conn, eldof, elpos, bmask,
                        xyz, Ntop, dof_top, bc0, bc1, xb, tab, a, mode, trans, val, rhs);
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
