@@ -1,0 +1,21 @@
+"""Builds tests/cpp/_build/mgsol_driver (the C++ host-layer driver) against the in-tree libfemus_hip.so."""
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+OUT = os.path.join(HERE, "_build", "mgsol_driver")
+
+
+def build():
+    os.makedirs(os.path.dirname(OUT), exist_ok=True)
+    lib_dir = os.path.join(ROOT, "exfemus_amd")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+           os.path.join(HERE, "mgsol_driver.cpp"), "-L", lib_dir, "-lfemus_hip",
+           "-Wl,-rpath,$ORIGIN/../../../exfemus_amd", "-o", OUT]
+    subprocess.run(cmd, check=True)
+    return OUT
+
+
+if __name__ == "__main__":
+    print(build())

@@ -45,3 +45,17 @@ def test_error_path_without_device():
     h = C.c_void_p()
     rc = lib.fm_ctx_create(-1, C.byref(h))
     assert rc < 0
+
+
+def test_cpp_host_layer_compiles():
+    """include/femus_hip.hpp (MGSolDA / MGSolT / MGSolNS over the C ABI) compiles with the host compiler,
+    together with the application-style driver that the GPU tests run."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no g++")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([gxx, "-fsyntax-only", "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(root, "include"),
+                        os.path.join(root, "tests", "cpp", "mgsol_driver.cpp")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

@@ -1,0 +1,62 @@
+"""The C++ host layer (include/femus_hip.hpp: MGSolDA / MGSolT / MGSolNS over the C ABI) driven like a FEMuS
+application (GenBc -> init -> GenIc -> MGTimeStep), by the compiled driver tests/cpp/mgsol_driver.cpp.
+
+Heat: the solution equals the FP64 direct solve of the oracle's assembly (to 1e-7).  Cavity: three Picard
+steps through the C++ layer equal the same steps through the Python mirror of the interface."""
+import functools
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from exfemus_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+DRIVER = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "_build", "mgsol_driver")
+
+
+def _run(*args):
+    if not os.path.exists(DRIVER):
+        pytest.fail("C++ driver not built: run __graft_entry__.build()")
+    out = subprocess.run([DRIVER, *map(str, args)], capture_output=True, text=True, timeout=180)
+    assert out.returncode == 0, out.stderr[-2000:]
+    return json.loads(out.stdout)
+
+
+@pytest.mark.parametrize("dim,nx,nlev", [(2, 2, 4), (3, 2, 3)])
+def test_cpp_heat_matches_oracle_direct_solve(oracle, dim, nx, nlev):
+    r = _run("heat", dim, nx, nlev)
+    om = oracle.Mesh(dim, [nx] * dim, [0.0] * dim, [1.0] * dim, nlev)
+    top = nlev - 1
+    d, n = oracle.init_dof(om, top, 1, 0)
+    b0, b1 = oracle.genbc(om, 1, 0, d, configs.heat_dirichlet_all)
+    rp, ci = oracle.system_pattern(om, top, 1, 0, d)
+    p = oracle.HeatParams(1.0, 1.0, 0.1, 0.0, 1.0, 1.0, (0.0, 0.0, 0.0))
+    v, rhs = oracle.heat_genmatrhs(om, top, 1, p, d, d, b0, b1, np.zeros(n), rp, ci)
+    xref = spla.spsolve(sp.csr_matrix((v, ci, rp), shape=(n, n)).tocsc(), rhs)
+    x = np.array(r["x"])
+    assert r["ndof"] == n and r["resid"] <= 1e-8
+    np.testing.assert_allclose(x, xref, rtol=1e-7, atol=1e-9 * np.abs(xref).max())
+
+
+def test_cpp_cavity_picard_matches_python_mirror(femus, ctx):
+    r = _run("cavity", 4, 3, 3)
+    m = femus.Mesh(2, [4, 4], [0, 0], [1, 1], 3)
+    ns = femus.MGSolNS(ctx, m, IRe=0.01, dt=0.1, time_coeff=1.0, upwind=0.1)
+    ns.GenBc(functools.partial(configs.cavity_bc, dim=2))
+    ns.init()
+    ns.GenIc(functools.partial(configs.cavity_ic, dim=2))
+    o = ns.opts(precision=femus.FM_PREC_FP64_FGMRES, smoother=femus.FM_SMOOTH_VANKA, vanka_colored=1, nu_pre=4,
+                nu_post=4, tol=1e-10, max_iter=200, gamma=1, cheb_lo=0.1)
+    for k in range(3):
+        rep = ns.MGTimeStep(0.1 * (k + 1), opts=o)
+        assert rep.converged == 1
+    x = ns.get_x()
+    xc = np.array(r["x"])
+    assert r["ndof"] == len(x)
+    np.testing.assert_allclose(xc, x, rtol=1e-9, atol=1e-11 * np.abs(x).max())
