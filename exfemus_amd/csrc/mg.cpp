@@ -443,6 +443,14 @@ __global__ void k_multi_add(int64_t n, int k, const double *__restrict__ Z, cons
   }
 }
 
+// w /= nrm[0] (device scalar; left unchanged when nrm[0] == 0)
+__global__ void k_scale_inv(int64_t n, double *w, const double *nrm) {
+  const double d = nrm[0];
+  if (!(d > 0.)) return;
+  const double r = 1.0 / d;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) w[i] *= r;
+}
+
 static int solve_fgmres(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &ops, fm_mg_report *rep, int adj) {
   fm_ctx *c = s->ctx;
   hipStream_t st = c->stream;
@@ -454,7 +462,8 @@ static int solve_fgmres(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &op
   double *V = nullptr, *Z = nullptr, *hd = nullptr;
   if (hipMalloc((void **)&V, (size_t)(m + 1) * n * sizeof(double)) != hipSuccess) return FM_ENOMEM;
   if (hipMalloc((void **)&Z, (size_t)m * n * sizeof(double)) != hipSuccess) { (void)hipFree(V); return FM_ENOMEM; }
-  if (hipMalloc((void **)&hd, 128 * sizeof(double)) != hipSuccess) { (void)hipFree(V); (void)hipFree(Z); return FM_ENOMEM; }
+  // hd: [0, 64) first Gram-Schmidt pass, [64, 128) second pass, [128] norm of the new vector, [192, ...) y
+  if (hipMalloc((void **)&hd, 256 * sizeof(double)) != hipSuccess) { (void)hipFree(V); (void)hipFree(Z); return FM_ENOMEM; }
   const int nb = 512;
   double *part = c->scratch;  // (m+1) * nb <= 4096 doubles: m <= 6 uses scratch; else allocate
   double *part_alloc = nullptr;
@@ -462,7 +471,7 @@ static int solve_fgmres(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &op
     if (hipMalloc((void **)&part_alloc, (size_t)(m + 1) * nb * sizeof(double)) != hipSuccess) { (void)hipFree(V); (void)hipFree(Z); (void)hipFree(hd); return FM_ENOMEM; }
     part = part_alloc;
   }
-  std::vector<double> H((size_t)(m + 1) * m, 0.), cs(m), sn(m), g(m + 1), y(m), hh(m + 1);
+  std::vector<double> H((size_t)(m + 1) * m, 0.), cs(m), sn(m), g(m + 1), y(m);
   CsrView<double> A = cview<double>(*ot.A);
   int rc = 0;
   double bn = dev_dot<double>(c, n, l->b, l->b, true);
@@ -502,20 +511,21 @@ static int solve_fgmres(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &op
           if ((rc = fmk::fill<double>(st, n, zj, 0.0)) != 0) goto done;
           if ((rc = fmk::add_f32(st, n, zj, l->xf, nullptr)) != 0) goto done;
         }
-        // w = A z_j ; classical Gram-Schmidt twice
+        // w = A z_j ; classical Gram-Schmidt twice and the normalisation, all on the device; one
+        // copy of the Hessenberg column back to the host per iteration
         if ((rc = fmk::spmv(st, A, zj, w)) != 0) goto done;
-        for (int k = 0; k <= j; k++) H[(size_t)k * m + j] = 0.;
         for (int pass = 0; pass < 2; pass++) {
           hipLaunchKernelGGL(k_multi_dot, dim3(nb), dim3(256), 4 * sizeof(double), st, n, j + 1, V, w, part);
-          hipLaunchKernelGGL(k_multi_sum, dim3(j + 1), dim3(256), 0, st, j + 1, nb, part, hd);
-          hipLaunchKernelGGL(k_multi_axpy, dim3(gv), dim3(256), 0, st, n, j + 1, V, hd, w);
-          if (hipMemcpyAsync(hh.data(), hd, (j + 1) * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess) { rc = FM_EHIP; goto done; }
-          if (hipStreamSynchronize(st) != hipSuccess) { rc = FM_EHIP; goto done; }
-          for (int k = 0; k <= j; k++) H[(size_t)k * m + j] += hh[k];
+          hipLaunchKernelGGL(k_multi_sum, dim3(j + 1), dim3(256), 0, st, j + 1, nb, part, hd + 64 * pass);
+          hipLaunchKernelGGL(k_multi_axpy, dim3(gv), dim3(256), 0, st, n, j + 1, V, hd + 64 * pass, w);
         }
-        double hn = dev_dot<double>(c, n, w, w, true);
+        if ((rc = fmk::dot_dev<double>(st, n, w, w, c->scratch, hd + 128, 1)) != 0) goto done;
+        hipLaunchKernelGGL(k_scale_inv, dim3(gv), dim3(256), 0, st, n, w, hd + 128);
+        if (hipMemcpyAsync(c->h_scratch, hd, 129 * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess) { rc = FM_EHIP; goto done; }
+        if (hipStreamSynchronize(st) != hipSuccess) { rc = FM_EHIP; goto done; }
+        for (int k = 0; k <= j; k++) H[(size_t)k * m + j] = c->h_scratch[k] + c->h_scratch[64 + k];
+        const double hn = c->h_scratch[128];
         H[(size_t)(j + 1) * m + j] = hn;
-        if (hn > 0.) { if ((rc = fmk::axpby<double>(st, n, 1.0 / hn, w, 0.0, w)) != 0) goto done; }
         // Givens
         for (int k = 0; k < j; k++) {
           double a = H[(size_t)k * m + j], b = H[(size_t)(k + 1) * m + j];
@@ -539,8 +549,8 @@ static int solve_fgmres(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &op
         for (int k = i + 1; k < j; k++) sacc -= H[(size_t)i * m + k] * y[k];
         y[i] = sacc / H[(size_t)i * m + i];
       }
-      if (hipMemcpyAsync(hd, y.data(), j * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
-      hipLaunchKernelGGL(k_multi_add, dim3(gv), dim3(256), 0, st, n, j, Z, hd, l->x);
+      if (hipMemcpyAsync(hd + 192, y.data(), j * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
+      hipLaunchKernelGGL(k_multi_add, dim3(gv), dim3(256), 0, st, n, j, Z, hd + 192, l->x);
     }
   }
   // true FP64 residual at exit
