@@ -122,7 +122,7 @@ struct fm_ctx {
 
 // a captured V-cycle (one per precision / option set / forward-or-adjoint)
 struct GraphEntry {
-  int prec, adj, xz, smoother, nu_pre, nu_post, correction, gamma, colored;
+  int prec, adj, xz, smoother, nu_pre, nu_post, correction, gamma, colored, cmult;
   double lo, hi, omega;
   hipGraphExec_t exec;
 };

@@ -252,7 +252,7 @@ static int vcycle(fm_system *s, const fm_mg_opts *opt, std::vector<LOps> &ops, i
     return 0;
   }
   int top = (int)ops.size() - 1;
-  int mult = L < top ? 2 : 1;
+  const int mult = L < top ? (opt->coarse_mult > 0 ? opt->coarse_mult : 2) : 1;
   FM_CHECK(smooth<T>(s, opt, L, o, opt->nu_pre * mult, xz));
   CsrView<T> A = cview<T>(*o.A);
   T *x = Vec<T>::x(l), *b = Vec<T>::b(l), *r = Vec<T>::r(l);
@@ -295,7 +295,7 @@ static int cycle_top(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &ops, 
   hipStream_t st = s->ctx->stream;
   for (auto &g : s->graphs)
     if (g.prec == prec && g.adj == adj && g.xz == (int)xz && g.smoother == o->smoother && g.nu_pre == o->nu_pre && g.nu_post == o->nu_post &&
-        g.correction == o->correction && g.gamma == o->gamma && g.colored == o->vanka_colored && g.lo == o->cheb_lo &&
+        g.correction == o->correction && g.gamma == o->gamma && g.cmult == o->coarse_mult && g.colored == o->vanka_colored && g.lo == o->cheb_lo &&
         g.hi == o->cheb_hi && g.omega == o->omega)
       return hipGraphLaunch(g.exec, st) == hipSuccess ? 0 : FM_EHIP;
   hipGraph_t graph = nullptr;
@@ -309,7 +309,7 @@ static int cycle_top(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &ops, 
   }
   GraphEntry g;
   g.prec = prec; g.adj = adj; g.xz = (int)xz; g.smoother = o->smoother; g.nu_pre = o->nu_pre; g.nu_post = o->nu_post;
-  g.correction = o->correction; g.gamma = o->gamma; g.colored = o->vanka_colored; g.lo = o->cheb_lo; g.hi = o->cheb_hi;
+  g.correction = o->correction; g.gamma = o->gamma; g.cmult = o->coarse_mult; g.colored = o->vanka_colored; g.lo = o->cheb_lo; g.hi = o->cheb_hi;
   g.omega = o->omega;
   e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);

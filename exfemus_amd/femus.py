@@ -67,7 +67,8 @@ class MGOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("smoother", C.c_int), ("nu_pre", C.c_int), ("nu_post", C.c_int),
                 ("correction", C.c_int), ("max_iter", C.c_int), ("fgmres_restart", C.c_int),
                 ("tol", C.c_double), ("cheb_lo", C.c_double), ("cheb_hi", C.c_double),
-                ("omega", C.c_double), ("vanka_colored", C.c_int), ("gamma", C.c_int), ("use_graph", C.c_int)]
+                ("omega", C.c_double), ("vanka_colored", C.c_int), ("gamma", C.c_int), ("use_graph", C.c_int),
+                ("coarse_mult", C.c_int)]
 
 
 class MGReport(C.Structure):
@@ -353,11 +354,11 @@ class MGSolDA:
     @staticmethod
     def opts(precision=FM_PREC_FP64, smoother=FM_SMOOTH_CHEBYSHEV, nu_pre=2, nu_post=2, correction=1,
              max_iter=100, fgmres_restart=30, tol=1e-8, cheb_lo=0.2, cheb_hi=1.05, omega=None, vanka_colored=0,
-             gamma=1, use_graph=1):
+             gamma=1, use_graph=1, coarse_mult=0):
         if omega is None:
             omega = 0.5 if smoother == FM_SMOOTH_VANKA else 0.6
         return MGOpts(precision, smoother, nu_pre, nu_post, correction, max_iter, fgmres_restart, tol,
-                      cheb_lo, cheb_hi, omega, vanka_colored, gamma, use_graph)
+                      cheb_lo, cheb_hi, omega, vanka_colored, gamma, use_graph, coarse_mult)
 
     def MGSetup(self, opts):
         _ck(lib().fm_mg_setup(self.h, C.byref(opts)))

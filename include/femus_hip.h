@@ -160,6 +160,7 @@ typedef struct {
   int vanka_colored;  /* 1 = multiplicative over 2^dim element colours       */
   int gamma;          /* coarse-grid visits per level: 1 = V-cycle, 2 = W (MGStep Gamma) */
   int use_graph;      /* 1 = capture each cycle once into a HIP graph and replay it */
+  int coarse_mult;    /* sweep multiplier below the top level; 0 = 2, the reference's MGStep rule */
 } fm_mg_opts;
 
 typedef struct {

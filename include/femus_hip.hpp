@@ -94,7 +94,7 @@ class MGSolDA {
       _node_dof[L].resize((size_t)(nq2 + nq1) * mesh.n_nodes());
       check(fm_system_node_dof(h_, L, _node_dof[L].data()), "fm_system_node_dof");
     }
-    opts = fm_mg_opts{FM_PREC_FP32_FGMRES, FM_SMOOTH_CHEBYSHEV, 2, 2, 1, 40, 30, 1e-8, 0.1, 1.05, 0.6, 0, 1, 1};
+    opts = fm_mg_opts{FM_PREC_FP32_FGMRES, FM_SMOOTH_CHEBYSHEV, 2, 2, 1, 40, 30, 1e-8, 0.1, 1.05, 0.6, 0, 1, 1, 0};
   }
   virtual ~MGSolDA() { if (h_) fm_system_destroy(h_); }
   MGSolDA(const MGSolDA &) = delete;
