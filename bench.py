@@ -286,7 +286,10 @@ def main():
     kw["use_graph"] = 0 if args.no_graph else 1
     om = s.opts(precision=prec_mixed, tol=args.tol, max_iter=200, **kw)
     o64 = s.opts(precision=prec_fp64, tol=args.tol, max_iter=200, **kw)
+    t0 = time.perf_counter()
     s.MGSetup(o64)
+    ctx.sync()
+    t_mgsetup = time.perf_counter() - t0
     zero = np.zeros(N)
 
     def run(o):
@@ -362,6 +365,7 @@ def main():
         "rel_l2_err_vs_fp64": errs,
         "final_rel_resid": reps[-1].rel_resid,
         "assembly_ms_all_levels": round(t_asm * 1e3, 2),
+        "mg_setup_ms": round(t_mgsetup * 1e3, 2),
         "spmv": {"fp64_us": round(sec64 * 1e6, 2), "fp64_GBs": round(gbs64, 1), "fp64_frac": round(gbs64 / HBM_PEAK_GBS, 4),
                  "fp32_us": round(sec32 * 1e6, 2), "fp32_GBs": round(gbs32, 1), "fp32_frac": round(gbs32 / HBM_PEAK_GBS, 4),
                  "bytes_alg_fp64": b64, "bytes_alg_fp32": b32,

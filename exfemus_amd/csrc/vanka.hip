@@ -27,9 +27,9 @@ __global__ __launch_bounds__(256) void k_vanka_invert(int nel, const int *__rest
                                                       int *__restrict__ status) {
   __shared__ double M[NL * NL];
   __shared__ int piv[NL];
-  __shared__ double s_best;
+  __shared__ double s_best, s_red[4];
   __shared__ int s_bi;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   for (int e = blockIdx.x; e < nel; e += gridDim.x) {
     __syncthreads();
     const int64_t base = (int64_t)e * NL * NL;
@@ -38,26 +38,39 @@ __global__ __launch_bounds__(256) void k_vanka_invert(int nel, const int *__rest
       int p = elpos[base + t];
       double v = p >= 0 ? Av[p] : 0.0;
       M[t] = v;
+      amax = fmax(amax, fabs(v));
     }
+    // block maximum of |M| (the singularity threshold)
+    for (int off = 32; off > 0; off >>= 1) amax = fmax(amax, __shfl_xor(amax, off, 64));
+    if (lane == 0) s_red[wid] = amax;
     __syncthreads();
     if (tid == 0) {
-      for (int t = 0; t < NL * NL; t++) amax = fmax(amax, fabs(M[t]));
-      s_best = amax;
+      double m = 0.;
+      for (int w = 0; w < (int)(blockDim.x >> 6); w++) m = fmax(m, s_red[w]);
+      s_best = m;
     }
     __syncthreads();
     const double tiny = 1e-14 * s_best;
     bool singular = false;
     for (int k = 0; k < NL; k++) {
-      if (tid == 0) {
-        int bi = k;
-        double bv = fabs(M[k * NL + k]);
-        for (int i = k + 1; i < NL; i++) {
-          double v = fabs(M[i * NL + k]);
+      // partial pivoting: the first wave finds the largest |M[i][k]|, i >= k (first index on ties)
+      if (tid < 64) {
+        int bi = NL;
+        double bv = -1.;
+        for (int i = k + lane; i < NL; i += 64) {
+          const double v = fabs(M[i * NL + k]);
           if (v > bv) { bv = v; bi = i; }
         }
-        s_bi = bi;
-        piv[k] = bi;
-        s_best = bv;
+        for (int off = 32; off > 0; off >>= 1) {
+          const double ov = __shfl_xor(bv, off, 64);
+          const int oi = __shfl_xor(bi, off, 64);
+          if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+        }
+        if (tid == 0) {
+          s_bi = bi;
+          piv[k] = bi;
+          s_best = bv;
+        }
       }
       __syncthreads();
       if (!(s_best > tiny)) { singular = true; break; }
