@@ -102,3 +102,44 @@ def lid_fourier_basis(ns, K, dim=2):
         for k in range(K):
             G[k, d] = np.sin((k + 1) * np.pi * xyz[0, node])
     return G
+
+
+def control_loop(problem, nu, a, eta=1.0, max_func_iter=50, toll_conv=1e-6, which="lid", bounds=None):
+    """Steepest-descent control driven by the discrete gradient, with the step rule of the reference's
+    optimal-control loop (MGEquationsSystem::eqnmap_timestep_loop_control, src/MGEquationsSystem.C:435-561):
+    solve the state, solve the adjoint, try a step of size eta along the negative gradient; a step that lowers
+    the functional is kept and eta grows by 1.5, a step that raises it is undone and eta shrinks by 0.66; stop
+    when two consecutive functionals differ by less than toll_conv (relative) or after max_func_iter trials.
+
+    which = "lid" moves the boundary-data coefficients a, "nu" the element viscosities nu (kept inside bounds,
+    a (lo, hi) pair, when given).  Returns (nu, a, history) with history = [(J, eta, accepted), ...]."""
+    nu = np.asarray(nu, np.float64).copy()
+    a = np.asarray(a, np.float64).copy()
+    J0 = problem.solve(nu, a)
+    g_nu, g_a = problem.gradient()
+    history = [(J0, eta, True)]
+    for _ in range(max_func_iter):
+        if which == "lid":
+            step = -eta * g_a
+            a1, nu1 = a + step, nu
+        else:
+            nu1 = nu - eta * g_nu * nu          # relative step on the (positive) viscosities
+            if bounds is not None:
+                nu1 = np.clip(nu1, bounds[0], bounds[1])
+            a1 = a
+        J1 = problem.solve(nu1, a1)
+        if abs(J0 - J1) / abs(J0) < toll_conv:
+            history.append((J1, eta, J1 < J0))
+            if J1 < J0:
+                nu, a, J0 = nu1, a1, J1
+            break
+        if J1 < J0:
+            nu, a, J0 = nu1, a1, J1
+            g_nu, g_a = problem.gradient()
+            history.append((J1, eta, True))
+            eta *= 1.5
+        else:
+            history.append((J1, eta, False))
+            eta *= 0.66
+    problem.solve(nu, a)                        # leave the state at the accepted control
+    return nu, a, history

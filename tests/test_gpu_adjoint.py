@@ -394,3 +394,18 @@ def test_channel_pressure_drop_gradient_fd(femus, ctx):
         assert abs(fd - g_nu[e]) <= 1e-5 * abs(g_nu[e]) + 1e-9 * abs(J0), (e, fd, g_nu[e])
     fd = (J_of(nu[top], a0 + h) - J_of(nu[top], a0 - h)) / (2 * h)
     assert abs(fd - g_a[0]) <= 1e-5 * abs(g_a[0])
+
+
+def test_control_loop_lowers_dissipation(femus, ctx):
+    """The optimal-control loop (reference step rule: x1.5 after a descent, x0.66 after an ascent) driven by the
+    GPU gradient lowers the cavity's dissipation through the lid coefficients, and every accepted step lowers it."""
+    from exfemus_amd import adjoint
+    m, ns, G, a0, nu, prob = _cavity_adjoint(femus, ctx, K=4)
+    J0 = prob.solve(nu, a0)
+    g_nu, g_a = prob.gradient()
+    eta = 0.1 * abs(J0) / max(np.linalg.norm(g_a) ** 2, 1e-300)
+    nu1, a1, hist = adjoint.control_loop(prob, nu, a0, eta=eta, max_func_iter=8, which="lid")
+    accepted = [h[0] for h in hist if h[2]]
+    assert len(accepted) >= 3
+    assert all(accepted[i + 1] < accepted[i] for i in range(len(accepted) - 1))
+    assert accepted[-1] < 0.9 * J0
