@@ -328,6 +328,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   __shared__ double s_phi[N2 * NG], s_psi[N1 * NG];
   __shared__ double s_xx[DIM * N2], s_uold[DIM * N2];
   __shared__ double s_JxW[NG], s_vel[NG * DIM], s_fup[NG * DIM], s_cl[NG * DIM], s_IJ[NG * DIM * DIM];
+  __shared__ double s_Tn[N2], s_Tq[NG];
   __shared__ int s_bcv[DIM * N2];
   const int tid = threadIdx.x;
   const double tcdt = a.tc / a.dt;
@@ -344,6 +345,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         s_uold[tid + d * N2] = x_old[kd];
         s_bcv[tid + d * N2] = bc0[kd];
       }
+      s_Tn[tid] = a.Tn ? a.Tn[node] : 0.0;
     }
     __syncthreads();
     const double nu = coef ? coef[e] : a.IRe;
@@ -352,13 +354,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       double IJ[DIM * DIM];
       const double det = jac_qp<DIM, N2, NG>(qp, s_xx, tab->dphi2, IJ);
       s_JxW[qp] = det * tab->w[qp];
-      double vg[DIM];
+      double vg[DIM], Tq = 0.;
       for (int d = 0; d < DIM; d++) vg[d] = 0.;
 #pragma unroll 3
       for (int sn = 0; sn < N2; sn++) {
         const double ph = s_phi[sn * NG + qp];
         for (int d = 0; d < DIM; d++) vg[d] += ph * s_uold[sn + d * N2];
+        Tq += ph * s_Tn[sn];
       }
+      s_Tq[qp] = Tq;
       double h_eff[DIM];
       for (int d = 0; d < DIM; d++) {
         double h = fabs(s_xx[d * N2 + 2 + 4 * (DIM - 2)] - s_xx[d * N2 + 0]);
@@ -483,7 +487,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         double F = 0.;
         for (int qp = 0; qp < NG; qp++) {
           const double phii = s_phi[i * NG + qp];
-          F += s_JxW[qp] * (tcdt * s_vel[qp * DIM + iv] * phii + a.IFr * a.dirg[iv] * phii + a.src[iv] * phii);
+          F += s_JxW[qp] * (tcdt * s_vel[qp * DIM + iv] * phii + a.IFr * a.dirg[iv] * phii + a.src[iv] * phii +
+                            a.buoy[iv] * s_Tq[qp] * phii);
         }
         rhs[eldof[(int64_t)e * NL + tid]] += F;
       }
@@ -748,7 +753,93 @@ __global__ __launch_bounds__(128) void k_ns_visc(int nel, const int *__restrict_
   }
 }
 
+// ===========================================================================
+// Nusselt number of a differentially heated box (top level):
+//   J = sum_e int_e (u_a T / kappa - k_e dT/dx_a)          (a = axis of the heated walls)
+// the heat flux through the box along axis a in units of the conductive one,
+// averaged over the box (at steady state the flux through every plane
+// x_a = const is the same, so this is the hot-wall Nusselt number).
+//   Jel[e] = J_e,  dJdk[e] = -int_e dT/dx_a,
+//   y[dof_i] += sum_q JxW (u_a phi_i / kappa - k_e d phi_i / dx_a)      (dJ/dT, colour by colour)
+// ===========================================================================
+template <int DIM>
+__global__ __launch_bounds__(64) void k_heat_nusselt(int nel, const int *__restrict__ elems, const int *__restrict__ conn,
+                                                     const int *__restrict__ eldof, const double *__restrict__ xyz, int Ntop,
+                                                     const int *__restrict__ dof_top, const double *__restrict__ vel,
+                                                     const double *__restrict__ T, const double *__restrict__ coef,
+                                                     double kappa, int axis, const FeTab *__restrict__ tab,
+                                                     double *__restrict__ Jel, double *__restrict__ dJdk,
+                                                     double *__restrict__ y) {
+  constexpr int N2 = DIM == 2 ? 9 : 27, NG = N2;
+  __shared__ double s_xx[DIM * N2], s_T[N2], s_u[N2];
+  __shared__ double s_gx[NG * N2], s_JxW[NG], s_uq[NG], s_Tq[NG], s_dT[NG];
+  __shared__ int s_dof[N2];
+  const int tid = threadIdx.x;
+  for (int ei = blockIdx.x; ei < nel; ei += gridDim.x) {
+    const int e = elems[ei];
+    __syncthreads();
+    if (tid < N2) {
+      const int node = conn[(int64_t)e * N2 + tid];
+      for (int d = 0; d < DIM; d++) s_xx[tid + d * N2] = xyz[node + (int64_t)d * Ntop];
+      s_T[tid] = T[dof_top[node]];
+      s_u[tid] = vel ? vel[node + (int64_t)axis * Ntop] : 0.0;
+      s_dof[tid] = eldof[(int64_t)e * N2 + tid];
+    }
+    __syncthreads();
+    const double ke = coef ? coef[e] : 1.0;
+    if (tid < NG) {
+      const int qp = tid;
+      double IJ[DIM * DIM];
+      const double det = jac_qp<DIM, N2, NG>(qp, s_xx, tab->dphi2, IJ);
+      s_JxW[qp] = det * tab->w[qp];
+      double Tq = 0., uq = 0., dT = 0.;
+      for (int sn = 0; sn < N2; sn++) {
+        const double ph = tab->phi2[sn * NG + qp];
+        Tq += ph * s_T[sn];
+        uq += ph * s_u[sn];
+        double g = 0.;
+        for (int j = 0; j < DIM; j++) g += IJ[j + axis * DIM] * tab->dphi2[sn * NG + qp + j * N2 * NG];
+        s_gx[qp * N2 + sn] = g;
+        dT += g * s_T[sn];
+      }
+      s_uq[qp] = uq;
+      s_Tq[qp] = Tq;
+      s_dT[qp] = dT;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double a0 = 0., a1 = 0.;
+      for (int q = 0; q < NG; q++) {
+        a0 += s_JxW[q] * (s_uq[q] * s_Tq[q] / kappa - ke * s_dT[q]);
+        a1 -= s_JxW[q] * s_dT[q];
+      }
+      if (Jel) Jel[e] = a0;
+      if (dJdk) dJdk[e] = a1;
+    }
+    if (y && tid < N2) {
+      double acc = 0.;
+      for (int q = 0; q < NG; q++)
+        acc += s_JxW[q] * (s_uq[q] * tab->phi2[tid * NG + q] / kappa - ke * s_gx[q * N2 + tid]);
+      y[s_dof[tid]] += acc;
+    }
+  }
+}
+
 namespace fmk {
+
+int heat_nusselt(hipStream_t s, int dim, int nel, const int *elems, const int *conn, const int *eldof, const double *xyz,
+                 int Ntop, const int *dof_top, const double *vel, const double *T, const double *coef, double kappa,
+                 int axis, const FeTab *tab, double *Jel, double *dJdk, double *y) {
+  if (nel <= 0) return 0;
+  unsigned g = (unsigned)(nel < 8192 ? nel : 8192);
+  if (dim == 2)
+    hipLaunchKernelGGL(k_heat_nusselt<2>, dim3(g), dim3(64), 0, s, nel, elems, conn, eldof, xyz, Ntop, dof_top, vel, T,
+                       coef, kappa, axis, tab, Jel, dJdk, y);
+  else
+    hipLaunchKernelGGL(k_heat_nusselt<3>, dim3(g), dim3(64), 0, s, nel, elems, conn, eldof, xyz, Ntop, dof_top, vel, T,
+                       coef, kappa, axis, tab, Jel, dJdk, y);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
 
 int ns_visc(hipStream_t s, int dim, int nel, const int *elems, const int *conn, const int *eldof, const double *xyz,
             int Ntop, const int *dof_top, const int *bc0, const double *U, const double *lam, const double *coef,

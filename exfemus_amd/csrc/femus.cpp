@@ -372,6 +372,7 @@ extern "C" int fm_system_destroy(fm_system *s) {
   for (auto &l : s->lev) fm_free_level(l);
   for (auto &l : s->alev) fm_free_level(l);
   dfree(s->d_bc0); dfree(s->d_bc1); dfree(s->d_dof_top); dfree(s->xyz); dfree(s->x_old); dfree(s->vel); dfree(s->xbc);
+  dfree(s->tnode);
   SysExtra &ex = extra(s);
   dfree(ex.d_tab);
   for (size_t i = 0; i < g_extra.size(); i++) if (g_extra[i].first == s) { g_extra.erase(g_extra.begin() + i); break; }
@@ -829,7 +830,8 @@ static int bc_rhs(fm_system *s, const void *params, const double *g, double *rhs
       if (p->dt == 0.) return FM_EINVAL;
       NsArgs a;
       a.IRe = p->IRe; a.dt = p->dt; a.tc = p->time_coeff; a.upwind = p->upwind; a.IFr = p->IFr;
-      for (int d = 0; d < 3; d++) { a.dirg[d] = p->dirg[d]; a.src[d] = p->src[d]; }
+      for (int d = 0; d < 3; d++) { a.dirg[d] = p->dirg[d]; a.src[d] = p->src[d]; a.buoy[d] = 0.; }
+      a.Tn = nullptr;
       FM_CHECK(fmk::ns_assemble(st, m.dim, n, el, l.conn, l.eldof, l.elpos, l.bmask, s->xyz, m.Ntop, s->d_dof_top,
                                 s->d_bc0, s->d_bc1, s->x_old, l.t, l.coef, ex.d_tab, a, 1, 0, 1, nullptr, l.d));
     }
@@ -846,6 +848,21 @@ extern "C" int fm_heat_bc_rhs(fm_system *s, const fm_heat_params *p, const doubl
 extern "C" int fm_ns_bc_rhs(fm_system *s, const fm_ns_params *p, const double *g, double *rhs) {
   if (!s || !s->mesh || s->nq2 != s->mesh->dim || s->nq1 != 1) return FM_EINVAL;
   return bc_rhs(s, p, g, rhs);
+}
+
+// Boussinesq coupling: body force coef * T in the NS momentum RHS, T a nodal
+// temperature field (top-level nodes); NULL T removes it.
+extern "C" int fm_ns_set_buoyancy(fm_system *s, const double *T_node, const double coef[3]) {
+  FM_CHECK(check_level(s, 0));
+  if (!s->mesh || s->nq2 != s->mesh->dim || s->nq1 != 1) return FM_EINVAL;
+  if (!T_node) { dfree(s->tnode); for (int d = 0; d < 3; d++) s->buoy[d] = 0.; return FM_OK; }
+  if (!coef) return FM_EINVAL;
+  const size_t N = (size_t)s->mesh->Ntop;
+  if (!s->tnode) FM_CHECK(dalloc(&s->tnode, N));
+  FM_HIP(hipMemcpyAsync(s->tnode, T_node, N * sizeof(double), hipMemcpyHostToDevice, s->ctx->stream));
+  FM_HIP(hipStreamSynchronize(s->ctx->stream));
+  for (int d = 0; d < 3; d++) s->buoy[d] = coef[d];
+  return FM_OK;
 }
 
 extern "C" int fm_system_set_velocity(fm_system *s, const double *vel) {
@@ -902,7 +919,8 @@ extern "C" int fm_ns_genmatrhs(fm_system *s, int L, int mode, const fm_ns_params
   hipStream_t st = s->ctx->stream;
   NsArgs a;
   a.IRe = p->IRe; a.dt = p->dt; a.tc = p->time_coeff; a.upwind = p->upwind; a.IFr = p->IFr;
-  for (int d = 0; d < 3; d++) { a.dirg[d] = p->dirg[d]; a.src[d] = p->src[d]; }
+  for (int d = 0; d < 3; d++) { a.dirg[d] = p->dirg[d]; a.src[d] = p->src[d]; a.buoy[d] = s->buoy[d]; }
+  a.Tn = s->tnode;
   FM_HIP(hipMemsetAsync(l.A.sell.v, 0, l.A.sell.nslot * sizeof(double), st));
   if (mode == 1) FM_HIP(hipMemsetAsync(l.b, 0, l.n * sizeof(double), st));
   SysExtra &ex = extra(s);
@@ -1090,7 +1108,8 @@ extern "C" int fm_ns_genmat_transpose(fm_system *s, int L, const fm_ns_params *p
   hipStream_t st = s->ctx->stream;
   NsArgs h;
   h.IRe = p->IRe; h.dt = p->dt; h.tc = p->time_coeff; h.upwind = p->upwind; h.IFr = p->IFr;
-  for (int d = 0; d < 3; d++) { h.dirg[d] = p->dirg[d]; h.src[d] = p->src[d]; }
+  for (int d = 0; d < 3; d++) { h.dirg[d] = p->dirg[d]; h.src[d] = p->src[d]; h.buoy[d] = 0.; }
+  h.Tn = nullptr;
   FM_HIP(hipMemsetAsync(a.A.sell.v, 0, a.A.sell.nslot * sizeof(double), st));
   SysExtra &ex = extra(s);
   for (int k = 0; k < l.ncolor; k++) {
@@ -1317,5 +1336,45 @@ extern "C" int fm_ns_pressure_drop(fm_system *s, const double *U, double *J, dou
   }
   if (J) *J = acc;
   if (dJdU) std::memcpy(dJdU, w.data(), nt * sizeof(double));
+  return FM_OK;
+}
+
+// Nusselt number of a differentially heated box along `axis` (top level):
+// J = sum_e int_e (u_a T / kappa - k_e dT/dx_a) with the nodal advection field
+// of fm_system_set_velocity, kappa = alpha0; dJ/dT (reference numbering) and
+// the partials dJ/dk_e = -int_e dT/dx_a.
+extern "C" int fm_heat_nusselt(fm_system *s, const fm_heat_params *p, int axis, const double *T_host, double *J,
+                               double *dJdT_host, double *dJdk_host) {
+  FM_CHECK(check_level(s, 0));
+  if (!p || !T_host || s->nq2 != 1 || s->nq1 != 0 || axis < 0 || axis >= s->mesh->dim || !(p->alpha0 > 0.)) return FM_EINVAL;
+  int top = (int)s->lev.size() - 1;
+  Level &l = s->lev[top];
+  hipStream_t st = s->ctx->stream;
+  const fm_mesh &m = *s->mesh;
+  SysExtra &ex = extra(s);
+  FM_HIP(hipMemcpyAsync(l.t, T_host, l.n * sizeof(double), hipMemcpyHostToDevice, st));
+  FM_CHECK(fmk::fill<double>(st, l.n, l.d, 0.0));
+  double *jel = nullptr, *dk = nullptr;
+  FM_CHECK(dalloc(&jel, l.nel));
+  if (dalloc(&dk, l.nel)) { dfree(jel); return FM_ENOMEM; }
+  int rc = 0;
+  for (int k = 0; k < l.ncolor && !rc; k++) {
+    int n = l.color_off[k + 1] - l.color_off[k];
+    rc = fmk::heat_nusselt(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldof, s->xyz, m.Ntop, s->d_dof_top,
+                           s->vel, l.t, l.coef, p->alpha0, axis, ex.d_tab, jel, dk, l.d);
+  }
+  std::vector<double> h(l.nel);
+  if (!rc) {
+    if (hipMemcpyAsync(h.data(), jel, l.nel * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess) rc = FM_EHIP;
+    if (!rc && dJdk_host && hipMemcpyAsync(dJdk_host, dk, l.nel * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess) rc = FM_EHIP;
+    if (!rc && dJdT_host && hipMemcpyAsync(dJdT_host, l.d, l.n * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess) rc = FM_EHIP;
+    if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = FM_EHIP;
+  }
+  dfree(jel);
+  dfree(dk);
+  if (rc) return rc < 0 ? rc : FM_EHIP;
+  double acc = 0.;
+  for (int e = 0; e < l.nel; e++) acc += h[e];
+  if (J) *J = acc;
   return FM_OK;
 }

@@ -140,6 +140,8 @@ struct fm_system {
   double *x_old = nullptr;                  // top-level x_old (device)
   double *vel = nullptr;                    // heat advection field [dim*Ntop] or nullptr
   double *xbc = nullptr;                    // Dirichlet data of the penalty RHS (nullptr: x_old)
+  double *tnode = nullptr;                  // Boussinesq: nodal temperature [Ntop] (nullptr: no buoyancy)
+  double buoy[3] = {0, 0, 0};
   std::vector<Level> lev;
   // transposed hierarchy for the discrete adjoint: A^T (same pattern as A),
   // restriction Prl[L+1]^T, prolongation Rst[L-1]^T, own work vectors

@@ -40,6 +40,8 @@ struct HeatArgs {
 struct NsArgs {
   double IRe, dt, tc, upwind, IFr;
   double dirg[3], src[3];
+  double buoy[3];          // Boussinesq body force buoy * T (T: nodal temperature field), zero when unused
+  const double *Tn;        // nodal temperature [Ntop] or nullptr
 };
 
 namespace fmk {
@@ -87,6 +89,9 @@ int ns_assemble(hipStream_t s, int dim, int nel_color, const int *elems, const i
                 const int *elpos, const unsigned char *bmask, const double *xyz, int Ntop, const int *dof_top,
                 const int *bc0, const int *bc1, const double *x_old, const double *xbc, const double *coef,
                 const FeTab *tab, NsArgs a, int mode, int trans, int bd_only, double *val, double *rhs);
+int heat_nusselt(hipStream_t s, int dim, int nel, const int *elems, const int *conn, const int *eldof, const double *xyz,
+                 int Ntop, const int *dof_top, const double *vel, const double *T, const double *coef, double kappa,
+                 int axis, const FeTab *tab, double *Jel, double *dJdk, double *y);
 int ns_visc(hipStream_t s, int dim, int nel, const int *elems, const int *conn, const int *eldof, const double *xyz,
             int Ntop, const int *dof_top, const int *bc0, const double *U, const double *lam, const double *coef,
             double nu0, const FeTab *tab, double *diss, double *adj, double *y, double ysc);

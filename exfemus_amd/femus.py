@@ -42,6 +42,7 @@ EXPORTS = [
     "fm_heat_grad_coef", "fm_bench_spmv", "fm_sync", "fm_heat_genmat_transpose", "fm_ns_genmat_transpose",
     "fm_get_adjoint_csr", "fm_heat_dissipation", "fm_bench_stream", "fm_ns_dissipation", "fm_ns_grad_coef",
     "fm_system_set_bcdata", "fm_heat_bc_rhs", "fm_ns_bc_rhs", "fm_ns_pressure_drop",
+    "fm_ns_set_buoyancy", "fm_heat_nusselt",
 ]
 
 
@@ -137,6 +138,8 @@ def lib():
     L.fm_heat_bc_rhs.argtypes = [vp, C.POINTER(HeatParams), f64p, f64p]
     L.fm_ns_bc_rhs.argtypes = [vp, C.POINTER(NSParams), f64p, f64p]
     L.fm_ns_pressure_drop.argtypes = [vp, f64p, C.POINTER(C.c_double), vp]
+    L.fm_ns_set_buoyancy.argtypes = [vp, vp, vp]
+    L.fm_heat_nusselt.argtypes = [vp, C.POINTER(HeatParams), C.c_int, f64p, C.POINTER(C.c_double), vp, vp]
     _LIB = L
     return L
 
@@ -317,6 +320,15 @@ class MGSolDA:
     def get_res(self, L):
         r = np.zeros(self._Dim[L]); _ck(lib().fm_get_res(self.h, L, r)); return r
 
+    def node_values(self, x, var=0):
+        """Nodal field of variable `var` on the top-level nodes from a top-level DoF vector (0 where absent)."""
+        N = self.mesh.N
+        nd = self.node_dof(self._NoLevels - 1)[var * N:(var + 1) * N]
+        out = np.zeros(N)
+        ok = nd >= 0
+        out[ok] = np.asarray(x)[nd[ok]]
+        return out
+
     def set_bcdata(self, g):
         """Dirichlet data of the penalty RHS (top-level DoFs); None = x_old (the reference's source)."""
         if g is None:
@@ -407,6 +419,16 @@ class MGSolT(MGSolDA):
     def GenMatTranspose(self, Level):
         _ck(lib().fm_heat_genmat_transpose(self.h, Level, C.byref(self.params)))
 
+    def nusselt(self, T, axis=0):
+        """Nusselt number along `axis` of a differentially heated box with the current advection field:
+        (J, dJ/dT, dJ/dk_e partials)."""
+        J = C.c_double()
+        g = np.zeros(self._Dim[-1])
+        dk = np.zeros(self.mesh.level(self._NoLevels - 1)[0])
+        _ck(lib().fm_heat_nusselt(self.h, C.byref(self.params), axis, np.ascontiguousarray(T, np.float64),
+                                  C.byref(J), g.ctypes.data_as(C.c_void_p), dk.ctypes.data_as(C.c_void_p)))
+        return J.value, g, dk
+
     def dissipation(self, T):
         """J = alpha0 sum_e k_e int |grad T|^2, dJ/dT, dJ/dk_e (partial) on the top level."""
         J = C.c_double()
@@ -461,6 +483,21 @@ class MGSolNS(MGSolDA):
         _ck(lib().fm_ns_dissipation(self.h, C.byref(self.params), np.ascontiguousarray(U, np.float64), C.byref(J),
                                     g.ctypes.data_as(C.c_void_p), dn.ctypes.data_as(C.c_void_p)))
         return J.value, g, dn
+
+    def set_buoyancy(self, T_node, coef):
+        """Boussinesq body force coef * T (nodal temperature on the top-level nodes); None removes it."""
+        if T_node is None:
+            _ck(lib().fm_ns_set_buoyancy(self.h, None, None))
+            return
+        t = np.ascontiguousarray(T_node, np.float64)
+        c = np.zeros(3); c[:len(coef)] = coef
+        _ck(lib().fm_ns_set_buoyancy(self.h, t.ctypes.data_as(C.c_void_p), c.ctypes.data_as(C.c_void_p)))
+        self._keep_T = t
+
+    def velocity_nodes(self, x=None):
+        """Nodal velocity [dim * Ntop] (component-major) from a top-level DoF vector (default: x)."""
+        x = self.get_x() if x is None else x
+        return np.concatenate([self.node_values(x, d) for d in range(self.mesh.dim)])
 
     def pressure_drop(self, U):
         """J = mean p(inlet x = lo) - mean p(outlet x = hi) and dJ/dU."""

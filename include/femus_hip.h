@@ -202,6 +202,16 @@ int fm_ns_dissipation(fm_system *s, const fm_ns_params *p, const double *U_host,
 /* Adjoint term of dJ/dnu_e for the Oseen (Picard) system: -lambda^T (dA/dnu_e) U */
 int fm_ns_grad_coef(fm_system *s, const fm_ns_params *p, const double *lambda_host,
                     const double *U_host, double *grad_host);
+/* Boussinesq coupling (new): body force coef * T in the NS momentum RHS,
+ * T_node a nodal temperature field on the top-level nodes [Ntop]; NULL
+ * removes it.  (The reference's nse application leaves the temperature
+ * dependence of the density commented out, MGSolverNS.C:344.)            */
+int fm_ns_set_buoyancy(fm_system *s, const double *T_node, const double coef[3]);
+/* Nusselt number of a differentially heated box along `axis`:
+ * J = sum_e int_e (u_a T / alpha0 - k_e dT/dx_a) (advection field of
+ * fm_system_set_velocity), dJ/dT and the partials dJ/dk_e               */
+int fm_heat_nusselt(fm_system *s, const fm_heat_params *p, int axis, const double *T_host, double *J,
+                    double *dJdT_host, double *dJdk_host);
 /* Pressure drop J = mean p(x = lo_x) - mean p(x = hi_x) on the top level
  * (face means of the Q1 pressure) and dJ/dU (linear functional)            */
 int fm_ns_pressure_drop(fm_system *s, const double *U_host, double *J, double *dJdU_host);

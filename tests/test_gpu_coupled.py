@@ -1,0 +1,78 @@
+"""Coupled heat + Navier-Stokes (Boussinesq) on the differentially heated box, and the adjoint of its Nusselt
+number with respect to the per-element conductivity.
+
+Pinned by: the conduction closed form (T = 1 - x, Nu = 1); the published benchmark Nusselt numbers of the
+differentially heated square (de Vahl Davis 1983: Ra 1e4 -> 2.243) and cube (Tric et al. 2000: Ra 1e4 ->
+2.0542); central differences of the steady heat problem with the velocity frozen."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+pytestmark = pytest.mark.gpu
+
+
+def test_nusselt_conduction_closed_form(femus, ctx):
+    from exfemus_amd import coupled
+    m = femus.Mesh(3, [2, 2, 2], [0, 0, 0], [1, 1, 1], 2)
+    T = femus.MGSolT(ctx, m, alpha0=0.01, dt=1.0, time_coeff=0.0, source=0.0, upwind=1.0)
+    T.GenBc(coupled.dhc_T_bc)
+    T.init()
+    x0 = T.GenIc(lambda xp: [1.0 - xp[0]])
+    for L in range(2):
+        T.GenMatRhs(0.0, L, 1 if L == 1 else 0)
+    rp, ci, v = T.csr(0, 1)
+    n = len(rp) - 1
+    Tsol = spla.spsolve(sp.csr_matrix((v, ci, rp), shape=(n, n)).tocsc(), T.get_rhs())
+    np.testing.assert_allclose(Tsol, x0, atol=1e-10)          # the linear profile is the discrete solution
+    J, dJdT, dJdk = T.nusselt(Tsol, axis=0)
+    assert abs(J - 1.0) < 1e-10
+    # J is linear in T: dJ/dT . T = J (no advection)
+    assert abs(dJdT @ Tsol - J) < 1e-10
+    assert abs(dJdk.sum() - 1.0) < 1e-10                      # sum_e -int_e dT/dx = 1 for T = 1 - x
+
+
+@pytest.mark.parametrize("dim,nx0,nlev,Ra,ref,rtol", [(2, 4, 3, 1e4, 2.243, 0.01), (3, 2, 3, 1e4, 2.0542, 0.03)])
+def test_dhc_benchmark_nusselt(femus, ctx, dim, nx0, nlev, Ra, ref, rtol):
+    from exfemus_amd import coupled
+    b = coupled.Boussinesq(ctx, dim, nx0, nlev, Ra)
+    steps = b.run(max_steps=300, tol=1e-7)
+    assert steps < 300
+    Nu = b.nusselt()[0]
+    assert abs(Nu - ref) <= rtol * ref, (Nu, ref)
+    b.ns.close()
+    b.T.close()
+
+
+def test_nusselt_conductivity_gradient_fd(femus, ctx):
+    """dNu/dk_e with the velocity of a converged Ra 1e4 cavity frozen, against central differences of the steady
+    heat problem (direct solves) within 1e-5."""
+    from exfemus_amd import configs, coupled
+    b = coupled.Boussinesq(ctx, 2, 4, 3, 1e4)
+    b.run(max_steps=300, tol=1e-7)
+    ne = b.mesh.level(2)[0]
+    k = configs.lognormal_k(ne, seed=5, sigma=0.5)
+    ks = configs.coarsen_coef(k, 3, 4)
+    for L in range(3):
+        b.T.set_elem_coef(L, ks[L])
+    Nu, grad, Ts = b.nusselt_gradient_k()
+    T = b.T
+    T.params.time_coeff = 0.0
+
+    def Nu_of(kt):
+        T.set_elem_coef(2, kt)
+        T.GenMatRhs(0.0, 2, 1)
+        rp, ci, v = T.csr(0, 2)
+        n = len(rp) - 1
+        x = spla.spsolve(sp.csr_matrix((v, ci, rp), shape=(n, n)).tocsc(), T.get_rhs())
+        return T.nusselt(x, axis=0)[0]
+
+    assert abs(Nu_of(k) - Nu) < 1e-9 * abs(Nu)
+    h = 1e-6
+    for e in np.argsort(-np.abs(grad * k))[:4]:
+        kp = k.copy(); kp[e] *= 1 + h
+        km = k.copy(); km[e] *= 1 - h
+        fd = (Nu_of(kp) - Nu_of(km)) / (2 * h * k[e])
+        assert abs(fd - grad[e]) <= 1e-5 * abs(grad[e]), (e, fd, grad[e])
+    b.ns.close()
+    b.T.close()
