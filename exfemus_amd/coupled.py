@@ -58,19 +58,34 @@ class Boussinesq:
         self.o_T = self.T.opts(precision=F.FM_PREC_FP32_FGMRES, nu_pre=3, nu_post=3, cheb_lo=0.1, tol=1e-10,
                                max_iter=200)
 
+    def state(self):
+        """(velocity-pressure vector, temperature vector) of the current step, top level, reference order."""
+        return self.ns.get_x().copy(), self.xT.copy()
+
+    def set_state(self, st):
+        U, T = st
+        self.ns.set_x(U)
+        self.ns.set_xold(U)
+        self.T.set_x(T)
+        self.T.set_xold(T)
+        self.T.set_velocity(self.ns.velocity_nodes(U))
+        self.xT = T.copy()
+
     def step(self, t=0.0):
-        """One pseudo-time step of each system; returns the relative changes (NS, T)."""
+        """One pseudo-time step of each system; returns the relative changes (NS, T).  The two multigrid
+        reports are kept in self.last_reps."""
         T_old = self.xT
         u_prev = self.ns.get_x()
         self.ns.set_buoyancy(self.T.node_values(T_old), self.buoy)
-        rep = self.ns.MGTimeStep(t, opts=self.o_ns)
-        if rep.converged != 1:
-            raise RuntimeError(f"NS step: {rep.iters} its, {rep.rel_resid:.2e}")
+        rep_ns = self.ns.MGTimeStep(t, opts=self.o_ns)
+        if rep_ns.converged != 1:
+            raise RuntimeError(f"NS step: {rep_ns.iters} its, {rep_ns.rel_resid:.2e}")
         u = self.ns.get_x()
         self.T.set_velocity(self.ns.velocity_nodes(u))
-        rep = self.T.MGTimeStep(t, opts=self.o_T)
-        if rep.converged != 1:
-            raise RuntimeError(f"T step: {rep.iters} its, {rep.rel_resid:.2e}")
+        rep_T = self.T.MGTimeStep(t, opts=self.o_T)
+        if rep_T.converged != 1:
+            raise RuntimeError(f"T step: {rep_T.iters} its, {rep_T.rel_resid:.2e}")
+        self.last_reps = (rep_ns, rep_T)
         T_new = self.T.get_x()
         self.xT = T_new
         du = np.linalg.norm(u - u_prev) / max(np.linalg.norm(u), 1e-300)

@@ -101,95 +101,13 @@ static int lambda_max(fm_ctx *c, const LOps &o, int iters, double *lam) {
   return 0;
 }
 
-// ---- coarse inverse: FP64 Gauss-Jordan with partial pivoting on the device --------
-// [W | I] -> [1 | W^-1], one pivot column per step: the pivot kernel (one block)
-// picks the first row of largest |W[i][k]|, i >= k, and flags a pivot below the
-// singularity threshold; the swap kernel exchanges rows k and p and scales the
-// new row k; column k is saved (the multipliers); the elimination kernel updates
-// every other row on the columns that can still change (W: j > k, I: all).
-struct GjInfo {
-  double piv;  // W[p][k] before the swap
-  int p, singular;
-};
-__global__ void k_gj_pivot(int n, int k, const double *__restrict__ W, double thr, GjInfo *info) {
-  __shared__ double sv[1024];
-  __shared__ int si[1024];
-  double bv = -1.;
-  int bi = n;
-  for (int i = k + threadIdx.x; i < n; i += blockDim.x) {
-    const double v = fabs(W[(int64_t)i * n + k]);
-    if (v > bv) { bv = v; bi = i; }
-  }
-  sv[threadIdx.x] = bv;
-  si[threadIdx.x] = bi;
-  __syncthreads();
-  for (int h = blockDim.x / 2; h > 0; h >>= 1) {
-    if ((int)threadIdx.x < h) {
-      const double ov = sv[threadIdx.x + h];
-      const int oi = si[threadIdx.x + h];
-      if (ov > sv[threadIdx.x] || (ov == sv[threadIdx.x] && oi < si[threadIdx.x])) { sv[threadIdx.x] = ov; si[threadIdx.x] = oi; }
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    const int p = si[0] < n ? si[0] : k;
-    info->p = p;
-    info->piv = W[(int64_t)p * n + k];
-    if (!(sv[0] > thr)) info->singular = 1;
-  }
-}
-__global__ void k_gj_swap_scale(int n, int k, double *W, double *I, const GjInfo *info) {
-  const int p = info->p;
-  const double piv = info->piv;
-  const double r = piv != 0. ? 1.0 / piv : 0.0;
-  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-    const double wk = W[(int64_t)k * n + j], wp = W[(int64_t)p * n + j];
-    const double ik = I[(int64_t)k * n + j], ip = I[(int64_t)p * n + j];
-    if (p != k) { W[(int64_t)p * n + j] = wk; I[(int64_t)p * n + j] = ik; }
-    W[(int64_t)k * n + j] = wp * r;
-    I[(int64_t)k * n + j] = ip * r;
-  }
-}
-__global__ void k_gj_col(int n, int k, const double *__restrict__ W, double *colk) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) colk[i] = W[(int64_t)i * n + k];
-}
-// columns c in [0, n - k - 1): W column k + 1 + c;  c in [n - k - 1, 2n - k - 1): I column c - (n - k - 1)
-__global__ void k_gj_elim(int n, int k, double *W, double *I, const double *__restrict__ colk) {
-  const int nw = n - k - 1, ncol = nw + n;
-  const int i = blockIdx.y;
-  if (i == k) return;
-  const double f = colk[i];
-  if (f == 0.) return;
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncol; c += gridDim.x * blockDim.x) {
-    if (c < nw) {
-      const int j = k + 1 + c;
-      W[(int64_t)i * n + j] -= f * W[(int64_t)k * n + j];
-    } else {
-      const int j = c - nw;
-      I[(int64_t)i * n + j] -= f * I[(int64_t)k * n + j];
-    }
-  }
-}
-__global__ void k_dense_init(int n, const int64_t *__restrict__ rp, const int *__restrict__ ci, const double *__restrict__ v,
-                             const int *__restrict__ rank, int pinned, double *W, double *I) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int r = rank[i];
-    if (r == pinned) {
-      W[(int64_t)r * n + r] = 1.0;
-    } else {
-      for (int64_t q = rp[i]; q < rp[i + 1]; q++) W[(int64_t)r * n + rank[ci[q]]] = v[q];
-    }
-    I[(int64_t)i * n + i] = 1.0;
-  }
-}
-__global__ void k_zero_col(int n, int c, double *I) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) I[(int64_t)i * n + c] = 0.0;
-}
-__global__ void k_to_f32_n(int64_t n, const double *a, float *b) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) b[i] = (float)a[i];
-}
+// ---- coarse inverse (coarse.hip): blocked exchange inversion on the device --------
+int fm_dense_inverse(hipStream_t st, const DevCSR &A, const int *d_rank, int pin, double thr, double **dAinv,
+                     float **dAinvf);
 
-// The dense inverse is laid out in the level's internal DoF order (rank[dof]).
+// The dense inverse is laid out in the level's internal DoF order (rank[dof]).  For a system with a Q1
+// (pressure) variable the last reference DoF may be pinned: in an enclosed flow the pressure is defined up
+// to a constant and the coarse solve then sets that DoF to 0.
 static int coarse_inverse(fm_system *s, const DevCSR &A, const Level &lv, double **dAinv, float **dAinvf) {
   const int n = (int)A.nr;
   hipStream_t st = s->ctx->stream;
@@ -200,49 +118,8 @@ static int coarse_inverse(fm_system *s, const DevCSR &A, const Level &lv, double
   FM_HIP(hipStreamSynchronize(st));
   double amax = 0.;
   for (double v : vals) amax = std::max(amax, std::fabs(v));
-  const size_t nn = (size_t)n * n;
-  double *W = nullptr, *colk = nullptr;
-  GjInfo *info = nullptr;
-  if (!*dAinv && hipMalloc((void **)dAinv, nn * sizeof(double)) != hipSuccess) return FM_ENOMEM;
-  if (!*dAinvf && hipMalloc((void **)dAinvf, nn * sizeof(float)) != hipSuccess) return FM_ENOMEM;
-  if (hipMalloc((void **)&W, nn * sizeof(double)) != hipSuccess) return FM_ENOMEM;
-  if (hipMalloc((void **)&colk, n * sizeof(double)) != hipSuccess) { (void)hipFree(W); return FM_ENOMEM; }
-  if (hipMalloc((void **)&info, sizeof(GjInfo)) != hipSuccess) { (void)hipFree(W); (void)hipFree(colk); return FM_ENOMEM; }
-  double *I = *dAinv;
-  int rc = 0, pinned = -1;
-  const unsigned g1 = (unsigned)std::min(1024, (n + 255) / 256);
-  for (int attempt = 0; attempt < 2 && !rc; attempt++) {
-    GjInfo h{0.0, 0, 0};
-    if (hipMemsetAsync(W, 0, nn * sizeof(double), st) != hipSuccess || hipMemsetAsync(I, 0, nn * sizeof(double), st) != hipSuccess ||
-        hipMemcpyAsync(info, &h, sizeof(GjInfo), hipMemcpyHostToDevice, st) != hipSuccess) { rc = FM_EHIP; break; }
-    hipLaunchKernelGGL(k_dense_init, dim3(g1), dim3(256), 0, st, n, A.rp, A.ci, A.v, lv.rank, pinned, W, I);
-    for (int k = 0; k < n; k++) {
-      hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(1024), 0, st, n, k, W, 1e-13 * amax, info);
-      hipLaunchKernelGGL(k_gj_swap_scale, dim3(g1), dim3(256), 0, st, n, k, W, I, info);
-      hipLaunchKernelGGL(k_gj_col, dim3(g1), dim3(256), 0, st, n, k, W, colk);
-      const int ncol = 2 * n - k - 1;
-      hipLaunchKernelGGL(k_gj_elim, dim3((unsigned)std::min(64, (ncol + 255) / 256), (unsigned)n), dim3(256), 0, st, n, k, W, I,
-                         colk);
-    }
-    if (hipMemcpyAsync(&h, info, sizeof(GjInfo), hipMemcpyDeviceToHost, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
-      rc = FM_EHIP;
-      break;
-    }
-    if (!h.singular) {
-      // the pinned row's equation is replaced by x = 0: its right-hand side is ignored
-      if (pinned >= 0) hipLaunchKernelGGL(k_zero_col, dim3(g1), dim3(256), 0, st, n, pinned, I);
-      hipLaunchKernelGGL(k_to_f32_n, dim3(1024), dim3(256), 0, st, (int64_t)nn, I, *dAinvf);
-      if (hipStreamSynchronize(st) != hipSuccess) rc = FM_EHIP;
-      break;
-    }
-    if (attempt == 1 || s->nq1 == 0) { fm_set_error("coarse matrix singular"); rc = FM_ESINGULAR; break; }
-    // enclosed flow: pressure defined up to a constant -> pin the last Q1 DoF
-    pinned = lv.h_rank[n - 1];
-  }
-  (void)hipFree(W);
-  (void)hipFree(colk);
-  (void)hipFree(info);
-  return rc;
+  const int pin = s->nq1 > 0 ? lv.h_rank[n - 1] : -1;
+  return fm_dense_inverse(st, A, lv.rank, pin, 1e-13 * amax, dAinv, dAinvf);
 }
 
 // ---- forward operator set
