@@ -21,102 +21,127 @@
 #include "fm_internal.h"
 #include "fm_kernels.h"
 
-template <int NL>
-__global__ __launch_bounds__(256) void k_vanka_invert(int nel, const int *__restrict__ elpos, const double *__restrict__ Av,
+// Inverse of every element patch matrix K_e (NL x NL, gathered from the level
+// matrix through the element -> slot map), FP64, by in-place exchange steps:
+// the step on pivot (r, k), a = M[r][k], swaps the roles of unknown k and
+// equation r,
+//   M[r][k] = 1/a, M[r][j] = -M[r][j]/a, M[i][k] = M[i][k]/a, M[i][j] -= M[i][k] M[r][j]/a,
+// with r chosen by partial pivoting among the rows not used yet (no row is
+// moved); afterwards (K_e^-1)[k][prow[q]] = M[prow[k]][q].  The matrix lives
+// in registers: thread (ty, tx) of a G x G grid owns the TR x TR tile at rows
+// ty*TR, columns tx*TR; each step publishes only the pivot row and column in
+// LDS (double-buffered, two barriers per step).
+template <int NL, int G, int TR>
+__global__ __launch_bounds__(G * G) void k_vanka_invert(int nel, const int *__restrict__ elpos, const double *__restrict__ Av,
                                                       double *__restrict__ vinv, float *__restrict__ vinvf,
                                                       int *__restrict__ status) {
-  __shared__ double M[NL * NL];
-  __shared__ int piv[NL];
-  __shared__ double s_best, s_red[4];
-  __shared__ int s_bi;
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  constexpr int NP = G * TR, NT = G * G, NW = (NT + 63) / 64;
+  static_assert(NP >= NL && TR <= 32, "tile grid does not cover the patch");
+  __shared__ double s_row[2][NP], s_col[2][NP], s_kv[2][G], s_red[NW];
+  __shared__ int s_ki[2][G], s_prow[NP], s_rowstep[NP];
+  const int tid = threadIdx.x, ty = tid / G, tx = tid % G, lane = tid & 63, wid = tid >> 6;
+  const int i0 = ty * TR, j0 = tx * TR;
   for (int e = blockIdx.x; e < nel; e += gridDim.x) {
-    __syncthreads();
     const int64_t base = (int64_t)e * NL * NL;
+    double m[TR][TR];
     double amax = 0.;
-    for (int t = tid; t < NL * NL; t += blockDim.x) {
-      int p = elpos[base + t];
-      double v = p >= 0 ? Av[p] : 0.0;
-      M[t] = v;
-      amax = fmax(amax, fabs(v));
-    }
-    // block maximum of |M| (the singularity threshold)
+#pragma unroll
+    for (int a = 0; a < TR; a++)
+#pragma unroll
+      for (int b = 0; b < TR; b++) {
+        const int i = i0 + a, j = j0 + b;
+        double v = 0.;
+        if (i < NL && j < NL) {
+          const int p = elpos[base + i * NL + j];
+          v = p >= 0 ? Av[p] : 0.0;
+        }
+        m[a][b] = v;
+        amax = fmax(amax, fabs(v));
+      }
     for (int off = 32; off > 0; off >>= 1) amax = fmax(amax, __shfl_xor(amax, off, 64));
+    __syncthreads();                   // the previous patch's readers of s_red / s_prow / s_rowstep are done
     if (lane == 0) s_red[wid] = amax;
     __syncthreads();
-    if (tid == 0) {
-      double m = 0.;
-      for (int w = 0; w < (int)(blockDim.x >> 6); w++) m = fmax(m, s_red[w]);
-      s_best = m;
-    }
-    __syncthreads();
-    const double tiny = 1e-14 * s_best;
+    double mx = 0.;
+    for (int w = 0; w < NW; w++) mx = fmax(mx, s_red[w]);
+    const double tiny = 1e-14 * mx;
+    unsigned used = 0;                 // bit a: row i0 + a already has a pivot
     bool singular = false;
     for (int k = 0; k < NL; k++) {
-      // partial pivoting: the first wave finds the largest |M[i][k]|, i >= k (first index on ties)
-      if (tid < 64) {
-        int bi = NL;
+      const int buf = k & 1, kt = k / TR, kb = k - kt * TR;
+      if (tx == kt) {                  // candidates of column k among this tile's free rows
         double bv = -1.;
-        for (int i = k + lane; i < NL; i += 64) {
-          const double v = fabs(M[i * NL + k]);
-          if (v > bv) { bv = v; bi = i; }
+        int bi = NP;
+#pragma unroll
+        for (int a = 0; a < TR; a++) {
+          double v = 0.;
+#pragma unroll
+          for (int b = 0; b < TR; b++) v = (b == kb) ? m[a][b] : v;
+          v = fabs(v);
+          if (i0 + a < NL && !((used >> a) & 1u) && v > bv) { bv = v; bi = i0 + a; }
         }
-        for (int off = 32; off > 0; off >>= 1) {
-          const double ov = __shfl_xor(bv, off, 64);
-          const int oi = __shfl_xor(bi, off, 64);
-          if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
-        }
-        if (tid == 0) {
-          s_bi = bi;
-          piv[k] = bi;
-          s_best = bv;
-        }
+        s_kv[buf][ty] = bv;
+        s_ki[buf][ty] = bi;
       }
       __syncthreads();
-      if (!(s_best > tiny)) { singular = true; break; }
-      const int bi = s_bi;
-      if (bi != k)
-        for (int j = tid; j < NL; j += blockDim.x) {
-          double t = M[k * NL + j]; M[k * NL + j] = M[bi * NL + j]; M[bi * NL + j] = t;
-        }
-      __syncthreads();
-      const double inv = 1.0 / M[k * NL + k];
-      __syncthreads();
-      for (int j = tid; j < NL; j += blockDim.x) M[k * NL + j] = (j == k ? 1.0 : M[k * NL + j]) * inv;
-      __syncthreads();
-      // eliminate column k from the other rows; column k of those rows becomes -f * inv
-      for (int t = tid; t < NL * NL; t += blockDim.x) {
-        const int i = t / NL, j = t % NL;
-        if (i == k) continue;
-        const double f = M[i * NL + k];
-        if (j == k) continue;
-        M[t] -= f * M[k * NL + j];
+      double bv = -1.;
+      int r = NP;
+      for (int t = 0; t < G; t++) {
+        const double v = s_kv[buf][t];
+        const int ii = s_ki[buf][t];
+        if (v > bv || (v == bv && ii < r)) { bv = v; r = ii; }
       }
+      if (!(bv > tiny)) { singular = true; break; }   // the same decision in every thread
+      const int ra_ = r - i0;          // row of the pivot inside this tile (valid when 0 <= ra_ < TR)
+#pragma unroll
+      for (int a = 0; a < TR; a++)
+        if (a == ra_)
+#pragma unroll
+          for (int b = 0; b < TR; b++) s_row[buf][j0 + b] = m[a][b];
+      if (tx == kt)
+#pragma unroll
+        for (int a = 0; a < TR; a++) {
+          double v = 0.;
+#pragma unroll
+          for (int b = 0; b < TR; b++) v = (b == kb) ? m[a][b] : v;
+          s_col[buf][i0 + a] = v;
+        }
+      if (tid == 0) { s_prow[k] = r; s_rowstep[r] = k; }
       __syncthreads();
-      for (int i = tid; i < NL; i += blockDim.x)
-        if (i != k) M[i * NL + k] = -M[i * NL + k] * M[k * NL + k];
-      __syncthreads();
+      const double ra = 1.0 / s_row[buf][k];
+      double cr[TR], rr[TR];
+#pragma unroll
+      for (int a = 0; a < TR; a++) cr[a] = s_col[buf][i0 + a] * ra;
+#pragma unroll
+      for (int b = 0; b < TR; b++) rr[b] = s_row[buf][j0 + b];
+#pragma unroll
+      for (int a = 0; a < TR; a++)
+#pragma unroll
+        for (int b = 0; b < TR; b++) {
+          const bool jc = (j0 + b == k);
+          if (a == ra_) m[a][b] = jc ? ra : -rr[b] * ra;
+          else m[a][b] = jc ? cr[a] : fma(-cr[a], rr[b], m[a][b]);
+        }
+      if (ra_ >= 0 && ra_ < TR) used |= 1u << ra_;
     }
     if (singular) {
       if (tid == 0) atomicAdd(status, 1);
-      for (int t = tid; t < NL * NL; t += blockDim.x) { vinv[base + t] = 0.0; vinvf[base + t] = 0.f; }
+      for (int t = tid; t < NL * NL; t += NT) { vinv[base + t] = 0.0; vinvf[base + t] = 0.f; }
       continue;
     }
-    // undo the row interchanges as column interchanges, last to first
-    for (int k = NL - 1; k >= 0; k--) {
-      const int p = piv[k];
-      if (p != k)
-        for (int i = tid; i < NL; i += blockDim.x) {
-          double t = M[i * NL + k]; M[i * NL + k] = M[i * NL + p]; M[i * NL + p] = t;
+    __syncthreads();
+    // transposed store: (K_e^-1)[rowstep[i]][prow[j]] = M[i][j] at prow[j] * NL + rowstep[i]
+#pragma unroll
+    for (int a = 0; a < TR; a++)
+#pragma unroll
+      for (int b = 0; b < TR; b++) {
+        const int i = i0 + a, j = j0 + b;
+        if (i < NL && j < NL) {
+          const int64_t o = base + (int64_t)s_prow[j] * NL + s_rowstep[i];
+          vinv[o] = m[a][b];
+          vinvf[o] = (float)m[a][b];
         }
-      __syncthreads();
-    }
-    // transposed: entry (i, j) of the inverse at j * NL + i
-    for (int t = tid; t < NL * NL; t += blockDim.x) {
-      const int i = t / NL, j = t % NL;
-      vinv[base + j * NL + i] = M[t];
-      vinvf[base + j * NL + i] = (float)M[t];
-    }
+      }
   }
 }
 
@@ -182,11 +207,11 @@ static int vanka_invert_level(fm_system *s, int L, const double *Av, double **st
   if (hipMalloc((void **)&status, sizeof(int)) != hipSuccess) return FM_ENOMEM;
   FM_HIP(hipMemsetAsync(status, 0, sizeof(int), st));
   FM_HIP(hipMemsetAsync(l.vwt, 0, l.n * sizeof(double), st));
-  unsigned g = (unsigned)(l.nel < 8192 ? l.nel : 8192);
+  unsigned g = (unsigned)(l.nel < 16384 ? l.nel : 16384);
   if (s->mesh->dim == 2)
-    hipLaunchKernelGGL(k_vanka_invert<22>, dim3(g), dim3(256), 0, st, l.nel, l.elpos, Av, *store, *storef, status);
+    hipLaunchKernelGGL((k_vanka_invert<22, 8, 3>), dim3(g), dim3(64), 0, st, l.nel, l.elpos, Av, *store, *storef, status);
   else
-    hipLaunchKernelGGL(k_vanka_invert<89>, dim3(g), dim3(256), 0, st, l.nel, l.elpos, Av, *store, *storef, status);
+    hipLaunchKernelGGL((k_vanka_invert<89, 16, 6>), dim3(g), dim3(256), 0, st, l.nel, l.elpos, Av, *store, *storef, status);
   FM_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_count_mult, dim3(1024), dim3(256), 0, st, l.nel, nl, l.eldofi, l.vwt);
   hipLaunchKernelGGL(k_recip, dim3(1024), dim3(256), 0, st, l.n, l.vwt);
@@ -260,4 +285,23 @@ int fm_vanka_apply64(fm_system *s, int L, const CsrView<double> &A, const double
 int fm_vanka_apply32(fm_system *s, int L, const CsrView<float> &A, const float *vinv, float *x, const float *b,
                      float *r, int its, double omega, int colored, bool xz) {
   return vanka_sweep<float>(s, L, A, vinv, x, b, r, its, omega, colored, xz);
+}
+
+// Diagnostic: one element's patch inverse as stored for the smoother (FP64, transposition undone) and the
+// patch DoFs in reference numbering, local order [u, v, (w), p].
+extern "C" int fm_vanka_patch(const fm_system *s, int L, int e, int *dofs, double *kinv) {
+  if (!s || !dofs || !kinv || L < 0 || L >= (int)s->lev.size()) return FM_EINVAL;
+  const Level &l = s->lev[L];
+  if (!s->mg_ready || !l.vinv) { fm_set_error("no Vanka inverses: run fm_mg_setup with the Vanka smoother"); return FM_ESTATE; }
+  if (e < 0 || e >= l.nel) return FM_EINVAL;
+  const int nl = s->nloc;
+  std::vector<double> t((size_t)nl * nl);
+  hipStream_t st = s->ctx->stream;
+  FM_HIP(hipSetDevice(s->ctx->device));
+  FM_HIP(hipMemcpyAsync(dofs, l.eldof + (int64_t)e * nl, nl * sizeof(int), hipMemcpyDeviceToHost, st));
+  FM_HIP(hipMemcpyAsync(t.data(), l.vinv + (int64_t)e * nl * nl, t.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  FM_HIP(hipStreamSynchronize(st));
+  for (int a = 0; a < nl; a++)
+    for (int b = 0; b < nl; b++) kinv[a * nl + b] = t[(size_t)b * nl + a];
+  return FM_OK;
 }

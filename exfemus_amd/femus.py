@@ -38,7 +38,7 @@ EXPORTS = [
     "fm_system_init", "fm_system_csr_nnz", "fm_system_csr_get", "fm_system_set_xold", "fm_system_get_x",
     "fm_system_set_x", "fm_system_get_rhs", "fm_system_set_rhs", "fm_system_set_elem_coef",
     "fm_system_set_velocity", "fm_heat_genmatrhs", "fm_ns_genmatrhs", "fm_spmv", "fm_residual",
-    "fm_restrict", "fm_prolong", "fm_get_res", "fm_mg_setup", "fm_mg_solve", "fm_adjoint_solve",
+    "fm_restrict", "fm_prolong", "fm_get_res", "fm_mg_setup", "fm_mg_solve", "fm_vanka_patch", "fm_adjoint_solve",
     "fm_heat_grad_coef", "fm_bench_spmv", "fm_sync", "fm_heat_genmat_transpose", "fm_ns_genmat_transpose",
     "fm_get_adjoint_csr", "fm_heat_dissipation", "fm_bench_stream", "fm_ns_dissipation", "fm_ns_grad_coef",
     "fm_system_set_bcdata", "fm_heat_bc_rhs", "fm_ns_bc_rhs", "fm_ns_pressure_drop",
@@ -123,6 +123,7 @@ def lib():
     for fn in ("fm_residual", "fm_restrict", "fm_prolong"):
         getattr(L, fn).argtypes = [vp, C.c_int]
     L.fm_mg_setup.argtypes = [vp, C.POINTER(MGOpts)]
+    L.fm_vanka_patch.argtypes = [vp, C.c_int, C.c_int, vp, vp]
     L.fm_mg_solve.argtypes = [vp, C.POINTER(MGOpts), C.POINTER(MGReport)]
     L.fm_adjoint_solve.argtypes = [vp, C.POINTER(MGOpts), f64p, f64p, C.POINTER(MGReport)]
     L.fm_heat_grad_coef.argtypes = [vp, C.POINTER(HeatParams), f64p, f64p, f64p]
@@ -374,6 +375,14 @@ class MGSolDA:
 
     def MGSetup(self, opts):
         _ck(lib().fm_mg_setup(self.h, C.byref(opts)))
+
+    def vanka_patch(self, L, e):
+        """(dofs, K_e^-1): the FP64 local inverse the Vanka smoother applies for element e on level L."""
+        nl = (self.nq2 * 27 if self.mesh.dim == 3 else self.nq2 * 9) + (self.nq1 * (8 if self.mesh.dim == 3 else 4))
+        dofs = np.zeros(nl, np.int32)
+        K = np.zeros((nl, nl))
+        _ck(lib().fm_vanka_patch(self.h, L, e, dofs.ctypes.data_as(C.c_void_p), K.ctypes.data_as(C.c_void_p)))
+        return dofs, K
 
     def MGSolve(self, Eps=1e-8, MaxIter=100, Gamma=1, Nc_pre=2, Nc_coarse=40, Nc_post=2, opts=None,
                 raise_on_noconv=False):

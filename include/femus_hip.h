@@ -173,6 +173,10 @@ typedef struct {
 
 int fm_mg_setup(fm_system *s, const fm_mg_opts *o);   /* eigen bounds, coarse LU, Vanka inverses */
 int fm_mg_solve(fm_system *s, const fm_mg_opts *o, fm_mg_report *rep);   /* x[top] in/out, b[top] */
+/* Diagnostic: the FP64 local inverse K_e^-1 the Vanka smoother applies for element e on level L
+ * (after fm_mg_setup with FM_SMOOTH_VANKA), row-major nloc x nloc, and the patch DoFs in reference
+ * numbering, local order [u, v, (w), p] (the local solves of VankaIter, contrib/laspack/itersolv.c:723-798). */
+int fm_vanka_patch(const fm_system *s, int L, int e, int *dofs, double *kinv);
 
 /* ---- discrete adjoint -------------------------------------------------- */
 /* Transpose-Jacobian assembly: the GenMatRhs kernels scatter K_e^T, giving

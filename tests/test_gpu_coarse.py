@@ -86,3 +86,35 @@ def test_coarse_inverse_enclosed_cavity_pinned(femus, ctx):
     xr = spla.spsolve(A[keep][:, keep].tocsc(), b[keep])
     np.testing.assert_allclose(x[:n - 1], xr, rtol=0, atol=1e-9 * np.abs(xr).max())
     s.close()
+
+
+@pytest.mark.parametrize("dim,nx,nlev", [(2, [2, 2], 3), (3, [2, 2, 2], 2)])
+def test_vanka_patch_inverses(femus, ctx, dim, nx, nlev):
+    """Every stored Vanka patch inverse (register-tiled exchange inversion with partial pivoting) inverts the
+    patch matrix gathered from the assembled level matrix, on every level."""
+    from exfemus_amd import configs
+    m = femus.Mesh(dim, nx, [0] * dim, [1] * dim, nlev)
+    s = femus.MGSolNS(ctx, m, IRe=0.01, dt=0.1, time_coeff=1.0, upwind=0.1)
+    s.GenBc(functools.partial(configs.cavity_bc, dim=dim))
+    s.init()
+    s.GenIc(functools.partial(configs.cavity_ic, dim=dim))
+    top = nlev - 1
+    for L in range(nlev):
+        s.GenMatRhs(0.0, L, 1 if L == top else 0)
+    s.MGSetup(s.opts(smoother=femus.FM_SMOOTH_VANKA))
+    rng = np.random.default_rng(0)
+    for L in range(1, nlev):
+        A = _csr_level(s, L)
+        ne = m.level(L)[0]
+        for e in sorted(set(rng.integers(0, ne, 6).tolist()) | {0, ne - 1}):
+            dofs, Ki = s.vanka_patch(L, e)
+            K = A[dofs][:, dofs].toarray()
+            err = np.abs(Ki @ K - np.eye(len(dofs))).max()
+            assert err < 1e-10 * max(1.0, np.linalg.cond(K) * 1e-6), (L, e, err)
+    s.close()
+
+
+def _csr_level(s, L):
+    rp, ci, v = s.csr(0, L)
+    n = len(rp) - 1
+    return sp.csr_matrix((v, ci, rp), shape=(n, n))
