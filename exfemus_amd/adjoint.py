@@ -15,6 +15,8 @@ A^T lambda = dJ/dU, then
     dJ/da_k  = lambda^T b_bd(G_k)      (b is linear in the boundary data).
 Everything runs through the C ABI; torch only carries the tensors.
 """
+import ctypes as C
+
 import numpy as np
 import torch
 
@@ -64,6 +66,143 @@ class OseenProblem:
         ns = self.ns
         U, dJdU, dJdnu = self.last
         lam, rep = ns.AdjointSolve(dJdU, self.opts)
+        if rep.converged != 1:
+            raise RuntimeError(f"adjoint multigrid did not converge: {rep.iters} its, {rep.rel_resid:.3e}")
+        g_nu = dJdnu + ns.grad_coef(lam, U)
+        g_a = np.array([lam @ ns.bc_rhs(Gk) for Gk in self.G])
+        return g_nu, g_a
+
+
+class SteadyNSProblem(OseenProblem):
+    """The steady state of the same discretisation (Galerkin: upwind 0; no time term):
+        R(U; nu, g) = A(U; nu) U - b(g) = 0,
+    the fixed point of the reference's Picard step (MGSolNS::MGTimeStep, MGSolverNS.C:722-756, iterated with
+    x_old <- x).  Solved by Picard steps until the relative update is below picard_tol, then Newton steps
+    J_N(U) dU = b - A(U) U with the Jacobian of fm_ns_genjac, multigrid-solved, until ||R|| <= tol ||b||.
+    The adjoint of J(U) at the steady state solves J_N(U)^T lambda = dJ/dU (transposed multigrid); then
+    dJ/dnu_e = partial - lambda^T (dA/dnu_e) U and dJ/da_k = lambda^T b(G_k).
+    Only the top level carries the Newton Jacobian (FGMRES measures its residual there); the coarse levels of
+    the preconditioner carry the Picard matrix at the same state with upwinding coarse_upwind (the Galerkin
+    Newton Jacobian of a coarse grid at high mesh Peclet number makes a poor coarse operator)."""
+
+    def __init__(self, ns, G, g0=None, functional="dissipation", opts=None, tol=1e-12, picard_tol=1e-3,
+                 max_picard=200, max_newton=30, coarse_upwind=0.1, newton_rtol=1e-6):
+        if ns.params.upwind != 0.0:
+            raise ValueError("the steady Newton Jacobian needs upwind = 0")
+        super().__init__(ns, G, g0, functional, opts)
+        self.tol, self.picard_tol = tol, picard_tol
+        self.max_picard, self.max_newton = max_picard, max_newton
+        self.coarse_upwind = coarse_upwind
+        self.newton_rtol = newton_rtol
+        self.U = None
+        self.stats = {}
+
+    def _residual(self, U):
+        ns = self.ns
+        top = self.nlev - 1
+        ns.set_xold(U)
+        ns.GenMatRhs(0.0, top, 1)
+        b = ns.get_rhs()
+        return b - ns.spmv(top, U), np.linalg.norm(b)
+
+    def _step_opts(self):
+        o = type(self.opts)()
+        C.pointer(o)[0] = self.opts
+        o.tol = self.newton_rtol
+        return o
+
+    def _coarse(self, transpose):
+        """Picard matrices (or their transposes) with upwinding on the coarse levels, at the current x_old."""
+        ns = self.ns
+        up = ns.params.upwind
+        ns.params.upwind = self.coarse_upwind
+        try:
+            for L in range(self.nlev - 1):
+                if transpose:
+                    ns.GenMatTranspose(L)
+                else:
+                    ns.GenMatRhs(0.0, L, 0)
+        finally:
+            ns.params.upwind = up
+
+    def steady_state(self, U0=None):
+        """Picard, then Newton, from U0 (default: the last steady state, else zero).  Returns U."""
+        ns = self.ns
+        top = self.nlev - 1
+        N = ns._Dim[top]
+        tc = ns.params.time_coeff
+        ns.params.time_coeff = 0.0
+        try:
+            U = np.zeros(N) if U0 is None else np.array(U0, np.float64)
+            npic = 0
+            if U0 is None:
+                for npic in range(1, self.max_picard + 1):
+                    ns.set_xold(U)
+                    for L in range(self.nlev):
+                        ns.GenMatRhs(0.0, L, 1 if L == top else 0)
+                    ns.set_x(U)
+                    rep = ns.MGSolve(opts=self.opts)
+                    if rep.converged != 1:
+                        raise RuntimeError(f"Picard step: {rep.iters} its, {rep.rel_resid:.2e}")
+                    U1 = ns.get_x()
+                    du = np.linalg.norm(U1 - U) / max(np.linalg.norm(U1), 1e-300)
+                    U = U1
+                    if du < self.picard_tol:
+                        break
+            rprev = np.inf
+            its_newton = []
+            for nnew in range(self.max_newton + 1):
+                r, bn = self._residual(U)
+                rn = np.linalg.norm(r)
+                # converged, or at the rounding floor of the residual (no longer halving below 1e-10)
+                if rn <= self.tol * bn or (rn <= 1e-10 * bn and rn > 0.5 * rprev):
+                    break
+                rprev = rn
+                if nnew == self.max_newton:
+                    raise RuntimeError(f"Newton: ||R||/||b|| = {np.linalg.norm(r) / bn:.2e} after {nnew} steps")
+                ns.GenJac(top, 0)
+                self._coarse(False)
+                ns.set_rhs(r)
+                ns.set_x(np.zeros(N))
+                rep = ns.MGSolve(opts=self._step_opts())
+                its_newton.append(rep.iters)
+                if rep.converged != 1 and not rep.rel_resid < 1e-3:      # inexact Newton: a reduced residual will do
+                    raise RuntimeError(f"Newton step: {rep.iters} its, {rep.rel_resid:.2e}")
+                U = U + ns.get_x()
+            self.stats = {"picard": npic, "newton": nnew, "newton_mg_its": its_newton,
+                          "rel_residual": float(np.linalg.norm(r) / bn)}
+        finally:
+            ns.params.time_coeff = tc
+        return U
+
+    def solve(self, nu, a, U0=None):
+        self._set(nu, a)
+        U = self.steady_state(U0)
+        ns = self.ns
+        ns.set_xold(U)
+        ns.set_x(U)
+        if self.functional == "dissipation":
+            J, dJdU, dJdnu = ns.dissipation(U)
+        else:
+            J, dJdU = ns.pressure_drop(U)
+            dJdnu = np.zeros(len(nu))
+        self.U = U
+        self.last = (U, dJdU, dJdnu)
+        return J
+
+    def gradient(self):
+        ns = self.ns
+        U, dJdU, dJdnu = self.last
+        ns.set_xold(U)
+        tc = ns.params.time_coeff
+        ns.params.time_coeff = 0.0
+        try:
+            ns.GenJac(self.nlev - 1, 1)
+            self._coarse(True)
+        finally:
+            ns.params.time_coeff = tc
+        lam, rep = ns.AdjointSolve(dJdU, self.opts)
+        self.stats["adjoint_mg_its"] = rep.iters
         if rep.converged != 1:
             raise RuntimeError(f"adjoint multigrid did not converge: {rep.iters} its, {rep.rel_resid:.3e}")
         g_nu = dJdnu + ns.grad_coef(lam, U)

@@ -312,7 +312,11 @@ __global__ void k_heat_bd(int nel, const int *__restrict__ elems, const int *__r
 //                    + JxW phi_i (tcdt phi_j + u_old.grad phi_j) ]
 //   (iv,jv): sum_q JxW (nu + f_up,iv u_iv u_jv) G_jv,i G_iv,j      (iv != jv)
 //   B^T:     -sum_q JxW psi_j G_iv,i        B: sum_q JxW psi_i G_jv,j
-// (MGSolverNS.C:364-489).  The DIM^2 velocity blocks are split into 3x3
+// (MGSolverNS.C:364-489).  With a.newton the velocity blocks also take the
+// derivative of the advection term with respect to the advecting field,
+//   (iv,jv): sum_q JxW phi_i phi_j du_iv/dx_jv   (u = x_old),
+// which makes the matrix the Newton Jacobian of the steady residual
+// A(u) u - b at u = x_old (upwinding off).  The DIM^2 velocity blocks are split into 3x3
 // register tiles (1x1 in 2D), 81 per block; B^T / B entries are one per
 // thread.  Velocity rows of Dirichlet DoFs are dropped (bc_vol), pressure
 // rows are not.
@@ -329,6 +333,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   __shared__ double s_xx[DIM * N2], s_uold[DIM * N2];
   __shared__ double s_JxW[NG], s_vel[NG * DIM], s_fup[NG * DIM], s_cl[NG * DIM], s_IJ[NG * DIM * DIM];
   __shared__ double s_Tn[N2], s_Tq[NG];
+  __shared__ double s_du[NG * DIM * DIM];   // [q][iv][jv] = du_iv/dx_jv (Newton term only)
   __shared__ int s_bcv[DIM * N2];
   const int tid = threadIdx.x;
   const double tcdt = a.tc / a.dt;
@@ -393,6 +398,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       }
     }
     __syncthreads();
+    if (a.newton) {
+      for (int t = tid; t < NG * DIM * DIM; t += blockDim.x) {
+        const int q = t / (DIM * DIM), iv = (t / DIM) % DIM, jv = t % DIM;
+        double g = 0.;
+        for (int sn = 0; sn < N2; sn++) g += s_G[(q * DIM + jv) * N2 + sn] * s_uold[sn + iv * N2];
+        s_du[t] = g;
+      }
+      __syncthreads();
+    }
     const int64_t pbase = (int64_t)e * NL * NL;
     // velocity blocks: DIM^2 blocks x 81 tiles
     for (int u = tid; u < DIM * DIM * 81; u += blockDim.x) {
@@ -407,9 +421,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 #pragma unroll 1
         for (int q = 0; q < NG; q++) {
           const double jw = s_JxW[q];
+          const double rc = tcdt + (a.newton ? s_du[(q * DIM + iv) * DIM + iv] : 0.);
           double bj[TI];
 #pragma unroll
-          for (int y = 0; y < TI; y++) bj[y] = tcdt * s_phi[(tj + 9 * y) * NG + q];
+          for (int y = 0; y < TI; y++) bj[y] = rc * s_phi[(tj + 9 * y) * NG + q];
 #pragma unroll
           for (int k = 0; k < DIM; k++) {
             const double *G = s_G + (q * DIM + k) * N2;
@@ -444,6 +459,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
           for (int x = 0; x < TI; x++)
 #pragma unroll
             for (int y = 0; y < TI; y++) K[x][y] += gi[x] * gj[y];
+          if (a.newton) {
+            const double c2 = s_JxW[q] * s_du[(q * DIM + iv) * DIM + jv];
+#pragma unroll
+            for (int x = 0; x < TI; x++) {
+              const double ai = c2 * s_phi[(ti + 9 * x) * NG + q];
+#pragma unroll
+              for (int y = 0; y < TI; y++) K[x][y] += ai * s_phi[(tj + 9 * y) * NG + q];
+            }
+          }
         }
       }
 #pragma unroll

@@ -1121,6 +1121,39 @@ extern "C" int fm_ns_genmat_transpose(fm_system *s, int L, const fm_ns_params *p
   return after_assembly_T(s, L);
 }
 
+// Newton Jacobian of the steady residual R(u) = A(u) u - b at u = x_old (the Oseen matrix of GenMatRhs plus the
+// derivative of the advection term with respect to the advecting field); trans = 1 scatters its transpose into the
+// adjoint matrices.  Upwinding must be off: its coefficient depends on |u| and is not differentiated here.
+extern "C" int fm_ns_genjac(fm_system *s, int L, int trans, const fm_ns_params *p) {
+  FM_CHECK(check_level(s, L));
+  const fm_mesh &m = *s->mesh;
+  if (!p || s->nq2 != m.dim || s->nq1 != 1 || p->dt == 0. || p->IRe <= 0. || (trans != 0 && trans != 1)) return FM_EINVAL;
+  if (p->upwind != 0.) { fm_set_error("fm_ns_genjac: the Newton Jacobian needs upwind = 0"); return FM_EINVAL; }
+  FM_HIP(hipSetDevice(s->ctx->device));
+  if (trans) FM_CHECK(ensure_adjoint(s));
+  Level &l = s->lev[L];
+  double *val = trans ? s->alev[L].A.sell.v : l.A.sell.v;
+  const int64_t nslot = trans ? s->alev[L].A.sell.nslot : l.A.sell.nslot;
+  hipStream_t st = s->ctx->stream;
+  NsArgs a;
+  a.IRe = p->IRe; a.dt = p->dt; a.tc = p->time_coeff; a.upwind = 0.; a.IFr = p->IFr;
+  for (int d = 0; d < 3; d++) { a.dirg[d] = p->dirg[d]; a.src[d] = p->src[d]; a.buoy[d] = 0.; }
+  a.Tn = nullptr;
+  a.newton = 1;
+  FM_HIP(hipMemsetAsync(val, 0, nslot * sizeof(double), st));
+  SysExtra &ex = extra(s);
+  for (int k = 0; k < l.ncolor; k++) {
+    int n = l.color_off[k + 1] - l.color_off[k];
+    FM_CHECK(fmk::ns_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldofi, l.elpos, l.bmask, s->xyz,
+                              m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, s->xbc, l.coef, ex.d_tab, a, 0, trans, 0,
+                              val, nullptr));
+  }
+  if (trans) return after_assembly_T(s, L);
+  FM_CHECK(after_assembly(s, L));
+  FM_HIP(hipStreamSynchronize(st));
+  return FM_OK;
+}
+
 extern "C" int fm_get_adjoint_csr(const fm_system *s, int L, double *val) {
   if (!s || L < 0 || L >= (int)s->alev.size() || !val) return FM_EINVAL;
   DevCSR &c = const_cast<DevCSR &>(s->alev[L].A);

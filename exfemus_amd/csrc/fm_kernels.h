@@ -42,6 +42,7 @@ struct NsArgs {
   double dirg[3], src[3];
   double buoy[3];          // Boussinesq body force buoy * T (T: nodal temperature field), zero when unused
   const double *Tn;        // nodal temperature [Ntop] or nullptr
+  int newton = 0;          // add the Newton term phi_i phi_j du_iv/dx_jv of the advection (steady Jacobian)
 };
 
 namespace fmk {

@@ -39,7 +39,7 @@ EXPORTS = [
     "fm_system_set_x", "fm_system_get_rhs", "fm_system_set_rhs", "fm_system_set_elem_coef",
     "fm_system_set_velocity", "fm_heat_genmatrhs", "fm_ns_genmatrhs", "fm_spmv", "fm_residual",
     "fm_restrict", "fm_prolong", "fm_get_res", "fm_mg_setup", "fm_mg_solve", "fm_vanka_patch", "fm_adjoint_solve",
-    "fm_heat_grad_coef", "fm_bench_spmv", "fm_sync", "fm_heat_genmat_transpose", "fm_ns_genmat_transpose",
+    "fm_heat_grad_coef", "fm_bench_spmv", "fm_sync", "fm_heat_genmat_transpose", "fm_ns_genmat_transpose", "fm_ns_genjac",
     "fm_get_adjoint_csr", "fm_heat_dissipation", "fm_bench_stream", "fm_ns_dissipation", "fm_ns_grad_coef",
     "fm_system_set_bcdata", "fm_heat_bc_rhs", "fm_ns_bc_rhs", "fm_ns_pressure_drop",
     "fm_ns_set_buoyancy", "fm_heat_nusselt",
@@ -129,6 +129,7 @@ def lib():
     L.fm_heat_grad_coef.argtypes = [vp, C.POINTER(HeatParams), f64p, f64p, f64p]
     L.fm_heat_genmat_transpose.argtypes = [vp, C.c_int, C.POINTER(HeatParams)]
     L.fm_ns_genmat_transpose.argtypes = [vp, C.c_int, C.POINTER(NSParams)]
+    L.fm_ns_genjac.argtypes = [vp, C.c_int, C.c_int, C.POINTER(NSParams)]
     L.fm_get_adjoint_csr.argtypes = [vp, C.c_int, f64p]
     L.fm_heat_dissipation.argtypes = [vp, C.POINTER(HeatParams), f64p, C.POINTER(C.c_double), vp, vp]
     L.fm_bench_spmv.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]
@@ -483,6 +484,10 @@ class MGSolNS(MGSolDA):
     # -- discrete adjoint of the Oseen (Picard) system ----------------------------
     def GenMatTranspose(self, Level):
         _ck(lib().fm_ns_genmat_transpose(self.h, Level, C.byref(self.params)))
+
+    def GenJac(self, Level, trans=0):
+        """Newton Jacobian of the steady residual at x_old (trans = 1: its transpose, for the adjoint solve)."""
+        _ck(lib().fm_ns_genjac(self.h, Level, trans, C.byref(self.params)))
 
     def dissipation(self, U):
         """J = sum_e nu_e int_e |grad u|^2, dJ/dU, dJ/dnu_e (partial) on the top level."""

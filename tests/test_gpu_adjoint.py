@@ -281,36 +281,27 @@ def test_cavity_lid_and_viscosity_gradient_fd(femus, ctx):
         assert abs(fd - g_a[k]) <= 1e-5 * abs(g_a[k]), (k, fd, g_a[k])
 
 
-def test_cavity_gradient_vs_cpu_reference(femus, oracle, ctx):
-    """The same gradient assembled and solved on the CPU: oracle Oseen matrix and RHS, sparse direct solves,
-    element Laplacians and viscous blocks from the oracle's element routines."""
-    m, ns, G, a0, nu, prob = _cavity_adjoint(femus, ctx)
-    prob.solve(nu, a0)
-    g_nu, g_a = prob.gradient()
-    nx, nlev, dim = (4, 4), m.nlev, 2
+def _cpu_cavity(oracle, nx, nlev, p, nu):
+    """Oracle mesh, DoF and BC maps, pattern, and per-element pieces of the top level: scalar Q2 Laplacians
+    (heat element routine, pure diffusion), velocity DoF lists, and the viscous part of the NS element matrix
+    (element matrix with the element viscosity minus the one with viscosity 0)."""
+    dim = 2
     top = nlev - 1
     om = oracle.Mesh(2, list(nx), [0, 0], [1, 1], nlev)
     d, n = oracle.init_dof(om, top, 2, 1)
     bc = functools.partial(configs.cavity_bc, dim=2)
     b0, b1 = oracle.genbc(om, 2, 1, d, bc)
     rp, ci = oracle.system_pattern(om, top, 2, 1, d)
-    p = oracle.NSParams(0.0025, 0.1, 0.0, 0.1, 0.0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0))
-    xold = a0 @ G
-    v, rhs = oracle.ns_genmatrhs(om, top, 1, p, d, d, b0, b1, xold, rp, ci, nu_elem=nu)
-    A = sp.csr_matrix((v, ci, rp), shape=(n, n)).tocsc()
-    U = spla.spsolve(A, rhs)
-    # element Laplacians (scalar Q2) from the heat element routine with a pure-diffusion parameter set
     dh, nh = oracle.init_dof(om, top, 1, 0)
     node_of_h = np.empty(nh, np.int64)
     Nn = om.N
     node_of_h[dh[:Nn][dh[:Nn] >= 0]] = np.flatnonzero(dh[:Nn] >= 0)
     ph = oracle.HeatParams(1.0, 1.0, 0.1, 0.0, 0.0, 0.0, (0.0, 0.0, 0.0))
     ones = np.ones(nh, np.int32)
-    ne = len(nu)
     nl = dim * 9 + 4
     zero = np.zeros(n)
     Kl, Dv, vd = [], [], []
-    for e in range(ne):
+    for e in range(len(nu)):
         KeM = np.zeros(27 * 27); FeM = np.zeros(27); hd = np.zeros(27, np.int32)
         oracle.lib().om_heat_element(om.h, top, e, 0, C.byref(ph), dh, ones, ones, np.zeros(nh), 1.0, None, KeM, FeM,
                                      hd, dh.ctypes.data_as(C.c_void_p))
@@ -323,19 +314,49 @@ def test_cavity_gradient_vs_cpu_reference(femus, oracle, ctx):
         oracle.lib().om_ns_element(om.h, top, e, 0, C.byref(p), d, b0, b1, zero, 0.0, M0, F, dd,
                                    d.ctypes.data_as(C.c_void_p))
         Dv.append(((M1 - M0)[:nl * nl].reshape(nl, nl)[:dim * 9, :dim * 9], dd[:dim * 9].copy()))
+    return om, d, n, b0, b1, rp, ci, (Kl, Dv, vd)
+
+
+def _cpu_dissipation(U, nu, pieces, n):
+    """J = sum_e nu_e sum_iv u_e,iv^T Klap_e u_e,iv and dJ/dU on the CPU."""
+    Kl, _, vd = pieces
     dJdU = np.zeros(n)
     J = 0.0
-    for e in range(ne):
-        for iv in range(dim):
+    for e in range(len(nu)):
+        for iv in range(2):
             ue = U[vd[e][iv]]
             J += nu[e] * ue @ Kl[e] @ ue
             dJdU[vd[e][iv]] += 2 * nu[e] * Kl[e] @ ue
-    lam = spla.spsolve(A.T.tocsc(), dJdU)
-    gc_nu = np.zeros(ne)
-    for e in range(ne):
-        part = sum(U[vd[e][iv]] @ Kl[e] @ U[vd[e][iv]] for iv in range(dim))
+    return J, dJdU
+
+
+def _cpu_grad_nu(U, lam, nu, pieces):
+    """dJ/dnu_e = partial - lambda^T (dA/dnu_e) U with the element viscous blocks."""
+    Kl, Dv, vd = pieces
+    g = np.zeros(len(nu))
+    for e in range(len(nu)):
+        part = sum(U[vd[e][iv]] @ Kl[e] @ U[vd[e][iv]] for iv in range(2))
         Dm, dd = Dv[e]
-        gc_nu[e] = part - lam[dd] @ (Dm @ U[dd])
+        g[e] = part - lam[dd] @ (Dm @ U[dd])
+    return g
+
+
+def test_cavity_gradient_vs_cpu_reference(femus, oracle, ctx):
+    """The same gradient assembled and solved on the CPU: oracle Oseen matrix and RHS, sparse direct solves,
+    element Laplacians and viscous blocks from the oracle's element routines."""
+    m, ns, G, a0, nu, prob = _cavity_adjoint(femus, ctx)
+    prob.solve(nu, a0)
+    g_nu, g_a = prob.gradient()
+    top = m.nlev - 1
+    p = oracle.NSParams(0.0025, 0.1, 0.0, 0.1, 0.0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0))
+    om, d, n, b0, b1, rp, ci, pieces = _cpu_cavity(oracle, (4, 4), m.nlev, p, nu)
+    xold = a0 @ G
+    v, rhs = oracle.ns_genmatrhs(om, top, 1, p, d, d, b0, b1, xold, rp, ci, nu_elem=nu)
+    A = sp.csr_matrix((v, ci, rp), shape=(n, n)).tocsc()
+    U = spla.spsolve(A, rhs)
+    J, dJdU = _cpu_dissipation(U, nu, pieces, n)
+    lam = spla.spsolve(A.T.tocsc(), dJdU)
+    gc_nu = _cpu_grad_nu(U, lam, nu, pieces)
     gc_a = np.zeros(len(a0))
     for k in range(len(a0)):
         _, rk = oracle.ns_genmatrhs(om, top, 1, p, d, d, b0, b1, G[k], rp, ci, nu_elem=nu)
@@ -343,6 +364,173 @@ def test_cavity_gradient_vs_cpu_reference(femus, oracle, ctx):
     assert abs(ns.dissipation(U)[0] - J) <= 1e-10 * abs(J)
     np.testing.assert_allclose(g_nu, gc_nu, rtol=1e-7, atol=1e-9 * np.abs(gc_nu).max())
     np.testing.assert_allclose(g_a, gc_a, rtol=1e-7, atol=1e-9 * np.abs(gc_a).max())
+
+
+# ---------------------------------------------------------------------------
+# Steady Navier-Stokes (Galerkin, upwind 0): Newton Jacobian of R(U) = A(U) U - b and the adjoint of a
+# functional of the steady state.
+# ---------------------------------------------------------------------------
+def _newton_check(femus, oracle, ctx, case):
+    """Returns (GPU Newton Jacobian, its transpose from the adjoint assembly, oracle directional derivative
+    function, U, pattern)."""
+    if case == "cavity2d":
+        dim, nx, nlev, lo, hi = 2, [2, 2], 2, [0, 0], [1, 1]
+        bc = functools.partial(configs.cavity_bc, dim=2)
+    else:
+        dim, nx, nlev, lo, hi = 3, [2, 1, 1], 2, [0, 0, 0], [4, 1, 1]
+        bc = configs.channel_bc
+    m = femus.Mesh(dim, nx, lo, hi, nlev)
+    ns = femus.MGSolNS(ctx, m, IRe=0.01, dt=0.1, time_coeff=0.0, upwind=0.0)
+    ns.GenBc(bc)
+    ns.init()
+    top = nlev - 1
+    rng = np.random.default_rng(11)
+    U = rng.standard_normal(ns._Dim[top])
+    nu = 0.01 * configs.lognormal_k(m.level(top)[0], seed=4, sigma=0.5)
+    ns.set_elem_coef(top, nu)
+    ns.set_xold(U)
+    ns.GenJac(top, 0)
+    Jg = _csr(ns, top)
+    ns.GenJac(top, 1)
+    rp, ci, _ = ns.csr(0, top)
+    JgT = sp.csr_matrix((ns.adjoint_csr(top), ci, rp), shape=Jg.shape)
+    om = oracle.Mesh(dim, nx, lo, hi, nlev)
+    d, n = oracle.init_dof(om, top, dim, 1)
+    b0, b1 = oracle.genbc(om, dim, 1, d, bc)
+    orp, oci = oracle.system_pattern(om, top, dim, 1, d)
+    p = oracle.NSParams(0.01, 0.1, 0.0, 0.0, 0.0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0))
+
+    def F(W):          # A_oracle(W) W: quadratic in W
+        v, _ = oracle.ns_genmatrhs(om, top, 1, p, d, d, b0, b1, W, orp, oci, nu_elem=nu)
+        return sp.csr_matrix((v, oci, orp), shape=(n, n)) @ W
+    return Jg, JgT, F, U
+
+
+@pytest.mark.parametrize("case", ["cavity2d", "channel3d"])
+def test_newton_jacobian_vs_oracle(femus, oracle, ctx, case):
+    """J_N delta equals the directional derivative of the oracle's A(W) W (central differences are exact for
+    a quadratic map), and the transposed Newton assembly is exactly its transpose."""
+    Jg, JgT, F, U = _newton_check(femus, oracle, ctx, case)
+    assert (JgT != Jg.T).nnz == 0
+    rng = np.random.default_rng(5)
+    for _ in range(3):
+        dl = rng.standard_normal(len(U))
+        eps = 1e-3
+        fd = (F(U + eps * dl) - F(U - eps * dl)) / (2 * eps)
+        jd = Jg @ dl
+        assert np.linalg.norm(jd - fd) <= 1e-10 * np.linalg.norm(fd)
+    # the Newton term is really there: J_N differs from the Picard matrix by more than rounding
+    assert np.linalg.norm(Jg @ U - F(U)) > 1e-3 * np.linalg.norm(F(U))
+
+
+def _steady_cavity(femus, ctx, nx=(8, 8), nlev=3, IRe=0.0025, K=16, sigma=0.5):
+    from exfemus_amd import adjoint
+    m = femus.Mesh(2, list(nx), [0, 0], [1, 1], nlev)
+    ns = femus.MGSolNS(ctx, m, IRe=IRe, dt=0.1, time_coeff=0.0, upwind=0.0)
+    ns.GenBc(functools.partial(configs.cavity_bc, dim=2))
+    ns.init()
+    G = adjoint.lid_fourier_basis(ns, K)
+    rng = np.random.default_rng(2026)
+    a0 = rng.standard_normal(K) / np.arange(1, K + 1)
+    a0[0] += 1.0                               # a leading sin(pi x) lid, |u| ~ 1: Re ~ 400 on average
+    ne = m.level(nlev - 1)[0]
+    nu = IRe * configs.lognormal_k(ne, seed=7, sigma=sigma)
+    opts = ns.opts(precision=3, smoother=femus.FM_SMOOTH_VANKA, vanka_colored=1, nu_pre=4, nu_post=4, tol=1e-13,
+                   max_iter=2000)
+    prob = adjoint.SteadyNSProblem(ns, G, functional="dissipation", opts=opts)
+    return m, ns, G, a0, nu, prob
+
+
+def test_steady_adjoint_gradient_fd(femus, ctx):
+    """Steady cavity (Re ~ 400, log-normal viscosity): dJ/dnu_e and dJ/da_k from the transposed Newton
+    Jacobian against central differences of the steady state (h = 1e-6, Newton from the base state)."""
+    m, ns, G, a0, nu, prob = _steady_cavity(femus, ctx)
+    J0 = prob.solve(nu, a0)
+    print("steady state:", prob.stats, flush=True)
+    assert prob.stats["rel_residual"] < 1e-10, prob.stats
+    U0 = prob.U.copy()
+    g_nu, g_a = prob.gradient()
+    print("adjoint:", prob.stats, flush=True)
+
+    def J_of(nu_t, a):
+        return prob.solve(nu_t, a, U0=U0)
+
+    h = 1e-6
+    for e in np.argsort(-np.abs(g_nu * nu))[:4]:
+        up = nu.copy(); up[e] *= 1 + h
+        um = nu.copy(); um[e] *= 1 - h
+        fd = (J_of(up, a0) - J_of(um, a0)) / (2 * h * nu[e])
+        assert abs(fd - g_nu[e]) <= 1e-5 * abs(g_nu[e]), (e, fd, g_nu[e], J0)
+    # a steady state is accurate to ~cond * eps (~1e-13 here), so central differences carry ~1e-7 absolute
+    # noise: the checked components are those well above it (the four largest |dJ/da_k|, as for nu_e)
+    for k in np.argsort(-np.abs(g_a))[:4]:
+        hk = h                                 # absolute step on the O(0.1-1) lid coefficients
+        ap = a0.copy(); ap[k] += hk
+        am = a0.copy(); am[k] -= hk
+        fd = (J_of(nu, ap) - J_of(nu, am)) / (2 * hk)
+        assert abs(fd - g_a[k]) <= 1e-5 * abs(g_a[k]), (k, fd, g_a[k])
+
+
+def test_steady_adjoint_vs_cpu_reference(femus, oracle, ctx):
+    """The steady-state gradient rebuilt on the CPU: Newton on the oracle's assembly, with the Jacobian
+    A(U) + C(U) where C(U) delta = A(delta) U - A(0) U is recovered column-group by column-group (groups of
+    columns sharing no row), sparse direct solves, oracle element routines for the gradient."""
+    m, ns, G, a0, nu, prob = _steady_cavity(femus, ctx)
+    prob.solve(nu, a0)
+    g_nu, g_a = prob.gradient()
+    top = m.nlev - 1
+    p = oracle.NSParams(0.0025, 0.1, 0.0, 0.0, 0.0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0))
+    om, d, n, b0, b1, rp, ci, pieces = _cpu_cavity(oracle, (8, 8), m.nlev, p, nu)
+    gbd = a0 @ G                                       # boundary data (the oracle reads it from x_old)
+    _, b = oracle.ns_genmatrhs(om, top, 1, p, d, d, b0, b1, gbd, rp, ci, nu_elem=nu)
+
+    def Aof(W):
+        v, _ = oracle.ns_genmatrhs(om, top, 1, p, d, d, b0, b1, W, rp, ci, nu_elem=nu)
+        return sp.csr_matrix((v, ci, rp), shape=(n, n))
+    pat = sp.csr_matrix((np.ones(len(ci)), ci, rp), shape=(n, n))
+    conflict = (pat.T @ pat).tocsr()                   # columns sharing a row
+    color = -np.ones(n, int)
+    for j in range(n):
+        used = set(color[conflict.indices[conflict.indptr[j]:conflict.indptr[j + 1]]].tolist())
+        c = 0
+        while c in used:
+            c += 1
+        color[j] = c
+    A0 = Aof(np.zeros(n))
+    U = prob.U.copy()
+    for _ in range(3):                                 # Newton on the CPU from the GPU state
+        AU = Aof(U)
+        rows, cols, vals = [], [], []
+        for c in range(color.max() + 1):
+            sel = color == c
+            dl = sel.astype(float)
+            Cd = Aof(dl) @ U - A0 @ U                  # = sum over the group's columns of C[:, j]
+            sub = pat[:, sel].tocoo()
+            jcol = np.flatnonzero(sel)[sub.col]
+            rows.append(sub.row); cols.append(jcol); vals.append(Cd[sub.row])
+        Cm = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
+        Jn = (AU + Cm).tocsc()
+        U = U + spla.spsolve(Jn, b - AU @ U)
+    assert np.linalg.norm(b - Aof(U) @ U) <= 1e-11 * np.linalg.norm(b)
+    # the enclosed cavity's pressure is defined up to a constant (the sparse LU picks its own)
+    N = m.N
+    pdofs = ns.node_dof(top)[2 * N:3 * N]
+    isp = np.zeros(n, bool)
+    isp[pdofs[pdofs >= 0]] = True
+    Ug = prob.U.copy()
+    U[isp] -= U[isp].mean()
+    Ug[isp] -= Ug[isp].mean()
+    assert np.linalg.norm(U[~isp] - Ug[~isp]) <= 1e-8 * np.linalg.norm(U[~isp])
+    assert np.linalg.norm(U[isp] - Ug[isp]) <= 1e-7 * np.linalg.norm(U[isp])
+    J, dJdU = _cpu_dissipation(U, nu, pieces, n)
+    lam = spla.spsolve(Jn.T.tocsc(), dJdU)
+    gc_nu = _cpu_grad_nu(U, lam, nu, pieces)
+    gc_a = np.zeros(len(a0))
+    for k in range(len(a0)):
+        _, rk = oracle.ns_genmatrhs(om, top, 1, p, d, d, b0, b1, G[k], rp, ci, nu_elem=nu)
+        gc_a[k] = lam @ rk
+    np.testing.assert_allclose(g_nu, gc_nu, rtol=1e-6, atol=1e-8 * np.abs(gc_nu).max())
+    np.testing.assert_allclose(g_a, gc_a, rtol=1e-6, atol=1e-8 * np.abs(gc_a).max())
 
 
 def test_cavity_gradcheck_torch(femus, ctx):
