@@ -86,14 +86,14 @@ class SteadyNSProblem(OseenProblem):
     Newton Jacobian of a coarse grid at high mesh Peclet number makes a poor coarse operator)."""
 
     def __init__(self, ns, G, g0=None, functional="dissipation", opts=None, tol=1e-12, picard_tol=1e-3,
-                 max_picard=200, max_newton=30, coarse_upwind=0.1, newton_rtol=1e-6):
+                 max_picard=200, max_newton=30, coarse_upwind=0.1, newton_rtol=1e-6, picard_rtol=1e-6):
         if ns.params.upwind != 0.0:
             raise ValueError("the steady Newton Jacobian needs upwind = 0")
         super().__init__(ns, G, g0, functional, opts)
         self.tol, self.picard_tol = tol, picard_tol
         self.max_picard, self.max_newton = max_picard, max_newton
         self.coarse_upwind = coarse_upwind
-        self.newton_rtol = newton_rtol
+        self.newton_rtol, self.picard_rtol = newton_rtol, picard_rtol
         self.U = None
         self.stats = {}
 
@@ -105,10 +105,11 @@ class SteadyNSProblem(OseenProblem):
         b = ns.get_rhs()
         return b - ns.spmv(top, U), np.linalg.norm(b)
 
-    def _step_opts(self):
+    def _step_opts(self, tol):
+        """The solver options with another relative tolerance (Picard and Newton steps need not be exact)."""
         o = type(self.opts)()
         C.pointer(o)[0] = self.opts
-        o.tol = self.newton_rtol
+        o.tol = tol
         return o
 
     def _coarse(self, transpose):
@@ -141,7 +142,7 @@ class SteadyNSProblem(OseenProblem):
                     for L in range(self.nlev):
                         ns.GenMatRhs(0.0, L, 1 if L == top else 0)
                     ns.set_x(U)
-                    rep = ns.MGSolve(opts=self.opts)
+                    rep = ns.MGSolve(opts=self._step_opts(self.picard_rtol))
                     if rep.converged != 1:
                         raise RuntimeError(f"Picard step: {rep.iters} its, {rep.rel_resid:.2e}")
                     U1 = ns.get_x()
@@ -164,7 +165,7 @@ class SteadyNSProblem(OseenProblem):
                 self._coarse(False)
                 ns.set_rhs(r)
                 ns.set_x(np.zeros(N))
-                rep = ns.MGSolve(opts=self._step_opts())
+                rep = ns.MGSolve(opts=self._step_opts(self.newton_rtol))
                 its_newton.append(rep.iters)
                 if rep.converged != 1 and not rep.rel_resid < 1e-3:      # inexact Newton: a reduced residual will do
                     raise RuntimeError(f"Newton step: {rep.iters} its, {rep.rel_resid:.2e}")

@@ -12,6 +12,8 @@ no-slip; T = 1 at x = 0 (hot), T = 0 at x = 1 (cold), adiabatic elsewhere.
 The Nusselt number is the box-averaged heat flux along x in units of the
 conductive one (fm_heat_nusselt).
 """
+import os
+
 import numpy as np
 
 from . import configs, femus as F
@@ -91,6 +93,34 @@ class Boussinesq:
         du = np.linalg.norm(u - u_prev) / max(np.linalg.norm(u), 1e-300)
         dT = np.linalg.norm(T_new - T_old) / max(np.linalg.norm(T_new), 1e-300)
         return du, dT
+
+    def names(self):
+        return ["u", "v", "w"][:self.dim] + ["p"], ["T"]
+
+    def print_step(self, outdir, step, time):
+        """sol.<step>.bin/.json with u, v, (w), p and T on the top level, its XDMF file, and the mesh file
+        once (print_u of both systems + print_soln_xmf)."""
+        from . import solio
+        os.makedirs(outdir, exist_ok=True)
+        top = self.mesh.nlev - 1
+        mbase = os.path.join(outdir, "mesh")
+        if not os.path.exists(mbase + ".json"):
+            solio.print_mesh(self.mesh, top, mbase)
+        base = os.path.join(outdir, f"sol.{step:04d}")
+        ns_names, T_names = self.names()
+        solio.print_u(self.mesh, top, base,
+                      [(ns_names, self.ns.node_dof(top), self.ns.get_x(), self.dim, 1, None),
+                       (T_names, self.T.node_dof(top), self.xT, 1, 0, None)])
+        solio.print_xmf(base + ".xmf", time, self.mesh, top, mbase, base, ns_names + T_names)
+        return base + ".xmf"
+
+    def read_step(self, outdir, step):
+        """Restart from sol.<step> (read_u of both systems); the next step continues from that state."""
+        base = os.path.join(outdir, f"sol.{step:04d}")
+        ns_names, T_names = self.names()
+        U = self.ns.read_u(base, ns_names)
+        self.xT = self.T.read_u(base, T_names)
+        self.T.set_velocity(self.ns.velocity_nodes(U))
 
     def run(self, max_steps=400, tol=1e-8, callback=None):
         """Pseudo-time steps until both relative changes fall below tol; returns the number of steps."""

@@ -377,6 +377,24 @@ class MGSolDA:
     def MGSetup(self, opts):
         _ck(lib().fm_mg_setup(self.h, C.byref(opts)))
 
+    def print_u(self, base, names, L=None, x=None, refvalue=None):
+        """Solution file of this system on level L (MGSolDA::print_u): nodal datasets named after the
+        variables; x defaults to the current solution of the top level."""
+        from . import solio
+        L = self._NoLevels - 1 if L is None else L
+        x = self.get_x(L) if x is None else x
+        return solio.print_u(self.mesh, L, base, [(names, self.node_dof(L), x, self.nq2, self.nq1, refvalue)])
+
+    def read_u(self, base, names, L=None, refvalue=None):
+        """Restart (MGSolDA::read_u): x_old and x of level L from a solution file written on that level."""
+        from . import solio
+        L = self._NoLevels - 1 if L is None else L
+        x = solio.read_u(self.mesh, L, base, names, self.node_dof(L), self.nq2, self.nq1, self._Dim[L], refvalue)
+        if L == self._NoLevels - 1:
+            self.set_xold(x)
+        self.set_x(x, L)
+        return x
+
     def vanka_patch(self, L, e):
         """(dofs, K_e^-1): the FP64 local inverse the Vanka smoother applies for element e on level L."""
         nl = (self.nq2 * 27 if self.mesh.dim == 3 else self.nq2 * 9) + (self.nq1 * (8 if self.mesh.dim == 3 else 4))

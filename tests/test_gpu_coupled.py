@@ -76,3 +76,28 @@ def test_nusselt_conductivity_gradient_fd(femus, ctx):
         assert abs(fd - grad[e]) <= 1e-5 * abs(grad[e]), (e, fd, grad[e])
     b.ns.close()
     b.T.close()
+
+
+def test_restart_from_solution_file(femus, ctx, tmp_path):
+    """print_step after 5 coupled steps, a fresh driver restarted with read_step, 3 more steps: the same state
+    as 8 uninterrupted steps (the files hold the full FP64 nodal state); the XDMF time file lists both steps."""
+    from exfemus_amd import coupled, solio
+    a = coupled.Boussinesq(ctx, 2, 2, 3, 1e4)
+    for k in range(5):
+        a.step(float(k + 1))
+    f5 = a.print_step(str(tmp_path), 5, 5.0)
+    for k in range(5, 8):
+        a.step(float(k + 1))
+    f8 = a.print_step(str(tmp_path), 8, 8.0)
+    solio.print_time_xmf(str(tmp_path / "time.xmf"), [f5, f8])
+    b = coupled.Boussinesq(ctx, 2, 2, 3, 1e4)
+    b.read_step(str(tmp_path), 5)
+    for k in range(5, 8):
+        b.step(float(k + 1))
+    Ua, Ta = a.state()
+    Ub, Tb = b.state()
+    np.testing.assert_allclose(Ub, Ua, rtol=0, atol=1e-12 * np.abs(Ua).max())
+    np.testing.assert_allclose(Tb, Ta, rtol=0, atol=1e-12)
+    for s in (a, b):
+        s.ns.close()
+        s.T.close()
