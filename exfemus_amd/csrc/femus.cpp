@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -205,8 +206,9 @@ extern "C" int fm_system_create(fm_ctx *ctx, const fm_mesh *m, int nq2, int nq1,
 
 static void free_sell(DevSELL &q) {
   dfree(q.sl_off); dfree(q.sl_c32); dfree(q.sl_w); dfree(q.perm); dfree(q.c16); dfree(q.base); dfree(q.c32);
+  dfree(q.sl_c8); dfree(q.c8); dfree(q.sl_c16);
   dfree(q.map); dfree(q.v); dfree(q.vf);
-  q.nsl = 0; q.nslot = 0; q.n32 = 0;
+  q.nsl = 0; q.nslot = 0; q.n32 = 0; q.n8 = 0; q.n16 = 0;
 }
 
 void fm_free_csr(DevCSR &c) {
@@ -264,7 +266,7 @@ int fm_build_sell(hipStream_t st, DevCSR &c, const int *ro, const int *cr, const
   c.sell.h_gslice = gslice;
   const int64_t nslot = off[nsl];
   std::vector<int> absc(nslot, -1), map(nslot, -1), base(nslot / C, 0);
-  std::vector<char> wide(nsl, 0);
+  std::vector<char> wide(nsl, 0), narrow(nsl, 1);
 #pragma omp parallel
   {
     std::vector<std::pair<int, int>> ent;
@@ -293,15 +295,23 @@ int fm_build_sell(hipStream_t st, DevCSR &c, const int *ro, const int *cr, const
         }
         base[off[s] / C + k] = mn;
         if (mx - mn > 65535) wide[s] = 1;
+        if (mx - mn > 255) narrow[s] = 0;
       }
     }
   }
-  std::vector<int64_t> o32(nsl, -1);
-  int64_t n32 = 0;
-  for (int64_t s = 0; s < nsl; s++)
+  // 8-bit offsets for the slices that allow them (FEMUS_SELL_C8=0 keeps every slice at 16 bits)
+  const char *c8env = getenv("FEMUS_SELL_C8");
+  const bool use8 = !(c8env && atoi(c8env) == 0);
+  std::vector<int64_t> o32(nsl, -1), o8(nsl, -1), o16(nsl, -1);
+  int64_t n32 = 0, n8 = 0, n16 = 0;
+  for (int64_t s = 0; s < nsl; s++) {
     if (wide[s]) { o32[s] = n32; n32 += (int64_t)wd[s] * C; }
-  std::vector<unsigned short> c16(nslot, 0);
+    else if (use8 && narrow[s]) { o8[s] = n8; n8 += (int64_t)((wd[s] + 3) / 4) * 4 * C; }
+    else { o16[s] = n16; n16 += (int64_t)((wd[s] + 1) / 2) * 2 * C; }
+  }
+  std::vector<unsigned short> c16(std::max<int64_t>(n16, 1), 0);
   std::vector<int> c32(std::max<int64_t>(n32, 1), 0);
+  std::vector<unsigned char> c8(std::max<int64_t>(n8, 1), 0);
 #pragma omp parallel for schedule(static)
   for (int64_t s = 0; s < nsl; s++)
     for (int k = 0; k < wd[s]; k++) {
@@ -309,22 +319,29 @@ int fm_build_sell(hipStream_t st, DevCSR &c, const int *ro, const int *cr, const
       for (int l = 0; l < C; l++) {
         int64_t slot = off[s] + (int64_t)k * C + l;
         const int cc = absc[slot] >= 0 ? absc[slot] : b;  // padding: the position's base column, value 0
+        // 8-bit (16-bit): positions 4g..4g+3 (2g, 2g+1) of lane l packed in one 32-bit word, the words of a
+        // group lane-contiguous
         if (wide[s]) c32[o32[s] + (int64_t)k * C + l] = cc;
-        else c16[slot] = (unsigned short)(cc - b);
+        else if (o8[s] >= 0) c8[o8[s] + (int64_t)(k / 4) * 4 * C + 4 * l + (k % 4)] = (unsigned char)(cc - b);
+        else c16[o16[s] + (int64_t)(k / 2) * 2 * C + 2 * l + (k % 2)] = (unsigned short)(cc - b);
       }
     }
   DevSELL &q = c.sell;
   free_sell(q);
-  q.nsl = nsl; q.nslot = nslot; q.n32 = n32;
+  q.nsl = nsl; q.nslot = nslot; q.n32 = n32; q.n8 = n8; q.n16 = n16;
   FM_CHECK(dalloc(&q.sl_off, nsl + 1)); FM_CHECK(dalloc(&q.sl_c32, nsl)); FM_CHECK(dalloc(&q.sl_w, nsl));
-  FM_CHECK(dalloc(&q.perm, nsl * C)); FM_CHECK(dalloc(&q.c16, nslot)); FM_CHECK(dalloc(&q.base, nslot / C));
+  FM_CHECK(dalloc(&q.sl_c8, nsl)); FM_CHECK(dalloc(&q.c8, c8.size())); FM_CHECK(dalloc(&q.sl_c16, nsl));
+  FM_CHECK(dalloc(&q.perm, nsl * C)); FM_CHECK(dalloc(&q.c16, c16.size())); FM_CHECK(dalloc(&q.base, nslot / C));
   FM_CHECK(dalloc(&q.c32, c32.size())); FM_CHECK(dalloc(&q.map, nslot)); FM_CHECK(dalloc(&q.v, nslot));
   FM_CHECK(dalloc(&q.vf, nslot));
   FM_HIP(hipMemcpyAsync(q.sl_off, off.data(), (nsl + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
   FM_HIP(hipMemcpyAsync(q.sl_c32, o32.data(), nsl * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  FM_HIP(hipMemcpyAsync(q.sl_c8, o8.data(), nsl * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  FM_HIP(hipMemcpyAsync(q.c8, c8.data(), c8.size(), hipMemcpyHostToDevice, st));
   FM_HIP(hipMemcpyAsync(q.sl_w, wd.data(), nsl * sizeof(int), hipMemcpyHostToDevice, st));
   FM_HIP(hipMemcpyAsync(q.perm, perm.data(), nsl * C * sizeof(int), hipMemcpyHostToDevice, st));
-  FM_HIP(hipMemcpyAsync(q.c16, c16.data(), nslot * sizeof(unsigned short), hipMemcpyHostToDevice, st));
+  FM_HIP(hipMemcpyAsync(q.c16, c16.data(), c16.size() * sizeof(unsigned short), hipMemcpyHostToDevice, st));
+  FM_HIP(hipMemcpyAsync(q.sl_c16, o16.data(), nsl * sizeof(int64_t), hipMemcpyHostToDevice, st));
   FM_HIP(hipMemcpyAsync(q.base, base.data(), (nslot / C) * sizeof(int), hipMemcpyHostToDevice, st));
   FM_HIP(hipMemcpyAsync(q.c32, c32.data(), c32.size() * sizeof(int), hipMemcpyHostToDevice, st));
   FM_HIP(hipMemcpyAsync(q.map, map.data(), nslot * sizeof(int), hipMemcpyHostToDevice, st));

@@ -32,17 +32,23 @@ struct fm_mesh {
 // 4096, slice s holds internal rows perm[64s .. 64s+63] with width sl_w[s];
 // entry k of lane l sits at slot sl_off[s] + 64k + l (column-major inside the
 // slice, so a wavefront's loads are contiguous).  Inside each row the entries
-// are sorted by internal column.  Columns are stored as 16-bit offsets from a
+// are sorted by internal column.  Columns are stored as offsets from a
 // per-slot-column base, base[sl_off[s]/64 + k] (the smallest column among the
-// 64 lanes at position k); a slice in which some position spans 65536 columns
-// or more keeps absolute 32-bit columns instead, at c32[sl_c32[s] + 64k + l]
-// (sl_c32[s] = -1 for 16-bit slices).  map[slot] is the CSR position of the
-// entry (-1 for padding; padding repeats the slot's base column with value 0).
+// 64 lanes at position k), in the narrowest of three classes per slice:
+//   8-bit  (every position spans <= 255 columns): offset of position k, lane l
+//          at byte c8[sl_c8[s] + 256 (k/4) + 4 l + k%4] (four positions per
+//          32-bit word, the 64 words of a group contiguous);
+//   16-bit (spans <= 65535): c16[sl_c16[s] + 128 (k/2) + 2 l + k%2];
+//   32-bit absolute columns otherwise: c32[sl_c32[s] + 64 k + l].
+// The offsets of the other classes are -1.  map[slot] is the CSR position of
+// the entry (-1 for padding; padding repeats the slot's base column with
+// value 0).
 struct DevSELL {
-  int64_t nsl = 0, nslot = 0, n32 = 0;
-  int64_t *sl_off = nullptr, *sl_c32 = nullptr;
+  int64_t nsl = 0, nslot = 0, n32 = 0, n8 = 0, n16 = 0;
+  int64_t *sl_off = nullptr, *sl_c32 = nullptr, *sl_c8 = nullptr, *sl_c16 = nullptr;
   int *sl_w = nullptr, *perm = nullptr, *base = nullptr, *c32 = nullptr, *map = nullptr;
   unsigned short *c16 = nullptr;
+  unsigned char *c8 = nullptr;
   double *v = nullptr;
   float *vf = nullptr;
   std::vector<int64_t> h_gslice;  // first slice of each row group (+ end)
@@ -173,7 +179,7 @@ extern const int fm_face3d[6][9];
 #include "fm_kernels.h"
 template <typename T> inline void fm_sell_fields(CsrView<T> &v, const DevSELL &q) {
   v.nsl = q.nsl; v.sl_off = q.sl_off; v.sl_w = q.sl_w; v.perm = q.perm; v.c16 = q.c16; v.base = q.base;
-  v.sl_c32 = q.sl_c32; v.c32 = q.c32;
+  v.sl_c32 = q.sl_c32; v.c32 = q.c32; v.sl_c8 = q.sl_c8; v.c8 = q.c8; v.sl_c16 = q.sl_c16;
 }
 template <typename T> CsrView<T> fm_view(const DevCSR &c);
 template <> inline CsrView<double> fm_view<double>(const DevCSR &c) {

@@ -12,9 +12,10 @@ struct CsrView {
   int lpr = 16;  // lanes per row
   // SELL-64 copy in internal DoF order (used when nsl > 0)
   int64_t nsl = 0;
-  const int64_t *sl_off = nullptr, *sl_c32 = nullptr;
+  const int64_t *sl_off = nullptr, *sl_c32 = nullptr, *sl_c8 = nullptr, *sl_c16 = nullptr;
   const int *sl_w = nullptr, *perm = nullptr, *base = nullptr, *c32 = nullptr;
   const unsigned short *c16 = nullptr;
+  const unsigned char *c8 = nullptr;
   const T *sv = nullptr;
   // optional restriction of the SELL product to the slices slist[0 .. nlist-1]
   const int *slist = nullptr;

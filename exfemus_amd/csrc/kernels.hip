@@ -78,15 +78,49 @@ __global__ __launch_bounds__(256) void k_spmv(int64_t nrows, const int64_t *__re
 // offsets are streamed with non-temporal loads so that the gathered vector
 // stays resident in L2 / Infinity Cache.  The slice index is made wave-uniform
 // (readfirstlane) so that the slice header and the per-position column bases
-// are scalar loads shared by the 64 lanes.  A 16-bit slice decodes column
-// base[k] + c16[slot]; a 32-bit slice reads c32.  With xcd_chunk > 0 the block
+// are scalar loads shared by the 64 lanes.  A slice decodes column
+// base[k] + c8[...] (8-bit slices), base[k] + c16[slot] (16-bit), or reads
+// absolute columns from c32 (32-bit).  With xcd_chunk > 0 the block
 // index is remapped so that the blocks the hardware places on one XCD (block
 // id modulo 8) walk one contiguous range of slices.
+// one lane's row of a slice whose column offsets are BITS wide (8 or 16), packed 32 / BITS positions per
+// 32-bit word: the word of group g (positions g * PER ..) of this lane is cw[64 g]
+template <typename T, int BITS, int UNR>
+__device__ __forceinline__ T sell_row_packed(const T *__restrict__ vp, const unsigned *__restrict__ cw,
+                                             const int *__restrict__ bp, int w, const T *__restrict__ xg) {
+  constexpr int PER = 32 / BITS;
+  constexpr unsigned MASK = (1u << BITS) - 1u;
+  static_assert(UNR % PER == 0, "the unroll must cover whole words");
+  T sum = 0;
+  int k = 0;
+  for (; k + UNR <= w; k += UNR) {
+    T v[UNR];
+    unsigned p[UNR / PER];
+#pragma unroll
+    for (int u = 0; u < UNR; u++) v[u] = __builtin_nontemporal_load(vp + (k + u) * 64);
+#pragma unroll
+    for (int g = 0; g < UNR / PER; g++) p[g] = __builtin_nontemporal_load(cw + (k / PER + g) * 64);
+    int c[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; u++) c[u] = bp[k + u] + (int)((p[u / PER] >> (BITS * (u % PER))) & MASK);
+    T xv[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; u++) xv[u] = xg[c[u]];
+#pragma unroll
+    for (int u = 0; u < UNR; u++) sum += v[u] * xv[u];
+  }
+  for (; k < w; k++)
+    sum += __builtin_nontemporal_load(vp + k * 64) * xg[bp[k] + (int)((cw[(k / PER) * 64] >> (BITS * (k % PER))) & MASK)];
+  return sum;
+}
+
 template <typename T, int EP, int UNR>
 __global__ __launch_bounds__(256) void k_sell(int nsl, const int64_t *__restrict__ sl_off,
                                               const int64_t *__restrict__ sl_c32, const int *__restrict__ sl_w,
                                               const int *__restrict__ perm, const unsigned short *__restrict__ c16,
                                               const int *__restrict__ base, const int *__restrict__ c32,
+                                              const int64_t *__restrict__ sl_c8, const unsigned char *__restrict__ c8,
+                                              const int64_t *__restrict__ sl_c16,
                                               const T *__restrict__ vals, const T *__restrict__ xg, EpArgs<T> ep,
                                               int xcd_chunk, const int *__restrict__ slist) {
   int blk = blockIdx.x;
@@ -99,29 +133,15 @@ __global__ __launch_bounds__(256) void k_sell(int nsl, const int64_t *__restrict
   const int w = sl_w[s];
   const int64_t off = sl_off[s];
   const int64_t o32 = sl_c32[s];
+  const int64_t o8 = sl_c8[s], o16 = sl_c16[s];
   const T *vp = vals + off + lane;
+  const int *bp = base + (off >> 6);
   T sum = 0;
   int k = 0;
-  if (o32 < 0) {
-    const unsigned short *cp = c16 + off + lane;
-    const int *bp = base + (off >> 6);
-    for (; k + UNR <= w; k += UNR) {
-      T v[UNR];
-      int c[UNR];
-#pragma unroll
-      for (int u = 0; u < UNR; u++) v[u] = __builtin_nontemporal_load(vp + (k + u) * 64);
-#pragma unroll
-      for (int u = 0; u < UNR; u++) c[u] = (int)__builtin_nontemporal_load(cp + (k + u) * 64);
-#pragma unroll
-      for (int u = 0; u < UNR; u++) c[u] += bp[k + u];
-      T xv[UNR];
-#pragma unroll
-      for (int u = 0; u < UNR; u++) xv[u] = xg[c[u]];
-#pragma unroll
-      for (int u = 0; u < UNR; u++) sum += v[u] * xv[u];
-    }
-    for (; k < w; k++)
-      sum += __builtin_nontemporal_load(vp + k * 64) * xg[bp[k] + (int)__builtin_nontemporal_load(cp + k * 64)];
+  if (o8 >= 0) {
+    sum = sell_row_packed<T, 8, UNR>(vp, reinterpret_cast<const unsigned *>(c8 + o8) + lane, bp, w, xg);
+  } else if (o16 >= 0) {
+    sum = sell_row_packed<T, 16, UNR>(vp, reinterpret_cast<const unsigned *>(c16 + o16) + lane, bp, w, xg);
   } else {
     const int *cp = c32 + o32 + lane;
     for (; k + UNR <= w; k += UNR) {
@@ -176,13 +196,13 @@ static int launch_spmv(hipStream_t s, const CsrView<T> &A, const T *x, const EpA
     const int nsl = (int)nwork;
     if (g_sell_unroll == 8)
       hipLaunchKernelGGL((k_sell<T, EP, 8>), grid, dim3(block), 0, s, nsl, A.sl_off, A.sl_c32, A.sl_w, A.perm, A.c16,
-                         A.base, A.c32, A.sv, x, ep, (int)chunk, A.slist);
+                         A.base, A.c32, A.sl_c8, A.c8, A.sl_c16, A.sv, x, ep, (int)chunk, A.slist);
     else if (g_sell_unroll == 16)
       hipLaunchKernelGGL((k_sell<T, EP, 16>), grid, dim3(block), 0, s, nsl, A.sl_off, A.sl_c32, A.sl_w, A.perm, A.c16,
-                         A.base, A.c32, A.sv, x, ep, (int)chunk, A.slist);
+                         A.base, A.c32, A.sl_c8, A.c8, A.sl_c16, A.sv, x, ep, (int)chunk, A.slist);
     else
       hipLaunchKernelGGL((k_sell<T, EP, 4>), grid, dim3(block), 0, s, nsl, A.sl_off, A.sl_c32, A.sl_w, A.perm, A.c16,
-                         A.base, A.c32, A.sv, x, ep, (int)chunk, A.slist);
+                         A.base, A.c32, A.sl_c8, A.c8, A.sl_c16, A.sv, x, ep, (int)chunk, A.slist);
     return hipGetLastError() == hipSuccess ? 0 : -2;
   }
   int64_t threads = A.nr * A.lpr;

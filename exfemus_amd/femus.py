@@ -28,7 +28,7 @@ FM_SMOOTH_CHEBYSHEV, FM_SMOOTH_JACOBI, FM_SMOOTH_VANKA, FM_SMOOTH_CHEBYSHEV4 = 0
 FM_ENOCONV = -3
 
 # storage format of the SELL SpMV (names the layout the PMC traffic figures were measured on)
-SPMV_FORMAT = "SELL-64 sigma4096 slab-grouped node order, 16-bit col offsets"
+SPMV_FORMAT = "SELL-64 sigma4096 slab-grouped node order, 8- or 16-bit col offsets per slice"
 
 # Exported entry points (the C ABI of include/femus_hip.h)
 EXPORTS = [
