@@ -467,6 +467,7 @@ def main():
     sec_copy = ctx.stream_copy(copy_bytes, reps=10)
     gbs_copy = 2 * copy_bytes / sec_copy / 1e9
     log(f"stream copy {gbs_copy:.0f} GB/s")
+    traffic32 = pmc_traffic(N, nnz, "fp32")
     out = {
         "metric": "MDoF/s to 1e-8 resid: FP32-MG+FP64 refine vs FP64 MG, 1 GPU; SpMV %HBM peak",
         "value": round(value, 3),
@@ -494,9 +495,13 @@ def main():
                  "fp32_us": round(sec32 * 1e6, 2), "fp32_GBs": round(gbs32, 1), "fp32_frac": round(gbs32 / HBM_PEAK_GBS, 4),
                  "bytes_alg_fp64": b64, "bytes_alg_fp32": b32,
                  "stream_copy_GBs": round(gbs_copy, 1), "fp64_frac_of_stream": round(gbs64 / gbs_copy, 4)},
-        # the mixed solve's dominant kernel is the FP32 SELL SpMV of the V-cycle
+        # the mixed solve's dominant kernel is the FP32 SELL SpMV of the V-cycle.  achieved / frac count the
+        # algorithmic bytes of a CSR product with int32 columns; the stored offsets are 8 or 16 bits wide, so
+        # the measured HBM traffic per launch (PMC) is lower and its rate is reported beside them
         "roofline": {"bound": "hbm", "achieved": round(gbs32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs32 / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(N, nnz, "fp32"),
+                     "frac": round(gbs32 / HBM_PEAK_GBS, 4), "traffic": traffic32,
+                     "traffic_GBs": None if traffic32 is None else round(traffic32 / sec32 / 1e9, 1),
+                     "traffic_frac": None if traffic32 is None else round(traffic32 / sec32 / 1e9 / HBM_PEAK_GBS, 4),
                      "bytes_alg": b32, "kernel": f"k_sell<float> y = A x ({F.SPMV_FORMAT}) on the top matrix"},
     }
     if not args.no_cpu and heat:
