@@ -38,6 +38,47 @@ def dhc_T_bc(xp, neum, flag):
         flag[0] = 0
 
 
+def fgmres(matvec, precond, b, tol=1e-10, maxit=100, restart=50):
+    """Right-preconditioned flexible GMRES on host vectors (Saad 1993): the operator and the preconditioner
+    (inexact multigrid solves) run on the GPU.  Returns (x, iterations, relative residual)."""
+    n = len(b)
+    x = np.zeros(n)
+    bn = np.linalg.norm(b)
+    if bn == 0.0:
+        return x, 0, 0.0
+    r = b.copy()
+    its = 0
+    while its < maxit:
+        beta = np.linalg.norm(r)
+        if beta <= tol * bn:
+            break
+        m = min(restart, maxit - its)
+        V = np.zeros((m + 1, n))
+        Z = np.zeros((m, n))
+        H = np.zeros((m + 1, m))
+        V[0] = r / beta
+        k = 0
+        for k in range(m):
+            Z[k] = precond(V[k])
+            w = matvec(Z[k])
+            for _ in range(2):                       # classical Gram-Schmidt, twice
+                h = V[:k + 1] @ w
+                w -= h @ V[:k + 1]
+                H[:k + 1, k] += h
+            H[k + 1, k] = np.linalg.norm(w)
+            its += 1
+            e1 = np.zeros(k + 2)
+            e1[0] = beta
+            y = np.linalg.lstsq(H[:k + 2, :k + 1], e1, rcond=None)[0]
+            res = np.linalg.norm(H[:k + 2, :k + 1] @ y - e1)
+            if H[k + 1, k] == 0.0 or res <= tol * bn:
+                break
+            V[k + 1] = w / H[k + 1, k]
+        x += y @ Z[:k + 1]
+        r = b - matvec(x)
+    return x, its, float(np.linalg.norm(r) / bn)
+
+
 class Boussinesq:
     def __init__(self, ctx, dim, nx0, nlev, Ra, Pr=0.71, dt=1.0, upwind_ns=0.1, upwind_T=1.0):
         self.dim = dim
@@ -135,6 +176,92 @@ class Boussinesq:
     def nusselt(self):
         """(Nu, dNu/dT, dNu/dk_e partials) at the current state, advection field frozen."""
         return self.T.nusselt(self.xT, axis=0)
+
+    def nusselt_gradient_coupled(self, tol=1e-10, maxit=100, coarse_upwind=0.1):
+        """dNu/dk_e of the steady coupled state, through the adjoint of the whole coupled system (velocity not
+        frozen): with R_NS(U, T) = 0, R_T(T, U) = 0 at the state,
+            [ J_NS^T     (dR_T/dU)^T ] [lam_U]   [dNu/dU]
+            [ (dR_NS/dT)^T    A_T^T  ] [lam_T] = [dNu/dT],   dNu/dk_e = partial - lam_T^T (dA_T/dk_e) T,
+        J_NS the Newton Jacobian of the NS residual, A_T the heat matrix at the state's velocity, dR_NS/dT = -coef M
+        (buoyancy), (dR_T/dU) the derivative of the advection.  Solved by flexible GMRES with the two transposed
+        multigrid solves as a block-diagonal preconditioner.  Both systems must be Galerkin (upwind 0), and the
+        state steady (the time terms are left out of the Jacobian).  Returns (Nu, gradient, info)."""
+        ns, T = self.ns, self.T
+        if ns.params.upwind != 0.0 or T.params.upwind != 0.0:
+            raise ValueError("the coupled adjoint needs upwind = 0 in both systems")
+        top = self.mesh.nlev - 1
+        N, dim = self.mesh.N, self.dim
+        U = ns.get_x()
+        Tv = self.xT.copy()
+        tcn, tct = ns.params.time_coeff, T.params.time_coeff
+        ns.params.time_coeff = 0.0
+        T.params.time_coeff = 0.0
+        try:
+            ns.set_xold(U)
+            ns.GenJac(top, 1)
+            up = ns.params.upwind
+            ns.params.upwind = coarse_upwind          # coarse levels: preconditioner only
+            try:
+                for L in range(top):
+                    ns.GenMatTranspose(L)
+            finally:
+                ns.params.upwind = up
+            T.set_velocity(ns.velocity_nodes(U))
+            T.set_xold(Tv)
+            for L in range(top + 1):
+                T.GenMatRhs(0.0, L, 1 if L == top else 0)
+                T.GenMatTranspose(L)
+        finally:
+            ns.params.time_coeff = tcn
+            T.params.time_coeff = tct
+        Nu, dJdT, dJdk = T.nusselt(Tv, axis=0)
+        ndU = ns.node_dof(top)
+        ndT = T.node_dof(top)[:N]
+        bc0U = ns.bc()[0]
+        nU, nT = len(U), len(Tv)
+        kappa, rhocp = T.params.alpha0, T.params.rhocp
+        ib = dim - 1                                  # the buoyant velocity component
+        coef = self.buoy[ib]
+        okT = ndT >= 0
+
+        def to_U(nodal, comp, out):
+            d = ndU[comp * N:(comp + 1) * N]
+            ok = d >= 0
+            out[d[ok]] += nodal[ok]
+
+        gU = np.zeros(nU)
+        to_U(T.coupling(0, 1.0 / kappa, T=Tv), 0, gU)
+        g = np.concatenate([gU, dJdT])
+
+        def matvec(v):
+            lU, lT = v[:nU], v[nU:]
+            yU = ns.adjoint_spmv(top, lU)
+            cT = T.coupling(1, rhocp, T=Tv, lam=lT)
+            for d in range(dim):
+                to_U(cT[d * N:(d + 1) * N], d, yU)
+            db = ndU[ib * N:(ib + 1) * N]
+            m = np.zeros(N)
+            okb = db >= 0
+            m[okb] = lU[db[okb]] * (bc0U[db[okb]] != 0)
+            Mm = T.coupling(2, -coef, m=m)
+            yT = T.adjoint_spmv(top, lT)
+            yT[ndT[okT]] += Mm[okT]
+            return np.concatenate([yU, yT])
+
+        o_ns = ns.opts(precision=F.FM_PREC_FP64_FGMRES, smoother=F.FM_SMOOTH_VANKA, vanka_colored=1, nu_pre=3,
+                       nu_post=3, coarse_mult=1, tol=1e-8, max_iter=300)
+        o_T = T.opts(precision=F.FM_PREC_FP64_FGMRES, nu_pre=3, nu_post=3, cheb_lo=0.1, tol=1e-8, max_iter=300)
+
+        def precond(r):
+            zU, _ = ns.AdjointSolve(r[:nU], o_ns)
+            zT, _ = T.AdjointSolve(r[nU:], o_T)
+            return np.concatenate([zU, zT])
+
+        lam, its, rel = fgmres(matvec, precond, g, tol=tol, maxit=maxit)
+        if not rel <= tol:
+            raise RuntimeError(f"coupled adjoint: {its} its, {rel:.2e}")
+        grad = dJdk + T.grad_coef(lam[nU:], Tv)
+        return Nu, grad, {"fgmres_its": its, "rel_resid": rel}
 
     def nusselt_gradient_k(self, opts=None):
         """Steady heat problem with the current velocity frozen (time term off): the temperature, its Nusselt

@@ -849,7 +849,91 @@ __global__ __launch_bounds__(64) void k_heat_nusselt(int nel, const int *__restr
   }
 }
 
+// ===========================================================================
+// Coupling terms of the Boussinesq system, on the heat system's top level,
+// node-space outputs (colour by colour: elements of one colour share no node):
+//   mode 0: out[j]            += scale sum_q JxW phi_j T_h                     dNu/du_axis (scale = 1/kappa)
+//   mode 1: out[d * Ntop + j] += scale sum_q JxW phi_j lam_h dT_h/dx_d         (dR_T/du)^T lam (scale = rho c_p),
+//           lam_h interpolating lam on the rows that carry the volume terms (bc_vol != 0)
+//   mode 2: out[j]            += scale sum_q JxW phi_j m_h                     M m (m nodal, Q2 mass matrix)
+// ===========================================================================
+template <int DIM>
+__global__ __launch_bounds__(64) void k_coupling(int nel, const int *__restrict__ elems, const int *__restrict__ conn,
+                                                 const double *__restrict__ xyz, int Ntop, const int *__restrict__ dof_top,
+                                                 const int *__restrict__ bc0, const double *__restrict__ T,
+                                                 const double *__restrict__ lam, const double *__restrict__ m, double scale,
+                                                 int mode, const FeTab *__restrict__ tab, double *__restrict__ out) {
+  constexpr int N2 = DIM == 2 ? 9 : 27, NG = N2;
+  __shared__ double s_xx[DIM * N2], s_T[N2], s_l[N2];
+  __shared__ double s_JxW[NG], s_a[NG], s_g[NG * DIM];
+  __shared__ int s_node[N2];
+  const int tid = threadIdx.x;
+  for (int ei = blockIdx.x; ei < nel; ei += gridDim.x) {
+    const int e = elems[ei];
+    __syncthreads();
+    if (tid < N2) {
+      const int node = conn[(int64_t)e * N2 + tid];
+      s_node[tid] = node;
+      for (int d = 0; d < DIM; d++) s_xx[tid + d * N2] = xyz[node + (int64_t)d * Ntop];
+      const int kd = dof_top[node];
+      s_T[tid] = mode == 2 ? m[node] : T[kd];
+      s_l[tid] = mode == 1 ? (bc0[kd] != 0 ? lam[kd] : 0.0) : 0.0;
+    }
+    __syncthreads();
+    if (tid < NG) {
+      const int qp = tid;
+      double IJ[DIM * DIM];
+      const double det = jac_qp<DIM, N2, NG>(qp, s_xx, tab->dphi2, IJ);
+      s_JxW[qp] = det * tab->w[qp];
+      double a = 0., lq = 0., g[DIM];
+      for (int d = 0; d < DIM; d++) g[d] = 0.;
+      for (int sn = 0; sn < N2; sn++) {
+        const double ph = tab->phi2[sn * NG + qp];
+        a += ph * s_T[sn];
+        lq += ph * s_l[sn];
+        if (mode == 1)
+          for (int d = 0; d < DIM; d++) {
+            double gd = 0.;
+            for (int j = 0; j < DIM; j++) gd += IJ[j + d * DIM] * tab->dphi2[sn * NG + qp + j * N2 * NG];
+            g[d] += gd * s_T[sn];
+          }
+      }
+      s_a[qp] = mode == 1 ? lq : a;
+      for (int d = 0; d < DIM; d++) s_g[qp * DIM + d] = g[d];
+    }
+    __syncthreads();
+    if (tid < N2) {
+      const int j = tid;
+      if (mode == 1) {
+        for (int d = 0; d < DIM; d++) {
+          double acc = 0.;
+          for (int q = 0; q < NG; q++) acc += s_JxW[q] * tab->phi2[j * NG + q] * s_a[q] * s_g[q * DIM + d];
+          out[(int64_t)d * Ntop + s_node[j]] += scale * acc;
+        }
+      } else {
+        double acc = 0.;
+        for (int q = 0; q < NG; q++) acc += s_JxW[q] * tab->phi2[j * NG + q] * s_a[q];
+        out[s_node[j]] += scale * acc;
+      }
+    }
+  }
+}
+
 namespace fmk {
+
+int coupling(hipStream_t s, int dim, int nel, const int *elems, const int *conn, const double *xyz, int Ntop,
+             const int *dof_top, const int *bc0, const double *T, const double *lam, const double *m, double scale, int mode,
+             const FeTab *tab, double *out) {
+  if (nel <= 0) return 0;
+  unsigned g = (unsigned)(nel < 8192 ? nel : 8192);
+  if (dim == 2)
+    hipLaunchKernelGGL(k_coupling<2>, dim3(g), dim3(64), 0, s, nel, elems, conn, xyz, Ntop, dof_top, bc0, T, lam, m, scale,
+                       mode, tab, out);
+  else
+    hipLaunchKernelGGL(k_coupling<3>, dim3(g), dim3(64), 0, s, nel, elems, conn, xyz, Ntop, dof_top, bc0, T, lam, m, scale,
+                       mode, tab, out);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
 
 int heat_nusselt(hipStream_t s, int dim, int nel, const int *elems, const int *conn, const int *eldof, const double *xyz,
                  int Ntop, const int *dof_top, const double *vel, const double *T, const double *coef, double kappa,

@@ -968,6 +968,27 @@ extern "C" int fm_spmv(fm_system *s, int L, const double *xh, double *yh) {
   FM_HIP(hipStreamSynchronize(st));
   return FM_OK;
 }
+// y = A^T x with the transposed matrix of the adjoint assembly, vectors in the reference DoF order
+extern "C" int fm_adjoint_spmv(fm_system *s, int L, const double *xh, double *yh) {
+  FM_CHECK(check_level(s, L));
+  if (!xh || !yh) return FM_EINVAL;
+  if (L >= (int)s->alev.size() || L >= (int)s->a_assembled.size() || !s->a_assembled[L]) {
+    fm_set_error("fm_adjoint_spmv: level not assembled transposed");
+    return FM_ESTATE;
+  }
+  Level &l = s->lev[L];
+  hipStream_t st = s->ctx->stream;
+  FM_HIP(hipSetDevice(s->ctx->device));
+  FM_HIP(hipMemcpyAsync(l.t, xh, l.n * sizeof(double), hipMemcpyHostToDevice, st));
+  DevCSR &c = s->alev[L].A;
+  FM_CHECK(fm_csr_from_sell(st, c));
+  CsrView<double> A = view<double>(c);
+  A.nsl = 0;  // CSR copy: reference numbering on both sides
+  FM_CHECK(fmk::spmv(st, A, l.t, l.d));
+  FM_HIP(hipMemcpyAsync(yh, l.d, l.n * sizeof(double), hipMemcpyDeviceToHost, st));
+  FM_HIP(hipStreamSynchronize(st));
+  return FM_OK;
+}
 extern "C" int fm_residual(fm_system *s, int L) {
   FM_CHECK(check_level(s, L));
   Level &l = s->lev[L];
@@ -1393,6 +1414,40 @@ extern "C" int fm_ns_pressure_drop(fm_system *s, const double *U, double *J, dou
 // J = sum_e int_e (u_a T / kappa - k_e dT/dx_a) with the nodal advection field
 // of fm_system_set_velocity, kappa = alpha0; dJ/dT (reference numbering) and
 // the partials dJ/dk_e = -int_e dT/dx_a.
+// Coupling products of the Boussinesq system on the heat system's top level (node-space outputs; see k_coupling):
+// mode 0: dNu/du_axis (needs T), mode 1: (dR_T/du)^T lam (T, lam; out has dim * Ntop entries), mode 2: M m (m nodal).
+extern "C" int fm_heat_coupling(fm_system *s, int mode, double scale, const double *T_host, const double *lam_host,
+                                const double *m_host, double *out_host) {
+  FM_CHECK(check_level(s, 0));
+  if (s->nq2 != 1 || s->nq1 != 0 || mode < 0 || mode > 2 || !out_host) return FM_EINVAL;
+  if ((mode != 2 && !T_host) || (mode == 1 && !lam_host) || (mode == 2 && !m_host)) return FM_EINVAL;
+  const int top = (int)s->lev.size() - 1;
+  Level &l = s->lev[top];
+  hipStream_t st = s->ctx->stream;
+  const fm_mesh &m = *s->mesh;
+  FM_HIP(hipSetDevice(s->ctx->device));
+  const int64_t nout = (int64_t)(mode == 1 ? m.dim : 1) * m.Ntop;
+  double *dm = nullptr, *dout = nullptr;
+  FM_CHECK(dalloc(&dout, nout));
+  if (mode == 2 && dalloc(&dm, m.Ntop)) { dfree(dout); return FM_ENOMEM; }
+  int rc = 0;
+  if (mode != 2 && hipMemcpyAsync(l.t, T_host, l.n * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess) rc = FM_EHIP;
+  if (!rc && mode == 1 && hipMemcpyAsync(l.d, lam_host, l.n * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess) rc = FM_EHIP;
+  if (!rc && mode == 2 && hipMemcpyAsync(dm, m_host, m.Ntop * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess) rc = FM_EHIP;
+  if (!rc) rc = fmk::fill<double>(st, nout, dout, 0.0);
+  SysExtra &ex = extra(s);
+  for (int k = 0; k < l.ncolor && !rc; k++) {
+    int n = l.color_off[k + 1] - l.color_off[k];
+    rc = fmk::coupling(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, s->xyz, m.Ntop, s->d_dof_top, s->d_bc0, l.t,
+                       l.d, dm, scale, mode, ex.d_tab, dout);
+  }
+  if (!rc && (hipMemcpyAsync(out_host, dout, nout * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+              hipStreamSynchronize(st) != hipSuccess)) rc = FM_EHIP;
+  dfree(dout);
+  dfree(dm);
+  return rc ? (rc < 0 ? rc : FM_EHIP) : FM_OK;
+}
+
 extern "C" int fm_heat_nusselt(fm_system *s, const fm_heat_params *p, int axis, const double *T_host, double *J,
                                double *dJdT_host, double *dJdk_host) {
   FM_CHECK(check_level(s, 0));

@@ -42,7 +42,7 @@ EXPORTS = [
     "fm_heat_grad_coef", "fm_bench_spmv", "fm_sync", "fm_heat_genmat_transpose", "fm_ns_genmat_transpose", "fm_ns_genjac",
     "fm_get_adjoint_csr", "fm_heat_dissipation", "fm_bench_stream", "fm_ns_dissipation", "fm_ns_grad_coef",
     "fm_system_set_bcdata", "fm_heat_bc_rhs", "fm_ns_bc_rhs", "fm_ns_pressure_drop",
-    "fm_ns_set_buoyancy", "fm_heat_nusselt",
+    "fm_ns_set_buoyancy", "fm_heat_nusselt", "fm_heat_coupling", "fm_adjoint_spmv",
 ]
 
 
@@ -142,6 +142,8 @@ def lib():
     L.fm_ns_pressure_drop.argtypes = [vp, f64p, C.POINTER(C.c_double), vp]
     L.fm_ns_set_buoyancy.argtypes = [vp, vp, vp]
     L.fm_heat_nusselt.argtypes = [vp, C.POINTER(HeatParams), C.c_int, f64p, C.POINTER(C.c_double), vp, vp]
+    L.fm_heat_coupling.argtypes = [vp, C.c_int, C.c_double, vp, vp, vp, vp]
+    L.fm_adjoint_spmv.argtypes = [vp, C.c_int, f64p, f64p]
     _LIB = L
     return L
 
@@ -426,6 +428,12 @@ class MGSolDA:
         _ck(rc, ok=(0, FM_ENOCONV))
         return lam, rep
 
+    def adjoint_spmv(self, L, x):
+        """A^T x with the transposed matrix of level L (after GenMatTranspose / GenJac(L, 1))."""
+        y = np.zeros(self._Dim[L])
+        _ck(lib().fm_adjoint_spmv(self.h, L, np.ascontiguousarray(x, np.float64), y))
+        return y
+
     def bench_spmv(self, L, prec, reps):
         s = C.c_double()
         _ck(lib().fm_bench_spmv(self.h, L, prec, reps, C.byref(s)))
@@ -446,6 +454,16 @@ class MGSolT(MGSolDA):
     # -- discrete adjoint ---------------------------------------------------------
     def GenMatTranspose(self, Level):
         _ck(lib().fm_heat_genmat_transpose(self.h, Level, C.byref(self.params)))
+
+    def coupling(self, mode, scale, T=None, lam=None, m=None):
+        """Node-space coupling products of the Boussinesq system (fm_heat_coupling)."""
+        N = self.mesh.N
+        out = np.zeros((self.mesh.dim if mode == 1 else 1) * N)
+        arr = lambda a: None if a is None else np.ascontiguousarray(a, np.float64)
+        T, lam, m = arr(T), arr(lam), arr(m)
+        ptr = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
+        _ck(lib().fm_heat_coupling(self.h, mode, scale, ptr(T), ptr(lam), ptr(m), out.ctypes.data_as(C.c_void_p)))
+        return out
 
     def nusselt(self, T, axis=0):
         """Nusselt number along `axis` of a differentially heated box with the current advection field:

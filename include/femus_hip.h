@@ -194,6 +194,9 @@ int fm_ns_genmat_transpose(fm_system *s, int L, const fm_ns_params *p);
  * transpose into the adjoint matrices (the adjoint of a functional of the steady state).  Requires
  * upwind = 0; time_coeff = 0 gives the steady Jacobian. */
 int fm_ns_genjac(fm_system *s, int L, int trans, const fm_ns_params *p);
+/* y = A^T x with the matrix of the transposed assembly of level L (reference DoF order): the products of a
+ * Krylov solve of a coupled adjoint system whose diagonal blocks are these transposed matrices. */
+int fm_adjoint_spmv(fm_system *s, int L, const double *x_host, double *y_host);
 int fm_get_adjoint_csr(const fm_system *s, int L, double *val);
 /* Transposed multigrid solve A^T lambda = g on the assembled hierarchy:
  * restriction and prolongation swap roles (Prl^T, Rst^T), smoothers and the
@@ -223,6 +226,15 @@ int fm_ns_set_buoyancy(fm_system *s, const double *T_node, const double coef[3])
  * fm_system_set_velocity), dJ/dT and the partials dJ/dk_e               */
 int fm_heat_nusselt(fm_system *s, const fm_heat_params *p, int axis, const double *T_host, double *J,
                     double *dJdT_host, double *dJdk_host);
+/* Coupling terms of the Boussinesq system (heat system s, top level; node-space outputs of Ntop entries, dim * Ntop
+ * for mode 1):  mode 0: out_j = scale sum_e int phi_j T        (dNu/du_axis with scale = 1/kappa; T_host needed)
+ *               mode 1: out_(d,j) = scale sum_e int phi_j lam dT/dx_d  ((dR_T/du)^T lam, scale = rho c_p; lam on
+ *                       the Dirichlet rows is ignored; T_host and lam_host needed)
+ *               mode 2: out_j = scale sum_e int phi_j m      (Q2 mass matrix times the nodal field m_host)
+ * The transposes of the off-diagonal blocks of the coupled Jacobian: the NS body force coef T (fm_ns_set_buoyancy)
+ * gives (dR_NS/dT)^T lam_U = -coef M (lam_U on the non-Dirichlet rows of the buoyant component). */
+int fm_heat_coupling(fm_system *s, int mode, double scale, const double *T_host, const double *lam_host,
+                     const double *m_host, double *out_host);
 /* Pressure drop J = mean p(x = lo_x) - mean p(x = hi_x) on the top level
  * (face means of the Q1 pressure) and dJ/dU (linear functional)            */
 int fm_ns_pressure_drop(fm_system *s, const double *U_host, double *J, double *dJdU_host);
