@@ -38,7 +38,7 @@ def dhc_T_bc(xp, neum, flag):
         flag[0] = 0
 
 
-def fgmres(matvec, precond, b, tol=1e-10, maxit=100, restart=50):
+def fgmres(matvec, precond, b, tol=1e-10, maxit=100, restart=50, callback=None):
     """Right-preconditioned flexible GMRES on host vectors (Saad 1993): the operator and the preconditioner
     (inexact multigrid solves) run on the GPU.  Returns (x, iterations, relative residual)."""
     n = len(b)
@@ -71,6 +71,8 @@ def fgmres(matvec, precond, b, tol=1e-10, maxit=100, restart=50):
             e1[0] = beta
             y = np.linalg.lstsq(H[:k + 2, :k + 1], e1, rcond=None)[0]
             res = np.linalg.norm(H[:k + 2, :k + 1] @ y - e1)
+            if callback:
+                callback(its, res / bn)
             if H[k + 1, k] == 0.0 or res <= tol * bn:
                 break
             V[k + 1] = w / H[k + 1, k]
@@ -80,8 +82,14 @@ def fgmres(matvec, precond, b, tol=1e-10, maxit=100, restart=50):
 
 
 class Boussinesq:
-    def __init__(self, ctx, dim, nx0, nlev, Ra, Pr=0.71, dt=1.0, upwind_ns=0.1, upwind_T=1.0):
+    def __init__(self, ctx, dim, nx0, nlev, Ra, Pr=0.71, dt=1.0, upwind_ns=0.1, upwind_T=1.0, coarse_upwind_ns=None,
+                 coarse_upwind_T=None):
+        """coarse_upwind_*: upwinding of the coarse levels of each system's multigrid (the preconditioner only;
+        None = the system's own).  With Galerkin top levels (upwind 0) an upwinded coarse operator keeps the
+        V-cycle effective where the coarse grids' mesh Peclet numbers exceed 1."""
         self.dim = dim
+        self.cu_ns = upwind_ns if coarse_upwind_ns is None else coarse_upwind_ns
+        self.cu_T = upwind_T if coarse_upwind_T is None else coarse_upwind_T
         self.mesh = F.Mesh(dim, [nx0] * dim, [0.0] * dim, [1.0] * dim, nlev)
         self.nu = np.sqrt(Pr / Ra)
         self.kappa = 1.0 / np.sqrt(Ra * Pr)
@@ -114,18 +122,41 @@ class Boussinesq:
         self.T.set_velocity(self.ns.velocity_nodes(U))
         self.xT = T.copy()
 
+    def _assemble(self, sysm, t, coarse_upwind, transpose=False):
+        """All levels of one system: the top with its rhs and own upwinding, coarse levels with coarse_upwind
+        (MGTimeStep's assembly, src/MGSolverBase.C:426-454); transpose = 1 adds the transposed matrices."""
+        top = self.mesh.nlev - 1
+        sysm.GenMatRhs(t, top, 1)
+        if transpose:
+            sysm.GenMatTranspose(top)
+        up = sysm.params.upwind
+        sysm.params.upwind = coarse_upwind
+        try:
+            for L in range(top):
+                sysm.GenMatRhs(t, L, 0)
+                if transpose:
+                    sysm.GenMatTranspose(L)
+        finally:
+            sysm.params.upwind = up
+
+    def _timestep(self, sysm, t, opts, coarse_upwind):
+        self._assemble(sysm, t, coarse_upwind)
+        rep = sysm.MGSolve(opts=opts)
+        sysm.set_xold(sysm.get_x())
+        return rep
+
     def step(self, t=0.0):
         """One pseudo-time step of each system; returns the relative changes (NS, T).  The two multigrid
         reports are kept in self.last_reps."""
         T_old = self.xT
         u_prev = self.ns.get_x()
         self.ns.set_buoyancy(self.T.node_values(T_old), self.buoy)
-        rep_ns = self.ns.MGTimeStep(t, opts=self.o_ns)
+        rep_ns = self._timestep(self.ns, t, self.o_ns, self.cu_ns)
         if rep_ns.converged != 1:
             raise RuntimeError(f"NS step: {rep_ns.iters} its, {rep_ns.rel_resid:.2e}")
         u = self.ns.get_x()
         self.T.set_velocity(self.ns.velocity_nodes(u))
-        rep_T = self.T.MGTimeStep(t, opts=self.o_T)
+        rep_T = self._timestep(self.T, t, self.o_T, self.cu_T)
         if rep_T.converged != 1:
             raise RuntimeError(f"T step: {rep_T.iters} its, {rep_T.rel_resid:.2e}")
         self.last_reps = (rep_ns, rep_T)
@@ -177,7 +208,8 @@ class Boussinesq:
         """(Nu, dNu/dT, dNu/dk_e partials) at the current state, advection field frozen."""
         return self.T.nusselt(self.xT, axis=0)
 
-    def nusselt_gradient_coupled(self, tol=1e-10, maxit=100, coarse_upwind=0.1):
+    def nusselt_gradient_coupled(self, tol=1e-10, maxit=100, inner_tol=1e-4, inner_maxit=15,
+                                 inner_precision=F.FM_PREC_FP32_FGMRES, callback=None):
         """dNu/dk_e of the steady coupled state, through the adjoint of the whole coupled system (velocity not
         frozen): with R_NS(U, T) = 0, R_T(T, U) = 0 at the state,
             [ J_NS^T     (dR_T/dU)^T ] [lam_U]   [dNu/dU]
@@ -200,7 +232,7 @@ class Boussinesq:
             ns.set_xold(U)
             ns.GenJac(top, 1)
             up = ns.params.upwind
-            ns.params.upwind = coarse_upwind          # coarse levels: preconditioner only
+            ns.params.upwind = self.cu_ns if self.cu_ns != 0.0 else 0.1   # coarse levels: preconditioner only
             try:
                 for L in range(top):
                     ns.GenMatTranspose(L)
@@ -208,9 +240,7 @@ class Boussinesq:
                 ns.params.upwind = up
             T.set_velocity(ns.velocity_nodes(U))
             T.set_xold(Tv)
-            for L in range(top + 1):
-                T.GenMatRhs(0.0, L, 1 if L == top else 0)
-                T.GenMatTranspose(L)
+            self._assemble(T, 0.0, self.cu_T, transpose=True)
         finally:
             ns.params.time_coeff = tcn
             T.params.time_coeff = tct
@@ -248,20 +278,24 @@ class Boussinesq:
             yT[ndT[okT]] += Mm[okT]
             return np.concatenate([yU, yT])
 
-        o_ns = ns.opts(precision=F.FM_PREC_FP64_FGMRES, smoother=F.FM_SMOOTH_VANKA, vanka_colored=1, nu_pre=3,
-                       nu_post=3, coarse_mult=1, tol=1e-8, max_iter=300)
-        o_T = T.opts(precision=F.FM_PREC_FP64_FGMRES, nu_pre=3, nu_post=3, cheb_lo=0.1, tol=1e-8, max_iter=300)
+        # inexact block solves, a bounded number of mixed-precision iterations each (flexible GMRES absorbs the
+        # variation; the outer residual is FP64)
+        o_ns = ns.opts(precision=inner_precision, smoother=F.FM_SMOOTH_VANKA, vanka_colored=1, nu_pre=3,
+                       nu_post=3, coarse_mult=1, tol=inner_tol, max_iter=inner_maxit)
+        o_T = T.opts(precision=inner_precision, nu_pre=3, nu_post=3, cheb_lo=0.1, tol=inner_tol, max_iter=inner_maxit)
+        inner = []
 
         def precond(r):
-            zU, _ = ns.AdjointSolve(r[:nU], o_ns)
-            zT, _ = T.AdjointSolve(r[nU:], o_T)
+            zU, rU = ns.AdjointSolve(r[:nU], o_ns)
+            zT, rT = T.AdjointSolve(r[nU:], o_T)
+            inner.append((rU.iters, rT.iters))
             return np.concatenate([zU, zT])
 
-        lam, its, rel = fgmres(matvec, precond, g, tol=tol, maxit=maxit)
+        lam, its, rel = fgmres(matvec, precond, g, tol=tol, maxit=maxit, callback=callback)
         if not rel <= tol:
             raise RuntimeError(f"coupled adjoint: {its} its, {rel:.2e}")
         grad = dJdk + T.grad_coef(lam[nU:], Tv)
-        return Nu, grad, {"fgmres_its": its, "rel_resid": rel}
+        return Nu, grad, {"fgmres_its": its, "rel_resid": rel, "inner_its": inner}
 
     def nusselt_gradient_k(self, opts=None):
         """Steady heat problem with the current velocity frozen (time term off): the temperature, its Nusselt
@@ -272,9 +306,7 @@ class Boussinesq:
         self.T.params.time_coeff = 0.0
         try:
             self.T.set_xold(self.xT)             # Dirichlet data
-            for L in range(self.mesh.nlev):
-                self.T.GenMatRhs(0.0, L, 1 if L == top else 0)
-                self.T.GenMatTranspose(L)
+            self._assemble(self.T, 0.0, self.cu_T, transpose=True)
             o = opts or self.T.opts(precision=F.FM_PREC_FP64_FGMRES, nu_pre=3, nu_post=3, cheb_lo=0.1, tol=1e-12,
                                     max_iter=200)
             self.T.set_x(self.xT)
