@@ -74,7 +74,7 @@ class OseenProblem:
 
 
 class SteadyNSProblem(OseenProblem):
-    """The steady state of the same discretisation (Galerkin: upwind 0; no time term):
+    """The steady state of the same discretisation (no time term; upwinding as set in the system's parameters):
         R(U; nu, g) = A(U; nu) U - b(g) = 0,
     the fixed point of the reference's Picard step (MGSolNS::MGTimeStep, MGSolverNS.C:722-756, iterated with
     x_old <- x).  Solved by Picard steps until the relative update is below picard_tol, then Newton steps
@@ -87,8 +87,6 @@ class SteadyNSProblem(OseenProblem):
 
     def __init__(self, ns, G, g0=None, functional="dissipation", opts=None, tol=1e-12, picard_tol=1e-3,
                  max_picard=200, max_newton=30, coarse_upwind=0.1, newton_rtol=1e-6, picard_rtol=1e-6):
-        if ns.params.upwind != 0.0:
-            raise ValueError("the steady Newton Jacobian needs upwind = 0")
         super().__init__(ns, G, g0, functional, opts)
         self.tol, self.picard_tol = tol, picard_tol
         self.max_picard, self.max_newton = max_picard, max_newton

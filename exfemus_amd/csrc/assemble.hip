@@ -313,10 +313,15 @@ __global__ void k_heat_bd(int nel, const int *__restrict__ elems, const int *__r
 //   (iv,jv): sum_q JxW (nu + f_up,iv u_iv u_jv) G_jv,i G_iv,j      (iv != jv)
 //   B^T:     -sum_q JxW psi_j G_iv,i        B: sum_q JxW psi_i G_jv,j
 // (MGSolverNS.C:364-489).  With a.newton the velocity blocks also take the
-// derivative of the advection term with respect to the advecting field,
-//   (iv,jv): sum_q JxW phi_i phi_j du_iv/dx_jv   (u = x_old),
-// which makes the matrix the Newton Jacobian of the steady residual
-// A(u) u - b at u = x_old (upwinding off).  The DIM^2 velocity blocks are split into 3x3
+// derivatives with respect to the advecting field (u = x_old) of the
+// advection term and of the upwind coefficients f_d(|u|) u_d^2, f_iv u_iv u_jv:
+//   (iv,m): sum_q JxW [ phi_i phi_s du_iv/dx_m + (sum_k G_k,i B_k^{iv,m}) phi_s ],
+//   B_k^{iv,m} = D_km du_iv/dx_k + E^{iv}_km du_k/dx_iv,
+//   D_km = f_k' (u_m/|u|) u_k^2 + 2 f_k u_k [k = m],
+//   E^{iv}_km = f_iv' (u_m/|u|) u_iv u_k + f_iv (u_k [m = iv] + u_iv [m = k]),
+// f_d = A_d L(c_d |u|) / |u|, L(P) = coth P - 1/P, so f_d' = A_d (c_d L'(P) |u| - L(P)) / |u|^2,
+// L'(P) = 1/P^2 - 1/sinh^2 P.  This makes the matrix the Newton Jacobian of the
+// steady residual A(u) u - b at u = x_old.  The DIM^2 velocity blocks are split into 3x3
 // register tiles (1x1 in 2D), 81 per block; B^T / B entries are one per
 // thread.  Velocity rows of Dirichlet DoFs are dropped (bc_vol), pressure
 // rows are not.
@@ -334,6 +339,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   __shared__ double s_JxW[NG], s_vel[NG * DIM], s_fup[NG * DIM], s_cl[NG * DIM], s_IJ[NG * DIM * DIM];
   __shared__ double s_Tn[N2], s_Tq[NG];
   __shared__ double s_du[NG * DIM * DIM];   // [q][iv][jv] = du_iv/dx_jv (Newton term only)
+  __shared__ double s_fupd[NG * DIM];        // [q][d] = d f_d / d|u| (Newton term with upwinding)
+  __shared__ double s_B[NG * DIM * DIM * DIM];  // [q][iv][m][k] (Newton term with upwinding)
   __shared__ int s_bcv[DIM * N2];
   const int tid = threadIdx.x;
   const double tcdt = a.tc / a.dt;
@@ -379,10 +386,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       mod2 = sqrt(mod2);
       for (int d = 0; d < DIM; d++) {
         const double Pe = 0.5 * mod2 * h_eff[d] / a.IRe;
-        const double fup = a.upwind * 0.5 * (1. / tanh(Pe) - 1. / Pe) * h_eff[d] / mod2;
+        const double Lp = 1. / tanh(Pe) - 1. / Pe;
+        const double fup = a.upwind * 0.5 * Lp * h_eff[d] / mod2;
         s_vel[qp * DIM + d] = vg[d];
         s_fup[qp * DIM + d] = fup;
         s_cl[qp * DIM + d] = nu + fup * vg[d] * vg[d];
+        if (a.newton) {
+          const double sh = sinh(Pe), dL = 1. / (Pe * Pe) - 1. / (sh * sh);
+          const double Ad = a.upwind * 0.5 * h_eff[d], cd = 0.5 * h_eff[d] / a.IRe;
+          s_fupd[qp * DIM + d] = Ad * (cd * dL * mod2 - Lp) / (mod2 * mod2);
+        }
       }
       for (int t = 0; t < DIM * DIM; t++) s_IJ[qp * DIM * DIM + t] = IJ[t];
     }
@@ -406,6 +419,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         s_du[t] = g;
       }
       __syncthreads();
+      if (a.upwind != 0.) {
+        for (int t = tid; t < NG * DIM * DIM * DIM; t += blockDim.x) {
+          const int q = t / (DIM * DIM * DIM), iv = (t / (DIM * DIM)) % DIM, m = (t / DIM) % DIM, k = t % DIM;
+          const double *v = s_vel + q * DIM;
+          double mod2 = 1.e-20;
+          for (int d = 0; d < DIM; d++) mod2 += v[d] * v[d];
+          mod2 = sqrt(mod2);
+          const double um = v[m] / mod2;
+          const double D = s_fupd[q * DIM + k] * um * v[k] * v[k] + (k == m ? 2. * s_fup[q * DIM + k] * v[k] : 0.);
+          const double E = s_fupd[q * DIM + iv] * um * v[iv] * v[k] +
+                           s_fup[q * DIM + iv] * ((m == iv ? v[k] : 0.) + (m == k ? v[iv] : 0.));
+          s_B[t] = D * s_du[(q * DIM + iv) * DIM + k] + E * s_du[(q * DIM + k) * DIM + iv];
+        }
+        __syncthreads();
+      }
     }
     const int64_t pbase = (int64_t)e * NL * NL;
     // velocity blocks: DIM^2 blocks x 81 tiles
@@ -468,6 +496,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
               for (int y = 0; y < TI; y++) K[x][y] += ai * s_phi[(tj + 9 * y) * NG + q];
             }
           }
+        }
+      }
+      if (a.newton && a.upwind != 0.) {
+        // derivative of the upwind coefficients: block (iv, m = jv)
+#pragma unroll 1
+        for (int q = 0; q < NG; q++) {
+          double gi[TI];
+#pragma unroll
+          for (int x = 0; x < TI; x++) gi[x] = 0.;
+#pragma unroll
+          for (int k = 0; k < DIM; k++) {
+            const double c = s_JxW[q] * s_B[((q * DIM + iv) * DIM + jv) * DIM + k];
+            const double *G = s_G + (q * DIM + k) * N2;
+#pragma unroll
+            for (int x = 0; x < TI; x++) gi[x] += c * G[ti + 9 * x];
+          }
+#pragma unroll
+          for (int x = 0; x < TI; x++)
+#pragma unroll
+            for (int y = 0; y < TI; y++) K[x][y] += gi[x] * s_phi[(tj + 9 * y) * NG + q];
         }
       }
 #pragma unroll

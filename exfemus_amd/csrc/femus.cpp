@@ -1160,13 +1160,12 @@ extern "C" int fm_ns_genmat_transpose(fm_system *s, int L, const fm_ns_params *p
 }
 
 // Newton Jacobian of the steady residual R(u) = A(u) u - b at u = x_old (the Oseen matrix of GenMatRhs plus the
-// derivative of the advection term with respect to the advecting field); trans = 1 scatters its transpose into the
-// adjoint matrices.  Upwinding must be off: its coefficient depends on |u| and is not differentiated here.
+// derivatives of the advection term and of the upwind coefficients with respect to the advecting field); trans = 1
+// scatters its transpose into the adjoint matrices.
 extern "C" int fm_ns_genjac(fm_system *s, int L, int trans, const fm_ns_params *p) {
   FM_CHECK(check_level(s, L));
   const fm_mesh &m = *s->mesh;
   if (!p || s->nq2 != m.dim || s->nq1 != 1 || p->dt == 0. || p->IRe <= 0. || (trans != 0 && trans != 1)) return FM_EINVAL;
-  if (p->upwind != 0.) { fm_set_error("fm_ns_genjac: the Newton Jacobian needs upwind = 0"); return FM_EINVAL; }
   FM_HIP(hipSetDevice(s->ctx->device));
   if (trans) FM_CHECK(ensure_adjoint(s));
   Level &l = s->lev[L];
@@ -1174,7 +1173,7 @@ extern "C" int fm_ns_genjac(fm_system *s, int L, int trans, const fm_ns_params *
   const int64_t nslot = trans ? s->alev[L].A.sell.nslot : l.A.sell.nslot;
   hipStream_t st = s->ctx->stream;
   NsArgs a;
-  a.IRe = p->IRe; a.dt = p->dt; a.tc = p->time_coeff; a.upwind = 0.; a.IFr = p->IFr;
+  a.IRe = p->IRe; a.dt = p->dt; a.tc = p->time_coeff; a.upwind = p->upwind; a.IFr = p->IFr;
   for (int d = 0; d < 3; d++) { a.dirg[d] = p->dirg[d]; a.src[d] = p->src[d]; a.buoy[d] = 0.; }
   a.Tn = nullptr;
   a.newton = 1;

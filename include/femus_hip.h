@@ -188,11 +188,11 @@ int fm_heat_genmat_transpose(fm_system *s, int L, const fm_heat_params *p);
 int fm_ns_genmat_transpose(fm_system *s, int L, const fm_ns_params *p);
 /* values of the assembled A[L]^T (pattern of A[L]) for checks              */
 /* Newton Jacobian of the steady Navier-Stokes residual R(U) = A(U) U - b at U = x_old: the GenMatRhs matrix
- * plus the derivative of the advection term (u_old . grad) u with respect to u_old, i.e. the velocity blocks
- * gain int phi_i phi_j du_iv/dx_jv (MGSolverNS.C:364-435 differentiated; the reference itself only has the
- * Picard matrix).  trans = 0: into the level matrix (a Newton step solves J dU = b - A(U) U); trans = 1: its
- * transpose into the adjoint matrices (the adjoint of a functional of the steady state).  Requires
- * upwind = 0; time_coeff = 0 gives the steady Jacobian. */
+ * plus the derivatives with respect to u_old of the advection term (u_old . grad) u (velocity blocks gain
+ * int phi_i phi_j du_iv/dx_jv) and of the upwind coefficients f_upwind(|u_old|) u_old,k u_old,l
+ * (MGSolverNS.C:330-435 differentiated; the reference itself only has the Picard matrix).  trans = 0: into the
+ * level matrix (a Newton step solves J dU = b - A(U) U); trans = 1: its transpose into the adjoint matrices
+ * (the adjoint of a functional of the steady state).  time_coeff = 0 gives the steady Jacobian. */
 int fm_ns_genjac(fm_system *s, int L, int trans, const fm_ns_params *p);
 /* y = A^T x with the matrix of the transposed assembly of level L (reference DoF order): the products of a
  * Krylov solve of a coupled adjoint system whose diagonal blocks are these transposed matrices. */

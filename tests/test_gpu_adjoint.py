@@ -370,7 +370,7 @@ def test_cavity_gradient_vs_cpu_reference(femus, oracle, ctx):
 # Steady Navier-Stokes (Galerkin, upwind 0): Newton Jacobian of R(U) = A(U) U - b and the adjoint of a
 # functional of the steady state.
 # ---------------------------------------------------------------------------
-def _newton_check(femus, oracle, ctx, case):
+def _newton_check(femus, oracle, ctx, case, upwind=0.0):
     """Returns (GPU Newton Jacobian, its transpose from the adjoint assembly, oracle directional derivative
     function, U, pattern)."""
     if case == "cavity2d":
@@ -380,7 +380,7 @@ def _newton_check(femus, oracle, ctx, case):
         dim, nx, nlev, lo, hi = 3, [2, 1, 1], 2, [0, 0, 0], [4, 1, 1]
         bc = configs.channel_bc
     m = femus.Mesh(dim, nx, lo, hi, nlev)
-    ns = femus.MGSolNS(ctx, m, IRe=0.01, dt=0.1, time_coeff=0.0, upwind=0.0)
+    ns = femus.MGSolNS(ctx, m, IRe=0.01, dt=0.1, time_coeff=0.0, upwind=upwind)
     ns.GenBc(bc)
     ns.init()
     top = nlev - 1
@@ -398,35 +398,36 @@ def _newton_check(femus, oracle, ctx, case):
     d, n = oracle.init_dof(om, top, dim, 1)
     b0, b1 = oracle.genbc(om, dim, 1, d, bc)
     orp, oci = oracle.system_pattern(om, top, dim, 1, d)
-    p = oracle.NSParams(0.01, 0.1, 0.0, 0.0, 0.0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0))
+    p = oracle.NSParams(0.01, 0.1, 0.0, upwind, 0.0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0))
 
-    def F(W):          # A_oracle(W) W: quadratic in W
+    def F(W):          # A_oracle(W) W
         v, _ = oracle.ns_genmatrhs(om, top, 1, p, d, d, b0, b1, W, orp, oci, nu_elem=nu)
         return sp.csr_matrix((v, oci, orp), shape=(n, n)) @ W
     return Jg, JgT, F, U
 
 
-@pytest.mark.parametrize("case", ["cavity2d", "channel3d"])
-def test_newton_jacobian_vs_oracle(femus, oracle, ctx, case):
-    """J_N delta equals the directional derivative of the oracle's A(W) W (central differences are exact for
-    a quadratic map), and the transposed Newton assembly is exactly its transpose."""
-    Jg, JgT, F, U = _newton_check(femus, oracle, ctx, case)
+@pytest.mark.parametrize("case,upwind", [("cavity2d", 0.0), ("channel3d", 0.0), ("cavity2d", 0.1), ("channel3d", 0.1)])
+def test_newton_jacobian_vs_oracle(femus, oracle, ctx, case, upwind):
+    """J_N delta equals the directional derivative of the oracle's A(W) W: without upwinding the map is
+    quadratic and central differences are exact; with the reference's upwinding (0.1, coth formula) they are
+    second-order accurate (eps = 1e-5).  The transposed Newton assembly is exactly its transpose."""
+    Jg, JgT, F, U = _newton_check(femus, oracle, ctx, case, upwind)
     assert (JgT != Jg.T).nnz == 0
     rng = np.random.default_rng(5)
     for _ in range(3):
         dl = rng.standard_normal(len(U))
-        eps = 1e-3
+        eps = 1e-3 if upwind == 0.0 else 1e-5
         fd = (F(U + eps * dl) - F(U - eps * dl)) / (2 * eps)
         jd = Jg @ dl
-        assert np.linalg.norm(jd - fd) <= 1e-10 * np.linalg.norm(fd)
+        assert np.linalg.norm(jd - fd) <= (1e-10 if upwind == 0.0 else 1e-8) * np.linalg.norm(fd)
     # the Newton term is really there: J_N differs from the Picard matrix by more than rounding
     assert np.linalg.norm(Jg @ U - F(U)) > 1e-3 * np.linalg.norm(F(U))
 
 
-def _steady_cavity(femus, ctx, nx=(8, 8), nlev=3, IRe=0.0025, K=16, sigma=0.5):
+def _steady_cavity(femus, ctx, nx=(8, 8), nlev=3, IRe=0.0025, K=16, sigma=0.5, upwind=0.0):
     from exfemus_amd import adjoint
     m = femus.Mesh(2, list(nx), [0, 0], [1, 1], nlev)
-    ns = femus.MGSolNS(ctx, m, IRe=IRe, dt=0.1, time_coeff=0.0, upwind=0.0)
+    ns = femus.MGSolNS(ctx, m, IRe=IRe, dt=0.1, time_coeff=0.0, upwind=upwind)
     ns.GenBc(functools.partial(configs.cavity_bc, dim=2))
     ns.init()
     G = adjoint.lid_fourier_basis(ns, K)
@@ -441,10 +442,12 @@ def _steady_cavity(femus, ctx, nx=(8, 8), nlev=3, IRe=0.0025, K=16, sigma=0.5):
     return m, ns, G, a0, nu, prob
 
 
-def test_steady_adjoint_gradient_fd(femus, ctx):
-    """Steady cavity (Re ~ 400, log-normal viscosity): dJ/dnu_e and dJ/da_k from the transposed Newton
-    Jacobian against central differences of the steady state (h = 1e-6, Newton from the base state)."""
-    m, ns, G, a0, nu, prob = _steady_cavity(femus, ctx)
+@pytest.mark.parametrize("upwind", [0.0, 0.1])
+def test_steady_adjoint_gradient_fd(femus, ctx, upwind):
+    """Steady cavity (Re ~ 400, log-normal viscosity; Galerkin and the reference's upwinding 0.1): dJ/dnu_e and
+    dJ/da_k from the transposed Newton Jacobian against central differences of the steady state (h = 1e-6,
+    Newton from the base state)."""
+    m, ns, G, a0, nu, prob = _steady_cavity(femus, ctx, upwind=upwind)
     J0 = prob.solve(nu, a0)
     print("steady state:", prob.stats, flush=True)
     assert prob.stats["rel_residual"] < 1e-10, prob.stats
