@@ -216,11 +216,10 @@ class Boussinesq:
             [ (dR_NS/dT)^T    A_T^T  ] [lam_T] = [dNu/dT],   dNu/dk_e = partial - lam_T^T (dA_T/dk_e) T,
         J_NS the Newton Jacobian of the NS residual, A_T the heat matrix at the state's velocity, dR_NS/dT = -coef M
         (buoyancy), (dR_T/dU) the derivative of the advection.  Solved by flexible GMRES with the two transposed
-        multigrid solves as a block-diagonal preconditioner.  Both systems must be Galerkin (upwind 0), and the
-        state steady (the time terms are left out of the Jacobian).  Returns (Nu, gradient, info)."""
+        multigrid solves as a block-diagonal preconditioner.  The upwind terms of both systems are differentiated
+        (fm_ns_genjac, fm_heat_coupling); the state must be steady (the time terms are left out of the Jacobian).
+        Returns (Nu, gradient, info)."""
         ns, T = self.ns, self.T
-        if ns.params.upwind != 0.0 or T.params.upwind != 0.0:
-            raise ValueError("the coupled adjoint needs upwind = 0 in both systems")
         top = self.mesh.nlev - 1
         N, dim = self.mesh.N, self.dim
         U = ns.get_x()

@@ -901,8 +901,11 @@ __global__ __launch_bounds__(64) void k_heat_nusselt(int nel, const int *__restr
 // Coupling terms of the Boussinesq system, on the heat system's top level,
 // node-space outputs (colour by colour: elements of one colour share no node):
 //   mode 0: out[j]            += scale sum_q JxW phi_j T_h                     dNu/du_axis (scale = 1/kappa)
-//   mode 1: out[d * Ntop + j] += scale sum_q JxW phi_j lam_h dT_h/dx_d         (dR_T/du)^T lam (scale = rho c_p),
-//           lam_h interpolating lam on the rows that carry the volume terms (bc_vol != 0)
+//   mode 1: out[m * Ntop + j] += sum_q JxW phi_j (scale lam_h dT_h/dx_m + sum_d D_dm dlam_h/dx_d dT_h/dx_d)
+//           = (dR_T/du)^T lam (scale = rho c_p; lam_h interpolating lam on the rows that carry the volume
+//           terms, bc_vol != 0); D_dm = f_d' (u_m/|u|) u_d^2 + 2 f_d u_d [d = m] is the derivative of the upwind
+//           diffusion f_d(|u|) u_d^2 of MGSolverT.C:290-300 (f_d = upwind rho c_p 0.5 L(Pe_d) h_d / |u|,
+//           Pe_d = 0.5 |u| h_d / alpha0, L(P) = coth P - 1/P)
 //   mode 2: out[j]            += scale sum_q JxW phi_j m_h                     M m (m nodal, Q2 mass matrix)
 // ===========================================================================
 template <int DIM>
@@ -910,10 +913,11 @@ __global__ __launch_bounds__(64) void k_coupling(int nel, const int *__restrict_
                                                  const double *__restrict__ xyz, int Ntop, const int *__restrict__ dof_top,
                                                  const int *__restrict__ bc0, const double *__restrict__ T,
                                                  const double *__restrict__ lam, const double *__restrict__ m, double scale,
-                                                 int mode, const FeTab *__restrict__ tab, double *__restrict__ out) {
+                                                 int mode, const double *__restrict__ vel, double upwind, double rhocp,
+                                                 double alpha0, const FeTab *__restrict__ tab, double *__restrict__ out) {
   constexpr int N2 = DIM == 2 ? 9 : 27, NG = N2;
-  __shared__ double s_xx[DIM * N2], s_T[N2], s_l[N2];
-  __shared__ double s_JxW[NG], s_a[NG], s_g[NG * DIM];
+  __shared__ double s_xx[DIM * N2], s_T[N2], s_l[N2], s_u[DIM * N2];
+  __shared__ double s_JxW[NG], s_a[NG], s_g[NG * DIM], s_c[NG * DIM];
   __shared__ int s_node[N2];
   const int tid = threadIdx.x;
   for (int ei = blockIdx.x; ei < nel; ei += gridDim.x) {
@@ -926,6 +930,7 @@ __global__ __launch_bounds__(64) void k_coupling(int nel, const int *__restrict_
       const int kd = dof_top[node];
       s_T[tid] = mode == 2 ? m[node] : T[kd];
       s_l[tid] = mode == 1 ? (bc0[kd] != 0 ? lam[kd] : 0.0) : 0.0;
+      for (int d = 0; d < DIM; d++) s_u[tid + d * N2] = (mode == 1 && vel) ? vel[node + (int64_t)d * Ntop] : 0.0;
     }
     __syncthreads();
     if (tid < NG) {
@@ -933,8 +938,8 @@ __global__ __launch_bounds__(64) void k_coupling(int nel, const int *__restrict_
       double IJ[DIM * DIM];
       const double det = jac_qp<DIM, N2, NG>(qp, s_xx, tab->dphi2, IJ);
       s_JxW[qp] = det * tab->w[qp];
-      double a = 0., lq = 0., g[DIM];
-      for (int d = 0; d < DIM; d++) g[d] = 0.;
+      double a = 0., lq = 0., g[DIM], gl[DIM], v[DIM];
+      for (int d = 0; d < DIM; d++) { g[d] = 0.; gl[d] = 0.; v[d] = 0.; }
       for (int sn = 0; sn < N2; sn++) {
         const double ph = tab->phi2[sn * NG + qp];
         a += ph * s_T[sn];
@@ -944,10 +949,36 @@ __global__ __launch_bounds__(64) void k_coupling(int nel, const int *__restrict_
             double gd = 0.;
             for (int j = 0; j < DIM; j++) gd += IJ[j + d * DIM] * tab->dphi2[sn * NG + qp + j * N2 * NG];
             g[d] += gd * s_T[sn];
+            gl[d] += gd * s_l[sn];
+            v[d] += ph * s_u[sn + d * N2];
           }
       }
       s_a[qp] = mode == 1 ? lq : a;
       for (int d = 0; d < DIM; d++) s_g[qp * DIM + d] = g[d];
+      if (mode == 1) {
+        // derivative of the upwind diffusion: c_m = sum_d D_dm dlam/dx_d dT/dx_d
+        double mod2 = 1.e-20;
+        for (int d = 0; d < DIM; d++) mod2 += v[d] * v[d];
+        mod2 = sqrt(mod2);
+        double f[DIM], fd[DIM];
+        for (int d = 0; d < DIM; d++) {
+          const double h = fabs(s_xx[d * N2 + 2 + 4 * (DIM - 2)] - s_xx[d * N2 + 0]);
+          const double h1 = fabs(s_xx[d * N2 + 3 + 4 * (DIM - 2)] - s_xx[d * N2 + 1]);
+          const double he = h < h1 ? h1 : h;
+          const double Pe = 0.5 * mod2 * he / alpha0, Lp = 1. / tanh(Pe) - 1. / Pe, sh = sinh(Pe);
+          const double Ad = upwind * rhocp * 0.5 * he, cd = 0.5 * he / alpha0;
+          f[d] = Ad * Lp / mod2;
+          fd[d] = Ad * (cd * (1. / (Pe * Pe) - 1. / (sh * sh)) * mod2 - Lp) / (mod2 * mod2);
+        }
+        for (int mm = 0; mm < DIM; mm++) {
+          double c = 0.;
+          for (int d = 0; d < DIM; d++) {
+            const double D = fd[d] * (v[mm] / mod2) * v[d] * v[d] + (d == mm ? 2. * f[d] * v[d] : 0.);
+            c += D * gl[d] * g[d];
+          }
+          s_c[qp * DIM + mm] = upwind != 0. ? c : 0.;
+        }
+      }
     }
     __syncthreads();
     if (tid < N2) {
@@ -955,8 +986,9 @@ __global__ __launch_bounds__(64) void k_coupling(int nel, const int *__restrict_
       if (mode == 1) {
         for (int d = 0; d < DIM; d++) {
           double acc = 0.;
-          for (int q = 0; q < NG; q++) acc += s_JxW[q] * tab->phi2[j * NG + q] * s_a[q] * s_g[q * DIM + d];
-          out[(int64_t)d * Ntop + s_node[j]] += scale * acc;
+          for (int q = 0; q < NG; q++)
+            acc += s_JxW[q] * tab->phi2[j * NG + q] * (scale * s_a[q] * s_g[q * DIM + d] + s_c[q * DIM + d]);
+          out[(int64_t)d * Ntop + s_node[j]] += acc;
         }
       } else {
         double acc = 0.;
@@ -971,15 +1003,15 @@ namespace fmk {
 
 int coupling(hipStream_t s, int dim, int nel, const int *elems, const int *conn, const double *xyz, int Ntop,
              const int *dof_top, const int *bc0, const double *T, const double *lam, const double *m, double scale, int mode,
-             const FeTab *tab, double *out) {
+             const double *vel, double upwind, double rhocp, double alpha0, const FeTab *tab, double *out) {
   if (nel <= 0) return 0;
   unsigned g = (unsigned)(nel < 8192 ? nel : 8192);
   if (dim == 2)
     hipLaunchKernelGGL(k_coupling<2>, dim3(g), dim3(64), 0, s, nel, elems, conn, xyz, Ntop, dof_top, bc0, T, lam, m, scale,
-                       mode, tab, out);
+                       mode, vel, upwind, rhocp, alpha0, tab, out);
   else
     hipLaunchKernelGGL(k_coupling<3>, dim3(g), dim3(64), 0, s, nel, elems, conn, xyz, Ntop, dof_top, bc0, T, lam, m, scale,
-                       mode, tab, out);
+                       mode, vel, upwind, rhocp, alpha0, tab, out);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

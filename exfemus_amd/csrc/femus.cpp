@@ -1415,10 +1415,11 @@ extern "C" int fm_ns_pressure_drop(fm_system *s, const double *U, double *J, dou
 // the partials dJ/dk_e = -int_e dT/dx_a.
 // Coupling products of the Boussinesq system on the heat system's top level (node-space outputs; see k_coupling):
 // mode 0: dNu/du_axis (needs T), mode 1: (dR_T/du)^T lam (T, lam; out has dim * Ntop entries), mode 2: M m (m nodal).
-extern "C" int fm_heat_coupling(fm_system *s, int mode, double scale, const double *T_host, const double *lam_host,
-                                const double *m_host, double *out_host) {
+extern "C" int fm_heat_coupling(fm_system *s, const fm_heat_params *p, int mode, double scale, const double *T_host,
+                                const double *lam_host, const double *m_host, double *out_host) {
   FM_CHECK(check_level(s, 0));
-  if (s->nq2 != 1 || s->nq1 != 0 || mode < 0 || mode > 2 || !out_host) return FM_EINVAL;
+  if (!p || s->nq2 != 1 || s->nq1 != 0 || mode < 0 || mode > 2 || !out_host || !(p->alpha0 > 0.)) return FM_EINVAL;
+  if (mode == 1 && p->upwind != 0. && !s->vel) { fm_set_error("fm_heat_coupling: upwind term needs the velocity"); return FM_ESTATE; }
   if ((mode != 2 && !T_host) || (mode == 1 && !lam_host) || (mode == 2 && !m_host)) return FM_EINVAL;
   const int top = (int)s->lev.size() - 1;
   Level &l = s->lev[top];
@@ -1438,7 +1439,7 @@ extern "C" int fm_heat_coupling(fm_system *s, int mode, double scale, const doub
   for (int k = 0; k < l.ncolor && !rc; k++) {
     int n = l.color_off[k + 1] - l.color_off[k];
     rc = fmk::coupling(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, s->xyz, m.Ntop, s->d_dof_top, s->d_bc0, l.t,
-                       l.d, dm, scale, mode, ex.d_tab, dout);
+                       l.d, dm, scale, mode, s->vel, p->upwind, p->rhocp, p->alpha0, ex.d_tab, dout);
   }
   if (!rc && (hipMemcpyAsync(out_host, dout, nout * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
               hipStreamSynchronize(st) != hipSuccess)) rc = FM_EHIP;

@@ -93,7 +93,7 @@ int ns_assemble(hipStream_t s, int dim, int nel_color, const int *elems, const i
                 const FeTab *tab, NsArgs a, int mode, int trans, int bd_only, double *val, double *rhs);
 int coupling(hipStream_t s, int dim, int nel, const int *elems, const int *conn, const double *xyz, int Ntop,
              const int *dof_top, const int *bc0, const double *T, const double *lam, const double *m, double scale, int mode,
-             const FeTab *tab, double *out);
+             const double *vel, double upwind, double rhocp, double alpha0, const FeTab *tab, double *out);
 int heat_nusselt(hipStream_t s, int dim, int nel, const int *elems, const int *conn, const int *eldof, const double *xyz,
                  int Ntop, const int *dof_top, const double *vel, const double *T, const double *coef, double kappa,
                  int axis, const FeTab *tab, double *Jel, double *dJdk, double *y);

@@ -142,7 +142,7 @@ def lib():
     L.fm_ns_pressure_drop.argtypes = [vp, f64p, C.POINTER(C.c_double), vp]
     L.fm_ns_set_buoyancy.argtypes = [vp, vp, vp]
     L.fm_heat_nusselt.argtypes = [vp, C.POINTER(HeatParams), C.c_int, f64p, C.POINTER(C.c_double), vp, vp]
-    L.fm_heat_coupling.argtypes = [vp, C.c_int, C.c_double, vp, vp, vp, vp]
+    L.fm_heat_coupling.argtypes = [vp, C.POINTER(HeatParams), C.c_int, C.c_double, vp, vp, vp, vp]
     L.fm_adjoint_spmv.argtypes = [vp, C.c_int, f64p, f64p]
     _LIB = L
     return L
@@ -462,7 +462,8 @@ class MGSolT(MGSolDA):
         arr = lambda a: None if a is None else np.ascontiguousarray(a, np.float64)
         T, lam, m = arr(T), arr(lam), arr(m)
         ptr = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
-        _ck(lib().fm_heat_coupling(self.h, mode, scale, ptr(T), ptr(lam), ptr(m), out.ctypes.data_as(C.c_void_p)))
+        _ck(lib().fm_heat_coupling(self.h, C.byref(self.params), mode, scale, ptr(T), ptr(lam), ptr(m),
+                                   out.ctypes.data_as(C.c_void_p)))
         return out
 
     def nusselt(self, T, axis=0):

@@ -226,15 +226,17 @@ int fm_ns_set_buoyancy(fm_system *s, const double *T_node, const double coef[3])
  * fm_system_set_velocity), dJ/dT and the partials dJ/dk_e               */
 int fm_heat_nusselt(fm_system *s, const fm_heat_params *p, int axis, const double *T_host, double *J,
                     double *dJdT_host, double *dJdk_host);
-/* Coupling terms of the Boussinesq system (heat system s, top level; node-space outputs of Ntop entries, dim * Ntop
- * for mode 1):  mode 0: out_j = scale sum_e int phi_j T        (dNu/du_axis with scale = 1/kappa; T_host needed)
- *               mode 1: out_(d,j) = scale sum_e int phi_j lam dT/dx_d  ((dR_T/du)^T lam, scale = rho c_p; lam on
- *                       the Dirichlet rows is ignored; T_host and lam_host needed)
- *               mode 2: out_j = scale sum_e int phi_j m      (Q2 mass matrix times the nodal field m_host)
+/* Coupling terms of the Boussinesq system (heat system s with parameters p, top level; node-space outputs of
+ * Ntop entries, dim * Ntop for mode 1):
+ *   mode 0: out_j = scale sum_e int phi_j T                 (dNu/du_axis with scale = 1/kappa; T_host needed)
+ *   mode 1: out_(m,j) = (dR_T/du_(m,j))^T lam: scale sum_e int phi_j lam dT/dx_m (advection, scale = rho c_p) plus
+ *           the derivative of the upwind diffusion f_upwind(|u|) u_d^2 (MGSolverT.C:290-300; p->upwind, p->rhocp,
+ *           p->alpha0, velocity of fm_system_set_velocity); lam on the Dirichlet rows is ignored; T_host, lam_host
+ *   mode 2: out_j = scale sum_e int phi_j m                 (Q2 mass matrix times the nodal field m_host)
  * The transposes of the off-diagonal blocks of the coupled Jacobian: the NS body force coef T (fm_ns_set_buoyancy)
  * gives (dR_NS/dT)^T lam_U = -coef M (lam_U on the non-Dirichlet rows of the buoyant component). */
-int fm_heat_coupling(fm_system *s, int mode, double scale, const double *T_host, const double *lam_host,
-                     const double *m_host, double *out_host);
+int fm_heat_coupling(fm_system *s, const fm_heat_params *p, int mode, double scale, const double *T_host,
+                     const double *lam_host, const double *m_host, double *out_host);
 /* Pressure drop J = mean p(x = lo_x) - mean p(x = hi_x) on the top level
  * (face means of the Q1 pressure) and dJ/dU (linear functional)            */
 int fm_ns_pressure_drop(fm_system *s, const double *U_host, double *J, double *dJdU_host);

@@ -147,12 +147,14 @@ def _steady_direct(b, U, Tv, tol=1e-15, maxit=200):
     return U, Tv, it + 1
 
 
-def test_coupled_nusselt_adjoint_fd(femus, ctx):
-    """dNu/dk_e through the adjoint of the whole steady Boussinesq system (Galerkin, Ra 1e4, 16^2) against
-    central differences of fully re-converged coupled states (h = 1e-6 relative) within 1e-5; the frozen-velocity
-    gradient differs from it (the coupling matters)."""
+@pytest.mark.parametrize("up_ns,up_T", [(0.0, 0.0), (0.1, 1.0)])
+def test_coupled_nusselt_adjoint_fd(femus, ctx, up_ns, up_T):
+    """dNu/dk_e through the adjoint of the whole steady Boussinesq system (Ra 1e4, 16^2; Galerkin and the
+    reference's upwinding, whose coefficients depend on the velocity) against central differences of fully
+    re-converged coupled states (h = 1e-6 relative) within 1e-5; the frozen-velocity gradient differs from it
+    (the coupling matters)."""
     from exfemus_amd import configs, coupled
-    b = coupled.Boussinesq(ctx, 2, 4, 3, 1e4, dt=5.0, upwind_ns=0.0, upwind_T=0.0)
+    b = coupled.Boussinesq(ctx, 2, 4, 3, 1e4, dt=5.0, upwind_ns=up_ns, upwind_T=up_T)
     ne = b.mesh.level(2)[0]
     k = configs.lognormal_k(ne, seed=5, sigma=0.3)
 
