@@ -140,8 +140,9 @@ def adjoint_workload(F, ctx, args):
     """C4: 2D lid-driven cavity 128^2 quad9 Q2-Q1 (16^2 coarse, 4 levels), Re 400: per-element viscosity
     log-normal around 0.0025 (sigma 1, seeded) and 16 Fourier coefficients of the lid profile
     u(x) = sum_k a_k sin(k pi x) (a_1 = 1 + noise, a_k ~ N(0,1)/k); J = viscous dissipation.
-    --adjoint-problem steady (default): J of the steady state R(U) = A(U) U - b = 0 (Galerkin), solved by
-    Picard then Newton; the adjoint uses the transposed Newton Jacobian.  oseen: J after one Oseen step with the
+    Upwinding 0.1 (the reference's UP_WIND_NS).  --adjoint-problem steady (default): J of the steady state
+    R(U) = A(U) U - b = 0, solved by Picard then Newton; the adjoint uses the transposed Newton Jacobian (upwind
+    coefficients differentiated).  oseen: J after one Oseen step with the
     advection frozen at the lid profile.  Times the forward solve, the transposed multigrid solve and the
     gradient assembly; checks the most sensitive components against central differences (h = 1e-6)."""
     import functools
@@ -150,7 +151,7 @@ def adjoint_workload(F, ctx, args):
     from exfemus_amd import adjoint, configs
     steady = args.adjoint_problem == "steady"
     m = F.Mesh(2, [16, 16], [0, 0], [1, 1], 4)
-    ns = F.MGSolNS(ctx, m, IRe=0.0025, dt=0.1, time_coeff=0.0, upwind=0.0 if steady else 0.1)
+    ns = F.MGSolNS(ctx, m, IRe=0.0025, dt=0.1, time_coeff=0.0, upwind=0.1)
     ns.GenBc(functools.partial(configs.cavity_bc, dim=2))
     ns.init()
     K = 16
@@ -219,7 +220,7 @@ def adjoint_workload(F, ctx, args):
     prob.solve(nu, a0, U0=U0) if steady else None
     t_total = float(np.median(t_fwd) + np.median(t_adj))
     N = ns._Dim[top]
-    what = ("the steady state (Galerkin, Picard + Newton; adjoint with the transposed Newton Jacobian)" if steady
+    what = ("the steady state (upwind 0.1, Picard + Newton; adjoint with the transposed Newton Jacobian)" if steady
             else "one Oseen step (advection frozen at the lid profile)")
     out = {"metric": "adjoint gradient of the dissipation w.r.t. 16384 element viscosities + 16 lid coefficients",
            "value": round(t_total * 1e3, 3), "unit": "ms per gradient (forward solve + adjoint solve + gradient)",
