@@ -390,8 +390,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="launch V-cycles eagerly instead of replaying HIP graphs")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on exactly one GPU; multi-GPU sharding is described in "
-                                   "DESIGN.md and not implemented"}))
+        print(json.dumps({"error": "exfemus_amd runs on exactly one GPU: multi-GPU sharding (subdomain halos "
+                                   "over RCCL) is not implemented"}))
         sys.exit(2)
     from exfemus_amd import femus as F
     ctx = F.Context(0)

@@ -142,7 +142,7 @@ def test_sparsity_blocks_bruteforce(oracle):
 
 
 def test_system_pattern_counts(oracle):
-    # C1 (2D cavity 32^2, 3 levels) top rows 9,539 and nnz 377,289
+    # 2D lid-driven cavity 32^2, 3 levels (Q2-Q1): 9,539 top rows and 377,289 nonzeros
     m = oracle.Mesh(2, [8, 8], [0, 0], [1, 1], 3)
     d, n = oracle.init_dof(m, 2, 2, 1)
     rp, ci = oracle.system_pattern(m, 2, 2, 1, d)
