@@ -435,14 +435,21 @@ static int solve_fgmres(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &op
   int total = 0;
   double rel = 1.;
   unsigned gv = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
+  bool res_fresh = false;  // V holds b - A x of the current x (the exit residual needs no product)
+  bool x_zero = dev_dot<double>(c, n, l->x, l->x, true) == 0.;
   if (!(bn > 0.)) { rel = 0.; goto done; }
   while (total < o->max_iter) {
-    if ((rc = fmk::resid(st, A, l->b, l->x, V)) != 0) goto done;
+    if (x_zero) {   // r = b without a product
+      x_zero = false;
+      if (hipMemcpyAsync(V, l->b, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
+    } else if ((rc = fmk::resid(st, A, l->b, l->x, V)) != 0) {
+      goto done;
+    }
     {
       double beta = dev_dot<double>(c, n, V, V, true);
       rel = beta / bn;
       if (total == 0) rep->hist[0] = rel;
-      if (rel <= o->tol || !std::isfinite(rel)) break;
+      if (rel <= o->tol || !std::isfinite(rel)) { res_fresh = true; break; }
       if ((rc = fmk::axpby<double>(st, n, 1.0 / beta, V, 0.0, V)) != 0) goto done;
       std::fill(g.begin(), g.end(), 0.);
       g[0] = beta;
@@ -510,9 +517,13 @@ static int solve_fgmres(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &op
       hipLaunchKernelGGL(k_multi_add, dim3(gv), dim3(256), 0, st, n, j, Z, hd + 192, l->x);
     }
   }
-  // true FP64 residual at exit
-  if ((rc = fmk::resid(st, A, l->b, l->x, l->res)) != 0) goto done;
-  rel = dev_dot<double>(c, n, l->res, l->res, true) / bn;
+  // true FP64 residual at exit (already in V when the convergence check at the loop top ended the solve)
+  if (res_fresh) {
+    if (hipMemcpyAsync(l->res, V, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
+  } else {
+    if ((rc = fmk::resid(st, A, l->b, l->x, l->res)) != 0) goto done;
+    rel = dev_dot<double>(c, n, l->res, l->res, true) / bn;
+  }
 done:
   (void)hipStreamSynchronize(st);
   (void)hipFree(V); (void)hipFree(Z); (void)hipFree(hd);
