@@ -16,6 +16,7 @@
 // test can turn failures into exceptions.  Header only; link libfemus_hip.so.
 #ifndef FEMUS_HIP_HPP
 #define FEMUS_HIP_HPP
+#include <algorithm>
 #include <cstdlib>
 #include <functional>
 #include <iostream>
@@ -160,6 +161,11 @@ class MGSolDA {
     x_old = get_x();
   }
 
+  // x_old of the top level (advection field and Dirichlet data of the next assembly)
+  void set_xold(const std::vector<double> &v) {
+    x_old = v;
+    check(fm_system_set_xold(h_, x_old.data()), "fm_system_set_xold");
+  }
   std::vector<double> get_x(int L = -1) const {
     if (L < 0) L = _NoLevels - 1;
     std::vector<double> v(_Dim[L]);
@@ -178,6 +184,28 @@ class MGSolDA {
   }
   void set_elem_coef(int L, const std::vector<double> &c) {
     check(fm_system_set_elem_coef(h_, L, c.data()), "fm_system_set_elem_coef");
+  }
+  // nodal values of variable ivar on the top-level nodes from a top-level DoF vector (0 where the variable is absent)
+  std::vector<double> node_values(const std::vector<double> &x, int ivar = 0) const {
+    const int N = mesh_.n_nodes();
+    std::vector<double> v(N, 0.0);
+    const std::vector<int> &nd = _node_dof[_NoLevels - 1];
+    for (int k = 0; k < N; k++) {
+      const int d = nd[k + (size_t)ivar * N];
+      if (d >= 0) v[k] = x[d];
+    }
+    return v;
+  }
+  // Dirichlet data of the penalty RHS separate from x_old (top level; empty: the reference's x_old)
+  void set_bcdata(const std::vector<double> &g) {
+    check(fm_system_set_bcdata(h_, g.empty() ? nullptr : g.data()), "fm_system_set_bcdata");
+  }
+  // transposed multigrid solve A^T lambda = g on the transposed hierarchy (GenMatTranspose on every level first)
+  std::vector<double> AdjointSolve(const std::vector<double> &g) {
+    std::vector<double> lam(_Dim.back(), 0.0);
+    const int rc = fm_adjoint_solve(h_, &opts, g.data(), lam.data(), &report);
+    if (rc != FM_OK && rc != FM_ENOCONV) check(rc, "fm_adjoint_solve");
+    return lam;
   }
   fm_system *handle() const { return h_; }
 
@@ -209,7 +237,36 @@ class MGSolT : public MGSolDA {
     (void)time;
     check(fm_heat_genmatrhs(h_, Level, mode, &par), "fm_heat_genmatrhs");
   }
+  // transpose-Jacobian assembly of level L (the discrete adjoint's operator)
+  void GenMatTranspose(int Level) { check(fm_heat_genmat_transpose(h_, Level, &par), "fm_heat_genmat_transpose"); }
+  // advecting velocity [dim * Ntop] (component-major nodal values); empty removes it
+  void set_velocity(const std::vector<double> &vel) {
+    check(fm_system_set_velocity(h_, vel.empty() ? nullptr : vel.data()), "fm_system_set_velocity");
+  }
+  // Nusselt number along axis of the top-level temperature T, with dNu/dT and the partials dNu/dk_e
+  double nusselt(const std::vector<double> &T, int axis, std::vector<double> *dJdT = nullptr,
+                 std::vector<double> *dJdk = nullptr) {
+    double J = 0.0;
+    if (dJdT) dJdT->assign(_Dim.back(), 0.0);
+    if (dJdk) dJdk->assign(n_elem_top(), 0.0);
+    check(fm_heat_nusselt(h_, &par, axis, T.data(), &J, dJdT ? dJdT->data() : nullptr, dJdk ? dJdk->data() : nullptr),
+          "fm_heat_nusselt");
+    return J;
+  }
+  // -lambda^T (dA/dk_e) T for every top-level element
+  std::vector<double> grad_coef(const std::vector<double> &lam, const std::vector<double> &T) {
+    std::vector<double> g(n_elem_top(), 0.0);
+    check(fm_heat_grad_coef(h_, &par, lam.data(), T.data(), g.data()), "fm_heat_grad_coef");
+    return g;
+  }
   fm_heat_params par;
+
+ private:
+  int n_elem_top() const {
+    int ne = 0, n2 = 0, n1 = 0;
+    check(fm_mesh_level(mesh_.get(), _NoLevels - 1, &ne, &n2, &n1), "fm_mesh_level");
+    return ne;
+  }
 };
 
 // MGSolNS: coupled Q2-Q1 Navier-Stokes (template_appl/nse/SRC/MGSolverNS.C, NS_EQUATIONS == 1)
@@ -223,6 +280,24 @@ class MGSolNS : public MGSolDA {
   void GenMatRhs(double time, int Level, int mode) override {
     (void)time;
     check(fm_ns_genmatrhs(h_, Level, mode, &par), "fm_ns_genmatrhs");
+  }
+  // transpose of the Oseen (Picard) matrix of level L
+  void GenMatTranspose(int Level) { check(fm_ns_genmat_transpose(h_, Level, &par), "fm_ns_genmat_transpose"); }
+  // Newton Jacobian of the steady residual at x_old (trans = 1: its transpose, for the adjoint)
+  void GenJac(int Level, int trans) { check(fm_ns_genjac(h_, Level, trans, &par), "fm_ns_genjac"); }
+  // Boussinesq body force coef * T from a nodal temperature field [Ntop]; empty removes it
+  void set_buoyancy(const std::vector<double> &T_node, const double coef[3]) {
+    check(fm_ns_set_buoyancy(h_, T_node.empty() ? nullptr : T_node.data(), coef), "fm_ns_set_buoyancy");
+  }
+  // nodal velocity [dim * Ntop] (component-major) of a top-level DoF vector
+  std::vector<double> velocity_nodes(const std::vector<double> &U) const {
+    const int N = mesh_.n_nodes(), dim = mesh_.dim();
+    std::vector<double> v((size_t)dim * N);
+    for (int d = 0; d < dim; d++) {
+      const std::vector<double> c = node_values(U, d);
+      std::copy(c.begin(), c.end(), v.begin() + (size_t)d * N);
+    }
+    return v;
   }
   fm_ns_params par;
 };

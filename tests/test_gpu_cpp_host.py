@@ -60,3 +60,23 @@ def test_cpp_cavity_picard_matches_python_mirror(femus, ctx):
     xc = np.array(r["x"])
     assert r["ndof"] == len(x)
     np.testing.assert_allclose(xc, x, rtol=1e-9, atol=1e-11 * np.abs(x).max())
+
+
+def test_cpp_boussinesq_matches_python_driver(femus, ctx):
+    """The coupled heat + Navier-Stokes time loop driven from C++ (set_buoyancy / set_velocity between the two
+    MGTimeStep calls, as MGEquationsSystem::eqnmap_timestep_loop steps coupled equations), the Nusselt number
+    and its frozen-velocity conductivity gradient (GenMatTranspose + AdjointSolve + grad_coef) equal the Python
+    driver's after the same 20 steps."""
+    from exfemus_amd import coupled
+    r = _run("dhc", 2, 4, 3, 1e4, 20)
+    b = coupled.Boussinesq(ctx, 2, 4, 3, 1e4)
+    for k in range(20):
+        b.step(float(k + 1))
+    Nu = b.nusselt()[0]
+    Nus, grad, _ = b.nusselt_gradient_k()
+    assert abs(r["Nu"] - Nu) <= 1e-7 * Nu
+    assert abs(r["Nu_steady"] - Nus) <= 1e-7 * Nus
+    g = np.array(r["grad"])
+    np.testing.assert_allclose(g, grad, rtol=1e-5, atol=1e-7 * np.abs(grad).max())
+    b.ns.close()
+    b.T.close()
