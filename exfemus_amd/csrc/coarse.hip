@@ -34,7 +34,6 @@
 namespace {
 
 constexpr int NB = 64;     // panel width
-constexpr int TS = 64;     // update tile (TS x TS outputs, 256 threads, 4 x 4 each)
 
 struct SwInfo {
   int singular, pinned;
@@ -165,48 +164,54 @@ __global__ __launch_bounds__(256) void k_sw_rows(int n, int k0, const double *__
   W[(int64_t)q * n + j] = r >= 0 ? M[(int64_t)r * n + j] : 0.0;
 }
 
-// M[i][j] -= sum_q L[i][q] W[q][j] for every column tile except the panel's;
-// L[i][q] = P[q][i] + (rowstep[i] == k0 + q).  Grid (ceil(n / TS), ceil(n / TS)).
+// M[i][j] -= sum_q L[i][q] W[q][j] for every column j outside the panel;
+// L[i][q] = P[q][i] + (rowstep[i] == k0 + q).  128 x 128 output tile per
+// workgroup, 8 x 8 per thread (rows ty + 16 a, columns tx + 16 b), the 64-deep
+// panel taken in two chunks of 32 through LDS.  Grid (ceil(n / 128), ceil(n / 128)).
+constexpr int UT = 128, UK = 32;
 __global__ __launch_bounds__(256) void k_sw_update(int n, int k0, int bw, double *__restrict__ M, const double *__restrict__ P,
                                                    const double *__restrict__ W, const int *__restrict__ rowstep,
                                                    const SwInfo *info) {
   if (info->singular) return;
-  const int j0 = blockIdx.x * TS, i0 = blockIdx.y * TS;
-  if (j0 == k0) return;          // k0 is a multiple of TS and bw <= TS: the panel is this tile column
-  __shared__ double sL[NB][TS], sW[NB][TS];
-  for (int e = threadIdx.x; e < NB * TS; e += 256) {
-    const int q = e / TS, x = e % TS;
-    const int i = i0 + x, j = j0 + x;
-    double l = 0., w = 0.;
-    if (q < bw) {
-      if (i < n) l = P[(int64_t)q * n + i] + (rowstep[i] == k0 + q ? 1.0 : 0.0);
-      if (j < n) w = W[(int64_t)q * n + j];
-    }
-    sL[q][x] = l;
-    sW[q][x] = w;
-  }
-  __syncthreads();
+  const int j0 = blockIdx.x * UT, i0 = blockIdx.y * UT;
+  __shared__ double sL[UK][UT], sW[UK][UT];
   const int ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
-  double acc[4][4] = {};
-  for (int q = 0; q < NB; q++) {
-    double lv[4], wv[4];
+  double acc[8][8] = {};
+  for (int c0 = 0; c0 < NB; c0 += UK) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < UK * UT; e += 256) {
+      const int q = c0 + e / UT, x = e % UT;
+      const int i = i0 + x, j = j0 + x;
+      double l = 0., w = 0.;
+      if (q < bw) {
+        if (i < n) l = P[(int64_t)q * n + i] + (rowstep[i] == k0 + q ? 1.0 : 0.0);
+        if (j < n) w = W[(int64_t)q * n + j];
+      }
+      sL[e / UT][x] = l;
+      sW[e / UT][x] = w;
+    }
+    __syncthreads();
+#pragma unroll 2
+    for (int q = 0; q < UK; q++) {
+      double lv[8], wv[8];
 #pragma unroll
-    for (int a = 0; a < 4; a++) lv[a] = sL[q][ty + 16 * a];
+      for (int a = 0; a < 8; a++) lv[a] = sL[q][ty + 16 * a];
 #pragma unroll
-    for (int b = 0; b < 4; b++) wv[b] = sW[q][tx + 16 * b];
+      for (int b = 0; b < 8; b++) wv[b] = sW[q][tx + 16 * b];
 #pragma unroll
-    for (int a = 0; a < 4; a++)
+      for (int a = 0; a < 8; a++)
 #pragma unroll
-      for (int b = 0; b < 4; b++) acc[a][b] = fma(lv[a], wv[b], acc[a][b]);
+        for (int b = 0; b < 8; b++) acc[a][b] = fma(lv[a], wv[b], acc[a][b]);
+    }
   }
 #pragma unroll
-  for (int a = 0; a < 4; a++) {
+  for (int a = 0; a < 8; a++) {
     const int i = i0 + ty + 16 * a;
     if (i >= n) continue;
 #pragma unroll
-    for (int b = 0; b < 4; b++) {
+    for (int b = 0; b < 8; b++) {
       const int j = j0 + tx + 16 * b;
-      if (j < n) M[(int64_t)i * n + j] -= acc[a][b];
+      if (j < n && (j < k0 || j >= k0 + bw)) M[(int64_t)i * n + j] -= acc[a][b];
     }
   }
 }
@@ -279,7 +284,8 @@ int fm_dense_inverse(hipStream_t st, const DevCSR &A, const int *d_rank, int pin
       std::swap(pin_buf, pout);
     }
     hipLaunchKernelGGL(k_sw_rows, dim3(g256, (unsigned)bw), dim3(256), 0, st, n, k0, M, prow, W, info);
-    hipLaunchKernelGGL(k_sw_update, dim3(g64, g64), dim3(256), 0, st, n, k0, bw, M, pin_buf, W, rowstep, info);
+    const unsigned gu = (unsigned)((n + UT - 1) / UT);
+    hipLaunchKernelGGL(k_sw_update, dim3(gu, gu), dim3(256), 0, st, n, k0, bw, M, pin_buf, W, rowstep, info);
     hipLaunchKernelGGL(k_sw_scatter, dim3(g64), dim3(256), 0, st, n, k0, bw, M, pin_buf, info);
   }
   SwInfo h{0, 0};
