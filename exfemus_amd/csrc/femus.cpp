@@ -17,6 +17,7 @@
 static thread_local std::string g_err;
 void fm_set_error(const std::string &msg) { g_err = msg; }
 extern "C" const char *fm_last_error(void) { return g_err.c_str(); }
+extern "C" int fm_set_sell_split(int W) { return fmk::sell_split(W); }
 
 #define FM_CHECK(x)                                 \
   do {                                              \

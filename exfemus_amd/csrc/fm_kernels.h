@@ -48,6 +48,7 @@ struct NsArgs {
 
 namespace fmk {
 int lanes_for(int64_t nnz, int64_t n);
+int sell_split(int W);  // SELL split threshold (slices); W < 0 reads it; returns the previous value
 int spmv(hipStream_t s, const CsrView<double> &A, const double *x, double *y);
 int spmv(hipStream_t s, const CsrView<float> &A, const float *x, float *y);
 int resid(hipStream_t s, const CsrView<double> &A, const double *b, const double *x, double *r);

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <cstdlib>
+#include <algorithm>
 #include "fm_kernels.h"
 
 #define WAVE 64
@@ -114,6 +115,47 @@ __device__ __forceinline__ T sell_row_packed(const T *__restrict__ vp, const uns
   return sum;
 }
 
+// sum over the positions [k0, k1) of one lane's row of slice s (k0 a multiple of UNR, so that it is also a
+// multiple of the 4 or 2 positions packed per 32-bit word of 8- or 16-bit offsets)
+template <typename T, int UNR>
+__device__ __forceinline__ T sell_range(int s, int lane, int k0, int k1, const int64_t *__restrict__ sl_off,
+                                        const int64_t *__restrict__ sl_c32, const unsigned short *__restrict__ c16,
+                                        const int *__restrict__ base, const int *__restrict__ c32,
+                                        const int64_t *__restrict__ sl_c8, const unsigned char *__restrict__ c8,
+                                        const int64_t *__restrict__ sl_c16, const T *__restrict__ vals,
+                                        const T *__restrict__ xg) {
+  const int64_t off = sl_off[s];
+  const int64_t o32 = sl_c32[s];
+  const int64_t o8 = sl_c8[s], o16 = sl_c16[s];
+  const T *vp = vals + off + lane + (int64_t)k0 * 64;
+  const int *bp = base + (off >> 6) + k0;
+  const int w = k1 - k0;
+  if (w <= 0) return T(0);
+  if (o8 >= 0)
+    return sell_row_packed<T, 8, UNR>(vp, reinterpret_cast<const unsigned *>(c8 + o8) + lane + (k0 / 4) * 64, bp, w, xg);
+  if (o16 >= 0)
+    return sell_row_packed<T, 16, UNR>(vp, reinterpret_cast<const unsigned *>(c16 + o16) + lane + (k0 / 2) * 64, bp,
+                                       w, xg);
+  const int *cp = c32 + o32 + lane + (int64_t)k0 * 64;
+  T sum = 0;
+  int k = 0;
+  for (; k + UNR <= w; k += UNR) {
+    T v[UNR];
+    int c[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; u++) v[u] = __builtin_nontemporal_load(vp + (k + u) * 64);
+#pragma unroll
+    for (int u = 0; u < UNR; u++) c[u] = __builtin_nontemporal_load(cp + (k + u) * 64);
+    T xv[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; u++) xv[u] = xg[c[u]];
+#pragma unroll
+    for (int u = 0; u < UNR; u++) sum += v[u] * xv[u];
+  }
+  for (; k < w; k++) sum += __builtin_nontemporal_load(vp + k * 64) * xg[__builtin_nontemporal_load(cp + k * 64)];
+  return sum;
+}
+
 template <typename T, int EP, int UNR>
 __global__ __launch_bounds__(256) void k_sell(int nsl, const int64_t *__restrict__ sl_off,
                                               const int64_t *__restrict__ sl_c32, const int *__restrict__ sl_w,
@@ -130,52 +172,70 @@ __global__ __launch_bounds__(256) void k_sell(int nsl, const int64_t *__restrict
   const int s = slist ? __builtin_amdgcn_readfirstlane(slist[w_idx]) : w_idx;
   const int lane = threadIdx.x & 63;
   const int row = perm[(int64_t)s * 64 + lane];
+  const T sum = sell_range<T, UNR>(s, lane, 0, sl_w[s], sl_off, sl_c32, c16, base, c32, sl_c8, c8, sl_c16, vals, xg);
+  if (row < 0) return;
+  apply_ep<T, EP>(row, sum, xg, ep);
+}
+
+// The same product for matrices with few slices (the coarser multigrid levels), where one wave per slice
+// leaves the chip mostly idle and a launch costs the latency of one wave walking its rows' ~100 positions
+// in sequence: S waves (one block of 64 S threads) share a slice, wave j summing the positions
+// [j c, (j + 1) c) with c = ceil(w / S) rounded up to the unroll; the partial sums meet in LDS and wave 0
+// adds them in wave order and applies the epilogue.
+template <typename T, int EP, int UNR, int S>
+__global__ __launch_bounds__(64 * S) void k_sell_split(int nsl, const int64_t *__restrict__ sl_off,
+                                                       const int64_t *__restrict__ sl_c32, const int *__restrict__ sl_w,
+                                                       const int *__restrict__ perm,
+                                                       const unsigned short *__restrict__ c16,
+                                                       const int *__restrict__ base, const int *__restrict__ c32,
+                                                       const int64_t *__restrict__ sl_c8,
+                                                       const unsigned char *__restrict__ c8,
+                                                       const int64_t *__restrict__ sl_c16, const T *__restrict__ vals,
+                                                       const T *__restrict__ xg, EpArgs<T> ep,
+                                                       const int *__restrict__ slist) {
+  __shared__ T part[S - 1][64];
+  const int w_idx = blockIdx.x;
+  if (w_idx >= nsl) return;  // whole block
+  const int s = slist ? __builtin_amdgcn_readfirstlane(slist[w_idx]) : w_idx;
+  const int lane = threadIdx.x & 63;
+  const int j = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int w = sl_w[s];
-  const int64_t off = sl_off[s];
-  const int64_t o32 = sl_c32[s];
-  const int64_t o8 = sl_c8[s], o16 = sl_c16[s];
-  const T *vp = vals + off + lane;
-  const int *bp = base + (off >> 6);
-  T sum = 0;
-  int k = 0;
-  if (o8 >= 0) {
-    sum = sell_row_packed<T, 8, UNR>(vp, reinterpret_cast<const unsigned *>(c8 + o8) + lane, bp, w, xg);
-  } else if (o16 >= 0) {
-    sum = sell_row_packed<T, 16, UNR>(vp, reinterpret_cast<const unsigned *>(c16 + o16) + lane, bp, w, xg);
-  } else {
-    const int *cp = c32 + o32 + lane;
-    for (; k + UNR <= w; k += UNR) {
-      T v[UNR];
-      int c[UNR];
+  const int c = ((w + S - 1) / S + UNR - 1) / UNR * UNR;
+  const int k0 = min(w, j * c), k1 = min(w, k0 + c);
+  T sum = sell_range<T, UNR>(s, lane, k0, k1, sl_off, sl_c32, c16, base, c32, sl_c8, c8, sl_c16, vals, xg);
+  if (j > 0) part[j - 1][lane] = sum;
+  __syncthreads();
+  if (j > 0) return;
 #pragma unroll
-      for (int u = 0; u < UNR; u++) v[u] = __builtin_nontemporal_load(vp + (k + u) * 64);
-#pragma unroll
-      for (int u = 0; u < UNR; u++) c[u] = __builtin_nontemporal_load(cp + (k + u) * 64);
-      T xv[UNR];
-#pragma unroll
-      for (int u = 0; u < UNR; u++) xv[u] = xg[c[u]];
-#pragma unroll
-      for (int u = 0; u < UNR; u++) sum += v[u] * xv[u];
-    }
-    for (; k < w; k++) sum += __builtin_nontemporal_load(vp + k * 64) * xg[__builtin_nontemporal_load(cp + k * 64)];
-  }
+  for (int q = 0; q < S - 1; q++) sum += part[q][lane];
+  const int row = perm[(int64_t)s * 64 + lane];
   if (row < 0) return;
   apply_ep<T, EP>(row, sum, xg, ep);
 }
 
 // launch-shape knobs of the SELL kernel (read once from the environment):
-// FEMUS_SELL_UNROLL = 4 | 8 | 16 (default 8), FEMUS_SELL_XCD = 0 | 1 (default 1).
+// FEMUS_SELL_UNROLL = 4 | 8 | 16 (default 8), FEMUS_SELL_XCD = 0 | 1 (default 1),
+// FEMUS_SELL_SPLIT = W (default 16384; 0 = off): a product over n slices with n < W runs k_sell_split with
+// S = the smallest of 2, 4, 8 for which n S >= W (8 at most).
 // Measured on MI355X, 3D heat hex27 64^3 top matrix (135M nnz) in the
 // node-lexicographic internal order: contiguous per-XCD slice ranges take the
 // FP64 SpMV from 389 to 301 us and the FP32 one from 249 to 187 us (each XCD's
 // L2 then holds the x window of its own range); unroll 8 beats 4 by ~2%.
-static int g_sell_unroll = -1, g_sell_xcd = -1;
+static int g_sell_unroll = -1, g_sell_xcd = -1, g_sell_split = -1;
 static void sell_knobs() {
   if (g_sell_unroll >= 0) return;
   const char *u = getenv("FEMUS_SELL_UNROLL");
   const char *x = getenv("FEMUS_SELL_XCD");
+  const char *p = getenv("FEMUS_SELL_SPLIT");
   g_sell_unroll = (u && atoi(u) == 4) ? 4 : ((u && atoi(u) == 16) ? 16 : 8);
   g_sell_xcd = (x && atoi(x) == 0) ? 0 : 1;
+  if (g_sell_split < 0) g_sell_split = p ? std::max(0, atoi(p)) : 16384;
+}
+
+template <typename T, int EP, int S>
+static void launch_split(hipStream_t s, const CsrView<T> &A, const T *x, const EpArgs<T> &ep, int nsl) {
+  hipLaunchKernelGGL((k_sell_split<T, EP, 8, S>), dim3((unsigned)nsl), dim3(64 * S), 0, s, nsl, A.sl_off, A.sl_c32,
+                     A.sl_w, A.perm, A.c16, A.base, A.c32, A.sl_c8, A.c8, A.sl_c16, A.sv, x, ep, A.slist);
 }
 
 template <typename T, int EP>
@@ -186,6 +246,14 @@ static int launch_spmv(hipStream_t s, const CsrView<T> &A, const T *x, const EpA
     sell_knobs();
     const int64_t nwork = A.slist ? A.nlist : A.nsl;
     if (nwork <= 0) return 0;
+    const int nsl = (int)nwork;
+    if (nwork < g_sell_split) {
+      const int S = nwork * 2 >= g_sell_split ? 2 : (nwork * 4 >= g_sell_split ? 4 : 8);
+      if (S == 2) launch_split<T, EP, 2>(s, A, x, ep, nsl);
+      else if (S == 4) launch_split<T, EP, 4>(s, A, x, ep, nsl);
+      else launch_split<T, EP, 8>(s, A, x, ep, nsl);
+      return hipGetLastError() == hipSuccess ? 0 : -2;
+    }
     int64_t nb = (nwork + 3) / 4;
     int64_t chunk = 0;
     if (g_sell_xcd && nb >= 64) {
@@ -193,7 +261,6 @@ static int launch_spmv(hipStream_t s, const CsrView<T> &A, const T *x, const EpA
       nb = chunk * 8;  // padded grid: surplus blocks map past the last slice and exit
     }
     dim3 grid((unsigned)nb);
-    const int nsl = (int)nwork;
     if (g_sell_unroll == 8)
       hipLaunchKernelGGL((k_sell<T, EP, 8>), grid, dim3(block), 0, s, nsl, A.sl_off, A.sl_c32, A.sl_w, A.perm, A.c16,
                          A.base, A.c32, A.sl_c8, A.c8, A.sl_c16, A.sv, x, ep, (int)chunk, A.slist);
@@ -222,6 +289,13 @@ static int launch_spmv(hipStream_t s, const CsrView<T> &A, const T *x, const EpA
 }
 
 namespace fmk {
+
+int sell_split(int W) {
+  sell_knobs();
+  const int old = g_sell_split;
+  if (W >= 0) g_sell_split = W;
+  return old;
+}
 
 int lanes_for(int64_t nnz, int64_t n) {
   double avg = n > 0 ? (double)nnz / (double)n : 1.0;

@@ -42,7 +42,7 @@ EXPORTS = [
     "fm_heat_grad_coef", "fm_bench_spmv", "fm_sync", "fm_heat_genmat_transpose", "fm_ns_genmat_transpose", "fm_ns_genjac",
     "fm_get_adjoint_csr", "fm_heat_dissipation", "fm_bench_stream", "fm_ns_dissipation", "fm_ns_grad_coef",
     "fm_system_set_bcdata", "fm_heat_bc_rhs", "fm_ns_bc_rhs", "fm_ns_pressure_drop",
-    "fm_ns_set_buoyancy", "fm_heat_nusselt", "fm_heat_coupling", "fm_adjoint_spmv",
+    "fm_ns_set_buoyancy", "fm_heat_nusselt", "fm_heat_coupling", "fm_adjoint_spmv", "fm_set_sell_split",
 ]
 
 
@@ -91,6 +91,7 @@ def lib():
     vp = C.c_void_p
     pp = C.POINTER(C.c_void_p)
     L.fm_last_error.restype = C.c_char_p
+    L.fm_set_sell_split.argtypes = [C.c_int]
     L.fm_ctx_create.argtypes = [C.c_int, pp]
     L.fm_ctx_destroy.argtypes = [vp]
     L.fm_sync.argtypes = [vp]

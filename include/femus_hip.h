@@ -38,6 +38,12 @@ typedef struct fm_system fm_system;
 int fm_ctx_create(int device, fm_ctx **out);
 int fm_ctx_destroy(fm_ctx *ctx);
 const char *fm_last_error(void);
+/* Launch-shape knob of the level products (no counterpart in the reference, whose
+ * products are sequential): products over a SELL matrix with fewer than W slices of
+ * 64 rows let 2, 4 or 8 wavefronts share a slice (default W = 16384, or the
+ * FEMUS_SELL_SPLIT environment variable; 0 = never).  W < 0 only reads the value.
+ * Returns the previous W.  HIP graphs captured before a change keep their shape. */
+int fm_set_sell_split(int W);
 
 /* ---- mesh hierarchy: gencase GenCase + printMesh -----------------------
  * USER_APPL/gencase/src/MGGenCase.C:63-229 (build_square/build_cube +

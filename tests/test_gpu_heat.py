@@ -226,3 +226,46 @@ def test_level_operators_vs_csr(femus, oracle, ctx, dim, nx, nlev):
             pr, pc, pv = sysT.csr(2, L)
             P = sp.csr_matrix((pv, pc, pr), shape=(n, sysT._Dim[L - 1]))
             np.testing.assert_allclose(sysT.get_res(L), P @ xc, rtol=1e-12, atol=1e-12 * np.abs(xc).max())
+
+
+def test_split_slice_products_match(femus, oracle, ctx):
+    """The level products with 2, 4 or 8 wavefronts sharing a slice (the launch used for levels with few
+    slices) against the one-wave-per-slice launch and the reference-order CSR products: the threshold W is
+    swept over powers of two so that every level runs with S = 1, 2, 4 and 8 in turn."""
+    pm, om, sysT, ks = _build(femus, oracle, ctx, 3, [2, 2, 2], 3)
+    top = 2
+    sysT.set_xold(np.zeros(sysT._Dim[top]))
+    for L in range(3):
+        sysT.GenMatRhs(0.0, L, 1 if L == top else 0)
+    rng = np.random.default_rng(5)
+    lib = femus.lib()
+    W0 = lib.fm_set_sell_split(-1)
+    try:
+        for L in range(3):
+            n, nc = sysT._Dim[L], sysT._Dim[L - 1] if L > 0 else 0
+            x, b = rng.standard_normal(n), rng.standard_normal(n)
+            xc = rng.standard_normal(nc) if L > 0 else None
+            rp, ci, v = sysT.csr(0, L)
+            ref_res = b - sp.csr_matrix((v, ci, rp), shape=(n, n)) @ x
+            outs = {}
+            for W in [0] + [2 ** k for k in range(0, 17)] + [2 ** 30]:
+                lib.fm_set_sell_split(W)
+                sysT.set_x(x, L)
+                sysT.set_rhs(b, L)
+                sysT.resid(L)
+                res = sysT.get_res(L)
+                got = [res]
+                if L > 0:
+                    sysT.restrict(L)
+                    got.append(sysT.get_rhs(L - 1))
+                    sysT.set_x(xc, L - 1)
+                    sysT.prolong(L)
+                    got.append(sysT.get_res(L))
+                outs[W] = got
+                np.testing.assert_allclose(res, ref_res, rtol=1e-12,
+                                           atol=1e-13 * (np.abs(b).max() + np.abs(v).sum()))
+            for W, got in outs.items():
+                for a, r in zip(got, outs[0]):
+                    np.testing.assert_allclose(a, r, rtol=1e-13, atol=1e-13 * np.abs(r).max(), err_msg=f"L={L} W={W}")
+    finally:
+        lib.fm_set_sell_split(W0)
