@@ -214,7 +214,8 @@ __global__ __launch_bounds__(64 * S) void k_sell_split(int nsl, const int64_t *_
 }
 
 // launch-shape knobs of the SELL kernel (read once from the environment):
-// FEMUS_SELL_UNROLL = 4 | 8 | 16 (default 8), FEMUS_SELL_XCD = 0 | 1 (default 1),
+// FEMUS_SELL_UNROLL = 4 | 8 | 16 (default 8; the split kernel runs 16 when it is 16, else 8),
+// FEMUS_SELL_XCD = 0 | 1 (default 1),
 // FEMUS_SELL_SPLIT = W (default 16384; 0 = off): a product over n slices with n < W runs k_sell_split with
 // S = the smallest of 2, 4, 8 for which n S >= W (8 at most).
 // Measured on MI355X, 3D heat hex27 64^3 top matrix (135M nnz) in the
@@ -234,8 +235,12 @@ static void sell_knobs() {
 
 template <typename T, int EP, int S>
 static void launch_split(hipStream_t s, const CsrView<T> &A, const T *x, const EpArgs<T> &ep, int nsl) {
-  hipLaunchKernelGGL((k_sell_split<T, EP, 8, S>), dim3((unsigned)nsl), dim3(64 * S), 0, s, nsl, A.sl_off, A.sl_c32,
-                     A.sl_w, A.perm, A.c16, A.base, A.c32, A.sl_c8, A.c8, A.sl_c16, A.sv, x, ep, A.slist);
+  if (g_sell_unroll == 16)
+    hipLaunchKernelGGL((k_sell_split<T, EP, 16, S>), dim3((unsigned)nsl), dim3(64 * S), 0, s, nsl, A.sl_off, A.sl_c32,
+                       A.sl_w, A.perm, A.c16, A.base, A.c32, A.sl_c8, A.c8, A.sl_c16, A.sv, x, ep, A.slist);
+  else
+    hipLaunchKernelGGL((k_sell_split<T, EP, 8, S>), dim3((unsigned)nsl), dim3(64 * S), 0, s, nsl, A.sl_off, A.sl_c32,
+                       A.sl_w, A.perm, A.c16, A.base, A.c32, A.sl_c8, A.c8, A.sl_c16, A.sv, x, ep, A.slist);
 }
 
 template <typename T, int EP>

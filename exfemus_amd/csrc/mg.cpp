@@ -355,26 +355,49 @@ static int solve_stationary(fm_system *s, const fm_mg_opts *o, std::vector<LOps>
 }
 
 // Flexible GMRES(m), right preconditioned by one V-cycle (FP32 or FP64).
-__global__ void k_multi_dot(int64_t n, int k, const double *__restrict__ V, const double *__restrict__ w, double *part) {
-  // part[j * gridDim.x + block] = sum_i V[j][i] * w[i]  for j < k
-  extern __shared__ double sh[];
-  for (int j = 0; j < k; j++) {
-    double s = 0.;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-      s += V[(int64_t)j * n + i] * w[i];
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double t = 0.;
-      for (int q = 0; q < (int)(blockDim.x >> 6); q++) t += sh[q];
-      part[(int64_t)j * gridDim.x + blockIdx.x] = t;
-    }
-    __syncthreads();
+// part[j * gridDim.x + block] = sum_i V_j[i] w[i] for j < kk, where V_j is row j of V for j < k and w itself
+// for j = k (kk = k + 1: the squared norm of w rides along).  Block row blockIdx.y takes the 8 dot products
+// j0 .. j0 + 7 (j0 = 8 blockIdx.y) in one sweep, reading w once for the eight.  With skip non-null the
+// launch does nothing when skip[0] == 0.
+__global__ void k_multi_dot(int64_t n, int k, int kk, const double *__restrict__ V, const double *__restrict__ w,
+                            double *part, const double *skip) {
+  if (skip && skip[0] == 0.) return;
+  __shared__ double sh[4][8];
+  const int j0 = 8 * blockIdx.y;
+  const int nj = min(8, kk - j0);
+  const double *vp[8];
+#pragma unroll
+  for (int t = 0; t < 8; t++) vp[t] = (j0 + t < k) ? V + (int64_t)(j0 + t) * n : w;
+  double s[8];
+#pragma unroll
+  for (int t = 0; t < 8; t++) s[t] = 0.;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double wi = w[i];
+#pragma unroll
+    for (int t = 0; t < 8; t++)
+      if (t < nj) s[t] += vp[t][i] * wi;
+  }
+#pragma unroll
+  for (int t = 0; t < 8; t++)
+    for (int off = 32; off > 0; off >>= 1) s[t] += __shfl_xor(s[t], off, 64);
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int t = 0; t < 8; t++) sh[threadIdx.x >> 6][t] = s[t];
+  }
+  __syncthreads();
+  if (threadIdx.x < nj) {
+    double t = 0.;
+    for (int q = 0; q < (int)(blockDim.x >> 6); q++) t += sh[q][threadIdx.x];
+    part[(int64_t)(j0 + threadIdx.x) * gridDim.x + blockIdx.x] = t;
   }
 }
-__global__ void k_multi_sum(int k, int nb, const double *part, double *out) {
+// out[j] = sum of the nb partial sums of dot product j (0 when skip is non-null and skip[0] == 0)
+__global__ void k_multi_sum(int k, int nb, const double *part, double *out, const double *skip) {
   int j = blockIdx.x;
+  if (skip && skip[0] == 0.) {
+    if (threadIdx.x == 0) out[j] = 0.;
+    return;
+  }
   double s = 0.;
   for (int i = threadIdx.x; i < nb; i += blockDim.x) s += part[(int64_t)j * nb + i];
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
@@ -383,8 +406,19 @@ __global__ void k_multi_sum(int k, int nb, const double *part, double *out) {
   __syncthreads();
   if (threadIdx.x == 0) out[j] = sh[0] + sh[1] + sh[2] + sh[3];
 }
-// w -= sum_j h[j] V[j]
-__global__ void k_multi_axpy(int64_t n, int k, const double *__restrict__ V, const double *__restrict__ h, double *w) {
+// Selective reorthogonalisation (Daniel, Gragg, Kaufman and Stewart, Math. Comp. 30 (1976), eta = 1/sqrt 2):
+// after one classical Gram-Schmidt pass with coefficients h[0 .. k) and ww = h[k] = |w|^2 before it,
+// |w'|^2 = ww - sum h_j^2; a second pass is needed only when |w'| < |w| / sqrt 2.  flag[0] = 1 or 0.
+__global__ void k_dgks(int k, const double *h, double *flag) {
+  double proj = 0.;
+  for (int j = 0; j < k; j++) proj += h[j] * h[j];
+  const double ww = h[k];
+  flag[0] = (ww - proj < 0.5 * ww) ? 1. : 0.;
+}
+// w -= sum_j h[j] V[j] (nothing when skip is non-null and skip[0] == 0)
+__global__ void k_multi_axpy(int64_t n, int k, const double *__restrict__ V, const double *__restrict__ h, double *w,
+                             const double *skip) {
+  if (skip && skip[0] == 0.) return;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     double s = w[i];
     for (int j = 0; j < k; j++) s -= h[j] * V[(int64_t)j * n + i];
@@ -421,8 +455,8 @@ static int solve_fgmres(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &op
   if (hipMalloc((void **)&Z, (size_t)m * n * sizeof(double)) != hipSuccess) { (void)hipFree(V); return FM_ENOMEM; }
   // hd: [0, 64) first Gram-Schmidt pass, [64, 128) second pass, [128] norm of the new vector, [192, ...) y
   if (hipMalloc((void **)&hd, 256 * sizeof(double)) != hipSuccess) { (void)hipFree(V); (void)hipFree(Z); return FM_ENOMEM; }
-  const int nb = 512;
-  double *part = c->scratch;  // (m+1) * nb <= 4096 doubles: m <= 6 uses scratch; else allocate
+  const int nb = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + 2047) / 2048));
+  double *part = c->scratch;  // (m + 1) nb partial sums: the context scratch when they fit, else allocated
   double *part_alloc = nullptr;
   if ((int64_t)(m + 1) * nb > 2048) {
     if (hipMalloc((void **)&part_alloc, (size_t)(m + 1) * nb * sizeof(double)) != hipSuccess) { (void)hipFree(V); (void)hipFree(Z); (void)hipFree(hd); return FM_ENOMEM; }
@@ -479,9 +513,13 @@ static int solve_fgmres(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &op
         // copy of the Hessenberg column back to the host per iteration
         if ((rc = fmk::spmv(st, A, zj, w)) != 0) goto done;
         for (int pass = 0; pass < 2; pass++) {
-          hipLaunchKernelGGL(k_multi_dot, dim3(nb), dim3(256), 4 * sizeof(double), st, n, j + 1, V, w, part);
-          hipLaunchKernelGGL(k_multi_sum, dim3(j + 1), dim3(256), 0, st, j + 1, nb, part, hd + 64 * pass);
-          hipLaunchKernelGGL(k_multi_axpy, dim3(gv), dim3(256), 0, st, n, j + 1, V, hd + 64 * pass, w);
+          // pass 0 also forms |w|^2 (kk = j + 2) for the reorthogonalisation test; pass 1 runs only if it asks
+          const int kk = pass == 0 ? j + 2 : j + 1;
+          const double *skip = pass == 0 ? nullptr : hd + 129;
+          hipLaunchKernelGGL(k_multi_dot, dim3(nb, (kk + 7) / 8), dim3(256), 0, st, n, j + 1, kk, V, w, part, skip);
+          hipLaunchKernelGGL(k_multi_sum, dim3(kk), dim3(256), 0, st, kk, nb, part, hd + 64 * pass, skip);
+          if (pass == 0) hipLaunchKernelGGL(k_dgks, dim3(1), dim3(1), 0, st, j + 1, hd, hd + 129);
+          hipLaunchKernelGGL(k_multi_axpy, dim3(gv), dim3(256), 0, st, n, j + 1, V, hd + 64 * pass, w, skip);
         }
         if ((rc = fmk::dot_dev<double>(st, n, w, w, c->scratch, hd + 128, 1)) != 0) goto done;
         hipLaunchKernelGGL(k_scale_inv, dim3(gv), dim3(256), 0, st, n, w, hd + 128);
