@@ -53,10 +53,16 @@ int spmv(hipStream_t s, const CsrView<double> &A, const double *x, double *y);
 int spmv(hipStream_t s, const CsrView<float> &A, const float *x, float *y);
 int resid(hipStream_t s, const CsrView<double> &A, const double *b, const double *x, double *r);
 int resid(hipStream_t s, const CsrView<float> &A, const float *b, const float *x, float *r);
+// Chebyshev step: r -= A dold; dnew = c1 dold + c2 dinv r; x += dnew (x += dold + dnew with xold = 1)
 int cheb_step(hipStream_t s, const CsrView<double> &A, double *r, double *x, const double *dold, double *dnew,
-              const double *dinv, double c1, double c2);
+              const double *dinv, double c1, double c2, int xold = 0);
 int cheb_step(hipStream_t s, const CsrView<float> &A, float *r, float *x, const float *dold, float *dnew,
-              const float *dinv, float c1, float c2);
+              const float *dinv, float c1, float c2, int xold = 0);
+// r = b - A x and d = a dinv r in one product (x unchanged)
+int resid_d(hipStream_t s, const CsrView<double> &A, const double *b, const double *x, double *r, double *d,
+            const double *dinv, double a);
+int resid_d(hipStream_t s, const CsrView<float> &A, const float *b, const float *x, float *r, float *d,
+            const float *dinv, float a);
 int addmul(hipStream_t s, const CsrView<double> &P, const double *xc, double *x, double alpha, const double *cond);
 int addmul(hipStream_t s, const CsrView<float> &P, const float *xc, float *x, float alpha, const double *cond);
 int jacobi_step(hipStream_t s, const CsrView<double> &A, const double *b, const double *x, double *xn,
@@ -66,6 +72,8 @@ int jacobi_step(hipStream_t s, const CsrView<float> &A, const float *b, const fl
 template <typename T> int fill(hipStream_t s, int64_t n, T *x, T a);
 template <typename T> int axpby(hipStream_t s, int64_t n, T a, T *x, T b, const T *y);
 template <typename T> int cheb_first(hipStream_t s, int64_t n, const T *dinv, const T *r, T *d, T *x, T a);
+template <typename T> int cheb_start(hipStream_t s, int64_t n, const T *dinv, const T *b, T *r, T *d, T *x, T a);
+int from_f32(hipStream_t s, int64_t n, const float *a, double *b);
 int to_f32(hipStream_t s, int64_t n, const double *a, float *b, const double *scale);
 int add_f32(hipStream_t s, int64_t n, double *x, const float *e, const double *scale);
 template <typename T> int dot_dev(hipStream_t s, int64_t n, const T *x, const T *y, double *part, double *out, int do_sqrt);

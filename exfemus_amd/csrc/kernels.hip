@@ -23,7 +23,7 @@
 // ---------------------------------------------------------------------------
 // SpMV core with epilogues
 // ---------------------------------------------------------------------------
-enum { EP_Y = 0, EP_RES = 1, EP_CHEB = 2, EP_ADD = 3, EP_JAC = 4 };
+enum { EP_Y = 0, EP_RES = 1, EP_CHEB = 2, EP_ADD = 3, EP_JAC = 4, EP_RESD = 5 };
 
 template <typename T>
 struct EpArgs {
@@ -33,8 +33,9 @@ struct EpArgs {
   T *dnew;       // EP_CHEB: new direction
   const T *dold; // EP_CHEB: old direction (also the gathered vector)
   const T *dinv; // EP_CHEB / EP_JAC
-  T c1, c2;      // EP_CHEB: dnew = c1*dold + c2*dinv*r ; EP_ADD: x += c1*(Px) ; EP_JAC: omega
+  T c1, c2;      // EP_CHEB: dnew = c1*dold + c2*dinv*r ; EP_ADD: x += c1*(Px) ; EP_JAC: omega ; EP_RESD: d = c1*dinv*r
   const double *cond;  // EP_ADD: if non-null, apply only when cond[0] < cond[1] (reference rule)
+  int xold;      // EP_CHEB: x += dold + dnew instead of x += dnew (dold not yet added to x)
 };
 
 template <typename T, int EP>
@@ -46,11 +47,17 @@ __device__ inline void apply_ep(int64_t row, T sum, const T *__restrict__ xg, co
   } else if (EP == EP_CHEB) {
     T r = ep.y[row] - sum;
     ep.y[row] = r;
-    T dn = ep.c1 * ep.dold[row] + ep.c2 * ep.dinv[row] * r;
+    const T dol = ep.dold[row];
+    T dn = ep.c1 * dol + ep.c2 * ep.dinv[row] * r;
     ep.dnew[row] = dn;
-    ep.x[row] += dn;
+    ep.x[row] += ep.xold ? dol + dn : dn;
   } else if (EP == EP_ADD) {
     if (ep.cond == nullptr || ep.cond[0] < ep.cond[1]) ep.y[row] += ep.c1 * sum;
+  } else if (EP == EP_RESD) {
+    // r = b - A x and the first smoothing direction d = c1 dinv r (x is the gathered vector: not written here)
+    const T r = ep.b[row] - sum;
+    ep.y[row] = r;
+    ep.dnew[row] = ep.c1 * ep.dinv[row] * r;
   } else if (EP == EP_JAC) {
     // x_new = x + omega * dinv * (b - A x); written to dnew (x read via xg)
     ep.dnew[row] = xg[row] + ep.c1 * ep.dinv[row] * (ep.b[row] - sum);
@@ -320,9 +327,15 @@ template <typename T> static int resid_t(hipStream_t s, const CsrView<T> &A, con
   return launch_spmv<T, EP_RES>(s, A, x, ep);
 }
 template <typename T> static int cheb_t(hipStream_t s, const CsrView<T> &A, T *r, T *x, const T *dold, T *dnew,
-                                        const T *dinv, T c1, T c2) {
+                                        const T *dinv, T c1, T c2, int xold) {
   EpArgs<T> ep{}; ep.y = r; ep.x = x; ep.dold = dold; ep.dnew = dnew; ep.dinv = dinv; ep.c1 = c1; ep.c2 = c2;
+  ep.xold = xold;
   return launch_spmv<T, EP_CHEB>(s, A, dold, ep);
+}
+template <typename T> static int resid_d_t(hipStream_t s, const CsrView<T> &A, const T *b, const T *x, T *r, T *d,
+                                           const T *dinv, T a) {
+  EpArgs<T> ep{}; ep.y = r; ep.b = b; ep.dnew = d; ep.dinv = dinv; ep.c1 = a;
+  return launch_spmv<T, EP_RESD>(s, A, x, ep);
 }
 template <typename T> static int addmul_t(hipStream_t s, const CsrView<T> &P, const T *xc, T *x, T alpha,
                                           const double *cond) {
@@ -340,9 +353,17 @@ int spmv(hipStream_t s, const CsrView<float> &A, const float *x, float *y) { ret
 int resid(hipStream_t s, const CsrView<double> &A, const double *b, const double *x, double *r) { return resid_t(s, A, b, x, r); }
 int resid(hipStream_t s, const CsrView<float> &A, const float *b, const float *x, float *r) { return resid_t(s, A, b, x, r); }
 int cheb_step(hipStream_t s, const CsrView<double> &A, double *r, double *x, const double *dold, double *dnew,
-              const double *dinv, double c1, double c2) { return cheb_t(s, A, r, x, dold, dnew, dinv, c1, c2); }
+              const double *dinv, double c1, double c2, int xold) {
+  return cheb_t(s, A, r, x, dold, dnew, dinv, c1, c2, xold);
+}
 int cheb_step(hipStream_t s, const CsrView<float> &A, float *r, float *x, const float *dold, float *dnew,
-              const float *dinv, float c1, float c2) { return cheb_t(s, A, r, x, dold, dnew, dinv, c1, c2); }
+              const float *dinv, float c1, float c2, int xold) {
+  return cheb_t(s, A, r, x, dold, dnew, dinv, c1, c2, xold);
+}
+int resid_d(hipStream_t s, const CsrView<double> &A, const double *b, const double *x, double *r, double *d,
+            const double *dinv, double a) { return resid_d_t(s, A, b, x, r, d, dinv, a); }
+int resid_d(hipStream_t s, const CsrView<float> &A, const float *b, const float *x, float *r, float *d,
+            const float *dinv, float a) { return resid_d_t(s, A, b, x, r, d, dinv, a); }
 int addmul(hipStream_t s, const CsrView<double> &P, const double *xc, double *x, double alpha, const double *cond) {
   return addmul_t(s, P, xc, x, alpha, cond);
 }
@@ -372,6 +393,20 @@ template <typename T> __global__ void k_cheb_first(int64_t n, const T *dinv, con
     d[i] = di;
     x[i] += di;
   }
+}
+// the first Chebyshev sweep from a zero iterate: r = b, d = a * dinv .* b, x = d
+template <typename T> __global__ void k_cheb_start(int64_t n, const T *dinv, const T *b, T *r, T *d, T *x, T a) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const T bi = b[i];
+    const T di = a * dinv[i] * bi;
+    r[i] = bi;
+    d[i] = di;
+    x[i] = di;
+  }
+}
+// y64 = (double) x32
+__global__ void k_f32_to_f64(int64_t n, const float *a, double *b) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) b[i] = (double)a[i];
 }
 __global__ void k_f64_to_f32(int64_t n, const double *a, float *b) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) b[i] = (float)a[i];
@@ -418,6 +453,18 @@ template <typename T> int cheb_first(hipStream_t s, int64_t n, const T *dinv, co
 }
 template int cheb_first<double>(hipStream_t, int64_t, const double *, const double *, double *, double *, double);
 template int cheb_first<float>(hipStream_t, int64_t, const float *, const float *, float *, float *, float);
+template <typename T> int cheb_start(hipStream_t s, int64_t n, const T *dinv, const T *b, T *r, T *d, T *x, T a) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_cheb_start<T>, vgrid(n), dim3(256), 0, s, n, dinv, b, r, d, x, a);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+template int cheb_start<double>(hipStream_t, int64_t, const double *, const double *, double *, double *, double *, double);
+template int cheb_start<float>(hipStream_t, int64_t, const float *, const float *, float *, float *, float *, float);
+int from_f32(hipStream_t s, int64_t n, const float *a, double *b) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_f32_to_f64, vgrid(n), dim3(256), 0, s, n, a, b);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
 int to_f32(hipStream_t s, int64_t n, const double *a, float *b, const double *scale) {
   if (n <= 0) return 0;
   if (scale) hipLaunchKernelGGL(k_f64_to_f32_scaled, vgrid(n), dim3(256), 0, s, n, a, b, scale);

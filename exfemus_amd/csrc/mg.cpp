@@ -136,7 +136,36 @@ static std::vector<LOps> forward_ops(fm_system *s) {
 }
 
 // ---- smoothers ----------------------------------------------------------
-// xz: x is zero on entry, so the first residual is b itself (a copy instead of a product)
+// Chebyshev sweeps: the first direction d_0 = a D^-1 r_0 and x += d_0 come either from one elementwise
+// kernel (xz: x is zero on entry, r_0 = b, x = d_0; x need not be cleared beforehand) or from the residual
+// product's epilogue, with x += d_0 folded into the next step (x += d_0 + d_1).
+template <typename T>
+static int cheb_sweeps(hipStream_t st, const CsrView<T> &A, const LOps &o, T *x, const T *b, T *r, T *d, T *d2,
+                       const T *dinv, int its, bool xz, double a0, const std::vector<std::pair<double, double>> &c) {
+  bool pending = false;  // d_0 not yet added to x
+  if (xz) {
+    FM_CHECK(fmk::cheb_start<T>(st, o.n, dinv, b, r, d, x, (T)a0));
+  } else if (its >= 2) {
+    FM_CHECK(fmk::resid_d(st, A, b, x, r, d, dinv, (T)a0));
+    pending = true;
+  } else {
+    FM_CHECK(fmk::resid(st, A, b, x, r));
+    FM_CHECK(fmk::cheb_first<T>(st, o.n, dinv, r, d, x, (T)a0));
+  }
+  for (int k = 1; k < its; k++) {
+    FM_CHECK(fmk::cheb_step(st, A, r, x, d, d2, dinv, (T)c[k - 1].first, (T)c[k - 1].second, pending ? 1 : 0));
+    pending = false;
+    std::swap(d, d2);
+  }
+  return 0;
+}
+
+// xz: x is zero on entry (for the Chebyshev smoothers its memory need not hold zeros: the first sweep writes x)
+template <typename T>
+static bool smoother_writes_x(const fm_mg_opts *opt, int its) {
+  return its > 0 && (opt->smoother == FM_SMOOTH_CHEBYSHEV || opt->smoother == FM_SMOOTH_CHEBYSHEV4);
+}
+
 template <typename T>
 static int smooth(fm_system *s, const fm_mg_opts *opt, int L, const LOps &o, int its, bool xz) {
   hipStream_t st = s->ctx->stream;
@@ -148,36 +177,22 @@ static int smooth(fm_system *s, const fm_mg_opts *opt, int L, const LOps &o, int
   if (opt->smoother == FM_SMOOTH_CHEBYSHEV) {
     double lo = opt->cheb_lo * o.lam, hi = opt->cheb_hi * o.lam;
     double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta, rho = 1. / sigma;
-    if (xz) {
-      if (hipMemcpyAsync(r, b, o.n * sizeof(T), hipMemcpyDeviceToDevice, st) != hipSuccess) return FM_EHIP;
-    } else {
-      FM_CHECK(fmk::resid(st, A, b, x, r));
-    }
-    FM_CHECK(fmk::cheb_first<T>(st, o.n, dinv, r, d, x, (T)(1.0 / theta)));
+    std::vector<std::pair<double, double>> c;
     for (int k = 2; k <= its; k++) {
       double rho_new = 1. / (2. * sigma - rho);
-      FM_CHECK(fmk::cheb_step(st, A, r, x, d, d2, dinv, (T)(rho_new * rho), (T)(2. * rho_new / delta)));
-      std::swap(d, d2);
+      c.emplace_back(rho_new * rho, 2. * rho_new / delta);
       rho = rho_new;
     }
-    return 0;
+    return cheb_sweeps<T>(st, A, o, x, b, r, d, d2, dinv, its, xz, 1.0 / theta, c);
   }
   if (opt->smoother == FM_SMOOTH_CHEBYSHEV4) {
     // d_0 = 4/(3 rho) D^-1 r_0; r_k = r_{k-1} - A d_{k-1};
     // d_k = (2k-1)/(2k+3) d_{k-1} + (8k+4)/((2k+3) rho) D^-1 r_k; x += d_k   (rho = cheb_hi * lambda_max)
     const double rho = opt->cheb_hi * o.lam;
-    if (xz) {
-      if (hipMemcpyAsync(r, b, o.n * sizeof(T), hipMemcpyDeviceToDevice, st) != hipSuccess) return FM_EHIP;
-    } else {
-      FM_CHECK(fmk::resid(st, A, b, x, r));
-    }
-    FM_CHECK(fmk::cheb_first<T>(st, o.n, dinv, r, d, x, (T)(4.0 / (3.0 * rho))));
-    for (int k = 1; k < its; k++) {
-      const double c1 = (2.0 * k - 1.0) / (2.0 * k + 3.0), c2 = (8.0 * k + 4.0) / ((2.0 * k + 3.0) * rho);
-      FM_CHECK(fmk::cheb_step(st, A, r, x, d, d2, dinv, (T)c1, (T)c2));
-      std::swap(d, d2);
-    }
-    return 0;
+    std::vector<std::pair<double, double>> c;
+    for (int k = 1; k < its; k++)
+      c.emplace_back((2.0 * k - 1.0) / (2.0 * k + 3.0), (8.0 * k + 4.0) / ((2.0 * k + 3.0) * rho));
+    return cheb_sweeps<T>(st, A, o, x, b, r, d, d2, dinv, its, xz, 4.0 / (3.0 * rho), c);
   }
   if (opt->smoother == FM_SMOOTH_JACOBI) {
     // ping-pong through t: x_{k+1} = x_k + omega D^-1 (b - A x_k)
@@ -224,7 +239,10 @@ static int vcycle(fm_system *s, const fm_mg_opts *opt, std::vector<LOps> &ops, i
     FM_CHECK(fmk::ratio(st, l->nrm + 0, l->nrm + 1, l->nrm + 3));   // fine rest
   }
   FM_CHECK(fmk::spmv(st, cview<T>(*oc.R), r, Vec<T>::b(lc)));
-  FM_CHECK(fmk::fill<T>(st, oc.n, Vec<T>::x(lc), (T)0));
+  // the coarse iterate starts at zero: cleared here unless the coarse solve (level 0) or the coarse level's
+  // first Chebyshev sweep writes it whole
+  const int mult_c = opt->coarse_mult > 0 ? opt->coarse_mult : 2;
+  if (L - 1 > 0 && !smoother_writes_x<T>(opt, opt->nu_pre * mult_c)) FM_CHECK(fmk::fill<T>(st, oc.n, Vec<T>::x(lc), (T)0));
   const int gamma = (L - 1 > 0 && opt->gamma > 1) ? opt->gamma : 1;
   for (int g = 0; g < gamma; g++) FM_CHECK(vcycle<T>(s, opt, ops, L - 1, g == 0));
   double alpha = 1.0;
@@ -497,17 +515,16 @@ static int solve_fgmres(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &op
           if (hipMemcpyAsync(l->sb, l->b, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
           if (hipMemcpyAsync(l->b, vj, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
           if (hipMemcpyAsync(l->sx, l->x, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
-          if ((rc = fmk::fill<double>(st, n, l->x, 0.0)) != 0) goto done;
+          if (!smoother_writes_x<double>(o, o->nu_pre) && (rc = fmk::fill<double>(st, n, l->x, 0.0)) != 0) goto done;
           if ((rc = cycle_top<double>(s, o, ops, adj, true)) != 0) goto done;
           if (hipMemcpyAsync(zj, l->x, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
           if (hipMemcpyAsync(l->x, l->sx, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
           if (hipMemcpyAsync(l->b, l->sb, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
         } else {
           if ((rc = fmk::to_f32(st, n, vj, l->bf, nullptr)) != 0) goto done;
-          if ((rc = fmk::fill<float>(st, n, l->xf, 0.f)) != 0) goto done;
+          if (!smoother_writes_x<float>(o, o->nu_pre) && (rc = fmk::fill<float>(st, n, l->xf, 0.f)) != 0) goto done;
           if ((rc = cycle_top<float>(s, o, ops, adj, true)) != 0) goto done;
-          if ((rc = fmk::fill<double>(st, n, zj, 0.0)) != 0) goto done;
-          if ((rc = fmk::add_f32(st, n, zj, l->xf, nullptr)) != 0) goto done;
+          if ((rc = fmk::from_f32(st, n, l->xf, zj)) != 0) goto done;
         }
         // w = A z_j ; classical Gram-Schmidt twice and the normalisation, all on the device; one
         // copy of the Hessenberg column back to the host per iteration
