@@ -105,9 +105,18 @@ __device__ inline double jac_sur(int qp, const double *xb, const FeTab *tab) {
 //  4. Entries of Dirichlet rows are dropped (bc_vol row mask) and the rest
 //     are added to A through the element->CSR position map.
 // ===========================================================================
+// SELL slot of entry (a, b) of element e's matrix: position elpos[e][a][b] within the row of local DoF a, whose
+// position 0 is slot elslot[e][a]; -1 where the pattern has no such entry (0xFFFF) or the DoF does not exist
+__device__ __forceinline__ int64_t el_slot(const int64_t *__restrict__ elslot, const unsigned short *__restrict__ elpos,
+                                           int64_t e, int nl, int a, int b) {
+  const unsigned k = elpos[(e * nl + a) * nl + b];
+  const int64_t r = elslot[e * nl + a];
+  return (k == 0xFFFFu || r < 0) ? -1 : r + 64 * (int64_t)k;
+}
+
 template <int DIM>
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) void k_heat_vol(int nel, const int *__restrict__ elems, const int *__restrict__ conn,
-                                                  const int *__restrict__ eldof, const int *__restrict__ elpos,
+                                                  const int *__restrict__ eldof, const unsigned short *__restrict__ elpos, const int64_t *__restrict__ elslot,
                                                   const double *__restrict__ xyz, int Ntop, const int *__restrict__ dof_top,
                                                   const int *__restrict__ bc0, const double *__restrict__ x_old,
                                                   const double *__restrict__ coef, const double *__restrict__ vel,
@@ -224,7 +233,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) void k
 #pragma unroll
         for (int v = 0; v < TI; v++) {
           const int j = tj + 9 * v;
-          const int p = elpos[pbase + (trans ? j * N2 + i : i * N2 + j)];
+          const int64_t p = el_slot(elslot, elpos, e, N2, trans ? j : i, trans ? i : j);
           if (p >= 0) val[p] += K[u][v];
         }
       }
@@ -245,7 +254,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) void k
 // with boundary sides do work).
 template <int DIM>
 __global__ void k_heat_bd(int nel, const int *__restrict__ elems, const int *__restrict__ conn, const int *__restrict__ eldof,
-                          const int *__restrict__ elpos, const unsigned char *__restrict__ bmask,
+                          const unsigned short *__restrict__ elpos, const int64_t *__restrict__ elslot, const unsigned char *__restrict__ bmask,
                           const double *__restrict__ xyz, int Ntop, const int *__restrict__ dof_top,
                           const int *__restrict__ bc0, const int *__restrict__ bc1, const double *__restrict__ x_old,
                           const FeTab *__restrict__ tab, HeatArgs a, int mode, int trans, double *__restrict__ val,
@@ -273,7 +282,7 @@ __global__ void k_heat_bd(int nel, const int *__restrict__ elems, const int *__r
       double Ipen = jac_sur<DIM, NB, NGB>(NGB - 1, xb, tab) / a.dt;
       for (int k = 0; k < NB; k++) {
         int lv = sur[k];
-        int p = elpos[pbase + lv * N2 + lv];
+        int64_t p = el_slot(elslot, elpos, e, N2, lv, lv);
         if (val && p >= 0) val[p] += Ipen;
         if (mode == 1) {
           int node = conn[(int64_t)e * N2 + lv];
@@ -292,7 +301,7 @@ __global__ void k_heat_bd(int nel, const int *__restrict__ elems, const int *__r
           if (bc_alpha == 0) continue;
           double phii = tab->bphi2[i * NGB + qp];
           for (int j = 0; j < NB; j++) {
-            int p = elpos[pbase + (trans ? sur[j] * N2 + lei : lei * N2 + sur[j])];
+            int64_t p = el_slot(elslot, elpos, e, N2, trans ? sur[j] : lei, trans ? lei : sur[j]);
             if (val && p >= 0) val[p] += dtxJxW * bc_alpha * a.alpha0 * phii * tab->bphi2[j * NGB + qp];
           }
         }
@@ -327,7 +336,7 @@ __global__ void k_heat_bd(int nel, const int *__restrict__ elems, const int *__r
 // rows are not.
 template <int DIM>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_ns_vol(int nel, const int *__restrict__ elems, const int *__restrict__ conn,
-                                                const int *__restrict__ eldof, const int *__restrict__ elpos,
+                                                const int *__restrict__ eldof, const unsigned short *__restrict__ elpos, const int64_t *__restrict__ elslot,
                                                 const double *__restrict__ xyz, int Ntop, const int *__restrict__ dof_top,
                                                 const int *__restrict__ bc0, const double *__restrict__ x_old,
                                                 const double *__restrict__ coef, const FeTab *__restrict__ tab, NsArgs a,
@@ -525,7 +534,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 #pragma unroll
         for (int y = 0; y < TI; y++) {
           const int col = jv * N2 + tj + 9 * y;
-          const int p = elpos[pbase + (trans ? col * NL + row : row * NL + col)];
+          const int64_t p = el_slot(elslot, elpos, e, NL, trans ? col : row, trans ? row : col);
           if (p >= 0) val[p] += K[x][y];
         }
       }
@@ -550,7 +559,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 #pragma unroll 3
         for (int q = 0; q < NG; q++) K += s_JxW[q] * s_psi[ip * NG + q] * s_G[(q * DIM + jv) * N2 + j];
       }
-      const int p = elpos[pbase + (trans ? col * NL + row : row * NL + col)];
+      const int64_t p = el_slot(elslot, elpos, e, NL, trans ? col : row, trans ? row : col);
       if (p >= 0) val[p] += K;
     }
     if (mode == 1 && tid < DIM * N2) {
@@ -571,7 +580,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 // NS boundary faces (penalty Dirichlet with normal/tangential bits; Neumann tangential/normal terms)
 template <int DIM>
 __global__ void k_ns_bd(int nel, const int *__restrict__ elems, const int *__restrict__ conn, const int *__restrict__ eldof,
-                        const int *__restrict__ elpos, const unsigned char *__restrict__ bmask, const double *__restrict__ xyz,
+                        const unsigned short *__restrict__ elpos, const int64_t *__restrict__ elslot, const unsigned char *__restrict__ bmask, const double *__restrict__ xyz,
                         int Ntop, const int *__restrict__ dof_top, const int *__restrict__ bc0, const int *__restrict__ bc1,
                         const double *__restrict__ x_old, const FeTab *__restrict__ tab, NsArgs a, int mode, int trans,
                         double *__restrict__ val, double *__restrict__ rhs) {
@@ -625,11 +634,11 @@ __global__ void k_ns_bd(int nel, const int *__restrict__ elems, const int *__res
             rhs[eldof[(int64_t)e * NL + row]] += bc_rhs * Ipen * ((1 - bc_n) * (1 - bc_tg) * vo[iv] + bc_n * vdotn * nrm[iv] +
                                                                     bc_tg * (vo[iv] - vdotn * nrm[iv]));
           if (!val) continue;
-          int p = elpos[pbase + row * NL + row];
+          int64_t p = el_slot(elslot, elpos, e, NL, row, row);
           if (p >= 0) val[p] += ((1 - bc_n) * (1 - bc_tg) + bc_tg) * Ipen + Ipen * nrm[iv] * nrm[iv] * (bc_n - bc_tg);
           for (int jv = iv + 1; jv < DIM; jv++) {
             const int col = sur[i] + jv * N2;
-            int pc = elpos[pbase + (trans ? col * NL + row : row * NL + col)];
+            int64_t pc = el_slot(elslot, elpos, e, NL, trans ? col : row, trans ? row : col);
             if (pc >= 0) val[pc] += Ipen * nrm[jv] * nrm[iv] * (bc_n - bc_tg);
           }
         }
@@ -646,11 +655,11 @@ __global__ void k_ns_bd(int nel, const int *__restrict__ elems, const int *__res
             double phii = tab->bphi2[i * NGB + qp];
             for (int j = 0; j < NB; j++) {
               double phij = tab->bphi2[j * NGB + qp];
-              int p = elpos[pbase + row * NL + row];
+              int64_t p = el_slot(elslot, elpos, e, NL, row, row);
               if (p >= 0) val[p] += dtJxW * bc_tg * phij * phii + dtJxW * nrm[iv] * nrm[iv] * phij * phii * (bc_n - bc_tg);
               for (int jv = iv + 1; jv < DIM; jv++) {
                 const int col = sur[j] + jv * N2;
-                int pc = elpos[pbase + (trans ? col * NL + row : row * NL + col)];
+                int64_t pc = el_slot(elslot, elpos, e, NL, trans ? col : row, trans ? row : col);
                 if (pc >= 0) val[pc] += dtJxW * nrm[jv] * nrm[iv] * phij * phii * (bc_n - bc_tg);
               }
             }
@@ -1065,7 +1074,7 @@ static inline unsigned nblk(int nel) {
 // xbc: Dirichlet data for the boundary-face penalty RHS (nullptr: x_old, as in the reference);
 // bd_only: boundary kernel alone (with val == nullptr: the boundary RHS alone)
 int heat_assemble(hipStream_t s, int dim, int nel, const int *elems, const int *conn, const int *eldof,
-                  const int *elpos, const unsigned char *bmask, const double *xyz, int Ntop, const int *dof_top,
+                  const unsigned short *elpos, const int64_t *elslot, const unsigned char *bmask, const double *xyz, int Ntop, const int *dof_top,
                   const int *bc0, const int *bc1, const double *x_old, const double *xbc, const double *coef,
                   const double *vel, const FeTab *tab, HeatArgs a, int mode, int trans, int bd_only, double *val,
                   double *rhs) {
@@ -1073,37 +1082,37 @@ int heat_assemble(hipStream_t s, int dim, int nel, const int *elems, const int *
   const double *xb = xbc ? xbc : x_old;
   if (dim == 2) {
     if (!bd_only)
-      hipLaunchKernelGGL(k_heat_vol<2>, dim3(nblk(nel)), dim3(128), 0, s, nel, elems, conn, eldof, elpos, xyz, Ntop,
+      hipLaunchKernelGGL(k_heat_vol<2>, dim3(nblk(nel)), dim3(128), 0, s, nel, elems, conn, eldof, elpos, elslot, xyz, Ntop,
                          dof_top, bc0, x_old, coef, vel, tab, a, mode, trans, val, rhs);
-    hipLaunchKernelGGL(k_heat_bd<2>, dim3((nel + 127) / 128), dim3(128), 0, s, nel, elems, conn, eldof, elpos, bmask,
+    hipLaunchKernelGGL(k_heat_bd<2>, dim3((nel + 127) / 128), dim3(128), 0, s, nel, elems, conn, eldof, elpos, elslot, bmask,
                        xyz, Ntop, dof_top, bc0, bc1, xb, tab, a, mode, trans, val, rhs);
   } else {
     if (!bd_only)
-      hipLaunchKernelGGL(k_heat_vol<3>, dim3(nblk(nel)), dim3(128), 0, s, nel, elems, conn, eldof, elpos, xyz, Ntop,
+      hipLaunchKernelGGL(k_heat_vol<3>, dim3(nblk(nel)), dim3(128), 0, s, nel, elems, conn, eldof, elpos, elslot, xyz, Ntop,
                          dof_top, bc0, x_old, coef, vel, tab, a, mode, trans, val, rhs);
-    hipLaunchKernelGGL(k_heat_bd<3>, dim3((nel + 127) / 128), dim3(128), 0, s, nel, elems, conn, eldof, elpos, bmask,
+    hipLaunchKernelGGL(k_heat_bd<3>, dim3((nel + 127) / 128), dim3(128), 0, s, nel, elems, conn, eldof, elpos, elslot, bmask,
                        xyz, Ntop, dof_top, bc0, bc1, xb, tab, a, mode, trans, val, rhs);
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 int ns_assemble(hipStream_t s, int dim, int nel, const int *elems, const int *conn, const int *eldof,
-                const int *elpos, const unsigned char *bmask, const double *xyz, int Ntop, const int *dof_top,
+                const unsigned short *elpos, const int64_t *elslot, const unsigned char *bmask, const double *xyz, int Ntop, const int *dof_top,
                 const int *bc0, const int *bc1, const double *x_old, const double *xbc, const double *coef,
                 const FeTab *tab, NsArgs a, int mode, int trans, int bd_only, double *val, double *rhs) {
   if (nel <= 0) return 0;
   const double *xb = xbc ? xbc : x_old;
   if (dim == 2) {
     if (!bd_only)
-      hipLaunchKernelGGL(k_ns_vol<2>, dim3(nblk(nel)), dim3(256), 0, s, nel, elems, conn, eldof, elpos, xyz, Ntop,
+      hipLaunchKernelGGL(k_ns_vol<2>, dim3(nblk(nel)), dim3(256), 0, s, nel, elems, conn, eldof, elpos, elslot, xyz, Ntop,
                          dof_top, bc0, x_old, coef, tab, a, mode, trans, val, rhs);
-    hipLaunchKernelGGL(k_ns_bd<2>, dim3((2 * nel + 127) / 128), dim3(128), 0, s, nel, elems, conn, eldof, elpos, bmask,
+    hipLaunchKernelGGL(k_ns_bd<2>, dim3((2 * nel + 127) / 128), dim3(128), 0, s, nel, elems, conn, eldof, elpos, elslot, bmask,
                        xyz, Ntop, dof_top, bc0, bc1, xb, tab, a, mode, trans, val, rhs);
   } else {
     if (!bd_only)
-      hipLaunchKernelGGL(k_ns_vol<3>, dim3(nblk(nel)), dim3(256), 0, s, nel, elems, conn, eldof, elpos, xyz, Ntop,
+      hipLaunchKernelGGL(k_ns_vol<3>, dim3(nblk(nel)), dim3(256), 0, s, nel, elems, conn, eldof, elpos, elslot, xyz, Ntop,
                          dof_top, bc0, x_old, coef, tab, a, mode, trans, val, rhs);
-    hipLaunchKernelGGL(k_ns_bd<3>, dim3((3 * nel + 127) / 128), dim3(128), 0, s, nel, elems, conn, eldof, elpos, bmask,
+    hipLaunchKernelGGL(k_ns_bd<3>, dim3((3 * nel + 127) / 128), dim3(128), 0, s, nel, elems, conn, eldof, elpos, elslot, bmask,
                        xyz, Ntop, dof_top, bc0, bc1, xb, tab, a, mode, trans, val, rhs);
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;

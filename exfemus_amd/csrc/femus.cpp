@@ -208,7 +208,7 @@ extern "C" int fm_system_create(fm_ctx *ctx, const fm_mesh *m, int nq2, int nq1,
 static void free_sell(DevSELL &q) {
   dfree(q.sl_off); dfree(q.sl_c32); dfree(q.sl_w); dfree(q.perm); dfree(q.c16); dfree(q.base); dfree(q.c32);
   dfree(q.sl_c8); dfree(q.c8); dfree(q.sl_c16);
-  dfree(q.map); dfree(q.v); dfree(q.vf);
+  dfree(q.pos); dfree(q.rbase); dfree(q.rslot); dfree(q.v); dfree(q.vf);
   q.nsl = 0; q.nslot = 0; q.n32 = 0; q.n8 = 0; q.n16 = 0;
 }
 
@@ -253,6 +253,8 @@ int fm_build_sell(hipStream_t st, DevCSR &c, const int *ro, const int *cr, const
   gslice.back() = (int64_t)perm.size() / C;
   const int64_t nsl = (int64_t)perm.size() / C;
   if (nsl >= ((int64_t)1 << 31) / C) { fm_set_error("SELL: too many rows"); return FM_EINVAL; }
+  for (int64_t i = 0; i < n; i++)
+    if (rlen(i) >= 0xFFFF) { fm_set_error("SELL: a row has more than 65,534 entries"); return FM_EINVAL; }
   std::vector<int64_t> off(nsl + 1, 0);
   std::vector<int> wd(nsl);
   for (int64_t s = 0; s < nsl; s++) {
@@ -266,7 +268,9 @@ int fm_build_sell(hipStream_t st, DevCSR &c, const int *ro, const int *cr, const
   }
   c.sell.h_gslice = gslice;
   const int64_t nslot = off[nsl];
-  std::vector<int> absc(nslot, -1), map(nslot, -1), base(nslot / C, 0);
+  std::vector<int> absc(nslot, -1), base(nslot / C, 0);
+  std::vector<unsigned short> pos(nslot, 0xFFFF);
+  std::vector<int64_t> rbase(n, 0), rslot(n, -1);
   std::vector<char> wide(nsl, 0), narrow(nsl, 1);
 #pragma omp parallel
   {
@@ -277,13 +281,15 @@ int fm_build_sell(hipStream_t st, DevCSR &c, const int *ro, const int *cr, const
         int i = perm[s * C + l];
         if (i < 0) continue;
         int64_t r = crow(i);
+        rbase[i] = rp[r];
+        rslot[i] = off[s] + l;
         ent.clear();
-        for (int64_t q = rp[r]; q < rp[r + 1]; q++) ent.emplace_back(cr ? cr[ci[q]] : ci[q], (int)q);
+        for (int64_t q = rp[r]; q < rp[r + 1]; q++) ent.emplace_back(cr ? cr[ci[q]] : ci[q], (int)(q - rp[r]));
         std::sort(ent.begin(), ent.end());
         for (size_t k = 0; k < ent.size(); k++) {
           int64_t slot = off[s] + (int64_t)k * C + l;
           absc[slot] = ent[k].first;
-          map[slot] = ent[k].second;
+          pos[slot] = (unsigned short)ent[k].second;
         }
       }
       for (int k = 0; k < wd[s]; k++) {
@@ -333,7 +339,8 @@ int fm_build_sell(hipStream_t st, DevCSR &c, const int *ro, const int *cr, const
   FM_CHECK(dalloc(&q.sl_off, nsl + 1)); FM_CHECK(dalloc(&q.sl_c32, nsl)); FM_CHECK(dalloc(&q.sl_w, nsl));
   FM_CHECK(dalloc(&q.sl_c8, nsl)); FM_CHECK(dalloc(&q.c8, c8.size())); FM_CHECK(dalloc(&q.sl_c16, nsl));
   FM_CHECK(dalloc(&q.perm, nsl * C)); FM_CHECK(dalloc(&q.c16, c16.size())); FM_CHECK(dalloc(&q.base, nslot / C));
-  FM_CHECK(dalloc(&q.c32, c32.size())); FM_CHECK(dalloc(&q.map, nslot)); FM_CHECK(dalloc(&q.v, nslot));
+  FM_CHECK(dalloc(&q.c32, c32.size())); FM_CHECK(dalloc(&q.pos, nslot)); FM_CHECK(dalloc(&q.v, nslot));
+  FM_CHECK(dalloc(&q.rbase, n)); FM_CHECK(dalloc(&q.rslot, n));
   FM_CHECK(dalloc(&q.vf, nslot));
   FM_HIP(hipMemcpyAsync(q.sl_off, off.data(), (nsl + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
   FM_HIP(hipMemcpyAsync(q.sl_c32, o32.data(), nsl * sizeof(int64_t), hipMemcpyHostToDevice, st));
@@ -345,36 +352,45 @@ int fm_build_sell(hipStream_t st, DevCSR &c, const int *ro, const int *cr, const
   FM_HIP(hipMemcpyAsync(q.sl_c16, o16.data(), nsl * sizeof(int64_t), hipMemcpyHostToDevice, st));
   FM_HIP(hipMemcpyAsync(q.base, base.data(), (nslot / C) * sizeof(int), hipMemcpyHostToDevice, st));
   FM_HIP(hipMemcpyAsync(q.c32, c32.data(), c32.size() * sizeof(int), hipMemcpyHostToDevice, st));
-  FM_HIP(hipMemcpyAsync(q.map, map.data(), nslot * sizeof(int), hipMemcpyHostToDevice, st));
+  FM_HIP(hipMemcpyAsync(q.pos, pos.data(), nslot * sizeof(unsigned short), hipMemcpyHostToDevice, st));
+  FM_HIP(hipMemcpyAsync(q.rbase, rbase.data(), n * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  FM_HIP(hipMemcpyAsync(q.rslot, rslot.data(), n * sizeof(int64_t), hipMemcpyHostToDevice, st));
   FM_HIP(hipStreamSynchronize(st));
   return fm_sync_sell(st, c);
 }
 
+static SellMap sell_map(const DevSELL &q) {
+  SellMap m;
+  m.nsl = q.nsl; m.sl_off = q.sl_off; m.sl_w = q.sl_w; m.perm = q.perm; m.rbase = q.rbase; m.pos = q.pos;
+  return m;
+}
+
 int fm_csr_from_sell(hipStream_t st, DevCSR &c) {
   if (c.sell.nsl == 0) return 0;
-  FM_CHECK(fmk::sell2csr(st, c.sell.nslot, c.sell.map, c.sell.v, c.v));
+  FM_CHECK(fmk::sell2csr(st, sell_map(c.sell), c.sell.v, c.v));
   FM_CHECK(fmk::cast_vals(st, c.nnz, c.v, c.vf));
   return 0;
 }
 
-// SELL slot of every element-matrix entry and of every diagonal: elpos holds
-// CSR positions on entry and SELL slots on exit
-static int slots_from_csr(hipStream_t st, DevCSR &A, const int *ord, int64_t nelpos, int *elpos, int *dslot) {
-  int *slot_of = nullptr;
-  FM_CHECK(dalloc(&slot_of, A.nnz));
-  int rc = hipMemsetAsync(slot_of, 0xff, A.nnz * sizeof(int), st) == hipSuccess ? 0 : FM_EHIP;
-  if (!rc) rc = fmk::inv_map(st, A.sell.nslot, A.sell.map, slot_of);
-  if (!rc && elpos) rc = fmk::remap(st, nelpos, elpos, slot_of);
-  if (!rc) rc = fmk::diag_slot(st, A.nr, A.rp, A.ci, ord, slot_of, dslot);
+// SELL slots of the diagonals (dslot, one per internal row) and, with eldof given, the element map of the level
+// (elpos / elslot): CSR entries are translated to SELL positions through a temporary per-entry table
+static int sell_slots(hipStream_t st, DevCSR &A, const int *ord, int nel, int nloc, const int *eldof,
+                      const int *eldofi, unsigned short *elpos, int64_t *elslot, int64_t *dslot) {
+  unsigned short *kof = nullptr;
+  FM_CHECK(dalloc(&kof, A.nnz));
+  int rc = fmk::sell_kof(st, sell_map(A.sell), kof);
+  if (!rc && eldof) rc = fmk::elpos(st, nel, nloc, eldof, A.rp, A.ci, kof, elpos);
+  if (!rc && eldof) rc = fmk::elslot(st, (int64_t)nel * nloc, eldofi, A.sell.rslot, elslot);
+  if (!rc) rc = fmk::diag_slot(st, A.nr, A.rp, A.ci, ord, A.sell.rslot, kof, dslot);
   if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = FM_EHIP;
-  dfree(slot_of);
+  dfree(kof);
   return rc;
 }
 
 // refresh SELL values (FP64 and FP32) from the CSR values
 int fm_sync_sell(hipStream_t st, DevCSR &c) {
   if (c.sell.nsl == 0) return 0;
-  return fmk::csr2sell(st, c.sell.nslot, c.sell.map, c.v, c.sell.v, c.sell.vf);
+  return fmk::csr2sell(st, sell_map(c.sell), c.v, c.sell.v, c.sell.vf);
 }
 
 void fm_drop_graphs(fm_system *s) {
@@ -494,7 +510,8 @@ void fm_free_level(Level &l) {
   free_csr(l.A); free_csr(l.R); free_csr(l.P);
   dfree(l.x); dfree(l.b); dfree(l.r); dfree(l.t); dfree(l.d); dfree(l.d2); dfree(l.dinv);
   dfree(l.xf); dfree(l.bf); dfree(l.rf); dfree(l.tf); dfree(l.df); dfree(l.d2f); dfree(l.dinvf);
-  dfree(l.conn); dfree(l.eldof); dfree(l.eldofi); dfree(l.elpos); dfree(l.bmask); dfree(l.color_elems); dfree(l.coef);
+  dfree(l.conn); dfree(l.eldof); dfree(l.eldofi); dfree(l.elpos); dfree(l.elslot); dfree(l.bmask); dfree(l.color_elems);
+  dfree(l.coef);
   dfree(l.ord); dfree(l.rank);
   dfree(l.Ainv); dfree(l.Ainvf); dfree(l.vinv); dfree(l.vinvf); dfree(l.vwt); dfree(l.vwtf); dfree(l.nrm); dfree(l.res);
   dfree(l.sb); dfree(l.sx); dfree(l.vone);
@@ -654,7 +671,6 @@ extern "C" int fm_system_init(fm_system *s) {
     std::vector<int64_t> rp;
     std::vector<int> ci;
     fm_pattern(m, L, s->nq2, s->nq1, s->node_dof[L], rp, ci);
-    if (rp[l.n] >= ((int64_t)1 << 31)) { fm_set_error("nnz exceeds int32 element map range"); return FM_EINVAL; }
     FM_CHECK(upload_csr(st, l.A, l.n, l.n, rp, ci, nullptr, l.h_ord, l.h_rank, &l.h_goff));
     FM_CHECK(colour_slices(st, l));
     // transfers
@@ -691,11 +707,13 @@ extern "C" int fm_system_init(fm_system *s) {
     for (int e = 0; e < l.nel; e++) ce[fillp[col[e]]++] = e;
     std::vector<int> eldi(eld.size());
     for (size_t q = 0; q < eld.size(); q++) eldi[q] = eld[q] >= 0 ? l.h_rank[eld[q]] : -1;
-    dfree(l.conn); dfree(l.eldof); dfree(l.eldofi); dfree(l.elpos); dfree(l.bmask); dfree(l.color_elems);
+    dfree(l.conn); dfree(l.eldof); dfree(l.eldofi); dfree(l.elpos); dfree(l.elslot); dfree(l.bmask);
+    dfree(l.color_elems);
     FM_CHECK(dalloc(&l.conn, (size_t)l.nel * m.n2));
     FM_CHECK(dalloc(&l.eldof, (size_t)l.nel * nloc));
     FM_CHECK(dalloc(&l.eldofi, (size_t)l.nel * nloc));
     FM_CHECK(dalloc(&l.elpos, (size_t)l.nel * nloc * nloc));
+    FM_CHECK(dalloc(&l.elslot, (size_t)l.nel * nloc));
     FM_CHECK(dalloc(&l.bmask, l.nel));
     FM_CHECK(dalloc(&l.color_elems, l.nel));
     FM_HIP(hipMemcpyAsync(l.conn, m.conn[L].data(), m.conn[L].size() * sizeof(int), hipMemcpyHostToDevice, st));
@@ -703,10 +721,9 @@ extern "C" int fm_system_init(fm_system *s) {
     FM_HIP(hipMemcpyAsync(l.eldofi, eldi.data(), eldi.size() * sizeof(int), hipMemcpyHostToDevice, st));
     FM_HIP(hipMemcpyAsync(l.bmask, bm.data(), bm.size(), hipMemcpyHostToDevice, st));
     FM_HIP(hipMemcpyAsync(l.color_elems, ce.data(), ce.size() * sizeof(int), hipMemcpyHostToDevice, st));
-    FM_CHECK(fmk::elpos(st, l.nel, nloc, l.eldof, l.A.rp, l.A.ci, l.elpos));
     dfree(l.dslot);
     FM_CHECK(dalloc(&l.dslot, l.n));
-    FM_CHECK(slots_from_csr(st, l.A, l.ord, (int64_t)l.nel * nloc * nloc, l.elpos, l.dslot));
+    FM_CHECK(sell_slots(st, l.A, l.ord, l.nel, nloc, l.eldof, l.eldofi, l.elpos, l.elslot, l.dslot));
     FM_HIP(hipStreamSynchronize(st));
   }
   fm_drop_graphs(s);
@@ -841,7 +858,7 @@ static int bc_rhs(fm_system *s, const void *params, const double *g, double *rhs
       a.alpha0 = p->alpha0; a.rhocp = p->rhocp; a.dt = p->dt; a.tc = p->time_coeff; a.source = p->source;
       a.upwind = p->upwind;
       for (int d = 0; d < 3; d++) a.vel_const[d] = p->vel_const[d];
-      FM_CHECK(fmk::heat_assemble(st, m.dim, n, el, l.conn, l.eldof, l.elpos, l.bmask, s->xyz, m.Ntop, s->d_dof_top,
+      FM_CHECK(fmk::heat_assemble(st, m.dim, n, el, l.conn, l.eldof, l.elpos, l.elslot, l.bmask, s->xyz, m.Ntop, s->d_dof_top,
                                   s->d_bc0, s->d_bc1, s->x_old, l.t, l.coef, s->vel, ex.d_tab, a, 1, 0, 1, nullptr, l.d));
     } else {
       const fm_ns_params *p = (const fm_ns_params *)params;
@@ -850,7 +867,7 @@ static int bc_rhs(fm_system *s, const void *params, const double *g, double *rhs
       a.IRe = p->IRe; a.dt = p->dt; a.tc = p->time_coeff; a.upwind = p->upwind; a.IFr = p->IFr;
       for (int d = 0; d < 3; d++) { a.dirg[d] = p->dirg[d]; a.src[d] = p->src[d]; a.buoy[d] = 0.; }
       a.Tn = nullptr;
-      FM_CHECK(fmk::ns_assemble(st, m.dim, n, el, l.conn, l.eldof, l.elpos, l.bmask, s->xyz, m.Ntop, s->d_dof_top,
+      FM_CHECK(fmk::ns_assemble(st, m.dim, n, el, l.conn, l.eldof, l.elpos, l.elslot, l.bmask, s->xyz, m.Ntop, s->d_dof_top,
                                 s->d_bc0, s->d_bc1, s->x_old, l.t, l.coef, ex.d_tab, a, 1, 0, 1, nullptr, l.d));
     }
   }
@@ -919,7 +936,7 @@ extern "C" int fm_heat_genmatrhs(fm_system *s, int L, int mode, const fm_heat_pa
   SysExtra &ex = extra(s);
   for (int k = 0; k < l.ncolor; k++) {
     int n = l.color_off[k + 1] - l.color_off[k];
-    FM_CHECK(fmk::heat_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldofi, l.elpos, l.bmask, s->xyz,
+    FM_CHECK(fmk::heat_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldofi, l.elpos, l.elslot, l.bmask, s->xyz,
                                 m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, s->xbc, l.coef, s->vel, ex.d_tab, a,
                                 mode, 0, 0, l.A.sell.v, l.b));
   }
@@ -944,7 +961,7 @@ extern "C" int fm_ns_genmatrhs(fm_system *s, int L, int mode, const fm_ns_params
   SysExtra &ex = extra(s);
   for (int k = 0; k < l.ncolor; k++) {
     int n = l.color_off[k + 1] - l.color_off[k];
-    FM_CHECK(fmk::ns_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldofi, l.elpos, l.bmask, s->xyz,
+    FM_CHECK(fmk::ns_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldofi, l.elpos, l.elslot, l.bmask, s->xyz,
                               m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, s->xbc, l.coef, ex.d_tab, a, mode, 0, 0,
                               l.A.sell.v, l.b));
   }
@@ -1083,7 +1100,7 @@ static int ensure_adjoint(fm_system *s) {
     FM_CHECK(colour_slices(st, a));
     if (a.A.sell.nslot != l.A.sell.nslot) { fm_set_error("adjoint SELL layout differs"); return FM_ESTATE; }
     FM_CHECK(dalloc(&a.dslot, a.n));
-    FM_CHECK(slots_from_csr(st, a.A, a.ord, 0, nullptr, a.dslot));
+    FM_CHECK(sell_slots(st, a.A, a.ord, 0, 0, nullptr, nullptr, nullptr, nullptr, a.dslot));
     if (L < top) {  // restriction of the adjoint = (prolongation of level L+1)^T
       const DevCSR &P = s->lev[L + 1].P;
       std::vector<int64_t> rpT; std::vector<int> ciT; std::vector<double> vT;
@@ -1130,7 +1147,7 @@ extern "C" int fm_heat_genmat_transpose(fm_system *s, int L, const fm_heat_param
   SysExtra &ex = extra(s);
   for (int k = 0; k < l.ncolor; k++) {
     int n = l.color_off[k + 1] - l.color_off[k];
-    FM_CHECK(fmk::heat_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldof, l.elpos, l.bmask, s->xyz,
+    FM_CHECK(fmk::heat_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldof, l.elpos, l.elslot, l.bmask, s->xyz,
                                 m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, s->xbc, l.coef, s->vel, ex.d_tab, h,
                                 0, 1, 0, a.A.sell.v, nullptr));
   }
@@ -1153,7 +1170,7 @@ extern "C" int fm_ns_genmat_transpose(fm_system *s, int L, const fm_ns_params *p
   SysExtra &ex = extra(s);
   for (int k = 0; k < l.ncolor; k++) {
     int n = l.color_off[k + 1] - l.color_off[k];
-    FM_CHECK(fmk::ns_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldof, l.elpos, l.bmask, s->xyz,
+    FM_CHECK(fmk::ns_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldof, l.elpos, l.elslot, l.bmask, s->xyz,
                               m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, s->xbc, l.coef, ex.d_tab, h, 0, 1, 0,
                               a.A.sell.v, nullptr));
   }
@@ -1182,7 +1199,7 @@ extern "C" int fm_ns_genjac(fm_system *s, int L, int trans, const fm_ns_params *
   SysExtra &ex = extra(s);
   for (int k = 0; k < l.ncolor; k++) {
     int n = l.color_off[k + 1] - l.color_off[k];
-    FM_CHECK(fmk::ns_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldofi, l.elpos, l.bmask, s->xyz,
+    FM_CHECK(fmk::ns_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldofi, l.elpos, l.elslot, l.bmask, s->xyz,
                               m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, s->xbc, l.coef, ex.d_tab, a, 0, trans, 0,
                               val, nullptr));
   }

@@ -46,7 +46,11 @@ struct fm_mesh {
 struct DevSELL {
   int64_t nsl = 0, nslot = 0, n32 = 0, n8 = 0, n16 = 0;
   int64_t *sl_off = nullptr, *sl_c32 = nullptr, *sl_c8 = nullptr, *sl_c16 = nullptr;
-  int *sl_w = nullptr, *perm = nullptr, *base = nullptr, *c32 = nullptr, *map = nullptr;
+  int *sl_w = nullptr, *perm = nullptr, *base = nullptr, *c32 = nullptr;
+  // slot -> CSR entry: slot (s, k, l) of internal row i = perm[64 s + l] holds CSR entry rbase[i] + pos[slot]
+  // (pos 0xFFFF: padding); rslot[i]: slot of position 0 of internal row i
+  unsigned short *pos = nullptr;
+  int64_t *rbase = nullptr, *rslot = nullptr;
   unsigned short *c16 = nullptr;
   unsigned char *c8 = nullptr;
   double *v = nullptr;
@@ -92,8 +96,11 @@ struct Level {
   int *conn = nullptr;     // nel * n2 node ids
   int *eldof = nullptr;    // nel * nloc dofs (reference numbering)
   int *eldofi = nullptr;   // nel * nloc dofs (internal numbering)
-  int *elpos = nullptr;    // nel * nloc * nloc SELL slots of A (the assembly writes the SELL values directly)
-  int *dslot = nullptr;    // n: SELL slot of the diagonal of internal row i
+  // element map of A (the assembly writes the SELL values directly): entry (a, b) of element e sits at SELL
+  // slot elslot[e][a] + 64 elpos[e][a][b] (elpos 0xFFFF: no entry)
+  unsigned short *elpos = nullptr;  // nel * nloc * nloc positions within the row
+  int64_t *elslot = nullptr;        // nel * nloc: slot of position 0 of the row of each element DoF (-1: none)
+  int64_t *dslot = nullptr;         // n: SELL slot of the diagonal of internal row i
   unsigned char *bmask = nullptr;  // nel: bit s set if side s on the boundary
   int ncolor = 0;
   std::vector<int> color_off; // element ranges per colour (elements reordered by colour)

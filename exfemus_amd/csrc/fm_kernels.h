@@ -22,6 +22,16 @@ struct CsrView {
   int64_t nlist = 0;
 };
 
+// SELL slot <-> CSR entry correspondence of one matrix: slot (s, k, l) of internal row i = perm[64 s + l] holds
+// CSR entry rbase[i] + pos[slot] (pos 0xFFFF: padding)
+struct SellMap {
+  int64_t nsl = 0;
+  const int64_t *sl_off = nullptr;
+  const int *sl_w = nullptr, *perm = nullptr;
+  const int64_t *rbase = nullptr;
+  const unsigned short *pos = nullptr;
+};
+
 // Reference-element tables (src/MGFE.C:195-924 layout: phi[s*ng+qp],
 // dphi[(s*ng+qp) + d*ns*ng]); face tables are the (dim-1) Q2 tables.
 struct FeTab {
@@ -80,24 +90,26 @@ template <typename T> int dot_dev(hipStream_t s, int64_t n, const T *x, const T 
 int ratio(hipStream_t s, const double *a, const double *b, double *out);
 template <typename T> int dense_mv(hipStream_t s, int n, const T *M, const T *b, T *y);
 int gather(hipStream_t s, int64_t n, const int *idx, const double *src, double *dst);
-int dinv_slot(hipStream_t s, int64_t n, const int *dslot, const double *sv, double *d64, float *d32);
-int sell2csr(hipStream_t s, int64_t nslot, const int *map, const double *sv, double *cv);
-int inv_map(hipStream_t s, int64_t nslot, const int *map, int *inv);
-int remap(hipStream_t s, int64_t n, int *io, const int *table);
-int diag_slot(hipStream_t s, int64_t n, const int64_t *rp, const int *ci, const int *ord, const int *slot_of, int *dslot);
+int dinv_slot(hipStream_t s, int64_t n, const int64_t *dslot, const double *sv, double *d64, float *d32);
+int sell2csr(hipStream_t s, const SellMap &m, const double *sv, double *cv);
+int csr2sell(hipStream_t s, const SellMap &m, const double *csr_v, double *sv, float *svf);
+int sell_kof(hipStream_t s, const SellMap &m, unsigned short *kof);
+int diag_slot(hipStream_t s, int64_t n, const int64_t *rp, const int *ci, const int *ord, const int64_t *rslot,
+              const unsigned short *kof, int64_t *dslot);
+int elslot(hipStream_t s, int64_t n, const int *eldofi, const int64_t *rslot, int64_t *out);
 int cast_vals(hipStream_t s, int64_t n, const double *v, float *vf);
-int elpos(hipStream_t s, int nel, int nloc, const int *eldof, const int64_t *rp, const int *ci, int *pos);
+int elpos(hipStream_t s, int nel, int nloc, const int *eldof, const int64_t *rp, const int *ci,
+          const unsigned short *kof, unsigned short *pos);
 int copy16(hipStream_t s, int64_t n16, const void *a, void *b);
-int csr2sell(hipStream_t s, int64_t nslot, const int *map, const double *csr_v, double *sv, float *svf);
 
 // assemble.hip
 int heat_assemble(hipStream_t s, int dim, int nel_color, const int *elems, const int *conn, const int *eldof,
-                  const int *elpos, const unsigned char *bmask, const double *xyz, int Ntop, const int *dof_top,
+                  const unsigned short *elpos, const int64_t *elslot, const unsigned char *bmask, const double *xyz, int Ntop, const int *dof_top,
                   const int *bc0, const int *bc1, const double *x_old, const double *xbc, const double *coef,
                   const double *vel, const FeTab *tab, HeatArgs a, int mode, int trans, int bd_only, double *val,
                   double *rhs);
 int ns_assemble(hipStream_t s, int dim, int nel_color, const int *elems, const int *conn, const int *eldof,
-                const int *elpos, const unsigned char *bmask, const double *xyz, int Ntop, const int *dof_top,
+                const unsigned short *elpos, const int64_t *elslot, const unsigned char *bmask, const double *xyz, int Ntop, const int *dof_top,
                 const int *bc0, const int *bc1, const double *x_old, const double *xbc, const double *coef,
                 const FeTab *tab, NsArgs a, int mode, int trans, int bd_only, double *val, double *rhs);
 int coupling(hipStream_t s, int dim, int nel, const int *elems, const int *conn, const double *xyz, int Ntop,

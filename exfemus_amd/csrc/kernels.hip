@@ -535,71 +535,111 @@ template int dense_mv<float>(hipStream_t, int, const float *, const float *, flo
 
 // ---- diagonal inverse
 // dinv[i] = 1 / A_ii read at the diagonal's SELL slot dslot[i] (internal row i), FP64 and FP32
-__global__ void k_dinv_slot(int64_t n, const int *__restrict__ dslot, const double *__restrict__ sv, double *d64,
+__global__ void k_dinv_slot(int64_t n, const int64_t *__restrict__ dslot, const double *__restrict__ sv, double *d64,
                             float *d32) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const int q = dslot[i];
+    const int64_t q = dslot[i];
     const double d = q >= 0 ? sv[q] : 0.0;
     const double r = d != 0. ? 1.0 / d : 0.0;
     d64[i] = r;
     d32[i] = (float)r;
   }
 }
-int dinv_slot(hipStream_t s, int64_t n, const int *dslot, const double *sv, double *d64, float *d32) {
+int dinv_slot(hipStream_t s, int64_t n, const int64_t *dslot, const double *sv, double *d64, float *d32) {
   if (n <= 0) return 0;
   hipLaunchKernelGGL(k_dinv_slot, vgrid(n), dim3(256), 0, s, n, dslot, sv, d64, d32);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
-// CSR values from the SELL copy: cv[map[slot]] = sv[slot]
-__global__ void k_sell2csr(int64_t nslot, const int *__restrict__ map, const double *__restrict__ sv, double *cv) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nslot; i += (int64_t)gridDim.x * blockDim.x) {
-    const int q = map[i];
-    if (q >= 0) cv[q] = sv[i];
+
+// ---- SELL <-> CSR entry correspondence: slot (s, k, l) of internal row i = perm[64 s + l] holds CSR entry
+// rbase[i] + pos[slot] (pos = 0xFFFF: padding).  One wave per slice, four slices per block.
+#define SELL_WALK(M, BODY)                                                                       \
+  const int64_t s_ = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);                              \
+  if (s_ >= (M).nsl) return;                                                                     \
+  const int l_ = threadIdx.x & 63;                                                               \
+  const int row = (M).perm[s_ * 64 + l_];                                                        \
+  if (row < 0) return;                                                                           \
+  const int w_ = (M).sl_w[s_];                                                                   \
+  const int64_t rb = (M).rbase[row];                                                             \
+  for (int k = 0; k < w_; k++) {                                                                 \
+    const int64_t slot = (M).sl_off[s_] + (int64_t)k * 64 + l_;                                  \
+    const unsigned t = (M).pos[slot];                                                            \
+    if (t == 0xFFFFu) continue;                                                                  \
+    BODY                                                                                         \
   }
+
+// CSR values from the SELL copy: cv[rbase + pos] = sv[slot]
+__global__ void k_sell2csr(SellMap m, const double *__restrict__ sv, double *cv) {
+  SELL_WALK(m, cv[rb + t] = sv[slot];)
 }
-int sell2csr(hipStream_t s, int64_t nslot, const int *map, const double *sv, double *cv) {
-  if (nslot <= 0) return 0;
-  hipLaunchKernelGGL(k_sell2csr, vgrid(nslot), dim3(256), 0, s, nslot, map, sv, cv);
+int sell2csr(hipStream_t s, const SellMap &m, const double *sv, double *cv) {
+  if (m.nsl <= 0) return 0;
+  hipLaunchKernelGGL(k_sell2csr, dim3((unsigned)((m.nsl + 3) / 4)), dim3(256), 0, s, m, sv, cv);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
-// inv[map[slot]] = slot (inv pre-filled with -1)
-__global__ void k_inv_map(int64_t nslot, const int *__restrict__ map, int *inv) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nslot; i += (int64_t)gridDim.x * blockDim.x) {
-    const int q = map[i];
-    if (q >= 0) inv[q] = (int)i;
+// SELL values (FP64 and FP32) from the CSR values; padding slots are zero
+__global__ void k_csr2sell(SellMap m, const double *__restrict__ cv, double *__restrict__ sv, float *__restrict__ svf) {
+  const int64_t s_ = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (s_ >= m.nsl) return;
+  const int l_ = threadIdx.x & 63;
+  const int row = m.perm[s_ * 64 + l_];
+  const int w_ = m.sl_w[s_];
+  const int64_t rb = row >= 0 ? m.rbase[row] : 0;
+  for (int k = 0; k < w_; k++) {
+    const int64_t slot = m.sl_off[s_] + (int64_t)k * 64 + l_;
+    const unsigned t = m.pos[slot];
+    const double v = (row >= 0 && t != 0xFFFFu) ? cv[rb + t] : 0.0;
+    sv[slot] = v;
+    svf[slot] = (float)v;
   }
 }
-int inv_map(hipStream_t s, int64_t nslot, const int *map, int *inv) {
-  if (nslot <= 0) return 0;
-  hipLaunchKernelGGL(k_inv_map, vgrid(nslot), dim3(256), 0, s, nslot, map, inv);
+int csr2sell(hipStream_t s, const SellMap &m, const double *csr_v, double *sv, float *svf) {
+  if (m.nsl <= 0) return 0;
+  hipLaunchKernelGGL(k_csr2sell, dim3((unsigned)((m.nsl + 3) / 4)), dim3(256), 0, s, m, csr_v, sv, svf);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
-// io[t] = table[io[t]] for io[t] >= 0
-__global__ void k_remap(int64_t n, int *io, const int *__restrict__ table) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const int p = io[i];
-    io[i] = p >= 0 ? table[p] : -1;
-  }
+// kof[rbase + pos] = k: the SELL position of every CSR entry (entries of a row are at most 65,535)
+__global__ void k_sell_kof(SellMap m, unsigned short *kof) {
+  SELL_WALK(m, kof[rb + t] = (unsigned short)k;)
 }
-int remap(hipStream_t s, int64_t n, int *io, const int *table) {
-  if (n <= 0) return 0;
-  hipLaunchKernelGGL(k_remap, vgrid(n), dim3(256), 0, s, n, io, table);
+int sell_kof(hipStream_t s, const SellMap &m, unsigned short *kof) {
+  if (m.nsl <= 0) return 0;
+  hipLaunchKernelGGL(k_sell_kof, dim3((unsigned)((m.nsl + 3) / 4)), dim3(256), 0, s, m, kof);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
-// dslot[i] = SELL slot of the diagonal entry of CSR row ord[i] (-1 if absent)
+#undef SELL_WALK
+// dslot[i] = SELL slot of the diagonal entry of internal row i (CSR row ord[i], columns sorted), -1 if absent
 __global__ void k_diag_slot(int64_t n, const int64_t *__restrict__ rp, const int *__restrict__ ci,
-                            const int *__restrict__ ord, const int *__restrict__ slot_of, int *dslot) {
+                            const int *__restrict__ ord, const int64_t *__restrict__ rslot,
+                            const unsigned short *__restrict__ kof, int64_t *dslot) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t r = ord[i];
-    int p = -1;
-    for (int64_t q = rp[r]; q < rp[r + 1]; q++)
-      if (ci[q] == r) { p = slot_of[q]; break; }
+    const int r = ord[i];
+    int64_t lo = rp[r], hi = rp[r + 1] - 1, p = -1;
+    while (lo <= hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      const int c = ci[mid];
+      if (c == r) { p = rslot[i] + 64 * (int64_t)kof[mid]; break; }
+      if (c < r) lo = mid + 1; else hi = mid - 1;
+    }
     dslot[i] = p;
   }
 }
-int diag_slot(hipStream_t s, int64_t n, const int64_t *rp, const int *ci, const int *ord, const int *slot_of, int *dslot) {
+int diag_slot(hipStream_t s, int64_t n, const int64_t *rp, const int *ci, const int *ord, const int64_t *rslot,
+              const unsigned short *kof, int64_t *dslot) {
   if (n <= 0) return 0;
-  hipLaunchKernelGGL(k_diag_slot, vgrid(n), dim3(256), 0, s, n, rp, ci, ord, slot_of, dslot);
+  hipLaunchKernelGGL(k_diag_slot, vgrid(n), dim3(256), 0, s, n, rp, ci, ord, rslot, kof, dslot);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+// elslot[t] = rslot[eldofi[t]]: SELL slot of position 0 of the row of each element DoF (-1: no DoF)
+__global__ void k_elslot(int64_t n, const int *__restrict__ eldofi, const int64_t *__restrict__ rslot, int64_t *elslot) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int d = eldofi[i];
+    elslot[i] = d >= 0 ? rslot[d] : -1;
+  }
+}
+int elslot(hipStream_t s, int64_t n, const int *eldofi, const int64_t *rslot, int64_t *out) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_elslot, vgrid(n), dim3(256), 0, s, n, eldofi, rslot, out);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 // dst[i] = src[idx[i]]  (reference <-> internal DoF order)
@@ -621,28 +661,33 @@ int cast_vals(hipStream_t s, int64_t n, const double *v, float *vf) {
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-// ---- element -> CSR position map (binary search in sorted rows)
+// ---- element map: SELL position (within its row) of every element-matrix entry, found by binary search in the
+// sorted CSR row and translated through kof; 0xFFFF where the pattern has no such entry
 __global__ void k_elpos(int nel, int nloc, const int *__restrict__ eldof, const int64_t *__restrict__ rp,
-                        const int *__restrict__ ci, int *__restrict__ pos) {
+                        const int *__restrict__ ci, const unsigned short *__restrict__ kof, unsigned short *__restrict__ pos) {
   int64_t total = (int64_t)nel * nloc * nloc;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
     int64_t e = t / ((int64_t)nloc * nloc);
     int ab = (int)(t - e * nloc * nloc), a = ab / nloc, b = ab % nloc;
     int row = eldof[e * nloc + a], col = eldof[e * nloc + b];
-    int64_t lo = rp[row], hi = rp[row + 1] - 1, found = -1;
-    while (lo <= hi) {
-      int64_t mid = (lo + hi) >> 1;
-      int c = ci[mid];
-      if (c == col) { found = mid; break; }
-      if (c < col) lo = mid + 1; else hi = mid - 1;
+    unsigned short k = 0xFFFF;
+    if (row >= 0 && col >= 0) {
+      int64_t lo = rp[row], hi = rp[row + 1] - 1;
+      while (lo <= hi) {
+        int64_t mid = (lo + hi) >> 1;
+        int c = ci[mid];
+        if (c == col) { k = kof[mid]; break; }
+        if (c < col) lo = mid + 1; else hi = mid - 1;
+      }
     }
-    pos[t] = (int)found;
+    pos[t] = k;
   }
 }
-int elpos(hipStream_t s, int nel, int nloc, const int *eldof, const int64_t *rp, const int *ci, int *pos) {
+int elpos(hipStream_t s, int nel, int nloc, const int *eldof, const int64_t *rp, const int *ci,
+          const unsigned short *kof, unsigned short *pos) {
   int64_t total = (int64_t)nel * nloc * nloc;
   int64_t b = (total + 255) / 256; if (b > 65536) b = 65536;
-  hipLaunchKernelGGL(k_elpos, dim3((unsigned)b), dim3(256), 0, s, nel, nloc, eldof, rp, ci, pos);
+  hipLaunchKernelGGL(k_elpos, dim3((unsigned)b), dim3(256), 0, s, nel, nloc, eldof, rp, ci, kof, pos);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
@@ -658,21 +703,6 @@ int copy16(hipStream_t s, int64_t n16, const void *a, void *b) {
   int64_t nb = (n16 + 1023) / 1024;
   nb = nb < 1 ? 1 : (nb > 65536 ? 65536 : nb);
   hipLaunchKernelGGL(k_copy16, dim3((unsigned)nb), dim3(256), 0, s, n16, (const dvec2 *)a, (dvec2 *)b);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
-}
-
-__global__ void k_csr2sell(int64_t nslot, const int *__restrict__ map, const double *__restrict__ cv,
-                           double *__restrict__ sv, float *__restrict__ svf) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nslot; i += (int64_t)gridDim.x * blockDim.x) {
-    int q = map[i];
-    double v = q >= 0 ? cv[q] : 0.0;
-    sv[i] = v;
-    svf[i] = (float)v;
-  }
-}
-int csr2sell(hipStream_t s, int64_t nslot, const int *map, const double *csr_v, double *sv, float *svf) {
-  if (nslot <= 0) return 0;
-  hipLaunchKernelGGL(k_csr2sell, vgrid(nslot), dim3(256), 0, s, nslot, map, csr_v, sv, svf);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -32,7 +32,8 @@
 // ty*TR, columns tx*TR; each step publishes only the pivot row and column in
 // LDS (double-buffered, two barriers per step).
 template <int NL, int G, int TR>
-__global__ __launch_bounds__(G * G) void k_vanka_invert(int nel, const int *__restrict__ elpos, const double *__restrict__ Av,
+__global__ __launch_bounds__(G * G) void k_vanka_invert(int nel, const unsigned short *__restrict__ elpos,
+                                                      const int64_t *__restrict__ elslot, const double *__restrict__ Av,
                                                       double *__restrict__ vinv, float *__restrict__ vinvf,
                                                       int *__restrict__ status) {
   constexpr int NP = G * TR, NT = G * G, NW = (NT + 63) / 64;
@@ -52,8 +53,9 @@ __global__ __launch_bounds__(G * G) void k_vanka_invert(int nel, const int *__re
         const int i = i0 + a, j = j0 + b;
         double v = 0.;
         if (i < NL && j < NL) {
-          const int p = elpos[base + i * NL + j];
-          v = p >= 0 ? Av[p] : 0.0;
+          const unsigned k = elpos[base + i * NL + j];
+          const int64_t r = elslot[(int64_t)e * NL + i];
+          v = (k != 0xFFFFu && r >= 0) ? Av[r + 64 * (int64_t)k] : 0.0;
         }
         m[a][b] = v;
         amax = fmax(amax, fabs(v));
@@ -209,9 +211,9 @@ static int vanka_invert_level(fm_system *s, int L, const double *Av, double **st
   FM_HIP(hipMemsetAsync(l.vwt, 0, l.n * sizeof(double), st));
   unsigned g = (unsigned)(l.nel < 16384 ? l.nel : 16384);
   if (s->mesh->dim == 2)
-    hipLaunchKernelGGL((k_vanka_invert<22, 8, 3>), dim3(g), dim3(64), 0, st, l.nel, l.elpos, Av, *store, *storef, status);
+    hipLaunchKernelGGL((k_vanka_invert<22, 8, 3>), dim3(g), dim3(64), 0, st, l.nel, l.elpos, l.elslot, Av, *store, *storef, status);
   else
-    hipLaunchKernelGGL((k_vanka_invert<89, 16, 6>), dim3(g), dim3(256), 0, st, l.nel, l.elpos, Av, *store, *storef, status);
+    hipLaunchKernelGGL((k_vanka_invert<89, 16, 6>), dim3(g), dim3(256), 0, st, l.nel, l.elpos, l.elslot, Av, *store, *storef, status);
   FM_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_count_mult, dim3(1024), dim3(256), 0, st, l.nel, nl, l.eldofi, l.vwt);
   hipLaunchKernelGGL(k_recip, dim3(1024), dim3(256), 0, st, l.n, l.vwt);
