@@ -285,7 +285,8 @@ def dhc_workload(F, ctx, args):
     mixed = (ns.opts(precision=F.FM_PREC_FP32_FGMRES, **kw_ns), T.opts(precision=F.FM_PREC_FP32_FGMRES, **kw_T))
     fp64 = (ns.opts(precision=F.FM_PREC_FP64_FGMRES, **kw_ns), T.opts(precision=F.FM_PREC_FP64_FGMRES, **kw_T))
     b.o_ns, b.o_T = mixed
-    log(f"dhc setup {time.perf_counter() - t0:.1f}s  N_ns={Nns} N_T={NT}")
+    mem_used = device_memory_used_gb()
+    log(f"dhc setup {time.perf_counter() - t0:.1f}s  N_ns={Nns} N_T={NT}  device memory in use {mem_used} GB")
     k = 0
     for _ in range(args.dhc_spinup):
         k += 1
@@ -322,21 +323,30 @@ def dhc_workload(F, ctx, args):
     log(f"mixed step {t_step * 1e3:.1f} ms (NS MG {np.median(smn) * 1e3:.1f}, T MG {np.median(smT) * 1e3:.1f}) its {itsm}; "
         f"fp64 step {np.median(w64) * 1e3:.1f} ms (NS MG {np.median(s64n) * 1e3:.1f}, T MG {np.median(s64T) * 1e3:.1f}) "
         f"its {its64}; err {errs}")
-    t1 = time.perf_counter()
-    nsteady = b.run(max_steps=args.dhc_steady_steps, tol=1e-8)
-    t_steady = time.perf_counter() - t1
-    log(f"steady after {nsteady} more steps ({t_steady:.1f} s)")
-    t1 = time.perf_counter()
-    Nu, grad, cinfo = b.nusselt_gradient_coupled(
-        tol=1e-8, callback=lambda it, r: log(f"coupled adjoint FGMRES {it}: {r:.2e}"))
-    ctx.sync()
-    t_grad = time.perf_counter() - t1
-    t1 = time.perf_counter()
-    Nu_f, grad_f, _ = b.nusselt_gradient_k()
-    ctx.sync()
-    t_grad_f = time.perf_counter() - t1
-    log(f"Nu {Nu:.5f}; coupled adjoint {t_grad * 1e3:.1f} ms ({cinfo['fgmres_its']} its); frozen-velocity adjoint {t_grad_f * 1e3:.1f} ms; "
-        f"|g_frozen - g| / |g| = {np.linalg.norm(grad_f - grad) / np.linalg.norm(grad):.3f}")
+    adj = {}
+    nsteady = None
+    if args.dhc_steady_steps > 0:   # 0: the timed steps only (no steady state, no adjoint)
+        t1 = time.perf_counter()
+        nsteady = b.run(max_steps=args.dhc_steady_steps, tol=1e-8)
+        t_steady = time.perf_counter() - t1
+        log(f"steady after {nsteady} more steps ({t_steady:.1f} s)")
+        t1 = time.perf_counter()
+        Nu, grad, cinfo = b.nusselt_gradient_coupled(
+            tol=1e-8, callback=lambda it, r: log(f"coupled adjoint FGMRES {it}: {r:.2e}"))
+        ctx.sync()
+        t_grad = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        Nu_f, grad_f, _ = b.nusselt_gradient_k()
+        ctx.sync()
+        t_grad_f = time.perf_counter() - t1
+        rel = float(np.linalg.norm(grad_f - grad) / np.linalg.norm(grad))
+        log(f"Nu {Nu:.5f}; coupled adjoint {t_grad * 1e3:.1f} ms ({cinfo['fgmres_its']} its); frozen-velocity adjoint "
+            f"{t_grad_f * 1e3:.1f} ms; |g_frozen - g| / |g| = {rel:.3f}")
+        adj = {"nusselt": Nu, "nusselt_grad_coupled_ms": round(t_grad * 1e3, 2),
+               "nusselt_grad_coupled_fgmres_its": cinfo["fgmres_its"],
+               "nusselt_grad_coupled_inner_its": [list(x) for x in cinfo["inner_its"]],
+               "nusselt_grad_norm": float(np.linalg.norm(grad)),
+               "nusselt_grad_frozen_velocity_ms": round(t_grad_f * 1e3, 2), "frozen_vs_coupled_rel_diff": rel}
     nnz = int(F.lib().fm_system_csr_nnz(ns.h, 0, top))
     sec32 = ns.bench_spmv(top, 1, args.spmv_reps)
     b32 = spmv_bytes(Nns, nnz, 4)
@@ -361,18 +371,27 @@ def dhc_workload(F, ctx, args):
            "speedup_vs_fp64": round(float(np.median(w64)) / t_step, 3),
            "ns_solve_speedup_vs_fp64": round(float(np.median(s64n)) / float(np.median(smn)), 3),
            "rel_l2_err_vs_fp64": errs,
-           "nusselt": Nu, "steady_after_steps": nsteady,
-           "nusselt_grad_coupled_ms": round(t_grad * 1e3, 2), "nusselt_grad_coupled_fgmres_its": cinfo["fgmres_its"],
-           "nusselt_grad_coupled_inner_its": [list(x) for x in cinfo["inner_its"]],
-           "nusselt_grad_norm": float(np.linalg.norm(grad)),
-           "nusselt_grad_frozen_velocity_ms": round(t_grad_f * 1e3, 2),
-           "frozen_vs_coupled_rel_diff": float(np.linalg.norm(grad_f - grad) / np.linalg.norm(grad)),
+           "steady_after_steps": nsteady, **adj,
+           "device_memory_used_GB": mem_used,
            "roofline": {"bound": "hbm", "achieved": round(gbs32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(gbs32 / HBM_PEAK_GBS, 4), "traffic": None, "bytes_alg": b32,
                         "kernel": f"k_sell<float> y = A x ({F.SPMV_FORMAT}) on the top Navier-Stokes matrix"}}
     ns.close()
     T.close()
     return out
+
+
+def device_memory_used_gb():
+    """Device memory in use on the current device (hipMemGetInfo), in GB."""
+    import ctypes as C
+    try:
+        hip = C.CDLL("libamdhip64.so")
+    except OSError:
+        return None
+    free, total = C.c_size_t(), C.c_size_t()
+    if hip.hipMemGetInfo(C.byref(free), C.byref(total)) != 0:
+        return None
+    return round((total.value - free.value) / 1e9, 1)
 
 
 def field_ids(s):

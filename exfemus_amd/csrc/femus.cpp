@@ -367,6 +367,15 @@ static SellMap sell_map(const DevSELL &q) {
 
 int fm_csr_from_sell(hipStream_t st, DevCSR &c) {
   if (c.sell.nsl == 0) return 0;
+  if (!c.v) {  // first request: the CSR value arrays are allocated here
+    FM_CHECK(dalloc(&c.v, c.nnz));
+    FM_CHECK(dalloc(&c.vf, c.nnz));
+  }
+  if (!c.ci) {  // column indices of an assembled matrix live on the host after set-up; uploaded on request
+    if ((int64_t)c.h_ci.size() != c.nnz) { fm_set_error("CSR columns missing"); return FM_ESTATE; }
+    FM_CHECK(dalloc(&c.ci, c.nnz));
+    FM_HIP(hipMemcpyAsync(c.ci, c.h_ci.data(), c.nnz * sizeof(int), hipMemcpyHostToDevice, st));
+  }
   FM_CHECK(fmk::sell2csr(st, sell_map(c.sell), c.sell.v, c.v));
   FM_CHECK(fmk::cast_vals(st, c.nnz, c.v, c.vf));
   return 0;
@@ -390,6 +399,10 @@ static int sell_slots(hipStream_t st, DevCSR &A, const int *ord, int nel, int nl
 // refresh SELL values (FP64 and FP32) from the CSR values
 int fm_sync_sell(hipStream_t st, DevCSR &c) {
   if (c.sell.nsl == 0) return 0;
+  if (!c.v) {  // no CSR values yet: the SELL values start at zero
+    FM_CHECK(fmk::fill<double>(st, c.sell.nslot, c.sell.v, 0.0));
+    return fmk::fill<float>(st, c.sell.nslot, c.sell.vf, 0.f);
+  }
   return fmk::csr2sell(st, sell_map(c.sell), c.v, c.sell.v, c.sell.vf);
 }
 
@@ -473,16 +486,13 @@ static int upload_csr(hipStream_t st, DevCSR &c, int64_t nr, int64_t nc, std::ve
   c.nr = nr; c.nc = nc; c.nnz = rp[nr];
   FM_CHECK(dalloc(&c.rp, nr + 1));
   FM_CHECK(dalloc(&c.ci, c.nnz));
-  FM_CHECK(dalloc(&c.v, c.nnz));
-  FM_CHECK(dalloc(&c.vf, c.nnz));
   FM_HIP(hipMemcpyAsync(c.rp, rp.data(), (nr + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
   FM_HIP(hipMemcpyAsync(c.ci, ci.data(), c.nnz * sizeof(int), hipMemcpyHostToDevice, st));
-  if (v) {
+  if (v) {  // fixed values (transfers); an assembled matrix gets its CSR values only on request (fm_csr_from_sell)
+    FM_CHECK(dalloc(&c.v, c.nnz));
+    FM_CHECK(dalloc(&c.vf, c.nnz));
     FM_HIP(hipMemcpyAsync(c.v, v->data(), c.nnz * sizeof(double), hipMemcpyHostToDevice, st));
     FM_CHECK(fmk::cast_vals(st, c.nnz, c.v, c.vf));
-  } else {
-    FM_HIP(hipMemsetAsync(c.v, 0, c.nnz * sizeof(double), st));
-    FM_HIP(hipMemsetAsync(c.vf, 0, c.nnz * sizeof(float), st));
   }
   FM_HIP(hipStreamSynchronize(st));
   c.h_rp.swap(rp);
@@ -724,6 +734,7 @@ extern "C" int fm_system_init(fm_system *s) {
     dfree(l.dslot);
     FM_CHECK(dalloc(&l.dslot, l.n));
     FM_CHECK(sell_slots(st, l.A, l.ord, l.nel, nloc, l.eldof, l.eldofi, l.elpos, l.elslot, l.dslot));
+    dfree(l.A.ci);  // the solver runs on the SELL copy; CSR requests re-upload the columns (fm_csr_from_sell)
     FM_HIP(hipStreamSynchronize(st));
   }
   fm_drop_graphs(s);
@@ -752,7 +763,10 @@ extern "C" int fm_system_csr_get(const fm_system *s, int which, int L, int64_t *
   if (which == 0 && val) FM_CHECK(fm_csr_from_sell(s->ctx->stream, const_cast<DevCSR &>(c)));
   FM_HIP(hipStreamSynchronize(s->ctx->stream));
   if (rowptr) FM_HIP(hipMemcpy(rowptr, c.rp, (c.nr + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
-  if (colind) FM_HIP(hipMemcpy(colind, c.ci, c.nnz * sizeof(int), hipMemcpyDeviceToHost));
+  if (colind) {
+    if (c.ci) FM_HIP(hipMemcpy(colind, c.ci, c.nnz * sizeof(int), hipMemcpyDeviceToHost));
+    else std::copy(c.h_ci.begin(), c.h_ci.end(), colind);
+  }
   if (val) FM_HIP(hipMemcpy(val, c.v, c.nnz * sizeof(double), hipMemcpyDeviceToHost));
   return FM_OK;
 }
@@ -1101,6 +1115,7 @@ static int ensure_adjoint(fm_system *s) {
     if (a.A.sell.nslot != l.A.sell.nslot) { fm_set_error("adjoint SELL layout differs"); return FM_ESTATE; }
     FM_CHECK(dalloc(&a.dslot, a.n));
     FM_CHECK(sell_slots(st, a.A, a.ord, 0, 0, nullptr, nullptr, nullptr, nullptr, a.dslot));
+    dfree(a.A.ci);
     if (L < top) {  // restriction of the adjoint = (prolongation of level L+1)^T
       const DevCSR &P = s->lev[L + 1].P;
       std::vector<int64_t> rpT; std::vector<int> ciT; std::vector<double> vT;
