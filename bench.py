@@ -53,8 +53,9 @@ def build_heat(F, ctx, nx0, nlev, k_sigma=1.0):
 
 def assembly_rate(F, ctx, s, heat, reps=5):
     """Top-level GenMatRhs (matrix + rhs) timed alone: elements/s and algorithmic GB/s, counting per element
-    matrix entry the 4-byte SELL slot read and the 8-byte value read-modify-write (the scatter), plus the
-    element's coordinates and nodal data (Q2 nodes x dim coordinates + nodal values, 8 B each)."""
+    matrix entry the 2-byte position in its SELL row and the 8-byte value read-modify-write (the scatter), per
+    element DoF the 8-byte row slot, plus the element's coordinates and nodal data (Q2 nodes x dim
+    coordinates + nodal values, 8 B each)."""
     m = s.mesh
     top = m.nlev - 1
     nel = m.level(top)[0]
@@ -68,7 +69,7 @@ def assembly_rate(F, ctx, s, heat, reps=5):
         ctx.sync()
         ts.append(time.perf_counter() - t0)
     t = float(np.median(ts))
-    bytes_el = nl * nl * (4 + 2 * 8) + n2 * (m.dim + (1 if heat else m.dim)) * 8
+    bytes_el = nl * nl * (2 + 2 * 8) + nl * 8 + n2 * (m.dim + (1 if heat else m.dim)) * 8
     return {"elements_top": nel, "ms_top": round(t * 1e3, 3), "Melements_per_s": round(nel / t / 1e6, 2),
             "GBs_alg": round(nel * bytes_el / t / 1e9, 1), "bytes_per_element": bytes_el}
 
