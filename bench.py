@@ -567,7 +567,8 @@ def main():
         "spmv": {"fp64_us": round(sec64 * 1e6, 2), "fp64_GBs": round(gbs64, 1), "fp64_frac": round(gbs64 / HBM_PEAK_GBS, 4),
                  "fp32_us": round(sec32 * 1e6, 2), "fp32_GBs": round(gbs32, 1), "fp32_frac": round(gbs32 / HBM_PEAK_GBS, 4),
                  "bytes_alg_fp64": b64, "bytes_alg_fp32": b32,
-                 "stream_copy_GBs": round(gbs_copy, 1), "fp64_frac_of_stream": round(gbs64 / gbs_copy, 4)},
+                 "stream_copy_GBs": round(gbs_copy, 1), "fp64_frac_of_stream": round(gbs64 / gbs_copy, 4),
+                 "storage": s.sell_info(0, top)},
         # the mixed solve's dominant kernel is the FP32 SELL SpMV of the V-cycle.  achieved / frac count the
         # algorithmic bytes of a CSR product with int32 columns; the stored offsets are 8 or 16 bits wide, so
         # the measured HBM traffic per launch (PMC) is lower and its rate is reported beside them

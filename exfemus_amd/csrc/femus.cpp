@@ -226,7 +226,9 @@ static void free_csr(DevCSR &c) { fm_free_csr(c); }
 // internal rows) start their own slices, so a product can be restricted to
 // the slices of chosen groups.
 int fm_build_sell(hipStream_t st, DevCSR &c, const int *ro, const int *cr, const std::vector<int64_t> *groups) {
-  const int C = 64, W = 4096;
+  // W: rows are length-sorted inside windows of W rows (FEMUS_SELL_SIGMA, default 4096; 64 = inside a slice)
+  const char *sg = getenv("FEMUS_SELL_SIGMA");
+  const int C = 64, W = sg && atoi(sg) >= 64 ? atoi(sg) : 4096;
   const int64_t n = c.nr;
   const std::vector<int64_t> &rp = c.h_rp;
   const std::vector<int> &ci = c.h_ci;
@@ -752,6 +754,15 @@ extern "C" int64_t fm_system_csr_nnz(const fm_system *s, int which, int L) {
   const Level &l = s->lev[L];
   const DevCSR &c = which == 0 ? l.A : (which == 1 ? l.R : l.P);
   return c.nnz;
+}
+
+extern "C" int fm_system_sell_info(const fm_system *s, int which, int L, int64_t *info) {
+  if (!s || !info || L < 0 || L >= (int)s->lev.size() || !s->initialized) return FM_EINVAL;
+  const Level &l = s->lev[L];
+  const DevCSR &c = which == 0 ? l.A : (which == 1 ? l.R : l.P);
+  info[0] = c.sell.nsl; info[1] = c.sell.nslot; info[2] = c.sell.n8; info[3] = c.sell.n16; info[4] = c.sell.n32;
+  info[5] = c.nnz;
+  return FM_OK;
 }
 
 extern "C" int fm_system_csr_get(const fm_system *s, int which, int L, int64_t *rowptr, int *colind, double *val) {

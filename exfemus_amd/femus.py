@@ -42,7 +42,7 @@ EXPORTS = [
     "fm_heat_grad_coef", "fm_bench_spmv", "fm_sync", "fm_heat_genmat_transpose", "fm_ns_genmat_transpose", "fm_ns_genjac",
     "fm_get_adjoint_csr", "fm_heat_dissipation", "fm_bench_stream", "fm_ns_dissipation", "fm_ns_grad_coef",
     "fm_system_set_bcdata", "fm_heat_bc_rhs", "fm_ns_bc_rhs", "fm_ns_pressure_drop",
-    "fm_ns_set_buoyancy", "fm_heat_nusselt", "fm_heat_coupling", "fm_adjoint_spmv", "fm_set_sell_split",
+    "fm_ns_set_buoyancy", "fm_heat_nusselt", "fm_heat_coupling", "fm_adjoint_spmv", "fm_set_sell_split", "fm_system_sell_info",
 ]
 
 
@@ -92,6 +92,7 @@ def lib():
     pp = C.POINTER(C.c_void_p)
     L.fm_last_error.restype = C.c_char_p
     L.fm_set_sell_split.argtypes = [C.c_int]
+    L.fm_system_sell_info.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
     L.fm_ctx_create.argtypes = [C.c_int, pp]
     L.fm_ctx_destroy.argtypes = [vp]
     L.fm_sync.argtypes = [vp]
@@ -357,6 +358,12 @@ class MGSolDA:
     # -- LA operators (NumericVectorM::resid / matrix_mult) -----------------------
     def spmv(self, L, x):
         y = np.zeros(self._Dim[L]); _ck(lib().fm_spmv(self.h, L, np.ascontiguousarray(x, np.float64), y)); return y
+
+    def sell_info(self, which, L):
+        """SELL storage of matrix `which` (0 A, 1 R, 2 P) on level L: slices, slots, 8/16/32-bit offset slots, nnz."""
+        info = np.zeros(6, np.int64)
+        _ck(lib().fm_system_sell_info(self.h, which, L, info.ctypes.data_as(C.c_void_p)))
+        return dict(zip(("slices", "slots", "slots_c8", "slots_c16", "slots_c32", "nnz"), (int(v) for v in info)))
 
     def resid(self, L):
         _ck(lib().fm_residual(self.h, L))

@@ -89,6 +89,10 @@ int fm_system_init(fm_system *s);
 int64_t fm_system_csr_nnz(const fm_system *s, int which, int L);
 int fm_system_csr_get(const fm_system *s, int which, int L, int64_t *rowptr, int *colind,
                       double *val);
+/* SELL storage of matrix `which` (0 A, 1 R, 2 P) on level L (no counterpart in the reference, whose
+ * LASPACK rows are per-row arrays): info = {slices of 64 rows, slots (entries + padding), slots with 8-bit,
+ * 16-bit and 32-bit column offsets, nonzeros}. */
+int fm_system_sell_info(const fm_system *s, int which, int L, int64_t info[6]);
 
 /* x_old (top level, DoF order): SparseMatrix fields read by GenMatRhs      */
 int fm_system_set_xold(fm_system *s, const double *x_old_top);
