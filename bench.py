@@ -333,7 +333,7 @@ def dhc_workload(F, ctx, args):
         log(f"steady after {nsteady} more steps ({t_steady:.1f} s)")
         t1 = time.perf_counter()
         Nu, grad, cinfo = b.nusselt_gradient_coupled(
-            tol=1e-8, callback=lambda it, r: log(f"coupled adjoint FGMRES {it}: {r:.2e}"))
+            tol=1e-8, callback=lambda it, r: log(f"coupled adjoint FGMRES {it}: {r:.2e}"), lean=True)
         ctx.sync()
         t_grad = time.perf_counter() - t1
         t1 = time.perf_counter()

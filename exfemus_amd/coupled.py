@@ -209,7 +209,7 @@ class Boussinesq:
         return self.T.nusselt(self.xT, axis=0)
 
     def nusselt_gradient_coupled(self, tol=1e-10, maxit=100, inner_tol=1e-4, inner_maxit=15,
-                                 inner_precision=F.FM_PREC_FP32_FGMRES, callback=None):
+                                 inner_precision=F.FM_PREC_FP32_FGMRES, callback=None, lean=False):
         """dNu/dk_e of the steady coupled state, through the adjoint of the whole coupled system (velocity not
         frozen): with R_NS(U, T) = 0, R_T(T, U) = 0 at the state,
             [ J_NS^T     (dR_T/dU)^T ] [lam_U]   [dNu/dU]
@@ -218,8 +218,18 @@ class Boussinesq:
         (buoyancy), (dR_T/dU) the derivative of the advection.  Solved by flexible GMRES with the two transposed
         multigrid solves as a block-diagonal preconditioner.  The upwind terms of both systems are differentiated
         (fm_ns_genjac, fm_heat_coupling); the state must be steady (the time terms are left out of the Jacobian).
-        Returns (Nu, gradient, info)."""
+        lean: release the forward NS multigrid data first and keep the transposed hierarchy's Vanka inverses in
+        the inner solves' precision only (the memory of the largest meshes).  Returns (Nu, gradient, info)."""
         ns, T = self.ns, self.T
+        if lean:
+            ns.release_mg()
+            fp32 = inner_precision in (F.FM_PREC_FP32_IR, F.FM_PREC_FP32_FGMRES)
+            store0 = ns.set_vanka_store(2 if fp32 else 1)
+            try:
+                return self.nusselt_gradient_coupled(tol, maxit, inner_tol, inner_maxit, inner_precision, callback)
+            finally:
+                ns.release_mg(adjoint=True)
+                ns.set_vanka_store(store0)
         top = self.mesh.nlev - 1
         N, dim = self.mesh.N, self.dim
         U = ns.get_x()

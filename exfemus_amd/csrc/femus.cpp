@@ -206,6 +206,11 @@ extern "C" int fm_system_create(fm_ctx *ctx, const fm_mesh *m, int nq2, int nq1,
 }
 
 static void free_sell(DevSELL &q) {
+  if (q.borrowed) {  // only the values are this matrix's own
+    dfree(q.v); dfree(q.vf);
+    q = DevSELL();
+    return;
+  }
   dfree(q.sl_off); dfree(q.sl_c32); dfree(q.sl_w); dfree(q.perm); dfree(q.c16); dfree(q.base); dfree(q.c32);
   dfree(q.sl_c8); dfree(q.c8); dfree(q.sl_c16);
   dfree(q.pos); dfree(q.rbase); dfree(q.rslot); dfree(q.v); dfree(q.vf);
@@ -526,7 +531,7 @@ void fm_free_level(Level &l) {
   dfree(l.coef);
   dfree(l.ord); dfree(l.rank);
   dfree(l.Ainv); dfree(l.Ainvf); dfree(l.vinv); dfree(l.vinvf); dfree(l.vwt); dfree(l.vwtf); dfree(l.nrm); dfree(l.res);
-  dfree(l.sb); dfree(l.sx); dfree(l.vone);
+  dfree(l.sb); dfree(l.sx); dfree(l.vone); dfree(l.cslist); dfree(l.dslot);
 }
 
 // Per colour: the SELL slices of the row groups whose signature contains the
@@ -1102,6 +1107,25 @@ static void transpose_host(int64_t nr, int64_t nc, const std::vector<int64_t> &r
     for (int64_t q = rp[i]; q < rp[i + 1]; q++) { int64_t t = f[ci[q]]++; ciT[t] = (int)i; vT[t] = v[q]; }
 }
 
+// A matrix with the pattern of fwd (the transposed operator of a structurally symmetric pattern): its own row
+// pointers and SELL values, fwd's SELL layout (column offsets, slot maps, row order) borrowed, the CSR columns
+// kept on the host until a caller asks for the CSR form.
+static int share_pattern(hipStream_t st, DevCSR &c, const DevCSR &fwd) {
+  free_csr(c);
+  c.nr = fwd.nr; c.nc = fwd.nc; c.nnz = fwd.nnz;
+  FM_CHECK(dalloc(&c.rp, c.nr + 1));
+  FM_HIP(hipMemcpyAsync(c.rp, fwd.rp, (c.nr + 1) * sizeof(int64_t), hipMemcpyDeviceToDevice, st));
+  c.h_rp = fwd.h_rp;
+  c.h_ci = fwd.h_ci;
+  c.sell = fwd.sell;
+  c.sell.borrowed = true;
+  c.sell.v = nullptr;
+  c.sell.vf = nullptr;
+  FM_CHECK(dalloc(&c.sell.v, c.sell.nslot));
+  FM_CHECK(dalloc(&c.sell.vf, c.sell.nslot));
+  return fm_sync_sell(st, c);  // no CSR values: zeros
+}
+
 static int ensure_adjoint(fm_system *s) {
   if (!s->alev.empty()) return 0;
   hipStream_t st = s->ctx->stream;
@@ -1117,16 +1141,12 @@ static int ensure_adjoint(fm_system *s) {
     FM_CHECK(dalloc(&a.ord, a.n)); FM_CHECK(dalloc(&a.rank, a.n));
     FM_HIP(hipMemcpyAsync(a.ord, l.h_ord.data(), a.n * sizeof(int), hipMemcpyHostToDevice, st));
     FM_HIP(hipMemcpyAsync(a.rank, l.h_rank.data(), a.n * sizeof(int), hipMemcpyHostToDevice, st));
-    std::vector<int64_t> rp = l.A.h_rp;
-    std::vector<int> ci = l.A.h_ci;
     a.h_goff = l.h_goff;
     a.h_gmask = l.h_gmask;
-    FM_CHECK(upload_csr(st, a.A, l.n, l.n, rp, ci, nullptr, l.h_ord, l.h_rank, &a.h_goff));
+    FM_CHECK(share_pattern(st, a.A, l.A));  // A^T has A's pattern: same SELL layout, own values
     FM_CHECK(colour_slices(st, a));
-    if (a.A.sell.nslot != l.A.sell.nslot) { fm_set_error("adjoint SELL layout differs"); return FM_ESTATE; }
     FM_CHECK(dalloc(&a.dslot, a.n));
-    FM_CHECK(sell_slots(st, a.A, a.ord, 0, 0, nullptr, nullptr, nullptr, nullptr, a.dslot));
-    dfree(a.A.ci);
+    FM_HIP(hipMemcpyAsync(a.dslot, l.dslot, a.n * sizeof(int64_t), hipMemcpyDeviceToDevice, st));
     if (L < top) {  // restriction of the adjoint = (prolongation of level L+1)^T
       const DevCSR &P = s->lev[L + 1].P;
       std::vector<int64_t> rpT; std::vector<int> ciT; std::vector<double> vT;

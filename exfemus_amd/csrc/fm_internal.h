@@ -51,6 +51,7 @@ struct DevSELL {
   // (pos 0xFFFF: padding); rslot[i]: slot of position 0 of internal row i
   unsigned short *pos = nullptr;
   int64_t *rbase = nullptr, *rslot = nullptr;
+  bool borrowed = false;  // layout arrays belong to another matrix of the same pattern (only v / vf are owned)
   unsigned short *c16 = nullptr;
   unsigned char *c8 = nullptr;
   double *v = nullptr;
@@ -166,6 +167,7 @@ struct fm_system {
   bool initialized = false;
   bool mg_ready = false;
   int mg_smoother = -1;
+  int vanka_store = 3;  // precisions of the stored Vanka inverses: bit 0 FP64 (FP64 cycles), bit 1 FP32 (FP32 cycles)
 };
 
 // gencase.cpp

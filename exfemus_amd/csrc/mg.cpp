@@ -312,6 +312,27 @@ extern "C" int fm_mg_setup(fm_system *s, const fm_mg_opts *o) {
   return FM_OK;
 }
 
+extern "C" int fm_system_set_vanka_store(fm_system *s, int mask) {
+  if (!s || mask < 0 || mask > 3) return FM_EINVAL;
+  const int old = s->vanka_store;
+  if (mask > 0) s->vanka_store = mask;
+  return old;
+}
+
+extern "C" int fm_mg_release(fm_system *s, int adjoint) {
+  if (!s) return FM_EINVAL;
+  FM_HIP(hipSetDevice(s->ctx->device));
+  FM_HIP(hipStreamSynchronize(s->ctx->stream));
+  fm_drop_graphs(s);  // captured cycles hold the addresses released here
+  for (Level &l : adjoint ? s->alev : s->lev) {
+    for (double **p : {&l.vinv, &l.Ainv}) if (*p) { (void)hipFree(*p); *p = nullptr; }
+    for (float **p : {&l.vinvf, &l.Ainvf}) if (*p) { (void)hipFree(*p); *p = nullptr; }
+  }
+  if (adjoint) s->adj_ready = false;
+  else s->mg_ready = false;
+  return FM_OK;
+}
+
 static int mg_setup_ops(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &ops, bool adjoint) {
   (void)adjoint;
   for (size_t L = 1; L < ops.size(); L++) {
@@ -589,12 +610,31 @@ done:
   return rc;
 }
 
+// Vanka smoothing needs the patch inverses in the cycle's precision: 0 when they are there, else 1
+static int vanka_store_missing(const fm_mg_opts *o, const std::vector<LOps> &ops) {
+  if (o->smoother != FM_SMOOTH_VANKA) return 0;
+  const bool fp32 = o->precision == FM_PREC_FP32_IR || o->precision == FM_PREC_FP32_FGMRES;
+  for (size_t L = 1; L < ops.size(); L++)
+    if (fp32 ? !ops[L].vinvf : !ops[L].vinv) return 1;
+  return 0;
+}
+static int vanka_store_error(const fm_mg_opts *o) {
+  const bool fp32 = o->precision == FM_PREC_FP32_IR || o->precision == FM_PREC_FP32_FGMRES;
+  fm_set_error(std::string("Vanka inverses for ") + (fp32 ? "FP32" : "FP64") +
+               " cycles are not kept (fm_system_set_vanka_store)");
+  return FM_ESTATE;
+}
+
 extern "C" int fm_mg_solve(fm_system *s, const fm_mg_opts *o, fm_mg_report *rep) {
   if (!s || !o || !rep) return FM_EINVAL;
   if (!s->initialized) return FM_ESTATE;
   FM_HIP(hipSetDevice(s->ctx->device));
   std::memset(rep, 0, sizeof(*rep));
   if (!s->mg_ready || s->mg_smoother != o->smoother) FM_CHECK(fm_mg_setup(s, o));
+  if (vanka_store_missing(o, forward_ops(s))) {
+    FM_CHECK(fm_mg_setup(s, o));  // e.g. the store mask changed since the last set-up
+    if (vanka_store_missing(o, forward_ops(s))) return vanka_store_error(o);
+  }
   std::vector<LOps> ops = forward_ops(s);
   Timer tm(s->ctx);
   int rc;
@@ -644,9 +684,10 @@ extern "C" int fm_adjoint_solve(fm_system *s, const fm_mg_opts *o, const double 
     op.n = a.n; op.A = &a.A; op.R = &a.R; op.P = &a.P; op.dinv = a.dinv; op.dinvf = a.dinvf; op.lam = a.lam;
     op.Ainv = a.Ainv; op.Ainvf = a.Ainvf; op.v = &a; op.vinv = a.vinv; op.vinvf = a.vinvf;
   }
-  if (!s->adj_ready || s->adj_smoother != o->smoother) {
+  if (!s->adj_ready || s->adj_smoother != o->smoother || vanka_store_missing(o, ops)) {
     fm_drop_graphs(s);
     FM_CHECK(mg_setup_ops(s, o, ops, true));
+    if (vanka_store_missing(o, ops)) return vanka_store_error(o);
     for (size_t L = 0; L < s->alev.size(); L++) {
       s->alev[L].lam = ops[L].lam; s->alev[L].Ainv = ops[L].Ainv; s->alev[L].Ainvf = ops[L].Ainvf;
     }

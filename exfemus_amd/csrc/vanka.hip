@@ -128,7 +128,10 @@ __global__ __launch_bounds__(G * G) void k_vanka_invert(int nel, const unsigned 
     }
     if (singular) {
       if (tid == 0) atomicAdd(status, 1);
-      for (int t = tid; t < NL * NL; t += NT) { vinv[base + t] = 0.0; vinvf[base + t] = 0.f; }
+      for (int t = tid; t < NL * NL; t += NT) {
+        if (vinv) vinv[base + t] = 0.0;
+        if (vinvf) vinvf[base + t] = 0.f;
+      }
       continue;
     }
     __syncthreads();
@@ -140,8 +143,8 @@ __global__ __launch_bounds__(G * G) void k_vanka_invert(int nel, const unsigned 
         const int i = i0 + a, j = j0 + b;
         if (i < NL && j < NL) {
           const int64_t o = base + (int64_t)s_prow[j] * NL + s_rowstep[i];
-          vinv[o] = m[a][b];
-          vinvf[o] = (float)m[a][b];
+          if (vinv) vinv[o] = m[a][b];
+          if (vinvf) vinvf[o] = (float)m[a][b];
         }
       }
   }
@@ -192,10 +195,15 @@ static int vanka_invert_level(fm_system *s, int L, const double *Av, double **st
   hipStream_t st = s->ctx->stream;
   const int nl = s->nloc;
   if (s->nq1 == 0) { fm_set_error("Vanka needs a Q2-Q1 (velocity-pressure) system"); return FM_EINVAL; }
-  if (!*store) {
+  // the inverses are kept in the precisions the system's store mask names (bit 0 FP64, bit 1 FP32); a store
+  // left out is released
+  const bool k64 = (s->vanka_store & 1) != 0, k32 = (s->vanka_store & 2) != 0;
+  if (!k64 && *store) { (void)hipFree(*store); *store = nullptr; }
+  if (!k32 && *storef) { (void)hipFree(*storef); *storef = nullptr; }
+  if (k64 && !*store) {
     if (hipMalloc((void **)store, (size_t)l.nel * nl * nl * sizeof(double)) != hipSuccess) { fm_set_error("Vanka alloc"); return FM_ENOMEM; }
   }
-  if (!*storef) {
+  if (k32 && !*storef) {
     if (hipMalloc((void **)storef, (size_t)l.nel * nl * nl * sizeof(float)) != hipSuccess) { fm_set_error("Vanka alloc"); return FM_ENOMEM; }
   }
   if (!l.vwt) {

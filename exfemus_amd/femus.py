@@ -43,6 +43,7 @@ EXPORTS = [
     "fm_get_adjoint_csr", "fm_heat_dissipation", "fm_bench_stream", "fm_ns_dissipation", "fm_ns_grad_coef",
     "fm_system_set_bcdata", "fm_heat_bc_rhs", "fm_ns_bc_rhs", "fm_ns_pressure_drop",
     "fm_ns_set_buoyancy", "fm_heat_nusselt", "fm_heat_coupling", "fm_adjoint_spmv", "fm_set_sell_split", "fm_system_sell_info",
+    "fm_system_set_vanka_store", "fm_mg_release",
 ]
 
 
@@ -93,6 +94,8 @@ def lib():
     L.fm_last_error.restype = C.c_char_p
     L.fm_set_sell_split.argtypes = [C.c_int]
     L.fm_system_sell_info.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+    L.fm_system_set_vanka_store.argtypes = [C.c_void_p, C.c_int]
+    L.fm_mg_release.argtypes = [C.c_void_p, C.c_int]
     L.fm_ctx_create.argtypes = [C.c_int, pp]
     L.fm_ctx_destroy.argtypes = [vp]
     L.fm_sync.argtypes = [vp]
@@ -358,6 +361,17 @@ class MGSolDA:
     # -- LA operators (NumericVectorM::resid / matrix_mult) -----------------------
     def spmv(self, L, x):
         y = np.zeros(self._Dim[L]); _ck(lib().fm_spmv(self.h, L, np.ascontiguousarray(x, np.float64), y)); return y
+
+    def set_vanka_store(self, mask):
+        """Precisions in which the Vanka patch inverses are kept (bit 0 FP64, bit 1 FP32); returns the previous mask."""
+        rc = lib().fm_system_set_vanka_store(self.h, int(mask))
+        if rc < 0:
+            _ck(rc)
+        return rc
+
+    def release_mg(self, adjoint=False):
+        """Free the patch and coarse inverses of the forward (or transposed) hierarchy."""
+        _ck(lib().fm_mg_release(self.h, 1 if adjoint else 0))
 
     def sell_info(self, which, L):
         """SELL storage of matrix `which` (0 A, 1 R, 2 P) on level L: slices, slots, 8/16/32-bit offset slots, nnz."""

@@ -183,6 +183,14 @@ typedef struct {
 
 int fm_mg_setup(fm_system *s, const fm_mg_opts *o);   /* eigen bounds, coarse LU, Vanka inverses */
 int fm_mg_solve(fm_system *s, const fm_mg_opts *o, fm_mg_report *rep);   /* x[top] in/out, b[top] */
+/* Memory controls of the multigrid data (no counterpart in the reference, whose smoother
+ * keeps no patch inverses).  fm_system_set_vanka_store: precisions in which set-up keeps the
+ * Vanka patch inverses, bit 0 FP64 (FP64 cycles), bit 1 FP32 (FP32 cycles), default 3; 0 only
+ * reads it; returns the previous mask.  A solve whose cycles need a precision left out fails
+ * with FM_ESTATE.  fm_mg_release: frees the patch and coarse inverses of the forward
+ * (adjoint = 0) or transposed (adjoint = 1) hierarchy; the next solve sets them up again. */
+int fm_system_set_vanka_store(fm_system *s, int mask);
+int fm_mg_release(fm_system *s, int adjoint);
 /* Diagnostic: the FP64 local inverse K_e^-1 the Vanka smoother applies for element e on level L
  * (after fm_mg_setup with FM_SMOOTH_VANKA), row-major nloc x nloc, and the patch DoFs in reference
  * numbering, local order [u, v, (w), p] (the local solves of VankaIter, contrib/laspack/itersolv.c:723-798). */

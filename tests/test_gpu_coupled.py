@@ -170,6 +170,11 @@ def test_coupled_nusselt_adjoint_fd(femus, ctx, up_ns, up_T):
     Nu, grad, info = b.nusselt_gradient_coupled()
     print("coupled adjoint:", Nu, info, flush=True)
     assert abs(Nu - b.nusselt()[0]) < 1e-12 * Nu
+    # the memory-lean variant (forward NS multigrid data released, transposed Vanka inverses kept in FP32 only)
+    # runs the same iterations
+    Nu2, grad2, info2 = b.nusselt_gradient_coupled(lean=True)
+    assert info2["fgmres_its"] == info["fgmres_its"] and Nu2 == Nu
+    assert np.linalg.norm(grad2 - grad) <= 1e-12 * np.linalg.norm(grad)
 
     def Nu_of(kt):
         set_k(kt)

@@ -145,6 +145,35 @@ def test_ns_mg_vanka_solution(femus, oracle, ctx, case, dim, nx, nlev, prec):
     assert ev < 1e-7 and ep < 1e-6, (ev, ep)
 
 
+def test_vanka_store_mask_and_release(femus, oracle, ctx):
+    """The Vanka patch inverses kept in one precision only: a solve whose cycles need the other precision fails
+    with FM_ESTATE and leaves the system usable; FP32-only inverses give the mixed solve the same iterations as
+    both stores; after fm_mg_release the next solve sets the hierarchy up again."""
+    pm, om, ns, bc, x0 = _setup(femus, oracle, ctx, "channel3d", 3, [4, 2, 2], 2)
+    for L in range(2):
+        ns.GenMatRhs(0.0, L, 1 if L == 1 else 0)
+    kw = dict(smoother=femus.FM_SMOOTH_VANKA, nu_pre=2, nu_post=2, tol=1e-9, max_iter=80, omega=0.5)
+    o32, o64 = ns.opts(precision=femus.FM_PREC_FP32_FGMRES, **kw), ns.opts(precision=femus.FM_PREC_FP64_FGMRES, **kw)
+
+    def solve(o):
+        ns.set_x(x0)
+        return ns.MGSolve(opts=o), ns.get_x()
+
+    r32, x32 = solve(o32)
+    r64, x64 = solve(o64)
+    assert ns.set_vanka_store(2) == 3
+    ns.release_mg()
+    r, x = solve(o32)
+    assert r.iters == r32.iters and np.array_equal(x, x32)
+    with pytest.raises(femus.FemusError) as e:
+        solve(o64)
+    assert e.value.code == -5 and "FP64" in str(e.value)
+    assert ns.set_vanka_store(3) == 2
+    r, x = solve(o64)
+    assert r.iters == r64.iters and np.array_equal(x, x64)
+    ns.close()
+
+
 def test_ns_picard_steps_match_oracle(femus, oracle, ctx):
     """Three MGTimeStep (assemble all levels, MGSolve, x_old <- x) on the cavity vs the oracle with
     direct solves (MGSolverNS.C:722-756)."""
