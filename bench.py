@@ -328,7 +328,8 @@ def dhc_workload(F, ctx, args):
     nsteady = None
     if args.dhc_steady_steps > 0:   # 0: the timed steps only (no steady state, no adjoint)
         t1 = time.perf_counter()
-        nsteady = b.run(max_steps=args.dhc_steady_steps, tol=1e-8)
+        nsteady = b.run(max_steps=args.dhc_steady_steps, tol=1e-8,
+                        callback=lambda j, du, dT: j % 5 == 4 and log(f"steady step {j + 1}: du {du:.2e} dT {dT:.2e}"))
         t_steady = time.perf_counter() - t1
         log(f"steady after {nsteady} more steps ({t_steady:.1f} s)")
         t1 = time.perf_counter()
