@@ -1028,12 +1028,12 @@ extern "C" int fm_adjoint_spmv(fm_system *s, int L, const double *xh, double *yh
   hipStream_t st = s->ctx->stream;
   FM_HIP(hipSetDevice(s->ctx->device));
   FM_HIP(hipMemcpyAsync(l.t, xh, l.n * sizeof(double), hipMemcpyHostToDevice, st));
-  DevCSR &c = s->alev[L].A;
-  FM_CHECK(fm_csr_from_sell(st, c));
-  CsrView<double> A = view<double>(c);
-  A.nsl = 0;  // CSR copy: reference numbering on both sides
-  FM_CHECK(fmk::spmv(st, A, l.t, l.d));
-  FM_HIP(hipMemcpyAsync(yh, l.d, l.n * sizeof(double), hipMemcpyDeviceToHost, st));
+  // the SELL copy of A^T in the level's internal order: x gathered in (d[i] = x[ord[i]]), the product, the
+  // result gathered back to the reference numbering (y[j] = d2[rank[j]]); no CSR values are formed
+  FM_CHECK(fmk::gather(st, l.n, l.ord, l.t, l.d));
+  FM_CHECK(fmk::spmv(st, view<double>(s->alev[L].A), l.d, l.d2));
+  FM_CHECK(fmk::gather(st, l.n, l.rank, l.d2, l.t));
+  FM_HIP(hipMemcpyAsync(yh, l.t, l.n * sizeof(double), hipMemcpyDeviceToHost, st));
   FM_HIP(hipStreamSynchronize(st));
   return FM_OK;
 }

@@ -59,6 +59,19 @@ def test_transposed_assembly_exact(femus, ctx, dim, nx, nlev):
         assert (AT != A.T).nnz == 0
 
 
+@pytest.mark.parametrize("dim,nx,nlev", [(2, [2, 2], 3), (3, [2, 2, 2], 2)])
+def test_adjoint_spmv(femus, ctx, dim, nx, nlev):
+    """y = A^T x through the C ABI (the transposed SELL copy, vectors in the reference numbering) against the
+    transpose of the forward CSR matrix."""
+    m, s, k = _heat(femus, ctx, dim, nx, nlev)
+    _assemble(s, nlev)
+    rng = np.random.default_rng(9)
+    for L in range(nlev):
+        A = _csr(s, L)
+        x = rng.standard_normal(A.shape[0])
+        np.testing.assert_allclose(s.adjoint_spmv(L, x), A.T @ x, rtol=1e-13, atol=1e-13 * abs(A).sum(axis=0).max())
+
+
 @pytest.mark.parametrize("prec", [3, 2])
 def test_adjoint_solve(femus, ctx, prec):
     dim, nx, nlev = 3, [2, 2, 2], 3
