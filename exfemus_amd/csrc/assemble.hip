@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include "fm_kernels.h"
+#include "fm_device.h"
 
 __constant__ int c_face2d[4][3] = {{0,1,4},{1,2,5},{2,3,6},{3,0,7}};
 __constant__ int c_face3d[6][9] = {
@@ -132,7 +133,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) void k
   const int tid = threadIdx.x;
   const double tcdt = a.tc * a.rhocp / a.dt;
   for (int t = tid; t < N2 * NG; t += blockDim.x) s_phi[t] = tab->phi2[t];
-  for (int ei = blockIdx.x; ei < nel; ei += gridDim.x) {
+  for (int ei = xcd_block(); ei < nel; ei += gridDim.x) {
     const int e = elems[ei];
     __syncthreads();
     if (tid < N2) {
@@ -355,7 +356,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   const double tcdt = a.tc / a.dt;
   for (int t = tid; t < N2 * NG; t += blockDim.x) s_phi[t] = tab->phi2[t];
   for (int t = tid; t < N1 * NG; t += blockDim.x) s_psi[t] = tab->phi1[t];
-  for (int ei = blockIdx.x; ei < nel; ei += gridDim.x) {
+  for (int ei = xcd_block(); ei < nel; ei += gridDim.x) {
     const int e = elems[ei];
     __syncthreads();
     if (tid < N2) {
@@ -690,7 +691,7 @@ __global__ __launch_bounds__(64) void k_heat_lap(int nel, const int *__restrict_
   __shared__ int s_bcv[N2], s_dof[N2];
   __shared__ double s_red[2 * N2];
   const int tid = threadIdx.x;
-  for (int ei = blockIdx.x; ei < nel; ei += gridDim.x) {
+  for (int ei = xcd_block(); ei < nel; ei += gridDim.x) {
     const int e = elems[ei];
     __syncthreads();
     if (tid < N2) {
@@ -769,7 +770,7 @@ __global__ __launch_bounds__(128) void k_ns_visc(int nel, const int *__restrict_
   __shared__ int s_bcv[DIM * N2], s_dof[DIM * N2];
   __shared__ double s_red[2 * DIM * N2 + NG];
   const int tid = threadIdx.x;
-  for (int ei = blockIdx.x; ei < nel; ei += gridDim.x) {
+  for (int ei = xcd_block(); ei < nel; ei += gridDim.x) {
     const int e = elems[ei];
     __syncthreads();
     if (tid < N2) {
@@ -856,7 +857,7 @@ __global__ __launch_bounds__(64) void k_heat_nusselt(int nel, const int *__restr
   __shared__ double s_gx[NG * N2], s_JxW[NG], s_uq[NG], s_Tq[NG], s_dT[NG];
   __shared__ int s_dof[N2];
   const int tid = threadIdx.x;
-  for (int ei = blockIdx.x; ei < nel; ei += gridDim.x) {
+  for (int ei = xcd_block(); ei < nel; ei += gridDim.x) {
     const int e = elems[ei];
     __syncthreads();
     if (tid < N2) {
@@ -929,7 +930,7 @@ __global__ __launch_bounds__(64) void k_coupling(int nel, const int *__restrict_
   __shared__ double s_JxW[NG], s_a[NG], s_g[NG * DIM], s_c[NG * DIM];
   __shared__ int s_node[N2];
   const int tid = threadIdx.x;
-  for (int ei = blockIdx.x; ei < nel; ei += gridDim.x) {
+  for (int ei = xcd_block(); ei < nel; ei += gridDim.x) {
     const int e = elems[ei];
     __syncthreads();
     if (tid < N2) {

@@ -20,6 +20,7 @@
 #include <vector>
 #include "fm_internal.h"
 #include "fm_kernels.h"
+#include "fm_device.h"
 
 // Inverse of every element patch matrix K_e (NL x NL, gathered from the level
 // matrix through the element -> slot map), FP64, by in-place exchange steps:
@@ -42,7 +43,7 @@ __global__ __launch_bounds__(G * G) void k_vanka_invert(int nel, const unsigned 
   __shared__ int s_ki[2][G], s_prow[NP], s_rowstep[NP];
   const int tid = threadIdx.x, ty = tid / G, tx = tid % G, lane = tid & 63, wid = tid >> 6;
   const int i0 = ty * TR, j0 = tx * TR;
-  for (int e = blockIdx.x; e < nel; e += gridDim.x) {
+  for (int e = xcd_block(); e < nel; e += gridDim.x) {
     const int64_t base = (int64_t)e * NL * NL;
     double m[TR][TR];
     double amax = 0.;
@@ -159,7 +160,7 @@ __global__ __launch_bounds__(128) void k_vanka_apply(int nel, const int *__restr
   __shared__ T re[NL];
   __shared__ int dof[NL];
   const int tid = threadIdx.x;
-  for (int ei = blockIdx.x; ei < nel; ei += gridDim.x) {
+  for (int ei = xcd_block(); ei < nel; ei += gridDim.x) {
     const int e = elems[ei];
     __syncthreads();
     for (int a = tid; a < NL; a += blockDim.x) {
