@@ -207,6 +207,14 @@ class MGSolDA {
     if (rc != FM_OK && rc != FM_ENOCONV) check(rc, "fm_adjoint_solve");
     return lam;
   }
+  // memory of the largest meshes: precisions in which the Vanka inverses are kept (bit 0 FP64, bit 1 FP32;
+  // returns the previous mask), and release of one hierarchy's patch and coarse inverses
+  int set_vanka_store(int mask) {
+    const int rc = fm_system_set_vanka_store(h_, mask);
+    if (rc < 0) check(rc, "fm_system_set_vanka_store");
+    return rc;
+  }
+  void release_mg(bool adjoint = false) { check(fm_mg_release(h_, adjoint ? 1 : 0), "fm_mg_release"); }
   fm_system *handle() const { return h_; }
 
   // public state of the reference (MGSolverBase.h:44-98)
