@@ -61,11 +61,17 @@ def fgmres(matvec, precond, b, tol=1e-10, maxit=100, restart=50, callback=None):
         for k in range(m):
             Z[k] = precond(V[k])
             w = matvec(Z[k])
-            for _ in range(2):                       # classical Gram-Schmidt, twice
+            # classical Gram-Schmidt; a second pass only when the first lost more than half of |w|^2 (Daniel,
+            # Gragg, Kaufman and Stewart, Math. Comp. 30 (1976))
+            w0 = np.linalg.norm(w)
+            for p in range(2):
                 h = V[:k + 1] @ w
                 w -= h @ V[:k + 1]
                 H[:k + 1, k] += h
-            H[k + 1, k] = np.linalg.norm(w)
+                wn = np.linalg.norm(w)
+                if p == 0 and wn >= w0 / np.sqrt(2.0):
+                    break
+            H[k + 1, k] = wn
             its += 1
             e1 = np.zeros(k + 2)
             e1[0] = beta
