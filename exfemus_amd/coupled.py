@@ -108,12 +108,24 @@ class Boussinesq:
         self.T.GenBc(dhc_T_bc)
         self.T.init()
         self.xT = self.T.GenIc(lambda xp: [1.0 - xp[0]])
+        self.T.set_x(self.xT)                    # the device iterate is the coupling source
         self.buoy = [0.0] * 3
         self.buoy[dim - 1] = 1.0                 # gravity along -y (2D) / -z (3D)
         self.o_ns = self.ns.opts(precision=F.FM_PREC_FP32_FGMRES, smoother=F.FM_SMOOTH_VANKA, vanka_colored=1,
                                  nu_pre=3, nu_post=3, coarse_mult=1, tol=1e-9, max_iter=200)
         self.o_T = self.T.opts(precision=F.FM_PREC_FP32_FGMRES, nu_pre=3, nu_post=3, cheb_lo=0.1, tol=1e-10,
                                max_iter=200)
+
+    @property
+    def xT(self):
+        """Temperature of the current step (top level, reference order), fetched from the device on demand."""
+        if self._xT is None:
+            self._xT = self.T.get_x()
+        return self._xT
+
+    @xT.setter
+    def xT(self, v):
+        self._xT = v
 
     def state(self):
         """(velocity-pressure vector, temperature vector) of the current step, top level, reference order."""
@@ -146,30 +158,25 @@ class Boussinesq:
             sysm.params.upwind = up
 
     def _timestep(self, sysm, t, opts, coarse_upwind):
+        """Assembly and solve of one system; x_old <- x and the relative change on the device (MGTimeStep)."""
         self._assemble(sysm, t, coarse_upwind)
         rep = sysm.MGSolve(opts=opts)
-        sysm.set_xold(sysm.get_x())
-        return rep
+        return rep, sysm.advance()
 
     def step(self, t=0.0):
         """One pseudo-time step of each system; returns the relative changes (NS, T).  The two multigrid
-        reports are kept in self.last_reps."""
-        T_old = self.xT
-        u_prev = self.ns.get_x()
-        self.ns.set_buoyancy(self.T.node_values(T_old), self.buoy)
-        rep_ns = self._timestep(self.ns, t, self.o_ns, self.cu_ns)
+        reports are kept in self.last_reps.  The coupling fields (the temperature driving the buoyancy, the
+        velocity advecting the heat) pass from one system to the other on the device."""
+        self.ns.couple(self.T, 0, self.buoy)
+        rep_ns, du = self._timestep(self.ns, t, self.o_ns, self.cu_ns)
         if rep_ns.converged != 1:
             raise RuntimeError(f"NS step: {rep_ns.iters} its, {rep_ns.rel_resid:.2e}")
-        u = self.ns.get_x()
-        self.T.set_velocity(self.ns.velocity_nodes(u))
-        rep_T = self._timestep(self.T, t, self.o_T, self.cu_T)
+        self.T.couple(self.ns, 1)
+        rep_T, dT = self._timestep(self.T, t, self.o_T, self.cu_T)
         if rep_T.converged != 1:
             raise RuntimeError(f"T step: {rep_T.iters} its, {rep_T.rel_resid:.2e}")
         self.last_reps = (rep_ns, rep_T)
-        T_new = self.T.get_x()
-        self.xT = T_new
-        du = np.linalg.norm(u - u_prev) / max(np.linalg.norm(u), 1e-300)
-        dT = np.linalg.norm(T_new - T_old) / max(np.linalg.norm(T_new), 1e-300)
+        self._xT = None
         return du, dT
 
     def names(self):

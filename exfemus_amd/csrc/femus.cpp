@@ -940,6 +940,56 @@ extern "C" int fm_system_set_velocity(fm_system *s, const double *vel) {
   return FM_OK;
 }
 
+// Coupling fields from another system's top-level solution, gathered on the device (both systems on the same
+// context and mesh): which 0 = the temperature of a heat system as dst's buoyancy field (coef as in
+// fm_ns_set_buoyancy), 1 = the velocity of an NS system as dst's advection field.
+extern "C" int fm_system_couple(fm_system *dst, const fm_system *src, int which, const double coef[3]) {
+  FM_CHECK(check_level(dst, 0));
+  FM_CHECK(check_level(src, 0));
+  if (dst->ctx != src->ctx || dst->mesh != src->mesh || which < 0 || which > 1) return FM_EINVAL;
+  const fm_mesh &m = *dst->mesh;
+  const size_t N = (size_t)m.Ntop;
+  const int top = m.nlev - 1;
+  const Level &ls = src->lev[top];
+  hipStream_t st = dst->ctx->stream;
+  FM_HIP(hipSetDevice(dst->ctx->device));
+  if (which == 0) {
+    if (!coef || dst->nq2 != m.dim || dst->nq1 != 1 || src->nq2 < 1) return FM_EINVAL;
+    if (!dst->tnode) FM_CHECK(dalloc(&dst->tnode, N));
+    FM_CHECK(fmk::node_gather(st, (int64_t)N, 1, src->d_dof_top, ls.rank, ls.x, dst->tnode));
+    for (int d = 0; d < 3; d++) dst->buoy[d] = coef[d];
+  } else {
+    if (src->nq2 < m.dim) return FM_EINVAL;
+    if (!dst->vel) FM_CHECK(dalloc(&dst->vel, (size_t)m.dim * N));
+    FM_CHECK(fmk::node_gather(st, (int64_t)N, m.dim, src->d_dof_top, ls.rank, ls.x, dst->vel));
+  }
+  FM_HIP(hipStreamSynchronize(st));
+  return FM_OK;
+}
+
+// Time-step advance on the device: x_old = x (top level, reference numbering); *rel_change (if non-null)
+// = |x - x_old| / |x| with the x_old before the copy.
+extern "C" int fm_system_advance(fm_system *s, double *rel_change) {
+  FM_CHECK(check_level(s, 0));
+  const int top = (int)s->lev.size() - 1;
+  Level &l = s->lev[top];
+  fm_ctx *c = s->ctx;
+  hipStream_t st = c->stream;
+  FM_HIP(hipSetDevice(c->device));
+  FM_CHECK(fmk::gather(st, l.n, l.rank, l.x, l.t));  // x in the reference numbering
+  if (rel_change) {
+    FM_HIP(hipMemcpyAsync(l.d, l.t, l.n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    FM_CHECK(fmk::axpby<double>(st, l.n, 1.0, l.d, -1.0, s->x_old));
+    FM_CHECK(fmk::dot_dev<double>(st, l.n, l.d, l.d, c->scratch, c->scratch + 2048, 1));
+    FM_CHECK(fmk::dot_dev<double>(st, l.n, l.t, l.t, c->scratch, c->scratch + 2049, 1));
+    FM_HIP(hipMemcpyAsync(c->h_scratch, c->scratch + 2048, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+  }
+  FM_HIP(hipMemcpyAsync(s->x_old, l.t, l.n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  FM_HIP(hipStreamSynchronize(st));
+  if (rel_change) *rel_change = c->h_scratch[0] / std::max(c->h_scratch[1], 1e-300);
+  return FM_OK;
+}
+
 // refresh FP32 values and diagonal inverses after an assembly
 static int after_assembly(fm_system *s, int L) {
   Level &l = s->lev[L];

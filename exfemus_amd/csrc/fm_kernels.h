@@ -90,6 +90,7 @@ template <typename T> int dot_dev(hipStream_t s, int64_t n, const T *x, const T 
 int ratio(hipStream_t s, const double *a, const double *b, double *out);
 template <typename T> int dense_mv(hipStream_t s, int n, const T *M, const T *b, T *y);
 int gather(hipStream_t s, int64_t n, const int *idx, const double *src, double *dst);
+int node_gather(hipStream_t s, int64_t N, int nvar, const int *dof, const int *rank, const double *x, double *out);
 int dinv_slot(hipStream_t s, int64_t n, const int64_t *dslot, const double *sv, double *d64, float *d32);
 int sell2csr(hipStream_t s, const SellMap &m, const double *sv, double *cv);
 int csr2sell(hipStream_t s, const SellMap &m, const double *csr_v, double *sv, float *svf);

@@ -642,6 +642,21 @@ int elslot(hipStream_t s, int64_t n, const int *eldofi, const int64_t *rslot, in
   hipLaunchKernelGGL(k_elslot, vgrid(n), dim3(256), 0, s, n, eldofi, rslot, out);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
+// nodal field from a level vector in internal order: out[v * N + k] = x[rank[dof[k + v * N]]] for the nvar
+// variables v (0 where the node carries no such DoF)
+__global__ void k_node_gather(int64_t N, int nvar, const int *__restrict__ dof, const int *__restrict__ rank,
+                              const double *__restrict__ x, double *__restrict__ out) {
+  const int64_t total = N * nvar;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int d = dof[t];
+    out[t] = d >= 0 ? x[rank[d]] : 0.0;
+  }
+}
+int node_gather(hipStream_t s, int64_t N, int nvar, const int *dof, const int *rank, const double *x, double *out) {
+  if (N <= 0 || nvar <= 0) return 0;
+  hipLaunchKernelGGL(k_node_gather, vgrid(N * nvar), dim3(256), 0, s, N, nvar, dof, rank, x, out);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
 // dst[i] = src[idx[i]]  (reference <-> internal DoF order)
 __global__ void k_gather(int64_t n, const int *__restrict__ idx, const double *__restrict__ src, double *__restrict__ dst) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)

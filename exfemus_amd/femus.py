@@ -43,7 +43,7 @@ EXPORTS = [
     "fm_get_adjoint_csr", "fm_heat_dissipation", "fm_bench_stream", "fm_ns_dissipation", "fm_ns_grad_coef",
     "fm_system_set_bcdata", "fm_heat_bc_rhs", "fm_ns_bc_rhs", "fm_ns_pressure_drop",
     "fm_ns_set_buoyancy", "fm_heat_nusselt", "fm_heat_coupling", "fm_adjoint_spmv", "fm_set_sell_split", "fm_system_sell_info",
-    "fm_system_set_vanka_store", "fm_mg_release",
+    "fm_system_set_vanka_store", "fm_mg_release", "fm_system_couple", "fm_system_advance",
 ]
 
 
@@ -96,6 +96,8 @@ def lib():
     L.fm_system_sell_info.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
     L.fm_system_set_vanka_store.argtypes = [C.c_void_p, C.c_int]
     L.fm_mg_release.argtypes = [C.c_void_p, C.c_int]
+    L.fm_system_couple.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+    L.fm_system_advance.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
     L.fm_ctx_create.argtypes = [C.c_int, pp]
     L.fm_ctx_destroy.argtypes = [vp]
     L.fm_sync.argtypes = [vp]
@@ -361,6 +363,18 @@ class MGSolDA:
     # -- LA operators (NumericVectorM::resid / matrix_mult) -----------------------
     def spmv(self, L, x):
         y = np.zeros(self._Dim[L]); _ck(lib().fm_spmv(self.h, L, np.ascontiguousarray(x, np.float64), y)); return y
+
+    def couple(self, src, which, coef=None):
+        """This system's coupling field from src's top-level solution, on the device: which 0 = src's temperature
+        as the buoyancy field (coef per direction), 1 = src's velocity as the advection field."""
+        c = np.zeros(3) if coef is None else np.asarray(coef, np.float64)
+        _ck(lib().fm_system_couple(self.h, src.h, int(which), c.ctypes.data_as(C.c_void_p)))
+
+    def advance(self):
+        """x_old = x on the top level (on the device); returns |x - x_old| / |x| before the copy."""
+        r = C.c_double()
+        _ck(lib().fm_system_advance(self.h, C.byref(r)))
+        return r.value
 
     def set_vanka_store(self, mask):
         """Precisions in which the Vanka patch inverses are kept (bit 0 FP64, bit 1 FP32); returns the previous mask."""

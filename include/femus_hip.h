@@ -121,6 +121,15 @@ typedef struct {
 int fm_system_set_elem_coef(fm_system *s, int L, const double *coef);
 /* nodal Q2 advection velocity for heat [dim*Ntop] (NULL = vel_const)       */
 int fm_system_set_velocity(fm_system *s, const double *vel_top);
+/* Coupling and time stepping without host round trips (no counterpart in the reference, whose
+ * systems exchange external fields through host arrays, src/MGSolverDA.C:101-543).
+ * fm_system_couple: dst's coupling field from src's top-level solution, gathered on the device (same
+ * context and mesh): which 0 = the temperature of a heat system as dst's buoyancy field (coef as in
+ * fm_ns_set_buoyancy), 1 = the velocity of a Navier-Stokes system as dst's advection field.
+ * fm_system_advance: x_old = x on the top level (MGTimeStep's x_old <- x); *rel_change (if not NULL)
+ * = |x - x_old| / |x| before the copy. */
+int fm_system_couple(fm_system *dst, const fm_system *src, int which, const double coef[3]);
+int fm_system_advance(fm_system *s, double *rel_change);
 /* Dirichlet data of the boundary-face penalty RHS, top-level DoFs in the
  * reference numbering.  The reference reads it from x_old
  * (template_appl/nse/SRC/MGSolverNS.C:532-601, MGSolverT.C:401-438); a

@@ -193,3 +193,32 @@ def test_coupled_nusselt_adjoint_fd(femus, ctx, up_ns, up_T):
     assert np.linalg.norm(gfrozen - grad) > 1e-2 * np.linalg.norm(grad)
     b.ns.close()
     b.T.close()
+
+
+def test_device_coupling_matches_host_path(femus, ctx):
+    """One coupled step with the coupling fields and x_old <- x on the device (fm_system_couple /
+    fm_system_advance) equals the same step through host arrays (nodal temperature and velocity gathered on the
+    host, set_buoyancy / set_velocity / set_xold): the same states bit for bit, the relative changes to 1e-12."""
+    from exfemus_amd import coupled
+    b = coupled.Boussinesq(ctx, 2, 4, 3, 1e4)
+    for k in range(3):
+        b.step(float(k + 1))
+    st = b.state()
+    du, dT = b.step(4.0)
+    U_dev, T_dev = b.state()
+    b.set_state(st)
+    u_prev, T_old = st
+    b.ns.set_buoyancy(b.T.node_values(T_old), b.buoy)
+    b._assemble(b.ns, 4.0, b.cu_ns)
+    assert b.ns.MGSolve(opts=b.o_ns).converged == 1
+    u = b.ns.get_x()
+    b.ns.set_xold(u)
+    b.T.set_velocity(b.ns.velocity_nodes(u))
+    b._assemble(b.T, 4.0, b.cu_T)
+    assert b.T.MGSolve(opts=b.o_T).converged == 1
+    Tn = b.T.get_x()
+    assert np.array_equal(u, U_dev) and np.array_equal(Tn, T_dev)
+    assert abs(du - np.linalg.norm(u - u_prev) / np.linalg.norm(u)) <= 1e-12 * du
+    assert abs(dT - np.linalg.norm(Tn - T_old) / np.linalg.norm(Tn)) <= 1e-12 * dT
+    b.ns.close()
+    b.T.close()
