@@ -207,6 +207,19 @@ class MGSolDA {
     if (rc != FM_OK && rc != FM_ENOCONV) check(rc, "fm_adjoint_solve");
     return lam;
   }
+  // coupling field from src's top-level solution gathered on the device (fm_system_couple: which 0 = src's
+  // temperature as this system's buoyancy field, 1 = src's velocity as its advection field)
+  void couple(const MGSolDA &src, int which, const double coef[3] = nullptr) {
+    const double zero[3] = {0.0, 0.0, 0.0};
+    check(fm_system_couple(h_, src.h_, which, coef ? coef : zero), "fm_system_couple");
+  }
+  // x_old <- x on the device (fm_system_advance); returns |x - x_old| / |x|.  The host copy x_old of this
+  // class is left as it is: call get_x() when it is needed.
+  double advance() {
+    double r = 0.0;
+    check(fm_system_advance(h_, &r), "fm_system_advance");
+    return r;
+  }
   // memory of the largest meshes: precisions in which the Vanka inverses are kept (bit 0 FP64, bit 1 FP32;
   // returns the previous mask), and release of one hierarchy's patch and coarse inverses
   int set_vanka_store(int mask) {
