@@ -77,3 +77,32 @@ def test_singular_coarse_level_and_noconv(femus, ctx):
                                  C.byref(femus.MGReport()))
     assert rc == FM_ENOCONV
     s2.close()
+
+
+def test_memory_and_coupling_controls_reject_bad_arguments(femus, ctx):
+    """fm_system_couple needs two systems of one mesh and a known field; the Vanka store mask is 1..3 (0 reads);
+    fm_system_sell_info and fm_system_advance need an initialised system; fm_mg_release on a hierarchy that was
+    never set up is a no-op."""
+    L = femus.lib()
+    m1 = femus.Mesh(2, [2, 2], [0, 0], [1, 1], 2)
+    m2 = femus.Mesh(2, [2, 2], [0, 0], [1, 1], 2)
+    a = femus.MGSolT(ctx, m1, alpha0=1.0, dt=0.1, time_coeff=0.0, source=1.0)
+    b = femus.MGSolT(ctx, m2, alpha0=1.0, dt=0.1, time_coeff=0.0, source=1.0)
+    info = np.zeros(6, np.int64)
+    assert L.fm_system_sell_info(a.h, 0, 0, info.ctypes.data_as(C.c_void_p)) == FM_EINVAL   # before init
+    r = C.c_double()
+    assert L.fm_system_advance(a.h, C.byref(r)) == FM_ESTATE                                  # before init
+    for s_ in (a, b):
+        s_.GenBc(configs.heat_dirichlet_all)
+        s_.init()
+    zero = np.zeros(3)
+    assert L.fm_system_couple(a.h, b.h, 1, zero.ctypes.data_as(C.c_void_p)) == FM_EINVAL     # different meshes
+    assert L.fm_system_couple(a.h, a.h, 5, zero.ctypes.data_as(C.c_void_p)) == FM_EINVAL     # unknown field
+    assert L.fm_system_couple(a.h, a.h, 0, zero.ctypes.data_as(C.c_void_p)) == FM_EINVAL     # heat has no buoyancy
+    assert L.fm_system_set_vanka_store(a.h, 7) == FM_EINVAL
+    assert L.fm_system_set_vanka_store(a.h, 0) == 3                                           # read only
+    assert L.fm_mg_release(a.h, 0) == 0 and L.fm_mg_release(a.h, 1) == 0
+    assert L.fm_system_sell_info(a.h, 0, 1, info.ctypes.data_as(C.c_void_p)) == 0
+    assert info[5] == a.csr(0, 1)[1].size and info[1] >= info[5] and info[0] * 64 >= a._Dim[1]
+    a.close()
+    b.close()
