@@ -281,7 +281,7 @@ def dhc_workload(F, ctx, args):
     top = nlev - 1
     Nns, NT = ns._Dim[top], T._Dim[top]
     nu = args.nu or 3
-    kw_ns = dict(smoother=F.FM_SMOOTH_VANKA, vanka_colored=1, nu_pre=nu, nu_post=nu, coarse_mult=1, tol=args.tol,
+    kw_ns = dict(smoother=F.FM_SMOOTH_VANKA, vanka_colored=args.vanka_colored, nu_pre=nu, nu_post=nu, coarse_mult=1, tol=args.tol,
                  max_iter=200, use_graph=0 if args.no_graph else 1, omega=args.omega)
     kw_T = dict(nu_pre=3, nu_post=3, cheb_lo=0.1, tol=args.tol, max_iter=200, use_graph=0 if args.no_graph else 1)
     mixed = (ns.opts(precision=F.FM_PREC_FP32_FGMRES, **kw_ns), T.opts(precision=F.FM_PREC_FP32_FGMRES, **kw_T))
@@ -426,6 +426,7 @@ def main():
     ap.add_argument("--outer", default="fgmres", choices=["ir", "fgmres"])
     ap.add_argument("--nu", type=int, default=None)
     ap.add_argument("--omega", type=float, default=None, help="Vanka damping (default 0.5)")
+    ap.add_argument("--vanka-colored", type=int, default=1, help="1: multiplicative over element colours, 0: additive")
     ap.add_argument("--cheb-lo", type=float, default=0.1)
     ap.add_argument("--coarse-mult", type=int, default=None,
                     help="sweep multiplier below the top level (0: the reference's doubling; default 1 for heat64)")
@@ -471,7 +472,8 @@ def main():
         # sweeps below the top; here that costs more than the iterations it saves: 409 vs 333 ms at C3)
         nu = args.nu or 3
         m, s, t_asm = build_ns(F, ctx, args.workload)
-        kw = dict(nu_pre=nu, nu_post=nu, smoother=F.FM_SMOOTH_VANKA, vanka_colored=1, coarse_mult=1, omega=args.omega)
+        kw = dict(nu_pre=nu, nu_post=nu, smoother=F.FM_SMOOTH_VANKA, vanka_colored=args.vanka_colored, coarse_mult=1,
+                  omega=args.omega)
         smoother = f"coloured element Vanka {nu}/{nu}, same sweeps on every level"
         workload = ("3D NS channel hex27 Q2-Q1 96x24x24, 4 levels, Re 400, Oseen step from the inflow profile"
                     if args.workload == "channel" else
