@@ -755,14 +755,14 @@ extern "C" int fm_system_init(fm_system *s) {
 }
 
 extern "C" int64_t fm_system_csr_nnz(const fm_system *s, int which, int L) {
-  if (!s || L < 0 || L >= (int)s->lev.size()) return -1;
+  if (!s || L < 0 || L >= (int)s->lev.size() || which < 0 || which > 2) return -1;
   const Level &l = s->lev[L];
   const DevCSR &c = which == 0 ? l.A : (which == 1 ? l.R : l.P);
   return c.nnz;
 }
 
 extern "C" int fm_system_sell_info(const fm_system *s, int which, int L, int64_t *info) {
-  if (!s || !info || L < 0 || L >= (int)s->lev.size() || !s->initialized) return FM_EINVAL;
+  if (!s || !info || L < 0 || L >= (int)s->lev.size() || !s->initialized || which < 0 || which > 2) return FM_EINVAL;
   const Level &l = s->lev[L];
   const DevCSR &c = which == 0 ? l.A : (which == 1 ? l.R : l.P);
   info[0] = c.sell.nsl; info[1] = c.sell.nslot; info[2] = c.sell.n8; info[3] = c.sell.n16; info[4] = c.sell.n32;
@@ -771,7 +771,7 @@ extern "C" int fm_system_sell_info(const fm_system *s, int which, int L, int64_t
 }
 
 extern "C" int fm_system_csr_get(const fm_system *s, int which, int L, int64_t *rowptr, int *colind, double *val) {
-  if (!s || L < 0 || L >= (int)s->lev.size() || !s->initialized) return FM_EINVAL;
+  if (!s || L < 0 || L >= (int)s->lev.size() || !s->initialized || which < 0 || which > 2) return FM_EINVAL;
   const Level &l = s->lev[L];
   const DevCSR &c = which == 0 ? l.A : (which == 1 ? l.R : l.P);
   if (c.nr == 0) return FM_EINVAL;

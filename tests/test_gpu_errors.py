@@ -102,6 +102,7 @@ def test_memory_and_coupling_controls_reject_bad_arguments(femus, ctx):
     assert L.fm_system_set_vanka_store(a.h, 7) == FM_EINVAL
     assert L.fm_system_set_vanka_store(a.h, 0) == 3                                           # read only
     assert L.fm_mg_release(a.h, 0) == 0 and L.fm_mg_release(a.h, 1) == 0
+    assert L.fm_system_sell_info(a.h, 3, 1, info.ctypes.data_as(C.c_void_p)) == FM_EINVAL    # no matrix 3
     assert L.fm_system_sell_info(a.h, 0, 1, info.ctypes.data_as(C.c_void_p)) == 0
     assert info[5] == a.csr(0, 1)[1].size and info[1] >= info[5] and info[0] * 64 >= a._Dim[1]
     a.close()
