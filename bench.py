@@ -272,7 +272,9 @@ def dhc_workload(F, ctx, args):
     Then the run continues to the steady state and the Nusselt number's gradient with respect to the 110,592
     (at 48^3) per-element conductivities is computed twice: through the adjoint of the whole coupled system
     (flexible GMRES preconditioned by the two transposed multigrid solves; the upwind coefficients are
-    differentiated) and, for comparison, with the velocity frozen (steady heat adjoint only)."""
+    differentiated) and, for comparison, with the velocity frozen (steady heat adjoint only).  --dhc-nlev 5 is
+    the 96^3 configuration (the coupled adjoint then runs memory-lean: forward NS multigrid data released,
+    transposed Vanka inverses in FP32 only); --dhc-steady-steps 0 stops after the timed steps."""
     from exfemus_amd import coupled
     nx0, nlev = args.dhc_nx0, args.dhc_nlev
     t0 = time.perf_counter()
