@@ -226,7 +226,6 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) void k
           for (int v = 0; v < TI; v++) K[u][v] += ai * bj[v];
         }
       }
-      const int64_t pbase = (int64_t)e * N2 * N2;
 #pragma unroll
       for (int u = 0; u < TI; u++) {
         const int i = ti + 9 * u;
@@ -266,7 +265,6 @@ __global__ void k_heat_bd(int nel, const int *__restrict__ elems, const int *__r
   int e = elems[ei];
   unsigned char bm = bmask[e];
   if (!bm) return;
-  const int64_t pbase = (int64_t)e * N2 * N2;
   for (int s = 0; s < NS; s++) {
     if (!((bm >> s) & 1)) continue;
     int sur[NB];
@@ -445,7 +443,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         __syncthreads();
       }
     }
-    const int64_t pbase = (int64_t)e * NL * NL;
     // velocity blocks: DIM^2 blocks x 81 tiles
     for (int u = tid; u < DIM * DIM * 81; u += blockDim.x) {
       const int blk = u / 81, t = u % 81;
@@ -596,7 +593,6 @@ __global__ void k_ns_bd(int nel, const int *__restrict__ elems, const int *__res
   int e = elems[ei];
   unsigned char bm = bmask[e];
   if (!bm) return;
-  const int64_t pbase = (int64_t)e * NL * NL;
   for (int s = 0; s < NS; s++) {
     if (!((bm >> s) & 1)) continue;
     int sur[NB], gn[NB];

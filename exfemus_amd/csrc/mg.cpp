@@ -424,7 +424,7 @@ __global__ void k_multi_dot(int64_t n, int k, int kk, const double *__restrict__
     for (int t = 0; t < 8; t++) sh[threadIdx.x >> 6][t] = s[t];
   }
   __syncthreads();
-  if (threadIdx.x < nj) {
+  if ((int)threadIdx.x < nj) {
     double t = 0.;
     for (int q = 0; q < (int)(blockDim.x >> 6); q++) t += sh[q][threadIdx.x];
     part[(int64_t)(j0 + threadIdx.x) * gridDim.x + blockIdx.x] = t;
