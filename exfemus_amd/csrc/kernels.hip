@@ -229,15 +229,20 @@ __global__ __launch_bounds__(64 * S) void k_sell_split(int nsl, const int64_t *_
 // node-lexicographic internal order: contiguous per-XCD slice ranges take the
 // FP64 SpMV from 389 to 301 us and the FP32 one from 249 to 187 us (each XCD's
 // L2 then holds the x window of its own range); unroll 8 beats 4 by ~2%.
-static int g_sell_unroll = -1, g_sell_xcd = -1, g_sell_split = -1;
+// FEMUS_SELL_LIST_SPLIT = 0 | 1 (default 1): products restricted to a slice list (the residual of one Vanka
+// colour) with W to 4 W entries run with S = 2.  Measured: the C5 step at 48^3 (top-level colour lists of
+// ~19,600 slices) 627 -> 611 ms mixed, 985 -> 941 ms FP64; at 96^3 (~154,000 slices) no gain.
+static int g_sell_unroll = -1, g_sell_xcd = -1, g_sell_split = -1, g_sell_list2 = -1;
 static void sell_knobs() {
   if (g_sell_unroll >= 0) return;
   const char *u = getenv("FEMUS_SELL_UNROLL");
   const char *x = getenv("FEMUS_SELL_XCD");
   const char *p = getenv("FEMUS_SELL_SPLIT");
+  const char *q = getenv("FEMUS_SELL_LIST_SPLIT");
   g_sell_unroll = (u && atoi(u) == 4) ? 4 : ((u && atoi(u) == 16) ? 16 : 8);
   g_sell_xcd = (x && atoi(x) == 0) ? 0 : 1;
   if (g_sell_split < 0) g_sell_split = p ? std::max(0, atoi(p)) : 16384;
+  g_sell_list2 = (q && atoi(q) == 0) ? 0 : 1;
 }
 
 template <typename T, int EP, int S>
@@ -264,6 +269,10 @@ static int launch_spmv(hipStream_t s, const CsrView<T> &A, const T *x, const EpA
       if (S == 2) launch_split<T, EP, 2>(s, A, x, ep, nsl);
       else if (S == 4) launch_split<T, EP, 4>(s, A, x, ep, nsl);
       else launch_split<T, EP, 8>(s, A, x, ep, nsl);
+      return hipGetLastError() == hipSuccess ? 0 : -2;
+    }
+    if (A.slist && g_sell_list2 && nwork < 4 * (int64_t)g_sell_split) {
+      launch_split<T, EP, 2>(s, A, x, ep, nsl);
       return hipGetLastError() == hipSuccess ? 0 : -2;
     }
     int64_t nb = (nwork + 3) / 4;
