@@ -13,6 +13,7 @@
 #include <vector>
 #include "fm_internal.h"
 #include "fm_kernels.h"
+#include "sell_host.h"
 
 static thread_local std::string g_err;
 void fm_set_error(const std::string &msg) { g_err = msg; }
@@ -231,115 +232,23 @@ static void free_csr(DevCSR &c) { fm_free_csr(c); }
 // internal rows) start their own slices, so a product can be restricted to
 // the slices of chosen groups.
 int fm_build_sell(hipStream_t st, DevCSR &c, const int *ro, const int *cr, const std::vector<int64_t> *groups) {
-  // W: rows are length-sorted inside windows of W rows (FEMUS_SELL_SIGMA, default 4096; 64 = inside a slice)
+  // W: rows are length-sorted inside windows of W rows (FEMUS_SELL_SIGMA, default 4096; 64 = inside a slice);
+  // FEMUS_SELL_C8=0 keeps every slice at 16-bit offsets or wider
   const char *sg = getenv("FEMUS_SELL_SIGMA");
   const int C = 64, W = sg && atoi(sg) >= 64 ? atoi(sg) : 4096;
-  const int64_t n = c.nr;
-  const std::vector<int64_t> &rp = c.h_rp;
-  const std::vector<int> &ci = c.h_ci;
-  if ((int64_t)rp.size() != n + 1) { fm_set_error("SELL build without host pattern"); return FM_ESTATE; }
-  auto crow = [&](int64_t i) -> int64_t { return ro ? ro[i] : i; };
-  auto rlen = [&](int64_t i) -> int { int64_t r = crow(i); return (int)(rp[r + 1] - rp[r]); };
-  std::vector<int64_t> gb = groups ? *groups : std::vector<int64_t>{0, n};
-  if (gb.front() != 0 || gb.back() != n) { fm_set_error("SELL row groups do not cover the rows"); return FM_ESTATE; }
-  // rows of each group, length-sorted inside windows of W that do not cross a group boundary; each group starts a
-  // new slice (its last slice is padded with empty lanes)
-  std::vector<int> perm;
-  perm.reserve(n + (int64_t)C * gb.size());
-  std::vector<int64_t> gslice(gb.size(), 0);
-  for (size_t g = 0; g + 1 < gb.size(); g++) {
-    gslice[g] = (int64_t)perm.size() / C;
-    const size_t start = perm.size();
-    for (int64_t i = gb[g]; i < gb[g + 1]; i++) perm.push_back((int)i);
-    for (size_t w0 = start; w0 < perm.size(); w0 += W) {
-      size_t w1 = std::min(perm.size(), w0 + W);
-      std::stable_sort(perm.begin() + w0, perm.begin() + w1, [&](int a, int b) { return rlen(a) > rlen(b); });
-    }
-    while (perm.size() % C) perm.push_back(-1);
-  }
-  gslice.back() = (int64_t)perm.size() / C;
-  const int64_t nsl = (int64_t)perm.size() / C;
-  if (nsl >= ((int64_t)1 << 31) / C) { fm_set_error("SELL: too many rows"); return FM_EINVAL; }
-  for (int64_t i = 0; i < n; i++)
-    if (rlen(i) >= 0xFFFF) { fm_set_error("SELL: a row has more than 65,534 entries"); return FM_EINVAL; }
-  std::vector<int64_t> off(nsl + 1, 0);
-  std::vector<int> wd(nsl);
-  for (int64_t s = 0; s < nsl; s++) {
-    int w = 0;
-    for (int l = 0; l < C; l++) {
-      const int r = perm[s * C + l];
-      if (r >= 0) w = std::max(w, rlen(r));
-    }
-    wd[s] = w;
-    off[s + 1] = off[s] + (int64_t)w * C;
-  }
-  c.sell.h_gslice = gslice;
-  const int64_t nslot = off[nsl];
-  std::vector<int> absc(nslot, -1), base(nslot / C, 0);
-  std::vector<unsigned short> pos(nslot, 0xFFFF);
-  std::vector<int64_t> rbase(n, 0), rslot(n, -1);
-  std::vector<char> wide(nsl, 0), narrow(nsl, 1);
-#pragma omp parallel
-  {
-    std::vector<std::pair<int, int>> ent;
-#pragma omp for schedule(dynamic, 64)
-    for (int64_t s = 0; s < nsl; s++) {
-      for (int l = 0; l < C; l++) {
-        int i = perm[s * C + l];
-        if (i < 0) continue;
-        int64_t r = crow(i);
-        rbase[i] = rp[r];
-        rslot[i] = off[s] + l;
-        ent.clear();
-        for (int64_t q = rp[r]; q < rp[r + 1]; q++) ent.emplace_back(cr ? cr[ci[q]] : ci[q], (int)(q - rp[r]));
-        std::sort(ent.begin(), ent.end());
-        for (size_t k = 0; k < ent.size(); k++) {
-          int64_t slot = off[s] + (int64_t)k * C + l;
-          absc[slot] = ent[k].first;
-          pos[slot] = (unsigned short)ent[k].second;
-        }
-      }
-      for (int k = 0; k < wd[s]; k++) {
-        int mn = INT32_MAX, mx = -1;
-        for (int l = 0; l < C; l++) {
-          int cc = absc[off[s] + (int64_t)k * C + l];
-          if (cc < 0) continue;
-          mn = std::min(mn, cc);
-          mx = std::max(mx, cc);
-        }
-        base[off[s] / C + k] = mn;
-        if (mx - mn > 65535) wide[s] = 1;
-        if (mx - mn > 255) narrow[s] = 0;
-      }
-    }
-  }
-  // 8-bit offsets for the slices that allow them (FEMUS_SELL_C8=0 keeps every slice at 16 bits)
   const char *c8env = getenv("FEMUS_SELL_C8");
   const bool use8 = !(c8env && atoi(c8env) == 0);
-  std::vector<int64_t> o32(nsl, -1), o8(nsl, -1), o16(nsl, -1);
-  int64_t n32 = 0, n8 = 0, n16 = 0;
-  for (int64_t s = 0; s < nsl; s++) {
-    if (wide[s]) { o32[s] = n32; n32 += (int64_t)wd[s] * C; }
-    else if (use8 && narrow[s]) { o8[s] = n8; n8 += (int64_t)((wd[s] + 3) / 4) * 4 * C; }
-    else { o16[s] = n16; n16 += (int64_t)((wd[s] + 1) / 2) * 2 * C; }
-  }
-  std::vector<unsigned short> c16(std::max<int64_t>(n16, 1), 0);
-  std::vector<int> c32(std::max<int64_t>(n32, 1), 0);
-  std::vector<unsigned char> c8(std::max<int64_t>(n8, 1), 0);
-#pragma omp parallel for schedule(static)
-  for (int64_t s = 0; s < nsl; s++)
-    for (int k = 0; k < wd[s]; k++) {
-      const int b = base[off[s] / C + k];
-      for (int l = 0; l < C; l++) {
-        int64_t slot = off[s] + (int64_t)k * C + l;
-        const int cc = absc[slot] >= 0 ? absc[slot] : b;  // padding: the position's base column, value 0
-        // 8-bit (16-bit): positions 4g..4g+3 (2g, 2g+1) of lane l packed in one 32-bit word, the words of a
-        // group lane-contiguous
-        if (wide[s]) c32[o32[s] + (int64_t)k * C + l] = cc;
-        else if (o8[s] >= 0) c8[o8[s] + (int64_t)(k / 4) * 4 * C + 4 * l + (k % 4)] = (unsigned char)(cc - b);
-        else c16[o16[s] + (int64_t)(k / 2) * 2 * C + 2 * l + (k % 2)] = (unsigned short)(cc - b);
-      }
-    }
+  const int64_t n = c.nr;
+  SellHost h;
+  std::string err;
+  const int hrc = sell_build_host(n, c.h_rp, c.h_ci, ro, cr, groups, W, use8, h, err);
+  if (hrc) { fm_set_error(err); return hrc == -1 ? FM_EINVAL : FM_ESTATE; }
+  c.sell.h_gslice = h.gslice;
+  const int64_t nsl = h.nsl, nslot = h.nslot, n8 = h.n8, n16 = h.n16, n32 = h.n32;
+  const std::vector<int64_t> &off = h.off, &o8 = h.o8, &o16 = h.o16, &o32 = h.o32, &rbase = h.rbase, &rslot = h.rslot;
+  const std::vector<int> &wd = h.wd, &perm = h.perm, &base = h.base, &c32 = h.c32;
+  const std::vector<unsigned short> &c16 = h.c16, &pos = h.pos;
+  const std::vector<unsigned char> &c8 = h.c8;
   DevSELL &q = c.sell;
   free_sell(q);
   q.nsl = nsl; q.nslot = nslot; q.n32 = n32; q.n8 = n8; q.n16 = n16;
