@@ -467,83 +467,6 @@ static int colour_slices(hipStream_t st, Level &l) {
   return 0;
 }
 
-// Internal DoF order of level L: the level's nodes in lexicographic order of
-// their grid coordinates (the axis with the most nodes slowest, so a row's
-// neighbours lie within ~2 node planes of the smallest cross-section), all
-// variables of a node adjacent (Q2 variables, then the Q1 variable where the
-// node carries one).  This keeps the columns of a SELL slice inside few
-// narrow windows, so the gather of x hits cache and the column offsets fit
-// 16 bits.
-//
-// With nmask (per node: bit c set when an element of colour c contains the
-// node) the domain is cut into slabs of SLAB node planes along the slowest
-// axis and, inside each slab, the nodes are grouped by that colour signature,
-// lexicographic inside a group; goff / gmask return the groups' DoF ranges
-// and signatures.  The rows a colour's elements touch are then whole groups,
-// so a coloured smoother can form the residual on those rows only, and the
-// slabs keep the gathered part of x local (grouping over the whole domain
-// made the full product 40% slower on the 96x24x24 channel).
-static int level_order(const fm_mesh &m, int L, int nv, const std::vector<int> &dofL, int64_t ndof,
-                       std::vector<int> &ord, std::vector<int> &rank, const std::vector<unsigned char> *nmask,
-                       std::vector<int64_t> &goff, std::vector<int> &gmask) {
-  const int N = m.Ntop, dim = m.dim;
-  std::vector<int> nodes;
-  nodes.reserve(m.nq2[L]);
-  for (int k = 0; k < N; k++) if (dofL[k] >= 0) nodes.push_back(k);
-  int axes[3] = {0, 1, 2};
-  int64_t cnt[3];
-  double h[3];
-  for (int d = 0; d < dim; d++) {
-    cnt[d] = (int64_t)m.nx0[d] << (m.nlev - 1);
-    h[d] = (m.hi[d] - m.lo[d]) / (2.0 * (double)cnt[d]);
-  }
-  std::stable_sort(axes, axes + dim, [&](int a, int b) { return cnt[a] > cnt[b] || (cnt[a] == cnt[b] && a > b); });
-  const int SLAB = 8;
-  std::vector<int64_t> key(N, 0);
-  std::vector<int> slab(N, 0), par(N, 0);
-  for (int k : nodes) {
-    int64_t kk = 0;
-    int pc = 0;
-    for (int t = 0; t < dim; t++) {
-      int d = axes[t];
-      int64_t q = std::llround((m.xyz[k + (size_t)d * N] - m.lo[d]) / h[d]);
-      kk = kk * (2 * cnt[d] + 1) + q;
-      pc = pc * 2 + (int)(q & 1);
-      if (t == 0) slab[k] = (int)(q / SLAB);
-    }
-    key[k] = kk;
-    par[k] = pc;
-  }
-  // group = (slab, signature): the colour signature for coupled systems, the node's parity class
-  // (vertex / edge / face / interior node of the Q2 lattice: rows of one class have one length) otherwise
-  auto sig = [&](int k) -> int { return nmask ? (int)(*nmask)[k] : par[k]; };
-  auto grp = [&](int k) -> int64_t { return (int64_t)slab[k] * 256 + sig(k); };
-  std::stable_sort(nodes.begin(), nodes.end(), [&](int a, int b) {
-    return grp(a) != grp(b) ? grp(a) < grp(b) : key[a] < key[b];
-  });
-  ord.clear();
-  ord.reserve(ndof);
-  goff.assign(1, 0);
-  gmask.clear();
-  for (size_t t = 0; t < nodes.size(); t++) {
-    const int k = nodes[t];
-    if (t > 0 && grp(k) != grp(nodes[t - 1])) { goff.push_back((int64_t)ord.size()); gmask.push_back(sig(nodes[t - 1])); }
-    for (int iv = 0; iv < nv; iv++) {
-      int d = dofL[k + (size_t)iv * N];
-      if (d >= 0) ord.push_back(d);
-    }
-  }
-  goff.push_back((int64_t)ord.size());
-  gmask.push_back(nodes.empty() ? 0 : sig(nodes.back()));
-  if ((int64_t)ord.size() != ndof) { fm_set_error("level order does not cover the DoFs"); return FM_ESTATE; }
-  rank.assign(ndof, -1);
-  for (int64_t i = 0; i < ndof; i++) {
-    if (ord[i] < 0 || ord[i] >= ndof || rank[ord[i]] >= 0) { fm_set_error("level order is not a permutation"); return FM_ESTATE; }
-    rank[ord[i]] = (int)i;
-  }
-  return 0;
-}
-
 extern "C" int fm_system_init(fm_system *s) {
   if (!s) return FM_EINVAL;
   fm_ctx *c = s->ctx;
@@ -576,16 +499,13 @@ extern "C" int fm_system_init(fm_system *s) {
     l.nel = m.nel[L];
     // colour signature of every node (coupled systems only: the coloured Vanka sweep uses it)
     std::vector<unsigned char> nmask;
-    if (s->nq1 > 0) {
-      nmask.assign(N, 0);
-      for (int e = 0; e < l.nel; e++) {
-        const int *g = &m.egrid[L][3 * (size_t)e];
-        const int cl = (g[0] & 1) + 2 * (g[1] & 1) + (m.dim == 3 ? 4 * (g[2] & 1) : 0);
-        for (int a = 0; a < m.n2; a++) nmask[m.conn[L][(size_t)e * m.n2 + a]] |= (unsigned char)(1 << cl);
-      }
+    if (s->nq1 > 0) fm_node_colours(m, L, nmask);
+    std::string err;
+    if (fm_level_order(m, L, nv, s->node_dof[L], l.n, l.h_ord, l.h_rank, s->nq1 > 0 ? &nmask : nullptr, l.h_goff,
+                       l.h_gmask, err)) {
+      fm_set_error(err);
+      return FM_ESTATE;
     }
-    FM_CHECK(level_order(m, L, nv, s->node_dof[L], l.n, l.h_ord, l.h_rank, s->nq1 > 0 ? &nmask : nullptr, l.h_goff,
-                         l.h_gmask));
     dfree(l.ord); dfree(l.rank);
     FM_CHECK(dalloc(&l.ord, l.n)); FM_CHECK(dalloc(&l.rank, l.n));
     FM_HIP(hipMemcpyAsync(l.ord, l.h_ord.data(), l.n * sizeof(int), hipMemcpyHostToDevice, st));
@@ -622,8 +542,7 @@ extern "C" int fm_system_init(fm_system *s) {
     for (int e = 0; e < l.nel; e++) {
       fm_elem_dofs(m, L, e, s->nq2, s->nq1, s->node_dof[L], &eld[(size_t)e * nloc]);
       for (int sd = 0; sd < m.nsides; sd++) if (m.neigh[L][(size_t)e * m.nsides + sd] == -1) bm[e] |= (unsigned char)(1 << sd);
-      const int *g = &m.egrid[L][3 * (size_t)e];
-      col[e] = (g[0] & 1) + 2 * (g[1] & 1) + (m.dim == 3 ? 4 * (g[2] & 1) : 0);
+      col[e] = fm_elem_colour(m, L, e);
     }
     l.ncolor = 1 << m.dim;
     l.color_off.assign(l.ncolor + 1, 0);

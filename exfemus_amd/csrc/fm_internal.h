@@ -5,6 +5,7 @@
 #include <string>
 #include <vector>
 #include "../../include/femus_hip.h"
+#include "fm_mesh.h"
 
 void fm_set_error(const std::string &msg);
 
@@ -16,16 +17,6 @@ void fm_set_error(const std::string &msg);
       return FM_EHIP;                                                          \
     }                                                                          \
   } while (0)
-
-// Host-side mesh hierarchy (gencase result; src/MGMesh.C layout).
-struct fm_mesh {
-  int dim = 0, nlev = 0, n2 = 0, n1 = 0, nsides = 0, nb2 = 0, nch = 0, Ntop = 0;
-  int nx0[3] = {1, 1, 1};
-  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-  std::vector<int> nq2, nq1, nel;
-  std::vector<double> xyz;                    // xyz[node + d*Ntop]
-  std::vector<std::vector<int>> conn, neigh, parent, egrid;  // egrid: element grid index (3 ints)
-};
 
 // Sliced ELLPACK copy (SELL-C-sigma, C = 64 = one wavefront) in the level's
 // internal DoF order (see Level::ord): rows sorted by length inside windows of
@@ -169,20 +160,6 @@ struct fm_system {
   int mg_smoother = -1;
   int vanka_store = 3;  // precisions of the stored Vanka inverses: bit 0 FP64 (FP64 cycles), bit 1 FP32 (FP32 cycles)
 };
-
-// gencase.cpp
-int fm_build_box(int dim, const int nx0[3], const double lo[3], const double hi[3], int nlev, fm_mesh *m);
-void fm_init_dof(const fm_mesh &m, int L, int nq2, int nq1, std::vector<int> &node_dof, int64_t &ndof);
-void fm_pattern(const fm_mesh &m, int L, int nq2, int nq1, const std::vector<int> &dofL,
-                std::vector<int64_t> &rp, std::vector<int> &ci);
-void fm_transfer(const fm_mesh &m, int L, int nq2, int nq1, bool rest, const std::vector<int> &dof_row,
-                 const std::vector<int> &dof_col, const std::vector<int> *dof_top, const std::vector<int> *bc0,
-                 std::vector<int64_t> &rp, std::vector<int> &ci, std::vector<double> &v);
-void fm_elem_dofs(const fm_mesh &m, int L, int e, int nq2, int nq1, const std::vector<int> &dof, int *out);
-extern const int fm_hex27_ref[27][3];
-extern const int fm_quad9_ref[9][2];
-extern const int fm_face2d[4][3];
-extern const int fm_face3d[6][9];
 
 // views of a device matrix for the kernel launchers (SELL copy when present)
 #include "fm_kernels.h"

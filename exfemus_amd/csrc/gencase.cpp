@@ -16,7 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
-#include "fm_internal.h"
+#include "fm_mesh.h"
 
 const int fm_quad9_ref[9][2] = {{-1,-1},{1,-1},{1,1},{-1,1},{0,-1},{1,0},{0,1},{-1,0},{0,0}};
 const int fm_hex27_ref[27][3] = {

@@ -1,5 +1,6 @@
 """Builds tests/cpp/_build/mgsol_driver (the C++ host-layer driver) against the in-tree libfemus_hip.so, and
-tests/cpp/_build/sell_layout_check (the host SELL layout checker)."""
+the host checkers tests/cpp/_build/sell_layout_check (SELL layout) and tests/cpp/_build/order_check (internal
+DoF order)."""
 import os
 import subprocess
 
@@ -9,13 +10,19 @@ OUT = os.path.join(HERE, "_build", "mgsol_driver")
 
 
 CHECK = os.path.join(HERE, "_build", "sell_layout_check")
+ORDER = os.path.join(HERE, "_build", "order_check")
+CSRC = os.path.join(ROOT, "exfemus_amd", "csrc")
 
 
 def build_check():
-    """The host SELL layout checker: sell_layout_check.cpp with the library's host layout builder (no GPU)."""
+    """The host checkers, each built with the library's own host units (no GPU): sell_layout_check.cpp with the
+    SELL layout builder, order_check.cpp with the mesh builder and the internal DoF order."""
     os.makedirs(os.path.dirname(CHECK), exist_ok=True)
     cmd = ["g++", "-O2", "-std=c++17", "-fopenmp", "-Wall", os.path.join(HERE, "sell_layout_check.cpp"),
-           os.path.join(ROOT, "exfemus_amd", "csrc", "sell_host.cpp"), "-o", CHECK]
+           os.path.join(CSRC, "sell_host.cpp"), "-o", CHECK]
+    subprocess.run(cmd, check=True)
+    cmd = ["g++", "-O2", "-std=c++17", "-fopenmp", "-Wall", os.path.join(HERE, "order_check.cpp"),
+           os.path.join(CSRC, "gencase.cpp"), os.path.join(CSRC, "order_host.cpp"), "-o", ORDER]
     subprocess.run(cmd, check=True)
     return CHECK
 
