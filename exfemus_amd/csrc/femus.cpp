@@ -40,6 +40,28 @@ template <typename T> static void dfree(T *&p) { if (p) { (void)hipFree(p); p = 
 // ===========================================================================
 // context
 // ===========================================================================
+// frees what a context holds (any subset of it: also a context whose creation failed half-way)
+static void ctx_release(fm_ctx *c) {
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  dfree(c->scratch);
+  if (c->h_scratch) (void)hipHostFree(c->h_scratch);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+static int ctx_setup(fm_ctx *c) {
+  FM_HIP(hipSetDevice(c->device));
+  FM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  FM_HIP(hipEventCreate(&c->ev0));
+  FM_HIP(hipEventCreate(&c->ev1));
+  FM_CHECK(dalloc(&c->scratch, 4096));
+  FM_HIP(hipHostMalloc((void **)&c->h_scratch, 4096 * sizeof(double), hipHostMallocDefault));
+  return FM_OK;
+}
+
 extern "C" int fm_ctx_create(int device, fm_ctx **out) {
   if (!out) return FM_EINVAL;
   int ndev = 0;
@@ -47,12 +69,8 @@ extern "C" int fm_ctx_create(int device, fm_ctx **out) {
   if (device < 0 || device >= ndev) return FM_EINVAL;
   fm_ctx *c = new fm_ctx();
   c->device = device;
-  FM_HIP(hipSetDevice(device));
-  FM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  FM_HIP(hipEventCreate(&c->ev0));
-  FM_HIP(hipEventCreate(&c->ev1));
-  FM_CHECK(dalloc(&c->scratch, 4096));
-  FM_HIP(hipHostMalloc((void **)&c->h_scratch, 4096 * sizeof(double), hipHostMallocDefault));
+  const int rc = ctx_setup(c);
+  if (rc != FM_OK) { ctx_release(c); return rc; }
   *out = c;
   return FM_OK;
 }
@@ -60,14 +78,7 @@ extern "C" int fm_ctx_create(int device, fm_ctx **out) {
 extern "C" int fm_ctx_destroy(fm_ctx *c) {
   if (!c) return FM_EINVAL;
   if (c->nsys > 0) { fm_set_error("fm_ctx_destroy: systems still alive on this context"); return FM_ESTATE; }
-  (void)hipSetDevice(c->device);
-  (void)hipStreamSynchronize(c->stream);
-  dfree(c->scratch);
-  if (c->h_scratch) (void)hipHostFree(c->h_scratch);
-  (void)hipEventDestroy(c->ev0);
-  (void)hipEventDestroy(c->ev1);
-  (void)hipStreamDestroy(c->stream);
-  delete c;
+  ctx_release(c);
   return FM_OK;
 }
 
@@ -178,17 +189,6 @@ static void build_tables(int dim, FeTab &t) {
 // ===========================================================================
 // system
 // ===========================================================================
-struct SysExtra {
-  FeTab *d_tab = nullptr;
-  int *d_elems_tmp = nullptr;
-};
-static std::vector<std::pair<fm_system *, SysExtra>> g_extra;  // tables per system
-static SysExtra &extra(fm_system *s) {
-  for (auto &p : g_extra) if (p.first == s) return p.second;
-  g_extra.push_back({s, SysExtra()});
-  return g_extra.back().second;
-}
-
 extern "C" int fm_system_create(fm_ctx *ctx, const fm_mesh *m, int nq2, int nq1, fm_system **out) {
   if (!ctx || !m || !out || nq2 < 0 || nq1 < 0 || nq2 + nq1 == 0 || nq2 + nq1 > 8) return FM_EINVAL;
   if (nq1 > 0 && nq2 == 0) return FM_EINVAL;
@@ -336,9 +336,7 @@ extern "C" int fm_system_destroy(fm_system *s) {
   for (auto &l : s->alev) fm_free_level(l);
   dfree(s->d_bc0); dfree(s->d_bc1); dfree(s->d_dof_top); dfree(s->xyz); dfree(s->x_old); dfree(s->vel); dfree(s->xbc);
   dfree(s->tnode);
-  SysExtra &ex = extra(s);
-  dfree(ex.d_tab);
-  for (size_t i = 0; i < g_extra.size(); i++) if (g_extra[i].first == s) { g_extra.erase(g_extra.begin() + i); break; }
+  dfree(s->d_tab);
   s->ctx->nsys--;
   delete s;
   return FM_OK;
@@ -486,12 +484,11 @@ extern "C" int fm_system_init(fm_system *s) {
   FM_HIP(hipMemcpyAsync(s->d_dof_top, s->node_dof[top].data(), (size_t)nv * N * sizeof(int), hipMemcpyHostToDevice, st));
   FM_HIP(hipMemcpyAsync(s->xyz, m.xyz.data(), m.xyz.size() * sizeof(double), hipMemcpyHostToDevice, st));
   FM_HIP(hipMemsetAsync(s->x_old, 0, nt * sizeof(double), st));
-  SysExtra &ex = extra(s);
-  if (!ex.d_tab) {
+  if (!s->d_tab) {
     FeTab t;
     build_tables(m.dim, t);
-    FM_CHECK(dalloc(&ex.d_tab, 1));
-    FM_HIP(hipMemcpy(ex.d_tab, &t, sizeof(FeTab), hipMemcpyHostToDevice));
+    FM_CHECK(dalloc(&s->d_tab, 1));
+    FM_HIP(hipMemcpy(s->d_tab, &t, sizeof(FeTab), hipMemcpyHostToDevice));
   }
   for (int L = 0; L < nlev; L++) {
     Level &l = s->lev[L];
@@ -701,7 +698,6 @@ static int bc_rhs(fm_system *s, const void *params, const double *g, double *rhs
   int top = (int)s->lev.size() - 1;
   Level &l = s->lev[top];
   hipStream_t st = s->ctx->stream;
-  SysExtra &ex = extra(s);
   FM_HIP(hipMemcpyAsync(l.t, g, l.n * sizeof(double), hipMemcpyHostToDevice, st));
   FM_CHECK(fmk::fill<double>(st, l.n, l.d, 0.0));
   const bool heat = s->nq2 == 1 && s->nq1 == 0;
@@ -717,7 +713,7 @@ static int bc_rhs(fm_system *s, const void *params, const double *g, double *rhs
       a.upwind = p->upwind;
       for (int d = 0; d < 3; d++) a.vel_const[d] = p->vel_const[d];
       FM_CHECK(fmk::heat_assemble(st, m.dim, n, el, l.conn, l.eldof, l.elpos, l.elslot, l.bmask, s->xyz, m.Ntop, s->d_dof_top,
-                                  s->d_bc0, s->d_bc1, s->x_old, l.t, l.coef, s->vel, ex.d_tab, a, 1, 0, 1, nullptr, l.d));
+                                  s->d_bc0, s->d_bc1, s->x_old, l.t, l.coef, s->vel, s->d_tab, a, 1, 0, 1, nullptr, l.d));
     } else {
       const fm_ns_params *p = (const fm_ns_params *)params;
       if (p->dt == 0.) return FM_EINVAL;
@@ -726,7 +722,7 @@ static int bc_rhs(fm_system *s, const void *params, const double *g, double *rhs
       for (int d = 0; d < 3; d++) { a.dirg[d] = p->dirg[d]; a.src[d] = p->src[d]; a.buoy[d] = 0.; }
       a.Tn = nullptr;
       FM_CHECK(fmk::ns_assemble(st, m.dim, n, el, l.conn, l.eldof, l.elpos, l.elslot, l.bmask, s->xyz, m.Ntop, s->d_dof_top,
-                                s->d_bc0, s->d_bc1, s->x_old, l.t, l.coef, ex.d_tab, a, 1, 0, 1, nullptr, l.d));
+                                s->d_bc0, s->d_bc1, s->x_old, l.t, l.coef, s->d_tab, a, 1, 0, 1, nullptr, l.d));
     }
   }
   FM_HIP(hipMemcpyAsync(rhs, l.d, l.n * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -841,11 +837,10 @@ extern "C" int fm_heat_genmatrhs(fm_system *s, int L, int mode, const fm_heat_pa
   for (int d = 0; d < 3; d++) a.vel_const[d] = p->vel_const[d];
   FM_HIP(hipMemsetAsync(l.A.sell.v, 0, l.A.sell.nslot * sizeof(double), st));
   if (mode == 1) FM_HIP(hipMemsetAsync(l.b, 0, l.n * sizeof(double), st));
-  SysExtra &ex = extra(s);
   for (int k = 0; k < l.ncolor; k++) {
     int n = l.color_off[k + 1] - l.color_off[k];
     FM_CHECK(fmk::heat_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldofi, l.elpos, l.elslot, l.bmask, s->xyz,
-                                m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, s->xbc, l.coef, s->vel, ex.d_tab, a,
+                                m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, s->xbc, l.coef, s->vel, s->d_tab, a,
                                 mode, 0, 0, l.A.sell.v, l.b));
   }
   FM_CHECK(after_assembly(s, L));
@@ -866,11 +861,10 @@ extern "C" int fm_ns_genmatrhs(fm_system *s, int L, int mode, const fm_ns_params
   a.Tn = s->tnode;
   FM_HIP(hipMemsetAsync(l.A.sell.v, 0, l.A.sell.nslot * sizeof(double), st));
   if (mode == 1) FM_HIP(hipMemsetAsync(l.b, 0, l.n * sizeof(double), st));
-  SysExtra &ex = extra(s);
   for (int k = 0; k < l.ncolor; k++) {
     int n = l.color_off[k + 1] - l.color_off[k];
     FM_CHECK(fmk::ns_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldofi, l.elpos, l.elslot, l.bmask, s->xyz,
-                              m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, s->xbc, l.coef, ex.d_tab, a, mode, 0, 0,
+                              m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, s->xbc, l.coef, s->d_tab, a, mode, 0, 0,
                               l.A.sell.v, l.b));
   }
   FM_CHECK(after_assembly(s, L));
@@ -1068,11 +1062,10 @@ extern "C" int fm_heat_genmat_transpose(fm_system *s, int L, const fm_heat_param
   h.alpha0 = p->alpha0; h.rhocp = p->rhocp; h.dt = p->dt; h.tc = p->time_coeff; h.source = p->source; h.upwind = p->upwind;
   for (int d = 0; d < 3; d++) h.vel_const[d] = p->vel_const[d];
   FM_HIP(hipMemsetAsync(a.A.sell.v, 0, a.A.sell.nslot * sizeof(double), st));
-  SysExtra &ex = extra(s);
   for (int k = 0; k < l.ncolor; k++) {
     int n = l.color_off[k + 1] - l.color_off[k];
     FM_CHECK(fmk::heat_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldof, l.elpos, l.elslot, l.bmask, s->xyz,
-                                m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, s->xbc, l.coef, s->vel, ex.d_tab, h,
+                                m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, s->xbc, l.coef, s->vel, s->d_tab, h,
                                 0, 1, 0, a.A.sell.v, nullptr));
   }
   return after_assembly_T(s, L);
@@ -1091,11 +1084,10 @@ extern "C" int fm_ns_genmat_transpose(fm_system *s, int L, const fm_ns_params *p
   for (int d = 0; d < 3; d++) { h.dirg[d] = p->dirg[d]; h.src[d] = p->src[d]; h.buoy[d] = 0.; }
   h.Tn = nullptr;
   FM_HIP(hipMemsetAsync(a.A.sell.v, 0, a.A.sell.nslot * sizeof(double), st));
-  SysExtra &ex = extra(s);
   for (int k = 0; k < l.ncolor; k++) {
     int n = l.color_off[k + 1] - l.color_off[k];
     FM_CHECK(fmk::ns_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldof, l.elpos, l.elslot, l.bmask, s->xyz,
-                              m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, s->xbc, l.coef, ex.d_tab, h, 0, 1, 0,
+                              m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, s->xbc, l.coef, s->d_tab, h, 0, 1, 0,
                               a.A.sell.v, nullptr));
   }
   return after_assembly_T(s, L);
@@ -1120,11 +1112,10 @@ extern "C" int fm_ns_genjac(fm_system *s, int L, int trans, const fm_ns_params *
   a.Tn = nullptr;
   a.newton = 1;
   FM_HIP(hipMemsetAsync(val, 0, nslot * sizeof(double), st));
-  SysExtra &ex = extra(s);
   for (int k = 0; k < l.ncolor; k++) {
     int n = l.color_off[k + 1] - l.color_off[k];
     FM_CHECK(fmk::ns_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldofi, l.elpos, l.elslot, l.bmask, s->xyz,
-                              m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, s->xbc, l.coef, ex.d_tab, a, 0, trans, 0,
+                              m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, s->xbc, l.coef, s->d_tab, a, 0, trans, 0,
                               val, nullptr));
   }
   if (trans) return after_assembly_T(s, L);
@@ -1155,12 +1146,11 @@ extern "C" int fm_heat_dissipation(fm_system *s, const fm_heat_params *p, const 
   FM_CHECK(fmk::fill<double>(st, l.n, l.d, 0.0));
   double *diss = nullptr;
   FM_CHECK(dalloc(&diss, l.nel));
-  SysExtra &ex = extra(s);
   int rc = 0;
   for (int k = 0; k < l.ncolor && !rc; k++) {
     int n = l.color_off[k + 1] - l.color_off[k];
     rc = fmk::heat_lap(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldof, s->xyz, m.Ntop, s->d_dof_top,
-                       s->d_bc0, l.t, nullptr, l.coef, ex.d_tab, diss, nullptr, l.d, 2.0 * p->alpha0);
+                       s->d_bc0, l.t, nullptr, l.coef, s->d_tab, diss, nullptr, l.d, 2.0 * p->alpha0);
   }
   std::vector<double> h(l.nel), hk(l.nel, 1.0);
   if (!rc) {
@@ -1193,9 +1183,8 @@ extern "C" int fm_heat_grad_coef(fm_system *s, const fm_heat_params *p, const do
   FM_HIP(hipMemcpyAsync(l.d2, lam_host, l.n * sizeof(double), hipMemcpyHostToDevice, st));
   double *adj = nullptr;
   FM_CHECK(dalloc(&adj, l.nel));
-  SysExtra &ex = extra(s);
   int rc = fmk::heat_lap(st, m.dim, l.nel, l.color_elems, l.conn, l.eldof, s->xyz, m.Ntop, s->d_dof_top, s->d_bc0,
-                         l.t, l.d2, l.coef, ex.d_tab, nullptr, adj, nullptr, 0.0);
+                         l.t, l.d2, l.coef, s->d_tab, nullptr, adj, nullptr, 0.0);
   std::vector<double> h(l.nel);
   if (!rc) {
     FM_HIP(hipMemcpyAsync(h.data(), adj, l.nel * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1246,12 +1235,11 @@ extern "C" int fm_ns_dissipation(fm_system *s, const fm_ns_params *p, const doub
   FM_CHECK(fmk::fill<double>(st, l.n, l.d, 0.0));
   double *diss = nullptr;
   FM_CHECK(dalloc(&diss, l.nel));
-  SysExtra &ex = extra(s);
   int rc = 0;
   for (int k = 0; k < l.ncolor && !rc; k++) {
     int n = l.color_off[k + 1] - l.color_off[k];
     rc = fmk::ns_visc(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldof, s->xyz, m.Ntop, s->d_dof_top,
-                      s->d_bc0, l.t, nullptr, l.coef, p->IRe, ex.d_tab, diss, nullptr, l.d, 2.0);
+                      s->d_bc0, l.t, nullptr, l.coef, p->IRe, s->d_tab, diss, nullptr, l.d, 2.0);
   }
   std::vector<double> h(l.nel), hk(l.nel, p->IRe);
   if (!rc) {
@@ -1284,9 +1272,8 @@ extern "C" int fm_ns_grad_coef(fm_system *s, const fm_ns_params *p, const double
   FM_HIP(hipMemcpyAsync(l.d2, lam_host, l.n * sizeof(double), hipMemcpyHostToDevice, st));
   double *adj = nullptr;
   FM_CHECK(dalloc(&adj, l.nel));
-  SysExtra &ex = extra(s);
   int rc = fmk::ns_visc(st, m.dim, l.nel, l.color_elems, l.conn, l.eldof, s->xyz, m.Ntop, s->d_dof_top, s->d_bc0,
-                        l.t, l.d2, l.coef, p->IRe, ex.d_tab, nullptr, adj, nullptr, 0.0);
+                        l.t, l.d2, l.coef, p->IRe, s->d_tab, nullptr, adj, nullptr, 0.0);
   std::vector<double> h(l.nel);
   if (!rc) {
     FM_HIP(hipMemcpyAsync(h.data(), adj, l.nel * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1377,11 +1364,10 @@ extern "C" int fm_heat_coupling(fm_system *s, const fm_heat_params *p, int mode,
   if (!rc && mode == 1 && hipMemcpyAsync(l.d, lam_host, l.n * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess) rc = FM_EHIP;
   if (!rc && mode == 2 && hipMemcpyAsync(dm, m_host, m.Ntop * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess) rc = FM_EHIP;
   if (!rc) rc = fmk::fill<double>(st, nout, dout, 0.0);
-  SysExtra &ex = extra(s);
   for (int k = 0; k < l.ncolor && !rc; k++) {
     int n = l.color_off[k + 1] - l.color_off[k];
     rc = fmk::coupling(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, s->xyz, m.Ntop, s->d_dof_top, s->d_bc0, l.t,
-                       l.d, dm, scale, mode, s->vel, p->upwind, p->rhocp, p->alpha0, ex.d_tab, dout);
+                       l.d, dm, scale, mode, s->vel, p->upwind, p->rhocp, p->alpha0, s->d_tab, dout);
   }
   if (!rc && (hipMemcpyAsync(out_host, dout, nout * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
               hipStreamSynchronize(st) != hipSuccess)) rc = FM_EHIP;
@@ -1398,7 +1384,6 @@ extern "C" int fm_heat_nusselt(fm_system *s, const fm_heat_params *p, int axis, 
   Level &l = s->lev[top];
   hipStream_t st = s->ctx->stream;
   const fm_mesh &m = *s->mesh;
-  SysExtra &ex = extra(s);
   FM_HIP(hipMemcpyAsync(l.t, T_host, l.n * sizeof(double), hipMemcpyHostToDevice, st));
   FM_CHECK(fmk::fill<double>(st, l.n, l.d, 0.0));
   double *jel = nullptr, *dk = nullptr;
@@ -1408,7 +1393,7 @@ extern "C" int fm_heat_nusselt(fm_system *s, const fm_heat_params *p, int axis, 
   for (int k = 0; k < l.ncolor && !rc; k++) {
     int n = l.color_off[k + 1] - l.color_off[k];
     rc = fmk::heat_nusselt(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldof, s->xyz, m.Ntop, s->d_dof_top,
-                           s->vel, l.t, l.coef, p->alpha0, axis, ex.d_tab, jel, dk, l.d);
+                           s->vel, l.t, l.coef, p->alpha0, axis, s->d_tab, jel, dk, l.d);
   }
   std::vector<double> h(l.nel);
   if (!rc) {

@@ -31,9 +31,8 @@ void fm_set_error(const std::string &msg);
 //          32-bit word, the 64 words of a group contiguous);
 //   16-bit (spans <= 65535): c16[sl_c16[s] + 128 (k/2) + 2 l + k%2];
 //   32-bit absolute columns otherwise: c32[sl_c32[s] + 64 k + l].
-// The offsets of the other classes are -1.  map[slot] is the CSR position of
-// the entry (-1 for padding; padding repeats the slot's base column with
-// value 0).
+// The offsets of the other classes are -1.  Padding repeats the slot's base
+// column with value 0; pos / rbase / rslot tie every slot to its CSR entry.
 struct DevSELL {
   int64_t nsl = 0, nslot = 0, n32 = 0, n8 = 0, n16 = 0;
   int64_t *sl_off = nullptr, *sl_c32 = nullptr, *sl_c8 = nullptr, *sl_c16 = nullptr;
@@ -132,6 +131,7 @@ struct GraphEntry {
   hipGraphExec_t exec;
 };
 
+struct FeTab;
 struct fm_system {
   fm_ctx *ctx = nullptr;
   const fm_mesh *mesh = nullptr;
@@ -159,6 +159,7 @@ struct fm_system {
   bool mg_ready = false;
   int mg_smoother = -1;
   int vanka_store = 3;  // precisions of the stored Vanka inverses: bit 0 FP64 (FP64 cycles), bit 1 FP32 (FP32 cycles)
+  FeTab *d_tab = nullptr;  // reference-element tables on the device (built at the first fm_system_init)
 };
 
 // views of a device matrix for the kernel launchers (SELL copy when present)
