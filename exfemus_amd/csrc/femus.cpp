@@ -16,18 +16,11 @@
 #include "sell_host.h"
 
 static thread_local std::string g_err;
-void fm_set_error(const std::string &msg) { g_err = msg; }
+static thread_local unsigned g_nerr = 0;
+void fm_set_error(const std::string &msg) { g_err = msg; g_nerr++; }
+unsigned fm_error_count() { return g_nerr; }
 extern "C" const char *fm_last_error(void) { return g_err.c_str(); }
 extern "C" int fm_set_sell_split(int W) { return fmk::sell_split(W); }
-
-#define FM_CHECK(x)                                 \
-  do {                                              \
-    int rc_ = (x);                                  \
-    if (rc_ != 0) {                                 \
-      if (g_err.empty()) fm_set_error(#x);          \
-      return rc_ < 0 ? rc_ : FM_EHIP;               \
-    }                                               \
-  } while (0)
 
 template <typename T> static int dalloc(T **p, size_t n) {
   if (n == 0) n = 1;
@@ -63,10 +56,10 @@ static int ctx_setup(fm_ctx *c) {
 }
 
 extern "C" int fm_ctx_create(int device, fm_ctx **out) {
-  if (!out) return FM_EINVAL;
+  if (!out) return fm_einval(__func__);
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) { fm_set_error("no HIP device"); return FM_EHIP; }
-  if (device < 0 || device >= ndev) return FM_EINVAL;
+  if (device < 0 || device >= ndev) return fm_einval(__func__);
   fm_ctx *c = new fm_ctx();
   c->device = device;
   const int rc = ctx_setup(c);
@@ -76,14 +69,14 @@ extern "C" int fm_ctx_create(int device, fm_ctx **out) {
 }
 
 extern "C" int fm_ctx_destroy(fm_ctx *c) {
-  if (!c) return FM_EINVAL;
+  if (!c) return fm_einval(__func__);
   if (c->nsys > 0) { fm_set_error("fm_ctx_destroy: systems still alive on this context"); return FM_ESTATE; }
   ctx_release(c);
   return FM_OK;
 }
 
 extern "C" int fm_sync(fm_ctx *c) {
-  if (!c) return FM_EINVAL;
+  if (!c) return fm_einval(__func__);
   FM_HIP(hipStreamSynchronize(c->stream));
   return FM_OK;
 }
@@ -93,7 +86,7 @@ extern "C" int fm_sync(fm_ctx *c) {
 // ===========================================================================
 extern "C" int fm_mesh_create_box(int dim, const int nx0[3], const double lo[3], const double hi[3], int nlev,
                                   fm_mesh **out) {
-  if (!out || !nx0 || !lo || !hi) return FM_EINVAL;
+  if (!out || !nx0 || !lo || !hi) return fm_einval(__func__);
   fm_mesh *m = new fm_mesh();
   int rc = fm_build_box(dim, nx0, lo, hi, nlev, m);
   if (rc) { delete m; fm_set_error("fm_mesh_create_box: invalid sizes"); return rc; }
@@ -102,24 +95,24 @@ extern "C" int fm_mesh_create_box(int dim, const int nx0[3], const double lo[3],
 }
 extern "C" int fm_mesh_destroy(fm_mesh *m) { delete m; return FM_OK; }
 extern "C" int fm_mesh_info(const fm_mesh *m, int *info) {
-  if (!m || !info) return FM_EINVAL;
+  if (!m || !info) return fm_einval(__func__);
   info[0] = m->dim; info[1] = m->nlev; info[2] = m->n2; info[3] = m->n1; info[4] = m->nsides; info[5] = m->Ntop;
   return FM_OK;
 }
 extern "C" int fm_mesh_level(const fm_mesh *m, int L, int *n_el, int *n_q2, int *n_q1) {
-  if (!m || L < 0 || L >= m->nlev) return FM_EINVAL;
+  if (!m || L < 0 || L >= m->nlev) return fm_einval(__func__);
   if (n_el) *n_el = m->nel[L];
   if (n_q2) *n_q2 = m->nq2[L];
   if (n_q1) *n_q1 = m->nq1[L];
   return FM_OK;
 }
 extern "C" int fm_mesh_coords(const fm_mesh *m, double *xyz) {
-  if (!m || !xyz) return FM_EINVAL;
+  if (!m || !xyz) return fm_einval(__func__);
   std::memcpy(xyz, m->xyz.data(), m->xyz.size() * sizeof(double));
   return FM_OK;
 }
 extern "C" int fm_mesh_conn(const fm_mesh *m, int L, int *conn, int *neigh, int *parent) {
-  if (!m || L < 0 || L >= m->nlev) return FM_EINVAL;
+  if (!m || L < 0 || L >= m->nlev) return fm_einval(__func__);
   if (conn) std::memcpy(conn, m->conn[L].data(), m->conn[L].size() * sizeof(int));
   if (neigh) std::memcpy(neigh, m->neigh[L].data(), m->neigh[L].size() * sizeof(int));
   if (parent) std::memcpy(parent, m->parent[L].data(), m->parent[L].size() * sizeof(int));
@@ -190,8 +183,8 @@ static void build_tables(int dim, FeTab &t) {
 // system
 // ===========================================================================
 extern "C" int fm_system_create(fm_ctx *ctx, const fm_mesh *m, int nq2, int nq1, fm_system **out) {
-  if (!ctx || !m || !out || nq2 < 0 || nq1 < 0 || nq2 + nq1 == 0 || nq2 + nq1 > 8) return FM_EINVAL;
-  if (nq1 > 0 && nq2 == 0) return FM_EINVAL;
+  if (!ctx || !m || !out || nq2 < 0 || nq1 < 0 || nq2 + nq1 == 0 || nq2 + nq1 > 8) return fm_einval(__func__);
+  if (nq1 > 0 && nq2 == 0) return fm_einval(__func__);
   fm_system *s = new fm_system();
   s->ctx = ctx; s->mesh = m; s->nq2 = nq2; s->nq1 = nq1;
   s->nloc = nq2 * m->n2 + nq1 * m->n1;
@@ -328,7 +321,7 @@ void fm_drop_graphs(fm_system *s) {
 }
 
 extern "C" int fm_system_destroy(fm_system *s) {
-  if (!s) return FM_EINVAL;
+  if (!s) return fm_einval(__func__);
   fm_drop_graphs(s);
   (void)hipSetDevice(s->ctx->device);
   (void)hipStreamSynchronize(s->ctx->stream);
@@ -347,7 +340,7 @@ extern "C" int64_t fm_system_ndof(const fm_system *s, int L) {
   return s->Dim[L];
 }
 extern "C" int fm_system_node_dof(const fm_system *s, int L, int *nd) {
-  if (!s || !nd || L < 0 || L >= (int)s->Dim.size()) return FM_EINVAL;
+  if (!s || !nd || L < 0 || L >= (int)s->Dim.size()) return fm_einval(__func__);
   std::memcpy(nd, s->node_dof[L].data(), s->node_dof[L].size() * sizeof(int));
   return FM_OK;
 }
@@ -356,7 +349,7 @@ extern "C" int fm_system_node_dof(const fm_system *s, int L, int *nd) {
 // default (bc_intern_read: all 1); nodes of top-level boundary faces take
 // bc_read(xp).  Q1 variables are set on face vertices only.
 extern "C" int fm_system_genbc(fm_system *s, fm_bc_fn fn, void *user) {
-  if (!s || !fn) return FM_EINVAL;
+  if (!s || !fn) return fm_einval(__func__);
   const fm_mesh &m = *s->mesh;
   int top = m.nlev - 1, N = m.Ntop, dim = m.dim, nv = s->nq2 + s->nq1;
   int nb1 = dim == 2 ? 2 : 4;
@@ -387,7 +380,7 @@ extern "C" int fm_system_genbc(fm_system *s, fm_bc_fn fn, void *user) {
 }
 
 extern "C" int fm_system_get_bc(const fm_system *s, int *bc0, int *bc1) {
-  if (!s) return FM_EINVAL;
+  if (!s) return fm_einval(__func__);
   if (bc0) std::memcpy(bc0, s->bc0.data(), s->bc0.size() * sizeof(int));
   if (bc1) std::memcpy(bc1, s->bc1.data(), s->bc1.size() * sizeof(int));
   return FM_OK;
@@ -466,7 +459,7 @@ static int colour_slices(hipStream_t st, Level &l) {
 }
 
 extern "C" int fm_system_init(fm_system *s) {
-  if (!s) return FM_EINVAL;
+  if (!s) return fm_einval(__func__);
   fm_ctx *c = s->ctx;
   FM_HIP(hipSetDevice(c->device));
   hipStream_t st = c->stream;
@@ -587,7 +580,7 @@ extern "C" int64_t fm_system_csr_nnz(const fm_system *s, int which, int L) {
 }
 
 extern "C" int fm_system_sell_info(const fm_system *s, int which, int L, int64_t *info) {
-  if (!s || !info || L < 0 || L >= (int)s->lev.size() || !s->initialized || which < 0 || which > 2) return FM_EINVAL;
+  if (!s || !info || L < 0 || L >= (int)s->lev.size() || !s->initialized || which < 0 || which > 2) return fm_einval(__func__);
   const Level &l = s->lev[L];
   const DevCSR &c = which == 0 ? l.A : (which == 1 ? l.R : l.P);
   info[0] = c.sell.nsl; info[1] = c.sell.nslot; info[2] = c.sell.n8; info[3] = c.sell.n16; info[4] = c.sell.n32;
@@ -596,10 +589,10 @@ extern "C" int fm_system_sell_info(const fm_system *s, int which, int L, int64_t
 }
 
 extern "C" int fm_system_csr_get(const fm_system *s, int which, int L, int64_t *rowptr, int *colind, double *val) {
-  if (!s || L < 0 || L >= (int)s->lev.size() || !s->initialized || which < 0 || which > 2) return FM_EINVAL;
+  if (!s || L < 0 || L >= (int)s->lev.size() || !s->initialized || which < 0 || which > 2) return fm_einval(__func__);
   const Level &l = s->lev[L];
   const DevCSR &c = which == 0 ? l.A : (which == 1 ? l.R : l.P);
-  if (c.nr == 0) return FM_EINVAL;
+  if (c.nr == 0) return fm_einval(__func__);
   FM_HIP(hipSetDevice(s->ctx->device));
   if (which == 0 && val) FM_CHECK(fm_csr_from_sell(s->ctx->stream, const_cast<DevCSR &>(c)));
   FM_HIP(hipStreamSynchronize(s->ctx->stream));
@@ -643,27 +636,27 @@ static int vec_from_host(fm_system *s, int L, const double *host, double *v) {
 }
 extern "C" int fm_system_get_x(const fm_system *s, int L, double *x) {
   FM_CHECK(check_level(s, L));
-  if (!x) return FM_EINVAL;
+  if (!x) return fm_einval(__func__);
   return vec_to_host(s, L, s->lev[L].x, x);
 }
 extern "C" int fm_system_set_x(fm_system *s, int L, const double *x) {
   FM_CHECK(check_level(s, L));
-  if (!x) return FM_EINVAL;
+  if (!x) return fm_einval(__func__);
   return vec_from_host(s, L, x, s->lev[L].x);
 }
 extern "C" int fm_system_get_rhs(const fm_system *s, int L, double *b) {
   FM_CHECK(check_level(s, L));
-  if (!b) return FM_EINVAL;
+  if (!b) return fm_einval(__func__);
   return vec_to_host(s, L, s->lev[L].b, b);
 }
 extern "C" int fm_system_set_rhs(fm_system *s, int L, const double *b) {
   FM_CHECK(check_level(s, L));
-  if (!b) return FM_EINVAL;
+  if (!b) return fm_einval(__func__);
   return vec_from_host(s, L, b, s->lev[L].b);
 }
 extern "C" int fm_get_res(const fm_system *s, int L, double *r) {
   FM_CHECK(check_level(s, L));
-  if (!r) return FM_EINVAL;
+  if (!r) return fm_einval(__func__);
   return vec_to_host(s, L, s->lev[L].res, r);
 }
 
@@ -693,7 +686,7 @@ extern "C" int fm_system_set_bcdata(fm_system *s, const double *g) {
 // numbering in and out), matrix untouched: the linear map g -> b_bd(g).
 static int bc_rhs(fm_system *s, const void *params, const double *g, double *rhs) {
   FM_CHECK(check_level(s, 0));
-  if (!params || !g || !rhs) return FM_EINVAL;
+  if (!params || !g || !rhs) return fm_einval(__func__);
   const fm_mesh &m = *s->mesh;
   int top = (int)s->lev.size() - 1;
   Level &l = s->lev[top];
@@ -701,13 +694,13 @@ static int bc_rhs(fm_system *s, const void *params, const double *g, double *rhs
   FM_HIP(hipMemcpyAsync(l.t, g, l.n * sizeof(double), hipMemcpyHostToDevice, st));
   FM_CHECK(fmk::fill<double>(st, l.n, l.d, 0.0));
   const bool heat = s->nq2 == 1 && s->nq1 == 0;
-  if (!heat && !(s->nq2 == m.dim && s->nq1 == 1)) return FM_EINVAL;
+  if (!heat && !(s->nq2 == m.dim && s->nq1 == 1)) return fm_einval(__func__);
   for (int k = 0; k < l.ncolor; k++) {
     int n = l.color_off[k + 1] - l.color_off[k];
     const int *el = l.color_elems + l.color_off[k];
     if (heat) {
       const fm_heat_params *p = (const fm_heat_params *)params;
-      if (p->dt == 0.) return FM_EINVAL;
+      if (p->dt == 0.) return fm_einval(__func__);
       HeatArgs a;
       a.alpha0 = p->alpha0; a.rhocp = p->rhocp; a.dt = p->dt; a.tc = p->time_coeff; a.source = p->source;
       a.upwind = p->upwind;
@@ -716,7 +709,7 @@ static int bc_rhs(fm_system *s, const void *params, const double *g, double *rhs
                                   s->d_bc0, s->d_bc1, s->x_old, l.t, l.coef, s->vel, s->d_tab, a, 1, 0, 1, nullptr, l.d));
     } else {
       const fm_ns_params *p = (const fm_ns_params *)params;
-      if (p->dt == 0.) return FM_EINVAL;
+      if (p->dt == 0.) return fm_einval(__func__);
       NsArgs a;
       a.IRe = p->IRe; a.dt = p->dt; a.tc = p->time_coeff; a.upwind = p->upwind; a.IFr = p->IFr;
       for (int d = 0; d < 3; d++) { a.dirg[d] = p->dirg[d]; a.src[d] = p->src[d]; a.buoy[d] = 0.; }
@@ -731,11 +724,11 @@ static int bc_rhs(fm_system *s, const void *params, const double *g, double *rhs
 }
 
 extern "C" int fm_heat_bc_rhs(fm_system *s, const fm_heat_params *p, const double *g, double *rhs) {
-  if (!s || s->nq2 != 1 || s->nq1 != 0) return FM_EINVAL;
+  if (!s || s->nq2 != 1 || s->nq1 != 0) return fm_einval(__func__);
   return bc_rhs(s, p, g, rhs);
 }
 extern "C" int fm_ns_bc_rhs(fm_system *s, const fm_ns_params *p, const double *g, double *rhs) {
-  if (!s || !s->mesh || s->nq2 != s->mesh->dim || s->nq1 != 1) return FM_EINVAL;
+  if (!s || !s->mesh || s->nq2 != s->mesh->dim || s->nq1 != 1) return fm_einval(__func__);
   return bc_rhs(s, p, g, rhs);
 }
 
@@ -743,9 +736,9 @@ extern "C" int fm_ns_bc_rhs(fm_system *s, const fm_ns_params *p, const double *g
 // temperature field (top-level nodes); NULL T removes it.
 extern "C" int fm_ns_set_buoyancy(fm_system *s, const double *T_node, const double coef[3]) {
   FM_CHECK(check_level(s, 0));
-  if (!s->mesh || s->nq2 != s->mesh->dim || s->nq1 != 1) return FM_EINVAL;
+  if (!s->mesh || s->nq2 != s->mesh->dim || s->nq1 != 1) return fm_einval(__func__);
   if (!T_node) { dfree(s->tnode); for (int d = 0; d < 3; d++) s->buoy[d] = 0.; return FM_OK; }
-  if (!coef) return FM_EINVAL;
+  if (!coef) return fm_einval(__func__);
   const size_t N = (size_t)s->mesh->Ntop;
   if (!s->tnode) FM_CHECK(dalloc(&s->tnode, N));
   FM_HIP(hipMemcpyAsync(s->tnode, T_node, N * sizeof(double), hipMemcpyHostToDevice, s->ctx->stream));
@@ -770,7 +763,7 @@ extern "C" int fm_system_set_velocity(fm_system *s, const double *vel) {
 extern "C" int fm_system_couple(fm_system *dst, const fm_system *src, int which, const double coef[3]) {
   FM_CHECK(check_level(dst, 0));
   FM_CHECK(check_level(src, 0));
-  if (dst->ctx != src->ctx || dst->mesh != src->mesh || which < 0 || which > 1) return FM_EINVAL;
+  if (dst->ctx != src->ctx || dst->mesh != src->mesh || which < 0 || which > 1) return fm_einval(__func__);
   const fm_mesh &m = *dst->mesh;
   const size_t N = (size_t)m.Ntop;
   const int top = m.nlev - 1;
@@ -778,12 +771,12 @@ extern "C" int fm_system_couple(fm_system *dst, const fm_system *src, int which,
   hipStream_t st = dst->ctx->stream;
   FM_HIP(hipSetDevice(dst->ctx->device));
   if (which == 0) {
-    if (!coef || dst->nq2 != m.dim || dst->nq1 != 1 || src->nq2 < 1) return FM_EINVAL;
+    if (!coef || dst->nq2 != m.dim || dst->nq1 != 1 || src->nq2 < 1) return fm_einval(__func__);
     if (!dst->tnode) FM_CHECK(dalloc(&dst->tnode, N));
     FM_CHECK(fmk::node_gather(st, (int64_t)N, 1, src->d_dof_top, ls.rank, ls.x, dst->tnode));
     for (int d = 0; d < 3; d++) dst->buoy[d] = coef[d];
   } else {
-    if (src->nq2 < m.dim) return FM_EINVAL;
+    if (src->nq2 < m.dim) return fm_einval(__func__);
     if (!dst->vel) FM_CHECK(dalloc(&dst->vel, (size_t)m.dim * N));
     FM_CHECK(fmk::node_gather(st, (int64_t)N, m.dim, src->d_dof_top, ls.rank, ls.x, dst->vel));
   }
@@ -827,7 +820,7 @@ static int after_assembly(fm_system *s, int L) {
 
 extern "C" int fm_heat_genmatrhs(fm_system *s, int L, int mode, const fm_heat_params *p) {
   FM_CHECK(check_level(s, L));
-  if (!p || s->nq2 != 1 || s->nq1 != 0 || p->dt == 0. || p->alpha0 <= 0.) return FM_EINVAL;
+  if (!p || s->nq2 != 1 || s->nq1 != 0 || p->dt == 0. || p->alpha0 <= 0.) return fm_einval(__func__);
   FM_HIP(hipSetDevice(s->ctx->device));
   Level &l = s->lev[L];
   hipStream_t st = s->ctx->stream;
@@ -851,7 +844,7 @@ extern "C" int fm_heat_genmatrhs(fm_system *s, int L, int mode, const fm_heat_pa
 extern "C" int fm_ns_genmatrhs(fm_system *s, int L, int mode, const fm_ns_params *p) {
   FM_CHECK(check_level(s, L));
   const fm_mesh &m = *s->mesh;
-  if (!p || s->nq2 != m.dim || s->nq1 != 1 || p->dt == 0. || p->IRe <= 0.) return FM_EINVAL;
+  if (!p || s->nq2 != m.dim || s->nq1 != 1 || p->dt == 0. || p->IRe <= 0.) return fm_einval(__func__);
   FM_HIP(hipSetDevice(s->ctx->device));
   Level &l = s->lev[L];
   hipStream_t st = s->ctx->stream;
@@ -891,7 +884,7 @@ extern "C" int fm_spmv(fm_system *s, int L, const double *xh, double *yh) {
 // y = A^T x with the transposed matrix of the adjoint assembly, vectors in the reference DoF order
 extern "C" int fm_adjoint_spmv(fm_system *s, int L, const double *xh, double *yh) {
   FM_CHECK(check_level(s, L));
-  if (!xh || !yh) return FM_EINVAL;
+  if (!xh || !yh) return fm_einval(__func__);
   if (L >= (int)s->alev.size() || L >= (int)s->a_assembled.size() || !s->a_assembled[L]) {
     fm_set_error("fm_adjoint_spmv: level not assembled transposed");
     return FM_ESTATE;
@@ -918,7 +911,7 @@ extern "C" int fm_residual(fm_system *s, int L) {
 }
 extern "C" int fm_restrict(fm_system *s, int L) {
   FM_CHECK(check_level(s, L));
-  if (L < 1) return FM_EINVAL;
+  if (L < 1) return fm_einval(__func__);
   Level &lc = s->lev[L - 1], &lf = s->lev[L];
   FM_CHECK(fmk::spmv(s->ctx->stream, view<double>(lc.R), lf.res, lc.b));
   FM_HIP(hipStreamSynchronize(s->ctx->stream));
@@ -926,7 +919,7 @@ extern "C" int fm_restrict(fm_system *s, int L) {
 }
 extern "C" int fm_prolong(fm_system *s, int L) {
   FM_CHECK(check_level(s, L));
-  if (L < 1) return FM_EINVAL;
+  if (L < 1) return fm_einval(__func__);
   Level &lc = s->lev[L - 1], &lf = s->lev[L];
   FM_CHECK(fmk::spmv(s->ctx->stream, view<double>(lf.P), lc.x, lf.res));
   FM_HIP(hipStreamSynchronize(s->ctx->stream));
@@ -935,7 +928,7 @@ extern "C" int fm_prolong(fm_system *s, int L) {
 
 extern "C" int fm_bench_spmv(fm_system *s, int L, int prec, int reps, double *sec) {
   FM_CHECK(check_level(s, L));
-  if (reps < 1 || !sec || prec < 0 || prec > 3) return FM_EINVAL;
+  if (reps < 1 || !sec || prec < 0 || prec > 3) return fm_einval(__func__);
   Level &l = s->lev[L];
   hipStream_t st = s->ctx->stream;
   // prec 0 / 1: SELL-64 FP64 / FP32 (the solver's format); 2 / 3: CSR FP64 / FP32
@@ -1052,7 +1045,7 @@ static int after_assembly_T(fm_system *s, int L) {
 
 extern "C" int fm_heat_genmat_transpose(fm_system *s, int L, const fm_heat_params *p) {
   FM_CHECK(check_level(s, L));
-  if (!p || s->nq2 != 1 || s->nq1 != 0 || p->dt == 0. || p->alpha0 <= 0.) return FM_EINVAL;
+  if (!p || s->nq2 != 1 || s->nq1 != 0 || p->dt == 0. || p->alpha0 <= 0.) return fm_einval(__func__);
   FM_HIP(hipSetDevice(s->ctx->device));
   FM_CHECK(ensure_adjoint(s));
   Level &l = s->lev[L], &a = s->alev[L];
@@ -1074,7 +1067,7 @@ extern "C" int fm_heat_genmat_transpose(fm_system *s, int L, const fm_heat_param
 extern "C" int fm_ns_genmat_transpose(fm_system *s, int L, const fm_ns_params *p) {
   FM_CHECK(check_level(s, L));
   const fm_mesh &m = *s->mesh;
-  if (!p || s->nq2 != m.dim || s->nq1 != 1 || p->dt == 0. || p->IRe <= 0.) return FM_EINVAL;
+  if (!p || s->nq2 != m.dim || s->nq1 != 1 || p->dt == 0. || p->IRe <= 0.) return fm_einval(__func__);
   FM_HIP(hipSetDevice(s->ctx->device));
   FM_CHECK(ensure_adjoint(s));
   Level &l = s->lev[L], &a = s->alev[L];
@@ -1099,7 +1092,7 @@ extern "C" int fm_ns_genmat_transpose(fm_system *s, int L, const fm_ns_params *p
 extern "C" int fm_ns_genjac(fm_system *s, int L, int trans, const fm_ns_params *p) {
   FM_CHECK(check_level(s, L));
   const fm_mesh &m = *s->mesh;
-  if (!p || s->nq2 != m.dim || s->nq1 != 1 || p->dt == 0. || p->IRe <= 0. || (trans != 0 && trans != 1)) return FM_EINVAL;
+  if (!p || s->nq2 != m.dim || s->nq1 != 1 || p->dt == 0. || p->IRe <= 0. || (trans != 0 && trans != 1)) return fm_einval(__func__);
   FM_HIP(hipSetDevice(s->ctx->device));
   if (trans) FM_CHECK(ensure_adjoint(s));
   Level &l = s->lev[L];
@@ -1125,7 +1118,7 @@ extern "C" int fm_ns_genjac(fm_system *s, int L, int trans, const fm_ns_params *
 }
 
 extern "C" int fm_get_adjoint_csr(const fm_system *s, int L, double *val) {
-  if (!s || L < 0 || L >= (int)s->alev.size() || !val) return FM_EINVAL;
+  if (!s || L < 0 || L >= (int)s->alev.size() || !val) return fm_einval(__func__);
   DevCSR &c = const_cast<DevCSR &>(s->alev[L].A);
   FM_CHECK(fm_csr_from_sell(s->ctx->stream, c));
   FM_HIP(hipStreamSynchronize(s->ctx->stream));
@@ -1137,7 +1130,7 @@ extern "C" int fm_get_adjoint_csr(const fm_system *s, int L, double *val) {
 extern "C" int fm_heat_dissipation(fm_system *s, const fm_heat_params *p, const double *T_host, double *J,
                                    double *dJdT_host, double *dJdk_host) {
   FM_CHECK(check_level(s, 0));
-  if (!p || !T_host || s->nq2 != 1 || s->nq1 != 0) return FM_EINVAL;
+  if (!p || !T_host || s->nq2 != 1 || s->nq1 != 0) return fm_einval(__func__);
   int top = (int)s->lev.size() - 1;
   Level &l = s->lev[top];
   hipStream_t st = s->ctx->stream;
@@ -1174,7 +1167,7 @@ extern "C" int fm_heat_dissipation(fm_system *s, const fm_heat_params *p, const 
 extern "C" int fm_heat_grad_coef(fm_system *s, const fm_heat_params *p, const double *lam_host, const double *T_host,
                                  double *grad_host) {
   FM_CHECK(check_level(s, 0));
-  if (!p || !lam_host || !T_host || !grad_host || s->nq2 != 1 || s->nq1 != 0) return FM_EINVAL;
+  if (!p || !lam_host || !T_host || !grad_host || s->nq2 != 1 || s->nq1 != 0) return fm_einval(__func__);
   int top = (int)s->lev.size() - 1;
   Level &l = s->lev[top];
   hipStream_t st = s->ctx->stream;
@@ -1199,7 +1192,7 @@ extern "C" int fm_heat_grad_coef(fm_system *s, const fm_heat_params *p, const do
 // STREAM copy of `bytes` (rounded down to 16 B) between two device buffers;
 // *sec = average kernel time over reps (HIP events on the context stream).
 extern "C" int fm_bench_stream(fm_ctx *c, int64_t bytes, int reps, double *sec) {
-  if (!c || bytes < 16 || reps < 1 || !sec) return FM_EINVAL;
+  if (!c || bytes < 16 || reps < 1 || !sec) return fm_einval(__func__);
   FM_HIP(hipSetDevice(c->device));
   int64_t n16 = bytes / 16;
   void *a = nullptr, *b = nullptr;
@@ -1226,7 +1219,7 @@ extern "C" int fm_bench_stream(fm_ctx *c, int64_t bytes, int reps, double *sec) 
 extern "C" int fm_ns_dissipation(fm_system *s, const fm_ns_params *p, const double *U_host, double *J,
                                  double *dJdU_host, double *dJdnu_host) {
   FM_CHECK(check_level(s, 0));
-  if (!p || !U_host || s->nq2 != s->mesh->dim || s->nq1 != 1) return FM_EINVAL;
+  if (!p || !U_host || s->nq2 != s->mesh->dim || s->nq1 != 1) return fm_einval(__func__);
   int top = (int)s->lev.size() - 1;
   Level &l = s->lev[top];
   hipStream_t st = s->ctx->stream;
@@ -1263,7 +1256,7 @@ extern "C" int fm_ns_dissipation(fm_system *s, const fm_ns_params *p, const doub
 extern "C" int fm_ns_grad_coef(fm_system *s, const fm_ns_params *p, const double *lam_host, const double *U_host,
                                double *grad_host) {
   FM_CHECK(check_level(s, 0));
-  if (!p || !lam_host || !U_host || !grad_host || s->nq2 != s->mesh->dim || s->nq1 != 1) return FM_EINVAL;
+  if (!p || !lam_host || !U_host || !grad_host || s->nq2 != s->mesh->dim || s->nq1 != 1) return fm_einval(__func__);
   int top = (int)s->lev.size() - 1;
   Level &l = s->lev[top];
   hipStream_t st = s->ctx->stream;
@@ -1293,7 +1286,7 @@ extern "C" int fm_ns_grad_coef(fm_system *s, const fm_ns_params *p, const double
 extern "C" int fm_ns_pressure_drop(fm_system *s, const double *U, double *J, double *dJdU) {
   FM_CHECK(check_level(s, 0));
   const fm_mesh &m = *s->mesh;
-  if (!U || s->nq2 != m.dim || s->nq1 != 1) return FM_EINVAL;
+  if (!U || s->nq2 != m.dim || s->nq1 != 1) return fm_einval(__func__);
   const int top = m.nlev - 1, N = m.Ntop, dim = m.dim, nvert = dim == 2 ? 2 : 4;
   const std::vector<int> &dof = s->node_dof[top];
   const int64_t nt = s->Dim[top];
@@ -1347,9 +1340,9 @@ extern "C" int fm_ns_pressure_drop(fm_system *s, const double *U, double *J, dou
 extern "C" int fm_heat_coupling(fm_system *s, const fm_heat_params *p, int mode, double scale, const double *T_host,
                                 const double *lam_host, const double *m_host, double *out_host) {
   FM_CHECK(check_level(s, 0));
-  if (!p || s->nq2 != 1 || s->nq1 != 0 || mode < 0 || mode > 2 || !out_host || !(p->alpha0 > 0.)) return FM_EINVAL;
+  if (!p || s->nq2 != 1 || s->nq1 != 0 || mode < 0 || mode > 2 || !out_host || !(p->alpha0 > 0.)) return fm_einval(__func__);
   if (mode == 1 && p->upwind != 0. && !s->vel) { fm_set_error("fm_heat_coupling: upwind term needs the velocity"); return FM_ESTATE; }
-  if ((mode != 2 && !T_host) || (mode == 1 && !lam_host) || (mode == 2 && !m_host)) return FM_EINVAL;
+  if ((mode != 2 && !T_host) || (mode == 1 && !lam_host) || (mode == 2 && !m_host)) return fm_einval(__func__);
   const int top = (int)s->lev.size() - 1;
   Level &l = s->lev[top];
   hipStream_t st = s->ctx->stream;
@@ -1379,7 +1372,7 @@ extern "C" int fm_heat_coupling(fm_system *s, const fm_heat_params *p, int mode,
 extern "C" int fm_heat_nusselt(fm_system *s, const fm_heat_params *p, int axis, const double *T_host, double *J,
                                double *dJdT_host, double *dJdk_host) {
   FM_CHECK(check_level(s, 0));
-  if (!p || !T_host || s->nq2 != 1 || s->nq1 != 0 || axis < 0 || axis >= s->mesh->dim || !(p->alpha0 > 0.)) return FM_EINVAL;
+  if (!p || !T_host || s->nq2 != 1 || s->nq1 != 0 || axis < 0 || axis >= s->mesh->dim || !(p->alpha0 > 0.)) return fm_einval(__func__);
   int top = (int)s->lev.size() - 1;
   Level &l = s->lev[top];
   hipStream_t st = s->ctx->stream;

@@ -8,6 +8,22 @@
 #include "fm_mesh.h"
 
 void fm_set_error(const std::string &msg);
+unsigned fm_error_count();  // fm_set_error calls so far on this thread
+// invalid-argument return of an entry point: names the function, so fm_last_error never reports an older failure
+inline int fm_einval(const char *where) {
+  fm_set_error(std::string(where) + ": invalid argument");
+  return FM_EINVAL;
+}
+// propagates a failing internal call; names the call when it set no message of its own
+#define FM_CHECK(x)                                                  \
+  do {                                                               \
+    const unsigned e0_ = fm_error_count();                           \
+    const int rc_ = (x);                                             \
+    if (rc_ != 0) {                                                  \
+      if (fm_error_count() == e0_) fm_set_error(#x);                 \
+      return rc_ < 0 ? rc_ : FM_EHIP;                                \
+    }                                                                \
+  } while (0)
 
 #define FM_HIP(call)                                                           \
   do {                                                                         \

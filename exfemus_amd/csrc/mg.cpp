@@ -17,12 +17,6 @@
 #include "fm_internal.h"
 #include "fm_kernels.h"
 
-#define FM_CHECK(x)                                 \
-  do {                                              \
-    int rc_ = (x);                                  \
-    if (rc_ != 0) return rc_ < 0 ? rc_ : FM_EHIP;   \
-  } while (0)
-
 int fm_vanka_setup(fm_system *s, int L, std::vector<struct LOps> *ops);
 int fm_vanka_sweep64(fm_system *s, int L, const struct LOps &o, double *x, const double *b, int its, double omega,
                      int colored, bool xz);
@@ -209,7 +203,7 @@ static int smooth(fm_system *s, const fm_mg_opts *opt, int L, const LOps &o, int
       return fm_vanka_sweep64(s, L, o, (double *)x, (const double *)b, its, opt->omega, opt->vanka_colored, xz);
     return fm_vanka_sweep32(s, L, o, (float *)x, (const float *)b, its, opt->omega, opt->vanka_colored, xz);
   }
-  return FM_EINVAL;
+  return fm_einval(__func__);
 }
 
 // ---- V-cycle (MGSolBase::MGStep) -----------------------------------------
@@ -313,14 +307,14 @@ extern "C" int fm_mg_setup(fm_system *s, const fm_mg_opts *o) {
 }
 
 extern "C" int fm_system_set_vanka_store(fm_system *s, int mask) {
-  if (!s || mask < 0 || mask > 3) return FM_EINVAL;
+  if (!s || mask < 0 || mask > 3) return fm_einval(__func__);
   const int old = s->vanka_store;
   if (mask > 0) s->vanka_store = mask;
   return old;
 }
 
 extern "C" int fm_mg_release(fm_system *s, int adjoint) {
-  if (!s) return FM_EINVAL;
+  if (!s) return fm_einval(__func__);
   FM_HIP(hipSetDevice(s->ctx->device));
   FM_HIP(hipStreamSynchronize(s->ctx->stream));
   fm_drop_graphs(s);  // captured cycles hold the addresses released here
@@ -626,7 +620,7 @@ static int vanka_store_error(const fm_mg_opts *o) {
 }
 
 extern "C" int fm_mg_solve(fm_system *s, const fm_mg_opts *o, fm_mg_report *rep) {
-  if (!s || !o || !rep) return FM_EINVAL;
+  if (!s || !o || !rep) return fm_einval(__func__);
   if (!s->initialized) return FM_ESTATE;
   FM_HIP(hipSetDevice(s->ctx->device));
   std::memset(rep, 0, sizeof(*rep));
@@ -672,7 +666,7 @@ int fm_vanka_sweep32(fm_system *s, int L, const LOps &o, float *x, const float *
 // transposed assembly, restriction Prl^T, prolongation Rst^T, smoothers and
 // coarse inverse of A^T), same outer solvers as the forward problem.
 extern "C" int fm_adjoint_solve(fm_system *s, const fm_mg_opts *o, const double *g, double *lam, fm_mg_report *rep) {
-  if (!s || !o || !g || !lam || !rep) return FM_EINVAL;
+  if (!s || !o || !g || !lam || !rep) return fm_einval(__func__);
   if (!s->initialized || s->alev.empty()) { fm_set_error("adjoint: assemble A^T on every level first"); return FM_ESTATE; }
   for (char a : s->a_assembled) if (!a) { fm_set_error("adjoint: A^T missing on a level"); return FM_ESTATE; }
   FM_HIP(hipSetDevice(s->ctx->device));

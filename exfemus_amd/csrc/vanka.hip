@@ -357,10 +357,10 @@ int fm_vanka_apply32(fm_system *s, int L, const CsrView<float> &A, const float *
 // Diagnostic: one element's patch inverse as stored for the smoother (FP64, transposition undone) and the
 // patch DoFs in reference numbering, local order [u, v, (w), p].
 extern "C" int fm_vanka_patch(const fm_system *s, int L, int e, int *dofs, double *kinv) {
-  if (!s || !dofs || !kinv || L < 0 || L >= (int)s->lev.size()) return FM_EINVAL;
+  if (!s || !dofs || !kinv || L < 0 || L >= (int)s->lev.size()) return fm_einval(__func__);
   const Level &l = s->lev[L];
   if (!s->mg_ready || !l.vinv) { fm_set_error("no Vanka inverses: run fm_mg_setup with the Vanka smoother"); return FM_ESTATE; }
-  if (e < 0 || e >= l.nel) return FM_EINVAL;
+  if (e < 0 || e >= l.nel) return fm_einval(__func__);
   const int nl = s->nloc;
   std::vector<double> t((size_t)nl * nl);
   hipStream_t st = s->ctx->stream;

@@ -47,6 +47,29 @@ def test_error_path_without_device():
     assert rc < 0
 
 
+def test_error_messages_name_the_failing_call():
+    """Every failing call leaves its own message for fm_last_error, never an older one (host-only mesh entry
+    points with bad arguments: no device needed)."""
+    if not os.path.exists(LIB):
+        pytest.skip("libfemus_hip.so not built")
+    lib = C.CDLL(LIB)
+    lib.fm_last_error.restype = C.c_char_p
+    lib.fm_mesh_create_box.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                       C.c_int, C.POINTER(C.c_void_p)]
+    lib.fm_mesh_info.argtypes = [C.c_void_p, C.c_void_p]
+    lib.fm_mesh_level.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    nx0 = (C.c_int * 3)(0, 1, 1)
+    lo = (C.c_double * 3)(0, 0, 0)
+    hi = (C.c_double * 3)(1, 1, 1)
+    h = C.c_void_p()
+    assert lib.fm_mesh_create_box(2, nx0, lo, hi, 3, C.byref(h)) == -1
+    assert b"fm_mesh_create_box" in lib.fm_last_error()
+    assert lib.fm_mesh_info(None, None) == -1
+    assert lib.fm_last_error() == b"fm_mesh_info: invalid argument"
+    assert lib.fm_mesh_level(None, 0, None, None, None) == -1
+    assert lib.fm_last_error() == b"fm_mesh_level: invalid argument"
+
+
 def test_cpp_host_layer_compiles():
     """include/femus_hip.hpp (MGSolDA / MGSolT / MGSolNS over the C ABI) compiles with the host compiler,
     together with the application-style driver that the GPU tests run."""
