@@ -3,7 +3,9 @@
  *
  * One context = one HIP device + one HIP stream; calls on a context are not
  * thread-safe.  Status codes: 0 = OK, <0 = error (see FM_E*); no exceptions
- * cross this boundary.  Host arrays are copied in; device memory is owned by
+ * cross this boundary.  After a failing call, fm_last_error() returns that
+ * call's message (kept per thread until the next failure on the thread).
+ * Host arrays are copied in; device memory is owned by
  * the library.  Vectors exchanged with the caller are in the reference DoF
  * order (MGSolDA::_node_dof, src/MGSolverDA.C:880-948).
  *
