@@ -155,6 +155,15 @@ def lib():
     return L
 
 
+def _vec(a, n, what):
+    """a as a contiguous float64 array of exactly n entries: the library reads n values through the pointer, so a
+    shorter array would be read past its end."""
+    v = np.ascontiguousarray(a, np.float64).reshape(-1)
+    if v.size != n:
+        raise ValueError(f"{what}: expected {n} values, got {v.size}")
+    return v
+
+
 def _ck(rc, ok=(0,)):
     if rc not in ok:
         raise FemusError(rc, lib().fm_last_error().decode(errors="replace"))
@@ -310,7 +319,7 @@ class MGSolDA:
         return rp, ci, v
 
     def set_xold(self, x):
-        _ck(lib().fm_system_set_xold(self.h, np.ascontiguousarray(x, np.float64)))
+        _ck(lib().fm_system_set_xold(self.h, _vec(x, self._Dim[-1], "x_old")))
 
     def get_x(self, L=None):
         L = self._NoLevels - 1 if L is None else L
@@ -318,7 +327,7 @@ class MGSolDA:
 
     def set_x(self, x, L=None):
         L = self._NoLevels - 1 if L is None else L
-        _ck(lib().fm_system_set_x(self.h, L, np.ascontiguousarray(x, np.float64)))
+        _ck(lib().fm_system_set_x(self.h, L, _vec(x, self._Dim[L], "x")))
 
     def get_rhs(self, L=None):
         L = self._NoLevels - 1 if L is None else L
@@ -326,7 +335,7 @@ class MGSolDA:
 
     def set_rhs(self, b, L=None):
         L = self._NoLevels - 1 if L is None else L
-        _ck(lib().fm_system_set_rhs(self.h, L, np.ascontiguousarray(b, np.float64)))
+        _ck(lib().fm_system_set_rhs(self.h, L, _vec(b, self._Dim[L], "rhs")))
 
     def get_res(self, L):
         r = np.zeros(self._Dim[L]); _ck(lib().fm_get_res(self.h, L, r)); return r
@@ -345,29 +354,31 @@ class MGSolDA:
         if g is None:
             _ck(lib().fm_system_set_bcdata(self.h, None))
         else:
-            c = np.ascontiguousarray(g, np.float64)
+            c = _vec(g, self._Dim[-1], "bcdata")
             _ck(lib().fm_system_set_bcdata(self.h, c.ctypes.data_as(C.c_void_p)))
 
     def bc_rhs(self, g):
         """Boundary-face penalty RHS of the top level for Dirichlet data g (linear in g)."""
         out = np.zeros(self._Dim[-1])
         fn = lib().fm_heat_bc_rhs if self.nq1 == 0 else lib().fm_ns_bc_rhs
-        _ck(fn(self.h, C.byref(self.params), np.ascontiguousarray(g, np.float64), out))
+        _ck(fn(self.h, C.byref(self.params), _vec(g, self._Dim[-1], "g"), out))
         return out
 
     def set_elem_coef(self, L, coef):
-        c = None if coef is None else np.ascontiguousarray(coef, np.float64)
+        c = None if coef is None else _vec(coef, self.mesh.level(L)[0], "element coefficients")
         _ck(lib().fm_system_set_elem_coef(self.h, L, None if c is None else c.ctypes.data_as(C.c_void_p)))
         self._keep_coef = c
 
     # -- LA operators (NumericVectorM::resid / matrix_mult) -----------------------
     def spmv(self, L, x):
-        y = np.zeros(self._Dim[L]); _ck(lib().fm_spmv(self.h, L, np.ascontiguousarray(x, np.float64), y)); return y
+        y = np.zeros(self._Dim[L]); _ck(lib().fm_spmv(self.h, L, _vec(x, self._Dim[L], "x"), y)); return y
 
     def couple(self, src, which, coef=None):
         """This system's coupling field from src's top-level solution, on the device: which 0 = src's temperature
         as the buoyancy field (coef per direction), 1 = src's velocity as the advection field."""
-        c = np.zeros(3) if coef is None else np.asarray(coef, np.float64)
+        c = np.zeros(3)
+        if coef is not None:
+            c[:len(coef)] = coef
         _ck(lib().fm_system_couple(self.h, src.h, int(which), c.ctypes.data_as(C.c_void_p)))
 
     def advance(self):
@@ -460,14 +471,14 @@ class MGSolDA:
         """Transposed multigrid solve A^T lambda = g (returns lambda, report)."""
         lam = np.zeros(self._Dim[-1])
         rep = MGReport()
-        rc = lib().fm_adjoint_solve(self.h, C.byref(opts), np.ascontiguousarray(g, np.float64), lam, C.byref(rep))
+        rc = lib().fm_adjoint_solve(self.h, C.byref(opts), _vec(g, self._Dim[-1], "g"), lam, C.byref(rep))
         _ck(rc, ok=(0, FM_ENOCONV))
         return lam, rep
 
     def adjoint_spmv(self, L, x):
         """A^T x with the transposed matrix of level L (after GenMatTranspose / GenJac(L, 1))."""
         y = np.zeros(self._Dim[L])
-        _ck(lib().fm_adjoint_spmv(self.h, L, np.ascontiguousarray(x, np.float64), y))
+        _ck(lib().fm_adjoint_spmv(self.h, L, _vec(x, self._Dim[L], "x"), y))
         return y
 
     def bench_spmv(self, L, prec, reps):
@@ -495,8 +506,10 @@ class MGSolT(MGSolDA):
         """Node-space coupling products of the Boussinesq system (fm_heat_coupling)."""
         N = self.mesh.N
         out = np.zeros((self.mesh.dim if mode == 1 else 1) * N)
-        arr = lambda a: None if a is None else np.ascontiguousarray(a, np.float64)
-        T, lam, m = arr(T), arr(lam), arr(m)
+        n = self._Dim[-1]
+        T = None if T is None else _vec(T, n, "T")
+        lam = None if lam is None else _vec(lam, n, "lam")
+        m = None if m is None else _vec(m, N, "m")
         ptr = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
         _ck(lib().fm_heat_coupling(self.h, C.byref(self.params), mode, scale, ptr(T), ptr(lam), ptr(m),
                                    out.ctypes.data_as(C.c_void_p)))
@@ -508,7 +521,7 @@ class MGSolT(MGSolDA):
         J = C.c_double()
         g = np.zeros(self._Dim[-1])
         dk = np.zeros(self.mesh.level(self._NoLevels - 1)[0])
-        _ck(lib().fm_heat_nusselt(self.h, C.byref(self.params), axis, np.ascontiguousarray(T, np.float64),
+        _ck(lib().fm_heat_nusselt(self.h, C.byref(self.params), axis, _vec(T, self._Dim[-1], "T"),
                                   C.byref(J), g.ctypes.data_as(C.c_void_p), dk.ctypes.data_as(C.c_void_p)))
         return J.value, g, dk
 
@@ -517,19 +530,19 @@ class MGSolT(MGSolDA):
         J = C.c_double()
         g = np.zeros(self._Dim[-1])
         dk = np.zeros(self.mesh.level(self._NoLevels - 1)[0])
-        _ck(lib().fm_heat_dissipation(self.h, C.byref(self.params), np.ascontiguousarray(T, np.float64), C.byref(J),
+        _ck(lib().fm_heat_dissipation(self.h, C.byref(self.params), _vec(T, self._Dim[-1], "T"), C.byref(J),
                                       g.ctypes.data_as(C.c_void_p), dk.ctypes.data_as(C.c_void_p)))
         return J.value, g, dk
 
     def grad_coef(self, lam, T):
         """-lambda^T (dA/dk_e) T for every top element."""
         out = np.zeros(self.mesh.level(self._NoLevels - 1)[0])
-        _ck(lib().fm_heat_grad_coef(self.h, C.byref(self.params), np.ascontiguousarray(lam, np.float64),
-                                    np.ascontiguousarray(T, np.float64), out))
+        _ck(lib().fm_heat_grad_coef(self.h, C.byref(self.params), _vec(lam, self._Dim[-1], "lam"),
+                                    _vec(T, self._Dim[-1], "T"), out))
         return out
 
     def set_velocity(self, vel_top):
-        v = None if vel_top is None else np.ascontiguousarray(vel_top, np.float64)
+        v = None if vel_top is None else _vec(vel_top, self.mesh.dim * self.mesh.N, "velocity")
         self._keep_vel = v
         _ck(lib().fm_system_set_velocity(self.h, None if v is None else v.ctypes.data_as(C.c_void_p)))
 
@@ -567,7 +580,7 @@ class MGSolNS(MGSolDA):
         J = C.c_double()
         g = np.zeros(self._Dim[-1])
         dn = np.zeros(self.mesh.level(self._NoLevels - 1)[0])
-        _ck(lib().fm_ns_dissipation(self.h, C.byref(self.params), np.ascontiguousarray(U, np.float64), C.byref(J),
+        _ck(lib().fm_ns_dissipation(self.h, C.byref(self.params), _vec(U, self._Dim[-1], "U"), C.byref(J),
                                     g.ctypes.data_as(C.c_void_p), dn.ctypes.data_as(C.c_void_p)))
         return J.value, g, dn
 
@@ -576,7 +589,7 @@ class MGSolNS(MGSolDA):
         if T_node is None:
             _ck(lib().fm_ns_set_buoyancy(self.h, None, None))
             return
-        t = np.ascontiguousarray(T_node, np.float64)
+        t = _vec(T_node, self.mesh.N, "nodal temperature")
         c = np.zeros(3); c[:len(coef)] = coef
         _ck(lib().fm_ns_set_buoyancy(self.h, t.ctypes.data_as(C.c_void_p), c.ctypes.data_as(C.c_void_p)))
         self._keep_T = t
@@ -590,15 +603,15 @@ class MGSolNS(MGSolDA):
         """J = mean p(inlet x = lo) - mean p(outlet x = hi) and dJ/dU."""
         J = C.c_double()
         g = np.zeros(self._Dim[-1])
-        _ck(lib().fm_ns_pressure_drop(self.h, np.ascontiguousarray(U, np.float64), C.byref(J),
+        _ck(lib().fm_ns_pressure_drop(self.h, _vec(U, self._Dim[-1], "U"), C.byref(J),
                                       g.ctypes.data_as(C.c_void_p)))
         return J.value, g
 
     def grad_coef(self, lam, U):
         """-lambda^T (dA/dnu_e) U for every top element."""
         out = np.zeros(self.mesh.level(self._NoLevels - 1)[0])
-        _ck(lib().fm_ns_grad_coef(self.h, C.byref(self.params), np.ascontiguousarray(lam, np.float64),
-                                  np.ascontiguousarray(U, np.float64), out))
+        _ck(lib().fm_ns_grad_coef(self.h, C.byref(self.params), _vec(lam, self._Dim[-1], "lam"),
+                                  _vec(U, self._Dim[-1], "U"), out))
         return out
 
     def MGTimeStep(self, time, opts=None):

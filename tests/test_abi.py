@@ -70,6 +70,19 @@ def test_error_messages_name_the_failing_call():
     assert lib.fm_last_error() == b"fm_mesh_level: invalid argument"
 
 
+def test_wrapper_checks_array_lengths():
+    """The Python mirror hands the library only arrays of the length it reads (no device needed)."""
+    import numpy as np
+    from exfemus_amd.femus import _vec
+    v = _vec(np.arange(6, dtype=np.int32).reshape(2, 3), 6, "x")
+    assert v.dtype == np.float64 and v.flags["C_CONTIGUOUS"] and v.shape == (6,)
+    v = _vec(np.arange(12.0).reshape(3, 4).T, 12, "x")     # non-contiguous input is copied
+    assert v.flags["C_CONTIGUOUS"] and np.array_equal(v, np.arange(12.0).reshape(3, 4).T.reshape(-1))
+    for bad in (np.zeros(5), np.zeros(7), []):
+        with pytest.raises(ValueError):
+            _vec(bad, 6, "x")
+
+
 def test_cpp_host_layer_compiles():
     """include/femus_hip.hpp (MGSolDA / MGSolT / MGSolNS over the C ABI) compiles with the host compiler,
     together with the application-style driver that the GPU tests run."""

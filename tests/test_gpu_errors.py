@@ -107,3 +107,27 @@ def test_memory_and_coupling_controls_reject_bad_arguments(femus, ctx):
     assert info[5] == a.csr(0, 1)[1].size and info[1] >= info[5] and info[0] * 64 >= a._Dim[1]
     a.close()
     b.close()
+
+
+def test_wrapper_rejects_arrays_of_the_wrong_length(femus, ctx):
+    """The Python mirror checks every host array it hands to the library against the length the library reads
+    (level DoFs, elements, nodes) and raises before any copy; the system stays usable."""
+    m = femus.Mesh(2, [2, 2], [0, 0], [1, 1], 2)
+    s = femus.MGSolT(ctx, m, alpha0=1.0, dt=0.1, time_coeff=0.0, source=1.0)
+    s.GenBc(configs.heat_dirichlet_all)
+    s.init()
+    n, nel, N = s._Dim[1], m.level(1)[0], m.N
+    for call in (lambda: s.set_x(np.zeros(n - 1)), lambda: s.set_rhs(np.zeros(n + 1), 0),
+                 lambda: s.set_xold(np.zeros(n - 1)), lambda: s.spmv(1, np.zeros(s._Dim[0])),
+                 lambda: s.set_elem_coef(1, np.ones(nel - 1)), lambda: s.set_velocity(np.zeros(N)),
+                 lambda: s.nusselt(np.zeros(n - 1)), lambda: s.coupling(2, 1.0, m=np.zeros(N - 1))):
+        with pytest.raises(ValueError):
+            call()
+    s.set_x(np.ones((1, n)))                                    # any shape with the right number of entries
+    assert np.array_equal(s.get_x(), np.ones(n))
+    s.set_xold(np.zeros(n))
+    for lev in range(2):
+        s.GenMatRhs(0.0, lev, 1 if lev == 1 else 0)
+    rep = s.MGSolve(opts=s.opts(precision=femus.FM_PREC_FP64_FGMRES, tol=1e-10, max_iter=50))
+    assert rep.converged == 1
+    s.close()
