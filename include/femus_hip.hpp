@@ -41,6 +41,14 @@ inline void set_error_handler(ErrorHandler h) { error_handler() = std::move(h); 
 inline void check(int rc, const char *where) {
   if (rc != FM_OK) error_handler()(rc, where, fm_last_error() ? fm_last_error() : "");
 }
+// true when a host vector holds the number of values the library reads through its pointer; otherwise reports
+// FM_EINVAL through the error handler and returns false (the call is then skipped)
+inline bool sized(const std::vector<double> &v, size_t n, const char *where) {
+  if (v.size() == n) return true;
+  error_handler()(FM_EINVAL, where,
+                  "expected " + std::to_string(n) + " values, got " + std::to_string(v.size()));
+  return false;
+}
 
 // One HIP device + stream (the reference's one MPI rank).
 class Context {
@@ -163,6 +171,7 @@ class MGSolDA {
 
   // x_old of the top level (advection field and Dirichlet data of the next assembly)
   void set_xold(const std::vector<double> &v) {
+    if (!sized(v, (size_t)_Dim.back(), "set_xold")) return;
     x_old = v;
     check(fm_system_set_xold(h_, x_old.data()), "fm_system_set_xold");
   }
@@ -174,6 +183,7 @@ class MGSolDA {
   }
   void set_x(const std::vector<double> &v, int L = -1) {
     if (L < 0) L = _NoLevels - 1;
+    if (L >= _NoLevels || !sized(v, (size_t)_Dim[L], "set_x")) return;
     check(fm_system_set_x(h_, L, v.data()), "fm_system_set_x");
   }
   std::vector<double> get_rhs(int L = -1) const {
@@ -183,12 +193,14 @@ class MGSolDA {
     return v;
   }
   void set_elem_coef(int L, const std::vector<double> &c) {
+    if (!sized(c, (size_t)n_elem(L), "set_elem_coef")) return;
     check(fm_system_set_elem_coef(h_, L, c.data()), "fm_system_set_elem_coef");
   }
   // nodal values of variable ivar on the top-level nodes from a top-level DoF vector (0 where the variable is absent)
   std::vector<double> node_values(const std::vector<double> &x, int ivar = 0) const {
     const int N = mesh_.n_nodes();
     std::vector<double> v(N, 0.0);
+    if (!sized(x, (size_t)_Dim.back(), "node_values")) return v;
     const std::vector<int> &nd = _node_dof[_NoLevels - 1];
     for (int k = 0; k < N; k++) {
       const int d = nd[k + (size_t)ivar * N];
@@ -198,11 +210,13 @@ class MGSolDA {
   }
   // Dirichlet data of the penalty RHS separate from x_old (top level; empty: the reference's x_old)
   void set_bcdata(const std::vector<double> &g) {
+    if (!g.empty() && !sized(g, (size_t)_Dim.back(), "set_bcdata")) return;
     check(fm_system_set_bcdata(h_, g.empty() ? nullptr : g.data()), "fm_system_set_bcdata");
   }
   // transposed multigrid solve A^T lambda = g on the transposed hierarchy (GenMatTranspose on every level first)
   std::vector<double> AdjointSolve(const std::vector<double> &g) {
     std::vector<double> lam(_Dim.back(), 0.0);
+    if (!sized(g, lam.size(), "AdjointSolve")) return lam;
     const int rc = fm_adjoint_solve(h_, &opts, g.data(), lam.data(), &report);
     if (rc != FM_OK && rc != FM_ENOCONV) check(rc, "fm_adjoint_solve");
     return lam;
@@ -243,6 +257,12 @@ class MGSolDA {
   Mesh &mesh_;
   int nq2_, nq1_;
   fm_system *h_ = nullptr;
+  int n_elem(int L) const {
+    int ne = 0, n2 = 0, n1 = 0;
+    check(fm_mesh_level(mesh_.get(), L, &ne, &n2, &n1), "fm_mesh_level");
+    return ne;
+  }
+  int n_elem_top() const { return n_elem(_NoLevels - 1); }
 
  private:
   static void bc_trampoline(const double *xp, int *neu, int *flag, void *self) {
@@ -262,12 +282,14 @@ class MGSolT : public MGSolDA {
   void GenMatTranspose(int Level) { check(fm_heat_genmat_transpose(h_, Level, &par), "fm_heat_genmat_transpose"); }
   // advecting velocity [dim * Ntop] (component-major nodal values); empty removes it
   void set_velocity(const std::vector<double> &vel) {
+    if (!vel.empty() && !sized(vel, (size_t)mesh_.dim() * mesh_.n_nodes(), "set_velocity")) return;
     check(fm_system_set_velocity(h_, vel.empty() ? nullptr : vel.data()), "fm_system_set_velocity");
   }
   // Nusselt number along axis of the top-level temperature T, with dNu/dT and the partials dNu/dk_e
   double nusselt(const std::vector<double> &T, int axis, std::vector<double> *dJdT = nullptr,
                  std::vector<double> *dJdk = nullptr) {
     double J = 0.0;
+    if (!sized(T, (size_t)_Dim.back(), "nusselt")) return J;
     if (dJdT) dJdT->assign(_Dim.back(), 0.0);
     if (dJdk) dJdk->assign(n_elem_top(), 0.0);
     check(fm_heat_nusselt(h_, &par, axis, T.data(), &J, dJdT ? dJdT->data() : nullptr, dJdk ? dJdk->data() : nullptr),
@@ -277,17 +299,11 @@ class MGSolT : public MGSolDA {
   // -lambda^T (dA/dk_e) T for every top-level element
   std::vector<double> grad_coef(const std::vector<double> &lam, const std::vector<double> &T) {
     std::vector<double> g(n_elem_top(), 0.0);
+    if (!sized(lam, (size_t)_Dim.back(), "grad_coef") || !sized(T, (size_t)_Dim.back(), "grad_coef")) return g;
     check(fm_heat_grad_coef(h_, &par, lam.data(), T.data(), g.data()), "fm_heat_grad_coef");
     return g;
   }
   fm_heat_params par;
-
- private:
-  int n_elem_top() const {
-    int ne = 0, n2 = 0, n1 = 0;
-    check(fm_mesh_level(mesh_.get(), _NoLevels - 1, &ne, &n2, &n1), "fm_mesh_level");
-    return ne;
-  }
 };
 
 // MGSolNS: coupled Q2-Q1 Navier-Stokes (template_appl/nse/SRC/MGSolverNS.C, NS_EQUATIONS == 1)
@@ -308,6 +324,7 @@ class MGSolNS : public MGSolDA {
   void GenJac(int Level, int trans) { check(fm_ns_genjac(h_, Level, trans, &par), "fm_ns_genjac"); }
   // Boussinesq body force coef * T from a nodal temperature field [Ntop]; empty removes it
   void set_buoyancy(const std::vector<double> &T_node, const double coef[3]) {
+    if (!T_node.empty() && !sized(T_node, (size_t)mesh_.n_nodes(), "set_buoyancy")) return;
     check(fm_ns_set_buoyancy(h_, T_node.empty() ? nullptr : T_node.data(), coef), "fm_ns_set_buoyancy");
   }
   // nodal velocity [dim * Ntop] (component-major) of a top-level DoF vector

@@ -83,6 +83,35 @@ def test_wrapper_checks_array_lengths():
             _vec(bad, 6, "x")
 
 
+def test_cpp_host_layer_checks_vector_sizes(tmp_path):
+    """femus_hip::sized (used by every setter of the C++ host layer before it passes a pointer) reports a
+    vector of the wrong length through the error handler as FM_EINVAL and lets the caller skip the call."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no g++")
+    src = tmp_path / "sized.cpp"
+    src.write_text(r'''
+#include "femus_hip.hpp"
+#include <cstdio>
+int main() {
+  int code = 0; std::string where, msg;
+  femus_hip::set_error_handler([&](int c, const std::string &w, const std::string &m) { code = c; where = w; msg = m; });
+  const bool ok = femus_hip::sized(std::vector<double>(4), 4, "a");
+  const bool bad = femus_hip::sized(std::vector<double>(3), 4, "set_x");
+  std::printf("%d %d %d %s|%s\n", ok, bad, code, where.c_str(), msg.c_str());
+  return 0;
+}
+''')
+    exe = tmp_path / "sized"
+    r = subprocess.run([gxx, "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                        str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=30).stdout.strip()
+    assert out == "1 0 -1 set_x|expected 4 values, got 3", out
+
+
 def test_cpp_host_layer_compiles():
     """include/femus_hip.hpp (MGSolDA / MGSolT / MGSolNS over the C ABI) compiles with the host compiler,
     together with the application-style driver that the GPU tests run."""
