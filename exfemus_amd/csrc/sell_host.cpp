@@ -6,9 +6,10 @@
 #include <utility>
 
 int sell_build_host(int64_t n, const std::vector<int64_t> &rp, const std::vector<int> &ci, const int *ro,
-                    const int *cr, const std::vector<int64_t> *groups, int W, bool use8, SellHost &out,
+                    const int *cr, const std::vector<int64_t> *groups, int W, int minbits, SellHost &out,
                     std::string &err) {
   const int C = 64;
+  if (minbits != 8 && minbits != 16 && minbits != 32) { err = "SELL: offset class must be 8, 16 or 32 bits"; return -1; }
   if ((int64_t)rp.size() != n + 1) { err = "SELL build without host pattern"; return -5; }
   auto crow = [&](int64_t i) -> int64_t { return ro ? ro[i] : i; };
   auto rlen = [&](int64_t i) -> int { int64_t r = crow(i); return (int)(rp[r + 1] - rp[r]); };
@@ -84,12 +85,17 @@ int sell_build_host(int64_t n, const std::vector<int64_t> &rp, const std::vector
       }
     }
   }
-  // 8-bit offsets for the slices that allow them (use8 = false keeps every slice at 16 bits)
+  // each slice in the narrowest class it allows, but never narrower than minbits (16: no 8-bit slices,
+  // 32: absolute columns everywhere)
+  for (int64_t s = 0; s < nsl; s++) {
+    if (minbits == 32) wide[s] = 1;
+    if (minbits >= 16) narrow[s] = 0;
+  }
   std::vector<int64_t> o32(nsl, -1), o8(nsl, -1), o16(nsl, -1);
   int64_t n32 = 0, n8 = 0, n16 = 0;
   for (int64_t s = 0; s < nsl; s++) {
     if (wide[s]) { o32[s] = n32; n32 += (int64_t)wd[s] * C; }
-    else if (use8 && narrow[s]) { o8[s] = n8; n8 += (int64_t)((wd[s] + 3) / 4) * 4 * C; }
+    else if (narrow[s]) { o8[s] = n8; n8 += (int64_t)((wd[s] + 3) / 4) * 4 * C; }
     else { o16[s] = n16; n16 += (int64_t)((wd[s] + 1) / 2) * 2 * C; }
   }
   std::vector<unsigned short> c16(std::max<int64_t>(n16, 1), 0);

@@ -9,8 +9,9 @@
 // identity), CSR column j becomes internal column cr[j] (nullptr: identity).  Rows are length-sorted inside
 // windows of W rows that never cross a row group (groups: boundaries in internal rows, nullptr: one group);
 // each group starts a new slice.  The entries of a row are sorted by internal column; column offsets are
-// stored per slice in the narrowest class (8 bits when every position of the slice spans <= 255 columns and
-// use8 is set, 16 bits when it spans <= 65,535, absolute 32-bit columns otherwise) from a per-position base.
+// stored per slice in the narrowest class (8 bits when every position of the slice spans <= 255 columns, 16
+// bits when it spans <= 65,535, absolute 32-bit columns otherwise) from a per-position base, but never in a
+// class narrower than minbits (8, 16 or 32: 32 stores absolute columns in every slice).
 struct SellHost {
   int64_t nsl = 0, nslot = 0, n8 = 0, n16 = 0, n32 = 0;
   std::vector<int> perm;          // 64 nsl: internal row of each lane (-1: padding)
@@ -29,5 +30,5 @@ struct SellHost {
 
 // returns 0, or -1 (sizes beyond the format) / -5 (inconsistent input) with err set
 int sell_build_host(int64_t n, const std::vector<int64_t> &rp, const std::vector<int> &ci, const int *ro,
-                    const int *cr, const std::vector<int64_t> *groups, int W, bool use8, SellHost &out,
+                    const int *cr, const std::vector<int64_t> *groups, int W, int minbits, SellHost &out,
                     std::string &err);

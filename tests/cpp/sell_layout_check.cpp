@@ -5,7 +5,7 @@
 // every CSR entry is held exactly once, that the decoded column (base + 8/16-bit offset, or the 32-bit column)
 // is the entry's internal column, that a row's columns increase with the position, that padding is marked,
 // and that the row slot bases and group slices are consistent.  Prints one JSON line.
-//   usage: sell_layout_check nx ny nz nv r seed groups W use8 permute
+//   usage: sell_layout_check nx ny nz nv r seed groups W minbits permute   (minbits 8, 16 or 32)
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -21,11 +21,12 @@ static int fail(const std::string &m) {
 }
 
 int main(int argc, char **argv) {
-  if (argc < 11) return fail("usage: nx ny nz nv r seed groups W use8 permute");
+  if (argc < 11) return fail("usage: nx ny nz nv r seed groups W minbits permute");
   const int nx = atoi(argv[1]), ny = atoi(argv[2]), nz = atoi(argv[3]), nv = atoi(argv[4]), r = atoi(argv[5]);
   const unsigned seed = (unsigned)atoi(argv[6]);
   const int ngroups = atoi(argv[7]), W = atoi(argv[8]);
-  const bool use8 = atoi(argv[9]) != 0, permute = atoi(argv[10]) != 0;
+  const int minbits = atoi(argv[9]);
+  const bool permute = atoi(argv[10]) != 0;
   const int64_t nn = (int64_t)nx * ny * nz, n = nn * nv;
   auto node = [&](int i, int j, int k) { return ((int64_t)k * ny + j) * nx + i; };
   // CSR pattern in the natural numbering (dof = node * nv + v), columns sorted
@@ -60,7 +61,7 @@ int main(int argc, char **argv) {
   }
   SellHost h;
   std::string err;
-  if (sell_build_host(n, rp, ci, ro.data(), cr.data(), &groups, W, use8, h, err)) return fail("build: " + err);
+  if (sell_build_host(n, rp, ci, ro.data(), cr.data(), &groups, W, minbits, h, err)) return fail("build: " + err);
   const int C = 64;
   if ((int64_t)h.perm.size() != h.nsl * C || (int64_t)h.off.size() != h.nsl + 1 || h.off[h.nsl] != h.nslot)
     return fail("slice arrays");

@@ -1,0 +1,111 @@
+"""The three column-offset classes of the SELL copies (8-bit and 16-bit offsets from a per-position base, absolute
+32-bit columns), each forced onto every slice of small matrices through fm_set_sell_offsets: the level products
+(residual, restriction, prolongation) on the one-wave-per-slice kernel and on the split-slice kernel must equal
+the reference-order CSR products and the oracle's LASPACK-order product, and the FP32 V-cycle (whose products
+decode the same offsets in FP32) must still solve to the FP64 direct solution."""
+import functools
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from exfemus_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+
+def _heat(femus, ctx, dim, nx, nlev):
+    m = femus.Mesh(dim, nx, [0.0] * dim, [1.0] * dim, nlev)
+    s = femus.MGSolT(ctx, m, alpha0=1.0, dt=0.1, time_coeff=0.0, source=1.0, upwind=1.0)
+    s.GenBc(configs.heat_dirichlet_all)
+    s.init()
+    ks = configs.coarsen_coef(configs.lognormal_k(m.level(nlev - 1)[0], sigma=1.0), nlev, 2 ** dim)
+    for L in range(nlev):
+        s.set_elem_coef(L, ks[L])
+    s.set_xold(np.zeros(s._Dim[nlev - 1]))
+    for L in range(nlev):
+        s.GenMatRhs(0.0, L, 1 if L == nlev - 1 else 0)
+    return m, s
+
+
+def _ns(femus, ctx, dim, nx, nlev):
+    m = femus.Mesh(dim, nx, [0.0] * dim, [1.0] * dim, nlev)
+    s = femus.MGSolNS(ctx, m, IRe=0.01, dt=0.1, time_coeff=1.0, upwind=0.1)
+    s.GenBc(functools.partial(configs.cavity_bc, dim=dim))
+    s.init()
+    s.GenIc(functools.partial(configs.cavity_ic, dim=dim))
+    for L in range(nlev):
+        s.GenMatRhs(0.0, L, 1 if L == nlev - 1 else 0)
+    return m, s
+
+
+def _check_products(femus, oracle, s, nlev, rng):
+    lib = femus.lib()
+    W0 = lib.fm_set_sell_split(-1)
+    try:
+        for L in range(nlev):
+            n = s._Dim[L]
+            rp, ci, v = s.csr(0, L)
+            A = sp.csr_matrix((v, ci, rp), shape=(n, n))
+            x, b = rng.standard_normal(n), rng.standard_normal(n)
+            ref = b - A @ x
+            # the same rows through LASPACK's product order (oracle) as a second reference
+            np.testing.assert_allclose(ref, b - oracle.spmv(rp, ci, v, x), rtol=1e-12,
+                                       atol=1e-13 * (np.abs(b).max() + np.abs(v).sum()))
+            for W in (0, 2 ** 30):    # one wave per slice; 8 waves per slice
+                lib.fm_set_sell_split(W)
+                s.set_x(x, L)
+                s.set_rhs(b, L)
+                s.resid(L)
+                np.testing.assert_allclose(s.get_res(L), ref, rtol=1e-12,
+                                           atol=1e-13 * (np.abs(b).max() + np.abs(v).sum()), err_msg=f"L={L} W={W}")
+                if L > 0:
+                    res = s.get_res(L)
+                    s.restrict(L)
+                    rr, rc, rv = s.csr(1, L - 1)
+                    R = sp.csr_matrix((rv, rc, rr), shape=(s._Dim[L - 1], n))
+                    np.testing.assert_allclose(s.get_rhs(L - 1), R @ res, rtol=1e-12, atol=1e-12 * np.abs(res).max())
+                    xc = rng.standard_normal(s._Dim[L - 1])
+                    s.set_x(xc, L - 1)
+                    s.prolong(L)
+                    pr, pc, pv = s.csr(2, L)
+                    P = sp.csr_matrix((pv, pc, pr), shape=(n, s._Dim[L - 1]))
+                    np.testing.assert_allclose(s.get_res(L), P @ xc, rtol=1e-12, atol=1e-12 * np.abs(xc).max())
+    finally:
+        lib.fm_set_sell_split(W0)
+
+
+@pytest.mark.parametrize("bits", [8, 16, 32])
+@pytest.mark.parametrize("physics,dim,nx,nlev", [("heat", 3, [2, 2, 2], 3), ("heat", 2, [2, 3], 4),
+                                                 ("ns", 3, [2, 2, 2], 2), ("ns", 2, [2, 2], 3)])
+def test_offset_class_products(femus, oracle, ctx, bits, physics, dim, nx, nlev):
+    lib = femus.lib()
+    old = lib.fm_set_sell_offsets(bits)
+    try:
+        m, s = (_heat if physics == "heat" else _ns)(femus, ctx, dim, nx, nlev)
+    finally:
+        lib.fm_set_sell_offsets(old)
+    for which in (0, 1, 2):
+        for L in range(nlev):
+            if (which == 1 and L == nlev - 1) or (which == 2 and L == 0):
+                continue
+            info = s.sell_info(which, L)
+            slots, c8, c16, c32, nnz = (info[k] for k in ("slots", "slots_c8", "slots_c16", "slots_c32", "nnz"))
+            assert c8 + c16 + c32 >= slots >= nnz
+            if bits >= 16:
+                assert c8 == 0
+            if bits == 32:
+                assert c16 == 0 and c32 == slots
+    _check_products(femus, oracle, s, nlev, np.random.default_rng(17 + bits))
+    if physics == "heat":
+        top = nlev - 1
+        rp, ci, v = s.csr(0, top)
+        n = s._Dim[top]
+        xref = spla.spsolve(sp.csr_matrix((v, ci, rp), shape=(n, n)).tocsc(), s.get_rhs())
+        for prec in (femus.FM_PREC_FP32_FGMRES, femus.FM_PREC_FP32_IR):
+            s.set_x(np.zeros(n))
+            rep = s.MGSolve(opts=s.opts(precision=prec, tol=1e-10, max_iter=60))
+            assert rep.converged == 1
+            assert np.linalg.norm(s.get_x() - xref) <= 1e-8 * np.linalg.norm(xref)
+    s.close()

@@ -1,7 +1,7 @@
 """The SELL-64 layout of the level products, built on the host (exfemus_amd/csrc/sell_host.cpp, the code the
 library runs before uploading), checked slot by slot by tests/cpp/sell_layout_check.cpp on structured patterns
 (Q2-shaped couplings of radius 1 and 2, 1 to 4 variables per node), natural and random internal orders, row
-groups, sort windows of 64 and 4096 rows, 8-bit offsets on and off: every CSR entry is held exactly once at a
+groups, sort windows of 64 and 4096 rows, the narrowest offset class allowed (8, 16 or 32 bits): every CSR entry is held exactly once at a
 position of its own row, decodes to its internal column, rows are sorted, padding is marked, and each group
 starts a slice."""
 import json
@@ -14,13 +14,14 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CHECK = os.path.join(HERE, "cpp", "_build", "sell_layout_check")
 
 CASES = [
-    # nx ny nz nv r seed groups W use8 permute
-    (10, 8, 6, 1, 1, 1, 1, 4096, 1, 0),
-    (10, 8, 6, 1, 2, 3, 5, 4096, 1, 1),
-    (7, 9, 5, 4, 2, 7, 11, 64, 1, 1),
-    (12, 5, 5, 3, 1, 2, 3, 4096, 0, 0),
-    (33, 17, 9, 1, 2, 4, 1, 4096, 1, 0),
-    (80, 80, 40, 1, 1, 5, 4, 4096, 1, 1),   # random order of 256,000 rows: slices with 32-bit columns
+    # nx ny nz nv r seed groups W minbits permute
+    (10, 8, 6, 1, 1, 1, 1, 4096, 8, 0),
+    (10, 8, 6, 1, 2, 3, 5, 4096, 8, 1),
+    (7, 9, 5, 4, 2, 7, 11, 64, 8, 1),
+    (12, 5, 5, 3, 1, 2, 3, 4096, 16, 0),
+    (33, 17, 9, 1, 2, 4, 1, 4096, 8, 0),
+    (10, 8, 6, 2, 2, 6, 3, 4096, 32, 0),     # absolute columns in every slice
+    (80, 80, 40, 1, 1, 5, 4, 4096, 8, 1),   # random order of 256,000 rows: slices with 32-bit columns
 ]
 
 
@@ -36,10 +37,12 @@ def test_sell_layout(args):
     res, rc = _run(args)
     assert rc == 0 and res["ok"], res
     assert res["c8"] + res["c16"] + res["c32"] >= res["slots"] >= res["nnz"]
-    nx, ny, nz, nv, r, seed, groups, W, use8, permute = args
-    if not use8:
+    nx, ny, nz, nv, r, seed, groups, W, minbits, permute = args
+    if minbits >= 16:
         assert res["c8"] == 0
-    if not permute and r == 1 and nv == 1 and use8:
+    if minbits == 32:
+        assert res["c16"] == 0 and res["c32"] == res["slots"]
+    if not permute and r == 1 and nv == 1 and minbits == 8:
         assert res["c8"] > res["c16"]        # natural order, narrow stencil: mostly 8-bit offsets
     if args == CASES[-1]:
         assert res["c32"] > 0
