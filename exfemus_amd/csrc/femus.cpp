@@ -21,6 +21,22 @@ void fm_set_error(const std::string &msg) { g_err = msg; g_nerr++; }
 unsigned fm_error_count() { return g_nerr; }
 extern "C" const char *fm_last_error(void) { return g_err.c_str(); }
 extern "C" int fm_set_sell_split(int W) { return fmk::sell_split(W); }
+// narrowest column-offset class of SELL copies built from now on (FEMUS_SELL_C8=0 starts it at 16)
+static int g_sell_minbits = -1;
+static int sell_minbits() {
+  if (g_sell_minbits < 0) {
+    const char *c8env = getenv("FEMUS_SELL_C8");
+    g_sell_minbits = (c8env && atoi(c8env) == 0) ? 16 : 8;
+  }
+  return g_sell_minbits;
+}
+extern "C" int fm_set_sell_offsets(int bits) {
+  const int old = sell_minbits();
+  if (bits < 0) return old;
+  if (bits != 8 && bits != 16 && bits != 32) return fm_einval(__func__);
+  g_sell_minbits = bits;
+  return old;
+}
 
 template <typename T> static int dalloc(T **p, size_t n) {
   if (n == 0) n = 1;
@@ -226,15 +242,13 @@ static void free_csr(DevCSR &c) { fm_free_csr(c); }
 // the slices of chosen groups.
 int fm_build_sell(hipStream_t st, DevCSR &c, const int *ro, const int *cr, const std::vector<int64_t> *groups) {
   // W: rows are length-sorted inside windows of W rows (FEMUS_SELL_SIGMA, default 4096; 64 = inside a slice);
-  // FEMUS_SELL_C8=0 keeps every slice at 16-bit offsets or wider
+  // column offsets no narrower than fm_set_sell_offsets allows
   const char *sg = getenv("FEMUS_SELL_SIGMA");
   const int C = 64, W = sg && atoi(sg) >= 64 ? atoi(sg) : 4096;
-  const char *c8env = getenv("FEMUS_SELL_C8");
-  const bool use8 = !(c8env && atoi(c8env) == 0);
   const int64_t n = c.nr;
   SellHost h;
   std::string err;
-  const int hrc = sell_build_host(n, c.h_rp, c.h_ci, ro, cr, groups, W, use8, h, err);
+  const int hrc = sell_build_host(n, c.h_rp, c.h_ci, ro, cr, groups, W, sell_minbits(), h, err);
   if (hrc) { fm_set_error(err); return hrc == -1 ? FM_EINVAL : FM_ESTATE; }
   c.sell.h_gslice = h.gslice;
   const int64_t nsl = h.nsl, nslot = h.nslot, n8 = h.n8, n16 = h.n16, n32 = h.n32;
@@ -325,6 +339,7 @@ extern "C" int fm_system_destroy(fm_system *s) {
   fm_drop_graphs(s);
   (void)hipSetDevice(s->ctx->device);
   (void)hipStreamSynchronize(s->ctx->stream);
+  fm_krylov_free(s);
   for (auto &l : s->lev) fm_free_level(l);
   for (auto &l : s->alev) fm_free_level(l);
   dfree(s->d_bc0); dfree(s->d_bc1); dfree(s->d_dof_top); dfree(s->xyz); dfree(s->x_old); dfree(s->vel); dfree(s->xbc);
@@ -612,6 +627,7 @@ static int check_level(const fm_system *s, int L) {
 
 extern "C" int fm_system_set_xold(fm_system *s, const double *x) {
   FM_CHECK(check_level(s, 0));
+  if (!x) return fm_einval(__func__);
   FM_HIP(hipMemcpyAsync(s->x_old, x, s->Dim.back() * sizeof(double), hipMemcpyHostToDevice, s->ctx->stream));
   FM_HIP(hipStreamSynchronize(s->ctx->stream));
   return FM_OK;
@@ -870,6 +886,7 @@ extern "C" int fm_ns_genmatrhs(fm_system *s, int L, int mode, const fm_ns_params
 // ===========================================================================
 extern "C" int fm_spmv(fm_system *s, int L, const double *xh, double *yh) {
   FM_CHECK(check_level(s, L));
+  if (!xh || !yh) return fm_einval(__func__);
   Level &l = s->lev[L];
   hipStream_t st = s->ctx->stream;
   FM_HIP(hipMemcpyAsync(l.t, xh, l.n * sizeof(double), hipMemcpyHostToDevice, st));
@@ -928,26 +945,42 @@ extern "C" int fm_prolong(fm_system *s, int L) {
 
 extern "C" int fm_bench_spmv(fm_system *s, int L, int prec, int reps, double *sec) {
   FM_CHECK(check_level(s, L));
-  if (reps < 1 || !sec || prec < 0 || prec > 3) return fm_einval(__func__);
+  if (reps < 1 || !sec || prec < 0 || prec > 5) return fm_einval(__func__);
   Level &l = s->lev[L];
   hipStream_t st = s->ctx->stream;
-  // prec 0 / 1: SELL-64 FP64 / FP32 (the solver's format); 2 / 3: CSR FP64 / FP32
+  // prec 0 / 1: SELL-64 FP64 / FP32 y = A x (the solver's format); 2 / 3: CSR FP64 / FP32 y = A x;
+  // 4 / 5: the SELL Chebyshev smoothing step FP64 / FP32 (r -= A d, d' = c1 d + c2 D^-1 r, x += d'), the
+  // product the V-cycle spends its time in.  The level's own work vectors are overwritten.
   CsrView<double> v64 = view<double>(l.A);
   CsrView<float> v32 = view<float>(l.A);
-  if (prec >= 2) {
+  if (prec == 2 || prec == 3) {
     FM_CHECK(fm_csr_from_sell(st, l.A));
     v64.nsl = 0; v32.nsl = 0;
   }
-  bool f32 = (prec & 1) != 0;
+  const bool f32 = (prec & 1) != 0, cheb = prec >= 4;
   FM_CHECK(fmk::fill<double>(st, l.n, l.t, 1.0));
   FM_CHECK(fmk::fill<float>(st, l.n, l.tf, 1.0f));
-  if (!f32) FM_CHECK(fmk::spmv(st, v64, l.t, l.d));
-  else FM_CHECK(fmk::spmv(st, v32, l.tf, l.df));
-  FM_HIP(hipEventRecord(s->ctx->ev0, st));
-  for (int k = 0; k < reps; k++) {
-    if (!f32) FM_CHECK(fmk::spmv(st, v64, l.t, l.d));
-    else FM_CHECK(fmk::spmv(st, v32, l.tf, l.df));
+  if (cheb) {
+    // a damped momentum-Jacobi recurrence (c1 = 0.3, c2 = 0.25 with the Jacobi diagonal): bounded iterates
+    FM_CHECK(fmk::dinv_slot(st, l.n, l.dslot, l.A.sell.v, l.dinv, l.dinvf));
+    FM_CHECK(fmk::fill<double>(st, l.n, l.r, 1.0)); FM_CHECK(fmk::fill<double>(st, l.n, l.x, 0.0));
+    FM_CHECK(fmk::fill<double>(st, l.n, l.d, 0.0)); FM_CHECK(fmk::fill<double>(st, l.n, l.d2, 0.0));
+    FM_CHECK(fmk::fill<float>(st, l.n, l.rf, 1.0f)); FM_CHECK(fmk::fill<float>(st, l.n, l.xf, 0.0f));
+    FM_CHECK(fmk::fill<float>(st, l.n, l.df, 0.0f)); FM_CHECK(fmk::fill<float>(st, l.n, l.d2f, 0.0f));
   }
+  int k2 = 0;
+  auto one = [&]() -> int {
+    if (cheb) {
+      const int k = k2++;
+      if (!f32) return fmk::cheb_step(st, v64, l.r, l.x, k & 1 ? l.d2 : l.d, k & 1 ? l.d : l.d2, l.dinv, 0.3, 0.25);
+      return fmk::cheb_step(st, v32, l.rf, l.xf, k & 1 ? l.d2f : l.df, k & 1 ? l.df : l.d2f, l.dinvf, 0.3f, 0.25f);
+    }
+    if (!f32) return fmk::spmv(st, v64, l.t, l.d);
+    return fmk::spmv(st, v32, l.tf, l.df);
+  };
+  FM_CHECK(one());
+  FM_HIP(hipEventRecord(s->ctx->ev0, st));
+  for (int k = 0; k < reps; k++) FM_CHECK(one());
   FM_HIP(hipEventRecord(s->ctx->ev1, st));
   FM_HIP(hipEventSynchronize(s->ctx->ev1));
   float ms = 0.f;

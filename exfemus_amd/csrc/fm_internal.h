@@ -143,6 +143,7 @@ struct fm_ctx {
 // a captured V-cycle (one per precision / option set / forward-or-adjoint)
 struct GraphEntry {
   int prec, adj, xz, smoother, nu_pre, nu_post, correction, gamma, colored, cmult;
+  int split;  // split-slice threshold of the level products at capture (the launch shapes it recorded)
   double lo, hi, omega;
   hipGraphExec_t exec;
 };
@@ -176,7 +177,16 @@ struct fm_system {
   int mg_smoother = -1;
   int vanka_store = 3;  // precisions of the stored Vanka inverses: bit 0 FP64 (FP64 cycles), bit 1 FP32 (FP32 cycles)
   FeTab *d_tab = nullptr;  // reference-element tables on the device (built at the first fm_system_init)
+  // flexible-GMRES workspace of the top level, kept from one solve to the next (fm_mg_release frees it):
+  // basis V [(km + 1) kn], preconditioned vectors Z [km kn], Gram-Schmidt partial sums, the Hessenberg
+  // state (coefficients, rotations, residual estimates) and a pinned mirror of the estimates with one event
+  // per Arnoldi step
+  double *kV = nullptr, *kZ = nullptr, *kpart = nullptr, *kH = nullptr, *kpin = nullptr;
+  int64_t kn = 0;
+  int km = 0, knb = 0;
+  std::vector<hipEvent_t> kev;
 };
+void fm_krylov_free(fm_system *s);
 
 // views of a device matrix for the kernel launchers (SELL copy when present)
 #include "fm_kernels.h"

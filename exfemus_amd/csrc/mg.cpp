@@ -261,9 +261,10 @@ static int cycle_top(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &ops, 
   const int top = (int)ops.size() - 1;
   if (!o->use_graph) return vcycle<T>(s, o, ops, top, xz);
   const int prec = sizeof(T) == 8 ? 64 : 32;
+  const int split = fmk::sell_split(-1);
   hipStream_t st = s->ctx->stream;
   for (auto &g : s->graphs)
-    if (g.prec == prec && g.adj == adj && g.xz == (int)xz && g.smoother == o->smoother && g.nu_pre == o->nu_pre && g.nu_post == o->nu_post &&
+    if (g.prec == prec && g.adj == adj && g.xz == (int)xz && g.split == split && g.smoother == o->smoother && g.nu_pre == o->nu_pre && g.nu_post == o->nu_post &&
         g.correction == o->correction && g.gamma == o->gamma && g.cmult == o->coarse_mult && g.colored == o->vanka_colored && g.lo == o->cheb_lo &&
         g.hi == o->cheb_hi && g.omega == o->omega)
       return hipGraphLaunch(g.exec, st) == hipSuccess ? 0 : FM_EHIP;
@@ -277,7 +278,7 @@ static int cycle_top(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &ops, 
     return rc ? rc : FM_EHIP;
   }
   GraphEntry g;
-  g.prec = prec; g.adj = adj; g.xz = (int)xz; g.smoother = o->smoother; g.nu_pre = o->nu_pre; g.nu_post = o->nu_post;
+  g.prec = prec; g.adj = adj; g.xz = (int)xz; g.split = split; g.smoother = o->smoother; g.nu_pre = o->nu_pre; g.nu_post = o->nu_post;
   g.correction = o->correction; g.gamma = o->gamma; g.cmult = o->coarse_mult; g.colored = o->vanka_colored; g.lo = o->cheb_lo; g.hi = o->cheb_hi;
   g.omega = o->omega;
   e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
@@ -318,6 +319,7 @@ extern "C" int fm_mg_release(fm_system *s, int adjoint) {
   FM_HIP(hipSetDevice(s->ctx->device));
   FM_HIP(hipStreamSynchronize(s->ctx->stream));
   fm_drop_graphs(s);  // captured cycles hold the addresses released here
+  fm_krylov_free(s);  // the flexible-GMRES workspace is shared by both hierarchies
   for (Level &l : adjoint ? s->alev : s->lev) {
     for (double **p : {&l.vinv, &l.Ainv}) if (*p) { (void)hipFree(*p); *p = nullptr; }
     for (float **p : {&l.vinvf, &l.Ainvf}) if (*p) { (void)hipFree(*p); *p = nullptr; }
@@ -475,6 +477,106 @@ __global__ void k_scale_inv(int64_t n, double *w, const double *nrm) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) w[i] *= r;
 }
 
+// Hessenberg state of one restart cycle, on the device (offsets into fm_system::kH, m = restart length):
+// H [(m + 1) m] row-major, cs [m], sn [m], g [m + 1], y [m], rel [m] (residual estimate after each step),
+// st [8] (st[0]: columns before the first breakdown, -1 while there is none), then the Gram-Schmidt scratch
+// hd [256]: [0, 64) first pass, [64, 128) second pass, [128] norm of the new vector, [129] reorthogonalisation flag
+struct KryLayout {
+  int64_t H, cs, sn, g, y, rel, st, hd, total;
+  explicit KryLayout(int m) {
+    H = 0; cs = H + (int64_t)(m + 1) * m; sn = cs + m; g = sn + m; y = g + m + 1; rel = y + m; st = rel + m;
+    hd = st + 8; total = hd + 256;
+  }
+};
+
+// start of a restart cycle: g = beta e_1 (beta = |r| from the device scalar), no breakdown yet
+__global__ void k_fgmres_start(int m, double *g, double *st, const double *beta) {
+  for (int i = threadIdx.x; i <= m; i += blockDim.x) g[i] = i == 0 ? beta[0] : 0.;
+  if (threadIdx.x == 0) st[0] = -1.;
+}
+// Column j of the Hessenberg matrix from the Gram-Schmidt coefficients (two passes) and the new vector's
+// norm, the earlier Givens rotations applied to it, the rotation that zeroes its subdiagonal, the rotated
+// right-hand side g and the residual estimate rel[j] = |g[j + 1]| / bn.  One thread: at most 60 columns.
+__global__ void k_arnoldi_col(int j, int m, const double *hd, double *H, double *cs, double *sn, double *g,
+                              double *rel, double *st, double bn) {
+  if (threadIdx.x != 0) return;
+  for (int k = 0; k <= j; k++) H[(int64_t)k * m + j] = hd[k] + hd[64 + k];
+  const double hn = hd[128];
+  H[(int64_t)(j + 1) * m + j] = hn;
+  for (int k = 0; k < j; k++) {
+    const double a = H[(int64_t)k * m + j], b = H[(int64_t)(k + 1) * m + j];
+    H[(int64_t)k * m + j] = cs[k] * a + sn[k] * b;
+    H[(int64_t)(k + 1) * m + j] = -sn[k] * a + cs[k] * b;
+  }
+  const double a = H[(int64_t)j * m + j], b = H[(int64_t)(j + 1) * m + j];
+  const double r = sqrt(a * a + b * b);
+  cs[j] = r > 0. ? a / r : 1.;
+  sn[j] = r > 0. ? b / r : 0.;
+  H[(int64_t)j * m + j] = r;
+  H[(int64_t)(j + 1) * m + j] = 0.;
+  g[j + 1] = -sn[j] * g[j];
+  g[j] = cs[j] * g[j];
+  rel[j] = fabs(g[j + 1]) / bn;
+  // a zero new vector (the Krylov space is invariant: the solution is in it) ends the usable columns
+  if (!(hn > 0.) && st[0] < 0.) st[0] = (double)(j + 1);
+}
+// y = H^-1 g over the first jn columns (fewer after a breakdown; the rest of y is zero)
+__global__ void k_fgmres_y(int jn, int m, const double *H, const double *g, double *y, const double *st) {
+  if (threadIdx.x != 0) return;
+  const int j = st[0] > 0. ? min(jn, (int)st[0]) : jn;
+  for (int i = jn - 1; i >= j; i--) y[i] = 0.;
+  for (int i = j - 1; i >= 0; i--) {
+    double acc = g[i];
+    for (int k = i + 1; k < j; k++) acc -= H[(int64_t)i * m + k] * y[k];
+    const double d = H[(int64_t)i * m + i];
+    y[i] = d != 0. ? acc / d : 0.;
+  }
+}
+
+void fm_krylov_free(fm_system *s) {
+  for (double **p : {&s->kV, &s->kZ, &s->kpart, &s->kH})
+    if (*p) { (void)hipFree(*p); *p = nullptr; }
+  if (s->kpin) { (void)hipHostFree(s->kpin); s->kpin = nullptr; }
+  for (hipEvent_t e : s->kev) (void)hipEventDestroy(e);
+  s->kev.clear();
+  s->kn = 0; s->km = 0; s->knb = 0;
+}
+
+// the workspace for restart length m and top-level size n (allocated once, reused while both stay)
+static int krylov_ensure(fm_system *s, int64_t n, int m, int nb) {
+  if (s->kV && s->kn == n && s->km == m && s->knb == nb) return 0;
+  FM_HIP(hipStreamSynchronize(s->ctx->stream));
+  fm_krylov_free(s);
+  const KryLayout K(m);
+  int rc = 0;
+  if (hipMalloc((void **)&s->kV, (size_t)(m + 1) * n * sizeof(double)) != hipSuccess ||
+      hipMalloc((void **)&s->kZ, (size_t)m * n * sizeof(double)) != hipSuccess ||
+      hipMalloc((void **)&s->kpart, (size_t)(m + 1) * nb * sizeof(double)) != hipSuccess ||
+      hipMalloc((void **)&s->kH, (size_t)K.total * sizeof(double)) != hipSuccess)
+    rc = FM_ENOMEM;
+  if (!rc && hipHostMalloc((void **)&s->kpin, (size_t)m * sizeof(double), hipHostMallocDefault) != hipSuccess) rc = FM_EHIP;
+  for (int j = 0; j < m && !rc; j++) {
+    hipEvent_t e;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) { rc = FM_EHIP; break; }
+    s->kev.push_back(e);
+  }
+  if (rc) {
+    fm_krylov_free(s);
+    fm_set_error("flexible GMRES workspace: allocation failed");
+    return rc;
+  }
+  s->kn = n; s->km = m; s->knb = nb;
+  return 0;
+}
+
+// Flexible GMRES(m), right preconditioned by one V-cycle (FP32 or FP64).  The Arnoldi steps run without
+// host round trips: the Gram-Schmidt coefficients, the Hessenberg column, its Givens rotation and the
+// residual estimate are formed on the device, and each step's estimate is copied to pinned memory behind
+// an event.  The host waits only at checkpoints: after a checkpoint with estimate r, the next
+// ceil(log(tol / 2 r) / log rho) steps are launched without waiting, rho being the fastest reduction per
+// step assumed possible: 1/20, or the fastest one seen if that was faster (GMRES accelerates, so the rate
+// of the first steps is no bound).  A step launched after the one that converged only costs time: the
+// least-squares update over more columns is at least as good.
 static int solve_fgmres(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &ops, fm_mg_report *rep, int adj) {
   fm_ctx *c = s->ctx;
   hipStream_t st = c->stream;
@@ -482,25 +584,19 @@ static int solve_fgmres(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &op
   LOps &ot = ops[top];
   Level *l = ot.v;
   int64_t n = ot.n;
-  int m = std::max(1, std::min(o->fgmres_restart > 0 ? o->fgmres_restart : 30, 60));
-  double *V = nullptr, *Z = nullptr, *hd = nullptr;
-  if (hipMalloc((void **)&V, (size_t)(m + 1) * n * sizeof(double)) != hipSuccess) return FM_ENOMEM;
-  if (hipMalloc((void **)&Z, (size_t)m * n * sizeof(double)) != hipSuccess) { (void)hipFree(V); return FM_ENOMEM; }
-  // hd: [0, 64) first Gram-Schmidt pass, [64, 128) second pass, [128] norm of the new vector, [192, ...) y
-  if (hipMalloc((void **)&hd, 256 * sizeof(double)) != hipSuccess) { (void)hipFree(V); (void)hipFree(Z); return FM_ENOMEM; }
+  const int m = std::max(1, std::min(o->fgmres_restart > 0 ? o->fgmres_restart : 30, 60));
   const int nb = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + 2047) / 2048));
-  double *part = c->scratch;  // (m + 1) nb partial sums: the context scratch when they fit, else allocated
-  double *part_alloc = nullptr;
-  if ((int64_t)(m + 1) * nb > 2048) {
-    if (hipMalloc((void **)&part_alloc, (size_t)(m + 1) * nb * sizeof(double)) != hipSuccess) { (void)hipFree(V); (void)hipFree(Z); (void)hipFree(hd); return FM_ENOMEM; }
-    part = part_alloc;
-  }
-  std::vector<double> H((size_t)(m + 1) * m, 0.), cs(m), sn(m), g(m + 1), y(m);
+  FM_CHECK(krylov_ensure(s, n, m, nb));
+  double *V = s->kV, *Z = s->kZ, *part = s->kpart;
+  const KryLayout K(m);
+  double *H = s->kH + K.H, *cs = s->kH + K.cs, *sn = s->kH + K.sn, *g = s->kH + K.g, *y = s->kH + K.y;
+  double *relh = s->kH + K.rel, *stt = s->kH + K.st, *hd = s->kH + K.hd;
   CsrView<double> A = cview<double>(*ot.A);
   int rc = 0;
   double bn = dev_dot<double>(c, n, l->b, l->b, true);
   int total = 0;
   double rel = 1.;
+  double rho = 0.05;  // fastest reduction of the estimate per step assumed possible (lowered if one is faster)
   unsigned gv = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
   bool res_fresh = false;  // V holds b - A x of the current x (the exit residual needs no product)
   bool x_zero = dev_dot<double>(c, n, l->x, l->x, true) == 0.;
@@ -513,19 +609,28 @@ static int solve_fgmres(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &op
       goto done;
     }
     {
-      double beta = dev_dot<double>(c, n, V, V, true);
+      // beta = |r| stays on the device for g; the host needs it for the restart test
+      if ((rc = fmk::dot_dev<double>(st, n, V, V, c->scratch, hd + 130, 1)) != 0) goto done;
+      if (hipMemcpyAsync(c->h_scratch, hd + 130, sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipStreamSynchronize(st) != hipSuccess) { rc = FM_EHIP; goto done; }
+      const double beta = c->h_scratch[0];
       rel = beta / bn;
       if (total == 0) rep->hist[0] = rel;
       if (rel <= o->tol || !std::isfinite(rel)) { res_fresh = true; break; }
-      if ((rc = fmk::axpby<double>(st, n, 1.0 / beta, V, 0.0, V)) != 0) goto done;
-      std::fill(g.begin(), g.end(), 0.);
-      g[0] = beta;
-      int j = 0;
-      for (; j < m && total < o->max_iter; j++) {
+      hipLaunchKernelGGL(k_scale_inv, dim3(gv), dim3(256), 0, st, n, V, hd + 130);
+      hipLaunchKernelGGL(k_fgmres_start, dim3(1), dim3(64), 0, st, m, g, stt, hd + 130);
+      // steps from an estimate r that cannot reach tol / 2 at the reduction rho per step, plus one (1 .. m)
+      auto lookahead = [&](double r) -> int {
+        const double need = std::ceil(std::log(0.5 * o->tol / r) / std::log(rho));
+        return need >= 1. ? (int)std::min(need, (double)m) : 1;
+      };
+      int j = 0, jv = -1, check_at = lookahead(rel) - 1;
+      double rv = rel;
+      bool stop = false;
+      while (j < m && total < o->max_iter && !stop) {
         double *vj = V + (size_t)j * n, *zj = Z + (size_t)j * n, *w = V + (size_t)(j + 1) * n;
         // z_j = M v_j
         if (o->precision == FM_PREC_FP64_FGMRES) {
-          // use level-top b/x of the FP64 cycle: save b in t
           // the FP64 cycle works on the level's own b / x: park them in sb / sx
           if (hipMemcpyAsync(l->sb, l->b, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
           if (hipMemcpyAsync(l->b, vj, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
@@ -541,8 +646,8 @@ static int solve_fgmres(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &op
           if ((rc = cycle_top<float>(s, o, ops, adj, true)) != 0) goto done;
           if ((rc = fmk::from_f32(st, n, l->xf, zj)) != 0) goto done;
         }
-        // w = A z_j ; classical Gram-Schmidt twice and the normalisation, all on the device; one
-        // copy of the Hessenberg column back to the host per iteration
+        // w = A z_j; classical Gram-Schmidt (second pass only when the DGKS test asks), normalisation and the
+        // Hessenberg column, all on the device
         if ((rc = fmk::spmv(st, A, zj, w)) != 0) goto done;
         for (int pass = 0; pass < 2; pass++) {
           // pass 0 also forms |w|^2 (kk = j + 2) for the reorthogonalisation test; pass 1 runs only if it asks
@@ -555,36 +660,30 @@ static int solve_fgmres(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &op
         }
         if ((rc = fmk::dot_dev<double>(st, n, w, w, c->scratch, hd + 128, 1)) != 0) goto done;
         hipLaunchKernelGGL(k_scale_inv, dim3(gv), dim3(256), 0, st, n, w, hd + 128);
-        if (hipMemcpyAsync(c->h_scratch, hd, 129 * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess) { rc = FM_EHIP; goto done; }
-        if (hipStreamSynchronize(st) != hipSuccess) { rc = FM_EHIP; goto done; }
-        for (int k = 0; k <= j; k++) H[(size_t)k * m + j] = c->h_scratch[k] + c->h_scratch[64 + k];
-        const double hn = c->h_scratch[128];
-        H[(size_t)(j + 1) * m + j] = hn;
-        // Givens
-        for (int k = 0; k < j; k++) {
-          double a = H[(size_t)k * m + j], b = H[(size_t)(k + 1) * m + j];
-          H[(size_t)k * m + j] = cs[k] * a + sn[k] * b;
-          H[(size_t)(k + 1) * m + j] = -sn[k] * a + cs[k] * b;
-        }
-        double a = H[(size_t)j * m + j], b = H[(size_t)(j + 1) * m + j];
-        double r = std::sqrt(a * a + b * b);
-        cs[j] = r > 0 ? a / r : 1.; sn[j] = r > 0 ? b / r : 0.;
-        H[(size_t)j * m + j] = r; H[(size_t)(j + 1) * m + j] = 0.;
-        g[j + 1] = -sn[j] * g[j];
-        g[j] = cs[j] * g[j];
+        hipLaunchKernelGGL(k_arnoldi_col, dim3(1), dim3(64), 0, st, j, m, hd, H, cs, sn, g, relh, stt, bn);
+        if (hipMemcpyAsync(s->kpin + j, relh + j, sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipEventRecord(s->kev[j], st) != hipSuccess) { rc = FM_EHIP; goto done; }
+        j++;
         total++;
-        rel = std::fabs(g[j + 1]) / bn;
-        if (total < 256) rep->hist[total] = rel;
-        if (rel <= o->tol * 0.5 || hn == 0.) { j++; break; }
+        if (j - 1 < check_at && j < m && total < o->max_iter) continue;
+        // checkpoint: read the estimates of the steps launched since the last one
+        if (hipEventSynchronize(s->kev[j - 1]) != hipSuccess) { rc = FM_EHIP; goto done; }
+        for (int q = jv + 1; q < j; q++) {
+          const double r = s->kpin[q], prev = q == jv + 1 ? rv : s->kpin[q - 1];
+          const int it = total - j + q + 1;
+          if (it < 256) rep->hist[it] = r;
+          if (prev > 0. && r > 0. && r < prev) rho = std::min(rho, r / prev);
+          if (r <= o->tol * 0.5 || !std::isfinite(r)) stop = true;
+        }
+        jv = j - 1;
+        rv = s->kpin[jv];
+        rel = rv;
+        check_at = jv + 1;
+        if (!stop && rv > 0.) check_at = jv + lookahead(rv);
       }
-      // solve H y = g, x += Z y
-      for (int i = j - 1; i >= 0; i--) {
-        double sacc = g[i];
-        for (int k = i + 1; k < j; k++) sacc -= H[(size_t)i * m + k] * y[k];
-        y[i] = sacc / H[(size_t)i * m + i];
-      }
-      if (hipMemcpyAsync(hd + 192, y.data(), j * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
-      hipLaunchKernelGGL(k_multi_add, dim3(gv), dim3(256), 0, st, n, j, Z, hd + 192, l->x);
+      // y = H^-1 g and x += Z y, on the device
+      hipLaunchKernelGGL(k_fgmres_y, dim3(1), dim3(64), 0, st, j, m, H, g, y, stt);
+      hipLaunchKernelGGL(k_multi_add, dim3(gv), dim3(256), 0, st, n, j, Z, y, l->x);
     }
   }
   // true FP64 residual at exit (already in V when the convergence check at the loop top ended the solve)
@@ -596,8 +695,6 @@ static int solve_fgmres(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &op
   }
 done:
   (void)hipStreamSynchronize(st);
-  (void)hipFree(V); (void)hipFree(Z); (void)hipFree(hd);
-  if (part_alloc) (void)hipFree(part_alloc);
   rep->iters = total;
   rep->rel_resid = rel;
   rep->converged = rel <= o->tol;

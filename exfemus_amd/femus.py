@@ -27,8 +27,8 @@ FM_PREC_FP64, FM_PREC_FP32_IR, FM_PREC_FP32_FGMRES, FM_PREC_FP64_FGMRES = 0, 1, 
 FM_SMOOTH_CHEBYSHEV, FM_SMOOTH_JACOBI, FM_SMOOTH_VANKA, FM_SMOOTH_CHEBYSHEV4 = 0, 1, 2, 3
 FM_ENOCONV = -3
 
-# storage format of the SELL SpMV (names the layout the PMC traffic figures were measured on)
-SPMV_FORMAT = "SELL-64 sigma4096 slab-grouped node order, 8- or 16-bit col offsets per slice"
+# storage format of the SELL SpMV
+SPMV_FORMAT = "SELL-64 sigma4096 slab-grouped node order, 8-, 16- or 32-bit col offsets per slice"
 
 # Exported entry points (the C ABI of include/femus_hip.h)
 EXPORTS = [
@@ -42,7 +42,7 @@ EXPORTS = [
     "fm_heat_grad_coef", "fm_bench_spmv", "fm_sync", "fm_heat_genmat_transpose", "fm_ns_genmat_transpose", "fm_ns_genjac",
     "fm_get_adjoint_csr", "fm_heat_dissipation", "fm_bench_stream", "fm_ns_dissipation", "fm_ns_grad_coef",
     "fm_system_set_bcdata", "fm_heat_bc_rhs", "fm_ns_bc_rhs", "fm_ns_pressure_drop",
-    "fm_ns_set_buoyancy", "fm_heat_nusselt", "fm_heat_coupling", "fm_adjoint_spmv", "fm_set_sell_split", "fm_system_sell_info",
+    "fm_ns_set_buoyancy", "fm_heat_nusselt", "fm_heat_coupling", "fm_adjoint_spmv", "fm_set_sell_split", "fm_set_sell_offsets", "fm_system_sell_info",
     "fm_system_set_vanka_store", "fm_mg_release", "fm_system_couple", "fm_system_advance",
 ]
 
@@ -93,6 +93,7 @@ def lib():
     pp = C.POINTER(C.c_void_p)
     L.fm_last_error.restype = C.c_char_p
     L.fm_set_sell_split.argtypes = [C.c_int]
+    L.fm_set_sell_offsets.argtypes = [C.c_int]
     L.fm_system_sell_info.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
     L.fm_system_set_vanka_store.argtypes = [C.c_void_p, C.c_int]
     L.fm_mg_release.argtypes = [C.c_void_p, C.c_int]

@@ -44,8 +44,16 @@ const char *fm_last_error(void);
  * products are sequential): products over a SELL matrix with fewer than W slices of
  * 64 rows let 2, 4 or 8 wavefronts share a slice (default W = 16384, or the
  * FEMUS_SELL_SPLIT environment variable; 0 = never).  W < 0 only reads the value.
- * Returns the previous W.  HIP graphs captured before a change keep their shape. */
+ * Returns the previous W.  Captured V-cycle graphs are keyed on W: a solve after a change captures
+ * (and keeps) a graph of the new shape. */
 int fm_set_sell_split(int W);
+/* Column-offset width knob of the SELL copies (no counterpart in the reference, whose LASPACK rows store
+ * full column indices): each slice keeps its columns as 8-bit or 16-bit offsets from a per-position base
+ * when its span allows, else as absolute 32-bit columns.  bits = 8 (default; FEMUS_SELL_C8=0 starts at 16)
+ * lets every class be used, 16 never uses 8-bit offsets, 32 stores absolute columns in every slice.
+ * Applies to matrices set up afterwards (fm_system_init); bits < 0 only reads it.  Returns the previous
+ * value, or FM_EINVAL for another width. */
+int fm_set_sell_offsets(int bits);
 
 /* ---- mesh hierarchy: gencase GenCase + printMesh -----------------------
  * USER_APPL/gencase/src/MGGenCase.C:63-229 (build_square/build_cube +
@@ -199,7 +207,9 @@ int fm_mg_solve(fm_system *s, const fm_mg_opts *o, fm_mg_report *rep);   /* x[to
  * Vanka patch inverses, bit 0 FP64 (FP64 cycles), bit 1 FP32 (FP32 cycles), default 3; 0 only
  * reads it; returns the previous mask.  A solve whose cycles need a precision left out fails
  * with FM_ESTATE.  fm_mg_release: frees the patch and coarse inverses of the forward
- * (adjoint = 0) or transposed (adjoint = 1) hierarchy; the next solve sets them up again. */
+ * (adjoint = 0) or transposed (adjoint = 1) hierarchy, and the flexible-GMRES workspace that solves
+ * keep between calls ((2 m + 1) top-level vectors for restart length m); the next solve sets them up
+ * again. */
 int fm_system_set_vanka_store(fm_system *s, int mask);
 int fm_mg_release(fm_system *s, int adjoint);
 /* Diagnostic: the FP64 local inverse K_e^-1 the Vanka smoother applies for element e on level L
@@ -271,9 +281,10 @@ int fm_heat_coupling(fm_system *s, const fm_heat_params *p, int mode, double sca
 int fm_ns_pressure_drop(fm_system *s, const double *U_host, double *J, double *dJdU_host);
 
 /* ---- timing helpers for benchmarks ------------------------------------- */
-/* Run `reps` SpMVs y = A[L] x on device buffers (prec 0 = FP64, 1 = FP32);
- * returns the average kernel time in seconds via *sec (HIP events on the
- * library stream). */
+/* Run `reps` products of A[L] on device buffers: prec 0 / 1 = y = A x on the SELL copy in FP64 / FP32,
+ * 2 / 3 = y = A x on the CSR copy, 4 / 5 = the SELL Chebyshev smoothing step (r -= A d,
+ * d' = c1 d + c2 D^-1 r, x += d') in FP64 / FP32; returns the average kernel time in seconds via *sec
+ * (HIP events on the library stream).  Overwrites the level's work vectors. */
 int fm_bench_spmv(fm_system *s, int L, int prec, int reps, double *sec);
 int fm_sync(fm_ctx *ctx);
 /* STREAM copy of `bytes` between two fresh device buffers (read + write),

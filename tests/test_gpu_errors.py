@@ -40,6 +40,18 @@ def test_invalid_arguments(femus, ctx):
     assert L.fm_vanka_patch(sysT.h, 1, 0, np.zeros(9, np.int32).ctypes.data_as(C.c_void_p),
                             np.zeros(81).ctypes.data_as(C.c_void_p)) == FM_ESTATE   # no Vanka inverses
     assert b"Vanka" in L.fm_last_error()
+    # null host pointers are invalid arguments that name the call (a second handle on the library takes raw
+    # pointers, the mirror's argument types would refuse None before the call)
+    raw = C.CDLL(femus.LIBPATH)
+    raw.fm_system_set_xold.argtypes = [C.c_void_p, C.c_void_p]
+    raw.fm_spmv.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+    buf = np.zeros(sysT._Dim[1])
+    assert raw.fm_system_set_xold(sysT.h, None) == FM_EINVAL
+    assert b"fm_system_set_xold" in L.fm_last_error()
+    assert raw.fm_spmv(sysT.h, 1, None, buf.ctypes.data_as(C.c_void_p)) == FM_EINVAL
+    assert raw.fm_spmv(sysT.h, 1, buf.ctypes.data_as(C.c_void_p), None) == FM_EINVAL
+    assert b"fm_spmv" in L.fm_last_error()
+    assert L.fm_set_sell_offsets(12) == FM_EINVAL
     # the system still works: assemble and solve
     sysT.set_xold(np.zeros(sysT._Dim[1]))
     for lev in range(2):
