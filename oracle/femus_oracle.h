@@ -85,6 +85,12 @@ typedef struct {
   double IFr;        /* _IFr                                                */
   double dirg[3];    /* gravity direction                                   */
   double src[3];     /* constant volume source                              */
+  /* Boussinesq body force (no reference counterpart: MGSolverNS.C:344 keeps the
+   * temperature term commented out): RHS of velocity row i of component iv gains
+   * int buoy[iv] T phi_i, T interpolated from the nodal field Tn (top-level
+   * nodes; NULL = none) */
+  double buoy[3];
+  const double *Tn;
 } om_ns_params;
 
 /* Assemble level L into a CSR with the given (sorted) pattern.  x_old_top is

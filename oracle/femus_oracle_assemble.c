@@ -8,7 +8,8 @@
  *   template_appl/nse/SRC/MGSolverNS.C:108-717 (MGSolNS::GenMatRhs, NS_EQUATIONS==1)
  *   src/MGSolverBase.C:336-353 (get_el_sol: fields read from top-level x_old)
  * Additions of this build (no reference counterpart): per-element k / nu
- * scaling, the steady switch time_coeff, a constant volume source.
+ * scaling, the steady switch time_coeff, a constant volume source, the
+ * Boussinesq body force buoy * T from a nodal temperature field.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -238,10 +239,11 @@ int om_ns_element(const om_mesh *m, int L, int iel, int mode, const om_ns_params
   const fe_t *f = m->dim == 2 ? &F2 : &F3;
   int dim = m->dim, n2 = m->n2, n1 = m->n1, N = m->Ntop, NF = n2;
   int nl = dim * n2 + n1;
-  double xx[81], h_eff[3], IJ[9], dphi[81], uold[3][27];
+  double xx[81], h_eff[3], IJ[9], dphi[81], uold[3][27], Tn[27];
   int bc_vol[4 * 27], bc_bd[4 * 27];
   const int *conn = m->conn[L] + iel * n2;
   elem_geom(m, L, iel, xx, h_eff);
+  for (int k = 0; k < n2; k++) Tn[k] = p->Tn ? p->Tn[conn[k]] : 0.;
   for (int iv = 0; iv < dim; iv++)
     for (int k = 0; k < n2; k++) {
       int kd = node_dof_top[conn[k] + iv * N];
@@ -262,10 +264,11 @@ int om_ns_element(const om_mesh *m, int L, int iel, int mode, const om_ns_params
     double JxW = det * f->w[qp];
     const double *phi = f->phi2;
     dphi_global(f, qp, IJ, dphi);
-    double vel_g[3] = {0., 0., 0.};
+    double vel_g[3] = {0., 0., 0.}, T_g = 0.;
     for (int s = 0; s < n2; s++) {
       double ph = phi[s * f->ng + qp];
       for (int d = 0; d < dim; d++) vel_g[d] += ph * uold[d][s];
+      T_g += ph * Tn[s];
     }
     double mod2 = 1.e-20;
     for (int d = 0; d < dim; d++) mod2 += vel_g[d] * vel_g[d];
@@ -282,7 +285,8 @@ int om_ns_element(const om_mesh *m, int L, int iel, int mode, const om_ns_params
         int indx = i + iv * n2;
         double dtxJxW = JxW * bc_vol[i + iv * NF];
         if (mode == 1)
-          FeM[indx] += dtxJxW * (tc * rho * vel_g[iv] * phii / dt + rho * p->IFr * p->dirg[iv] * phii + p->src[iv] * phii);
+          FeM[indx] += dtxJxW * (tc * rho * vel_g[iv] * phii / dt + rho * p->IFr * p->dirg[iv] * phii + p->src[iv] * phii +
+                                 p->buoy[iv] * T_g * phii);
         for (int j = 0; j < n2; j++) {
           double phij = phi[j * f->ng + qp];
           double Lap = 0., Adv = 0.;

@@ -26,9 +26,19 @@ class HeatParams(C.Structure):
 
 
 class NSParams(C.Structure):
+    """om_ns_params; buoy / Tn (Boussinesq body force from a nodal temperature field) default to none.  Keep
+    the array passed as Tn alive while the structure is in use (set_buoyancy stores a reference)."""
     _fields_ = [("IRe", C.c_double), ("dt", C.c_double), ("time_coeff", C.c_double),
                 ("upwind", C.c_double), ("IFr", C.c_double), ("dirg", C.c_double * 3),
-                ("src", C.c_double * 3)]
+                ("src", C.c_double * 3), ("buoy", C.c_double * 3), ("Tn", C.c_void_p)]
+
+    def set_buoyancy(self, T_node, coef):
+        self._Tn = None if T_node is None else np.ascontiguousarray(T_node, np.float64)
+        self.Tn = None if self._Tn is None else self._Tn.ctypes.data
+        c = [0.0, 0.0, 0.0]
+        c[:len(coef)] = coef
+        self.buoy = (C.c_double * 3)(*c)
+        return self
 
 
 class MGOpts(C.Structure):
