@@ -74,23 +74,58 @@ def assembly_rate(F, ctx, s, heat, reps=5):
             "GBs_alg": round(nel * bytes_el / t / 1e9, 1), "bytes_per_element": bytes_el}
 
 
-def pmc_traffic(n, nnz, key):
-    """HBM bytes per top-level SpMV launch from the committed rocprofv3 PMC summary, when it was measured on
-    this matrix and this storage format; otherwise None."""
-    from exfemus_amd import femus as F
+def layout_key(kernel, info):
+    """Key of a SELL layout for the committed PMC summary: the kernel and every count that sets its bytes."""
+    return "|".join([kernel] + [str(info[k]) for k in ("slices", "slots", "slots_c8", "slots_c16", "slots_c32",
+                                                        "nnz")])
+
+
+def pmc_traffic(kernel, info):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE,
+    scaled by the copy kernel's readings), when it was measured on this exact SELL layout; otherwise None."""
     path = os.path.join(ROOT, "profiles", "sell_heat64_pmc_traffic.json")
     try:
         d = json.load(open(path))
     except (OSError, ValueError):
         return None
-    if d.get("n_rows") != n or d.get("nnz") != nnz or d.get("format") != F.SPMV_FORMAT:
-        return None
-    return d.get(key, {}).get("traffic_bytes")
+    return d.get("layouts", {}).get(layout_key(kernel, info), {}).get("traffic_bytes")
 
 
-def spmv_bytes(n, nnz, v):
+def spmv_bytes(n, nnz, v, nvec=2):
+    """Algorithmic bytes of one CSR product with int32 columns: values and columns once, row pointers, and
+    nvec vector streams of n entries (x gathered once + y written once = 2)."""
     p = 8 if nnz > 2 ** 31 else 4
-    return nnz * (v + 4) + (n + 1) * p + n * v + n * v
+    return nnz * (v + 4) + (n + 1) * p + nvec * n * v
+
+
+def sell_bytes(info, n, v, nvec=2):
+    """Algorithmic bytes of one product in the SELL-64 format as stored: every slot's value (padding
+    included), its 8-, 16- or 32-bit column offset, one int32 base per 64 slots, the slice headers (offsets
+    into values and the three offset arrays, width: 36 B) and the slice's 64 row indices, plus nvec vector
+    streams of n entries."""
+    return (info["slots"] * v + info["slots_c8"] + 2 * info["slots_c16"] + 4 * info["slots_c32"] +
+            info["slots"] // 64 * 4 + info["slices"] * (36 + 64 * 4) + nvec * n * v)
+
+
+def roofline_measure(s, top, N, reps):
+    """Top-level products timed with HIP events on the library stream: y = A x (FP64, FP32) and the Chebyshev
+    smoothing step (FP32: the kernel the mixed solve spends most time in, and FP64), with their format and
+    CSR-equivalent byte counts."""
+    info = s.sell_info(0, top)
+    nnz = info["nnz"]
+    out = {}
+    for name, prec, v, nvec in (("y_fp64", 0, 8, 2), ("y_fp32", 1, 4, 2), ("cheb_fp64", 4, 8, 7),
+                                ("cheb_fp32", 5, 4, 7)):
+        sec = s.bench_spmv(top, prec, reps)
+        bf, bc = sell_bytes(info, N, v, nvec), spmv_bytes(N, nnz, v, nvec)
+        out[name] = {"us": round(sec * 1e6, 2), "bytes": bf, "GBs": round(bf / sec / 1e9, 1),
+                     "frac": round(bf / sec / 1e9 / HBM_PEAK_GBS, 4), "csr_equiv_bytes": bc,
+                     "csr_equiv_frac": round(bc / sec / 1e9 / HBM_PEAK_GBS, 4)}
+    return info, out
+
+
+KERNELS = {"y_fp64": "k_sell<double, 0, 8>", "y_fp32": "k_sell<float, 0, 8>", "cheb_fp64": "k_sell<double, 2, 8>",
+           "cheb_fp32": "k_sell<float, 2, 8>"}
 
 
 def cpu_baseline(budget_s=25.0):
@@ -441,6 +476,8 @@ def main():
     ap.add_argument("--dhc-spinup", type=int, default=8)
     ap.add_argument("--dhc-steady-steps", type=int, default=200)
     ap.add_argument("--no-graph", action="store_true", help="launch V-cycles eagerly instead of replaying HIP graphs")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="heat64: only the top-level product timings and their layout keys (for the PMC runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "exfemus_amd runs on exactly one GPU: multi-GPU sharding (subdomain halos "
@@ -486,6 +523,15 @@ def main():
     N = s._Dim[top]
     nel_top = m.level(top)[0]
     log(f"setup {time.perf_counter() - t0:.1f}s  N_top={N}  assembly(all levels)={t_asm * 1e3:.1f} ms")
+    if args.roofline_only:
+        info, rf = roofline_measure(s, top, N, args.spmv_reps)
+        out = {k: dict(r, kernel=KERNELS[k], layout_key=layout_key(KERNELS[k], info)) for k, r in rf.items()}
+        # the 1 GiB copy (k_copy16): the yardstick that scales the PMC byte counters
+        out["copy"] = {"GBs": round(2 * (1 << 30) / ctx.stream_copy(1 << 30, reps=3) / 1e9, 1), "bytes_each_way": 1 << 30}
+        s.close()
+        ctx.close()
+        print(json.dumps(out))
+        return
     prec_mixed = F.FM_PREC_FP32_IR if args.outer == "ir" else F.FM_PREC_FP32_FGMRES
     prec_fp64 = F.FM_PREC_FP64 if args.outer == "ir" else F.FM_PREC_FP64_FGMRES
     kw["use_graph"] = 0 if args.no_graph else 1
@@ -554,22 +600,19 @@ def main():
         log(f"pcie-inclusive: {pcie['value']} MDoF/s ({pcie['ms_per_solve']} ms per solve)")
     except Exception as e:  # the metric above stands without it
         pcie = {"error": str(e)}
-    # SpMV roofline on the top matrix (FP64 and FP32 values)
-    nnz = int(F.lib().fm_system_csr_nnz(s.h, 0, top))
-    sec64 = s.bench_spmv(top, 0, args.spmv_reps)
-    sec32 = s.bench_spmv(top, 1, args.spmv_reps)
-    b64 = spmv_bytes(N, nnz, 8)
-    b32 = spmv_bytes(N, nnz, 4)
-    gbs64 = b64 / sec64 / 1e9
-    gbs32 = b32 / sec32 / 1e9
-    log(f"spmv fp64 {sec64 * 1e6:.1f} us {gbs64:.0f} GB/s ({gbs64 / HBM_PEAK_GBS:.3f}); "
-        f"fp32 {sec32 * 1e6:.1f} us {gbs32:.0f} GB/s ({gbs32 / HBM_PEAK_GBS:.3f})")
+    # roofline of the top-level products: the FP32 Chebyshev step dominates the mixed solve (8 of the 9
+    # top-level products of an iteration); y = A x in both precisions beside it
+    info, rf = roofline_measure(s, top, N, args.spmv_reps)
+    nnz = info["nnz"]
+    for k, r in rf.items():
+        log(f"{k}: {r['us']} us, {r['GBs']} GB/s format bytes ({r['frac']}), CSR-equivalent {r['csr_equiv_frac']}")
     # STREAM copy yardstick: 1 GiB read + 1 GiB written per launch
     copy_bytes = 1 << 30
     sec_copy = ctx.stream_copy(copy_bytes, reps=10)
     gbs_copy = 2 * copy_bytes / sec_copy / 1e9
     log(f"stream copy {gbs_copy:.0f} GB/s")
-    traffic32 = pmc_traffic(N, nnz, "fp32")
+    dom = rf["cheb_fp32"]
+    traffic = pmc_traffic(KERNELS["cheb_fp32"], info)
     out = {
         "metric": "MDoF/s to 1e-8 resid: FP32-MG+FP64 refine vs FP64 MG, 1 GPU; SpMV %HBM peak",
         "value": round(value, 3),
@@ -595,19 +638,18 @@ def main():
         "assembly_ms_all_levels": round(t_asm * 1e3, 2),
         "assembly_top": asm,
         "mg_setup_ms": round(t_mgsetup * 1e3, 2),
-        "spmv": {"fp64_us": round(sec64 * 1e6, 2), "fp64_GBs": round(gbs64, 1), "fp64_frac": round(gbs64 / HBM_PEAK_GBS, 4),
-                 "fp32_us": round(sec32 * 1e6, 2), "fp32_GBs": round(gbs32, 1), "fp32_frac": round(gbs32 / HBM_PEAK_GBS, 4),
-                 "bytes_alg_fp64": b64, "bytes_alg_fp32": b32,
-                 "stream_copy_GBs": round(gbs_copy, 1), "fp64_frac_of_stream": round(gbs64 / gbs_copy, 4),
-                 "storage": s.sell_info(0, top)},
-        # the mixed solve's dominant kernel is the FP32 SELL SpMV of the V-cycle.  achieved / frac count the
-        # algorithmic bytes of a CSR product with int32 columns; the stored offsets are 8 or 16 bits wide, so
-        # the measured HBM traffic per launch (PMC) is lower and its rate is reported beside them
-        "roofline": {"bound": "hbm", "achieved": round(gbs32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs32 / HBM_PEAK_GBS, 4), "traffic": traffic32,
-                     "traffic_GBs": None if traffic32 is None else round(traffic32 / sec32 / 1e9, 1),
-                     "traffic_frac": None if traffic32 is None else round(traffic32 / sec32 / 1e9 / HBM_PEAK_GBS, 4),
-                     "bytes_alg": b32, "kernel": f"k_sell<float> y = A x ({F.SPMV_FORMAT}) on the top matrix"},
+        "spmv": {**{k: r for k, r in rf.items()}, "stream_copy_GBs": round(gbs_copy, 1), "storage": info,
+                 "kernels": KERNELS},
+        # achieved = the bytes the SELL format moves per launch (values, 8/16/32-bit offsets, bases, slice
+        # headers, the seven vector streams of the step) / the launch time; csr_equiv_frac counts a CSR product
+        # with int32 columns instead (what the metric's SpMV definition assumes)
+        "roofline": {"bound": "hbm", "achieved": dom["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": dom["frac"], "traffic": traffic,
+                     "traffic_frac": None if traffic is None else round(traffic / (dom["us"] * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+                     "bytes_per_launch": dom["bytes"], "us_per_launch": dom["us"],
+                     "csr_equiv_frac": dom["csr_equiv_frac"],
+                     "kernel": f"{KERNELS['cheb_fp32']}: Chebyshev step r -= A d, d' = c1 d + c2 D^-1 r, x += d' "
+                               f"({F.SPMV_FORMAT}) on the top matrix"},
     }
     if not args.no_cpu and heat:
         try:
