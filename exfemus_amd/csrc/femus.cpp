@@ -61,8 +61,10 @@ static void ctx_release(fm_ctx *c) {
   delete c;
 }
 
+int fm_noalloc_attach(int device);  // noalloc.hip
 static int ctx_setup(fm_ctx *c) {
   FM_HIP(hipSetDevice(c->device));
+  FM_CHECK(fm_noalloc_attach(c->device));
   FM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   FM_HIP(hipEventCreate(&c->ev0));
   FM_HIP(hipEventCreate(&c->ev1));

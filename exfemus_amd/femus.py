@@ -43,7 +43,7 @@ EXPORTS = [
     "fm_get_adjoint_csr", "fm_heat_dissipation", "fm_bench_stream", "fm_ns_dissipation", "fm_ns_grad_coef",
     "fm_system_set_bcdata", "fm_heat_bc_rhs", "fm_ns_bc_rhs", "fm_ns_pressure_drop",
     "fm_ns_set_buoyancy", "fm_heat_nusselt", "fm_heat_coupling", "fm_adjoint_spmv", "fm_set_sell_split", "fm_set_sell_offsets", "fm_system_sell_info",
-    "fm_system_set_vanka_store", "fm_mg_release", "fm_system_couple", "fm_system_advance",
+    "fm_system_set_vanka_store", "fm_mg_release", "fm_system_couple", "fm_system_advance", "fm_heat_nusselt_noalloc",
 ]
 
 
@@ -150,6 +150,7 @@ def lib():
     L.fm_ns_pressure_drop.argtypes = [vp, f64p, C.POINTER(C.c_double), vp]
     L.fm_ns_set_buoyancy.argtypes = [vp, vp, vp]
     L.fm_heat_nusselt.argtypes = [vp, C.POINTER(HeatParams), C.c_int, f64p, C.POINTER(C.c_double), vp, vp]
+    L.fm_heat_nusselt_noalloc.argtypes = [vp, C.POINTER(HeatParams), C.c_int, f64p, C.POINTER(C.c_double), vp, vp]
     L.fm_heat_coupling.argtypes = [vp, C.POINTER(HeatParams), C.c_int, C.c_double, vp, vp, vp, vp]
     L.fm_adjoint_spmv.argtypes = [vp, C.c_int, f64p, f64p]
     _LIB = L

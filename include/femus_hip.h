@@ -265,6 +265,14 @@ int fm_ns_set_buoyancy(fm_system *s, const double *T_node, const double coef[3])
  * fm_system_set_velocity), dJ/dT and the partials dJ/dk_e               */
 int fm_heat_nusselt(fm_system *s, const fm_heat_params *p, int axis, const double *T_host, double *J,
                     double *dJdT_host, double *dJdk_host);
+/* fm_heat_nusselt without allocating: the same functional, gradient and partials, with the per-element
+ * scratch (contributions and conductivity partials) in a fixed device array of the library (16 MB per
+ * device, resolved once per device at fm_ctx_create and held until the process ends) instead of device
+ * memory allocated per call.  No counterpart in the reference: for callers that cannot allocate device
+ * memory (HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  At most 1,048,576
+ * top-level elements (FM_EINVAL beyond); calls that overlap on one device race on the shared array.     */
+int fm_heat_nusselt_noalloc(fm_system *s, const fm_heat_params *p, int axis, const double *T_host, double *J,
+                            double *dJdT_host, double *dJdk_host);
 /* Coupling terms of the Boussinesq system (heat system s with parameters p, top level; node-space outputs of
  * Ntop entries, dim * Ntop for mode 1):
  *   mode 0: out_j = scale sum_e int phi_j T                 (dNu/du_axis with scale = 1/kappa; T_host needed)
