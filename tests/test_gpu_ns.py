@@ -250,3 +250,43 @@ def test_ns_level_operators_vs_csr(femus, oracle, ctx, case, dim, nx, nlev):
             pr, pc, pv = ns.csr(2, L)
             P = sp.csr_matrix((pv, pc, pr), shape=(n, ns._Dim[L - 1]))
             np.testing.assert_allclose(ns.get_res(L), P @ xc, rtol=1e-12, atol=1e-12 * np.abs(xc).max())
+
+
+def test_ns_channel_re400_picard_steps_match_oracle(femus, oracle, ctx):
+    """The channel at C3's Reynolds number (IRe = 1/400, the reference's upwinding 0.1), 8 x 4 x 4 top: the
+    assembled Oseen system of every level against the oracle (1e-12), and three MGTimeStep pseudo-time steps
+    (mixed-precision multigrid with the coloured Vanka smoother) against the oracle's steps with direct solves."""
+    IRe = 1.0 / 400.0
+    dim, nx, nlev = 3, [4, 2, 2], 2
+    pm, om, ns, bc, x0 = _setup(femus, oracle, ctx, "channel3d", dim, nx, nlev, IRe=IRe)
+    top = nlev - 1
+    dofs, b0, b1 = _oracle_maps(oracle, om, dim, nlev, bc)
+    p = oracle.NSParams(IRe, 0.1, 1.0, 0.1, 0.0, (0, 0, 0), (0, 0, 0))
+    rng = np.random.default_rng(13)
+    xr = x0 + 0.3 * rng.standard_normal(len(x0)) * (b0 == 1)
+    ns.set_xold(xr)
+    for L in range(nlev):
+        mode = 1 if L == top else 0
+        ns.GenMatRhs(0.0, L, mode)
+        rp, ci = oracle.system_pattern(om, L, dim, 1, dofs[L])
+        ov, orhs = oracle.ns_genmatrhs(om, L, mode, p, dofs[L], dofs[top], b0, b1, xr, rp, ci)
+        np.testing.assert_allclose(ns.csr(0, L)[2], ov, rtol=1e-12, atol=1e-13 * np.abs(ov).max())
+        if mode == 1:
+            np.testing.assert_allclose(ns.get_rhs(L), orhs, rtol=1e-12, atol=1e-13 * np.abs(orhs).max())
+    o = ns.opts(precision=femus.FM_PREC_FP32_FGMRES, smoother=femus.FM_SMOOTH_VANKA, vanka_colored=1, nu_pre=3,
+                nu_post=3, coarse_mult=1, tol=1e-11, max_iter=100)
+    rp, ci = oracle.system_pattern(om, top, dim, 1, dofs[top])
+    n = len(rp) - 1
+    xo = x0.copy()
+    for step in range(3):
+        ns.set_x(xo)
+        ns.set_xold(xo)
+        rep = ns.MGTimeStep(0.1 * step, opts=o)
+        assert rep.converged == 1, (step, rep.iters, rep.rel_resid)
+        xg = ns.get_x()
+        v, rhs = oracle.ns_genmatrhs(om, top, 1, p, dofs[top], dofs[top], b0, b1, xo, rp, ci)
+        xo_new = spla.spsolve(sp.csr_matrix((v, ci, rp), shape=(n, n)).tocsc(), rhs)
+        ev, ep = _compare(ns, xg, xo_new, dim, False)
+        assert ev < 1e-9 and ep < 1e-8, (step, ev, ep)
+        xo = xo_new
+    ns.close()
