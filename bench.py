@@ -128,17 +128,30 @@ KERNELS = {"y_fp64": "k_sell<double, 0, 8>", "y_fp32": "k_sell<float, 0, 8>", "c
            "cheb_fp32": "k_sell<float, 2, 8>"}
 
 
-def cpu_baseline(budget_s=25.0):
-    """Oracle (restated reference CPU path) on a bounded heat sample, 1 core (coarse k: arithmetic mean)."""
+# Cycles the oracle's V-cycle (GMRES(30)+ILU(0) 8/8 smoothing, the reference path) needs on the benchmark's
+# hierarchy to reach ||b - A x|| / ||b|| <= 1e-8, from one full run of the same solve (deterministic: the same
+# on any host; 9.13e-9 after cycle 40, 13.5 s per cycle on a 2.7 GHz Xeon core); the bench times a bounded
+# number of its cycles and scales.
+CPU_CYCLES_TO_TOL = 40
+
+
+def cpu_baseline(cycles=1):
+    """Oracle (restated reference CPU path) on the benchmark's own workload (3D heat hex27 64^3, 5 levels, the
+    same log-normal k and geometric-mean coarse coefficients), one core: GMRES(30)+ILU(0) 8/8 smoothing (x2 below
+    the top), 60-iteration GMRES coarse solve, the reference's 0.95 conditional correction.  The assembly, the
+    transfers and the ILU factors (the smoother's set-up, kept by LASPACK across smoother calls, as the GPU's
+    multigrid set-up is kept across solves) are built first and not timed; `cycles` V-cycles from x = 0 are
+    timed, and the throughput to 1e-8 uses the cycle count of a full run."""
     from oracle import oracle as O
     from exfemus_amd import configs
-    nx0, nlev = [4, 4, 4], 4
+    nx0, nlev = [4, 4, 4], 5
+    t0 = time.perf_counter()
     m = O.Mesh(3, nx0, [0, 0, 0], [1, 1, 1], nlev)
     top = nlev - 1
     dofs = [O.init_dof(m, L, 1, 0)[0] for L in range(nlev)]
     b0, b1 = O.genbc(m, 1, 0, dofs[top], configs.heat_dirichlet_all)
     k_top = configs.lognormal_k(m.level(top)[0])
-    ks = configs.coarsen_coef(k_top, nlev, 8)
+    ks = configs.coarsen_coef(k_top, nlev, 8, mean="geometric")
     p = O.HeatParams(1.0, 1.0, 0.1, 0.0, 1.0, 1.0, (0.0, 0.0, 0.0))
     H = O.Hier(nlev)
     xold = np.zeros(len(b0))
@@ -155,17 +168,48 @@ def cpu_baseline(budget_s=25.0):
         if L > 0:
             q = O.system_prol(m, L, 1, 0, dofs[L], dofs[L - 1])
             H.set_P(L, q[0], q[1], q[2], len(dofs[L - 1][dofs[L - 1] >= 0]))
+        del rp, ci, v
+        log(f"cpu leg: level {L} assembled ({time.perf_counter() - t0:.0f} s)")
     n = len(b)
-    # warm the ILU factors (LASPACK caches them across smoother calls)
-    H.solve(b, smoother=0, coarse=0, correction=0, nu_pre=8, nu_post=8, max_cycles=1, eps=1e-30, ref_exit=0)
+    t_setup = time.perf_counter() - t0
+    # the ILU factors of every level are formed by the first smoother call
     t0 = time.perf_counter()
-    x, hist = H.solve(b, smoother=0, coarse=0, correction=0, nu_pre=8, nu_post=8, max_cycles=40, eps=1e-8,
+    H.solve(b, smoother=0, coarse=0, correction=0, nu_pre=8, nu_post=8, max_cycles=1, eps=1e-30, ref_exit=0)
+    t_ilu = time.perf_counter() - t0
+    log(f"cpu leg: ILU factors and a first cycle {t_ilu:.0f} s")
+    t0 = time.perf_counter()
+    x, hist = H.solve(b, smoother=0, coarse=0, correction=0, nu_pre=8, nu_post=8, max_cycles=cycles, eps=1e-8,
                       ref_exit=0)
     dt = time.perf_counter() - t0
-    return {"value": round(n / dt / 1e6, 6), "unit": "MDoF/s", "cores": 1, "kind": "port",
-            "sample": f"heat hex27 {nx0[0] * 8}^3 top ({n} DoF), {nlev} levels, GMRES(30)+ILU(0) 8/8 sweeps "
-                      f"(x2 below top), 60-it GMRES coarse, 0.95 conditional correction; {len(hist)} cycles to "
-                      f"{hist[-1]:.2e} in {dt:.2f} s"}
+    t_cycle = dt / len(hist)
+    ncyc = len(hist) if hist[-1] <= 1e-8 else CPU_CYCLES_TO_TOL
+    import platform
+    model = platform.processor() or platform.machine()
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": None if ncyc is None else round(n / (t_cycle * ncyc) / 1e6, 6), "unit": "MDoF/s", "cores": 1,
+            "kind": "port", "s_per_cycle": round(t_cycle, 3), "cycles_timed": len(hist),
+            "rel_resid_after": [float(h) for h in hist], "cycles_to_1e-8": ncyc,
+            "cycles_to_1e-8_source": "this run" if hist[-1] <= 1e-8 else "a full run of the same solve",
+            "untimed_setup_s": round(t_setup, 1), "untimed_ilu_s": round(t_ilu, 1), "cpu": model,
+            "sample": f"the metric's workload itself: heat hex27 64^3 top ({n} DoF), 5 levels, oracle V-cycle with "
+                      f"GMRES(30)+ILU(0) 8/8 sweeps (x2 below top), 60-it GMRES coarse, 0.95 conditional correction; "
+                      f"{len(hist)} timed cycle(s) of {t_cycle:.2f} s from x = 0, throughput = N / (s per cycle x "
+                      f"cycles to 1e-8)"}
+
+
+def _cpu_baseline_child(cycles):
+    """The CPU leg in a child process started before the GPU is touched (one host core, no thread pools), so
+    that it runs while the GPU legs are timed."""
+    import subprocess
+    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1")
+    return subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-baseline-only", "--cpu-cycles",
+                             str(cycles)], stdout=subprocess.PIPE, stderr=None, text=True, env=env)
 
 
 def build_ns(F, ctx, workload):
@@ -476,13 +520,21 @@ def main():
     ap.add_argument("--dhc-spinup", type=int, default=8)
     ap.add_argument("--dhc-steady-steps", type=int, default=200)
     ap.add_argument("--no-graph", action="store_true", help="launch V-cycles eagerly instead of replaying HIP graphs")
+    ap.add_argument("--cpu-baseline-only", action="store_true", help="only the CPU leg (run as a child process)")
+    ap.add_argument("--cpu-cycles", type=int, default=1, help="V-cycles the CPU leg times")
     ap.add_argument("--roofline-only", action="store_true",
                     help="heat64: only the top-level product timings and their layout keys (for the PMC runs)")
     args = ap.parse_args()
+    if args.cpu_baseline_only:
+        print(json.dumps(cpu_baseline(args.cpu_cycles)))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "exfemus_amd runs on exactly one GPU: multi-GPU sharding (subdomain halos "
                                    "over RCCL) is not implemented"}))
         sys.exit(2)
+    cpu_child = None
+    if not args.no_cpu and args.workload == "heat64" and not args.roofline_only and args.nx0 == 4 and args.nlev == 5:
+        cpu_child = _cpu_baseline_child(args.cpu_cycles)
     from exfemus_amd import femus as F
     ctx = F.Context(0)
     if args.workload in ("cavity-adjoint", "dhc"):
@@ -651,10 +703,14 @@ def main():
                      "kernel": f"{KERNELS['cheb_fp32']}: Chebyshev step r -= A d, d' = c1 d + c2 D^-1 r, x += d' "
                                f"({F.SPMV_FORMAT}) on the top matrix"},
     }
-    if not args.no_cpu and heat:
+    if cpu_child is not None:
+        log("waiting for the CPU leg")
         try:
-            out["cpu_baseline"] = cpu_baseline()
+            so, _ = cpu_child.communicate(timeout=900)
+            lines = [l for l in so.splitlines() if l.startswith("{")]
+            out["cpu_baseline"] = json.loads(lines[-1]) if lines else {"error": f"CPU leg exited {cpu_child.returncode}"}
         except Exception as e:  # the oracle may be unbuilt on a bare box
+            cpu_child.kill()
             out["cpu_baseline"] = {"error": str(e)}
     s.close()
     ctx.close()

@@ -111,6 +111,21 @@ extern "C" int fm_mesh_create_box(int dim, const int nx0[3], const double lo[3],
   *out = m;
   return FM_OK;
 }
+extern "C" int fm_mesh_create(int dim, int nlev, int n_nodes, const double *xyz, const int *off_el, const int *el_map,
+                              const int *el_neighbor, const int *nodes_q2, const int *nodes_q1, const int *parent,
+                              const int *child, fm_mesh **out) {
+  if (!out) return fm_einval(__func__);
+  fm_mesh *m = new fm_mesh();
+  std::string err;
+  if (fm_mesh_from_arrays(dim, nlev, n_nodes, xyz, off_el, el_map, el_neighbor, nodes_q2, nodes_q1, parent, child, m,
+                          err)) {
+    delete m;
+    fm_set_error(err);
+    return FM_EINVAL;
+  }
+  *out = m;
+  return FM_OK;
+}
 extern "C" int fm_mesh_destroy(fm_mesh *m) { delete m; return FM_OK; }
 extern "C" int fm_mesh_info(const fm_mesh *m, int *info) {
   if (!m || !info) return fm_einval(__func__);
@@ -212,8 +227,65 @@ extern "C" int fm_system_create(fm_ctx *ctx, const fm_mesh *m, int nq2, int nq1,
   int64_t nt = s->Dim[m->nlev - 1];
   s->bc0.assign(nt, 1); s->bc1.assign(nt, 1);
   s->lev.resize(m->nlev);
+  s->u_pat.resize(m->nlev); s->u_rst.resize(m->nlev); s->u_prl.resize(m->nlev);
   ctx->nsys++;
   *out = s;
+  return FM_OK;
+}
+
+// A caller's CSR (reference numbering) checked and stored with sorted columns; vals may be NULL (pattern only)
+static int user_csr(const char *fn, int64_t nr, int64_t nc, const int64_t *rp, const int *ci, const double *v,
+                    UserCsr &u) {
+  if (!rp || !ci || rp[0] != 0) return fm_einval(fn);
+  for (int64_t i = 0; i < nr; i++)
+    if (rp[i + 1] < rp[i]) return fm_einval(fn);
+  const int64_t nnz = rp[nr];
+  u.rp.assign(rp, rp + nr + 1);
+  u.ci.resize(nnz);
+  u.v.assign(v ? nnz : 0, 0.);
+  std::vector<std::pair<int, double>> row;
+  for (int64_t i = 0; i < nr; i++) {
+    row.clear();
+    for (int64_t q = rp[i]; q < rp[i + 1]; q++) {
+      if (ci[q] < 0 || ci[q] >= nc) { fm_set_error(std::string(fn) + ": column out of range in row " + std::to_string(i)); return FM_EINVAL; }
+      row.emplace_back(ci[q], v ? v[q] : 0.);
+    }
+    std::sort(row.begin(), row.end(), [](const std::pair<int, double> &a, const std::pair<int, double> &b) { return a.first < b.first; });
+    for (size_t k = 0; k < row.size(); k++) {
+      if (k > 0 && row[k].first == row[k - 1].first) { fm_set_error(std::string(fn) + ": repeated column in row " + std::to_string(i)); return FM_EINVAL; }
+      u.ci[rp[i] + k] = row[k].first;
+      if (v) u.v[rp[i] + k] = row[k].second;
+    }
+  }
+  u.nc = nc;
+  u.set = true;
+  return 0;
+}
+
+// MGSolDA::ReadMatrix (src/MGSolverDA.C:1785-2004): the pattern of A on level L from the caller (rows and columns in
+// the reference DoF numbering of level L).  It must hold every coupling of the level's elements (checked at
+// fm_system_init); extra entries stay zero.  NULL rowptr drops it (the pattern is built from the mesh again).
+extern "C" int fm_system_set_pattern(fm_system *s, int L, const int64_t *rowptr, const int *colind) {
+  if (!s || L < 0 || L >= (int)s->Dim.size()) return fm_einval(__func__);
+  if (!rowptr) { s->u_pat[L] = UserCsr(); s->initialized = false; return FM_OK; }
+  FM_CHECK(user_csr(__func__, s->Dim[L], s->Dim[L], rowptr, colind, nullptr, s->u_pat[L]));
+  s->initialized = false;
+  return FM_OK;
+}
+
+// MGSolDA::ReadRest / ReadProl (src/MGSolverDA.C:2236-2444, 2011-2230): which = 1: Rst[L] (coarse L <- fine L+1;
+// fm_system_init multiplies its rows by the Dirichlet mask bc0 % 2 of the row's top-level DoF, as ReadRest does),
+// which = 2: Prl[L] (fine L <- coarse L-1); reference numbering of both levels.  NULL rowptr drops it.
+extern "C" int fm_system_set_transfer(fm_system *s, int which, int L, const int64_t *rowptr, const int *colind,
+                                      const double *val) {
+  if (!s || L < 0 || L >= (int)s->Dim.size() || (which != 1 && which != 2)) return fm_einval(__func__);
+  const int top = (int)s->Dim.size() - 1;
+  if ((which == 1 && L >= top) || (which == 2 && L == 0)) return fm_einval(__func__);
+  UserCsr &u = which == 1 ? s->u_rst[L] : s->u_prl[L];
+  if (!rowptr) { u = UserCsr(); s->initialized = false; return FM_OK; }
+  if (!val) return fm_einval(__func__);
+  FM_CHECK(user_csr(__func__, s->Dim[L], s->Dim[which == 1 ? L + 1 : L - 1], rowptr, colind, val, u));
+  s->initialized = false;
   return FM_OK;
 }
 
@@ -475,6 +547,41 @@ static int colour_slices(hipStream_t st, Level &l) {
   return 0;
 }
 
+// every coupling of the level's elements must be an entry of the caller's pattern (the assembly writes through
+// the element -> entry map and would drop what the pattern lacks)
+static int check_user_pattern(const fm_system *s, int L) {
+  const fm_mesh &m = *s->mesh;
+  const UserCsr &u = s->u_pat[L];
+  std::vector<int> ed(s->nloc);
+  for (int e = 0; e < m.nel[L]; e++) {
+    fm_elem_dofs(m, L, e, s->nq2, s->nq1, s->node_dof[L], ed.data());
+    for (int a = 0; a < s->nloc; a++)
+      for (int b = 0; b < s->nloc; b++) {
+        const int *c0 = u.ci.data() + u.rp[ed[a]], *c1 = u.ci.data() + u.rp[ed[a] + 1];
+        if (!std::binary_search(c0, c1, ed[b])) {
+          fm_set_error("fm_system_init: the caller's pattern of level " + std::to_string(L) + " lacks entry (" +
+                       std::to_string(ed[a]) + ", " + std::to_string(ed[b]) + ") of element " + std::to_string(e));
+          return FM_EINVAL;
+        }
+      }
+  }
+  return 0;
+}
+
+// ReadRest's row mask (src/MGSolverDA.C:2422-2424): rows of a coarse DoF whose top-level DoF is Dirichlet vanish
+static void mask_user_rest(const fm_system *s, int L, const std::vector<int64_t> &rp, std::vector<double> &v) {
+  const fm_mesh &m = *s->mesh;
+  const int N = m.Ntop, top = m.nlev - 1, nv = s->nq2 + s->nq1;
+  for (int iv = 0; iv < nv; iv++)
+    for (int k = 0; k < N; k++) {
+      const int row = s->node_dof[L][k + (size_t)iv * N];
+      if (row < 0) continue;
+      const int td = s->node_dof[top][k + (size_t)iv * N];
+      if (td >= 0 && s->bc0[td] % 2 == 0)
+        for (int64_t q = rp[row]; q < rp[row + 1]; q++) v[q] = 0.;
+    }
+}
+
 extern "C" int fm_system_init(fm_system *s) {
   if (!s) return fm_einval(__func__);
   fm_ctx *c = s->ctx;
@@ -520,21 +627,36 @@ extern "C" int fm_system_init(fm_system *s) {
   }
   for (int L = 0; L < nlev; L++) {
     Level &l = s->lev[L];
-    // pattern
+    // pattern: the caller's (ReadMatrix) or the element couplings of the mesh
     std::vector<int64_t> rp;
     std::vector<int> ci;
-    fm_pattern(m, L, s->nq2, s->nq1, s->node_dof[L], rp, ci);
+    if (s->u_pat[L].set) {
+      FM_CHECK(check_user_pattern(s, L));
+      rp = s->u_pat[L].rp;
+      ci = s->u_pat[L].ci;
+    } else {
+      fm_pattern(m, L, s->nq2, s->nq1, s->node_dof[L], rp, ci);
+    }
     FM_CHECK(upload_csr(st, l.A, l.n, l.n, rp, ci, nullptr, l.h_ord, l.h_rank, &l.h_goff));
     FM_CHECK(colour_slices(st, l));
-    // transfers
+    // transfers: the caller's (ReadRest with the Dirichlet row mask, ReadProl) or built from the refinement
     if (L < top) {
       std::vector<int64_t> trp; std::vector<int> tci; std::vector<double> tv;
-      fm_transfer(m, L, s->nq2, s->nq1, true, s->node_dof[L], s->node_dof[L + 1], &s->node_dof[top], &s->bc0, trp, tci, tv);
+      if (s->u_rst[L].set) {
+        trp = s->u_rst[L].rp; tci = s->u_rst[L].ci; tv = s->u_rst[L].v;
+        mask_user_rest(s, L, trp, tv);
+      } else {
+        fm_transfer(m, L, s->nq2, s->nq1, true, s->node_dof[L], s->node_dof[L + 1], &s->node_dof[top], &s->bc0, trp, tci, tv);
+      }
       FM_CHECK(upload_csr(st, l.R, l.n, s->Dim[L + 1], trp, tci, &tv, l.h_ord, s->lev[L + 1].h_rank));
     }
     if (L > 0) {
       std::vector<int64_t> trp; std::vector<int> tci; std::vector<double> tv;
-      fm_transfer(m, L, s->nq2, s->nq1, false, s->node_dof[L], s->node_dof[L - 1], nullptr, nullptr, trp, tci, tv);
+      if (s->u_prl[L].set) {
+        trp = s->u_prl[L].rp; tci = s->u_prl[L].ci; tv = s->u_prl[L].v;
+      } else {
+        fm_transfer(m, L, s->nq2, s->nq1, false, s->node_dof[L], s->node_dof[L - 1], nullptr, nullptr, trp, tci, tv);
+      }
       FM_CHECK(upload_csr(st, l.P, l.n, s->Dim[L - 1], trp, tci, &tv, l.h_ord, s->lev[L - 1].h_rank));
     }
     // vectors
@@ -551,7 +673,7 @@ extern "C" int fm_system_init(fm_system *s) {
       for (int sd = 0; sd < m.nsides; sd++) if (m.neigh[L][(size_t)e * m.nsides + sd] == -1) bm[e] |= (unsigned char)(1 << sd);
       col[e] = fm_elem_colour(m, L, e);
     }
-    l.ncolor = 1 << m.dim;
+    l.ncolor = fm_ncolours(m, L);
     l.color_off.assign(l.ncolor + 1, 0);
     for (int e = 0; e < l.nel; e++) l.color_off[col[e] + 1]++;
     for (int k = 0; k < l.ncolor; k++) l.color_off[k + 1] += l.color_off[k];
@@ -671,6 +793,50 @@ extern "C" int fm_system_set_rhs(fm_system *s, int L, const double *b) {
   FM_CHECK(check_level(s, L));
   if (!b) return fm_einval(__func__);
   return vec_from_host(s, L, b, s->lev[L].b);
+}
+// Device-pointer variants: the caller's vector (reference numbering) lives in device memory of the context's
+// device and is gathered to / from the level's internal order on the device, with no host copy
+static int check_dev_ptr(const char *fn, const void *p) {
+  if (!p) return fm_einval(fn);
+  hipPointerAttribute_t at;
+  if (hipPointerGetAttributes(&at, p) != hipSuccess || (at.type != hipMemoryTypeDevice && at.type != hipMemoryTypeManaged)) {
+    (void)hipGetLastError();
+    fm_set_error(std::string(fn) + ": not a device pointer");
+    return FM_EINVAL;
+  }
+  return 0;
+}
+static int vec_to_dev(const fm_system *s, int L, const double *v, double *dev) {
+  FM_HIP(hipSetDevice(s->ctx->device));
+  FM_CHECK(fmk::gather(s->ctx->stream, s->lev[L].n, s->lev[L].rank, v, dev));
+  FM_HIP(hipStreamSynchronize(s->ctx->stream));
+  return 0;
+}
+static int vec_from_dev(fm_system *s, int L, const double *dev, double *v) {
+  FM_HIP(hipSetDevice(s->ctx->device));
+  FM_CHECK(fmk::gather(s->ctx->stream, s->lev[L].n, s->lev[L].ord, dev, v));
+  FM_HIP(hipStreamSynchronize(s->ctx->stream));
+  return 0;
+}
+extern "C" int fm_system_get_x_dev(const fm_system *s, int L, double *x_dev) {
+  FM_CHECK(check_level(s, L));
+  FM_CHECK(check_dev_ptr(__func__, x_dev));
+  return vec_to_dev(s, L, s->lev[L].x, x_dev);
+}
+extern "C" int fm_system_set_x_dev(fm_system *s, int L, const double *x_dev) {
+  FM_CHECK(check_level(s, L));
+  FM_CHECK(check_dev_ptr(__func__, x_dev));
+  return vec_from_dev(s, L, x_dev, s->lev[L].x);
+}
+extern "C" int fm_system_get_rhs_dev(const fm_system *s, int L, double *b_dev) {
+  FM_CHECK(check_level(s, L));
+  FM_CHECK(check_dev_ptr(__func__, b_dev));
+  return vec_to_dev(s, L, s->lev[L].b, b_dev);
+}
+extern "C" int fm_system_set_rhs_dev(fm_system *s, int L, const double *b_dev) {
+  FM_CHECK(check_level(s, L));
+  FM_CHECK(check_dev_ptr(__func__, b_dev));
+  return vec_from_dev(s, L, b_dev, s->lev[L].b);
 }
 extern "C" int fm_get_res(const fm_system *s, int L, double *r) {
   FM_CHECK(check_level(s, L));

@@ -148,6 +148,16 @@ struct GraphEntry {
   hipGraphExec_t exec;
 };
 
+// An operator handed in by the caller in the reference numbering (ReadMatrix / ReadRest / ReadProl): CSR with
+// sorted columns; values only for the transfers
+struct UserCsr {
+  bool set = false;
+  int64_t nc = 0;
+  std::vector<int64_t> rp;
+  std::vector<int> ci;
+  std::vector<double> v;
+};
+
 struct FeTab;
 struct fm_system {
   fm_ctx *ctx = nullptr;
@@ -156,6 +166,7 @@ struct fm_system {
   std::vector<std::vector<int>> node_dof;   // per level, (nq2+nq1)*Ntop
   std::vector<int64_t> Dim;
   std::vector<int> bc0, bc1;                // top-level DoFs
+  std::vector<UserCsr> u_pat, u_rst, u_prl;  // per level: caller's pattern of A, restriction, prolongation
   int *d_bc0 = nullptr, *d_bc1 = nullptr;   // device copies (top DoFs)
   int *d_dof_top = nullptr;                 // node_dof[top] on device
   double *xyz = nullptr;                    // device coords

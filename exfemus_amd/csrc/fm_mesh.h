@@ -14,6 +14,12 @@ struct fm_mesh {
   std::vector<int> nq2, nq1, nel;
   std::vector<double> xyz;                    // xyz[node + d*Ntop]
   std::vector<std::vector<int>> conn, neigh, parent, egrid;  // egrid: element grid index (3 ints)
+  // lattice: built by fm_build_box (egrid, nx0, lo, hi describe the nodes); false for a hierarchy handed in by a
+  // caller (fm_mesh_from_arrays), whose element colours and child maps are stored instead
+  bool lattice = true;
+  std::vector<std::vector<unsigned char>> ecol;  // per level: element colour (caller meshes)
+  std::vector<std::vector<int>> kids;            // per level L >= 1: kids[L][p * nch + c] = child c of parent p
+  std::vector<std::vector<unsigned char>> chid;  // per level L >= 1: index of each element among its parent's children
 };
 
 // gencase.cpp
@@ -34,9 +40,28 @@ extern const int fm_face3d[6][9];
 // (2^dim colours).  Two elements of one colour are at least two elements apart along some axis, so they
 // share no node.
 inline int fm_elem_colour(const fm_mesh &m, int L, int e) {
+  if (!m.lattice) return m.ecol[L][e];
   const int *g = &m.egrid[L][3 * (size_t)e];
   return (g[0] & 1) + 2 * (g[1] & 1) + (m.dim == 3 ? 4 * (g[2] & 1) : 0);
 }
+// number of element colours of level L
+inline int fm_ncolours(const fm_mesh &m, int L) {
+  if (m.lattice) return 1 << m.dim;
+  int c = 0;
+  for (unsigned char v : m.ecol[L]) c = v + 1 > c ? v + 1 : c;
+  return c;
+}
+
+// A caller's hierarchy in MGMesh's layout (src/MGMesh.C:493-899): nlev levels, Ntop nodes whose coordinates are
+// xyz[node + d Ntop]; level L holds the elements off_el[L] .. off_el[L+1]-1 of el_map (n2 top-level node ids per
+// element in libMesh Quad9 / Hex27 local order) and el_neigh (nsides level-local neighbours, -1 on the boundary);
+// nq2[L] / nq1[L] Q2 / Q1 nodes of level L (the first nq2[L] / nq1[L] top-level ids); parent / child the level
+// L-1 parent of every element of L >= 1 and its index among the parent's 2^dim children (embedding order of
+// MGGeomEl), both NULL for one level.  Validates, colours the elements (no two of one colour share a node) and
+// builds the child maps.  Returns 0, or -1 with err set.
+int fm_mesh_from_arrays(int dim, int nlev, int Ntop, const double *xyz, const int *off_el, const int *el_map,
+                        const int *el_neigh, const int *nq2, const int *nq1, const int *parent, const int *child,
+                        fm_mesh *m, std::string &err);
 
 // order_host.cpp
 // Colour signature of every node of level L: bit c set when an element of colour c contains the node

@@ -61,17 +61,23 @@ int fm_build_box(int dim, const int nx0[3], const double lo[3], const double hi[
           m->egrid[0][3 * e] = i; m->egrid[0][3 * e + 1] = j; m->egrid[0][3 * e + 2] = k;
         }
   }
+  m->kids.assign(nlev, {});
+  m->chid.assign(nlev, {});
   for (int L = 1; L < nlev; L++) {
     int np = m->nel[L - 1], nc = np * m->nch;
     m->nel[L] = nc;
     m->egrid[L].resize(3 * (size_t)nc);
     m->parent[L].resize(nc);
+    m->kids[L].resize(nc);
+    m->chid[L].resize(nc);
     for (int p = 0; p < np; p++)
       for (int c = 0; c < m->nch; c++) {
         int ce = p * m->nch + c;
         for (int d = 0; d < 3; d++)
           m->egrid[L][3 * ce + d] = d < dim ? 2 * m->egrid[L - 1][3 * p + d] + ((c >> d) & 1) : 0;
         m->parent[L][ce] = p;
+        m->kids[L][ce] = ce;
+        m->chid[L][ce] = (unsigned char)c;
       }
   }
   // node of element e (level L), local k -> top-grid linear index
@@ -286,7 +292,7 @@ void fm_transfer(const fm_mesh &m, int L, int nq2, int nq1, bool rest, const std
       for (int e = 0; e < m.nel[L]; e++)
         for (int kk = 0; kk < ns; kk++) { int r = m.conn[L][(size_t)e * m.n2 + kk]; last[r] = e; lastk[r] = kk; }
       for (int r = 0; r < nr; r++) {
-        int e = last[r], kk = lastk[r], par = m.parent[L][e], ch = e - par * nch;
+        int e = last[r], kk = lastk[r], par = m.parent[L][e], ch = m.chid[L][e];
         for (int kj = 0; kj < ns; kj++) {
           double val = E[((size_t)ch * ns + kk) * ns + kj];
           if (std::fabs(val) > 1.e-8) blk[t][r].push_back({m.conn[L - 1][(size_t)par * m.n2 + kj], val});
@@ -310,7 +316,7 @@ void fm_transfer(const fm_mesh &m, int L, int nq2, int nq1, bool rest, const std
             for (int k = 0; k < ns; k++) if (m.conn[L][(size_t)e * m.n2 + k] == r) kk = k;
             if (kk < 0) continue;  // r is not among this element's first ns nodes (Q1 block)
             for (int ich = 0; ich < nch; ich++) {
-              int ch = e * nch + ich;
+              int ch = m.kids[L + 1][(size_t)e * nch + ich];
               for (int kj = 0; kj < ns; kj++) {
                 double val = E[((size_t)ich * ns + kj) * ns + kk];
                 if (std::fabs(val) <= 1.e-6) continue;
@@ -345,4 +351,128 @@ void fm_transfer(const fm_mesh &m, int L, int nq2, int nq1, bool rest, const std
       for (auto &pr : blk[t][i]) { ci[q] = dof_col[pr.first + (size_t)iv * N]; v[q] = mask * pr.second; q++; }
     }
   }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// A hierarchy handed in by the caller in MGMesh's layout (src/MGMesh.C:493-899, include/MGMesh.h:65-80)
+// ---------------------------------------------------------------------------------------------------------------
+// Greedy colouring of level L's elements (in element order, the smallest colour no earlier element sharing a node
+// holds); returns the number of colours
+static int greedy_colours(const fm_mesh &m, int L, std::vector<unsigned char> &col) {
+  const int ne = m.nel[L], n2 = m.n2;
+  std::vector<int64_t> off;
+  std::vector<int> els;
+  node_elems(m, L, m.nq2[L], off, els);
+  col.assign(ne, 0);
+  int nc = 0;
+  for (int e = 0; e < ne; e++) {
+    unsigned used = 0;
+    for (int k = 0; k < n2; k++) {
+      const int nd = m.conn[L][(size_t)e * n2 + k];
+      for (int64_t q = off[nd]; q < off[nd + 1]; q++)
+        if (els[q] < e) used |= 1u << col[els[q]];
+    }
+    int c = 0;
+    while (c < 31 && ((used >> c) & 1)) c++;
+    col[e] = (unsigned char)c;
+    nc = std::max(nc, c + 1);
+  }
+  return nc;
+}
+
+// true when no two elements of one colour share a node
+static bool colouring_valid(const fm_mesh &m, int L, const std::vector<unsigned char> &col) {
+  std::vector<int> seen((size_t)m.nq2[L] * 8, -1);
+  for (int e = 0; e < m.nel[L]; e++)
+    for (int k = 0; k < m.n2; k++) {
+      const int nd = m.conn[L][(size_t)e * m.n2 + k];
+      int &s = seen[(size_t)nd * 8 + col[e]];
+      if (s >= 0 && s != e) return false;
+      s = e;
+    }
+  return true;
+}
+
+int fm_mesh_from_arrays(int dim, int nlev, int Ntop, const double *xyz, const int *off_el, const int *el_map,
+                        const int *el_neigh, const int *nq2, const int *nq1, const int *parent, const int *child,
+                        fm_mesh *m, std::string &err) {
+  auto bad = [&](const std::string &w) { err = "fm_mesh_create: " + w; return -1; };
+  if ((dim != 2 && dim != 3) || nlev < 1 || nlev > 12 || Ntop < 1) return bad("dim, levels or node count out of range");
+  if (!xyz || !off_el || !el_map || !el_neigh || !nq2 || !nq1) return bad("missing array");
+  if (nlev > 1 && (!parent || !child)) return bad("a multilevel hierarchy needs parent and child arrays");
+  m->dim = dim; m->nlev = nlev;
+  m->n2 = dim == 2 ? 9 : 27; m->n1 = dim == 2 ? 4 : 8; m->nsides = 2 * dim;
+  m->nb2 = dim == 2 ? 3 : 9; m->nch = 1 << dim;
+  m->Ntop = Ntop;
+  m->lattice = false;
+  if (off_el[0] != 0) return bad("off_el[0] must be 0");
+  m->nel.assign(nlev, 0); m->nq2.assign(nlev, 0); m->nq1.assign(nlev, 0);
+  for (int L = 0; L < nlev; L++) {
+    if (off_el[L + 1] <= off_el[L]) return bad("level " + std::to_string(L) + " has no elements");
+    m->nel[L] = off_el[L + 1] - off_el[L];
+    m->nq2[L] = nq2[L]; m->nq1[L] = nq1[L];
+    if (nq1[L] < 1 || nq1[L] > nq2[L] || nq2[L] > Ntop || (L > 0 && nq2[L] < nq2[L - 1]))
+      return bad("node counts of level " + std::to_string(L) + " inconsistent");
+  }
+  if (nq2[nlev - 1] != Ntop) return bad("the top level must hold every node");
+  m->xyz.assign(xyz, xyz + (size_t)dim * Ntop);
+  for (int d = 0; d < 3; d++) { m->nx0[d] = 1; m->lo[d] = 0.; m->hi[d] = 0.; }
+  for (int d = 0; d < dim; d++) {
+    m->lo[d] = *std::min_element(m->xyz.begin() + (size_t)d * Ntop, m->xyz.begin() + (size_t)(d + 1) * Ntop);
+    m->hi[d] = *std::max_element(m->xyz.begin() + (size_t)d * Ntop, m->xyz.begin() + (size_t)(d + 1) * Ntop);
+  }
+  m->conn.assign(nlev, {}); m->neigh.assign(nlev, {}); m->parent.assign(nlev, {}); m->egrid.assign(nlev, {});
+  m->kids.assign(nlev, {}); m->chid.assign(nlev, {}); m->ecol.assign(nlev, {});
+  for (int L = 0; L < nlev; L++) {
+    const int ne = m->nel[L];
+    const int *cm = el_map + (size_t)off_el[L] * m->n2, *nb = el_neigh + (size_t)off_el[L] * m->nsides;
+    m->conn[L].assign(cm, cm + (size_t)ne * m->n2);
+    m->neigh[L].assign(nb, nb + (size_t)ne * m->nsides);
+    std::vector<char> used2(nq2[L], 0), used1(nq1[L], 0);
+    for (int e = 0; e < ne; e++) {
+      for (int k = 0; k < m->n2; k++) {
+        const int nd = m->conn[L][(size_t)e * m->n2 + k];
+        if (nd < 0 || nd >= nq2[L]) return bad("level " + std::to_string(L) + " element node outside the level's nodes");
+        if (k < m->n1) {
+          if (nd >= nq1[L]) return bad("level " + std::to_string(L) + " element vertex outside the level's Q1 nodes");
+          used1[nd] = 1;
+        }
+        used2[nd] = 1;
+      }
+      for (int sd = 0; sd < m->nsides; sd++) {
+        const int q = m->neigh[L][(size_t)e * m->nsides + sd];
+        if (q < -1 || q >= ne) return bad("neighbour index out of range");
+      }
+    }
+    for (char u : used2) if (!u) return bad("a node of level " + std::to_string(L) + " belongs to no element");
+    for (char u : used1) if (!u) return bad("a Q1 node of level " + std::to_string(L) + " is no element vertex");
+    m->parent[L].assign(ne, -1);
+    if (L > 0) {
+      const int np = m->nel[L - 1];
+      m->kids[L].assign((size_t)np * m->nch, -1);
+      m->chid[L].assign(ne, 0);
+      for (int e = 0; e < ne; e++) {
+        const int p = parent[off_el[L] + e], c = child[off_el[L] + e];
+        if (p < 0 || p >= np || c < 0 || c >= m->nch) return bad("parent or child index out of range");
+        int &slot = m->kids[L][(size_t)p * m->nch + c];
+        if (slot >= 0) return bad("two elements are the same child of one parent");
+        slot = e;
+        m->parent[L][e] = p;
+        m->chid[L][e] = (unsigned char)c;
+      }
+      for (int k : m->kids[L]) if (k < 0) return bad("a parent misses a child");
+    }
+    // colours: the child index (a consistently oriented refinement gives 2^dim colours of non-touching
+    // elements), else greedy
+    bool ok = false;
+    if (L > 0) {
+      m->ecol[L] = m->chid[L];
+      ok = colouring_valid(*m, L, m->ecol[L]);
+    }
+    if (!ok) {
+      const int nc = greedy_colours(*m, L, m->ecol[L]);
+      if (nc > 8) return bad("level " + std::to_string(L) + " needs more than 8 element colours");
+    }
+  }
+  return 0;
 }

@@ -30,6 +30,50 @@ void fm_node_colours(const fm_mesh &m, int L, std::vector<unsigned char> &nmask)
 // made the full product 40% slower on the 96x24x24 channel).  Without nmask
 // the groups are the slabs' parity classes (vertex / edge / face / interior
 // node of the Q2 lattice).
+// Node keys of a level without a lattice (a caller's mesh): a Cuthill-McKee numbering of the node graph (nodes
+// joined when they share an element), started from a node of least degree and visiting neighbours in order of
+// increasing degree; slab = breadth-first level / SLAB, so the groups play the role of the lattice's slabs of
+// node planes.
+static void graph_keys(const fm_mesh &m, int L, std::vector<int64_t> &key, std::vector<int> &slab, int SLAB) {
+  const int nn = m.nq2[L], ne = m.nel[L], n2 = m.n2;
+  std::vector<int64_t> off(nn + 1, 0);
+  for (int e = 0; e < ne; e++)
+    for (int a = 0; a < n2; a++) off[m.conn[L][(size_t)e * n2 + a] + 1]++;
+  for (int k = 0; k < nn; k++) off[k + 1] += off[k];
+  std::vector<int> els(off[nn]);
+  {
+    std::vector<int64_t> f(off.begin(), off.end() - 1);
+    for (int e = 0; e < ne; e++)
+      for (int a = 0; a < n2; a++) els[f[m.conn[L][(size_t)e * n2 + a]]++] = e;
+  }
+  auto neighbours = [&](int k, std::vector<int> &out) {
+    out.clear();
+    for (int64_t q = off[k]; q < off[k + 1]; q++)
+      for (int a = 0; a < n2; a++) out.push_back(m.conn[L][(size_t)els[q] * n2 + a]);
+    std::sort(out.begin(), out.end());
+    out.erase(std::unique(out.begin(), out.end()), out.end());
+  };
+  std::vector<int> deg(nn), nb;
+  for (int k = 0; k < nn; k++) { neighbours(k, nb); deg[k] = (int)nb.size(); }
+  std::vector<int> num(nn, -1), lvl(nn, 0);
+  int cnt = 0;
+  for (int start = 0; start < nn; start++) {  // every connected component
+    if (num[start] >= 0) continue;
+    int s0 = start;
+    for (int k = 0; k < nn; k++) if (num[k] < 0 && deg[k] < deg[s0]) s0 = k;
+    std::vector<int> queue{s0};
+    num[s0] = cnt++;
+    for (size_t h = 0; h < queue.size(); h++) {
+      const int k = queue[h];
+      neighbours(k, nb);
+      std::stable_sort(nb.begin(), nb.end(), [&](int a, int b) { return deg[a] < deg[b]; });
+      for (int j : nb)
+        if (num[j] < 0) { num[j] = cnt++; lvl[j] = lvl[k] + 1; queue.push_back(j); }
+    }
+  }
+  for (int k = 0; k < nn; k++) { key[k] = num[k]; slab[k] = lvl[k] / SLAB; }
+}
+
 int fm_level_order(const fm_mesh &m, int L, int nv, const std::vector<int> &dofL, int64_t ndof,
                    std::vector<int> &ord, std::vector<int> &rank, const std::vector<unsigned char> *nmask,
                    std::vector<int64_t> &goff, std::vector<int> &gmask, std::string &err) {
@@ -37,6 +81,37 @@ int fm_level_order(const fm_mesh &m, int L, int nv, const std::vector<int> &dofL
   std::vector<int> nodes;
   nodes.reserve(m.nq2[L]);
   for (int k = 0; k < N; k++) if (dofL[k] >= 0) nodes.push_back(k);
+  if (!m.lattice) {
+    // no lattice: Cuthill-McKee keys, slabs of breadth-first levels, groups by colour signature (coupled systems)
+    std::vector<int64_t> key(N, 0);
+    std::vector<int> slab(N, 0);
+    graph_keys(m, L, key, slab, 4);
+    auto sig = [&](int k) -> int { return nmask ? (int)(*nmask)[k] : 0; };
+    auto grp = [&](int k) -> int64_t { return (int64_t)slab[k] * 256 + sig(k); };
+    std::stable_sort(nodes.begin(), nodes.end(), [&](int a, int b) {
+      return grp(a) != grp(b) ? grp(a) < grp(b) : key[a] < key[b];
+    });
+    ord.clear();
+    goff.assign(1, 0);
+    gmask.clear();
+    for (size_t t = 0; t < nodes.size(); t++) {
+      const int k = nodes[t];
+      if (t > 0 && grp(k) != grp(nodes[t - 1])) { goff.push_back((int64_t)ord.size()); gmask.push_back(sig(nodes[t - 1])); }
+      for (int iv = 0; iv < nv; iv++) {
+        int d = dofL[k + (size_t)iv * N];
+        if (d >= 0) ord.push_back(d);
+      }
+    }
+    goff.push_back((int64_t)ord.size());
+    gmask.push_back(nodes.empty() ? 0 : sig(nodes.back()));
+    if ((int64_t)ord.size() != ndof) { err = "level order does not cover the DoFs"; return FM_ESTATE; }
+    rank.assign(ndof, -1);
+    for (int64_t i = 0; i < ndof; i++) {
+      if (ord[i] < 0 || ord[i] >= ndof || rank[ord[i]] >= 0) { err = "level order is not a permutation"; return FM_ESTATE; }
+      rank[ord[i]] = (int)i;
+    }
+    return 0;
+  }
   int axes[3] = {0, 1, 2};
   int64_t cnt[3];
   double h[3];

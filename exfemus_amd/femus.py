@@ -44,6 +44,8 @@ EXPORTS = [
     "fm_system_set_bcdata", "fm_heat_bc_rhs", "fm_ns_bc_rhs", "fm_ns_pressure_drop",
     "fm_ns_set_buoyancy", "fm_heat_nusselt", "fm_heat_coupling", "fm_adjoint_spmv", "fm_set_sell_split", "fm_set_sell_offsets", "fm_system_sell_info",
     "fm_system_set_vanka_store", "fm_mg_release", "fm_system_couple", "fm_system_advance", "fm_heat_nusselt_noalloc",
+    "fm_mesh_create", "fm_system_set_pattern", "fm_system_set_transfer", "fm_system_get_x_dev", "fm_system_set_x_dev",
+    "fm_system_get_rhs_dev", "fm_system_set_rhs_dev",
 ]
 
 
@@ -104,6 +106,11 @@ def lib():
     L.fm_sync.argtypes = [vp]
     L.fm_mesh_create_box.argtypes = [C.c_int, i32p, f64p, f64p, C.c_int, pp]
     L.fm_mesh_destroy.argtypes = [vp]
+    L.fm_mesh_create.argtypes = [C.c_int, C.c_int, C.c_int, f64p, i32p, i32p, i32p, i32p, i32p, vp, vp, pp]
+    L.fm_system_set_pattern.argtypes = [vp, C.c_int, vp, vp]
+    L.fm_system_set_transfer.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
+    for fn in ("fm_system_get_x_dev", "fm_system_set_x_dev", "fm_system_get_rhs_dev", "fm_system_set_rhs_dev"):
+        getattr(L, fn).argtypes = [vp, C.c_int, C.c_void_p]
     L.fm_mesh_info.argtypes = [vp, i32p]
     L.fm_mesh_level.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.fm_mesh_coords.argtypes = [vp, f64p]
@@ -212,9 +219,32 @@ class Mesh:
         h = np.zeros(3); h[:dim] = hi
         self.h = C.c_void_p()
         _ck(lib().fm_mesh_create_box(dim, nx, l, h, nlev, C.byref(self.h)))
+        self._info()
+
+    def _info(self):
         info = np.zeros(6, np.int32)
         _ck(lib().fm_mesh_info(self.h, info))
         self.dim, self.nlev, self.n2, self.n1, self.nsides, self.N = (int(v) for v in info)
+
+    @classmethod
+    def from_arrays(cls, dim, xyz, off_el, el_map, el_neighbor, nodes_q2, nodes_q1, parent=None, child=None):
+        """A caller's hierarchy in MGMesh's layout (fm_mesh_create): xyz [dim][N], off_el [nlev+1], el_map
+        [elements][9|27] top-level node ids, el_neighbor [elements][2 dim] level-local (-1 boundary), node counts per
+        level, and for levels >= 1 each element's parent (level L-1 index) and child index."""
+        self = cls.__new__(cls)
+        self.h = C.c_void_p()
+        xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1)
+        off = np.ascontiguousarray(off_el, np.int32)
+        nlev = len(off) - 1
+        i32 = lambda a: np.ascontiguousarray(a, np.int32).reshape(-1)
+        par = None if parent is None else i32(parent)
+        ch = None if child is None else i32(child)
+        self._keep = (par, ch)
+        ptr = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
+        _ck(lib().fm_mesh_create(dim, nlev, len(xyz) // dim, xyz, off, i32(el_map), i32(el_neighbor), i32(nodes_q2),
+                                 i32(nodes_q1), ptr(par), ptr(ch), C.byref(self.h)))
+        self._info()
+        return self
 
     def level(self, L):
         a, b, c = C.c_int(), C.c_int(), C.c_int()
@@ -338,6 +368,30 @@ class MGSolDA:
     def set_rhs(self, b, L=None):
         L = self._NoLevels - 1 if L is None else L
         _ck(lib().fm_system_set_rhs(self.h, L, _vec(b, self._Dim[L], "rhs")))
+
+    def set_pattern(self, L, rowptr, colind):
+        """The caller's pattern of A on level L (ReadMatrix), before init(); None drops it."""
+        if rowptr is None:
+            _ck(lib().fm_system_set_pattern(self.h, L, None, None))
+            return
+        rp = np.ascontiguousarray(rowptr, np.int64); ci = np.ascontiguousarray(colind, np.int32)
+        if rp.size != self._Dim[L] + 1 or ci.size != rp[-1]:
+            raise ValueError("pattern: row pointers / columns of the wrong length")
+        _ck(lib().fm_system_set_pattern(self.h, L, rp.ctypes.data_as(C.c_void_p), ci.ctypes.data_as(C.c_void_p)))
+
+    def set_transfer(self, which, L, rowptr, colind, val):
+        """The caller's restriction (which 1, Rst[L]) or prolongation (2, Prl[L]) in the reference numbering."""
+        rp = np.ascontiguousarray(rowptr, np.int64); ci = np.ascontiguousarray(colind, np.int32)
+        v = np.ascontiguousarray(val, np.float64)
+        if rp.size != self._Dim[L] + 1 or ci.size != rp[-1] or v.size != rp[-1]:
+            raise ValueError("transfer: arrays of the wrong length")
+        _ck(lib().fm_system_set_transfer(self.h, which, L, rp.ctypes.data_as(C.c_void_p), ci.ctypes.data_as(C.c_void_p),
+                                         v.ctypes.data_as(C.c_void_p)))
+
+    def dev_vector(self, fn, L, ptr):
+        """fm_system_{get,set}_{x,rhs}_dev with a device address (e.g. a torch tensor's data_ptr())."""
+        L = self._NoLevels - 1 if L is None else L
+        _ck(getattr(lib(), fn)(self.h, L, C.c_void_p(int(ptr))))
 
     def get_res(self, L):
         r = np.zeros(self._Dim[L]); _ck(lib().fm_get_res(self.h, L, r)); return r

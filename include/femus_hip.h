@@ -61,6 +61,19 @@ int fm_set_sell_offsets(int bits);
  * dim = 2 (Quad9) or 3 (Hex27); nx0 = coarse intervals; nlev levels.      */
 int fm_mesh_create_box(int dim, const int nx0[3], const double lo[3], const double hi[3],
                        int nlev, fm_mesh **out);
+/* A caller's hierarchy in MGMesh's layout (MGMesh::read_c, src/MGMesh.C:493-899; include/MGMesh.h:65-80),
+ * one subdomain: n_nodes top-level nodes with coordinates xyz[node + d*n_nodes] (_xyz); level L holds the
+ * elements off_el[L] .. off_el[L+1]-1 (_off_el) of el_map (n2 = 9 / 27 top-level node ids per element in
+ * libMesh Quad9 / Hex27 local order, _el_map[0]) and el_neighbor (2*dim level-local neighbours per element,
+ * -1 on the boundary, _el_neighbor[0]); nodes_q2[L] / nodes_q1[L] are the Q2 / Q1 node counts of level L
+ * (_NoNodes, _off_nd[0] / [1]: the level's nodes are the first ids of the top numbering); parent[e] and
+ * child[e] give each element of levels L >= 1 its level L-1 parent and its index among that parent's 2^dim
+ * children (the embedding order of MGGeomEl, src/MGGeomEl.C:182-690), from which the transfers are built
+ * unless the caller hands them in (fm_system_set_transfer); both may be NULL for one level.  Elements may be
+ * distorted or curved (the Jacobians are general).  FM_EINVAL with a message for an inconsistent hierarchy. */
+int fm_mesh_create(int dim, int nlev, int n_nodes, const double *xyz, const int *off_el, const int *el_map,
+                   const int *el_neighbor, const int *nodes_q2, const int *nodes_q1, const int *parent,
+                   const int *child, fm_mesh **out);
 int fm_mesh_destroy(fm_mesh *m);
 /* info[0..5] = dim, nlev, n2, n1, nsides, n_nodes_top   (MGMesh::read_c) */
 int fm_mesh_info(const fm_mesh *m, int *info);
@@ -94,6 +107,17 @@ int fm_system_get_bc(const fm_system *s, int *bc0, int *bc1);
  * restriction with Dirichlet mask (ReadRest, src/MGSolverDA.C:2236-2444),
  * prolongation (ReadProl, :2011-2230).  Builds device CSR + element maps.  */
 int fm_system_init(fm_system *s);
+/* Caller-supplied operators for fm_system_init, in the reference DoF numbering of each level (call before
+ * fm_system_init; NULL rowptr drops one again).
+ * fm_system_set_pattern: MGSolDA::ReadMatrix (src/MGSolverDA.C:1785-2004; SparseMatrixM::update_sparsity_pattern,
+ *   contrib/matrix/sparse_matrixM.h:49-186): the pattern of A[L]; it must hold every coupling of the level's
+ *   elements (fm_system_init fails with FM_EINVAL naming the missing entry), extra entries stay zero.
+ * fm_system_set_transfer: ReadRest / ReadProl (src/MGSolverDA.C:2236-2444, 2011-2230): which 1 = Rst[L] (coarse
+ *   L <- fine L+1; its rows are multiplied by the Dirichlet mask as ReadRest does), 2 = Prl[L] (fine L <-
+ *   coarse L-1), with values.  Repeated columns in a row are refused.                                        */
+int fm_system_set_pattern(fm_system *s, int L, const int64_t *rowptr, const int *colind);
+int fm_system_set_transfer(fm_system *s, int which, int L, const int64_t *rowptr, const int *colind,
+                           const double *val);
 /* Pattern / transfer export for parity checks (host arrays).
  * which: 0 = A[L] (sorted CSR), 1 = Rst[L] (coarse L <- fine L+1), 2 = Prl[L] */
 int64_t fm_system_csr_nnz(const fm_system *s, int which, int L);
@@ -110,6 +134,12 @@ int fm_system_get_x(const fm_system *s, int L, double *x);
 int fm_system_set_x(fm_system *s, int L, const double *x);
 int fm_system_get_rhs(const fm_system *s, int L, double *b);
 int fm_system_set_rhs(fm_system *s, int L, const double *b);
+/* The same four exchanges with vectors in device memory of the context's device (NumericVectorM data already
+ * on the GPU: no host copy, no PCIe); a host pointer is refused with FM_EINVAL.  Complete on return.      */
+int fm_system_get_x_dev(const fm_system *s, int L, double *x_dev);
+int fm_system_set_x_dev(fm_system *s, int L, const double *x_dev);
+int fm_system_get_rhs_dev(const fm_system *s, int L, double *b_dev);
+int fm_system_set_rhs_dev(fm_system *s, int L, const double *b_dev);
 
 /* ---- assembly (GenMatRhs) ---------------------------------------------- */
 typedef struct {

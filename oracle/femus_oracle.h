@@ -35,6 +35,7 @@ void om_mesh_info(const om_mesh *m, int *info);
 /* per level: n_el, n_nodes(Q2), n_nodes_q1 */
 void om_mesh_level(const om_mesh *m, int L, int *n_el, int *nq2, int *nq1);
 void om_mesh_coords(const om_mesh *m, double *xyz);
+void om_mesh_set_coords(om_mesh *m, const double *xyz);
 void om_mesh_conn(const om_mesh *m, int L, int *conn, int *neigh, int *parent);
 /* libMesh-style id and (lev, levp) keys of every top node (ordering audit) */
 void om_mesh_node_keys(const om_mesh *m, int *libmesh_id, int *lev, int *levp);

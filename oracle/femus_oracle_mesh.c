@@ -280,6 +280,9 @@ void om_mesh_level(const om_mesh *m, int L, int *n_el, int *nq2, int *nq1) {
   *n_el = m->nel[L]; *nq2 = m->nq2[L]; *nq1 = m->nq1[L];
 }
 void om_mesh_coords(const om_mesh *m, double *xyz) { memcpy(xyz, m->xyz, sizeof(double) * m->dim * m->Ntop); }
+/* moved nodes (a distorted or curved hierarchy with the same connectivity): the element routines read the
+ * geometry from xyz only, so every level follows */
+void om_mesh_set_coords(om_mesh *m, const double *xyz) { memcpy(m->xyz, xyz, sizeof(double) * m->dim * m->Ntop); }
 void om_mesh_conn(const om_mesh *m, int L, int *conn, int *neigh, int *parent) {
   if (conn) memcpy(conn, m->conn[L], sizeof(int) * m->nel[L] * m->n2);
   if (neigh) memcpy(neigh, m->neigh[L], sizeof(int) * m->nel[L] * m->nsides);

@@ -62,6 +62,7 @@ def lib():
         L.om_mesh_info.argtypes = [vp, i32p]
         L.om_mesh_level.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.om_mesh_coords.argtypes = [vp, f64p]
+        L.om_mesh_set_coords.argtypes = [vp, f64p]
         L.om_mesh_conn.argtypes = [vp, C.c_int, vp, vp, vp]
         L.om_mesh_node_keys.argtypes = [vp, i32p, i32p, i32p]
         L.om_fe_tables.argtypes = [C.c_int, C.c_int, vp, vp, vp, C.POINTER(C.c_int)]
@@ -153,6 +154,10 @@ class Mesh:
         x = np.zeros(self.dim * self.N)
         lib().om_mesh_coords(self.h, x)
         return x.reshape(self.dim, self.N)
+
+    def set_coords(self, xyz):
+        """Move the nodes (same connectivity): xyz[d][node]."""
+        lib().om_mesh_set_coords(self.h, np.ascontiguousarray(xyz, np.float64).reshape(-1))
 
     def conn(self, L):
         ne = self.level(L)[0]

@@ -4,6 +4,8 @@ import ctypes as C
 import os
 import re
 
+import numpy as np
+
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -124,3 +126,54 @@ def test_cpp_host_layer_compiles():
     r = subprocess.run([gxx, "-fsyntax-only", "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(root, "include"),
                         os.path.join(root, "tests", "cpp", "mgsol_driver.cpp")], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_caller_mesh_through_the_host_abi():
+    """fm_mesh_create takes a caller's hierarchy in MGMesh's layout without a device: a box hierarchy read back
+    through fm_mesh_conn / fm_mesh_coords, its elements renumbered, goes back in and comes out unchanged; an
+    inconsistent hierarchy (a vertex outside the level's Q1 nodes, a missing child) is refused with a message."""
+    if not os.path.exists(LIB):
+        pytest.skip("libfemus_hip.so not built")
+    from exfemus_amd import femus as F
+    box = F.Mesh(3, [2, 1, 2], [0, 0, 0], [1, 1, 1], 2)
+    rng = np.random.default_rng(0)
+    counts = [box.level(L) for L in range(2)]
+    off, emap, eneigh, par, chd, perms = [0], [], [], [], [], []
+    for L in range(2):
+        conn, neigh, parent = box.conn(L)
+        ne = conn.shape[0]
+        p = rng.permutation(ne); inv = np.argsort(p); perms.append(p)
+        emap.append(conn[inv]); nb = neigh[inv]
+        eneigh.append(np.where(nb >= 0, p[np.maximum(nb, 0)], -1))
+        if L:
+            par.append(perms[0][parent[inv]]); chd.append(inv - parent[inv] * 8)
+        else:
+            par.append(np.full(ne, -1)); chd.append(np.zeros(ne, int))
+        off.append(off[-1] + ne)
+    args = (3, box.coords(), off, np.concatenate(emap), np.concatenate(eneigh), [c[1] for c in counts],
+            [c[2] for c in counts], np.concatenate(par), np.concatenate(chd))
+    m = F.Mesh.from_arrays(*args)
+    assert (m.dim, m.nlev, m.N) == (3, 2, box.N)
+    np.testing.assert_array_equal(m.coords(), box.coords())
+    for L in range(2):
+        assert m.level(L) == box.level(L)
+        c, nb, pa = m.conn(L)
+        np.testing.assert_array_equal(c, emap[L])
+        np.testing.assert_array_equal(nb, eneigh[L])
+        if L:
+            np.testing.assert_array_equal(pa, par[L])
+    bad = list(args)
+    bad_map = np.concatenate(emap).copy()
+    bad_map[0] = counts[0][1] - 1          # level-0 vertex beyond the level's Q1 nodes
+    bad[3] = bad_map
+    with pytest.raises(F.FemusError) as e:
+        F.Mesh.from_arrays(*bad)
+    assert "vertex" in str(e.value) or "no element" in str(e.value)
+    bad = list(args)
+    ch = np.concatenate(chd).copy()
+    sib = np.where(par[1] == par[1][0])[0]  # level-1 elements of one parent
+    ch[off[1] + sib[1]] = ch[off[1] + sib[0]]   # two of them claim the same child slot
+    bad[8] = ch
+    with pytest.raises(F.FemusError) as e:
+        F.Mesh.from_arrays(*bad)
+    assert "same child" in str(e.value)

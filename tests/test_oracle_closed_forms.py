@@ -523,3 +523,78 @@ def test_poiseuille_reproduced_exactly(oracle):
         np.testing.assert_allclose(x[dd[dd >= 0]], ex[dd >= 0], atol=1e-11)
     pd = d[2 * N:3 * N]
     np.testing.assert_allclose(x[pd[pd >= 0]], -8 * nu * xyz[0][pd >= 0], atol=1e-10)
+
+
+# ----------------------------------------------------------------------------- curved elements
+def curved_map(xyz):
+    """(x, y + 0.2 x^2, z + 0.3 x y): curved, volume preserving (det DF = 1), and in the Q2 space, so the Q2
+    isoparametric elements hold it exactly: the discrete domain has volume 1 and linear fields are exact."""
+    out = np.array(xyz, dtype=np.float64, copy=True)
+    out[1] = xyz[1] + 0.2 * xyz[0] ** 2
+    if xyz.shape[0] == 3:
+        out[2] = xyz[2] + 0.3 * xyz[0] * xyz[1]
+    return out
+
+
+def _natural_bc(nv):
+    def bc(xp, neum, flag):
+        for iv in range(nv):
+            neum[iv], flag[iv] = 1, 0
+    return bc
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_curved_heat_stiffness_on_linear_fields(oracle, dim):
+    """On a curved mesh the element Jacobians are full (off-diagonal inverse terms): the assembled stiffness
+    (alpha0 = 1, no advection, steady, natural boundaries) satisfies u^T K v = (c . d) |Omega| = c . d for the
+    linear fields u = c . x, v = d . x; a transposed inverse Jacobian fails it."""
+    m = oracle.Mesh(dim, [2] * dim, [0.0] * dim, [1.0] * dim, 1)
+    m.set_coords(curved_map(m.coords()))
+    xyz = m.coords()
+    d, n = oracle.init_dof(m, 0, 1, 0)
+    b0, b1 = oracle.genbc(m, 1, 0, d, _natural_bc(1))
+    rp, ci = oracle.system_pattern(m, 0, 1, 0, d)
+    p = oracle.HeatParams(1.0, 1.0, 0.1, 0.0, 0.0, 0.0, (0.0, 0.0, 0.0))
+    v, _ = oracle.heat_genmatrhs(m, 0, 1, p, d, d, b0, b1, np.zeros(n), rp, ci)
+    K = sp.csr_matrix((v, ci, rp), shape=(n, n))
+    rng = np.random.default_rng(8)
+    for _ in range(3):
+        c, dd = rng.standard_normal(dim), rng.standard_normal(dim)
+        u, w = np.zeros(n), np.zeros(n)
+        u[d[:m.N]], w[d[:m.N]] = c @ xyz, dd @ xyz
+        assert abs(w @ (K @ u) - c @ dd) <= 1e-12 * (1 + abs(c @ dd))
+    # constants are in the kernel
+    one = np.ones(n)
+    np.testing.assert_allclose(K @ one, 0.0, atol=1e-13)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_curved_viscous_tensor_and_divergence_on_linear_fields(oracle, dim):
+    """The NS viscous block on the curved mesh with linear velocity fields u = G x, v = H x (natural boundaries,
+    no advection, steady): v^T K u = nu (H : G + tr(H G)) |Omega| (the weak form of -div(nu (grad u + grad u^T)));
+    the pressure rows give 1^T B u = tr(G) |Omega| (partition of unity of the Q1 shapes)."""
+    m = oracle.Mesh(dim, [2] * dim, [0.0] * dim, [1.0] * dim, 1)
+    m.set_coords(curved_map(m.coords()))
+    xyz = m.coords()
+    N = m.N
+    d, n = oracle.init_dof(m, 0, dim, 1)
+    b0, b1 = oracle.genbc(m, dim, 1, d, _natural_bc(dim + 1))
+    rp, ci = oracle.system_pattern(m, 0, dim, 1, d)
+    nu = 0.37
+    p = oracle.NSParams(nu, 0.1, 0.0, 0.0, 0.0, (0, 0, 0), (0, 0, 0))
+    v, _ = oracle.ns_genmatrhs(m, 0, 1, p, d, d, b0, b1, np.zeros(n), rp, ci)
+    K = sp.csr_matrix((v, ci, rp), shape=(n, n))
+    nq2 = m.level(0)[1]
+    vel = np.concatenate([d[iv * N:iv * N + nq2] for iv in range(dim)])
+    pre = d[dim * N:dim * N + m.level(0)[2]]
+    rng = np.random.default_rng(9)
+    for _ in range(3):
+        G, H = rng.standard_normal((dim, dim)), rng.standard_normal((dim, dim))
+        u, w = np.zeros(n), np.zeros(n)
+        for iv in range(dim):
+            u[d[iv * N:iv * N + nq2]] = G[iv] @ xyz[:, :nq2]
+            w[d[iv * N:iv * N + nq2]] = H[iv] @ xyz[:, :nq2]
+        Ku = K @ u
+        ref = nu * (np.sum(H * G) + np.trace(H @ G))
+        assert abs(w[vel] @ Ku[vel] - ref) <= 1e-12 * (1 + abs(ref))
+        assert abs(Ku[pre].sum() - np.trace(G)) <= 1e-12 * (1 + abs(np.trace(G)))
