@@ -56,9 +56,10 @@ inline int fm_ncolours(const fm_mesh &m, int L) {
 // xyz[node + d Ntop]; level L holds the elements off_el[L] .. off_el[L+1]-1 of el_map (n2 top-level node ids per
 // element in libMesh Quad9 / Hex27 local order) and el_neigh (nsides level-local neighbours, -1 on the boundary);
 // nq2[L] / nq1[L] Q2 / Q1 nodes of level L (the first nq2[L] / nq1[L] top-level ids); parent / child the level
-// L-1 parent of every element of L >= 1 and its index among the parent's 2^dim children (embedding order of
-// MGGeomEl), both NULL for one level.  Validates, colours the elements (no two of one colour share a node) and
-// builds the child maps.  Returns 0, or -1 with err set.
+// L-1 parent of every element of L >= 1 and its index among the parent's 2^dim children (corner bits (c >> d) & 1
+// along axis d of the parent's reference element), or both NULL: then they are inferred from the nested node sets.
+// Validates, colours the elements (no two of one colour share a node) and builds the child maps.  Returns 0, or -1
+// with err set.
 int fm_mesh_from_arrays(int dim, int nlev, int Ntop, const double *xyz, const int *off_el, const int *el_map,
                         const int *el_neigh, const int *nq2, const int *nq1, const int *parent, const int *child,
                         fm_mesh *m, std::string &err);

@@ -393,13 +393,59 @@ static bool colouring_valid(const fm_mesh &m, int L, const std::vector<unsigned 
   return true;
 }
 
+// Parent and child index of every element of level L from the nested node sets alone: the vertices of a child
+// are Q2 nodes of its parent, at the parent's reference points (b_d - 1) + (r_d + 1) / 2 for child c with corner
+// bits b_d = (c >> d) & 1 and vertex reference coordinates r; the first (coarse element, child) pair that holds
+// every vertex there is taken.  Fails when an element has none (a child not oriented like its parent).
+static int infer_parents(const fm_mesh &m, int L, std::vector<int> &par, std::vector<int> &ch, std::string &err) {
+  const int dim = m.dim, n1 = m.n1, n2 = m.n2, nch = m.nch;
+  // at[c][i]: the parent's local node at vertex i of child c
+  std::vector<int> at((size_t)nch * n1, -1);
+  for (int c = 0; c < nch; c++)
+    for (int i = 0; i < n1; i++) {
+      int q[3] = {0, 0, 0};
+      for (int d = 0; d < dim; d++) q[d] = (((c >> d) & 1) - 1) + (refc(dim, i, d) + 1) / 2;
+      for (int k = 0; k < n2; k++) {
+        bool same = true;
+        for (int d = 0; d < dim; d++) same = same && refc(dim, k, d) == q[d];
+        if (same) at[(size_t)c * n1 + i] = k;
+      }
+    }
+  std::vector<int64_t> off;
+  std::vector<int> els;
+  node_elems(m, L - 1, m.nq2[L - 1], off, els);
+  const int ne = m.nel[L];
+  par.assign(ne, -1);
+  ch.assign(ne, -1);
+  for (int e = 0; e < ne; e++) {
+    const int *cf = &m.conn[L][(size_t)e * n2];
+    const int v0 = cf[0];
+    if (v0 >= m.nq2[L - 1]) { err = "fm_mesh_create: a vertex of a level " + std::to_string(L) + " element is no coarse node"; return -1; }
+    for (int64_t q = off[v0]; q < off[v0 + 1] && par[e] < 0; q++) {
+      const int p = els[q];
+      const int *cc = &m.conn[L - 1][(size_t)p * n2];
+      for (int c = 0; c < nch && par[e] < 0; c++) {
+        bool ok = true;
+        for (int i = 0; i < n1 && ok; i++) ok = cf[i] == cc[at[(size_t)c * n1 + i]];
+        if (ok) { par[e] = p; ch[e] = c; }
+      }
+    }
+    if (par[e] < 0) {
+      err = "fm_mesh_create: no parent of level " + std::to_string(L) + " element " + std::to_string(e) +
+            " holds its vertices at a child's positions (pass parent and child)";
+      return -1;
+    }
+  }
+  return 0;
+}
+
 int fm_mesh_from_arrays(int dim, int nlev, int Ntop, const double *xyz, const int *off_el, const int *el_map,
                         const int *el_neigh, const int *nq2, const int *nq1, const int *parent, const int *child,
                         fm_mesh *m, std::string &err) {
   auto bad = [&](const std::string &w) { err = "fm_mesh_create: " + w; return -1; };
   if ((dim != 2 && dim != 3) || nlev < 1 || nlev > 12 || Ntop < 1) return bad("dim, levels or node count out of range");
   if (!xyz || !off_el || !el_map || !el_neigh || !nq2 || !nq1) return bad("missing array");
-  if (nlev > 1 && (!parent || !child)) return bad("a multilevel hierarchy needs parent and child arrays");
+  if ((parent == nullptr) != (child == nullptr)) return bad("parent and child arrays go together");
   m->dim = dim; m->nlev = nlev;
   m->n2 = dim == 2 ? 9 : 27; m->n1 = dim == 2 ? 4 : 8; m->nsides = 2 * dim;
   m->nb2 = dim == 2 ? 3 : 9; m->nch = 1 << dim;
@@ -451,8 +497,13 @@ int fm_mesh_from_arrays(int dim, int nlev, int Ntop, const double *xyz, const in
       const int np = m->nel[L - 1];
       m->kids[L].assign((size_t)np * m->nch, -1);
       m->chid[L].assign(ne, 0);
+      std::vector<int> ipar, ich;
+      if (!parent) {
+        std::string e2;
+        if (infer_parents(*m, L, ipar, ich, e2)) return bad(e2);
+      }
       for (int e = 0; e < ne; e++) {
-        const int p = parent[off_el[L] + e], c = child[off_el[L] + e];
+        const int p = parent ? parent[off_el[L] + e] : ipar[e], c = child ? child[off_el[L] + e] : ich[e];
         if (p < 0 || p >= np || c < 0 || c >= m->nch) return bad("parent or child index out of range");
         int &slot = m->kids[L][(size_t)p * m->nch + c];
         if (slot >= 0) return bad("two elements are the same child of one parent");

@@ -68,9 +68,12 @@ int fm_mesh_create_box(int dim, const int nx0[3], const double lo[3], const doub
  * -1 on the boundary, _el_neighbor[0]); nodes_q2[L] / nodes_q1[L] are the Q2 / Q1 node counts of level L
  * (_NoNodes, _off_nd[0] / [1]: the level's nodes are the first ids of the top numbering); parent[e] and
  * child[e] give each element of levels L >= 1 its level L-1 parent and its index among that parent's 2^dim
- * children (the embedding order of MGGeomEl, src/MGGeomEl.C:182-690), from which the transfers are built
- * unless the caller hands them in (fm_system_set_transfer); both may be NULL for one level.  Elements may be
- * distorted or curved (the Jacobians are general).  FM_EINVAL with a message for an inconsistent hierarchy. */
+ * children (corner bits (c >> d) & 1 along axis d of the parent's reference element: the embedding tables of
+ * MGGeomEl, src/MGGeomEl.C:182-690), from which the transfers are built unless the caller hands them in
+ * (fm_system_set_transfer).  With parent = child = NULL both are inferred from the nested node sets (the
+ * reference's mesh file does not hold them: a child's vertices are its parent's nodes at the child's corner).
+ * Elements may be distorted or curved (the Jacobians are general).  FM_EINVAL with a message for an
+ * inconsistent hierarchy. */
 int fm_mesh_create(int dim, int nlev, int n_nodes, const double *xyz, const int *off_el, const int *el_map,
                    const int *el_neighbor, const int *nodes_q2, const int *nodes_q1, const int *parent,
                    const int *child, fm_mesh **out);

@@ -162,6 +162,9 @@ def test_caller_mesh_through_the_host_abi():
         np.testing.assert_array_equal(nb, eneigh[L])
         if L:
             np.testing.assert_array_equal(pa, par[L])
+    # without parent / child arrays (the reference's mesh file has none) they are inferred from the nested nodes
+    m2 = F.Mesh.from_arrays(*args[:7])
+    np.testing.assert_array_equal(m2.conn(1)[2], par[1])
     bad = list(args)
     bad_map = np.concatenate(emap).copy()
     bad_map[0] = counts[0][1] - 1          # level-0 vertex beyond the level's Q1 nodes
