@@ -153,6 +153,51 @@ extern "C" int fm_mesh_conn(const fm_mesh *m, int L, int *conn, int *neigh, int 
 }
 
 // ===========================================================================
+// preprocessor output (the datasets of gencase's mesh and operator files)
+// ===========================================================================
+static bool gencase_kind(int k) { return k == 0 || k == 1 || k == 2; }
+extern "C" int fm_gencase_matrix_block(const fm_mesh *m, int L, int rkind, int ckind, int *len, int *pos,
+                                       int64_t *nnz) {
+  if (!m || L < 0 || L >= m->nlev || !gencase_kind(rkind) || !gencase_kind(ckind) || !len || !nnz)
+    return fm_einval(__func__);
+  std::vector<int> ln, ps;
+  *nnz = fm_gencase_block(*m, L, rkind, ckind, ln, pos ? &ps : nullptr);
+  std::memcpy(len, ln.data(), ln.size() * sizeof(int));
+  if (pos) std::memcpy(pos, ps.data(), ps.size() * sizeof(int));
+  return FM_OK;
+}
+extern "C" int fm_gencase_transfer_block(const fm_mesh *m, int L, int kind, int rest, int *len, int *pos,
+                                         double *val, int64_t *nnz) {
+  if (!m || !gencase_kind(kind) || !len || !nnz || (pos == nullptr) != (val == nullptr)) return fm_einval(__func__);
+  if (rest ? (L < 0 || L + 1 >= m->nlev) : (L < 1 || L >= m->nlev)) {
+    fm_set_error(std::string(__func__) + (rest ? ": restriction needs 0 <= L < nlev-1" : ": prolongation needs 1 <= L < nlev"));
+    return FM_EINVAL;
+  }
+  std::vector<std::vector<std::pair<int, double>>> blk;
+  if (kind == 0) fm_gencase_transfer_kk(*m, L, rest != 0, blk);
+  else fm_transfer_block(*m, L, kind, rest != 0, blk);
+  int64_t q = 0;
+  len[0] = 0;
+  for (size_t r = 0; r < blk.size(); r++) {
+    for (auto &pr : blk[r]) {
+      if (pos) { pos[q] = pr.first; val[q] = pr.second; }
+      q++;
+    }
+    len[r + 1] = (int)q;
+  }
+  *nnz = q;
+  return FM_OK;
+}
+extern "C" int fm_gencase_boundary(const fm_mesh *m, int L, int *nodes, int *n_bd) {
+  if (!m || L < 0 || L >= m->nlev || !n_bd) return fm_einval(__func__);
+  std::vector<int> nd;
+  fm_boundary_mesh(*m, L, nd);
+  *n_bd = (int)(nd.size() / m->nb2);
+  if (nodes) std::memcpy(nodes, nd.data(), nd.size() * sizeof(int));
+  return FM_OK;
+}
+
+// ===========================================================================
 // FE tables (closed-form Q2/Q1 Lagrange at 3-point Gauss-Legendre points)
 // ===========================================================================
 static double l2f(int c, double x) { return c < 0 ? .5 * x * (x - 1.) : (c > 0 ? .5 * x * (x + 1.) : 1. - x * x); }

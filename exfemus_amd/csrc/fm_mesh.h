@@ -31,6 +31,12 @@ void fm_transfer(const fm_mesh &m, int L, int nq2, int nq1, bool rest, const std
                  const std::vector<int> &dof_col, const std::vector<int> *dof_top, const std::vector<int> *bc0,
                  std::vector<int64_t> &rp, std::vector<int> &ci, std::vector<double> &v);
 void fm_elem_dofs(const fm_mesh &m, int L, int e, int nq2, int nq1, const std::vector<int> &dof, int *out);
+// node-level operators and the boundary mesh in gencase's order (the preprocessor's output files)
+void fm_transfer_block(const fm_mesh &m, int L, int kind, bool rest,
+                       std::vector<std::vector<std::pair<int, double>>> &blk);
+int64_t fm_gencase_block(const fm_mesh &m, int L, int rkind, int ckind, std::vector<int> &len, std::vector<int> *pos);
+void fm_gencase_transfer_kk(const fm_mesh &m, int L, bool rest, std::vector<std::vector<std::pair<int, double>>> &blk);
+void fm_boundary_mesh(const fm_mesh &m, int L, std::vector<int> &nodes);
 extern const int fm_hex27_ref[27][3];
 extern const int fm_quad9_ref[9][2];
 extern const int fm_face2d[4][3];

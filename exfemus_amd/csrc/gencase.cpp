@@ -270,66 +270,72 @@ static void embed_tables(int dim, int kind, std::vector<double> &E) {
     }
 }
 
+// Node-level transfer block of one kind (2: Q2 nodes, 1: Q1 nodes) on level L: per row node the (column node,
+// value) pairs in the order gencase emits them.  rest = false: prolongation (fine L <- coarse L-1), the last fine
+// element containing the node defines its row (MGGenCase.C:2308-2390, entries |E| <= 1e-8 dropped); rest = true:
+// restriction (coarse L <- fine L+1), over the coarse elements containing the row node in element order, their
+// children's nodes in order, the first occurrence kept (MGGenCase.C:2452-2566, |E| <= 1e-6 dropped).  Unmasked.
+void fm_transfer_block(const fm_mesh &m, int L, int kind, bool rest, std::vector<std::vector<std::pair<int, double>>> &blk) {
+  const int dim = m.dim, nch = m.nch, ns = kind == 2 ? m.n2 : m.n1;
+  const int nr = kind == 2 ? m.nq2[L] : m.nq1[L];
+  std::vector<double> E;
+  embed_tables(dim, kind, E);
+  blk.assign(nr, {});
+  if (!rest) {
+    std::vector<int> last(nr, -1), lastk(nr, -1);
+    for (int e = 0; e < m.nel[L]; e++)
+      for (int kk = 0; kk < ns; kk++) { int r = m.conn[L][(size_t)e * m.n2 + kk]; last[r] = e; lastk[r] = kk; }
+    for (int r = 0; r < nr; r++) {
+      int e = last[r], kk = lastk[r], par = m.parent[L][e], ch = m.chid[L][e];
+      for (int kj = 0; kj < ns; kj++) {
+        double val = E[((size_t)ch * ns + kk) * ns + kj];
+        if (std::fabs(val) > 1.e-8) blk[r].push_back({m.conn[L - 1][(size_t)par * m.n2 + kj], val});
+      }
+    }
+    return;
+  }
+  std::vector<int64_t> off;
+  std::vector<int> els;
+  node_elems(m, L, m.nq2[L], off, els);
+  const int nfine = m.nq2[L + 1];
+#pragma omp parallel
+  {
+    std::vector<int> mark(nfine, -1);
+#pragma omp for schedule(dynamic, 256)
+    for (int r = 0; r < nr; r++) {
+      auto &row = blk[r];
+      for (int64_t q = off[r]; q < off[r + 1]; q++) {
+        int e = els[q], kk = -1;
+        for (int k = 0; k < ns; k++) if (m.conn[L][(size_t)e * m.n2 + k] == r) kk = k;
+        if (kk < 0) continue;  // r is not among this element's first ns nodes (Q1 block)
+        for (int ich = 0; ich < nch; ich++) {
+          int ch = m.kids[L + 1][(size_t)e * nch + ich];
+          for (int kj = 0; kj < ns; kj++) {
+            double val = E[((size_t)ich * ns + kj) * ns + kk];
+            if (std::fabs(val) <= 1.e-6) continue;
+            int col = m.conn[L + 1][(size_t)ch * m.n2 + kj];
+            if (mark[col] == r) continue;
+            mark[col] = r;
+            row.push_back({col, val});
+          }
+        }
+      }
+    }
+  }
+}
+
 // Transfer operators in DoF space.  rest=true: R (coarse L <- fine L+1), rows
 // masked by bc0 % 2 of the coarse node's top DoF; rest=false: P (fine L <- coarse L-1).
 void fm_transfer(const fm_mesh &m, int L, int nq2, int nq1, bool rest, const std::vector<int> &dof_row,
                  const std::vector<int> &dof_col, const std::vector<int> *dof_top, const std::vector<int> *bc0,
                  std::vector<int64_t> &rp, std::vector<int> &ci, std::vector<double> &v) {
-  int N = m.Ntop, dim = m.dim, nch = m.nch, nv = nq2 + nq1;
+  int N = m.Ntop, nv = nq2 + nq1;
   int64_t nrows = (int64_t)nq2 * m.nq2[L] + (int64_t)nq1 * m.nq1[L];
   // per node-kind block rows: list of (col node, value)
   std::vector<std::vector<std::pair<int, double>>> blk[2];
   for (int t = 0; t < 2; t++) {
     if ((t == 0 && nq2 == 0) || (t == 1 && nq1 == 0)) continue;
-    int kind = t == 0 ? 2 : 1, ns = kind == 2 ? m.n2 : m.n1;
-    int nr = kind == 2 ? m.nq2[L] : m.nq1[L];
-    std::vector<double> E;
-    embed_tables(dim, kind, E);
-    blk[t].assign(nr, {});
-    if (!rest) {
-      // last fine element (in level order) containing the node defines its row
-      std::vector<int> last(nr, -1), lastk(nr, -1);
-      for (int e = 0; e < m.nel[L]; e++)
-        for (int kk = 0; kk < ns; kk++) { int r = m.conn[L][(size_t)e * m.n2 + kk]; last[r] = e; lastk[r] = kk; }
-      for (int r = 0; r < nr; r++) {
-        int e = last[r], kk = lastk[r], par = m.parent[L][e], ch = m.chid[L][e];
-        for (int kj = 0; kj < ns; kj++) {
-          double val = E[((size_t)ch * ns + kk) * ns + kj];
-          if (std::fabs(val) > 1.e-8) blk[t][r].push_back({m.conn[L - 1][(size_t)par * m.n2 + kj], val});
-        }
-      }
-    } else {
-      // row r collects, over the coarse elements containing r (in element
-      // order), their children's nodes in order: first occurrence kept.
-      std::vector<int64_t> off;
-      std::vector<int> els;
-      node_elems(m, L, m.nq2[L], off, els);
-      int nfine = m.nq2[L + 1];
-#pragma omp parallel
-      {
-        std::vector<int> mark(nfine, -1);
-#pragma omp for schedule(dynamic, 256)
-        for (int r = 0; r < nr; r++) {
-          auto &row = blk[t][r];
-          for (int64_t q = off[r]; q < off[r + 1]; q++) {
-            int e = els[q], kk = -1;
-            for (int k = 0; k < ns; k++) if (m.conn[L][(size_t)e * m.n2 + k] == r) kk = k;
-            if (kk < 0) continue;  // r is not among this element's first ns nodes (Q1 block)
-            for (int ich = 0; ich < nch; ich++) {
-              int ch = m.kids[L + 1][(size_t)e * nch + ich];
-              for (int kj = 0; kj < ns; kj++) {
-                double val = E[((size_t)ich * ns + kj) * ns + kk];
-                if (std::fabs(val) <= 1.e-6) continue;
-                int col = m.conn[L + 1][(size_t)ch * m.n2 + kj];
-                if (mark[col] == r) continue;
-                mark[col] = r;
-                row.push_back({col, val});
-              }
-            }
-          }
-        }
-      }
-    }
+    fm_transfer_block(m, L, t == 0 ? 2 : 1, rest, blk[t]);
   }
   rp.assign(nrows + 1, 0);
   for (int iv = 0; iv < nv; iv++) {
@@ -485,9 +491,12 @@ int fm_mesh_from_arrays(int dim, int nlev, int Ntop, const double *xyz, const in
         }
         used2[nd] = 1;
       }
+      // neighbours are global element ids (gencase's EL_NEIG numbering) of the same level, -1 on the boundary
       for (int sd = 0; sd < m->nsides; sd++) {
-        const int q = m->neigh[L][(size_t)e * m->nsides + sd];
-        if (q < -1 || q >= ne) return bad("neighbour index out of range");
+        int &q = m->neigh[L][(size_t)e * m->nsides + sd];
+        if (q != -1 && (q < off_el[L] || q >= off_el[L + 1]))
+          return bad("level " + std::to_string(L) + " neighbour outside the level's elements");
+        if (q != -1) q -= off_el[L];
       }
     }
     for (char u : used2) if (!u) return bad("a node of level " + std::to_string(L) + " belongs to no element");
@@ -526,4 +535,95 @@ int fm_mesh_from_arrays(int dim, int nlev, int Ntop, const double *xyz, const in
     }
   }
   return 0;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Preprocessor output of gencase (the datasets of mesh.h5 and of the Matrix / Prol / Rest files)
+// ---------------------------------------------------------------------------------------------------------------
+// Rows / columns of one kind on level L: 2 = the level's Q2 nodes, 1 = its Q1 nodes, 0 = its elements (one
+// piecewise-constant DoF each, NDOF_K = 1)
+static int kind_count(const fm_mesh &m, int L, int kind) {
+  return kind == 2 ? m.nq2[L] : (kind == 1 ? m.nq1[L] : m.nel[L]);
+}
+
+// Block of the matrix pattern of level L in gencase's column order (compute_mat_qq, MGGenCase.C:1781-1853, and the
+// piecewise blocks compute_mat_qk / _kq / _lk / _kl / _kk, :1855-2307).  Node rows (rkind 2 / 1) list, over the
+// elements holding the row node among their first n(rkind) local nodes in element order, the elements' first
+// n(ckind) local nodes in local order (ckind 2 / 1) or the element itself (ckind 0), each column at its first
+// occurrence.  Element rows (rkind 0) list the element's nodes that are level-L nodes of kind ckind in local order
+// (all n2 local nodes are scanned and mapped through the level's node map, so for ckind 1 only the vertices remain)
+// or, for ckind 0, the element itself.  len: rows + 1 prefix counts; pos may be NULL (count only).
+int64_t fm_gencase_block(const fm_mesh &m, int L, int rkind, int ckind, std::vector<int> &len, std::vector<int> *pos) {
+  const int nrows = kind_count(m, L, rkind), ncols = kind_count(m, L, ckind);
+  len.assign(nrows + 1, 0);
+  if (pos) pos->clear();
+  int64_t count = 0;
+  if (rkind == 0) {
+    for (int e = 0; e < nrows; e++) {
+      const int *c = &m.conn[L][(size_t)e * m.n2];
+      if (ckind == 0) {
+        if (pos) pos->push_back(e);
+        count++;
+      } else {
+        for (int k = 0; k < m.n2; k++)
+          if (c[k] < ncols) { if (pos) pos->push_back(c[k]); count++; }
+      }
+      len[e + 1] = (int)count;
+    }
+    return count;
+  }
+  const int nr = rkind == 2 ? m.n2 : m.n1, nc = ckind == 2 ? m.n2 : (ckind == 1 ? m.n1 : 0);
+  std::vector<int64_t> off;
+  std::vector<int> els;
+  node_elems(m, L, m.nq2[L], off, els);
+  std::vector<int> seen(ncols, -1), row;
+  for (int r = 0; r < nrows; r++) {
+    row.clear();
+    for (int64_t q = off[r]; q < off[r + 1]; q++) {
+      const int e = els[q];
+      const int *c = &m.conn[L][(size_t)e * m.n2];
+      bool holds = false;
+      for (int k = 0; k < nr; k++) holds = holds || c[k] == r;
+      if (!holds) continue;
+      if (ckind == 0) {
+        if (seen[e] != r) { seen[e] = r; row.push_back(e); }
+        continue;
+      }
+      for (int k = 0; k < nc; k++)
+        if (seen[c[k]] != r) { seen[c[k]] = r; row.push_back(c[k]); }
+    }
+    count += (int64_t)row.size();
+    len[r + 1] = (int)count;
+    if (pos) pos->insert(pos->end(), row.begin(), row.end());
+  }
+  return count;
+}
+
+// Piecewise-constant transfers (compute_prol_kk, MGGenCase.C:2392-2448: a fine element takes its parent's value;
+// compute_res_kk, :2568-2631: a coarse element sums its children in child order), value 1 each.
+void fm_gencase_transfer_kk(const fm_mesh &m, int L, bool rest, std::vector<std::vector<std::pair<int, double>>> &blk) {
+  blk.assign(m.nel[L], {});
+  for (int e = 0; e < m.nel[L]; e++) {
+    if (!rest) { blk[e].push_back({m.parent[L][e], 1.}); continue; }
+    for (int ich = 0; ich < m.nch; ich++) blk[e].push_back({m.kids[L + 1][(size_t)e * m.nch + ich], 1.});
+  }
+}
+
+// Boundary elements of level L as gencase lists them (MGGenCase.C:330-380, 550-570): over the level's elements in
+// order, every side without a neighbour, its nodes in the side node order of libMesh's build_side (libMesh 0.9.4
+// Quad9 / Hex27 side_nodes_map, which orients every side outwards).
+static const int g_side2d[4][3] = {{0, 1, 4}, {1, 2, 5}, {2, 3, 6}, {3, 0, 7}};
+static const int g_side3d[6][9] = {{0, 3, 2, 1, 11, 10, 9, 8, 20}, {0, 1, 5, 4, 8, 13, 16, 12, 21},
+                                   {1, 2, 6, 5, 9, 14, 17, 13, 22}, {2, 3, 7, 6, 10, 15, 18, 14, 23},
+                                   {3, 0, 4, 7, 11, 12, 19, 15, 24}, {4, 5, 6, 7, 16, 17, 18, 19, 25}};
+void fm_boundary_mesh(const fm_mesh &m, int L, std::vector<int> &nodes) {
+  nodes.clear();
+  for (int e = 0; e < m.nel[L]; e++)
+    for (int sd = 0; sd < m.nsides; sd++) {
+      if (m.neigh[L][(size_t)e * m.nsides + sd] != -1) continue;
+      for (int k = 0; k < m.nb2; k++) {
+        const int ln = m.dim == 2 ? g_side2d[sd][k] : g_side3d[sd][k];
+        nodes.push_back(m.conn[L][(size_t)e * m.n2 + ln]);
+      }
+    }
 }

@@ -45,7 +45,8 @@ EXPORTS = [
     "fm_ns_set_buoyancy", "fm_heat_nusselt", "fm_heat_coupling", "fm_adjoint_spmv", "fm_set_sell_split", "fm_set_sell_offsets", "fm_system_sell_info",
     "fm_system_set_vanka_store", "fm_mg_release", "fm_system_couple", "fm_system_advance", "fm_heat_nusselt_noalloc",
     "fm_mesh_create", "fm_system_set_pattern", "fm_system_set_transfer", "fm_system_get_x_dev", "fm_system_set_x_dev",
-    "fm_system_get_rhs_dev", "fm_system_set_rhs_dev",
+    "fm_system_get_rhs_dev", "fm_system_set_rhs_dev", "fm_gencase_matrix_block", "fm_gencase_transfer_block",
+    "fm_gencase_boundary",
 ]
 
 
@@ -115,6 +116,9 @@ def lib():
     L.fm_mesh_level.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.fm_mesh_coords.argtypes = [vp, f64p]
     L.fm_mesh_conn.argtypes = [vp, C.c_int, vp, vp, vp]
+    L.fm_gencase_matrix_block.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, C.POINTER(C.c_int64)]
+    L.fm_gencase_transfer_block.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, C.POINTER(C.c_int64)]
+    L.fm_gencase_boundary.argtypes = [vp, C.c_int, vp, C.POINTER(C.c_int)]
     L.fm_system_create.argtypes = [vp, vp, C.c_int, C.c_int, pp]
     L.fm_system_destroy.argtypes = [vp]
     L.fm_system_ndof.restype = C.c_int64
@@ -229,8 +233,9 @@ class Mesh:
     @classmethod
     def from_arrays(cls, dim, xyz, off_el, el_map, el_neighbor, nodes_q2, nodes_q1, parent=None, child=None):
         """A caller's hierarchy in MGMesh's layout (fm_mesh_create): xyz [dim][N], off_el [nlev+1], el_map
-        [elements][9|27] top-level node ids, el_neighbor [elements][2 dim] level-local (-1 boundary), node counts per
-        level, and for levels >= 1 each element's parent (level L-1 index) and child index."""
+        [elements][9|27] top-level node ids, el_neighbor [elements][2 dim] global element ids of the same level
+        (gencase's EL_NEIG numbering, -1 on the boundary), node counts per level, and for levels >= 1 each element's
+        parent (level L-1 index) and child index."""
         self = cls.__new__(cls)
         self.h = C.c_void_p()
         xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1)
@@ -262,6 +267,43 @@ class Mesh:
         _ck(lib().fm_mesh_conn(self.h, L, c.ctypes.data_as(C.c_void_p), nb.ctypes.data_as(C.c_void_p),
                                p.ctypes.data_as(C.c_void_p)))
         return c.reshape(ne, self.n2), nb.reshape(ne, self.nsides), p
+
+    def _rows(self, L, kind):
+        ne, nq2, nq1 = self.level(L)
+        return {2: nq2, 1: nq1, 0: ne}[kind]
+
+    def gencase_matrix_block(self, L, rkind, ckind):
+        """Block (rkind, ckind) of level L's matrix pattern in gencase's order (fm_gencase_matrix_block): the
+        LEN (row offsets) and POS (columns) datasets.  Kinds: 2 Q2 nodes, 1 Q1 nodes, 0 elements."""
+        ln = np.zeros(self._rows(L, rkind) + 1, np.int32)
+        nnz = C.c_int64()
+        vp = lambda a: a.ctypes.data_as(C.c_void_p)
+        _ck(lib().fm_gencase_matrix_block(self.h, L, rkind, ckind, vp(ln), None, C.byref(nnz)))
+        pos = np.zeros(nnz.value, np.int32)
+        _ck(lib().fm_gencase_matrix_block(self.h, L, rkind, ckind, vp(ln), vp(pos), C.byref(nnz)))
+        return ln, pos
+
+    def gencase_transfer_block(self, L, kind, rest):
+        """Prolongation (rest False: fine L <- coarse L-1) or restriction (rest True: coarse L <- fine L+1) block
+        of one kind in gencase's order (fm_gencase_transfer_block): LEN, POS, VAL."""
+        ln = np.zeros(self._rows(L, kind) + 1, np.int32)
+        nnz = C.c_int64()
+        vp = lambda a: a.ctypes.data_as(C.c_void_p)
+        _ck(lib().fm_gencase_transfer_block(self.h, L, kind, int(bool(rest)), vp(ln), None, None, C.byref(nnz)))
+        pos = np.zeros(nnz.value, np.int32)
+        val = np.zeros(nnz.value, np.float64)
+        _ck(lib().fm_gencase_transfer_block(self.h, L, kind, int(bool(rest)), vp(ln), vp(pos), vp(val),
+                                            C.byref(nnz)))
+        return ln, pos, val
+
+    def boundary(self, L):
+        """Boundary mesh of level L (fm_gencase_boundary): [sides][3 | 9] top-level node ids."""
+        n = C.c_int()
+        _ck(lib().fm_gencase_boundary(self.h, L, None, C.byref(n)))
+        nb2 = 3 if self.dim == 2 else 9
+        nodes = np.zeros(n.value * nb2, np.int32)
+        _ck(lib().fm_gencase_boundary(self.h, L, nodes.ctypes.data_as(C.c_void_p), C.byref(n)))
+        return nodes.reshape(n.value, nb2)
 
     def __del__(self):
         try:

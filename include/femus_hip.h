@@ -64,10 +64,11 @@ int fm_mesh_create_box(int dim, const int nx0[3], const double lo[3], const doub
 /* A caller's hierarchy in MGMesh's layout (MGMesh::read_c, src/MGMesh.C:493-899; include/MGMesh.h:65-80),
  * one subdomain: n_nodes top-level nodes with coordinates xyz[node + d*n_nodes] (_xyz); level L holds the
  * elements off_el[L] .. off_el[L+1]-1 (_off_el) of el_map (n2 = 9 / 27 top-level node ids per element in
- * libMesh Quad9 / Hex27 local order, _el_map[0]) and el_neighbor (2*dim level-local neighbours per element,
- * -1 on the boundary, _el_neighbor[0]); nodes_q2[L] / nodes_q1[L] are the Q2 / Q1 node counts of level L
+ * libMesh Quad9 / Hex27 local order, _el_map[0]) and el_neighbor (2*dim neighbours per element in libMesh
+ * side order as global element ids off_el[L] .. off_el[L+1]-1 of the same level, -1 on the boundary:
+ * _el_neighbor[0], the EL_NEIG dataset of gencase, MGGenCase.C:1178-1194); nodes_q2[L] / nodes_q1[L] are the Q2 / Q1 node counts of level L
  * (_NoNodes, _off_nd[0] / [1]: the level's nodes are the first ids of the top numbering); parent[e] and
- * child[e] give each element of levels L >= 1 its level L-1 parent and its index among that parent's 2^dim
+ * child[e] give each element of levels L >= 1 its level L-1 parent (level-local index) and its index among that parent's 2^dim
  * children (corner bits (c >> d) & 1 along axis d of the parent's reference element: the embedding tables of
  * MGGeomEl, src/MGGeomEl.C:182-690), from which the transfers are built unless the caller hands them in
  * (fm_system_set_transfer).  With parent = child = NULL both are inferred from the nested node sets (the
@@ -85,8 +86,27 @@ int fm_mesh_level(const fm_mesh *m, int L, int *n_el, int *n_q2, int *n_q1);
 /* coordinates SoA xyz[node + d*Ntop]  (MGMesh::_xyz, include/MGMesh.h:65-80) */
 int fm_mesh_coords(const fm_mesh *m, double *xyz);
 /* connectivity / neighbours / parents of level L (MGMesh::get_el_nod_conn
- * src/MGMesh.C:216-236, get_el_neighbor :261-275); any pointer may be NULL */
+ * src/MGMesh.C:216-236, get_el_neighbor :261-275), neighbours and parents as level-local
+ * indices; any pointer may be NULL */
 int fm_mesh_conn(const fm_mesh *m, int L, int *conn, int *neigh, int *parent);
+
+/* ---- preprocessor output: the datasets gencase writes ---------------------
+ * Kinds: 2 = the level's Q2 nodes, 1 = its Q1 nodes, 0 = its elements (one piecewise-constant DoF each,
+ * NDOF_K = 1).  Indices are level-local (gencase's node maps MAP<l> are the identity on the level's nodes).
+ * Block (rkind, ckind) of level L's matrix pattern, in gencase's column order (MGGenCase::compute_matrix
+ * :1370-1551 with compute_mat_qq / _qk / _kq / _lk / _kl / _kk :1781-2307; the datasets LEN<rc> = len[rows+1]
+ * and POS<rc> = pos[*nnz] of Matrix<L>).  With pos = NULL only len and *nnz are filled. */
+int fm_gencase_matrix_block(const fm_mesh *m, int L, int rkind, int ckind, int *len, int *pos, int64_t *nnz);
+/* Prolongation block of one kind (rest = 0: fine L <- coarse L-1, Prol<L-1>_<L>, compute_prol_qq / _kk
+ * :2308-2448) or restriction block (rest = 1: coarse L <- fine L+1, Rest<L+1>_<L>, compute_res_qq / _kk
+ * :2452-2631): LEN = len[rows+1], POS = pos[*nnz], VAL = val[*nnz], unmasked.  pos and val both NULL: count
+ * only. */
+int fm_gencase_transfer_block(const fm_mesh *m, int L, int kind, int rest, int *len, int *pos, double *val,
+                              int64_t *nnz);
+/* Boundary mesh of level L (printMesh :358-380 and :555-570, print_mesh_h5 /ELEMS/FEM1 :1246-1300): every side
+ * without a neighbour, over the level's elements in order and their sides in libMesh order, as its nb2 = 3 / 9
+ * nodes in the order of libMesh's build_side; *n_bd gets the count, nodes[n_bd * nb2] may be NULL. */
+int fm_gencase_boundary(const fm_mesh *m, int L, int *nodes, int *n_bd);
 
 /* ---- system (one MGSolDA equation) --------------------------------------
  * nq2 Q2 variables (u,v,w or T) followed by nq1 Q1 variables (p):

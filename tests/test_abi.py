@@ -144,7 +144,7 @@ def test_caller_mesh_through_the_host_abi():
         ne = conn.shape[0]
         p = rng.permutation(ne); inv = np.argsort(p); perms.append(p)
         emap.append(conn[inv]); nb = neigh[inv]
-        eneigh.append(np.where(nb >= 0, p[np.maximum(nb, 0)], -1))
+        eneigh.append(np.where(nb >= 0, p[np.maximum(nb, 0)] + off[-1], -1))
         if L:
             par.append(perms[0][parent[inv]]); chd.append(inv - parent[inv] * 8)
         else:
@@ -159,7 +159,7 @@ def test_caller_mesh_through_the_host_abi():
         assert m.level(L) == box.level(L)
         c, nb, pa = m.conn(L)
         np.testing.assert_array_equal(c, emap[L])
-        np.testing.assert_array_equal(nb, eneigh[L])
+        np.testing.assert_array_equal(nb, np.where(eneigh[L] >= 0, eneigh[L] - off[L], -1))
         if L:
             np.testing.assert_array_equal(pa, par[L])
     # without parent / child arrays (the reference's mesh file has none) they are inferred from the nested nodes

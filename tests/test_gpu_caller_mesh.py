@@ -60,7 +60,7 @@ def caller_mesh(femus, dim, nx, nlev, seed, curved=True, permuted=True):
         pe.append(p)
         emap.append(new_id[conn[inv]])
         nb = neigh[inv]
-        eneigh.append(np.where(nb >= 0, p[np.maximum(nb, 0)], -1))
+        eneigh.append(np.where(nb >= 0, p[np.maximum(nb, 0)] + off[-1], -1))
         if L > 0:
             old_par = parent[inv]
             par.append(pe[L - 1][old_par])
