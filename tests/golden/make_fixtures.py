@@ -6,8 +6,9 @@
 2. laspack_*.npz: inputs/outputs of the reference's vendored LASPACK library
    (contrib/laspack, compiled from its sources into oracle/_ref by
    `make -C oracle ref`): Mul_QV, ILUPrecond and GMRESIter(+ILUPrecond) on a
-   small assembled heat matrix and a small Navier-Stokes-like non-symmetric
-   matrix produced by the oracle.
+   small assembled heat matrix (heat2d) and a small Navier-Stokes Oseen
+   saddle-point matrix with a 1e-3 pressure diagonal (ns2d), both assembled by
+   the oracle.
 Usage: python tests/golden/make_fixtures.py
 """
 import ctypes as C
@@ -57,8 +58,27 @@ def laspack_vectors():
     k = configs.lognormal_k(m.level(1)[0])
     p = O.HeatParams(1.0, 1.0, 0.1, 1.0, 1.0, 1.0, (0.3, -0.2, 0.0))
     v, rhs = O.heat_genmatrhs(m, 1, 1, p, dof, dof, b0, b1, np.linspace(0, 1, n), rp, ci, k_elem=k)
-    # NS 2D cavity 1 level (saddle point, ILU needs nonzero diagonals: add a small p-p diagonal)
     cases = {"heat2d": (rp, ci, v, rhs)}
+    # NS 2D cavity, top of 2 levels, Oseen matrix at a nonzero velocity (saddle point, non-symmetric); the
+    # pressure rows have no diagonal in the reference matrix and LASPACK's ILU(0) divides by it, so 1e-3 is
+    # added on the p-p diagonal (the pattern holds it: the element couples every DoF with every other)
+    dn, nn = O.init_dof(m, 1, 2, 1)
+    c0, c1 = O.genbc(m, 2, 1, dn, configs.cavity_bc)
+    rpn, cin = O.system_pattern(m, 1, 2, 1, dn)
+    xo = np.zeros(nn)
+    xyz = m.coords()
+    N = m.N
+    for k in range(m.level(1)[1]):
+        xo[dn[k]] = np.sin(np.pi * xyz[0, k]) * xyz[1, k]
+        xo[dn[k + N]] = -0.5 * xyz[0, k] * (1 - xyz[1, k])
+    pn = O.NSParams(0.01, 0.1, 1.0, 1.0, 0.0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0))
+    vn, bn = O.ns_genmatrhs(m, 1, 1, pn, dn, dn, c0, c1, xo, rpn, cin)
+    npr = m.level(1)[2]
+    for k in range(npr):
+        r = dn[k + 2 * N]
+        q = rpn[r] + int(np.flatnonzero(cin[rpn[r]:rpn[r + 1]] == r)[0])
+        vn[q] += 1e-3
+    cases["ns2d"] = (rpn, cin, vn, bn)
     rng = np.random.default_rng(3)
     for name, (rp_, ci_, v_, b_) in cases.items():
         nn = len(rp_) - 1

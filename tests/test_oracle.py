@@ -304,14 +304,16 @@ def _golden(name):
     return np.load(os.path.join(GOLD, f"laspack_{name}.npz"))
 
 
-def test_spmv_matches_laspack(oracle):
-    g = _golden("heat2d")
+@pytest.mark.parametrize("name", ["heat2d", "ns2d"])
+def test_spmv_matches_laspack(oracle, name):
+    g = _golden(name)
     y = oracle.spmv(g["rp"], g["ci"], g["v"], g["x"])
     np.testing.assert_allclose(y, g["spmv"], rtol=1e-14, atol=1e-14 * np.abs(g["spmv"]).max())
 
 
-def test_gmres_ilu_matches_laspack(oracle):
-    g = _golden("heat2d")
+@pytest.mark.parametrize("name", ["heat2d", "ns2d"])
+def test_gmres_ilu_matches_laspack(oracle, name):
+    g = _golden(name)
     for its, key in ((8, "gmres8"), (45, "gmres45")):
         x, it, acc = oracle.gmres_ilu(g["rp"], g["ci"], g["v"], g["b"], np.zeros(len(g["b"])), its)
         assert it == int(g[key + "_its"])
@@ -319,11 +321,12 @@ def test_gmres_ilu_matches_laspack(oracle):
         assert abs(acc - float(g[key + "_acc"])) <= 1e-6 * float(g[key + "_acc"]) + 1e-14
 
 
-def test_ilu_matches_live_laspack(oracle):
+@pytest.mark.parametrize("name", ["heat2d", "ns2d"])
+def test_ilu_matches_live_laspack(oracle, name):
     R = oracle.ref_lib()
     if R is None:
         pytest.skip("oracle/_ref not built (reference sources absent)")
-    g = _golden("heat2d")
+    g = _golden(name)
     n = len(g["b"])
     yi = np.zeros(n)
     R.lr_ilu_apply(n, g["rp"], g["ci"], g["v"], g["x"], yi)
