@@ -522,6 +522,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="launch V-cycles eagerly instead of replaying HIP graphs")
     ap.add_argument("--cpu-baseline-only", action="store_true", help="only the CPU leg (run as a child process)")
     ap.add_argument("--cpu-cycles", type=int, default=1, help="V-cycles the CPU leg times")
+    ap.add_argument("--assembly-only", type=int, default=0, metavar="REPS",
+                    help="only the top-level GenMatRhs, timed REPS times (heat64 or channel; for the PMC runs)")
     ap.add_argument("--roofline-only", action="store_true",
                     help="heat64: only the top-level product timings and their layout keys (for the PMC runs)")
     args = ap.parse_args()
@@ -533,7 +535,8 @@ def main():
                                    "over RCCL) is not implemented"}))
         sys.exit(2)
     cpu_child = None
-    if not args.no_cpu and args.workload == "heat64" and not args.roofline_only and args.nx0 == 4 and args.nlev == 5:
+    if (not args.no_cpu and args.workload == "heat64" and not args.roofline_only and not args.assembly_only
+            and args.nx0 == 4 and args.nlev == 5):
         cpu_child = _cpu_baseline_child(args.cpu_cycles)
     from exfemus_amd import femus as F
     ctx = F.Context(0)
@@ -575,6 +578,14 @@ def main():
     N = s._Dim[top]
     nel_top = m.level(top)[0]
     log(f"setup {time.perf_counter() - t0:.1f}s  N_top={N}  assembly(all levels)={t_asm * 1e3:.1f} ms")
+    if args.assembly_only:
+        asm = assembly_rate(F, ctx, s, heat, reps=args.assembly_only)
+        # the 256 MiB copy (k_copy16): the yardstick that scales the PMC byte counters
+        cp = {"GBs": round(2 * (1 << 28) / ctx.stream_copy(1 << 28, reps=3) / 1e9, 1), "bytes_each_way": 1 << 28}
+        s.close()
+        ctx.close()
+        print(json.dumps({"workload": args.workload, "assembly_top": asm, "copy": cp}))
+        return
     if args.roofline_only:
         info, rf = roofline_measure(s, top, N, args.spmv_reps)
         out = {k: dict(r, kernel=KERNELS[k], layout_key=layout_key(KERNELS[k], info)) for k, r in rf.items()}

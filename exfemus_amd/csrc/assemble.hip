@@ -115,7 +115,7 @@ __device__ __forceinline__ int64_t el_slot(const int64_t *__restrict__ elslot, c
   return (k == 0xFFFFu || r < 0) ? -1 : r + 64 * (int64_t)k;
 }
 
-template <int DIM>
+template <int DIM, bool ADV>
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) void k_heat_vol(int nel, const int *__restrict__ elems, const int *__restrict__ conn,
                                                   const int *__restrict__ eldof, const unsigned short *__restrict__ elpos, const int64_t *__restrict__ elslot,
                                                   const double *__restrict__ xyz, int Ntop, const int *__restrict__ dof_top,
@@ -133,20 +133,52 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) void k
   const int tid = threadIdx.x;
   const double tcdt = a.tc * a.rhocp / a.dt;
   for (int t = tid; t < N2 * NG; t += blockDim.x) s_phi[t] = tab->phi2[t];
-  for (int ei = xcd_block(); ei < nel; ei += gridDim.x) {
+  // nodal data of the block's next element, loaded one element ahead by the first N2 threads
+  double g_x[DIM], g_v[DIM], g_T = 0.;
+  int g_bc = 0;
+  auto load_nodal = [&](int e) {
+    const int node = conn[(int64_t)e * N2 + tid];
+    const int kd = dof_top[node];
+    for (int d = 0; d < DIM; d++) {
+      g_x[d] = xyz[node + (int64_t)d * Ntop];
+      g_v[d] = vel ? vel[node + (int64_t)d * Ntop] : a.vel_const[d];
+    }
+    g_T = x_old[kd];
+    g_bc = bc0[kd];
+  };
+  int ei = xcd_block();
+  if (ei < nel && tid < N2) load_nodal(elems[ei]);
+  for (; ei < nel; ei += gridDim.x) {
     const int e = elems[ei];
-    __syncthreads();
+    // the slots of this thread's element-matrix entries and their current values, read before the element is
+    // computed (elements of one colour share no node, so no other element of the launch touches these slots)
+    const int ti = tid / 9, tj = tid % 9;
+    int64_t pp[TI][TI];
+    double pv[TI][TI];
+    if (tid < 81) {
+#pragma unroll
+      for (int u = 0; u < TI; u++)
+#pragma unroll
+        for (int v = 0; v < TI; v++) {
+          const int i = ti + 9 * u, j = tj + 9 * v;
+          pp[u][v] = el_slot(elslot, elpos, e, N2, trans ? j : i, trans ? i : j);
+          pv[u][v] = pp[u][v] >= 0 ? val[pp[u][v]] : 0.;
+        }
+    }
+    int rdof = -1;
+    double rold = 0.;
+    if (mode == 1 && tid < N2) {
+      rdof = eldof[(int64_t)e * N2 + tid];
+      rold = rhs[rdof];
+    }
+    __syncthreads();  // the previous element is done with the shared arrays
     if (tid < N2) {
-      int node = conn[(int64_t)e * N2 + tid];
-      int kd = dof_top[node];
-      for (int d = 0; d < DIM; d++) {
-        s_xx[tid + d * N2] = xyz[node + (int64_t)d * Ntop];
-        s_veln[tid + d * N2] = vel ? vel[node + (int64_t)d * Ntop] : a.vel_const[d];
-      }
-      s_Told[tid] = x_old[kd];
-      s_bcv[tid] = bc0[kd];
+      for (int d = 0; d < DIM; d++) { s_xx[tid + d * N2] = g_x[d]; s_veln[tid + d * N2] = g_v[d]; }
+      s_Told[tid] = g_T;
+      s_bcv[tid] = g_bc;
     }
     __syncthreads();
+    if (ei + (int)gridDim.x < nel && tid < N2) load_nodal(elems[ei + gridDim.x]);
     const double ke = coef ? coef[e] : 1.0;
     if (tid < NG) {
       const int qp = tid;
@@ -193,7 +225,6 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) void k
     }
     __syncthreads();
     if (tid < 81) {
-      const int ti = tid / 9, tj = tid % 9;
       double K[TI][TI];
 #pragma unroll
       for (int u = 0; u < TI; u++)
@@ -202,40 +233,43 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) void k
 #pragma unroll 1
       for (int q = 0; q < NG; q++) {
         double bj[TI];
+        if (ADV) {
 #pragma unroll
-        for (int v = 0; v < TI; v++) bj[v] = tcdt * s_phi[(tj + 9 * v) * NG + q];
+          for (int v = 0; v < TI; v++) bj[v] = tcdt * s_phi[(tj + 9 * v) * NG + q];
+        }
 #pragma unroll
         for (int d = 0; d < DIM; d++) {
           const double *G = s_G + (q * DIM + d) * N2;
-          const double w = s_w[q * DIM + d], rv = a.rhocp * s_vq[q * DIM + d];
+          const double w = s_w[q * DIM + d];
           double gi[TI], gj[TI];
 #pragma unroll
           for (int u = 0; u < TI; u++) { gi[u] = w * G[ti + 9 * u]; gj[u] = G[tj + 9 * u]; }
+          if (ADV) {
+            const double rv = a.rhocp * s_vq[q * DIM + d];
 #pragma unroll
-          for (int v = 0; v < TI; v++) bj[v] += rv * gj[v];
+            for (int v = 0; v < TI; v++) bj[v] += rv * gj[v];
+          }
 #pragma unroll
           for (int u = 0; u < TI; u++)
 #pragma unroll
             for (int v = 0; v < TI; v++) K[u][v] += gi[u] * gj[v];
         }
-        const double jw = s_JxW[q];
+        if (ADV) {
+          const double jw = s_JxW[q];
 #pragma unroll
-        for (int u = 0; u < TI; u++) {
-          const double ai = jw * s_phi[(ti + 9 * u) * NG + q];
+          for (int u = 0; u < TI; u++) {
+            const double ai = jw * s_phi[(ti + 9 * u) * NG + q];
 #pragma unroll
-          for (int v = 0; v < TI; v++) K[u][v] += ai * bj[v];
+            for (int v = 0; v < TI; v++) K[u][v] += ai * bj[v];
+          }
         }
       }
 #pragma unroll
       for (int u = 0; u < TI; u++) {
-        const int i = ti + 9 * u;
-        if (s_bcv[i] == 0) continue;
+        if (s_bcv[ti + 9 * u] == 0) continue;
 #pragma unroll
-        for (int v = 0; v < TI; v++) {
-          const int j = tj + 9 * v;
-          const int64_t p = el_slot(elslot, elpos, e, N2, trans ? j : i, trans ? i : j);
-          if (p >= 0) val[p] += K[u][v];
-        }
+        for (int v = 0; v < TI; v++)
+          if (pp[u][v] >= 0) val[pp[u][v]] = pv[u][v] + K[u][v];
       }
     }
     if (mode == 1 && tid < N2) {
@@ -244,9 +278,250 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) void k
         double F = 0.;
 #pragma unroll 3
         for (int qp = 0; qp < NG; qp++) F += s_JxW[qp] * s_phi[i * NG + qp] * (tcdt * s_Tg[qp] + a.source);
-        rhs[eldof[(int64_t)e * N2 + i]] += F;
+        rhs[rdof] = rold + F;
       }
     }
+  }
+}
+
+// ===========================================================================
+// Heat volume kernel, one wavefront per element
+//
+// A 256-thread block stages the reference tables once for its four waves (dphi/dxi at the Gauss points,
+// R[q][k][s], and phi[q][s]); afterwards every wave takes its own elements and never waits for the others.
+// Per element:
+//  1. the DIM x DIM Jacobian entries at the NG Gauss points (MGFE::Jac3D / Jac2D, src/MGFE.C:1066-1166) as
+//     NG DIM^2 dot products over the element's nodes, spread over the 64 lanes;
+//  2. per Gauss point (one lane each): the inverse Jacobian M_dk = dxi_k/dx_d, JxW, the interpolated velocity
+//     and T_old, the upwind coefficients, and from them the weights of MGSolverT.C:228-330 folded into the
+//     reference frame: the symmetric C_q,kl = sum_d JxW (alpha0 k_e + f_up,d v_d^2) M_dk M_dl and the
+//     advection vector vt_q,k = rhocp sum_d v_d M_dk;
+//  3. the element matrix K_ij = sum_q [ sum_kl C_q,kl R_qki R_qlj + a_qi b_qj ] with a_qi = JxW phi_i and
+//     b_qj = tcdt phi_j + sum_k vt_q,k R_qkj (the mass / advection part only when present), each lane owning a
+//     TI x TJ tile (rows i + 9u, columns TJ t + v: 63 tiles of 3 x 4 of the 27 x 27 hex27 matrix, 27 tiles of
+//     1 x 3 of the 9 x 9 quad9 one);
+//  4. Dirichlet rows dropped (the bc_vol row mask); the tile either added to A through the element -> SELL
+//     slot map (coloured launches: the current values are read before step 1, since elements of one colour
+//     share no slot) or written to the element-matrix buffer E[e][row][col] (TOE; row / column swapped for
+//     the transposed matrix), and the element right-hand side likewise.
+// ===========================================================================
+__device__ __forceinline__ void wave_lds_sync() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); __builtin_amdgcn_wave_barrier(); }
+
+template <int DIM, bool ADV, bool TOE>
+__global__ __launch_bounds__(256) void k_heat_el(int nel, const int *__restrict__ elems, const int *__restrict__ conn,
+                                                 const int *__restrict__ eldof, const unsigned short *__restrict__ elpos,
+                                                 const int64_t *__restrict__ elslot, const double *__restrict__ xyz, int Ntop,
+                                                 const int *__restrict__ dof_top, const int *__restrict__ bc0,
+                                                 const double *__restrict__ x_old, const double *__restrict__ coef,
+                                                 const double *__restrict__ vel, const FeTab *__restrict__ tab, HeatArgs a,
+                                                 int mode, int trans, double *__restrict__ val, double *__restrict__ rhs,
+                                                 double *__restrict__ E, double *__restrict__ Fe) {
+  constexpr int N2 = DIM == 2 ? 9 : 27, NG = N2, DD = DIM * DIM, NC = DIM == 2 ? 3 : 6;
+  constexpr int TI = DIM == 2 ? 1 : 3, TJ = DIM == 2 ? 3 : 4, NTI = N2 / TI, NTJ = (N2 + TJ - 1) / TJ;
+  constexpr int NW = 4;  // waves per block
+  __shared__ double s_R[NG * DIM * N2];  // [q][k][s]
+  __shared__ double s_phi[NG * N2];      // [q][s]
+  __shared__ double s_xx[NW][DIM * N2], s_vn[NW][ADV ? DIM * N2 : 1], s_Tn[NW][N2];
+  __shared__ double s_jb[NW][NG * DD], s_C[NW][NG * NC], s_vt[NW][ADV ? NG * DIM : 1], s_JxW[NW][NG], s_src[NW][NG];
+  __shared__ int s_bcv[NW][N2];
+  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  for (int t = tid; t < NG * DIM * N2; t += blockDim.x) {
+    const int q = t / (DIM * N2), k = (t / N2) % DIM, sn = t % N2;
+    s_R[t] = tab->dphi2[sn * NG + q + k * N2 * NG];
+  }
+  for (int t = tid; t < NG * N2; t += blockDim.x) s_phi[t] = tab->phi2[(t % N2) * NG + t / N2];
+  __syncthreads();
+  const double tcdt = a.tc * a.rhocp / a.dt;
+  double *xx = s_xx[wv], *vn = s_vn[wv], *Tn = s_Tn[wv], *jb = s_jb[wv], *Cq = s_C[wv], *vt = s_vt[wv];
+  double *JxWq = s_JxW[wv], *srcq = s_src[wv];
+  int *bcv = s_bcv[wv];
+  const int ti = lane / NTJ, tj = lane % NTJ;
+  const bool tile = lane < NTI * NTJ;
+  const int nwaves = gridDim.x * NW;
+  for (int ei = xcd_block() * NW + wv; ei < nel; ei += nwaves) {
+    const int e = elems ? elems[ei] : ei;
+    // slots of this lane's entries and their current values (scatter into A)
+    int64_t pp[TI][TJ];
+    double pv[TI][TJ];
+    if (!TOE && tile) {
+#pragma unroll
+      for (int u = 0; u < TI; u++)
+#pragma unroll
+        for (int v = 0; v < TJ; v++) {
+          const int i = ti + NTI * u, j = TJ * tj + v;
+          pp[u][v] = j < N2 ? el_slot(elslot, elpos, e, N2, trans ? j : i, trans ? i : j) : -1;
+          pv[u][v] = pp[u][v] >= 0 ? val[pp[u][v]] : 0.;
+        }
+    }
+    if (lane < N2) {
+      const int node = conn[(int64_t)e * N2 + lane];
+      const int kd = dof_top[node];
+      for (int d = 0; d < DIM; d++) {
+        xx[lane + d * N2] = xyz[node + (int64_t)d * Ntop];
+        if (ADV) vn[lane + d * N2] = vel ? vel[node + (int64_t)d * Ntop] : a.vel_const[d];
+      }
+      Tn[lane] = x_old[kd];
+      bcv[lane] = bc0[kd];
+    }
+    wave_lds_sync();
+    // 1. Jacobian entries J[q][c][k] = sum_s x_c,s dphi_s/dxi_k (q)
+    for (int p = lane; p < NG * DD; p += 64) {
+      const int q = p / DD, c = (p / DIM) % DIM, k = p % DIM;
+      const double *R = s_R + (q * DIM + k) * N2, *x = xx + c * N2;
+      double J = 0.;
+#pragma unroll 9
+      for (int sn = 0; sn < N2; sn++) J += x[sn] * R[sn];
+      jb[p] = J;
+    }
+    wave_lds_sync();
+    // 2. per Gauss point
+    if (lane < NG) {
+      const int q = lane;
+      const double *J = jb + q * DD;
+      double M[DIM][DIM], det;
+      if (DIM == 2) {
+        det = J[0] * J[3] - J[2] * J[1];
+        const double id = 1. / det;
+        M[0][0] = J[3] * id; M[0][1] = -J[2] * id; M[1][0] = -J[1] * id; M[1][1] = J[0] * id;
+      } else {
+        // J[c*3+k] = dx_c/dxi_k; M[d][k] = (J^-1)[k][d]
+        const double c00 = J[4] * J[8] - J[5] * J[7], c01 = J[5] * J[6] - J[3] * J[8], c02 = J[3] * J[7] - J[4] * J[6];
+        det = J[0] * c00 + J[1] * c01 + J[2] * c02;
+        const double id = 1. / det;
+        M[0][0] = c00 * id; M[0][1] = c01 * id; M[0][2] = c02 * id;
+        M[1][0] = (J[2] * J[7] - J[1] * J[8]) * id; M[1][1] = (J[0] * J[8] - J[2] * J[6]) * id;
+        M[1][2] = (J[1] * J[6] - J[0] * J[7]) * id;
+        M[2][0] = (J[1] * J[5] - J[2] * J[4]) * id; M[2][1] = (J[2] * J[3] - J[0] * J[5]) * id;
+        M[2][2] = (J[0] * J[4] - J[1] * J[3]) * id;
+      }
+      const double JxW = det * tab->w[q];
+      const double *ph = s_phi + q * N2;
+      double Tg = 0.;
+#pragma unroll 9
+      for (int sn = 0; sn < N2; sn++) Tg += ph[sn] * Tn[sn];
+      double w[DIM];
+      const double ke = coef ? coef[e] : 1.0;
+      for (int d = 0; d < DIM; d++) w[d] = JxW * a.alpha0 * ke;
+      if (ADV) {
+        double vg[DIM];
+        for (int d = 0; d < DIM; d++) vg[d] = 0.;
+#pragma unroll 9
+        for (int sn = 0; sn < N2; sn++)
+          for (int d = 0; d < DIM; d++) vg[d] += ph[sn] * vn[sn + d * N2];
+        double mod2 = 1.e-20;
+        for (int d = 0; d < DIM; d++) mod2 += vg[d] * vg[d];
+        mod2 = sqrt(mod2);
+        for (int d = 0; d < DIM; d++) {
+          const double h = fabs(xx[d * N2 + 2 + 4 * (DIM - 2)] - xx[d * N2 + 0]);
+          const double h1 = fabs(xx[d * N2 + 3 + 4 * (DIM - 2)] - xx[d * N2 + 1]);
+          const double h_eff = h < h1 ? h1 : h;
+          const double Pe = 0.5 * mod2 * h_eff / a.alpha0;
+          const double fup = a.upwind * a.rhocp * 0.5 * (1. / tanh(Pe) - 1. / Pe) * h_eff / mod2;
+          w[d] = JxW * (a.alpha0 * ke + fup * vg[d] * vg[d]);
+        }
+        for (int k = 0; k < DIM; k++) {
+          double t = 0.;
+          for (int d = 0; d < DIM; d++) t += vg[d] * M[d][k];
+          vt[q * DIM + k] = a.rhocp * t;
+        }
+      }
+      int c = 0;
+      for (int k = 0; k < DIM; k++)
+        for (int l = k; l < DIM; l++) {
+          double t = 0.;
+          for (int d = 0; d < DIM; d++) t += w[d] * M[d][k] * M[d][l];
+          Cq[q * NC + c++] = t;
+        }
+      JxWq[q] = JxW;
+      srcq[q] = JxW * (tcdt * Tg + a.source);
+    }
+    wave_lds_sync();
+    // 3. element matrix tile
+    if (tile) {
+      double K[TI][TJ];
+#pragma unroll
+      for (int u = 0; u < TI; u++)
+#pragma unroll
+        for (int v = 0; v < TJ; v++) K[u][v] = 0.;
+#pragma unroll 1
+      for (int q = 0; q < NG; q++) {
+        const double *R = s_R + q * DIM * N2;
+        double C[DIM][DIM];
+        {
+          int c = 0;
+          for (int k = 0; k < DIM; k++)
+            for (int l = k; l < DIM; l++) { C[k][l] = Cq[q * NC + c]; C[l][k] = C[k][l]; c++; }
+        }
+        double rc[TJ][DIM];
+#pragma unroll
+        for (int v = 0; v < TJ; v++) {
+          const int j = TJ * tj + v;
+#pragma unroll
+          for (int k = 0; k < DIM; k++) rc[v][k] = j < N2 ? R[k * N2 + j] : 0.;
+        }
+#pragma unroll
+        for (int u = 0; u < TI; u++) {
+          const int i = ti + NTI * u;
+          double H[DIM];
+#pragma unroll
+          for (int l = 0; l < DIM; l++) H[l] = 0.;
+#pragma unroll
+          for (int k = 0; k < DIM; k++) {
+            const double r = R[k * N2 + i];
+#pragma unroll
+            for (int l = 0; l < DIM; l++) H[l] += C[k][l] * r;
+          }
+#pragma unroll
+          for (int v = 0; v < TJ; v++)
+#pragma unroll
+            for (int l = 0; l < DIM; l++) K[u][v] += H[l] * rc[v][l];
+        }
+        if (ADV) {
+          const double *ph = s_phi + q * N2;
+          double b[TJ];
+#pragma unroll
+          for (int v = 0; v < TJ; v++) {
+            const int j = TJ * tj + v;
+            double t = j < N2 ? tcdt * ph[j] : 0.;
+#pragma unroll
+            for (int k = 0; k < DIM; k++) t += vt[q * DIM + k] * rc[v][k];
+            b[v] = t;
+          }
+          const double jw = JxWq[q];
+#pragma unroll
+          for (int u = 0; u < TI; u++) {
+            const double ai = jw * ph[ti + NTI * u];
+#pragma unroll
+            for (int v = 0; v < TJ; v++) K[u][v] += ai * b[v];
+          }
+        }
+      }
+      // 4. output
+#pragma unroll
+      for (int u = 0; u < TI; u++) {
+        const int i = ti + NTI * u;
+        const bool keep = bcv[i] != 0;
+#pragma unroll
+        for (int v = 0; v < TJ; v++) {
+          const int j = TJ * tj + v;
+          if (j >= N2) continue;
+          if (TOE) {
+            E[((int64_t)e * N2 + (trans ? j : i)) * N2 + (trans ? i : j)] = keep ? K[u][v] : 0.;
+          } else if (keep && pp[u][v] >= 0) {
+            val[pp[u][v]] = pv[u][v] + K[u][v];
+          }
+        }
+      }
+    }
+    if (mode == 1 && lane < N2) {
+      const int i = lane;
+      double F = 0.;
+#pragma unroll 9
+      for (int q = 0; q < NG; q++) F += s_phi[q * N2 + i] * srcq[q];
+      if (TOE) Fe[(int64_t)e * N2 + i] = bcv[i] != 0 ? F : 0.;
+      else if (bcv[i] != 0) rhs[eldof[(int64_t)e * N2 + i]] += F;
+    }
+    wave_lds_sync();
   }
 }
 
@@ -1063,6 +1338,12 @@ int heat_lap(hipStream_t s, int dim, int nel, const int *elems, const int *conn,
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
+// blocks of the one-wave-per-element kernels (four waves per block)
+static inline unsigned nblk_w(int nel) {
+  int b = (nel + 3) / 4;
+  b = b < 8192 ? b : 8192;
+  return (unsigned)(b < 1 ? 1 : b);
+}
 static inline unsigned nblk(int nel) {
   int b = nel < 4096 ? nel : 4096;
   return (unsigned)(b < 1 ? 1 : b);
@@ -1077,16 +1358,22 @@ int heat_assemble(hipStream_t s, int dim, int nel, const int *elems, const int *
                   double *rhs) {
   if (nel <= 0) return 0;
   const double *xb = xbc ? xbc : x_old;
+  // the mass / advection part of the element matrix is zero without a time term and without a velocity
+  const bool adv = a.tc * a.rhocp != 0. || vel || a.vel_const[0] != 0. || a.vel_const[1] != 0. || a.vel_const[2] != 0.;
   if (dim == 2) {
-    if (!bd_only)
-      hipLaunchKernelGGL(k_heat_vol<2>, dim3(nblk(nel)), dim3(128), 0, s, nel, elems, conn, eldof, elpos, elslot, xyz, Ntop,
-                         dof_top, bc0, x_old, coef, vel, tab, a, mode, trans, val, rhs);
+    if (!bd_only) {
+      auto kern = adv ? k_heat_el<2, true, false> : k_heat_el<2, false, false>;
+      hipLaunchKernelGGL(kern, dim3(nblk_w(nel)), dim3(256), 0, s, nel, elems, conn, eldof, elpos, elslot, xyz, Ntop, dof_top,
+                         bc0, x_old, coef, vel, tab, a, mode, trans, val, rhs, nullptr, nullptr);
+    }
     hipLaunchKernelGGL(k_heat_bd<2>, dim3((nel + 127) / 128), dim3(128), 0, s, nel, elems, conn, eldof, elpos, elslot, bmask,
                        xyz, Ntop, dof_top, bc0, bc1, xb, tab, a, mode, trans, val, rhs);
   } else {
-    if (!bd_only)
-      hipLaunchKernelGGL(k_heat_vol<3>, dim3(nblk(nel)), dim3(128), 0, s, nel, elems, conn, eldof, elpos, elslot, xyz, Ntop,
-                         dof_top, bc0, x_old, coef, vel, tab, a, mode, trans, val, rhs);
+    if (!bd_only) {
+      auto kern = adv ? k_heat_el<3, true, false> : k_heat_el<3, false, false>;
+      hipLaunchKernelGGL(kern, dim3(nblk_w(nel)), dim3(256), 0, s, nel, elems, conn, eldof, elpos, elslot, xyz, Ntop, dof_top,
+                         bc0, x_old, coef, vel, tab, a, mode, trans, val, rhs, nullptr, nullptr);
+    }
     hipLaunchKernelGGL(k_heat_bd<3>, dim3((nel + 127) / 128), dim3(128), 0, s, nel, elems, conn, eldof, elpos, elslot, bmask,
                        xyz, Ntop, dof_top, bc0, bc1, xb, tab, a, mode, trans, val, rhs);
   }
