@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfemus_hip.so")
-SOURCES = ["kernels.hip", "assemble.hip", "femus.cpp", "mg.cpp", "vanka.hip", "coarse.hip", "gencase.cpp", "sell_host.cpp", "order_host.cpp", "noalloc.hip"]
+SOURCES = ["kernels.hip", "assemble.hip", "femus.cpp", "mg.cpp", "vanka.hip", "coarse.hip", "gencase.cpp", "sell_host.cpp", "order_host.cpp", "noalloc.hip", "egather_host.cpp"]
 ARCH = os.environ.get("FEMUS_ARCH", "gfx950")
 
 

@@ -308,7 +308,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) void k
 __device__ __forceinline__ void wave_lds_sync() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); __builtin_amdgcn_wave_barrier(); }
 
 template <int DIM, bool ADV, bool TOE>
-__global__ __launch_bounds__(256) void k_heat_el(int nel, const int *__restrict__ elems, const int *__restrict__ conn,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_heat_el(int nel, const int *__restrict__ elems, const int *__restrict__ conn,
                                                  const int *__restrict__ eldof, const unsigned short *__restrict__ elpos,
                                                  const int64_t *__restrict__ elslot, const double *__restrict__ xyz, int Ntop,
                                                  const int *__restrict__ dof_top, const int *__restrict__ bc0,
@@ -316,20 +316,33 @@ __global__ __launch_bounds__(256) void k_heat_el(int nel, const int *__restrict_
                                                  const double *__restrict__ vel, const FeTab *__restrict__ tab, HeatArgs a,
                                                  int mode, int trans, double *__restrict__ val, double *__restrict__ rhs,
                                                  double *__restrict__ E, double *__restrict__ Fe) {
-  constexpr int N2 = DIM == 2 ? 9 : 27, NG = N2, DD = DIM * DIM, NC = DIM == 2 ? 3 : 6;
+  constexpr int N2 = DIM == 2 ? 9 : 27, NG = N2, DD = DIM * DIM, NC = DIM == 2 ? 4 : 6;
+  // node-indexed LDS rows padded to an even length (zeros beyond N2): 16-byte loads of node pairs
+  constexpr int NP = N2 + 1;
+  // lane tiles: rows TI ti .. TI ti + TI - 1, columns TJ tj .. TJ tj + TJ - 1 (j < N2)
   constexpr int TI = DIM == 2 ? 1 : 3, TJ = DIM == 2 ? 3 : 4, NTI = N2 / TI, NTJ = (N2 + TJ - 1) / TJ;
   constexpr int NW = 4;  // waves per block
-  __shared__ double s_R[NG * DIM * N2];  // [q][k][s]
-  __shared__ double s_phi[NG * N2];      // [q][s]
-  __shared__ double s_xx[NW][DIM * N2], s_vn[NW][ADV ? DIM * N2 : 1], s_Tn[NW][N2];
-  __shared__ double s_jb[NW][NG * DD], s_C[NW][NG * NC], s_vt[NW][ADV ? NG * DIM : 1], s_JxW[NW][NG], s_src[NW][NG];
+  __shared__ __align__(16) double s_R[NG * DIM * NP];  // [q][k][s]
+  __shared__ __align__(16) double s_phi[NG * NP];      // [q][s]
+  __shared__ __align__(16) double s_xx[NW][DIM * NP];
+  __shared__ __align__(16) double s_vn[NW][ADV ? DIM * NP : 2];
+  __shared__ __align__(16) double s_Tn[NW][NP];
+  __shared__ __align__(16) double s_jb[NW][NG * DD];
+  __shared__ __align__(16) double s_C[NW][NG * NC];
+  __shared__ __align__(16) double s_vt[NW][ADV ? NG * DIM : 2];
+  __shared__ double s_JxW[NW][NG], s_src[NW][NG];
   __shared__ int s_bcv[NW][N2];
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
-  for (int t = tid; t < NG * DIM * N2; t += blockDim.x) {
-    const int q = t / (DIM * N2), k = (t / N2) % DIM, sn = t % N2;
-    s_R[t] = tab->dphi2[sn * NG + q + k * N2 * NG];
+  for (int t = tid; t < NG * DIM * NP; t += blockDim.x) {
+    const int q = t / (DIM * NP), k = (t / NP) % DIM, sn = t % NP;
+    s_R[t] = sn < N2 ? tab->dphi2[sn * NG + q + k * N2 * NG] : 0.;
   }
-  for (int t = tid; t < NG * N2; t += blockDim.x) s_phi[t] = tab->phi2[(t % N2) * NG + t / N2];
+  for (int t = tid; t < NG * NP; t += blockDim.x) s_phi[t] = t % NP < N2 ? tab->phi2[(t % NP) * NG + t / NP] : 0.;
+  for (int t = tid; t < NW * DIM; t += blockDim.x) {
+    s_xx[t / DIM][(t % DIM) * NP + N2] = 0.;
+    if (ADV) s_vn[t / DIM][(t % DIM) * NP + N2] = 0.;
+  }
+  if (tid < NW) s_Tn[tid][N2] = 0.;
   __syncthreads();
   const double tcdt = a.tc * a.rhocp / a.dt;
   double *xx = s_xx[wv], *vn = s_vn[wv], *Tn = s_Tn[wv], *jb = s_jb[wv], *Cq = s_C[wv], *vt = s_vt[wv];
@@ -338,6 +351,18 @@ __global__ __launch_bounds__(256) void k_heat_el(int nel, const int *__restrict_
   const int ti = lane / NTJ, tj = lane % NTJ;
   const bool tile = lane < NTI * NTJ;
   const int nwaves = gridDim.x * NW;
+  // sum_s a[s] b[s] over the padded node rows, two nodes per 16-byte load
+  auto dotp = [](const double *x, const double *y) {
+    const double2 *x2 = (const double2 *)x, *y2 = (const double2 *)y;
+    double t0 = 0., t1 = 0.;
+#pragma unroll 7
+    for (int p = 0; p < NP / 2; p++) {
+      const double2 u = x2[p], w = y2[p];
+      t0 += u.x * w.x;
+      t1 += u.y * w.y;
+    }
+    return t0 + t1;
+  };
   for (int ei = xcd_block() * NW + wv; ei < nel; ei += nwaves) {
     const int e = elems ? elems[ei] : ei;
     // slots of this lane's entries and their current values (scatter into A)
@@ -348,7 +373,7 @@ __global__ __launch_bounds__(256) void k_heat_el(int nel, const int *__restrict_
       for (int u = 0; u < TI; u++)
 #pragma unroll
         for (int v = 0; v < TJ; v++) {
-          const int i = ti + NTI * u, j = TJ * tj + v;
+          const int i = TI * ti + u, j = TJ * tj + v;
           pp[u][v] = j < N2 ? el_slot(elslot, elpos, e, N2, trans ? j : i, trans ? i : j) : -1;
           pv[u][v] = pp[u][v] >= 0 ? val[pp[u][v]] : 0.;
         }
@@ -357,21 +382,18 @@ __global__ __launch_bounds__(256) void k_heat_el(int nel, const int *__restrict_
       const int node = conn[(int64_t)e * N2 + lane];
       const int kd = dof_top[node];
       for (int d = 0; d < DIM; d++) {
-        xx[lane + d * N2] = xyz[node + (int64_t)d * Ntop];
-        if (ADV) vn[lane + d * N2] = vel ? vel[node + (int64_t)d * Ntop] : a.vel_const[d];
+        xx[lane + d * NP] = xyz[node + (int64_t)d * Ntop];
+        if (ADV) vn[lane + d * NP] = vel ? vel[node + (int64_t)d * Ntop] : a.vel_const[d];
       }
       Tn[lane] = x_old[kd];
       bcv[lane] = bc0[kd];
     }
     wave_lds_sync();
     // 1. Jacobian entries J[q][c][k] = sum_s x_c,s dphi_s/dxi_k (q)
+#pragma unroll 1
     for (int p = lane; p < NG * DD; p += 64) {
       const int q = p / DD, c = (p / DIM) % DIM, k = p % DIM;
-      const double *R = s_R + (q * DIM + k) * N2, *x = xx + c * N2;
-      double J = 0.;
-#pragma unroll 9
-      for (int sn = 0; sn < N2; sn++) J += x[sn] * R[sn];
-      jb[p] = J;
+      jb[p] = dotp(xx + c * NP, s_R + (q * DIM + k) * NP);
     }
     wave_lds_sync();
     // 2. per Gauss point
@@ -395,25 +417,20 @@ __global__ __launch_bounds__(256) void k_heat_el(int nel, const int *__restrict_
         M[2][2] = (J[0] * J[4] - J[1] * J[3]) * id;
       }
       const double JxW = det * tab->w[q];
-      const double *ph = s_phi + q * N2;
-      double Tg = 0.;
-#pragma unroll 9
-      for (int sn = 0; sn < N2; sn++) Tg += ph[sn] * Tn[sn];
+      const double *ph = s_phi + q * NP;
+      const double Tg = dotp(ph, Tn);
       double w[DIM];
       const double ke = coef ? coef[e] : 1.0;
       for (int d = 0; d < DIM; d++) w[d] = JxW * a.alpha0 * ke;
       if (ADV) {
         double vg[DIM];
-        for (int d = 0; d < DIM; d++) vg[d] = 0.;
-#pragma unroll 9
-        for (int sn = 0; sn < N2; sn++)
-          for (int d = 0; d < DIM; d++) vg[d] += ph[sn] * vn[sn + d * N2];
+        for (int d = 0; d < DIM; d++) vg[d] = dotp(ph, vn + d * NP);
         double mod2 = 1.e-20;
         for (int d = 0; d < DIM; d++) mod2 += vg[d] * vg[d];
         mod2 = sqrt(mod2);
         for (int d = 0; d < DIM; d++) {
-          const double h = fabs(xx[d * N2 + 2 + 4 * (DIM - 2)] - xx[d * N2 + 0]);
-          const double h1 = fabs(xx[d * N2 + 3 + 4 * (DIM - 2)] - xx[d * N2 + 1]);
+          const double h = fabs(xx[d * NP + 2 + 4 * (DIM - 2)] - xx[d * NP + 0]);
+          const double h1 = fabs(xx[d * NP + 3 + 4 * (DIM - 2)] - xx[d * NP + 1]);
           const double h_eff = h < h1 ? h1 : h;
           const double Pe = 0.5 * mod2 * h_eff / a.alpha0;
           const double fup = a.upwind * a.rhocp * 0.5 * (1. / tanh(Pe) - 1. / Pe) * h_eff / mod2;
@@ -425,6 +442,7 @@ __global__ __launch_bounds__(256) void k_heat_el(int nel, const int *__restrict_
           vt[q * DIM + k] = a.rhocp * t;
         }
       }
+      // C_q in (00, 01, 02, 11, 12, 22) order (2D: 00, 01, 11 and a zero pad)
       int c = 0;
       for (int k = 0; k < DIM; k++)
         for (int l = k; l < DIM; l++) {
@@ -432,6 +450,7 @@ __global__ __launch_bounds__(256) void k_heat_el(int nel, const int *__restrict_
           for (int d = 0; d < DIM; d++) t += w[d] * M[d][k] * M[d][l];
           Cq[q * NC + c++] = t;
         }
+      if (DIM == 2) Cq[q * NC + 3] = 0.;
       JxWq[q] = JxW;
       srcq[q] = JxW * (tcdt * Tg + a.source);
     }
@@ -445,29 +464,40 @@ __global__ __launch_bounds__(256) void k_heat_el(int nel, const int *__restrict_
         for (int v = 0; v < TJ; v++) K[u][v] = 0.;
 #pragma unroll 1
       for (int q = 0; q < NG; q++) {
-        const double *R = s_R + q * DIM * N2;
+        const double *R = s_R + q * DIM * NP;
         double C[DIM][DIM];
         {
+          const double2 *c2 = (const double2 *)(Cq + q * NC);
+          double cc[NC];
+#pragma unroll
+          for (int t = 0; t < NC / 2; t++) { const double2 u = c2[t]; cc[2 * t] = u.x; cc[2 * t + 1] = u.y; }
           int c = 0;
+#pragma unroll
           for (int k = 0; k < DIM; k++)
-            for (int l = k; l < DIM; l++) { C[k][l] = Cq[q * NC + c]; C[l][k] = C[k][l]; c++; }
+#pragma unroll
+            for (int l = k; l < DIM; l++) { C[k][l] = cc[c]; C[l][k] = cc[c]; c++; }
         }
         double rc[TJ][DIM];
 #pragma unroll
-        for (int v = 0; v < TJ; v++) {
-          const int j = TJ * tj + v;
+        for (int k = 0; k < DIM; k++) {
+          const double *Rk = R + k * NP + TJ * tj;
+          if (TJ == 4) {
+            const double2 u0 = ((const double2 *)Rk)[0], u1 = ((const double2 *)Rk)[1];
+            rc[0][k] = u0.x; rc[1][k] = u0.y; rc[2 % TJ][k] = u1.x; rc[3 % TJ][k] = u1.y;
+          } else {
 #pragma unroll
-          for (int k = 0; k < DIM; k++) rc[v][k] = j < N2 ? R[k * N2 + j] : 0.;
+            for (int v = 0; v < TJ; v++) rc[v][k] = Rk[v];
+          }
         }
 #pragma unroll
         for (int u = 0; u < TI; u++) {
-          const int i = ti + NTI * u;
+          const int i = TI * ti + u;
           double H[DIM];
 #pragma unroll
           for (int l = 0; l < DIM; l++) H[l] = 0.;
 #pragma unroll
           for (int k = 0; k < DIM; k++) {
-            const double r = R[k * N2 + i];
+            const double r = R[k * NP + i];
 #pragma unroll
             for (int l = 0; l < DIM; l++) H[l] += C[k][l] * r;
           }
@@ -477,12 +507,11 @@ __global__ __launch_bounds__(256) void k_heat_el(int nel, const int *__restrict_
             for (int l = 0; l < DIM; l++) K[u][v] += H[l] * rc[v][l];
         }
         if (ADV) {
-          const double *ph = s_phi + q * N2;
+          const double *ph = s_phi + q * NP;
           double b[TJ];
 #pragma unroll
           for (int v = 0; v < TJ; v++) {
-            const int j = TJ * tj + v;
-            double t = j < N2 ? tcdt * ph[j] : 0.;
+            double t = tcdt * ph[TJ * tj + v];
 #pragma unroll
             for (int k = 0; k < DIM; k++) t += vt[q * DIM + k] * rc[v][k];
             b[v] = t;
@@ -490,7 +519,7 @@ __global__ __launch_bounds__(256) void k_heat_el(int nel, const int *__restrict_
           const double jw = JxWq[q];
 #pragma unroll
           for (int u = 0; u < TI; u++) {
-            const double ai = jw * ph[ti + NTI * u];
+            const double ai = jw * ph[TI * ti + u];
 #pragma unroll
             for (int v = 0; v < TJ; v++) K[u][v] += ai * b[v];
           }
@@ -499,7 +528,7 @@ __global__ __launch_bounds__(256) void k_heat_el(int nel, const int *__restrict_
       // 4. output
 #pragma unroll
       for (int u = 0; u < TI; u++) {
-        const int i = ti + NTI * u;
+        const int i = TI * ti + u;
         const bool keep = bcv[i] != 0;
 #pragma unroll
         for (int v = 0; v < TJ; v++) {
@@ -517,7 +546,7 @@ __global__ __launch_bounds__(256) void k_heat_el(int nel, const int *__restrict_
       const int i = lane;
       double F = 0.;
 #pragma unroll 9
-      for (int q = 0; q < NG; q++) F += s_phi[q * N2 + i] * srcq[q];
+      for (int q = 0; q < NG; q++) F += s_phi[q * NP + i] * srcq[q];
       if (TOE) Fe[(int64_t)e * N2 + i] = bcv[i] != 0 ? F : 0.;
       else if (bcv[i] != 0) rhs[eldof[(int64_t)e * N2 + i]] += F;
     }
@@ -527,17 +556,24 @@ __global__ __launch_bounds__(256) void k_heat_el(int nel, const int *__restrict_
 
 // Heat boundary faces: one thread per element of the colour (only elements
 // with boundary sides do work).
+// E != nullptr: the element-buffer assembly -- the face terms are added to the element's own block of E and Fe
+// (every element of the level in one launch, elems == nullptr)
 template <int DIM>
 __global__ void k_heat_bd(int nel, const int *__restrict__ elems, const int *__restrict__ conn, const int *__restrict__ eldof,
                           const unsigned short *__restrict__ elpos, const int64_t *__restrict__ elslot, const unsigned char *__restrict__ bmask,
                           const double *__restrict__ xyz, int Ntop, const int *__restrict__ dof_top,
                           const int *__restrict__ bc0, const int *__restrict__ bc1, const double *__restrict__ x_old,
                           const FeTab *__restrict__ tab, HeatArgs a, int mode, int trans, double *__restrict__ val,
-                          double *__restrict__ rhs) {
+                          double *__restrict__ rhs, double *__restrict__ E, double *__restrict__ Fe) {
   constexpr int N2 = DIM == 2 ? 9 : 27, NB = DIM == 2 ? 3 : 9, NGB = NB, NS = 2 * DIM;
   int ei = blockIdx.x * blockDim.x + threadIdx.x;
   if (ei >= nel) return;
-  int e = elems[ei];
+  int e = elems ? elems[ei] : ei;
+  auto add = [&](int row, int col, double x) {
+    if (E) { E[((int64_t)e * N2 + row) * N2 + col] += x; return; }
+    const int64_t p = el_slot(elslot, elpos, e, N2, row, col);
+    if (val && p >= 0) val[p] += x;
+  };
   unsigned char bm = bmask[e];
   if (!bm) return;
   for (int s = 0; s < NS; s++) {
@@ -556,11 +592,12 @@ __global__ void k_heat_bd(int nel, const int *__restrict__ elems, const int *__r
       double Ipen = jac_sur<DIM, NB, NGB>(NGB - 1, xb, tab) / a.dt;
       for (int k = 0; k < NB; k++) {
         int lv = sur[k];
-        int64_t p = el_slot(elslot, elpos, e, N2, lv, lv);
-        if (val && p >= 0) val[p] += Ipen;
+        if (val || E) add(lv, lv, Ipen);
         if (mode == 1) {
           int node = conn[(int64_t)e * N2 + lv];
-          rhs[eldof[(int64_t)e * N2 + lv]] += (bc_s % 2) * Ipen * x_old[dof_top[node]];
+          const double f = (bc_s % 2) * Ipen * x_old[dof_top[node]];
+          if (Fe) Fe[(int64_t)e * N2 + lv] += f;
+          else rhs[eldof[(int64_t)e * N2 + lv]] += f;
         }
       }
     } else if (bc1[ckd] != 0) {
@@ -574,10 +611,8 @@ __global__ void k_heat_bd(int nel, const int *__restrict__ elems, const int *__r
           int bc_alpha = (bcs & 2) >> 1;
           if (bc_alpha == 0) continue;
           double phii = tab->bphi2[i * NGB + qp];
-          for (int j = 0; j < NB; j++) {
-            int64_t p = el_slot(elslot, elpos, e, N2, trans ? sur[j] : lei, trans ? lei : sur[j]);
-            if (val && p >= 0) val[p] += dtxJxW * bc_alpha * a.alpha0 * phii * tab->bphi2[j * NGB + qp];
-          }
+          for (int j = 0; j < NB; j++)
+            add(trans ? sur[j] : lei, trans ? lei : sur[j], dtxJxW * bc_alpha * a.alpha0 * phii * tab->bphi2[j * NGB + qp]);
         }
       }
     }
@@ -608,13 +643,16 @@ __global__ void k_heat_bd(int nel, const int *__restrict__ elems, const int *__r
 // register tiles (1x1 in 2D), 81 per block; B^T / B entries are one per
 // thread.  Velocity rows of Dirichlet DoFs are dropped (bc_vol), pressure
 // rows are not.
-template <int DIM>
+// TOE: every entry of the element matrix (zeros included) written to E[e][row][col] (row / column swapped for
+// the transposed matrix) and the element rhs to Fe[e][row], for all elements in one launch (elems == nullptr)
+template <int DIM, bool TOE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_ns_vol(int nel, const int *__restrict__ elems, const int *__restrict__ conn,
                                                 const int *__restrict__ eldof, const unsigned short *__restrict__ elpos, const int64_t *__restrict__ elslot,
                                                 const double *__restrict__ xyz, int Ntop, const int *__restrict__ dof_top,
                                                 const int *__restrict__ bc0, const double *__restrict__ x_old,
                                                 const double *__restrict__ coef, const FeTab *__restrict__ tab, NsArgs a,
-                                                int mode, int trans, double *__restrict__ val, double *__restrict__ rhs) {
+                                                int mode, int trans, double *__restrict__ val, double *__restrict__ rhs,
+                                                double *__restrict__ E, double *__restrict__ Fe) {
   constexpr int N2 = DIM == 2 ? 9 : 27, N1 = DIM == 2 ? 4 : 8, NG = N2, NL = DIM * N2 + N1, TI = DIM == 2 ? 1 : 3;
   __shared__ double s_G[NG * DIM * N2];  // [q][d][i]
   __shared__ double s_phi[N2 * NG], s_psi[N1 * NG];
@@ -630,7 +668,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   for (int t = tid; t < N2 * NG; t += blockDim.x) s_phi[t] = tab->phi2[t];
   for (int t = tid; t < N1 * NG; t += blockDim.x) s_psi[t] = tab->phi1[t];
   for (int ei = xcd_block(); ei < nel; ei += gridDim.x) {
-    const int e = elems[ei];
+    const int e = elems ? elems[ei] : ei;
+    // entry (row, col) of the element matrix: E (zero in Dirichlet rows) or added to A through the slot map
+    auto put = [&](int row, int col, double x, bool keep) {
+      if (TOE) {
+        E[((int64_t)e * NL + (trans ? col : row)) * NL + (trans ? row : col)] = keep ? x : 0.;
+      } else if (keep) {
+        const int64_t p = el_slot(elslot, elpos, e, NL, trans ? col : row, trans ? row : col);
+        if (p >= 0) val[p] += x;
+      }
+    };
     __syncthreads();
     if (tid < N2) {
       int node = conn[(int64_t)e * N2 + tid];
@@ -803,13 +850,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 #pragma unroll
       for (int x = 0; x < TI; x++) {
         const int row = iv * N2 + ti + 9 * x;
-        if (s_bcv[row] == 0) continue;
+        const bool keep = s_bcv[row] != 0;
+        if (!TOE && !keep) continue;
 #pragma unroll
-        for (int y = 0; y < TI; y++) {
-          const int col = jv * N2 + tj + 9 * y;
-          const int64_t p = el_slot(elslot, elpos, e, NL, trans ? col : row, trans ? row : col);
-          if (p >= 0) val[p] += K[x][y];
-        }
+        for (int y = 0; y < TI; y++) put(row, jv * N2 + tj + 9 * y, K[x][y], keep);
       }
     }
     // B^T (velocity rows, pressure columns) and B (pressure rows, velocity columns)
@@ -817,11 +861,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     for (int u = tid; u < 2 * NBT; u += blockDim.x) {
       int row, col;
       double K = 0.;
+      bool keep = true;
       if (u < NBT) {
         row = u / N1;
         const int j = u % N1, iv = row / N2, i = row % N2;
-        if (s_bcv[row] == 0) continue;
         col = DIM * N2 + j;
+        keep = s_bcv[row] != 0;
+        if (!keep) {
+          if (TOE) put(row, col, 0., false);
+          continue;
+        }
 #pragma unroll 3
         for (int q = 0; q < NG; q++) K -= s_JxW[q] * s_psi[j * NG + q] * s_G[(q * DIM + iv) * N2 + i];
       } else {
@@ -832,21 +881,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 #pragma unroll 3
         for (int q = 0; q < NG; q++) K += s_JxW[q] * s_psi[ip * NG + q] * s_G[(q * DIM + jv) * N2 + j];
       }
-      const int64_t p = el_slot(elslot, elpos, e, NL, trans ? col : row, trans ? row : col);
-      if (p >= 0) val[p] += K;
+      put(row, col, K, keep);
     }
+    // the pressure-pressure block is zero (no stabilisation)
+    if (TOE)
+      for (int u = tid; u < N1 * N1; u += blockDim.x) put(DIM * N2 + u / N1, DIM * N2 + u % N1, 0., false);
     if (mode == 1 && tid < DIM * N2) {
       const int iv = tid / N2, i = tid % N2;
-      if (s_bcv[i + iv * N2] != 0) {
-        double F = 0.;
+      const bool keep = s_bcv[i + iv * N2] != 0;
+      double F = 0.;
+      if (keep) {
         for (int qp = 0; qp < NG; qp++) {
           const double phii = s_phi[i * NG + qp];
           F += s_JxW[qp] * (tcdt * s_vel[qp * DIM + iv] * phii + a.IFr * a.dirg[iv] * phii + a.src[iv] * phii +
                             a.buoy[iv] * s_Tq[qp] * phii);
         }
-        rhs[eldof[(int64_t)e * NL + tid]] += F;
       }
+      if (TOE) Fe[(int64_t)e * NL + tid] = F;
+      else if (keep) rhs[eldof[(int64_t)e * NL + tid]] += F;
     }
+    if (TOE && mode == 1 && tid < N1) Fe[(int64_t)e * NL + DIM * N2 + tid] = 0.;
   }
 }
 
@@ -856,7 +910,7 @@ __global__ void k_ns_bd(int nel, const int *__restrict__ elems, const int *__res
                         const unsigned short *__restrict__ elpos, const int64_t *__restrict__ elslot, const unsigned char *__restrict__ bmask, const double *__restrict__ xyz,
                         int Ntop, const int *__restrict__ dof_top, const int *__restrict__ bc0, const int *__restrict__ bc1,
                         const double *__restrict__ x_old, const FeTab *__restrict__ tab, NsArgs a, int mode, int trans,
-                        double *__restrict__ val, double *__restrict__ rhs) {
+                        double *__restrict__ val, double *__restrict__ rhs, double *__restrict__ E, double *__restrict__ Fe) {
   constexpr int N2 = DIM == 2 ? 9 : 27, N1 = DIM == 2 ? 4 : 8, NB = DIM == 2 ? 3 : 9, NGB = NB, NS = 2 * DIM;
   constexpr int NL = DIM * N2 + N1;
   // one thread per (element, velocity component): a thread writes only the rows of its own component
@@ -865,9 +919,14 @@ __global__ void k_ns_bd(int nel, const int *__restrict__ elems, const int *__res
   const int tix = blockIdx.x * blockDim.x + threadIdx.x;
   const int ei = tix / DIM, iv = tix % DIM;
   if (ei >= nel) return;
-  int e = elems[ei];
+  int e = elems ? elems[ei] : ei;
   unsigned char bm = bmask[e];
   if (!bm) return;
+  auto add = [&](int row, int col, double x) {
+    if (E) { E[((int64_t)e * NL + row) * NL + col] += x; return; }
+    const int64_t p = el_slot(elslot, elpos, e, NL, row, col);
+    if (p >= 0) val[p] += x;
+  };
   for (int s = 0; s < NS; s++) {
     if (!((bm >> s) & 1)) continue;
     int sur[NB], gn[NB];
@@ -902,19 +961,20 @@ __global__ void k_ns_bd(int nel, const int *__restrict__ elems, const int *__res
           int bc_rhs = (bcs & 4) >> 2, bc_tg = (bcs & 2) >> 1, bc_n = bcs % 2;
           double vo[DIM], vdotn = 0.;
           for (int k = 0; k < DIM; k++) { vo[k] = x_old[dof_top[gn[i] + k * Ntop]]; vdotn += nrm[k] * vo[k]; }
-          if (mode == 1)
-            rhs[eldof[(int64_t)e * NL + row]] += bc_rhs * Ipen * ((1 - bc_n) * (1 - bc_tg) * vo[iv] + bc_n * vdotn * nrm[iv] +
-                                                                    bc_tg * (vo[iv] - vdotn * nrm[iv]));
-          if (!val) continue;
-          int64_t p = el_slot(elslot, elpos, e, NL, row, row);
-          if (p >= 0) val[p] += ((1 - bc_n) * (1 - bc_tg) + bc_tg) * Ipen + Ipen * nrm[iv] * nrm[iv] * (bc_n - bc_tg);
+          if (mode == 1) {
+            const double f = bc_rhs * Ipen * ((1 - bc_n) * (1 - bc_tg) * vo[iv] + bc_n * vdotn * nrm[iv] +
+                                              bc_tg * (vo[iv] - vdotn * nrm[iv]));
+            if (Fe) Fe[(int64_t)e * NL + row] += f;
+            else rhs[eldof[(int64_t)e * NL + row]] += f;
+          }
+          if (!val && !E) continue;
+          add(row, row, ((1 - bc_n) * (1 - bc_tg) + bc_tg) * Ipen + Ipen * nrm[iv] * nrm[iv] * (bc_n - bc_tg));
           for (int jv = iv + 1; jv < DIM; jv++) {
             const int col = sur[i] + jv * N2;
-            int64_t pc = el_slot(elslot, elpos, e, NL, trans ? col : row, trans ? row : col);
-            if (pc >= 0) val[pc] += Ipen * nrm[jv] * nrm[iv] * (bc_n - bc_tg);
+            add(trans ? col : row, trans ? row : col, Ipen * nrm[jv] * nrm[iv] * (bc_n - bc_tg));
           }
         }
-      } else if (bc1[ckd] != 0 && val) {
+      } else if (bc1[ckd] != 0 && (val || E)) {
         for (int qp = 0; qp < NGB; qp++) {
           double JxW = jac_sur<DIM, NB, NGB>(qp, xb, tab) * tab->bw[qp];
           for (int i = 0; i < NB; i++) {
@@ -927,12 +987,10 @@ __global__ void k_ns_bd(int nel, const int *__restrict__ elems, const int *__res
             double phii = tab->bphi2[i * NGB + qp];
             for (int j = 0; j < NB; j++) {
               double phij = tab->bphi2[j * NGB + qp];
-              int64_t p = el_slot(elslot, elpos, e, NL, row, row);
-              if (p >= 0) val[p] += dtJxW * bc_tg * phij * phii + dtJxW * nrm[iv] * nrm[iv] * phij * phii * (bc_n - bc_tg);
+              add(row, row, dtJxW * bc_tg * phij * phii + dtJxW * nrm[iv] * nrm[iv] * phij * phii * (bc_n - bc_tg));
               for (int jv = iv + 1; jv < DIM; jv++) {
                 const int col = sur[j] + jv * N2;
-                int64_t pc = el_slot(elslot, elpos, e, NL, trans ? col : row, trans ? row : col);
-                if (pc >= 0) val[pc] += dtJxW * nrm[jv] * nrm[iv] * phij * phii * (bc_n - bc_tg);
+                add(trans ? col : row, trans ? row : col, dtJxW * nrm[jv] * nrm[iv] * phij * phii * (bc_n - bc_tg));
               }
             }
           }
@@ -1367,7 +1425,7 @@ int heat_assemble(hipStream_t s, int dim, int nel, const int *elems, const int *
                          bc0, x_old, coef, vel, tab, a, mode, trans, val, rhs, nullptr, nullptr);
     }
     hipLaunchKernelGGL(k_heat_bd<2>, dim3((nel + 127) / 128), dim3(128), 0, s, nel, elems, conn, eldof, elpos, elslot, bmask,
-                       xyz, Ntop, dof_top, bc0, bc1, xb, tab, a, mode, trans, val, rhs);
+                       xyz, Ntop, dof_top, bc0, bc1, xb, tab, a, mode, trans, val, rhs, nullptr, nullptr);
   } else {
     if (!bd_only) {
       auto kern = adv ? k_heat_el<3, true, false> : k_heat_el<3, false, false>;
@@ -1375,7 +1433,7 @@ int heat_assemble(hipStream_t s, int dim, int nel, const int *elems, const int *
                          bc0, x_old, coef, vel, tab, a, mode, trans, val, rhs, nullptr, nullptr);
     }
     hipLaunchKernelGGL(k_heat_bd<3>, dim3((nel + 127) / 128), dim3(128), 0, s, nel, elems, conn, eldof, elpos, elslot, bmask,
-                       xyz, Ntop, dof_top, bc0, bc1, xb, tab, a, mode, trans, val, rhs);
+                       xyz, Ntop, dof_top, bc0, bc1, xb, tab, a, mode, trans, val, rhs, nullptr, nullptr);
   }
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
@@ -1388,17 +1446,163 @@ int ns_assemble(hipStream_t s, int dim, int nel, const int *elems, const int *co
   const double *xb = xbc ? xbc : x_old;
   if (dim == 2) {
     if (!bd_only)
-      hipLaunchKernelGGL(k_ns_vol<2>, dim3(nblk(nel)), dim3(256), 0, s, nel, elems, conn, eldof, elpos, elslot, xyz, Ntop,
-                         dof_top, bc0, x_old, coef, tab, a, mode, trans, val, rhs);
+      hipLaunchKernelGGL((k_ns_vol<2, false>), dim3(nblk(nel)), dim3(256), 0, s, nel, elems, conn, eldof, elpos, elslot, xyz,
+                         Ntop, dof_top, bc0, x_old, coef, tab, a, mode, trans, val, rhs, nullptr, nullptr);
     hipLaunchKernelGGL(k_ns_bd<2>, dim3((2 * nel + 127) / 128), dim3(128), 0, s, nel, elems, conn, eldof, elpos, elslot, bmask,
-                       xyz, Ntop, dof_top, bc0, bc1, xb, tab, a, mode, trans, val, rhs);
+                       xyz, Ntop, dof_top, bc0, bc1, xb, tab, a, mode, trans, val, rhs, nullptr, nullptr);
   } else {
     if (!bd_only)
-      hipLaunchKernelGGL(k_ns_vol<3>, dim3(nblk(nel)), dim3(256), 0, s, nel, elems, conn, eldof, elpos, elslot, xyz, Ntop,
-                         dof_top, bc0, x_old, coef, tab, a, mode, trans, val, rhs);
+      hipLaunchKernelGGL((k_ns_vol<3, false>), dim3(nblk(nel)), dim3(256), 0, s, nel, elems, conn, eldof, elpos, elslot, xyz,
+                         Ntop, dof_top, bc0, x_old, coef, tab, a, mode, trans, val, rhs, nullptr, nullptr);
     hipLaunchKernelGGL(k_ns_bd<3>, dim3((3 * nel + 127) / 128), dim3(128), 0, s, nel, elems, conn, eldof, elpos, elslot, bmask,
-                       xyz, Ntop, dof_top, bc0, bc1, xb, tab, a, mode, trans, val, rhs);
+                       xyz, Ntop, dof_top, bc0, bc1, xb, tab, a, mode, trans, val, rhs, nullptr, nullptr);
   }
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+// ---- element-buffer assembly ------------------------------------------------------------------------------
+// All elements of the level in one launch each: the element matrices (E) and right-hand sides (Fe), then the
+// boundary faces added to the same element blocks; egather sums them into the SELL slots.
+int heat_assemble_e(hipStream_t s, int dim, int nel, const int *conn, const unsigned char *bmask, const double *xyz, int Ntop,
+                    const int *dof_top, const int *bc0, const int *bc1, const double *x_old, const double *xbc,
+                    const double *coef, const double *vel, const FeTab *tab, HeatArgs a, int mode, int trans, double *E,
+                    double *Fe) {
+  if (nel <= 0) return 0;
+  const double *xb = xbc ? xbc : x_old;
+  const bool adv = a.tc * a.rhocp != 0. || vel || a.vel_const[0] != 0. || a.vel_const[1] != 0. || a.vel_const[2] != 0.;
+  if (dim == 2) {
+    auto kern = adv ? k_heat_el<2, true, true> : k_heat_el<2, false, true>;
+    hipLaunchKernelGGL(kern, dim3(nblk_w(nel)), dim3(256), 0, s, nel, nullptr, conn, nullptr, nullptr, nullptr, xyz, Ntop,
+                       dof_top, bc0, x_old, coef, vel, tab, a, mode, trans, nullptr, nullptr, E, Fe);
+    hipLaunchKernelGGL(k_heat_bd<2>, dim3((nel + 127) / 128), dim3(128), 0, s, nel, nullptr, conn, nullptr, nullptr, nullptr,
+                       bmask, xyz, Ntop, dof_top, bc0, bc1, xb, tab, a, mode, trans, nullptr, nullptr, E, Fe);
+  } else {
+    auto kern = adv ? k_heat_el<3, true, true> : k_heat_el<3, false, true>;
+    hipLaunchKernelGGL(kern, dim3(nblk_w(nel)), dim3(256), 0, s, nel, nullptr, conn, nullptr, nullptr, nullptr, xyz, Ntop,
+                       dof_top, bc0, x_old, coef, vel, tab, a, mode, trans, nullptr, nullptr, E, Fe);
+    hipLaunchKernelGGL(k_heat_bd<3>, dim3((nel + 127) / 128), dim3(128), 0, s, nel, nullptr, conn, nullptr, nullptr, nullptr,
+                       bmask, xyz, Ntop, dof_top, bc0, bc1, xb, tab, a, mode, trans, nullptr, nullptr, E, Fe);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int ns_assemble_e(hipStream_t s, int dim, int nel, const int *conn, const unsigned char *bmask, const double *xyz, int Ntop,
+                  const int *dof_top, const int *bc0, const int *bc1, const double *x_old, const double *xbc,
+                  const double *coef, const FeTab *tab, NsArgs a, int mode, int trans, double *E, double *Fe) {
+  if (nel <= 0) return 0;
+  const double *xb = xbc ? xbc : x_old;
+  if (dim == 2) {
+    hipLaunchKernelGGL((k_ns_vol<2, true>), dim3(nblk(nel)), dim3(256), 0, s, nel, nullptr, conn, nullptr, nullptr, nullptr,
+                       xyz, Ntop, dof_top, bc0, x_old, coef, tab, a, mode, trans, nullptr, nullptr, E, Fe);
+    hipLaunchKernelGGL(k_ns_bd<2>, dim3((2 * nel + 127) / 128), dim3(128), 0, s, nel, nullptr, conn, nullptr, nullptr, nullptr,
+                       bmask, xyz, Ntop, dof_top, bc0, bc1, xb, tab, a, mode, trans, nullptr, nullptr, E, Fe);
+  } else {
+    hipLaunchKernelGGL((k_ns_vol<3, true>), dim3(nblk(nel)), dim3(256), 0, s, nel, nullptr, conn, nullptr, nullptr, nullptr,
+                       xyz, Ntop, dof_top, bc0, x_old, coef, tab, a, mode, trans, nullptr, nullptr, E, Fe);
+    hipLaunchKernelGGL(k_ns_bd<3>, dim3((3 * nel + 127) / 128), dim3(128), 0, s, nel, nullptr, conn, nullptr, nullptr, nullptr,
+                       bmask, xyz, Ntop, dof_top, bc0, bc1, xb, tab, a, mode, trans, nullptr, nullptr, E, Fe);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+// One block per row block (lg lanes of one slice).  Everything the block reads is staged in LDS first with
+// independent loads -- its element rows (E), its contribution stream (cidx), the count and stream start of each
+// of its slots -- so a block waits for memory once; then every slot is the sum of its contributions in element
+// order, written in FP64 and FP32 (each group of lg lanes at one position is contiguous), and the rhs of each row
+// the sum of its element rows' Fe entries.  LDS layout (bytes): rows [0, oc), stream [oc, os), starts [os, on),
+// counts [on, on + W lg), then the block's element row ids (8-byte aligned).
+__global__ __launch_bounds__(256) void k_egather(int lg, int nl, int oc, int os, int on, const int64_t *__restrict__ sl_off,
+                                                 const int *__restrict__ sl_w, const int *__restrict__ perm,
+                                                 const int64_t *__restrict__ lseg, const int *__restrict__ seg,
+                                                 const int64_t *__restrict__ bbase, const unsigned char *__restrict__ cnt,
+                                                 const unsigned short *__restrict__ cst, const unsigned short *__restrict__ cidx,
+                                                 const double *__restrict__ E, const double *__restrict__ Fe,
+                                                 double *__restrict__ v, float *__restrict__ vf, double *__restrict__ rhs) {
+  extern __shared__ double smem[];
+  unsigned char *sb = (unsigned char *)smem;
+  double *se = smem;
+  unsigned short *sc = (unsigned short *)(sb + oc), *ss = (unsigned short *)(sb + os);
+  unsigned char *sn = sb + on;
+  const int64_t g = xcd_block();  // the row blocks of a slice on one XCD: their partial lines meet in one L2
+  const int gpb = 64 / lg;
+  const int64_t sl = g / gpb;
+  const int l0 = (int)(g % gpb) * lg;
+  const int64_t s0 = lseg[g * lg], s1 = lseg[(g + 1) * lg];
+  const int nseg = (int)(s1 - s0);
+  const int64_t c0 = bbase[g], c1 = bbase[g + 1];
+  const int nc = (int)(c1 - c0);
+  const int W = sl_w[sl], nit = W * lg;
+  const int64_t off = sl_off[sl];
+  const int tid = threadIdx.x;
+  // the loads of each stage are issued in batches of 8 per thread before their LDS stores
+  constexpr int B = 8, NT = 256;
+  int *sg = (int *)(sb + on + ((nit + 7) / 8) * 8);  // element row ids (after the counts)
+  for (int t0 = tid; t0 < nseg; t0 += B * NT) {
+    int r[B];
+#pragma unroll
+    for (int u = 0; u < B; u++) r[u] = t0 + u * NT < nseg ? seg[s0 + t0 + u * NT] : 0;
+#pragma unroll
+    for (int u = 0; u < B; u++) if (t0 + u * NT < nseg) sg[t0 + u * NT] = r[u];
+  }
+  for (int t0 = tid; t0 < nc; t0 += B * NT) {
+    unsigned short r[B];
+#pragma unroll
+    for (int u = 0; u < B; u++) r[u] = t0 + u * NT < nc ? cidx[c0 + t0 + u * NT] : 0;
+#pragma unroll
+    for (int u = 0; u < B; u++) if (t0 + u * NT < nc) sc[t0 + u * NT] = r[u];
+  }
+  for (int t0 = tid; t0 < nit; t0 += B * NT) {
+    unsigned short r[B];
+    unsigned char q[B];
+#pragma unroll
+    for (int u = 0; u < B; u++) {
+      const int t = t0 + u * NT;
+      const int64_t slot = off + 64 * (int64_t)(t / lg) + l0 + t % lg;
+      r[u] = t < nit ? cst[slot] : 0;
+      q[u] = t < nit ? cnt[slot] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < B; u++) if (t0 + u * NT < nit) { ss[t0 + u * NT] = r[u]; sn[t0 + u * NT] = q[u]; }
+  }
+  __syncthreads();
+  for (int t0 = tid; t0 < nseg * nl; t0 += B * NT) {
+    double r[B];
+#pragma unroll
+    for (int u = 0; u < B; u++) {
+      const int t = t0 + u * NT;
+      r[u] = t < nseg * nl ? E[(int64_t)sg[t / nl] * nl + t % nl] : 0.;
+    }
+#pragma unroll
+    for (int u = 0; u < B; u++) if (t0 + u * NT < nseg * nl) se[t0 + u * NT] = r[u];
+  }
+  if (rhs && tid < lg) {
+    const int row = perm[64 * sl + l0 + tid];
+    if (row >= 0) {
+      double f = 0.;
+      for (int64_t q = lseg[g * lg + tid]; q < lseg[g * lg + tid + 1]; q++) f += Fe[sg[q - s0]];
+      rhs[row] = f;
+    }
+  }
+  __syncthreads();
+#pragma unroll 4
+  for (int t = tid; t < nit; t += NT) {
+    const int64_t slot = off + 64 * (int64_t)(t / lg) + l0 + t % lg;
+    const int c = sn[t], st = ss[t];
+    double sum = 0.;
+    for (int m = 0; m < c; m++) sum += se[sc[st + m]];
+    v[slot] = sum;
+    vf[slot] = (float)sum;
+  }
+}
+
+int egather(hipStream_t s, int64_t nblk, int lg, int nl, int oc, int os, int on, size_t lds, const int64_t *sl_off,
+            const int *sl_w, const int *perm, const int64_t *lseg, const int *seg, const int64_t *bbase,
+            const unsigned char *cnt, const unsigned short *cst, const unsigned short *cidx, const double *E,
+            const double *Fe, double *v, float *vf, double *rhs) {
+  if (nblk <= 0) return 0;
+  if (nblk > 0x7fffffff) return -1;
+  hipLaunchKernelGGL(k_egather, dim3((unsigned)nblk), dim3(256), lds, s, lg, nl, oc, os, on, sl_off, sl_w, perm, lseg, seg,
+                     bbase, cnt, cst, cidx, E, Fe, v, vf, rhs);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

@@ -21,6 +21,17 @@ void fm_set_error(const std::string &msg) { g_err = msg; g_nerr++; }
 unsigned fm_error_count() { return g_nerr; }
 extern "C" const char *fm_last_error(void) { return g_err.c_str(); }
 extern "C" int fm_set_sell_split(int W) { return fmk::sell_split(W); }
+
+// Element-buffer assembly (element matrices to a buffer, then a gather into the SELL slots) for the levels whose
+// buffers fit the budget; 0 keeps every level on the coloured scatter.
+static int g_asm_gather = 1;
+static constexpr size_t kGatherBudget = (size_t)12 << 30;  // element buffers + gather index of one system
+extern "C" int fm_set_assembly_gather(int on) {
+  if (on != 0 && on != 1) return fm_einval(__func__);
+  const int old = g_asm_gather;
+  g_asm_gather = on;
+  return old;
+}
 // narrowest column-offset class of SELL copies built from now on (FEMUS_SELL_C8=0 starts it at 16)
 static int g_sell_minbits = -1;
 static int sell_minbits() {
@@ -461,6 +472,7 @@ extern "C" int fm_system_destroy(fm_system *s) {
   fm_krylov_free(s);
   for (auto &l : s->lev) fm_free_level(l);
   for (auto &l : s->alev) fm_free_level(l);
+  dfree(s->ebuf); dfree(s->febuf);
   dfree(s->d_bc0); dfree(s->d_bc1); dfree(s->d_dof_top); dfree(s->xyz); dfree(s->x_old); dfree(s->vel); dfree(s->xbc);
   dfree(s->tnode);
   dfree(s->d_tab);
@@ -566,6 +578,7 @@ void fm_free_level(Level &l) {
   dfree(l.ord); dfree(l.rank);
   dfree(l.Ainv); dfree(l.Ainvf); dfree(l.vinv); dfree(l.vinvf); dfree(l.vwt); dfree(l.vwtf); dfree(l.nrm); dfree(l.res);
   dfree(l.sb); dfree(l.sx); dfree(l.vone); dfree(l.cslist); dfree(l.dslot);
+  fm_free_egather(l.eg);
 }
 
 // Per colour: the SELL slices of the row groups whose signature contains the
@@ -728,6 +741,7 @@ extern "C" int fm_system_init(fm_system *s) {
     for (size_t q = 0; q < eld.size(); q++) eldi[q] = eld[q] >= 0 ? l.h_rank[eld[q]] : -1;
     dfree(l.conn); dfree(l.eldof); dfree(l.eldofi); dfree(l.elpos); dfree(l.elslot); dfree(l.bmask);
     dfree(l.color_elems);
+    fm_free_egather(l.eg);  // built from the slot map and the SELL layout made here
     FM_CHECK(dalloc(&l.conn, (size_t)l.nel * m.n2));
     FM_CHECK(dalloc(&l.eldof, (size_t)l.nel * nloc));
     FM_CHECK(dalloc(&l.eldofi, (size_t)l.nel * nloc));
@@ -1036,11 +1050,105 @@ extern "C" int fm_system_advance(fm_system *s, double *rel_change) {
   return FM_OK;
 }
 
-// refresh FP32 values and diagonal inverses after an assembly
-static int after_assembly(fm_system *s, int L) {
+void fm_free_egather(EGather &g) {
+  for (void *p : {(void *)g.lseg, (void *)g.bbase, (void *)g.seg, (void *)g.cnt, (void *)g.cst, (void *)g.cidx})
+    if (p) (void)hipFree(p);
+  g = EGather();
+}
+
+template <typename T>
+static int eg_upload(hipStream_t st, T **d, const std::vector<T> &h) {
+  if (hipMalloc((void **)d, std::max<size_t>(h.size(), 1) * sizeof(T)) != hipSuccess) return -1;
+  if (!h.empty() && hipMemcpyAsync(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st) != hipSuccess) return -1;
+  return 0;
+}
+
+int fm_build_egather(hipStream_t st, Level &l, int nl, size_t max_lds, EGather &g, std::string &err) {
+  const DevSELL &S = l.A.sell;
+  const int64_t nrow_el = (int64_t)l.nel * nl;
+  std::vector<unsigned short> pos((size_t)nrow_el * nl);
+  std::vector<int64_t> slot0(nrow_el), off(S.nsl + 1);
+  std::vector<int> wd(S.nsl);
+  if (hipMemcpyAsync(pos.data(), l.elpos, pos.size() * sizeof(unsigned short), hipMemcpyDeviceToHost, st) ||
+      hipMemcpyAsync(slot0.data(), l.elslot, slot0.size() * sizeof(int64_t), hipMemcpyDeviceToHost, st) ||
+      hipMemcpyAsync(off.data(), S.sl_off, off.size() * sizeof(int64_t), hipMemcpyDeviceToHost, st) ||
+      hipMemcpyAsync(wd.data(), S.sl_w, wd.size() * sizeof(int), hipMemcpyDeviceToHost, st) || hipStreamSynchronize(st)) {
+    err = "element-buffer gather: download of the slot map failed";
+    return -1;
+  }
+  if (off.back() != S.nslot) { err = "element-buffer gather: slice offsets do not cover the slots"; return -1; }
+  EGatherPlan p;
+  if (egather_plan(l.nel, nl, pos, slot0, off, wd, max_lds, p, err)) return -1;
+  g = EGather();
+  g.lg = p.lg; g.nl = nl; g.maxseg = p.maxseg; g.maxctr = p.maxctr; g.maxit = p.maxit;
+  g.nblk = p.nblk; g.nseg = p.nseg; g.nctr = p.nctr;
+  // bbase holds nblk + 1 offsets: a block reads its own and the next one (the end of its stream)
+  if (eg_upload(st, &g.lseg, p.lseg) || eg_upload(st, &g.bbase, p.bbase) || eg_upload(st, &g.seg, p.seg) ||
+      eg_upload(st, &g.cnt, p.cnt) || eg_upload(st, &g.cst, p.cst) || eg_upload(st, &g.cidx, p.cidx) ||
+      hipStreamSynchronize(st) != hipSuccess) {
+    fm_free_egather(g);
+    err = "element-buffer gather: upload failed";
+    return -1;
+  }
+  g.bytes = p.lseg.size() * 8 + p.bbase.size() * 8 + p.seg.size() * 4 + p.cnt.size() + p.cst.size() * 2 + p.cidx.size() * 2;
+  g.built = true;
+  return 0;
+}
+
+// Whether level L assembles through the element buffer: builds the gather at the first call, sizes the
+// system's element buffers; false (the coloured scatter) when switched off, over budget or not buildable.
+static int egather_ready(fm_system *s, int L, bool &use) {
+  use = false;
+  Level &l = s->lev[L];
+  if (!g_asm_gather || l.eg.unusable || l.nel <= 0) return 0;
+  const int nl = s->nloc;
+  const size_t ne = (size_t)l.nel * nl * nl, nf = (size_t)l.nel * nl;
+  if (!l.eg.built) {
+    size_t held = std::max(s->ebuf_n, ne) * 8 + std::max(s->febuf_n, nf) * 8;
+    for (const Level &o : s->lev) held += o.eg.bytes;
+    // index: per slot count + start, per contribution (at most one per element entry) an LDS position
+    const size_t index = (size_t)l.A.sell.nslot * 3 + ne * 2 + nf * 12;
+    if (held + index > kGatherBudget) { l.eg.unusable = true; return 0; }
+    std::string err;
+    // at most 24 KB of staging per block: six or more blocks resident per CU
+    if (fm_build_egather(s->ctx->stream, l, nl, 24 * 1024, l.eg, err)) {
+      (void)hipGetLastError();
+      l.eg.unusable = true;
+      return 0;
+    }
+  }
+  if (s->ebuf_n < ne) {
+    dfree(s->ebuf);
+    s->ebuf_n = 0;
+    FM_CHECK(dalloc(&s->ebuf, ne));
+    s->ebuf_n = ne;
+  }
+  if (s->febuf_n < nf) {
+    dfree(s->febuf);
+    s->febuf_n = 0;
+    FM_CHECK(dalloc(&s->febuf, nf));
+    s->febuf_n = nf;
+  }
+  use = true;
+  return 0;
+}
+
+// sum the element buffers of level L into the values (FP64 + FP32) of A (or of A^T: same layout) and the rhs
+static int run_egather(fm_system *s, int L, DevSELL &target, double *rhs) {
+  Level &l = s->lev[L];
+  const EGather &g = l.eg;
+  const DevSELL &S = l.A.sell;
+  size_t oc, os, on;
+  const size_t lds = egather_lds(g.maxseg, g.nl, g.maxctr, g.maxit, &oc, &os, &on);
+  return fmk::egather(s->ctx->stream, g.nblk, g.lg, g.nl, (int)oc, (int)os, (int)on, lds, S.sl_off, S.sl_w, S.perm,
+                      g.lseg, g.seg, g.bbase, g.cnt, g.cst, g.cidx, s->ebuf, s->febuf, target.v, target.vf, rhs);
+}
+
+// refresh FP32 values and diagonal inverses after an assembly (the gather writes the FP32 values itself)
+static int after_assembly(fm_system *s, int L, bool gathered = false) {
   Level &l = s->lev[L];
   hipStream_t st = s->ctx->stream;
-  FM_CHECK(fmk::cast_vals(st, l.A.sell.nslot, l.A.sell.v, l.A.sell.vf));
+  if (!gathered) FM_CHECK(fmk::cast_vals(st, l.A.sell.nslot, l.A.sell.v, l.A.sell.vf));
   FM_CHECK(fmk::dinv_slot(st, l.n, l.dslot, l.A.sell.v, l.dinv, l.dinvf));
   l.lam = 0.;
   s->mg_ready = false;
@@ -1057,15 +1165,23 @@ extern "C" int fm_heat_genmatrhs(fm_system *s, int L, int mode, const fm_heat_pa
   HeatArgs a;
   a.alpha0 = p->alpha0; a.rhocp = p->rhocp; a.dt = p->dt; a.tc = p->time_coeff; a.source = p->source; a.upwind = p->upwind;
   for (int d = 0; d < 3; d++) a.vel_const[d] = p->vel_const[d];
-  FM_HIP(hipMemsetAsync(l.A.sell.v, 0, l.A.sell.nslot * sizeof(double), st));
-  if (mode == 1) FM_HIP(hipMemsetAsync(l.b, 0, l.n * sizeof(double), st));
-  for (int k = 0; k < l.ncolor; k++) {
-    int n = l.color_off[k + 1] - l.color_off[k];
-    FM_CHECK(fmk::heat_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldofi, l.elpos, l.elslot, l.bmask, s->xyz,
-                                m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, s->xbc, l.coef, s->vel, s->d_tab, a,
-                                mode, 0, 0, l.A.sell.v, l.b));
+  bool gather = false;
+  FM_CHECK(egather_ready(s, L, gather));
+  if (gather) {
+    FM_CHECK(fmk::heat_assemble_e(st, m.dim, l.nel, l.conn, l.bmask, s->xyz, m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1,
+                                  s->x_old, s->xbc, l.coef, s->vel, s->d_tab, a, mode, 0, s->ebuf, s->febuf));
+    FM_CHECK(run_egather(s, L, l.A.sell, mode == 1 ? l.b : nullptr));
+  } else {
+    FM_HIP(hipMemsetAsync(l.A.sell.v, 0, l.A.sell.nslot * sizeof(double), st));
+    if (mode == 1) FM_HIP(hipMemsetAsync(l.b, 0, l.n * sizeof(double), st));
+    for (int k = 0; k < l.ncolor; k++) {
+      int n = l.color_off[k + 1] - l.color_off[k];
+      FM_CHECK(fmk::heat_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldofi, l.elpos, l.elslot, l.bmask,
+                                  s->xyz, m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, s->xbc, l.coef, s->vel,
+                                  s->d_tab, a, mode, 0, 0, l.A.sell.v, l.b));
+    }
   }
-  FM_CHECK(after_assembly(s, L));
+  FM_CHECK(after_assembly(s, L, gather));
   FM_HIP(hipStreamSynchronize(st));
   return FM_OK;
 }
@@ -1081,15 +1197,23 @@ extern "C" int fm_ns_genmatrhs(fm_system *s, int L, int mode, const fm_ns_params
   a.IRe = p->IRe; a.dt = p->dt; a.tc = p->time_coeff; a.upwind = p->upwind; a.IFr = p->IFr;
   for (int d = 0; d < 3; d++) { a.dirg[d] = p->dirg[d]; a.src[d] = p->src[d]; a.buoy[d] = s->buoy[d]; }
   a.Tn = s->tnode;
-  FM_HIP(hipMemsetAsync(l.A.sell.v, 0, l.A.sell.nslot * sizeof(double), st));
-  if (mode == 1) FM_HIP(hipMemsetAsync(l.b, 0, l.n * sizeof(double), st));
-  for (int k = 0; k < l.ncolor; k++) {
-    int n = l.color_off[k + 1] - l.color_off[k];
-    FM_CHECK(fmk::ns_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldofi, l.elpos, l.elslot, l.bmask, s->xyz,
-                              m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, s->xbc, l.coef, s->d_tab, a, mode, 0, 0,
-                              l.A.sell.v, l.b));
+  bool gather = false;
+  FM_CHECK(egather_ready(s, L, gather));
+  if (gather) {
+    FM_CHECK(fmk::ns_assemble_e(st, m.dim, l.nel, l.conn, l.bmask, s->xyz, m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1,
+                                s->x_old, s->xbc, l.coef, s->d_tab, a, mode, 0, s->ebuf, s->febuf));
+    FM_CHECK(run_egather(s, L, l.A.sell, mode == 1 ? l.b : nullptr));
+  } else {
+    FM_HIP(hipMemsetAsync(l.A.sell.v, 0, l.A.sell.nslot * sizeof(double), st));
+    if (mode == 1) FM_HIP(hipMemsetAsync(l.b, 0, l.n * sizeof(double), st));
+    for (int k = 0; k < l.ncolor; k++) {
+      int n = l.color_off[k + 1] - l.color_off[k];
+      FM_CHECK(fmk::ns_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldofi, l.elpos, l.elslot, l.bmask,
+                                s->xyz, m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, s->xbc, l.coef, s->d_tab, a,
+                                mode, 0, 0, l.A.sell.v, l.b));
+    }
   }
-  FM_CHECK(after_assembly(s, L));
+  FM_CHECK(after_assembly(s, L, gather));
   FM_HIP(hipStreamSynchronize(st));
   return FM_OK;
 }
@@ -1277,10 +1401,10 @@ static int ensure_adjoint(fm_system *s) {
   return 0;
 }
 
-static int after_assembly_T(fm_system *s, int L) {
+static int after_assembly_T(fm_system *s, int L, bool gathered = false) {
   Level &a = s->alev[L];
   hipStream_t st = s->ctx->stream;
-  FM_CHECK(fmk::cast_vals(st, a.A.sell.nslot, a.A.sell.v, a.A.sell.vf));
+  if (!gathered) FM_CHECK(fmk::cast_vals(st, a.A.sell.nslot, a.A.sell.v, a.A.sell.vf));
   FM_CHECK(fmk::dinv_slot(st, a.n, a.dslot, a.A.sell.v, a.dinv, a.dinvf));
   a.lam = 0.;
   s->a_assembled[L] = 1;
@@ -1300,14 +1424,22 @@ extern "C" int fm_heat_genmat_transpose(fm_system *s, int L, const fm_heat_param
   HeatArgs h;
   h.alpha0 = p->alpha0; h.rhocp = p->rhocp; h.dt = p->dt; h.tc = p->time_coeff; h.source = p->source; h.upwind = p->upwind;
   for (int d = 0; d < 3; d++) h.vel_const[d] = p->vel_const[d];
-  FM_HIP(hipMemsetAsync(a.A.sell.v, 0, a.A.sell.nslot * sizeof(double), st));
-  for (int k = 0; k < l.ncolor; k++) {
-    int n = l.color_off[k + 1] - l.color_off[k];
-    FM_CHECK(fmk::heat_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldof, l.elpos, l.elslot, l.bmask, s->xyz,
-                                m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, s->xbc, l.coef, s->vel, s->d_tab, h,
-                                0, 1, 0, a.A.sell.v, nullptr));
+  bool gather = false;
+  FM_CHECK(egather_ready(s, L, gather));
+  if (gather) {
+    FM_CHECK(fmk::heat_assemble_e(st, m.dim, l.nel, l.conn, l.bmask, s->xyz, m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1,
+                                  s->x_old, s->xbc, l.coef, s->vel, s->d_tab, h, 0, 1, s->ebuf, s->febuf));
+    FM_CHECK(run_egather(s, L, a.A.sell, nullptr));
+  } else {
+    FM_HIP(hipMemsetAsync(a.A.sell.v, 0, a.A.sell.nslot * sizeof(double), st));
+    for (int k = 0; k < l.ncolor; k++) {
+      int n = l.color_off[k + 1] - l.color_off[k];
+      FM_CHECK(fmk::heat_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldof, l.elpos, l.elslot, l.bmask,
+                                  s->xyz, m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, s->xbc, l.coef, s->vel,
+                                  s->d_tab, h, 0, 1, 0, a.A.sell.v, nullptr));
+    }
   }
-  return after_assembly_T(s, L);
+  return after_assembly_T(s, L, gather);
 }
 
 extern "C" int fm_ns_genmat_transpose(fm_system *s, int L, const fm_ns_params *p) {
@@ -1322,14 +1454,22 @@ extern "C" int fm_ns_genmat_transpose(fm_system *s, int L, const fm_ns_params *p
   h.IRe = p->IRe; h.dt = p->dt; h.tc = p->time_coeff; h.upwind = p->upwind; h.IFr = p->IFr;
   for (int d = 0; d < 3; d++) { h.dirg[d] = p->dirg[d]; h.src[d] = p->src[d]; h.buoy[d] = 0.; }
   h.Tn = nullptr;
-  FM_HIP(hipMemsetAsync(a.A.sell.v, 0, a.A.sell.nslot * sizeof(double), st));
-  for (int k = 0; k < l.ncolor; k++) {
-    int n = l.color_off[k + 1] - l.color_off[k];
-    FM_CHECK(fmk::ns_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldof, l.elpos, l.elslot, l.bmask, s->xyz,
-                              m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, s->xbc, l.coef, s->d_tab, h, 0, 1, 0,
-                              a.A.sell.v, nullptr));
+  bool gather = false;
+  FM_CHECK(egather_ready(s, L, gather));
+  if (gather) {
+    FM_CHECK(fmk::ns_assemble_e(st, m.dim, l.nel, l.conn, l.bmask, s->xyz, m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1,
+                                s->x_old, s->xbc, l.coef, s->d_tab, h, 0, 1, s->ebuf, s->febuf));
+    FM_CHECK(run_egather(s, L, a.A.sell, nullptr));
+  } else {
+    FM_HIP(hipMemsetAsync(a.A.sell.v, 0, a.A.sell.nslot * sizeof(double), st));
+    for (int k = 0; k < l.ncolor; k++) {
+      int n = l.color_off[k + 1] - l.color_off[k];
+      FM_CHECK(fmk::ns_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldof, l.elpos, l.elslot, l.bmask,
+                                s->xyz, m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, s->xbc, l.coef, s->d_tab, h,
+                                0, 1, 0, a.A.sell.v, nullptr));
+    }
   }
-  return after_assembly_T(s, L);
+  return after_assembly_T(s, L, gather);
 }
 
 // Newton Jacobian of the steady residual R(u) = A(u) u - b at u = x_old (the Oseen matrix of GenMatRhs plus the
@@ -1350,15 +1490,23 @@ extern "C" int fm_ns_genjac(fm_system *s, int L, int trans, const fm_ns_params *
   for (int d = 0; d < 3; d++) { a.dirg[d] = p->dirg[d]; a.src[d] = p->src[d]; a.buoy[d] = 0.; }
   a.Tn = nullptr;
   a.newton = 1;
-  FM_HIP(hipMemsetAsync(val, 0, nslot * sizeof(double), st));
-  for (int k = 0; k < l.ncolor; k++) {
-    int n = l.color_off[k + 1] - l.color_off[k];
-    FM_CHECK(fmk::ns_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldofi, l.elpos, l.elslot, l.bmask, s->xyz,
-                              m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, s->xbc, l.coef, s->d_tab, a, 0, trans, 0,
-                              val, nullptr));
+  bool gather = false;
+  FM_CHECK(egather_ready(s, L, gather));
+  if (gather) {
+    FM_CHECK(fmk::ns_assemble_e(st, m.dim, l.nel, l.conn, l.bmask, s->xyz, m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1,
+                                s->x_old, s->xbc, l.coef, s->d_tab, a, 0, trans, s->ebuf, s->febuf));
+    FM_CHECK(run_egather(s, L, trans ? s->alev[L].A.sell : l.A.sell, nullptr));
+  } else {
+    FM_HIP(hipMemsetAsync(val, 0, nslot * sizeof(double), st));
+    for (int k = 0; k < l.ncolor; k++) {
+      int n = l.color_off[k + 1] - l.color_off[k];
+      FM_CHECK(fmk::ns_assemble(st, m.dim, n, l.color_elems + l.color_off[k], l.conn, l.eldofi, l.elpos, l.elslot, l.bmask,
+                                s->xyz, m.Ntop, s->d_dof_top, s->d_bc0, s->d_bc1, s->x_old, s->xbc, l.coef, s->d_tab, a, 0,
+                                trans, 0, val, nullptr));
+    }
   }
-  if (trans) return after_assembly_T(s, L);
-  FM_CHECK(after_assembly(s, L));
+  if (trans) return after_assembly_T(s, L, gather);
+  FM_CHECK(after_assembly(s, L, gather));
   FM_HIP(hipStreamSynchronize(st));
   return FM_OK;
 }

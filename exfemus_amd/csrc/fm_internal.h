@@ -6,6 +6,7 @@
 #include <vector>
 #include "../../include/femus_hip.h"
 #include "fm_mesh.h"
+#include "egather_host.h"
 
 void fm_set_error(const std::string &msg);
 unsigned fm_error_count();  // fm_set_error calls so far on this thread
@@ -79,6 +80,20 @@ struct DevCSR {
   DevSELL sell;
 };
 
+// Element-buffer assembly of one level: the element matrices go to a buffer E[e][row][col] (NL x NL doubles
+// per element, row / column in the orientation of the target matrix), and a gather sums them into the SELL
+// slots (the plan of egather_host.h, on the device).
+struct EGather {
+  bool built = false, unusable = false;
+  int lg = 0, nl = 0, maxseg = 0, maxctr = 0, maxit = 0;  // per block: element rows, contributions, slots
+  int64_t nblk = 0, nseg = 0, nctr = 0;
+  int64_t *lseg = nullptr, *bbase = nullptr;
+  int *seg = nullptr;
+  unsigned char *cnt = nullptr;
+  unsigned short *cst = nullptr, *cidx = nullptr;
+  size_t bytes = 0;
+};
+
 struct Level {
   int64_t n = 0;           // _Dim[L]
   int nel = 0;
@@ -128,7 +143,13 @@ struct Level {
   double *res = nullptr;
   // saved rhs / iterate while an outer Krylov method borrows the level's cycle vectors
   double *sb = nullptr, *sx = nullptr;
+  EGather eg;              // element-buffer assembly (built at the first assembly that uses it)
 };
+
+// the element-buffer gather of a level built from its slot map and SELL layout (femus.cpp); 0, or -1 with err
+// set when the blocks cannot be staged in max_lds bytes
+int fm_build_egather(hipStream_t st, Level &l, int nl, size_t max_lds, EGather &g, std::string &err);
+void fm_free_egather(EGather &g);
 
 struct fm_ctx {
   int device = 0;
@@ -196,6 +217,10 @@ struct fm_system {
   int64_t kn = 0;
   int km = 0, knb = 0;
   std::vector<hipEvent_t> kev;
+  // element-matrix and element-rhs buffers of the element-buffer assembly (sized for the largest level that
+  // uses it; shared by the levels and by the transposed assemblies)
+  double *ebuf = nullptr, *febuf = nullptr;
+  size_t ebuf_n = 0, febuf_n = 0;
 };
 void fm_krylov_free(fm_system *s);
 

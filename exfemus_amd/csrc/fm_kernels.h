@@ -113,6 +113,19 @@ int ns_assemble(hipStream_t s, int dim, int nel_color, const int *elems, const i
                 const unsigned short *elpos, const int64_t *elslot, const unsigned char *bmask, const double *xyz, int Ntop, const int *dof_top,
                 const int *bc0, const int *bc1, const double *x_old, const double *xbc, const double *coef,
                 const FeTab *tab, NsArgs a, int mode, int trans, int bd_only, double *val, double *rhs);
+// element-buffer assembly: element matrices / rhs of every element to E [nel][nl][nl] and Fe [nel][nl] (face
+// terms included), then egather sums them into the SELL values (FP64 and FP32) and the rhs
+int heat_assemble_e(hipStream_t s, int dim, int nel, const int *conn, const unsigned char *bmask, const double *xyz, int Ntop,
+                    const int *dof_top, const int *bc0, const int *bc1, const double *x_old, const double *xbc,
+                    const double *coef, const double *vel, const FeTab *tab, HeatArgs a, int mode, int trans, double *E,
+                    double *Fe);
+int ns_assemble_e(hipStream_t s, int dim, int nel, const int *conn, const unsigned char *bmask, const double *xyz, int Ntop,
+                  const int *dof_top, const int *bc0, const int *bc1, const double *x_old, const double *xbc,
+                  const double *coef, const FeTab *tab, NsArgs a, int mode, int trans, double *E, double *Fe);
+int egather(hipStream_t s, int64_t nblk, int lg, int nl, int oc, int os, int on, size_t lds, const int64_t *sl_off,
+            const int *sl_w, const int *perm, const int64_t *lseg, const int *seg, const int64_t *bbase,
+            const unsigned char *cnt, const unsigned short *cst, const unsigned short *cidx, const double *E,
+            const double *Fe, double *v, float *vf, double *rhs);
 int coupling(hipStream_t s, int dim, int nel, const int *elems, const int *conn, const double *xyz, int Ntop,
              const int *dof_top, const int *bc0, const double *T, const double *lam, const double *m, double scale, int mode,
              const double *vel, double upwind, double rhocp, double alpha0, const FeTab *tab, double *out);

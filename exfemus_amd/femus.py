@@ -46,7 +46,7 @@ EXPORTS = [
     "fm_system_set_vanka_store", "fm_mg_release", "fm_system_couple", "fm_system_advance", "fm_heat_nusselt_noalloc",
     "fm_mesh_create", "fm_system_set_pattern", "fm_system_set_transfer", "fm_system_get_x_dev", "fm_system_set_x_dev",
     "fm_system_get_rhs_dev", "fm_system_set_rhs_dev", "fm_gencase_matrix_block", "fm_gencase_transfer_block",
-    "fm_gencase_boundary",
+    "fm_gencase_boundary", "fm_set_assembly_gather",
 ]
 
 
@@ -97,6 +97,7 @@ def lib():
     L.fm_last_error.restype = C.c_char_p
     L.fm_set_sell_split.argtypes = [C.c_int]
     L.fm_set_sell_offsets.argtypes = [C.c_int]
+    L.fm_set_assembly_gather.argtypes = [C.c_int]
     L.fm_system_sell_info.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
     L.fm_system_set_vanka_store.argtypes = [C.c_void_p, C.c_int]
     L.fm_mg_release.argtypes = [C.c_void_p, C.c_int]

@@ -54,6 +54,12 @@ int fm_set_sell_split(int W);
  * Applies to matrices set up afterwards (fm_system_init); bits < 0 only reads it.  Returns the previous
  * value, or FM_EINVAL for another width. */
 int fm_set_sell_offsets(int bits);
+/* Assembly route of GenMatRhs and of the transposed / Newton assemblies: on = 1 (default) writes the element
+ * matrices of every element to an element buffer and sums them into the SELL values with one gather per level
+ * (deterministic, element order) wherever the buffers of the system stay within a 12 GiB budget; on = 0 adds the
+ * element matrices colour by colour straight into the SELL values (also the route of the levels over budget).
+ * Returns the previous setting, or FM_EINVAL. */
+int fm_set_assembly_gather(int on);
 
 /* ---- mesh hierarchy: gencase GenCase + printMesh -----------------------
  * USER_APPL/gencase/src/MGGenCase.C:63-229 (build_square/build_cube +
