@@ -1,16 +1,22 @@
 // assemble.hip -- GenMatRhs on gfx950: heat (Q2) and coupled Navier-Stokes
-// (Q2-Q1 Taylor-Hood) element matrices scattered into CSR.
+// (Q2-Q1 Taylor-Hood) element matrices and their sum into the level matrix.
 //
-// Volume kernels: one workgroup per element (grid-stride over the elements of
-// one colour).  Reference tables (phi, dphi/dxi, weights) are staged in LDS
-// once per workgroup; per element the 27 (9) Gauss-point threads build the
-// Jacobian, its inverse, JxW and the global gradients (MGFE::Jac3D/Jac2D,
-// get_dphi_gl_g: src/MGFE.C:1066-1166, 1496-1519) into LDS, then all threads
-// accumulate element-matrix entries over the Gauss points and add them to A
-// through the precomputed element->CSR position map.  Elements of one colour
-// share no node, so no atomics are needed and results are deterministic.
-// Boundary faces (penalty Dirichlet, Robin/Neumann) run in a second kernel
-// over the boundary elements of the same colour, after the volume kernel.
+// Two routes share the element kernels.  The element-buffer route (the default)
+// computes every element of a level in one launch and writes its matrix and
+// right-hand side to a buffer (E, Fe); the boundary-face kernel adds the face
+// terms to the same element blocks; k_egather then sums the buffer into the
+// SELL slots, one block per group of rows staged in LDS, in element order.  The
+// coloured route adds each element matrix straight into the SELL values through
+// the element -> slot map, one launch per element colour (elements of one
+// colour share no node, so no atomics are needed); it serves the levels whose
+// element buffer would not fit.
+//
+// Heat volume: one wavefront per element, reference tables staged once per
+// block (k_heat_el).  NS volume: one 256-thread workgroup per element (k_ns_vol):
+// Jacobian, inverse, JxW and global gradients (MGFE::Jac3D/Jac2D,
+// get_dphi_gl_g: src/MGFE.C:1066-1166, 1496-1519) in LDS, element-matrix tiles
+// in registers.  Boundary faces (penalty Dirichlet, Robin/Neumann): one thread
+// per element (and velocity component).
 //
 // Heat:  template_appl/nse/SRC/MGSolverT.C:228-343 (volume), :366-507 (faces)
 // NS:    template_appl/nse/SRC/MGSolverNS.C:285-495 (volume), :503-690 (faces)
@@ -88,24 +94,6 @@ __device__ inline double jac_sur(int qp, const double *xb, const FeTab *tab) {
   }
 }
 
-// ===========================================================================
-// Heat volume kernel
-//
-// One 128-thread workgroup per element (grid-stride over one colour).
-//  1. 27 (9) Gauss-point threads: Jacobian, its inverse, JxW, T_old and the
-//     advection velocity at the point, the diffusion weights
-//     w_qd = JxW (alpha0 k_e + f_up v_d^2) (MGSolverT.C:228-330).
-//  2. All threads: global gradients G[q][d][i] into LDS (the reference
-//     shape values phi[i][q] are staged in LDS once per workgroup).
-//  3. The element matrix is the rank-(DIM+1)*NG product
-//        K_ij = sum_q ( sum_d w_qd G_qdi G_qdj + a_qi b_qj ),
-//     a_qi = JxW phi_i, b_qj = tcdt phi_j + rhocp v.grad phi_j (formed in
-//     registers from G), computed by 81 threads, each owning a 3x3 tile
-//     (i, i+9, i+18) x (j, j+9, j+18) of the 27x27 hex27 matrix (1x1 of the
-//     9x9 quad9 one): 6 LDS reads feed 9 FMAs per (q, d).
-//  4. Entries of Dirichlet rows are dropped (bc_vol row mask) and the rest
-//     are added to A through the element->CSR position map.
-// ===========================================================================
 // SELL slot of entry (a, b) of element e's matrix: position elpos[e][a][b] within the row of local DoF a, whose
 // position 0 is slot elslot[e][a]; -1 where the pattern has no such entry (0xFFFF) or the DoF does not exist
 __device__ __forceinline__ int64_t el_slot(const int64_t *__restrict__ elslot, const unsigned short *__restrict__ elpos,
@@ -113,175 +101,6 @@ __device__ __forceinline__ int64_t el_slot(const int64_t *__restrict__ elslot, c
   const unsigned k = elpos[(e * nl + a) * nl + b];
   const int64_t r = elslot[e * nl + a];
   return (k == 0xFFFFu || r < 0) ? -1 : r + 64 * (int64_t)k;
-}
-
-template <int DIM, bool ADV>
-__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4))) void k_heat_vol(int nel, const int *__restrict__ elems, const int *__restrict__ conn,
-                                                  const int *__restrict__ eldof, const unsigned short *__restrict__ elpos, const int64_t *__restrict__ elslot,
-                                                  const double *__restrict__ xyz, int Ntop, const int *__restrict__ dof_top,
-                                                  const int *__restrict__ bc0, const double *__restrict__ x_old,
-                                                  const double *__restrict__ coef, const double *__restrict__ vel,
-                                                  const FeTab *__restrict__ tab, HeatArgs a, int mode, int trans,
-                                                  double *__restrict__ val, double *__restrict__ rhs) {
-  constexpr int N2 = DIM == 2 ? 9 : 27, NG = N2, TI = DIM == 2 ? 1 : 3;
-  __shared__ double s_G[NG * DIM * N2];  // [q][d][i]
-  __shared__ double s_phi[N2 * NG];      // [i][q]
-  __shared__ double s_w[NG * DIM], s_IJ[NG * DIM * DIM], s_vq[NG * DIM];
-  __shared__ double s_JxW[NG], s_Tg[NG];
-  __shared__ double s_xx[DIM * N2], s_Told[N2], s_veln[DIM * N2];
-  __shared__ int s_bcv[N2];
-  const int tid = threadIdx.x;
-  const double tcdt = a.tc * a.rhocp / a.dt;
-  for (int t = tid; t < N2 * NG; t += blockDim.x) s_phi[t] = tab->phi2[t];
-  // nodal data of the block's next element, loaded one element ahead by the first N2 threads
-  double g_x[DIM], g_v[DIM], g_T = 0.;
-  int g_bc = 0;
-  auto load_nodal = [&](int e) {
-    const int node = conn[(int64_t)e * N2 + tid];
-    const int kd = dof_top[node];
-    for (int d = 0; d < DIM; d++) {
-      g_x[d] = xyz[node + (int64_t)d * Ntop];
-      g_v[d] = vel ? vel[node + (int64_t)d * Ntop] : a.vel_const[d];
-    }
-    g_T = x_old[kd];
-    g_bc = bc0[kd];
-  };
-  int ei = xcd_block();
-  if (ei < nel && tid < N2) load_nodal(elems[ei]);
-  for (; ei < nel; ei += gridDim.x) {
-    const int e = elems[ei];
-    // the slots of this thread's element-matrix entries and their current values, read before the element is
-    // computed (elements of one colour share no node, so no other element of the launch touches these slots)
-    const int ti = tid / 9, tj = tid % 9;
-    int64_t pp[TI][TI];
-    double pv[TI][TI];
-    if (tid < 81) {
-#pragma unroll
-      for (int u = 0; u < TI; u++)
-#pragma unroll
-        for (int v = 0; v < TI; v++) {
-          const int i = ti + 9 * u, j = tj + 9 * v;
-          pp[u][v] = el_slot(elslot, elpos, e, N2, trans ? j : i, trans ? i : j);
-          pv[u][v] = pp[u][v] >= 0 ? val[pp[u][v]] : 0.;
-        }
-    }
-    int rdof = -1;
-    double rold = 0.;
-    if (mode == 1 && tid < N2) {
-      rdof = eldof[(int64_t)e * N2 + tid];
-      rold = rhs[rdof];
-    }
-    __syncthreads();  // the previous element is done with the shared arrays
-    if (tid < N2) {
-      for (int d = 0; d < DIM; d++) { s_xx[tid + d * N2] = g_x[d]; s_veln[tid + d * N2] = g_v[d]; }
-      s_Told[tid] = g_T;
-      s_bcv[tid] = g_bc;
-    }
-    __syncthreads();
-    if (ei + (int)gridDim.x < nel && tid < N2) load_nodal(elems[ei + gridDim.x]);
-    const double ke = coef ? coef[e] : 1.0;
-    if (tid < NG) {
-      const int qp = tid;
-      double IJ[DIM * DIM];
-      const double det = jac_qp<DIM, N2, NG>(qp, s_xx, tab->dphi2, IJ);
-      const double JxW = det * tab->w[qp];
-      s_JxW[qp] = JxW;
-      double Tg = 0., vg[DIM];
-      for (int d = 0; d < DIM; d++) vg[d] = 0.;
-#pragma unroll 3
-      for (int sn = 0; sn < N2; sn++) {
-        const double ph = s_phi[sn * NG + qp];
-        Tg += ph * s_Told[sn];
-        for (int d = 0; d < DIM; d++) vg[d] += ph * s_veln[sn + d * N2];
-      }
-      double h_eff[DIM];
-      for (int d = 0; d < DIM; d++) {
-        double h = fabs(s_xx[d * N2 + 2 + 4 * (DIM - 2)] - s_xx[d * N2 + 0]);
-        double h1 = fabs(s_xx[d * N2 + 3 + 4 * (DIM - 2)] - s_xx[d * N2 + 1]);
-        h_eff[d] = h < h1 ? h1 : h;
-      }
-      double mod2 = 1.e-20;
-      for (int d = 0; d < DIM; d++) mod2 += vg[d] * vg[d];
-      mod2 = sqrt(mod2);
-      s_Tg[qp] = Tg;
-      for (int d = 0; d < DIM; d++) {
-        const double Pe = 0.5 * mod2 * h_eff[d] / a.alpha0;
-        const double fup = a.upwind * a.rhocp * 0.5 * (1. / tanh(Pe) - 1. / Pe) * h_eff[d] / mod2;
-        s_vq[qp * DIM + d] = vg[d];
-        s_w[qp * DIM + d] = JxW * (a.alpha0 * ke + fup * vg[d] * vg[d]);
-      }
-      for (int t = 0; t < DIM * DIM; t++) s_IJ[qp * DIM * DIM + t] = IJ[t];
-    }
-    __syncthreads();
-    for (int pq = tid; pq < NG * N2; pq += blockDim.x) {
-      const int q = pq / N2, i = pq % N2;
-      double gr[DIM];
-      for (int d = 0; d < DIM; d++) gr[d] = tab->dphi2[i * NG + q + d * N2 * NG];
-      for (int d = 0; d < DIM; d++) {
-        double g = 0.;
-        for (int j = 0; j < DIM; j++) g += s_IJ[q * DIM * DIM + j + d * DIM] * gr[j];
-        s_G[(q * DIM + d) * N2 + i] = g;
-      }
-    }
-    __syncthreads();
-    if (tid < 81) {
-      double K[TI][TI];
-#pragma unroll
-      for (int u = 0; u < TI; u++)
-#pragma unroll
-        for (int v = 0; v < TI; v++) K[u][v] = 0.;
-#pragma unroll 1
-      for (int q = 0; q < NG; q++) {
-        double bj[TI];
-        if (ADV) {
-#pragma unroll
-          for (int v = 0; v < TI; v++) bj[v] = tcdt * s_phi[(tj + 9 * v) * NG + q];
-        }
-#pragma unroll
-        for (int d = 0; d < DIM; d++) {
-          const double *G = s_G + (q * DIM + d) * N2;
-          const double w = s_w[q * DIM + d];
-          double gi[TI], gj[TI];
-#pragma unroll
-          for (int u = 0; u < TI; u++) { gi[u] = w * G[ti + 9 * u]; gj[u] = G[tj + 9 * u]; }
-          if (ADV) {
-            const double rv = a.rhocp * s_vq[q * DIM + d];
-#pragma unroll
-            for (int v = 0; v < TI; v++) bj[v] += rv * gj[v];
-          }
-#pragma unroll
-          for (int u = 0; u < TI; u++)
-#pragma unroll
-            for (int v = 0; v < TI; v++) K[u][v] += gi[u] * gj[v];
-        }
-        if (ADV) {
-          const double jw = s_JxW[q];
-#pragma unroll
-          for (int u = 0; u < TI; u++) {
-            const double ai = jw * s_phi[(ti + 9 * u) * NG + q];
-#pragma unroll
-            for (int v = 0; v < TI; v++) K[u][v] += ai * bj[v];
-          }
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < TI; u++) {
-        if (s_bcv[ti + 9 * u] == 0) continue;
-#pragma unroll
-        for (int v = 0; v < TI; v++)
-          if (pp[u][v] >= 0) val[pp[u][v]] = pv[u][v] + K[u][v];
-      }
-    }
-    if (mode == 1 && tid < N2) {
-      const int i = tid;
-      if (s_bcv[i] != 0) {
-        double F = 0.;
-#pragma unroll 3
-        for (int qp = 0; qp < NG; qp++) F += s_JxW[qp] * s_phi[i * NG + qp] * (tcdt * s_Tg[qp] + a.source);
-        rhs[rdof] = rold + F;
-      }
-    }
-  }
 }
 
 // ===========================================================================
