@@ -52,15 +52,18 @@ def build_heat(F, ctx, nx0, nlev, k_sigma=1.0):
 
 
 def assembly_rate(F, ctx, s, heat, reps=5):
-    """Top-level GenMatRhs (matrix + rhs) timed alone: elements/s and algorithmic GB/s, counting per element
-    matrix entry the 2-byte position in its SELL row and the 8-byte value read-modify-write (the scatter), per
-    element DoF the 8-byte row slot, plus the element's coordinates and nodal data (Q2 nodes x dim
-    coordinates + nodal values, 8 B each)."""
+    """Top-level GenMatRhs (matrix + rhs) timed alone: elements/s and two byte rates.  GBs_out: the bytes an
+    assembly has to produce -- every SELL slot of the level matrix in FP64 and FP32 (12 B) and the rhs (8 B per
+    DoF) -- plus the element inputs (per element its Q2 nodes' coordinates and nodal data, 8 B each).  GBs_route:
+    the bytes the element-buffer route moves -- the element matrices written and read back (2 x NL^2 x 8 B per
+    element), the gather's index (per slot a count and a start, 3 B; per element-matrix entry a 2-byte LDS
+    position), the outputs and the inputs."""
     m = s.mesh
     top = m.nlev - 1
     nel = m.level(top)[0]
     n2 = 27 if m.dim == 3 else 9
     nl = n2 if heat else m.dim * n2 + (8 if m.dim == 3 else 4)
+    info = s.sell_info(0, top)
     ts = []
     for _ in range(reps):
         ctx.sync()
@@ -69,9 +72,13 @@ def assembly_rate(F, ctx, s, heat, reps=5):
         ctx.sync()
         ts.append(time.perf_counter() - t0)
     t = float(np.median(ts))
-    bytes_el = nl * nl * (2 + 2 * 8) + nl * 8 + n2 * (m.dim + (1 if heat else m.dim)) * 8
+    n = s._Dim[top]
+    inputs = nel * n2 * (m.dim + (1 if heat else m.dim)) * 8
+    out = info["slots"] * 12 + n * 8
+    route = 2 * nel * nl * nl * 8 + info["slots"] * 3 + nel * nl * nl * 2 + out + inputs
     return {"elements_top": nel, "ms_top": round(t * 1e3, 3), "Melements_per_s": round(nel / t / 1e6, 2),
-            "GBs_alg": round(nel * bytes_el / t / 1e9, 1), "bytes_per_element": bytes_el}
+            "GBs_out": round((out + inputs) / t / 1e9, 1), "GBs_route": round(route / t / 1e9, 1),
+            "bytes_out": out + inputs, "bytes_route": route}
 
 
 def layout_key(kernel, info):
