@@ -80,6 +80,7 @@ int jacobi_step(hipStream_t s, const CsrView<double> &A, const double *b, const 
 int jacobi_step(hipStream_t s, const CsrView<float> &A, const float *b, const float *x, float *xn,
                 const float *dinv, float omega);
 template <typename T> int fill(hipStream_t s, int64_t n, T *x, T a);
+template <typename T> int copy(hipStream_t s, int64_t n, const T *x, T *y);  // y = x
 template <typename T> int axpby(hipStream_t s, int64_t n, T a, T *x, T b, const T *y);
 template <typename T> int cheb_first(hipStream_t s, int64_t n, const T *dinv, const T *r, T *d, T *x, T a);
 template <typename T> int cheb_start(hipStream_t s, int64_t n, const T *dinv, const T *b, T *r, T *d, T *x, T a);

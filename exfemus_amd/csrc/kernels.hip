@@ -390,6 +390,10 @@ int jacobi_step(hipStream_t s, const CsrView<float> &A, const float *b, const fl
 template <typename T> __global__ void k_fill(int64_t n, T *x, T a) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) x[i] = a;
 }
+// y = x (a kernel, not a copy command: a captured V-cycle graph then holds kernel nodes only)
+template <typename T> __global__ void k_copy(int64_t n, const T *x, T *y) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) y[i] = x[i];
+}
 // x = a*x + b*y  (both same type)
 template <typename T> __global__ void k_axpby(int64_t n, T a, T *x, T b, const T *y) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -448,6 +452,13 @@ template <typename T> int fill(hipStream_t s, int64_t n, T *x, T a) {
 }
 template int fill<double>(hipStream_t, int64_t, double *, double);
 template int fill<float>(hipStream_t, int64_t, float *, float);
+template <typename T> int copy(hipStream_t s, int64_t n, const T *x, T *y) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_copy<T>, vgrid(n), dim3(256), 0, s, n, x, y);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+template int copy<double>(hipStream_t, int64_t, const double *, double *);
+template int copy<float>(hipStream_t, int64_t, const float *, float *);
 template <typename T> int axpby(hipStream_t s, int64_t n, T a, T *x, T b, const T *y) {
   if (n <= 0) return 0;
   hipLaunchKernelGGL(k_axpby<T>, vgrid(n), dim3(256), 0, s, n, a, x, b, y);

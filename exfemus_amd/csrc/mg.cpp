@@ -193,8 +193,7 @@ static int smooth(fm_system *s, const fm_mg_opts *opt, int L, const LOps &o, int
     T *t = Vec<T>::t(l);
     for (int k = 0; k < its; k++) {
       FM_CHECK(fmk::jacobi_step(st, A, b, x, t, dinv, (T)opt->omega));
-      hipError_t e = hipMemcpyAsync(x, t, o.n * sizeof(T), hipMemcpyDeviceToDevice, st);
-      if (e != hipSuccess) return FM_EHIP;
+      FM_CHECK(fmk::copy<T>(st, o.n, t, x));
     }
     return 0;
   }

@@ -308,7 +308,7 @@ static int vanka_sweep(fm_system *s, int L, const CsrView<T> &A, const T *vinv, 
   auto residual = [&](int c) -> int {
     if (fresh) {
       fresh = false;
-      return hipMemcpyAsync(r, b, l.n * sizeof(T), hipMemcpyDeviceToDevice, st) == hipSuccess ? 0 : FM_EHIP;
+      return fmk::copy<T>(st, l.n, b, r) == 0 ? 0 : FM_EHIP;
     }
     if (c >= 0 && l.cslist && c + 1 < (int)l.cs_off.size()) {
       CsrView<T> Ac = A;

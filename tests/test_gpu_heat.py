@@ -165,10 +165,12 @@ def test_mg_matches_oracle_cycles(femus, oracle, ctx, dim, nx, nlev):
         np.testing.assert_allclose(sysT.get_x(), xo, rtol=1e-7, atol=1e-12 * np.abs(xo).max())
 
 
+@pytest.mark.parametrize("smoother", [0, 1])
 @pytest.mark.parametrize("prec", [0, 1, 2, 3])
-def test_graph_replay_matches_eager(femus, oracle, ctx, prec):
+def test_graph_replay_matches_eager(femus, oracle, ctx, prec, smoother):
     """V-cycles replayed from the captured HIP graph give bitwise the same iterates as eager launches,
-    over repeated solves with eager outer-loop work and host synchronisation between the replays."""
+    over repeated solves with eager outer-loop work and host synchronisation between the replays
+    (Chebyshev and damped Jacobi smoothing; the Jacobi sweep's copy runs as a kernel inside the graph)."""
     pm, om, sysT, ks = _build(femus, oracle, ctx, 3, [2, 2, 2], 3)
     top = 2
     sysT.set_xold(np.zeros(sysT._Dim[top]))
@@ -177,7 +179,8 @@ def test_graph_replay_matches_eager(femus, oracle, ctx, prec):
     res = {}
     for g in (0, 1, 1, 0, 1):
         sysT.set_x(np.zeros(sysT._Dim[top]))
-        rep = sysT.MGSolve(opts=sysT.opts(precision=prec, tol=1e-10, max_iter=60, use_graph=g))
+        rep = sysT.MGSolve(opts=sysT.opts(precision=prec, smoother=smoother, tol=1e-10, max_iter=200,
+                                          use_graph=g))
         x = sysT.get_x()
         key = (rep.iters, tuple(rep.hist[:rep.iters]))
         res.setdefault(g, []).append((key, x))
