@@ -1186,6 +1186,41 @@ extern "C" int fm_heat_genmatrhs(fm_system *s, int L, int mode, const fm_heat_pa
   return FM_OK;
 }
 
+// A caller's own values of A[L] (SparseMatrixM::zero and set(i, j, v) over the whole pattern,
+// contrib/matrix/sparse_matrixM.h:113-116, before LinearSolverM::solve, contrib/matrix/linear_solverM.h:150-154):
+// val holds the nnz values in the CSR order of the level's pattern as fm_system_csr_get (which 0) reports it
+// (reference numbering, columns sorted).  They replace the FP64 and FP32 SELL values and the Jacobi diagonal;
+// the multigrid set-up is rebuilt at the next solve.
+static int set_matrix(fm_system *s, int L, const double *val, bool dev) {
+  Level &l = s->lev[L];
+  hipStream_t st = s->ctx->stream;
+  FM_HIP(hipSetDevice(s->ctx->device));
+  if (l.A.nr == 0 || l.A.sell.nsl == 0) { fm_set_error("set_matrix: level has no matrix"); return FM_ESTATE; }
+  const double *src = val;
+  if (!dev) {  // staged in the CSR value array, which holds CSR values only on request anyway
+    if (!l.A.v) {
+      FM_CHECK(dalloc(&l.A.v, l.A.nnz));
+      FM_CHECK(dalloc(&l.A.vf, l.A.nnz));
+    }
+    FM_HIP(hipMemcpyAsync(l.A.v, val, l.A.nnz * sizeof(double), hipMemcpyHostToDevice, st));
+    src = l.A.v;
+  }
+  FM_CHECK(fmk::csr2sell(st, sell_map(l.A.sell), src, l.A.sell.v, l.A.sell.vf));
+  FM_CHECK(after_assembly(s, L, true));
+  FM_HIP(hipStreamSynchronize(st));
+  return FM_OK;
+}
+extern "C" int fm_system_set_matrix(fm_system *s, int L, const double *val) {
+  FM_CHECK(check_level(s, L));
+  if (!val) return fm_einval(__func__);
+  return set_matrix(s, L, val, false);
+}
+extern "C" int fm_system_set_matrix_dev(fm_system *s, int L, const double *val_dev) {
+  FM_CHECK(check_level(s, L));
+  FM_CHECK(check_dev_ptr(__func__, val_dev));
+  return set_matrix(s, L, val_dev, true);
+}
+
 extern "C" int fm_ns_genmatrhs(fm_system *s, int L, int mode, const fm_ns_params *p) {
   FM_CHECK(check_level(s, L));
   const fm_mesh &m = *s->mesh;

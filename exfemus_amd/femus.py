@@ -46,7 +46,7 @@ EXPORTS = [
     "fm_system_set_vanka_store", "fm_mg_release", "fm_system_couple", "fm_system_advance", "fm_heat_nusselt_noalloc",
     "fm_mesh_create", "fm_system_set_pattern", "fm_system_set_transfer", "fm_system_get_x_dev", "fm_system_set_x_dev",
     "fm_system_get_rhs_dev", "fm_system_set_rhs_dev", "fm_gencase_matrix_block", "fm_gencase_transfer_block",
-    "fm_gencase_boundary", "fm_set_assembly_gather",
+    "fm_gencase_boundary", "fm_set_assembly_gather", "fm_system_set_matrix", "fm_system_set_matrix_dev",
 ]
 
 
@@ -111,7 +111,8 @@ def lib():
     L.fm_mesh_create.argtypes = [C.c_int, C.c_int, C.c_int, f64p, i32p, i32p, i32p, i32p, i32p, vp, vp, pp]
     L.fm_system_set_pattern.argtypes = [vp, C.c_int, vp, vp]
     L.fm_system_set_transfer.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
-    for fn in ("fm_system_get_x_dev", "fm_system_set_x_dev", "fm_system_get_rhs_dev", "fm_system_set_rhs_dev"):
+    for fn in ("fm_system_get_x_dev", "fm_system_set_x_dev", "fm_system_get_rhs_dev", "fm_system_set_rhs_dev",
+               "fm_system_set_matrix", "fm_system_set_matrix_dev"):
         getattr(L, fn).argtypes = [vp, C.c_int, C.c_void_p]
     L.fm_mesh_info.argtypes = [vp, i32p]
     L.fm_mesh_level.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
@@ -392,6 +393,14 @@ class MGSolDA:
         _ck(lib().fm_system_csr_get(self.h, which, L, rp.ctypes.data_as(C.c_void_p), ci.ctypes.data_as(C.c_void_p),
                                     v.ctypes.data_as(C.c_void_p)))
         return rp, ci, v
+
+    def set_matrix(self, L, val):
+        """The caller's values of A on level L in the CSR order of csr(0, L) (SparseMatrixM::set over the pattern)."""
+        nnz = int(lib().fm_system_csr_nnz(self.h, 0, L))
+        v = np.ascontiguousarray(val, np.float64)
+        if v.size != nnz:
+            raise ValueError(f"set_matrix: {v.size} values for a pattern of {nnz}")
+        _ck(lib().fm_system_set_matrix(self.h, L, v.ctypes.data_as(C.c_void_p)))
 
     def set_xold(self, x):
         _ck(lib().fm_system_set_xold(self.h, _vec(x, self._Dim[-1], "x_old")))

@@ -156,6 +156,14 @@ int fm_system_csr_get(const fm_system *s, int which, int L, int64_t *rowptr, int
  * LASPACK rows are per-row arrays): info = {slices of 64 rows, slots (entries + padding), slots with 8-bit,
  * 16-bit and 32-bit column offsets, nonzeros}. */
 int fm_system_sell_info(const fm_system *s, int which, int L, int64_t info[6]);
+/* A caller-assembled A[L]: SparseMatrixM::zero + set(i, j, v) over the whole pattern
+ * (contrib/matrix/sparse_matrixM.h:113-116), as a matrix handed to LinearSolverM::solve
+ * (contrib/matrix/linear_solverM.h:150-154).  val = the nnz values in the CSR order of fm_system_csr_get
+ * (which 0: reference numbering, sorted columns, the level's own or fm_system_set_pattern's pattern).  They
+ * replace the level's FP64 / FP32 values and Jacobi diagonal; fm_mg_solve rebuilds its set-up.  The _dev form
+ * takes the values in device memory of the context's device (a host pointer is refused).  Complete on return. */
+int fm_system_set_matrix(fm_system *s, int L, const double *val);
+int fm_system_set_matrix_dev(fm_system *s, int L, const double *val_dev);
 
 /* x_old (top level, DoF order): SparseMatrix fields read by GenMatRhs      */
 int fm_system_set_xold(fm_system *s, const double *x_old_top);

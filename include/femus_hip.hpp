@@ -192,6 +192,16 @@ class MGSolDA {
     check(fm_system_get_rhs(h_, L, v.data()), "fm_system_get_rhs");
     return v;
   }
+  void set_rhs(const std::vector<double> &v, int L = -1) {
+    if (L < 0) L = _NoLevels - 1;
+    if (L >= _NoLevels || !sized(v, (size_t)_Dim[L], "set_rhs")) return;
+    check(fm_system_set_rhs(h_, L, v.data()), "fm_system_set_rhs");
+  }
+  // a caller-assembled A[L]: values in the CSR order of the level's pattern (fm_system_set_matrix)
+  void set_matrix(int L, const std::vector<double> &val) {
+    if (L < 0 || L >= _NoLevels || !sized(val, (size_t)fm_system_csr_nnz(h_, 0, L), "set_matrix")) return;
+    check(fm_system_set_matrix(h_, L, val.data()), "fm_system_set_matrix");
+  }
   void set_elem_coef(int L, const std::vector<double> &c) {
     if (!sized(c, (size_t)n_elem(L), "set_elem_coef")) return;
     check(fm_system_set_elem_coef(h_, L, c.data()), "fm_system_set_elem_coef");
