@@ -294,19 +294,15 @@ def test_caller_pattern_missing_entry_and_bad_mesh(femus, oracle, ctx):
 
 
 def _hip():
-    """The HIP runtime the library itself uses, for device buffers of the caller."""
+    """The HIP runtime the library itself is bound to, for device buffers of the caller.
+
+    The symbols are looked up through the library's own handle (dlsym on a handle searches the object and its
+    dependencies), so they come from the runtime copy the library uses.  A process can hold two copies (torch
+    bundles one with the same soname, and which copy the library binds to depends on the import order); a
+    runtime opened by file name can then be the other copy, which does not share the library's device state."""
     import ctypes as C
-    import glob
-    import os
-    for path in ["libamdhip64.so"] + sorted(glob.glob(os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib",
-                                                                    "libamdhip64.so*"))):
-        try:
-            h = C.CDLL(path)
-            break
-        except OSError:
-            continue
-    else:
-        pytest.skip("HIP runtime library not found")
+    from exfemus_amd import femus
+    h = C.CDLL(femus.LIBPATH)
     h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
     h.hipFree.argtypes = [C.c_void_p]
     h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
