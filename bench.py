@@ -84,7 +84,7 @@ def assembly_rate(F, ctx, s, heat, reps=5):
 def layout_key(kernel, info):
     """Key of a SELL layout for the committed PMC summary: the kernel and every count that sets its bytes."""
     return "|".join([kernel] + [str(info[k]) for k in ("slices", "slots", "slots_c8", "slots_c16", "slots_c32",
-                                                        "nnz")])
+                                                        "nnz", "slots_c2")])
 
 
 def pmc_traffic(kernel, info):
@@ -107,11 +107,11 @@ def spmv_bytes(n, nnz, v, nvec=2):
 
 def sell_bytes(info, n, v, nvec=2):
     """Algorithmic bytes of one product in the SELL-64 format as stored: every slot's value (padding
-    included), its 8-, 16- or 32-bit column offset, one int32 base per 64 slots, the slice headers (offsets
-    into values and the three offset arrays, width: 36 B) and the slice's 64 row indices, plus nvec vector
-    streams of n entries."""
-    return (info["slots"] * v + info["slots_c8"] + 2 * info["slots_c16"] + 4 * info["slots_c32"] +
-            info["slots"] // 64 * 4 + info["slices"] * (36 + 64 * 4) + nvec * n * v)
+    included), its 2-bit lane-predicted residual or 8-, 16- or 32-bit column offset, one int32 base per 64
+    slots, the slice headers (offsets into values and the four column arrays, width: 44 B) and the slice's 64
+    row indices, plus nvec vector streams of n entries."""
+    return (info["slots"] * v + info.get("slots_c2", 0) // 4 + info["slots_c8"] + 2 * info["slots_c16"] +
+            4 * info["slots_c32"] + info["slots"] // 64 * 4 + info["slices"] * (44 + 64 * 4) + nvec * n * v)
 
 
 def roofline_measure(s, top, N, reps):

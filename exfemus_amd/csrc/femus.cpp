@@ -32,19 +32,20 @@ extern "C" int fm_set_assembly_gather(int on) {
   g_asm_gather = on;
   return old;
 }
-// narrowest column-offset class of SELL copies built from now on (FEMUS_SELL_C8=0 starts it at 16)
+// narrowest column class of SELL copies built from now on (FEMUS_SELL_P2=0 starts it at 8, FEMUS_SELL_C8=0 at 16)
 static int g_sell_minbits = -1;
 static int sell_minbits() {
   if (g_sell_minbits < 0) {
     const char *c8env = getenv("FEMUS_SELL_C8");
-    g_sell_minbits = (c8env && atoi(c8env) == 0) ? 16 : 8;
+    const char *p2env = getenv("FEMUS_SELL_P2");
+    g_sell_minbits = (c8env && atoi(c8env) == 0) ? 16 : ((p2env && atoi(p2env) == 0) ? 8 : 2);
   }
   return g_sell_minbits;
 }
 extern "C" int fm_set_sell_offsets(int bits) {
   const int old = sell_minbits();
   if (bits < 0) return old;
-  if (bits != 8 && bits != 16 && bits != 32) return fm_einval(__func__);
+  if (bits != 2 && bits != 8 && bits != 16 && bits != 32) return fm_einval(__func__);
   g_sell_minbits = bits;
   return old;
 }
@@ -352,9 +353,9 @@ static void free_sell(DevSELL &q) {
     return;
   }
   dfree(q.sl_off); dfree(q.sl_c32); dfree(q.sl_w); dfree(q.perm); dfree(q.c16); dfree(q.base); dfree(q.c32);
-  dfree(q.sl_c8); dfree(q.c8); dfree(q.sl_c16);
+  dfree(q.sl_c8); dfree(q.c8); dfree(q.sl_c16); dfree(q.sl_c2); dfree(q.c2);
   dfree(q.pos); dfree(q.rbase); dfree(q.rslot); dfree(q.v); dfree(q.vf);
-  q.nsl = 0; q.nslot = 0; q.n32 = 0; q.n8 = 0; q.n16 = 0;
+  q.nsl = 0; q.nslot = 0; q.n32 = 0; q.n8 = 0; q.n16 = 0; q.n2 = 0;
 }
 
 void fm_free_csr(DevCSR &c) {
@@ -366,8 +367,9 @@ static void free_csr(DevCSR &c) { fm_free_csr(c); }
 // SELL-64 copy of a CSR pattern in internal DoF order: internal row i holds CSR
 // row ro[i], CSR column j becomes internal column cr[j]; sigma window 4096 rows
 // (rows sorted by decreasing length inside each window), entries of a row
-// sorted by internal column, 16-bit column offsets from a per-position base
-// with a 32-bit fallback per slice.  Optional row groups (boundaries in
+// sorted by internal column, columns per slice in the narrowest class it allows
+// (2-bit lane-predicted, 8- or 16-bit offsets from a per-position base, absolute
+// 32-bit columns).  Optional row groups (boundaries in
 // internal rows) start their own slices, so a product can be restricted to
 // the slices of chosen groups.
 int fm_build_sell(hipStream_t st, DevCSR &c, const int *ro, const int *cr, const std::vector<int64_t> *groups) {
@@ -381,14 +383,17 @@ int fm_build_sell(hipStream_t st, DevCSR &c, const int *ro, const int *cr, const
   const int hrc = sell_build_host(n, c.h_rp, c.h_ci, ro, cr, groups, W, sell_minbits(), h, err);
   if (hrc) { fm_set_error(err); return hrc == -1 ? FM_EINVAL : FM_ESTATE; }
   c.sell.h_gslice = h.gslice;
-  const int64_t nsl = h.nsl, nslot = h.nslot, n8 = h.n8, n16 = h.n16, n32 = h.n32;
+  const int64_t nsl = h.nsl, nslot = h.nslot, n8 = h.n8, n16 = h.n16, n32 = h.n32, n2 = h.n2;
   const std::vector<int64_t> &off = h.off, &o8 = h.o8, &o16 = h.o16, &o32 = h.o32, &rbase = h.rbase, &rslot = h.rslot;
   const std::vector<int> &wd = h.wd, &perm = h.perm, &base = h.base, &c32 = h.c32;
   const std::vector<unsigned short> &c16 = h.c16, &pos = h.pos;
   const std::vector<unsigned char> &c8 = h.c8;
   DevSELL &q = c.sell;
   free_sell(q);
-  q.nsl = nsl; q.nslot = nslot; q.n32 = n32; q.n8 = n8; q.n16 = n16;
+  q.nsl = nsl; q.nslot = nslot; q.n32 = n32; q.n8 = n8; q.n16 = n16; q.n2 = n2;
+  FM_CHECK(dalloc(&q.sl_c2, nsl)); FM_CHECK(dalloc(&q.c2, h.c2.size()));
+  FM_HIP(hipMemcpyAsync(q.sl_c2, h.o2.data(), nsl * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  FM_HIP(hipMemcpyAsync(q.c2, h.c2.data(), h.c2.size() * sizeof(unsigned), hipMemcpyHostToDevice, st));
   FM_CHECK(dalloc(&q.sl_off, nsl + 1)); FM_CHECK(dalloc(&q.sl_c32, nsl)); FM_CHECK(dalloc(&q.sl_w, nsl));
   FM_CHECK(dalloc(&q.sl_c8, nsl)); FM_CHECK(dalloc(&q.c8, c8.size())); FM_CHECK(dalloc(&q.sl_c16, nsl));
   FM_CHECK(dalloc(&q.perm, nsl * C)); FM_CHECK(dalloc(&q.c16, c16.size())); FM_CHECK(dalloc(&q.base, nslot / C));
@@ -782,7 +787,7 @@ extern "C" int fm_system_sell_info(const fm_system *s, int which, int L, int64_t
   const Level &l = s->lev[L];
   const DevCSR &c = which == 0 ? l.A : (which == 1 ? l.R : l.P);
   info[0] = c.sell.nsl; info[1] = c.sell.nslot; info[2] = c.sell.n8; info[3] = c.sell.n16; info[4] = c.sell.n32;
-  info[5] = c.nnz;
+  info[5] = c.nnz; info[6] = c.sell.n2;
   return FM_OK;
 }
 

@@ -41,18 +41,21 @@ inline int fm_einval(const char *where) {
 // entry k of lane l sits at slot sl_off[s] + 64k + l (column-major inside the
 // slice, so a wavefront's loads are contiguous).  Inside each row the entries
 // are sorted by internal column.  Columns are stored as offsets from a
-// per-slot-column base, base[sl_off[s]/64 + k] (the smallest column among the
-// 64 lanes at position k), in the narrowest of three classes per slice:
+// per-slot-column base, base[sl_off[s]/64 + k], in the narrowest of four classes per slice:
+//   2-bit lane-predicted (every column of position k is base + l + d, d in [0, 3], base = the smallest
+//          column - lane): d of position k, lane l in bits 2 (k%16) of c2[sl_c2[s] + 64 (k/16) + l];
+// otherwise base is the smallest column among the 64 lanes at position k and
 //   8-bit  (every position spans <= 255 columns): offset of position k, lane l
 //          at byte c8[sl_c8[s] + 256 (k/4) + 4 l + k%4] (four positions per
 //          32-bit word, the 64 words of a group contiguous);
 //   16-bit (spans <= 65535): c16[sl_c16[s] + 128 (k/2) + 2 l + k%2];
 //   32-bit absolute columns otherwise: c32[sl_c32[s] + 64 k + l].
-// The offsets of the other classes are -1.  Padding repeats the slot's base
-// column with value 0; pos / rbase / rslot tie every slot to its CSR entry.
+// The offsets of the other classes are -1.  Padding decodes to one of the
+// position's real columns with value 0; pos / rbase / rslot tie every slot to its CSR entry.
 struct DevSELL {
-  int64_t nsl = 0, nslot = 0, n32 = 0, n8 = 0, n16 = 0;
-  int64_t *sl_off = nullptr, *sl_c32 = nullptr, *sl_c8 = nullptr, *sl_c16 = nullptr;
+  int64_t nsl = 0, nslot = 0, n32 = 0, n8 = 0, n16 = 0, n2 = 0;
+  int64_t *sl_off = nullptr, *sl_c32 = nullptr, *sl_c8 = nullptr, *sl_c16 = nullptr, *sl_c2 = nullptr;
+  unsigned *c2 = nullptr;
   int *sl_w = nullptr, *perm = nullptr, *base = nullptr, *c32 = nullptr;
   // slot -> CSR entry: slot (s, k, l) of internal row i = perm[64 s + l] holds CSR entry rbase[i] + pos[slot]
   // (pos 0xFFFF: padding); rslot[i]: slot of position 0 of internal row i
@@ -228,7 +231,7 @@ void fm_krylov_free(fm_system *s);
 #include "fm_kernels.h"
 template <typename T> inline void fm_sell_fields(CsrView<T> &v, const DevSELL &q) {
   v.nsl = q.nsl; v.sl_off = q.sl_off; v.sl_w = q.sl_w; v.perm = q.perm; v.c16 = q.c16; v.base = q.base;
-  v.sl_c32 = q.sl_c32; v.c32 = q.c32; v.sl_c8 = q.sl_c8; v.c8 = q.c8; v.sl_c16 = q.sl_c16;
+  v.sl_c32 = q.sl_c32; v.c32 = q.c32; v.sl_c8 = q.sl_c8; v.c8 = q.c8; v.sl_c16 = q.sl_c16; v.sl_c2 = q.sl_c2; v.c2 = q.c2;
 }
 template <typename T> CsrView<T> fm_view(const DevCSR &c);
 template <> inline CsrView<double> fm_view<double>(const DevCSR &c) {

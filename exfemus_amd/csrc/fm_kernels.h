@@ -12,8 +12,9 @@ struct CsrView {
   int lpr = 16;  // lanes per row
   // SELL-64 copy in internal DoF order (used when nsl > 0)
   int64_t nsl = 0;
-  const int64_t *sl_off = nullptr, *sl_c32 = nullptr, *sl_c8 = nullptr, *sl_c16 = nullptr;
+  const int64_t *sl_off = nullptr, *sl_c32 = nullptr, *sl_c8 = nullptr, *sl_c16 = nullptr, *sl_c2 = nullptr;
   const int *sl_w = nullptr, *perm = nullptr, *base = nullptr, *c32 = nullptr;
+  const unsigned *c2 = nullptr;
   const unsigned short *c16 = nullptr;
   const unsigned char *c8 = nullptr;
   const T *sv = nullptr;

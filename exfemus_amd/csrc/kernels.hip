@@ -117,8 +117,71 @@ __device__ __forceinline__ T sell_row_packed(const T *__restrict__ vp, const uns
 #pragma unroll
     for (int u = 0; u < UNR; u++) sum += v[u] * xv[u];
   }
-  for (; k < w; k++)
-    sum += __builtin_nontemporal_load(vp + k * 64) * xg[bp[k] + (int)((cw[(k / PER) * 64] >> (BITS * (k % PER))) & MASK)];
+  // the last w mod UNR positions (Q2 rows hold 27, 45, 75 or 125 entries) as one batch under wave-uniform
+  // guards, so that their loads are in flight together instead of one dependent chain per position; the
+  // terms are added in the same order as above
+  const int rem = w - k;
+  if (rem > 0) {
+    T v[UNR], xv[UNR];
+    unsigned p[UNR / PER];
+#pragma unroll
+    for (int u = 0; u < UNR; u++)
+      if (u < rem) v[u] = __builtin_nontemporal_load(vp + (k + u) * 64);
+#pragma unroll
+    for (int g = 0; g < UNR / PER; g++)
+      if (g * PER < rem) p[g] = __builtin_nontemporal_load(cw + (k / PER + g) * 64);
+#pragma unroll
+    for (int u = 0; u < UNR; u++)
+      if (u < rem) xv[u] = xg[bp[k + u] + (int)((p[u / PER] >> (BITS * (u % PER))) & MASK)];
+#pragma unroll
+    for (int u = 0; u < UNR; u++)
+      if (u < rem) sum += v[u] * xv[u];
+  }
+  return sum;
+}
+
+// one lane's row of a lane-predicted slice over the positions [k0, k1): the column of position k is
+// bp[k] + lane + a 2-bit residual, bits 2 (k % 16) of the lane's word cw[64 (k / 16)].  A batch's columns need
+// only that word and the scalar bases, so the word of the next 16 positions is loaded one chunk ahead and a
+// batch issues its value loads and its gathers together (one HBM round trip per batch, not HBM then L2).
+template <typename T, int UNR>
+__device__ __forceinline__ T sell_row_pred(const T *__restrict__ vp, const unsigned *__restrict__ cw,
+                                           const int *__restrict__ bp, int lane, int k0, int k1,
+                                           const T *__restrict__ xg) {
+  static_assert(16 % UNR == 0, "a batch lies inside one residual word");
+  T sum = 0;
+  if (k0 >= k1) return sum;
+  unsigned p = __builtin_nontemporal_load(cw + (k0 >> 4) * 64);
+  int k = k0;
+  while (k < k1) {
+    const int kend = min(k1, ((k >> 4) + 1) << 4);   // end of this word's positions
+    unsigned pn = 0;
+    if (kend < k1) pn = __builtin_nontemporal_load(cw + (kend >> 4) * 64);
+    for (; k + UNR <= kend; k += UNR) {
+      T v[UNR], xv[UNR];
+#pragma unroll
+      for (int u = 0; u < UNR; u++) v[u] = __builtin_nontemporal_load(vp + (k + u) * 64);
+#pragma unroll
+      for (int u = 0; u < UNR; u++) xv[u] = xg[bp[k + u] + lane + (int)((p >> (2 * ((k + u) & 15))) & 3u)];
+#pragma unroll
+      for (int u = 0; u < UNR; u++) sum += v[u] * xv[u];
+    }
+    const int rem = kend - k;  // the word's last positions as one guarded batch
+    if (rem > 0) {
+      T v[UNR], xv[UNR];
+#pragma unroll
+      for (int u = 0; u < UNR; u++)
+        if (u < rem) v[u] = __builtin_nontemporal_load(vp + (k + u) * 64);
+#pragma unroll
+      for (int u = 0; u < UNR; u++)
+        if (u < rem) xv[u] = xg[bp[k + u] + lane + (int)((p >> (2 * ((k + u) & 15))) & 3u)];
+#pragma unroll
+      for (int u = 0; u < UNR; u++)
+        if (u < rem) sum += v[u] * xv[u];
+      k = kend;
+    }
+    p = pn;
+  }
   return sum;
 }
 
@@ -129,9 +192,12 @@ __device__ __forceinline__ T sell_range(int s, int lane, int k0, int k1, const i
                                         const int64_t *__restrict__ sl_c32, const unsigned short *__restrict__ c16,
                                         const int *__restrict__ base, const int *__restrict__ c32,
                                         const int64_t *__restrict__ sl_c8, const unsigned char *__restrict__ c8,
-                                        const int64_t *__restrict__ sl_c16, const T *__restrict__ vals,
+                                        const int64_t *__restrict__ sl_c16, const int64_t *__restrict__ sl_c2,
+                                        const unsigned *__restrict__ c2, const T *__restrict__ vals,
                                         const T *__restrict__ xg) {
   const int64_t off = sl_off[s];
+  const int64_t o2 = sl_c2[s];
+  if (o2 >= 0) return sell_row_pred<T, UNR>(vals + off + lane, c2 + o2 + lane, base + (off >> 6), lane, k0, k1, xg);
   const int64_t o32 = sl_c32[s];
   const int64_t o8 = sl_c8[s], o16 = sl_c16[s];
   const T *vp = vals + off + lane + (int64_t)k0 * 64;
@@ -159,7 +225,23 @@ __device__ __forceinline__ T sell_range(int s, int lane, int k0, int k1, const i
 #pragma unroll
     for (int u = 0; u < UNR; u++) sum += v[u] * xv[u];
   }
-  for (; k < w; k++) sum += __builtin_nontemporal_load(vp + k * 64) * xg[__builtin_nontemporal_load(cp + k * 64)];
+  const int rem = w - k;  // the remainder as one guarded batch, as in sell_row_packed
+  if (rem > 0) {
+    T v[UNR], xv[UNR];
+    int c[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; u++)
+      if (u < rem) {
+        v[u] = __builtin_nontemporal_load(vp + (k + u) * 64);
+        c[u] = __builtin_nontemporal_load(cp + (k + u) * 64);
+      }
+#pragma unroll
+    for (int u = 0; u < UNR; u++)
+      if (u < rem) xv[u] = xg[c[u]];
+#pragma unroll
+    for (int u = 0; u < UNR; u++)
+      if (u < rem) sum += v[u] * xv[u];
+  }
   return sum;
 }
 
@@ -169,7 +251,8 @@ __global__ __launch_bounds__(256) void k_sell(int nsl, const int64_t *__restrict
                                               const int *__restrict__ perm, const unsigned short *__restrict__ c16,
                                               const int *__restrict__ base, const int *__restrict__ c32,
                                               const int64_t *__restrict__ sl_c8, const unsigned char *__restrict__ c8,
-                                              const int64_t *__restrict__ sl_c16,
+                                              const int64_t *__restrict__ sl_c16, const int64_t *__restrict__ sl_c2,
+                                              const unsigned *__restrict__ c2,
                                               const T *__restrict__ vals, const T *__restrict__ xg, EpArgs<T> ep,
                                               int xcd_chunk, const int *__restrict__ slist) {
   int blk = blockIdx.x;
@@ -179,7 +262,8 @@ __global__ __launch_bounds__(256) void k_sell(int nsl, const int64_t *__restrict
   const int s = slist ? __builtin_amdgcn_readfirstlane(slist[w_idx]) : w_idx;
   const int lane = threadIdx.x & 63;
   const int row = perm[(int64_t)s * 64 + lane];
-  const T sum = sell_range<T, UNR>(s, lane, 0, sl_w[s], sl_off, sl_c32, c16, base, c32, sl_c8, c8, sl_c16, vals, xg);
+  const T sum = sell_range<T, UNR>(s, lane, 0, sl_w[s], sl_off, sl_c32, c16, base, c32, sl_c8, c8, sl_c16, sl_c2, c2,
+                                         vals, xg);
   if (row < 0) return;
   apply_ep<T, EP>(row, sum, xg, ep);
 }
@@ -197,9 +281,10 @@ __global__ __launch_bounds__(64 * S) void k_sell_split(int nsl, const int64_t *_
                                                        const int *__restrict__ base, const int *__restrict__ c32,
                                                        const int64_t *__restrict__ sl_c8,
                                                        const unsigned char *__restrict__ c8,
-                                                       const int64_t *__restrict__ sl_c16, const T *__restrict__ vals,
-                                                       const T *__restrict__ xg, EpArgs<T> ep,
-                                                       const int *__restrict__ slist) {
+                                                       const int64_t *__restrict__ sl_c16,
+                                                       const int64_t *__restrict__ sl_c2, const unsigned *__restrict__ c2,
+                                                       const T *__restrict__ vals, const T *__restrict__ xg,
+                                                       EpArgs<T> ep, const int *__restrict__ slist) {
   __shared__ T part[S - 1][64];
   const int w_idx = blockIdx.x;
   if (w_idx >= nsl) return;  // whole block
@@ -209,7 +294,8 @@ __global__ __launch_bounds__(64 * S) void k_sell_split(int nsl, const int64_t *_
   const int w = sl_w[s];
   const int c = ((w + S - 1) / S + UNR - 1) / UNR * UNR;
   const int k0 = min(w, j * c), k1 = min(w, k0 + c);
-  T sum = sell_range<T, UNR>(s, lane, k0, k1, sl_off, sl_c32, c16, base, c32, sl_c8, c8, sl_c16, vals, xg);
+  T sum = sell_range<T, UNR>(s, lane, k0, k1, sl_off, sl_c32, c16, base, c32, sl_c8, c8, sl_c16, sl_c2, c2, vals,
+                                   xg);
   if (j > 0) part[j - 1][lane] = sum;
   __syncthreads();
   if (j > 0) return;
@@ -249,10 +335,10 @@ template <typename T, int EP, int S>
 static void launch_split(hipStream_t s, const CsrView<T> &A, const T *x, const EpArgs<T> &ep, int nsl) {
   if (g_sell_unroll == 16)
     hipLaunchKernelGGL((k_sell_split<T, EP, 16, S>), dim3((unsigned)nsl), dim3(64 * S), 0, s, nsl, A.sl_off, A.sl_c32,
-                       A.sl_w, A.perm, A.c16, A.base, A.c32, A.sl_c8, A.c8, A.sl_c16, A.sv, x, ep, A.slist);
+                       A.sl_w, A.perm, A.c16, A.base, A.c32, A.sl_c8, A.c8, A.sl_c16, A.sl_c2, A.c2, A.sv, x, ep, A.slist);
   else
     hipLaunchKernelGGL((k_sell_split<T, EP, 8, S>), dim3((unsigned)nsl), dim3(64 * S), 0, s, nsl, A.sl_off, A.sl_c32,
-                       A.sl_w, A.perm, A.c16, A.base, A.c32, A.sl_c8, A.c8, A.sl_c16, A.sv, x, ep, A.slist);
+                       A.sl_w, A.perm, A.c16, A.base, A.c32, A.sl_c8, A.c8, A.sl_c16, A.sl_c2, A.c2, A.sv, x, ep, A.slist);
 }
 
 template <typename T, int EP>
@@ -284,13 +370,13 @@ static int launch_spmv(hipStream_t s, const CsrView<T> &A, const T *x, const EpA
     dim3 grid((unsigned)nb);
     if (g_sell_unroll == 8)
       hipLaunchKernelGGL((k_sell<T, EP, 8>), grid, dim3(block), 0, s, nsl, A.sl_off, A.sl_c32, A.sl_w, A.perm, A.c16,
-                         A.base, A.c32, A.sl_c8, A.c8, A.sl_c16, A.sv, x, ep, (int)chunk, A.slist);
+                         A.base, A.c32, A.sl_c8, A.c8, A.sl_c16, A.sl_c2, A.c2, A.sv, x, ep, (int)chunk, A.slist);
     else if (g_sell_unroll == 16)
       hipLaunchKernelGGL((k_sell<T, EP, 16>), grid, dim3(block), 0, s, nsl, A.sl_off, A.sl_c32, A.sl_w, A.perm, A.c16,
-                         A.base, A.c32, A.sl_c8, A.c8, A.sl_c16, A.sv, x, ep, (int)chunk, A.slist);
+                         A.base, A.c32, A.sl_c8, A.c8, A.sl_c16, A.sl_c2, A.c2, A.sv, x, ep, (int)chunk, A.slist);
     else
       hipLaunchKernelGGL((k_sell<T, EP, 4>), grid, dim3(block), 0, s, nsl, A.sl_off, A.sl_c32, A.sl_w, A.perm, A.c16,
-                         A.base, A.c32, A.sl_c8, A.c8, A.sl_c16, A.sv, x, ep, (int)chunk, A.slist);
+                         A.base, A.c32, A.sl_c8, A.c8, A.sl_c16, A.sl_c2, A.c2, A.sv, x, ep, (int)chunk, A.slist);
     return hipGetLastError() == hipSuccess ? 0 : -2;
   }
   int64_t threads = A.nr * A.lpr;

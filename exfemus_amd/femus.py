@@ -507,10 +507,12 @@ class MGSolDA:
         _ck(lib().fm_mg_release(self.h, 1 if adjoint else 0))
 
     def sell_info(self, which, L):
-        """SELL storage of matrix `which` (0 A, 1 R, 2 P) on level L: slices, slots, 8/16/32-bit offset slots, nnz."""
-        info = np.zeros(6, np.int64)
+        """SELL storage of matrix `which` (0 A, 1 R, 2 P) on level L: slices, slots, 8/16/32-bit offset slots, nnz,
+        2-bit lane-predicted slots."""
+        info = np.zeros(7, np.int64)
         _ck(lib().fm_system_sell_info(self.h, which, L, info.ctypes.data_as(C.c_void_p)))
-        return dict(zip(("slices", "slots", "slots_c8", "slots_c16", "slots_c32", "nnz"), (int(v) for v in info)))
+        return dict(zip(("slices", "slots", "slots_c8", "slots_c16", "slots_c32", "nnz", "slots_c2"),
+                        (int(v) for v in info)))
 
     def resid(self, L):
         _ck(lib().fm_residual(self.h, L))

@@ -48,9 +48,11 @@ const char *fm_last_error(void);
  * (and keeps) a graph of the new shape. */
 int fm_set_sell_split(int W);
 /* Column-offset width knob of the SELL copies (no counterpart in the reference, whose LASPACK rows store
- * full column indices): each slice keeps its columns as 8-bit or 16-bit offsets from a per-position base
- * when its span allows, else as absolute 32-bit columns.  bits = 8 (default; FEMUS_SELL_C8=0 starts at 16)
- * lets every class be used, 16 never uses 8-bit offsets, 32 stores absolute columns in every slice.
+ * full column indices): each slice keeps its columns as 2-bit residuals from base + lane (when every column of
+ * a position is base + lane + 0..3, as on the rows of structured grids), as 8-bit or 16-bit offsets from a
+ * per-position base when its span allows, else as absolute 32-bit columns.  bits = 2 (default; FEMUS_SELL_P2=0
+ * starts at 8, FEMUS_SELL_C8=0 at 16) lets every class be used, 8 never uses the lane-predicted class, 16 no
+ * 8-bit offsets either, 32 stores absolute columns in every slice.
  * Applies to matrices set up afterwards (fm_system_init); bits < 0 only reads it.  Returns the previous
  * value, or FM_EINVAL for another width. */
 int fm_set_sell_offsets(int bits);
@@ -154,8 +156,9 @@ int fm_system_csr_get(const fm_system *s, int which, int L, int64_t *rowptr, int
                       double *val);
 /* SELL storage of matrix `which` (0 A, 1 R, 2 P) on level L (no counterpart in the reference, whose
  * LASPACK rows are per-row arrays): info = {slices of 64 rows, slots (entries + padding), slots with 8-bit,
- * 16-bit and 32-bit column offsets, nonzeros}. */
-int fm_system_sell_info(const fm_system *s, int which, int L, int64_t info[6]);
+ * 16-bit and 32-bit column offsets, nonzeros, slots with 2-bit lane-predicted columns}; the per-class counts
+ * are of the stored arrays (8-bit offsets padded to 4 positions, 16-bit to 2, 2-bit to 16 per lane). */
+int fm_system_sell_info(const fm_system *s, int which, int L, int64_t info[7]);
 /* A caller-assembled A[L]: SparseMatrixM::zero + set(i, j, v) over the whole pattern
  * (contrib/matrix/sparse_matrixM.h:113-116), as a matrix handed to LinearSolverM::solve
  * (contrib/matrix/linear_solverM.h:150-154).  val = the nnz values in the CSR order of fm_system_csr_get

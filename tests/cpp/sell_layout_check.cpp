@@ -2,10 +2,11 @@
 // patterns: a box of nx x ny x nz nodes with nv variables per node, every DoF coupled to all DoFs of the nodes
 // within Chebyshev distance r (the shape of Q2 rows), rows renumbered by a random permutation (or none), row
 // groups at random boundaries.  Checks, for every slot, that the entry it holds is a CSR entry of its row, that
-// every CSR entry is held exactly once, that the decoded column (base + 8/16-bit offset, or the 32-bit column)
-// is the entry's internal column, that a row's columns increase with the position, that padding is marked,
-// and that the row slot bases and group slices are consistent.  Prints one JSON line.
-//   usage: sell_layout_check nx ny nz nv r seed groups W minbits permute   (minbits 8, 16 or 32)
+// every CSR entry is held exactly once, that the decoded column (base + lane + 2-bit residual, base + 8/16-bit
+// offset, or the 32-bit column) is the entry's internal column, that a row's columns increase with the
+// position, that padding is marked and decodes to a column inside the vector, that no slice sits in a class
+// below the floor, and that the row slot bases and group slices are consistent.  Prints one JSON line.
+//   usage: sell_layout_check nx ny nz nv r seed groups W minbits permute   (minbits 2, 8, 16 or 32)
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -66,12 +67,21 @@ int main(int argc, char **argv) {
   if ((int64_t)h.perm.size() != h.nsl * C || (int64_t)h.off.size() != h.nsl + 1 || h.off[h.nsl] != h.nslot)
     return fail("slice arrays");
   std::vector<char> seen(rp[n], 0), row_seen(n, 0);
-  int64_t s8 = 0, s16 = 0, s32 = 0;
+  int64_t s8 = 0, s16 = 0, s32 = 0, s2 = 0;
+  // the column a slot decodes to, as the product kernels decode it
+  auto decode = [&](int64_t s, int k, int l) -> int64_t {
+    const int b = h.base[h.off[s] / C + k];
+    if (h.o2[s] >= 0) return (int64_t)b + l + ((h.c2[h.o2[s] + (int64_t)(k / 16) * C + l] >> (2 * (k % 16))) & 3u);
+    if (h.o8[s] >= 0) return b + h.c8[h.o8[s] + (int64_t)(k / 4) * 4 * C + 4 * l + (k % 4)];
+    if (h.o16[s] >= 0) return b + h.c16[h.o16[s] + (int64_t)(k / 2) * 2 * C + 2 * l + (k % 2)];
+    return h.c32[h.o32[s] + (int64_t)k * C + l];
+  };
   for (int64_t s = 0; s < h.nsl; s++) {
-    const int cls = (h.o8[s] >= 0) + (h.o16[s] >= 0) + (h.o32[s] >= 0);
+    const int cls = (h.o8[s] >= 0) + (h.o16[s] >= 0) + (h.o32[s] >= 0) + (h.o2[s] >= 0);
     if (cls != 1) return fail("slice " + std::to_string(s) + " has no single offset class");
     if (h.off[s + 1] - h.off[s] != (int64_t)h.wd[s] * C) return fail("slice width");
-    (h.o8[s] >= 0 ? s8 : h.o16[s] >= 0 ? s16 : s32) += (int64_t)h.wd[s] * C;
+    if (h.o2[s] >= 0 && minbits > 2) return fail("lane-predicted slice below the class floor");
+    (h.o2[s] >= 0 ? s2 : h.o8[s] >= 0 ? s8 : h.o16[s] >= 0 ? s16 : s32) += (int64_t)h.wd[s] * C;
     for (int l = 0; l < C; l++) {
       const int i = h.perm[s * C + l];
       if (i >= 0) {
@@ -80,22 +90,20 @@ int main(int argc, char **argv) {
         if (h.rbase[i] != rp[ro[i]]) return fail("row CSR base");
       }
       const int len = i >= 0 ? (int)(rp[ro[i] + 1] - rp[ro[i]]) : 0;
-      int prev = -1;
+      int64_t prev = -1;
       for (int k = 0; k < h.wd[s]; k++) {
         const int64_t slot = h.off[s] + (int64_t)k * C + l;
         const unsigned t = h.pos[slot];
         if (k >= len) {
           if (t != 0xFFFFu) return fail("padding slot not marked");
+          const int64_t pc = decode(s, k, l);   // gathered with value 0: must lie inside the vector
+          if (pc < 0 || pc >= n) return fail("padding slot decodes outside the columns");
           continue;
         }
         if (t == 0xFFFFu || (int)t >= len) return fail("slot position out of its row");
         const int64_t q = h.rbase[i] + t;
         if (seen[q]++) return fail("CSR entry held twice");
-        int col;
-        const int b = h.base[h.off[s] / C + k];
-        if (h.o8[s] >= 0) col = b + h.c8[h.o8[s] + (int64_t)(k / 4) * 4 * C + 4 * l + (k % 4)];
-        else if (h.o16[s] >= 0) col = b + h.c16[h.o16[s] + (int64_t)(k / 2) * 2 * C + 2 * l + (k % 2)];
-        else col = h.c32[h.o32[s] + (int64_t)k * C + l];
+        const int64_t col = decode(s, k, l);
         if (col != cr[ci[q]]) return fail("decoded column differs from the entry's internal column");
         if (col <= prev) return fail("row columns not increasing");
         prev = col;
@@ -110,8 +118,8 @@ int main(int argc, char **argv) {
       return fail("group does not start its slice");
   }
   std::printf("{\"ok\": true, \"rows\": %lld, \"nnz\": %lld, \"slices\": %lld, \"slots\": %lld, \"c8\": %lld, "
-              "\"c16\": %lld, \"c32\": %lld, \"groups\": %zu}\n",
+              "\"c16\": %lld, \"c32\": %lld, \"c2\": %lld, \"groups\": %zu}\n",
               (long long)n, (long long)rp[n], (long long)h.nsl, (long long)h.nslot, (long long)s8, (long long)s16,
-              (long long)s32, groups.size() - 1);
+              (long long)s32, (long long)s2, groups.size() - 1);
   return 0;
 }

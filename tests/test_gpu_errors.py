@@ -100,7 +100,7 @@ def test_memory_and_coupling_controls_reject_bad_arguments(femus, ctx):
     m2 = femus.Mesh(2, [2, 2], [0, 0], [1, 1], 2)
     a = femus.MGSolT(ctx, m1, alpha0=1.0, dt=0.1, time_coeff=0.0, source=1.0)
     b = femus.MGSolT(ctx, m2, alpha0=1.0, dt=0.1, time_coeff=0.0, source=1.0)
-    info = np.zeros(6, np.int64)
+    info = np.zeros(7, np.int64)
     assert L.fm_system_sell_info(a.h, 0, 0, info.ctypes.data_as(C.c_void_p)) == FM_EINVAL   # before init
     r = C.c_double()
     assert L.fm_system_advance(a.h, C.byref(r)) == FM_ESTATE                                  # before init

@@ -1,7 +1,8 @@
 """The benchmark's configuration at its full size (3D heat hex27, 64^3 top, 5 levels, 2,146,689 DoF,
 135,005,697 nonzeros, log-normal k) through the C ABI, checked by properties that do not need the oracle to
 run at that size: the SELL residual of every level equals the host product of the level's CSR matrix
-(fm_system_csr_get, scipy) on random vectors, the top level exercises all three column-offset classes, the
+(fm_system_csr_get, scipy) on random vectors, the top level exercises all four column classes (most slices
+lane-predicted), the
 mixed-precision solution's FP64 residual formed on the host is below the tolerance, and it agrees with the
 all-FP64 multigrid solution to 1e-7."""
 import numpy as np
@@ -27,6 +28,7 @@ def test_heat64_five_levels_residuals_and_solution(femus, ctx):
         s.GenMatRhs(0.0, L, 1 if L == top else 0)
     info = s.sell_info(0, top)
     assert s._Dim[top] == 2146689 and info["nnz"] == 135005697
+    assert info["slots_c2"] > 0.9 * info["slots"]          # lane-predicted class on most slices
     assert info["slots_c8"] > 0 and info["slots_c16"] > 0 and info["slots_c32"] > 0
     rng = np.random.default_rng(64)
     for L in range(nlev):

@@ -1,5 +1,6 @@
-"""The three column-offset classes of the SELL copies (8-bit and 16-bit offsets from a per-position base, absolute
-32-bit columns), each forced onto every slice of small matrices through fm_set_sell_offsets: the level products
+"""The column classes of the SELL copies (2-bit residuals from base + lane, 8-bit and 16-bit offsets from a
+per-position base, absolute 32-bit columns), each made the narrowest allowed on small matrices through
+fm_set_sell_offsets (32 forces absolute columns onto every slice): the level products
 (residual, restriction, prolongation) on the one-wave-per-slice kernel and on the split-slice kernel must equal
 the reference-order CSR products and the oracle's LASPACK-order product, and the FP32 V-cycle (whose products
 decode the same offsets in FP32) must still solve to the FP64 direct solution."""
@@ -15,12 +16,12 @@ from exfemus_amd import configs
 pytestmark = pytest.mark.gpu
 
 
-def _heat(femus, ctx, dim, nx, nlev):
+def _heat(femus, ctx, dim, nx, nlev, mean="arithmetic"):
     m = femus.Mesh(dim, nx, [0.0] * dim, [1.0] * dim, nlev)
     s = femus.MGSolT(ctx, m, alpha0=1.0, dt=0.1, time_coeff=0.0, source=1.0, upwind=1.0)
     s.GenBc(configs.heat_dirichlet_all)
     s.init()
-    ks = configs.coarsen_coef(configs.lognormal_k(m.level(nlev - 1)[0], sigma=1.0), nlev, 2 ** dim)
+    ks = configs.coarsen_coef(configs.lognormal_k(m.level(nlev - 1)[0], sigma=1.0), nlev, 2 ** dim, mean=mean)
     for L in range(nlev):
         s.set_elem_coef(L, ks[L])
     s.set_xold(np.zeros(s._Dim[nlev - 1]))
@@ -76,7 +77,7 @@ def _check_products(femus, oracle, s, nlev, rng):
         lib.fm_set_sell_split(W0)
 
 
-@pytest.mark.parametrize("bits", [8, 16, 32])
+@pytest.mark.parametrize("bits", [2, 8, 16, 32])
 @pytest.mark.parametrize("physics,dim,nx,nlev", [("heat", 3, [2, 2, 2], 3), ("heat", 2, [2, 3], 4),
                                                  ("ns", 3, [2, 2, 2], 2), ("ns", 2, [2, 2], 3)])
 def test_offset_class_products(femus, oracle, ctx, bits, physics, dim, nx, nlev):
@@ -91,21 +92,75 @@ def test_offset_class_products(femus, oracle, ctx, bits, physics, dim, nx, nlev)
             if (which == 1 and L == nlev - 1) or (which == 2 and L == 0):
                 continue
             info = s.sell_info(which, L)
-            slots, c8, c16, c32, nnz = (info[k] for k in ("slots", "slots_c8", "slots_c16", "slots_c32", "nnz"))
-            assert c8 + c16 + c32 >= slots >= nnz
+            slots, c2, c8, c16, c32, nnz = (info[k] for k in ("slots", "slots_c2", "slots_c8", "slots_c16",
+                                                               "slots_c32", "nnz"))
+            assert c2 + c8 + c16 + c32 >= slots >= nnz
+            if bits >= 8:
+                assert c2 == 0
             if bits >= 16:
                 assert c8 == 0
             if bits == 32:
                 assert c16 == 0 and c32 == slots
     _check_products(femus, oracle, s, nlev, np.random.default_rng(17 + bits))
     if physics == "heat":
-        top = nlev - 1
-        rp, ci, v = s.csr(0, top)
-        n = s._Dim[top]
-        xref = spla.spsolve(sp.csr_matrix((v, ci, rp), shape=(n, n)).tocsc(), s.get_rhs())
-        for prec in (femus.FM_PREC_FP32_FGMRES, femus.FM_PREC_FP32_IR):
-            s.set_x(np.zeros(n))
-            rep = s.MGSolve(opts=s.opts(precision=prec, tol=1e-10, max_iter=60))
+        _check_solve(femus, s, nlev)
+    s.close()
+
+
+def _check_solve(femus, s, nlev):
+    top = nlev - 1
+    rp, ci, v = s.csr(0, top)
+    n = s._Dim[top]
+    xref = spla.spsolve(sp.csr_matrix((v, ci, rp), shape=(n, n)).tocsc(), s.get_rhs())
+    for prec in (femus.FM_PREC_FP32_FGMRES, femus.FM_PREC_FP32_IR):
+        s.set_x(np.zeros(n))
+        rep = s.MGSolve(opts=s.opts(precision=prec, tol=1e-10, max_iter=60))
+        assert rep.converged == 1
+        assert np.linalg.norm(s.get_x() - xref) <= 1e-8 * np.linalg.norm(xref)
+
+
+def test_lane_predicted_class(femus, oracle, ctx):
+    """2D heat with 128^2 elements on the top level: its node classes run in lines of 64-65 nodes, so most
+    top-level slices hold 64 consecutive nodes of one line and take the 2-bit lane-predicted class (column =
+    base + lane + 0..3, padding decoding to a real column of its position).  The products on both kernels as for
+    the other classes; against the same hierarchy with the class switched off (8-bit offsets), which sums every
+    row in the same column order, the residual and the solves agree bit for bit (same iterations, same
+    iterates), and the solves reach the direct solution."""
+    lib = femus.lib()
+    nlev, top = 6, 5
+    layouts = {}
+    for bits in (2, 8):
+        old = lib.fm_set_sell_offsets(bits)
+        try:
+            layouts[bits] = _heat(femus, ctx, 2, [4, 4], nlev, mean="geometric")[1]
+        finally:
+            lib.fm_set_sell_offsets(old)
+    s, s8 = layouts[2], layouts[8]
+    info = s.sell_info(0, top)
+    assert info["slots_c2"] >= 0.9 * info["slots"]
+    assert s8.sell_info(0, top)["slots_c2"] == 0
+    _check_products(femus, oracle, s, nlev, np.random.default_rng(2))
+    rng = np.random.default_rng(5)
+    x, b = rng.standard_normal(s._Dim[top]), rng.standard_normal(s._Dim[top])
+    for t in (s, s8):
+        t.set_x(x, top); t.set_rhs(b, top); t.resid(top)
+    np.testing.assert_array_equal(s.get_res(top), s8.get_res(top))
+    for t in (s, s8):
+        t.GenMatRhs(0.0, top, 1)      # the assembled right-hand side again
+    rp, ci, v = s.csr(0, top)
+    n = s._Dim[top]
+    xref = spla.spsolve(sp.csr_matrix((v, ci, rp), shape=(n, n)).tocsc(), s.get_rhs())
+    # flexible GMRES in both precisions: on this 2D log-normal hierarchy the plain V-cycle iteration (FP64 or FP32
+    # refinement) is not contractive, in either layout
+    for prec in (femus.FM_PREC_FP32_FGMRES, femus.FM_PREC_FP64_FGMRES):
+        out = []
+        for t in (s, s8):
+            t.set_x(np.zeros(n))
+            rep = t.MGSolve(opts=t.opts(precision=prec, tol=1e-10, max_iter=200))
             assert rep.converged == 1
-            assert np.linalg.norm(s.get_x() - xref) <= 1e-8 * np.linalg.norm(xref)
+            out.append((rep.iters, t.get_x()))
+        assert out[0][0] == out[1][0]
+        np.testing.assert_array_equal(out[0][1], out[1][1])
+        assert np.linalg.norm(out[0][1] - xref) <= 1e-8 * np.linalg.norm(xref)
+    s8.close()
     s.close()
