@@ -230,7 +230,7 @@ void fm_krylov_free(fm_system *s);
 // views of a device matrix for the kernel launchers (SELL copy when present)
 #include "fm_kernels.h"
 template <typename T> inline void fm_sell_fields(CsrView<T> &v, const DevSELL &q) {
-  v.nsl = q.nsl; v.sl_off = q.sl_off; v.sl_w = q.sl_w; v.perm = q.perm; v.c16 = q.c16; v.base = q.base;
+  v.nsl = q.nsl; v.nslot = q.nslot; v.sl_off = q.sl_off; v.sl_w = q.sl_w; v.perm = q.perm; v.c16 = q.c16; v.base = q.base;
   v.sl_c32 = q.sl_c32; v.c32 = q.c32; v.sl_c8 = q.sl_c8; v.c8 = q.c8; v.sl_c16 = q.sl_c16; v.sl_c2 = q.sl_c2; v.c2 = q.c2;
 }
 template <typename T> CsrView<T> fm_view(const DevCSR &c);

@@ -11,7 +11,7 @@ struct CsrView {
   const T *v = nullptr;
   int lpr = 16;  // lanes per row
   // SELL-64 copy in internal DoF order (used when nsl > 0)
-  int64_t nsl = 0;
+  int64_t nsl = 0, nslot = 0;
   const int64_t *sl_off = nullptr, *sl_c32 = nullptr, *sl_c8 = nullptr, *sl_c16 = nullptr, *sl_c2 = nullptr;
   const int *sl_w = nullptr, *perm = nullptr, *base = nullptr, *c32 = nullptr;
   const unsigned *c2 = nullptr;

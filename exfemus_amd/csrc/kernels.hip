@@ -91,9 +91,18 @@ __global__ __launch_bounds__(256) void k_spmv(int64_t nrows, const int64_t *__re
 // absolute columns from c32 (32-bit).  With xcd_chunk > 0 the block
 // index is remapped so that the blocks the hardware places on one XCD (block
 // id modulo 8) walk one contiguous range of slices.
+// matrix stream loads: non-temporal (NT) for matrices that are read once per pass and would only evict the
+// gathered vector, ordinary for matrices small enough to stay in the Infinity Cache between the back-to-back
+// products of one smoothing burst
+template <bool NT, typename P>
+__device__ __forceinline__ P ldm(const P *p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+
 // one lane's row of a slice whose column offsets are BITS wide (8 or 16), packed 32 / BITS positions per
 // 32-bit word: the word of group g (positions g * PER ..) of this lane is cw[64 g]
-template <typename T, int BITS, int UNR>
+template <typename T, int BITS, int UNR, bool NT>
 __device__ __forceinline__ T sell_row_packed(const T *__restrict__ vp, const unsigned *__restrict__ cw,
                                              const int *__restrict__ bp, int w, const T *__restrict__ xg) {
   constexpr int PER = 32 / BITS;
@@ -105,9 +114,9 @@ __device__ __forceinline__ T sell_row_packed(const T *__restrict__ vp, const uns
     T v[UNR];
     unsigned p[UNR / PER];
 #pragma unroll
-    for (int u = 0; u < UNR; u++) v[u] = __builtin_nontemporal_load(vp + (k + u) * 64);
+    for (int u = 0; u < UNR; u++) v[u] = ldm<NT>(vp + (k + u) * 64);
 #pragma unroll
-    for (int g = 0; g < UNR / PER; g++) p[g] = __builtin_nontemporal_load(cw + (k / PER + g) * 64);
+    for (int g = 0; g < UNR / PER; g++) p[g] = ldm<NT>(cw + (k / PER + g) * 64);
     int c[UNR];
 #pragma unroll
     for (int u = 0; u < UNR; u++) c[u] = bp[k + u] + (int)((p[u / PER] >> (BITS * (u % PER))) & MASK);
@@ -126,10 +135,10 @@ __device__ __forceinline__ T sell_row_packed(const T *__restrict__ vp, const uns
     unsigned p[UNR / PER];
 #pragma unroll
     for (int u = 0; u < UNR; u++)
-      if (u < rem) v[u] = __builtin_nontemporal_load(vp + (k + u) * 64);
+      if (u < rem) v[u] = ldm<NT>(vp + (k + u) * 64);
 #pragma unroll
     for (int g = 0; g < UNR / PER; g++)
-      if (g * PER < rem) p[g] = __builtin_nontemporal_load(cw + (k / PER + g) * 64);
+      if (g * PER < rem) p[g] = ldm<NT>(cw + (k / PER + g) * 64);
 #pragma unroll
     for (int u = 0; u < UNR; u++)
       if (u < rem) xv[u] = xg[bp[k + u] + (int)((p[u / PER] >> (BITS * (u % PER))) & MASK)];
@@ -144,23 +153,23 @@ __device__ __forceinline__ T sell_row_packed(const T *__restrict__ vp, const uns
 // bp[k] + lane + a 2-bit residual, bits 2 (k % 16) of the lane's word cw[64 (k / 16)].  A batch's columns need
 // only that word and the scalar bases, so the word of the next 16 positions is loaded one chunk ahead and a
 // batch issues its value loads and its gathers together (one HBM round trip per batch, not HBM then L2).
-template <typename T, int UNR>
+template <typename T, int UNR, bool NT>
 __device__ __forceinline__ T sell_row_pred(const T *__restrict__ vp, const unsigned *__restrict__ cw,
                                            const int *__restrict__ bp, int lane, int k0, int k1,
                                            const T *__restrict__ xg) {
   static_assert(16 % UNR == 0, "a batch lies inside one residual word");
   T sum = 0;
   if (k0 >= k1) return sum;
-  unsigned p = __builtin_nontemporal_load(cw + (k0 >> 4) * 64);
+  unsigned p = ldm<NT>(cw + (k0 >> 4) * 64);
   int k = k0;
   while (k < k1) {
     const int kend = min(k1, ((k >> 4) + 1) << 4);   // end of this word's positions
     unsigned pn = 0;
-    if (kend < k1) pn = __builtin_nontemporal_load(cw + (kend >> 4) * 64);
+    if (kend < k1) pn = ldm<NT>(cw + (kend >> 4) * 64);
     for (; k + UNR <= kend; k += UNR) {
       T v[UNR], xv[UNR];
 #pragma unroll
-      for (int u = 0; u < UNR; u++) v[u] = __builtin_nontemporal_load(vp + (k + u) * 64);
+      for (int u = 0; u < UNR; u++) v[u] = ldm<NT>(vp + (k + u) * 64);
 #pragma unroll
       for (int u = 0; u < UNR; u++) xv[u] = xg[bp[k + u] + lane + (int)((p >> (2 * ((k + u) & 15))) & 3u)];
 #pragma unroll
@@ -171,7 +180,7 @@ __device__ __forceinline__ T sell_row_pred(const T *__restrict__ vp, const unsig
       T v[UNR], xv[UNR];
 #pragma unroll
       for (int u = 0; u < UNR; u++)
-        if (u < rem) v[u] = __builtin_nontemporal_load(vp + (k + u) * 64);
+        if (u < rem) v[u] = ldm<NT>(vp + (k + u) * 64);
 #pragma unroll
       for (int u = 0; u < UNR; u++)
         if (u < rem) xv[u] = xg[bp[k + u] + lane + (int)((p >> (2 * ((k + u) & 15))) & 3u)];
@@ -187,7 +196,7 @@ __device__ __forceinline__ T sell_row_pred(const T *__restrict__ vp, const unsig
 
 // sum over the positions [k0, k1) of one lane's row of slice s (k0 a multiple of UNR, so that it is also a
 // multiple of the 4 or 2 positions packed per 32-bit word of 8- or 16-bit offsets)
-template <typename T, int UNR>
+template <typename T, int UNR, bool NT>
 __device__ __forceinline__ T sell_range(int s, int lane, int k0, int k1, const int64_t *__restrict__ sl_off,
                                         const int64_t *__restrict__ sl_c32, const unsigned short *__restrict__ c16,
                                         const int *__restrict__ base, const int *__restrict__ c32,
@@ -197,7 +206,7 @@ __device__ __forceinline__ T sell_range(int s, int lane, int k0, int k1, const i
                                         const T *__restrict__ xg) {
   const int64_t off = sl_off[s];
   const int64_t o2 = sl_c2[s];
-  if (o2 >= 0) return sell_row_pred<T, UNR>(vals + off + lane, c2 + o2 + lane, base + (off >> 6), lane, k0, k1, xg);
+  if (o2 >= 0) return sell_row_pred<T, UNR, NT>(vals + off + lane, c2 + o2 + lane, base + (off >> 6), lane, k0, k1, xg);
   const int64_t o32 = sl_c32[s];
   const int64_t o8 = sl_c8[s], o16 = sl_c16[s];
   const T *vp = vals + off + lane + (int64_t)k0 * 64;
@@ -205,9 +214,9 @@ __device__ __forceinline__ T sell_range(int s, int lane, int k0, int k1, const i
   const int w = k1 - k0;
   if (w <= 0) return T(0);
   if (o8 >= 0)
-    return sell_row_packed<T, 8, UNR>(vp, reinterpret_cast<const unsigned *>(c8 + o8) + lane + (k0 / 4) * 64, bp, w, xg);
+    return sell_row_packed<T, 8, UNR, NT>(vp, reinterpret_cast<const unsigned *>(c8 + o8) + lane + (k0 / 4) * 64, bp, w, xg);
   if (o16 >= 0)
-    return sell_row_packed<T, 16, UNR>(vp, reinterpret_cast<const unsigned *>(c16 + o16) + lane + (k0 / 2) * 64, bp,
+    return sell_row_packed<T, 16, UNR, NT>(vp, reinterpret_cast<const unsigned *>(c16 + o16) + lane + (k0 / 2) * 64, bp,
                                        w, xg);
   const int *cp = c32 + o32 + lane + (int64_t)k0 * 64;
   T sum = 0;
@@ -216,9 +225,9 @@ __device__ __forceinline__ T sell_range(int s, int lane, int k0, int k1, const i
     T v[UNR];
     int c[UNR];
 #pragma unroll
-    for (int u = 0; u < UNR; u++) v[u] = __builtin_nontemporal_load(vp + (k + u) * 64);
+    for (int u = 0; u < UNR; u++) v[u] = ldm<NT>(vp + (k + u) * 64);
 #pragma unroll
-    for (int u = 0; u < UNR; u++) c[u] = __builtin_nontemporal_load(cp + (k + u) * 64);
+    for (int u = 0; u < UNR; u++) c[u] = ldm<NT>(cp + (k + u) * 64);
     T xv[UNR];
 #pragma unroll
     for (int u = 0; u < UNR; u++) xv[u] = xg[c[u]];
@@ -232,8 +241,8 @@ __device__ __forceinline__ T sell_range(int s, int lane, int k0, int k1, const i
 #pragma unroll
     for (int u = 0; u < UNR; u++)
       if (u < rem) {
-        v[u] = __builtin_nontemporal_load(vp + (k + u) * 64);
-        c[u] = __builtin_nontemporal_load(cp + (k + u) * 64);
+        v[u] = ldm<NT>(vp + (k + u) * 64);
+        c[u] = ldm<NT>(cp + (k + u) * 64);
       }
 #pragma unroll
     for (int u = 0; u < UNR; u++)
@@ -262,7 +271,7 @@ __global__ __launch_bounds__(256) void k_sell(int nsl, const int64_t *__restrict
   const int s = slist ? __builtin_amdgcn_readfirstlane(slist[w_idx]) : w_idx;
   const int lane = threadIdx.x & 63;
   const int row = perm[(int64_t)s * 64 + lane];
-  const T sum = sell_range<T, UNR>(s, lane, 0, sl_w[s], sl_off, sl_c32, c16, base, c32, sl_c8, c8, sl_c16, sl_c2, c2,
+  const T sum = sell_range<T, UNR, true>(s, lane, 0, sl_w[s], sl_off, sl_c32, c16, base, c32, sl_c8, c8, sl_c16, sl_c2, c2,
                                          vals, xg);
   if (row < 0) return;
   apply_ep<T, EP>(row, sum, xg, ep);
@@ -273,7 +282,7 @@ __global__ __launch_bounds__(256) void k_sell(int nsl, const int64_t *__restrict
 // in sequence: S waves (one block of 64 S threads) share a slice, wave j summing the positions
 // [j c, (j + 1) c) with c = ceil(w / S) rounded up to the unroll; the partial sums meet in LDS and wave 0
 // adds them in wave order and applies the epilogue.
-template <typename T, int EP, int UNR, int S>
+template <typename T, int EP, int UNR, int S, bool NT>
 __global__ __launch_bounds__(64 * S) void k_sell_split(int nsl, const int64_t *__restrict__ sl_off,
                                                        const int64_t *__restrict__ sl_c32, const int *__restrict__ sl_w,
                                                        const int *__restrict__ perm,
@@ -294,7 +303,7 @@ __global__ __launch_bounds__(64 * S) void k_sell_split(int nsl, const int64_t *_
   const int w = sl_w[s];
   const int c = ((w + S - 1) / S + UNR - 1) / UNR * UNR;
   const int k0 = min(w, j * c), k1 = min(w, k0 + c);
-  T sum = sell_range<T, UNR>(s, lane, k0, k1, sl_off, sl_c32, c16, base, c32, sl_c8, c8, sl_c16, sl_c2, c2, vals,
+  T sum = sell_range<T, UNR, NT>(s, lane, k0, k1, sl_off, sl_c32, c16, base, c32, sl_c8, c8, sl_c16, sl_c2, c2, vals,
                                    xg);
   if (j > 0) part[j - 1][lane] = sum;
   __syncthreads();
@@ -318,7 +327,11 @@ __global__ __launch_bounds__(64 * S) void k_sell_split(int nsl, const int64_t *_
 // FEMUS_SELL_LIST_SPLIT = 0 | 1 (default 1): products restricted to a slice list (the residual of one Vanka
 // colour) with W to 4 W entries run with S = 2.  Measured: the C5 step at 48^3 (top-level colour lists of
 // ~19,600 slices) 627 -> 611 ms mixed, 985 -> 941 ms FP64; at 96^3 (~154,000 slices) no gain.
+// FEMUS_SELL_TEMPORAL_MB = M (default 200; 0 = never): split-slice products over a matrix whose values and
+// column offsets take at most M MB read it with ordinary loads, so that it stays in the 256 MB Infinity Cache
+// across the back-to-back products of a smoothing burst; larger matrices stream with non-temporal loads.
 static int g_sell_unroll = -1, g_sell_xcd = -1, g_sell_split = -1, g_sell_list2 = -1;
+static int64_t g_sell_temporal = -1;
 static void sell_knobs() {
   if (g_sell_unroll >= 0) return;
   const char *u = getenv("FEMUS_SELL_UNROLL");
@@ -329,16 +342,26 @@ static void sell_knobs() {
   g_sell_xcd = (x && atoi(x) == 0) ? 0 : 1;
   if (g_sell_split < 0) g_sell_split = p ? std::max(0, atoi(p)) : 16384;
   g_sell_list2 = (q && atoi(q) == 0) ? 0 : 1;
+  const char *tm = getenv("FEMUS_SELL_TEMPORAL_MB");
+  g_sell_temporal = (int64_t)(tm ? std::max(0, atoi(tm)) : 200) << 20;
 }
 
 template <typename T, int EP, int S>
 static void launch_split(hipStream_t s, const CsrView<T> &A, const T *x, const EpArgs<T> &ep, int nsl) {
+  // ordinary loads for a matrix whose values and offsets fit the cache budget (FEMUS_SELL_TEMPORAL_MB)
+  const bool nt = A.nslot * (int64_t)(sizeof(T) + 1) > g_sell_temporal;
   if (g_sell_unroll == 16)
-    hipLaunchKernelGGL((k_sell_split<T, EP, 16, S>), dim3((unsigned)nsl), dim3(64 * S), 0, s, nsl, A.sl_off, A.sl_c32,
-                       A.sl_w, A.perm, A.c16, A.base, A.c32, A.sl_c8, A.c8, A.sl_c16, A.sl_c2, A.c2, A.sv, x, ep, A.slist);
+    hipLaunchKernelGGL((k_sell_split<T, EP, 16, S, true>), dim3((unsigned)nsl), dim3(64 * S), 0, s, nsl, A.sl_off,
+                       A.sl_c32, A.sl_w, A.perm, A.c16, A.base, A.c32, A.sl_c8, A.c8, A.sl_c16, A.sl_c2, A.c2, A.sv, x, ep,
+                       A.slist);
+  else if (nt)
+    hipLaunchKernelGGL((k_sell_split<T, EP, 8, S, true>), dim3((unsigned)nsl), dim3(64 * S), 0, s, nsl, A.sl_off,
+                       A.sl_c32, A.sl_w, A.perm, A.c16, A.base, A.c32, A.sl_c8, A.c8, A.sl_c16, A.sl_c2, A.c2, A.sv, x, ep,
+                       A.slist);
   else
-    hipLaunchKernelGGL((k_sell_split<T, EP, 8, S>), dim3((unsigned)nsl), dim3(64 * S), 0, s, nsl, A.sl_off, A.sl_c32,
-                       A.sl_w, A.perm, A.c16, A.base, A.c32, A.sl_c8, A.c8, A.sl_c16, A.sl_c2, A.c2, A.sv, x, ep, A.slist);
+    hipLaunchKernelGGL((k_sell_split<T, EP, 8, S, false>), dim3((unsigned)nsl), dim3(64 * S), 0, s, nsl, A.sl_off,
+                       A.sl_c32, A.sl_w, A.perm, A.c16, A.base, A.c32, A.sl_c8, A.c8, A.sl_c16, A.sl_c2, A.c2, A.sv, x, ep,
+                       A.slist);
 }
 
 template <typename T, int EP>
