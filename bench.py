@@ -569,13 +569,17 @@ def main():
                     f"FP32-MG+FP64-{args.outer.upper()} to {args.tol:g}")
         data = "synthetic (seeded log-normal k, f = 1; coarse-level k = geometric mean of the children)"
     else:
-        # coloured Vanka 3/3 with the same sweep count on every level (the reference's MGStep doubles the
-        # sweeps below the top; here that costs more than the iterations it saves: 409 vs 333 ms at C3)
-        nu = args.nu or 3
+        # coloured Vanka with the same sweep count on every level (the reference's MGStep doubles the sweeps
+        # below the top; here that costs more than the iterations it saves: 409 vs 333 ms at C3).  Channel: 2/2
+        # sweeps damped by 0.7 (23 iterations, 233 ms; 3/3 by 0.5: 18 iterations, 262 ms; 2/2 by 0.5: 36; one
+        # sweep does not converge); cavity: 3/3 by 0.5
+        chan = args.workload == "channel"
+        nu = args.nu or (2 if chan else 3)
+        omega = args.omega if args.omega is not None else (0.7 if chan else 0.5)
         m, s, t_asm = build_ns(F, ctx, args.workload)
         kw = dict(nu_pre=nu, nu_post=nu, smoother=F.FM_SMOOTH_VANKA, vanka_colored=args.vanka_colored, coarse_mult=1,
-                  omega=args.omega)
-        smoother = f"coloured element Vanka {nu}/{nu}, same sweeps on every level"
+                  omega=omega)
+        smoother = f"coloured element Vanka {nu}/{nu} (damping {omega:g}), same sweeps on every level"
         workload = ("3D NS channel hex27 Q2-Q1 96x24x24, 4 levels, Re 400, Oseen step from the inflow profile"
                     if args.workload == "channel" else
                     "2D NS lid-driven cavity quad9 Q2-Q1 32^2, 3 levels, Re 100, Oseen step from the lid data")
