@@ -513,7 +513,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--outer", default="fgmres", choices=["ir", "fgmres"])
     ap.add_argument("--nu", type=int, default=None)
-    ap.add_argument("--omega", type=float, default=None, help="Vanka damping (default 0.5)")
+    ap.add_argument("--omega", type=float, default=None, help="Vanka damping (default 0.7 channel, 0.5 otherwise)")
+    ap.add_argument("--gamma", type=int, default=1, help="cycle index below the top (1 V-cycle, 2 W-cycle)")
     ap.add_argument("--vanka-colored", type=int, default=1, help="1: multiplicative over element colours, 0: additive")
     ap.add_argument("--cheb-lo", type=float, default=0.1)
     ap.add_argument("--coarse-mult", type=int, default=None,
@@ -562,7 +563,7 @@ def main():
         # the same sweep count on every level (the reference's MGStep doubles them below the top; here the
         # doubled coarse sweeps cost more than they save: 35.5 vs 29.0 ms, 11 iterations either way)
         cm = 1 if args.coarse_mult is None else args.coarse_mult
-        kw = dict(nu_pre=nu, nu_post=nu, cheb_lo=args.cheb_lo, coarse_mult=cm, smoother=args.smoother)
+        kw = dict(nu_pre=nu, nu_post=nu, cheb_lo=args.cheb_lo, coarse_mult=cm, smoother=args.smoother, gamma=args.gamma)
         smoother = (f"Chebyshev-Jacobi {nu}/{nu}" + (" (4th kind)" if args.smoother == F.FM_SMOOTH_CHEBYSHEV4 else "") +
                     (", same sweeps on every level" if cm == 1 else ", sweeps doubled below the top" if cm in (0, 2) else ""))
         workload = (f"3D heat hex27 {args.nx0 * 16}^3, {args.nlev} levels, lognormal k, "
@@ -577,8 +578,8 @@ def main():
         nu = args.nu or (2 if chan else 3)
         omega = args.omega if args.omega is not None else (0.7 if chan else 0.5)
         m, s, t_asm = build_ns(F, ctx, args.workload)
-        kw = dict(nu_pre=nu, nu_post=nu, smoother=F.FM_SMOOTH_VANKA, vanka_colored=args.vanka_colored, coarse_mult=1,
-                  omega=omega)
+        kw = dict(nu_pre=nu, nu_post=nu, smoother=F.FM_SMOOTH_VANKA, vanka_colored=args.vanka_colored,
+                  coarse_mult=1 if args.coarse_mult is None else args.coarse_mult, omega=omega, gamma=args.gamma)
         smoother = f"coloured element Vanka {nu}/{nu} (damping {omega:g}), same sweeps on every level"
         workload = ("3D NS channel hex27 Q2-Q1 96x24x24, 4 levels, Re 400, Oseen step from the inflow profile"
                     if args.workload == "channel" else
