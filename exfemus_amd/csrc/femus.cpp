@@ -1115,8 +1115,12 @@ static int egather_ready(fm_system *s, int L, bool &use) {
     const size_t index = (size_t)l.A.sell.nslot * 3 + ne * 2 + nf * 12;
     if (held + index > kGatherBudget) { l.eg.unusable = true; return 0; }
     std::string err;
-    // at most 24 KB of staging per block: six or more blocks resident per CU
-    if (fm_build_egather(s->ctx->stream, l, nl, 24 * 1024, l.eg, err)) {
+    // staging per block (FEMUS_EGATHER_LDS_KB): 24 KB for scalar Q2 rows (six or more blocks resident per CU;
+    // 56 KB took the 64^3 heat gather from 3.49 to 3.79 ms), 56 KB for Q2-Q1 rows, whose element rows fill 24 KB
+    // with two lanes (96x24x24 channel: 9.41 -> 8.60 ms, four or eight lanes per block)
+    const char *lk = getenv("FEMUS_EGATHER_LDS_KB");
+    const size_t lds = (size_t)(lk && atoi(lk) >= 8 && atoi(lk) <= 64 ? atoi(lk) : (nl > 32 ? 56 : 24)) * 1024;
+    if (fm_build_egather(s->ctx->stream, l, nl, lds, l.eg, err)) {
       (void)hipGetLastError();
       l.eg.unusable = true;
       return 0;

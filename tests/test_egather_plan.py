@@ -29,6 +29,7 @@ CASES = [
     ((3, 2, 2, 1, 2, 1, 0, 4096, 8192, 4, 1), 2),
     ((2, 2, 2, 1, 4, 1, 0, 4096, 49152, 6, 0), 16),
     ((3, 1, 2, 1, 3, 3, 1, 512, 20000, 7, 1), 2),
+    ((3, 2, 2, 2, 3, 3, 1, 4096, 57344, 5, 0), 4),    # the library's Q2-Q1 budget (56 KB)
 ]
 
 
