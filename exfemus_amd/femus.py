@@ -28,7 +28,8 @@ FM_SMOOTH_CHEBYSHEV, FM_SMOOTH_JACOBI, FM_SMOOTH_VANKA, FM_SMOOTH_CHEBYSHEV4 = 0
 FM_ENOCONV = -3
 
 # storage format of the SELL SpMV
-SPMV_FORMAT = "SELL-64 sigma4096 slab-grouped node order, 8-, 16- or 32-bit col offsets per slice"
+SPMV_FORMAT = ("SELL-64 sigma4096 slab-grouped node order; columns per slice as 2-bit lane-predicted residuals or "
+               "8-, 16- or 32-bit offsets")
 
 # Exported entry points (the C ABI of include/femus_hip.h)
 EXPORTS = [
