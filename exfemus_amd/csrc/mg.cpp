@@ -425,38 +425,18 @@ __global__ void k_multi_dot(int64_t n, int k, int kk, const double *__restrict__
     part[(int64_t)(j0 + threadIdx.x) * gridDim.x + blockIdx.x] = t;
   }
 }
-// out[j] = sum of the nb partial sums of dot product j (0 when skip is non-null and skip[0] == 0)
-__global__ void k_multi_sum(int k, int nb, const double *part, double *out, const double *skip) {
-  int j = blockIdx.x;
-  if (skip && skip[0] == 0.) {
-    if (threadIdx.x == 0) out[j] = 0.;
-    return;
-  }
-  double s = 0.;
-  for (int i = threadIdx.x; i < nb; i += blockDim.x) s += part[(int64_t)j * nb + i];
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-  __shared__ double sh[4];
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) out[j] = sh[0] + sh[1] + sh[2] + sh[3];
-}
-// Selective reorthogonalisation (Daniel, Gragg, Kaufman and Stewart, Math. Comp. 30 (1976), eta = 1/sqrt 2):
-// after one classical Gram-Schmidt pass with coefficients h[0 .. k) and ww = h[k] = |w|^2 before it,
-// |w'|^2 = ww - sum h_j^2; a second pass is needed only when |w'| < |w| / sqrt 2.  flag[0] = 1 or 0.
-__global__ void k_dgks(int k, const double *h, double *flag) {
-  double proj = 0.;
-  for (int j = 0; j < k; j++) proj += h[j] * h[j];
-  const double ww = h[k];
-  flag[0] = (ww - proj < 0.5 * ww) ? 1. : 0.;
-}
-// w -= sum_j h[j] V[j] (nothing when skip is non-null and skip[0] == 0)
+// w = (w - sum_{j < k} h[j] V[j]) scale[0], and its FP32 copy in f when f is non-null (nothing when skip is non-null and
+// skip[0] == 0)
 __global__ void k_multi_axpy(int64_t n, int k, const double *__restrict__ V, const double *__restrict__ h, double *w,
-                             const double *skip) {
+                             const double *scale, float *f, const double *skip) {
   if (skip && skip[0] == 0.) return;
+  const double sc = scale[0];
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     double s = w[i];
     for (int j = 0; j < k; j++) s -= h[j] * V[(int64_t)j * n + i];
+    s *= sc;
     w[i] = s;
+    if (f) f[i] = (float)s;
   }
 }
 // x += sum_j y[j] Z[j]
@@ -479,7 +459,8 @@ __global__ void k_scale_inv(int64_t n, double *w, const double *nrm) {
 // Hessenberg state of one restart cycle, on the device (offsets into fm_system::kH, m = restart length):
 // H [(m + 1) m] row-major, cs [m], sn [m], g [m + 1], y [m], rel [m] (residual estimate after each step),
 // st [8] (st[0]: columns before the first breakdown, -1 while there is none), then the Gram-Schmidt scratch
-// hd [256]: [0, 64) first pass, [64, 128) second pass, [128] norm of the new vector, [129] reorthogonalisation flag
+// hd [256]: [0, 64) first pass, [64, 128) second pass, [128] norm of the new vector, [129] reorthogonalisation flag,
+// [130] |r| at a restart, [131] scale of the Gram-Schmidt update
 struct KryLayout {
   int64_t H, cs, sn, g, y, rel, st, hd, total;
   explicit KryLayout(int m) {
@@ -496,9 +477,8 @@ __global__ void k_fgmres_start(int m, double *g, double *st, const double *beta)
 // Column j of the Hessenberg matrix from the Gram-Schmidt coefficients (two passes) and the new vector's
 // norm, the earlier Givens rotations applied to it, the rotation that zeroes its subdiagonal, the rotated
 // right-hand side g and the residual estimate rel[j] = |g[j + 1]| / bn.  One thread: at most 60 columns.
-__global__ void k_arnoldi_col(int j, int m, const double *hd, double *H, double *cs, double *sn, double *g,
-                              double *rel, double *st, double bn) {
-  if (threadIdx.x != 0) return;
+__device__ void arnoldi_col(int j, int m, const double *hd, double *H, double *cs, double *sn, double *g,
+                            double *rel, double *st, double bn) {
   for (int k = 0; k <= j; k++) H[(int64_t)k * m + j] = hd[k] + hd[64 + k];
   const double hn = hd[128];
   H[(int64_t)(j + 1) * m + j] = hn;
@@ -519,6 +499,45 @@ __global__ void k_arnoldi_col(int j, int m, const double *hd, double *H, double 
   // a zero new vector (the Krylov space is invariant: the solution is in it) ends the usable columns
   if (!(hn > 0.) && st[0] < 0.) st[0] = (double)(j + 1);
 }
+// One Gram-Schmidt pass of Arnoldi step j, after k_multi_dot has left the nb block partials of the j + 2 dot
+// products (w with V_0 .. V_j, then |w|^2): the four waves sum them (coefficients to hd[64 pass + t]); pass 0 takes
+// the selective reorthogonalisation test (Daniel, Gragg, Kaufman and Stewart, Math. Comp. 30 (1976), eta = 1/sqrt 2):
+// a second pass (hd[129] = 1) only when |w'|^2 = |w|^2 - sum h^2 < |w|^2 / 2.  The pass that completes the step
+// (pass 0 without a second pass, or pass 1) takes the new vector's norm from the same identity -- no cancellation:
+// the test bounds it below by half of |w|^2 -- forms the Hessenberg column and its rotation (arnoldi_col) and sets
+// the update's scale hd[131] = 1 / norm (1 while a second pass follows, or when the norm is zero).  Pass 1 does
+// nothing when no second pass was asked for.
+__global__ void k_gs_step(int j, int m, int nb, const double *part, double *hd, int pass, double *H, double *cs,
+                          double *sn, double *g, double *rel, double *st, double bn) {
+  if (pass == 1 && hd[129] == 0.) return;
+  double *out = hd + 64 * pass;
+  const int kk = j + 2, wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+  for (int t = wv; t < kk; t += (int)(blockDim.x >> 6)) {
+    double s = 0.;
+    for (int i = ln; i < nb; i += 64) s += part[(int64_t)t * nb + i];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (ln == 0) out[t] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double proj = 0.;
+  for (int t = 0; t <= j; t++) proj += out[t] * out[t];
+  const double ww = out[j + 1];
+  if (pass == 0) {
+    const bool again = ww - proj < 0.5 * ww;
+    hd[129] = again ? 1. : 0.;
+    if (again) {
+      hd[131] = 1.;
+      return;
+    }
+    for (int t = 0; t <= j; t++) hd[64 + t] = 0.;
+  }
+  const double hn = ww - proj > 0. ? sqrt(ww - proj) : 0.;
+  hd[128] = hn;
+  hd[131] = hn > 0. ? 1. / hn : 1.;
+  arnoldi_col(j, m, hd, H, cs, sn, g, rel, st, bn);
+}
+
 // y = H^-1 g over the first jn columns (fewer after a breakdown; the rest of y is zero)
 __global__ void k_fgmres_y(int jn, int m, const double *H, const double *g, double *y, const double *st) {
   if (threadIdx.x != 0) return;
@@ -640,26 +659,23 @@ static int solve_fgmres(fm_system *s, const fm_mg_opts *o, std::vector<LOps> &op
           if (hipMemcpyAsync(l->x, l->sx, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
           if (hipMemcpyAsync(l->b, l->sb, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) { rc = FM_EHIP; goto done; }
         } else {
-          if ((rc = fmk::to_f32(st, n, vj, l->bf, nullptr)) != 0) goto done;
+          // the FP32 copy of v_j: v_0 here, v_j (j > 0) written by the update that formed it
+          if (j == 0 && (rc = fmk::to_f32(st, n, vj, l->bf, nullptr)) != 0) goto done;
           if (!smoother_writes_x<float>(o, o->nu_pre) && (rc = fmk::fill<float>(st, n, l->xf, 0.f)) != 0) goto done;
           if ((rc = cycle_top<float>(s, o, ops, adj, true)) != 0) goto done;
           if ((rc = fmk::from_f32(st, n, l->xf, zj)) != 0) goto done;
         }
-        // w = A z_j; classical Gram-Schmidt (second pass only when the DGKS test asks), normalisation and the
-        // Hessenberg column, all on the device
+        // w = A z_j; classical Gram-Schmidt (second pass only when the DGKS test asks), the norm, the Hessenberg
+        // column and the normalisation (with the FP32 copy of v_{j+1} for the next FP32 cycle), all on the device:
+        // per pass one dot sweep (|w|^2 riding along), one single-block step, one update
         if ((rc = fmk::spmv(st, A, zj, w)) != 0) goto done;
+        float *fnext = o->precision == FM_PREC_FP32_FGMRES ? l->bf : nullptr;
         for (int pass = 0; pass < 2; pass++) {
-          // pass 0 also forms |w|^2 (kk = j + 2) for the reorthogonalisation test; pass 1 runs only if it asks
-          const int kk = pass == 0 ? j + 2 : j + 1;
           const double *skip = pass == 0 ? nullptr : hd + 129;
-          hipLaunchKernelGGL(k_multi_dot, dim3(nb, (kk + 7) / 8), dim3(256), 0, st, n, j + 1, kk, V, w, part, skip);
-          hipLaunchKernelGGL(k_multi_sum, dim3(kk), dim3(256), 0, st, kk, nb, part, hd + 64 * pass, skip);
-          if (pass == 0) hipLaunchKernelGGL(k_dgks, dim3(1), dim3(1), 0, st, j + 1, hd, hd + 129);
-          hipLaunchKernelGGL(k_multi_axpy, dim3(gv), dim3(256), 0, st, n, j + 1, V, hd + 64 * pass, w, skip);
+          hipLaunchKernelGGL(k_multi_dot, dim3(nb, (j + 2 + 7) / 8), dim3(256), 0, st, n, j + 1, j + 2, V, w, part, skip);
+          hipLaunchKernelGGL(k_gs_step, dim3(1), dim3(256), 0, st, j, m, nb, part, hd, pass, H, cs, sn, g, relh, stt, bn);
+          hipLaunchKernelGGL(k_multi_axpy, dim3(gv), dim3(256), 0, st, n, j + 1, V, hd + 64 * pass, w, hd + 131, fnext, skip);
         }
-        if ((rc = fmk::dot_dev<double>(st, n, w, w, c->scratch, hd + 128, 1)) != 0) goto done;
-        hipLaunchKernelGGL(k_scale_inv, dim3(gv), dim3(256), 0, st, n, w, hd + 128);
-        hipLaunchKernelGGL(k_arnoldi_col, dim3(1), dim3(64), 0, st, j, m, hd, H, cs, sn, g, relh, stt, bn);
         if (hipMemcpyAsync(s->kpin + j, relh + j, sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
             hipEventRecord(s->kev[j], st) != hipSuccess) { rc = FM_EHIP; goto done; }
         j++;
