@@ -675,32 +675,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         for (int y = 0; y < TI; y++) put(row, jv * N2 + tj + 9 * y, K[x][y], keep);
       }
     }
-    // B^T (velocity rows, pressure columns) and B (pressure rows, velocity columns)
+    // B^T (velocity rows, pressure columns) and B (pressure rows, velocity columns): B = -(B^T)^T entry for entry
+    // (the same products in the same order), so each pair is formed once; the B^T entry is dropped in Dirichlet
+    // velocity rows, the B entry is always kept
     constexpr int NBT = DIM * N2 * N1;
-    for (int u = tid; u < 2 * NBT; u += blockDim.x) {
-      int row, col;
+    for (int u = tid; u < NBT; u += blockDim.x) {
+      const int row = u / N1, j = u % N1, iv = row / N2, i = row % N2;
       double K = 0.;
-      bool keep = true;
-      if (u < NBT) {
-        row = u / N1;
-        const int j = u % N1, iv = row / N2, i = row % N2;
-        col = DIM * N2 + j;
-        keep = s_bcv[row] != 0;
-        if (!keep) {
-          if (TOE) put(row, col, 0., false);
-          continue;
-        }
 #pragma unroll 3
-        for (int q = 0; q < NG; q++) K -= s_JxW[q] * s_psi[j * NG + q] * s_G[(q * DIM + iv) * N2 + i];
-      } else {
-        const int u2 = u - NBT, ip = u2 / (DIM * N2);
-        col = u2 % (DIM * N2);
-        const int jv = col / N2, j = col % N2;
-        row = DIM * N2 + ip;
-#pragma unroll 3
-        for (int q = 0; q < NG; q++) K += s_JxW[q] * s_psi[ip * NG + q] * s_G[(q * DIM + jv) * N2 + j];
-      }
-      put(row, col, K, keep);
+      for (int q = 0; q < NG; q++) K -= s_JxW[q] * s_psi[j * NG + q] * s_G[(q * DIM + iv) * N2 + i];
+      put(row, DIM * N2 + j, K, s_bcv[row] != 0);
+      put(DIM * N2 + j, row, -K, true);
     }
     // the pressure-pressure block is zero (no stabilisation)
     if (TOE)
