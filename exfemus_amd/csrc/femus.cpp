@@ -750,8 +750,10 @@ extern "C" int fm_system_init(fm_system *s) {
     FM_CHECK(dalloc(&l.conn, (size_t)l.nel * m.n2));
     FM_CHECK(dalloc(&l.eldof, (size_t)l.nel * nloc));
     FM_CHECK(dalloc(&l.eldofi, (size_t)l.nel * nloc));
-    FM_CHECK(dalloc(&l.elpos, (size_t)l.nel * nloc * nloc));
-    FM_CHECK(dalloc(&l.elslot, (size_t)l.nel * nloc));
+    FM_CHECK(dalloc(&l.elpos, (size_t)l.nel * nloc * nloc + FM_ELMAP_PAD));
+    FM_CHECK(dalloc(&l.elslot, (size_t)l.nel * nloc + FM_ELMAP_PAD));
+    FM_HIP(hipMemsetAsync(l.elpos + (size_t)l.nel * nloc * nloc, 0xFF, FM_ELMAP_PAD * sizeof(unsigned short), st));
+    FM_HIP(hipMemsetAsync(l.elslot + (size_t)l.nel * nloc, 0xFF, FM_ELMAP_PAD * sizeof(int64_t), st));
     FM_CHECK(dalloc(&l.bmask, l.nel));
     FM_CHECK(dalloc(&l.color_elems, l.nel));
     FM_HIP(hipMemcpyAsync(l.conn, m.conn[L].data(), m.conn[L].size() * sizeof(int), hipMemcpyHostToDevice, st));

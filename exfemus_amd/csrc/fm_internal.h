@@ -97,6 +97,10 @@ struct EGather {
   size_t bytes = 0;
 };
 
+// entries past the end of the element -> slot maps: the 3D patch inversion reads its 89 x 89 map block as a
+// 96 x 96 tile (up to 95 * 89 + 95 = 8550 entries from a patch's first one, 629 past the last patch's block)
+constexpr size_t FM_ELMAP_PAD = 1024;
+
 struct Level {
   int64_t n = 0;           // _Dim[L]
   int nel = 0;
@@ -122,7 +126,8 @@ struct Level {
   int *eldof = nullptr;    // nel * nloc dofs (reference numbering)
   int *eldofi = nullptr;   // nel * nloc dofs (internal numbering)
   // element map of A (the assembly writes the SELL values directly): entry (a, b) of element e sits at SELL
-  // slot elslot[e][a] + 64 elpos[e][a][b] (elpos 0xFFFF: no entry)
+  // slot elslot[e][a] + 64 elpos[e][a][b] (elpos 0xFFFF: no entry); both arrays carry FM_ELMAP_PAD entries
+  // more (0xFFFF / -1), read by the Vanka patch inversion's padded tiles
   unsigned short *elpos = nullptr;  // nel * nloc * nloc positions within the row
   int64_t *elslot = nullptr;        // nel * nloc: slot of position 0 of the row of each element DoF (-1: none)
   int64_t *dslot = nullptr;         // n: SELL slot of the diagonal of internal row i

@@ -24,112 +24,236 @@
 #include "fm_kernels.h"
 #include "fm_device.h"
 
-// Inverse of every element patch matrix K_e (NL x NL, gathered from the level
-// matrix through the element -> slot map), FP64, by in-place exchange steps:
-// the step on pivot (r, k), a = M[r][k], swaps the roles of unknown k and
+// Inverse of every element patch matrix K_e (NL x NL, gathered from the level matrix through the element ->
+// slot map), FP64, by exchange steps: the step on pivot (r, k), a = M[r][k], swaps the roles of unknown k and
 // equation r,
 //   M[r][k] = 1/a, M[r][j] = -M[r][j]/a, M[i][k] = M[i][k]/a, M[i][j] -= M[i][k] M[r][j]/a,
-// with r chosen by partial pivoting among the rows not used yet (no row is
-// moved); afterwards (K_e^-1)[k][prow[q]] = M[prow[k]][q].  The matrix lives
-// in registers: thread (ty, tx) of a G x G grid owns the TR x TR tile at rows
-// ty*TR, columns tx*TR; each step publishes only the pivot row and column in
-// LDS (double-buffered, two barriers per step).
+// with r chosen by partial pivoting among the rows not used yet (no row is moved; ties go to the lower row);
+// afterwards (K_e^-1)[k][piv[q]] = M[piv[k]][q].
+//
+// The steps run in blocks of TR columns.  The matrix lives in registers: thread (ty, tx) of a G x G grid owns
+// the TR x TR tile at rows ty*TR, columns tx*TR (96 x 96 for the 89 x 89 patch).  For block kt the wavefront
+// holding tile column kt takes the block's TR columns through LDS (rows lane and lane + 64 per lane) and runs
+// their TR exchange steps alone: pivot search by a wavefront arg-max (DPP inside rows of 16 lanes, a swizzle
+// across row pairs, lanes 0 and 32), the pivot row broadcast by readlane, no barrier.  Exchange steps compose,
+// so for the pivot rows P and the panel columns K of the block,
+//   M'[i][j] = (i in P ? 0 : M[i][j]) - sum_q M'[i][K_q] M[P_q][j]        (j outside K),
+// where M'[., K] is the factored panel; every tile takes that rank-TR update after the pivot rows M[P_q][.]
+// are published (two barriers per block instead of two per step).  Entries -delta in the published pivot
+// rows at the panel's own columns make the same update leave the factored panel in the panel's tiles.
+// The gather reads the element map for rows and columns up to G*TR (the map is padded by FM_ELMAP_PAD).
+// waves_per_eu(3): 168 registers, three 256-thread patches per CU.
+
+// lane l's copy of v (l uniform)
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// one arg-max stage against the lane that the DPP control CTRL names (ties to the smaller index)
+template <int CTRL>
+__device__ __forceinline__ void argmax_dpp(double &v, int &i) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)u, CTRL, 0xF, 0xF, false);
+  const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, false);
+  const int oi = __builtin_amdgcn_mov_dpp(i, CTRL, 0xF, 0xF, false);
+  const double ov = __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+  const bool take = (ov > v) | ((ov == v) & (oi < i));
+  v = take ? ov : v;
+  i = take ? oi : i;
+}
+
+// largest v over the wavefront, ties to the smallest i; every lane gets the result
+__device__ __forceinline__ void wave_argmax(double &v, int &i) {
+  argmax_dpp<0xB1>(v, i);              // quad_perm [1,0,3,2]
+  argmax_dpp<0x4E>(v, i);              // quad_perm [2,3,0,1]
+  argmax_dpp<0x141>(v, i);             // row_half_mirror
+  argmax_dpp<0x140>(v, i);             // row_mirror: every lane holds its row's result
+  {                                    // rows 0/1 and 2/3: swizzle xor 16 inside each half
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_ds_swizzle((int)(unsigned)u, 0x401F);
+    const unsigned hi = (unsigned)__builtin_amdgcn_ds_swizzle((int)(unsigned)(u >> 32), 0x401F);
+    const int oi = __builtin_amdgcn_ds_swizzle(i, 0x401F);
+    const double ov = __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+    const bool take = (ov > v) | ((ov == v) & (oi < i));
+    v = take ? ov : v;
+    i = take ? oi : i;
+  }
+  const double v0 = readlane_d(v, 0), v1 = readlane_d(v, 32);
+  const int i0 = __builtin_amdgcn_readlane(i, 0), i1 = __builtin_amdgcn_readlane(i, 32);
+  const bool second = (v1 > v0) | ((v1 == v0) & (i1 < i0));
+  v = second ? v1 : v0;
+  i = second ? i1 : i0;
+}
+
+// LDS writes of this wavefront visible to its own later LDS reads (no other wavefront involved)
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 template <int NL, int G, int TR>
-__global__ __launch_bounds__(G * G) void k_vanka_invert(int nel, const unsigned short *__restrict__ elpos,
-                                                      const int64_t *__restrict__ elslot, const double *__restrict__ Av,
-                                                      double *__restrict__ vinv, float *__restrict__ vinvf,
-                                                      int *__restrict__ status) {
-  constexpr int NP = G * TR, NT = G * G, NW = (NT + 63) / 64;
-  static_assert(NP >= NL && TR <= 32, "tile grid does not cover the patch");
-  __shared__ double s_row[2][NP], s_col[2][NP], s_kv[2][G], s_red[NW];
-  __shared__ int s_ki[2][G], s_prow[NP], s_rowstep[NP];
-  const int tid = threadIdx.x, ty = tid / G, tx = tid % G, lane = tid & 63, wid = tid >> 6;
+__global__ __launch_bounds__(G * G) __attribute__((amdgpu_waves_per_eu(3))) void k_vanka_invert(
+    int nel, const unsigned short *__restrict__ elpos, const int64_t *__restrict__ elslot,
+    const double *__restrict__ Av, double *__restrict__ vinv, float *__restrict__ vinvf, int *__restrict__ status) {
+  constexpr int NP = G * TR, NT = G * G, NW = NT / 64, NR = (NP + 63) / 64, NB = (NL + TR - 1) / TR;
+  constexpr int GW = 64 / G;           // tile columns per wavefront
+  static_assert(NP >= NL && NT % 64 == 0 && 64 % G == 0 && NP - 1 + (NP - 1) * NL < NL * NL + FM_ELMAP_PAD,
+                "tile grid");
+  __shared__ __align__(16) double s_pan[2][TR][NP];   // the block's columns after its exchange steps
+  __shared__ __align__(16) double s_a12[2][TR][NP];   // the block's pivot rows before them
+  __shared__ double s_red[NW];
+  __shared__ int s_piv[NP], s_step[NP], s_flag;        // pivot row of step k, step of pivot row r, singular
+  const int tid = threadIdx.x, tx = tid / G, ty = tid % G, lane = tid & 63, wid = tid >> 6;
   const int i0 = ty * TR, j0 = tx * TR;
   for (int e = xcd_block(); e < nel; e += gridDim.x) {
     const int64_t base = (int64_t)e * NL * NL;
     double m[TR][TR];
     double amax = 0.;
+    {
+      // branch-free gather in halves of the tile: all element-map loads of a half before its value loads;
+      // entries outside the patch or the pattern read slot 0 and are replaced by 0
+      constexpr int TH = (TR + 1) / 2;
+      const unsigned short *ep = elpos + base + i0 * NL + j0;
+      const int64_t *sp = elslot + (int64_t)e * NL + i0;
 #pragma unroll
-    for (int a = 0; a < TR; a++)
+      for (int h = 0; h < TR; h += TH) {
+        unsigned short kk[TH][TR];
+        int64_t rs[TH];
 #pragma unroll
-      for (int b = 0; b < TR; b++) {
-        const int i = i0 + a, j = j0 + b;
-        double v = 0.;
-        if (i < NL && j < NL) {
-          const unsigned k = elpos[base + i * NL + j];
-          const int64_t r = elslot[(int64_t)e * NL + i];
-          v = (k != 0xFFFFu && r >= 0) ? Av[r + 64 * (int64_t)k] : 0.0;
+        for (int a = 0; a < TH; a++) {
+          rs[a] = sp[h + a];
+#pragma unroll
+          for (int b = 0; b < TR; b++) kk[a][b] = ep[(h + a) * NL + b];
         }
-        m[a][b] = v;
-        amax = fmax(amax, fabs(v));
+#pragma unroll
+        for (int a = 0; a < TH; a++)
+#pragma unroll
+          for (int b = 0; b < TR; b++) {
+            const bool ok = i0 + h + a < NL && j0 + b < NL && rs[a] >= 0 && kk[a][b] != 0xFFFFu;
+            const double v = Av[ok ? rs[a] + 64 * (int64_t)kk[a][b] : 0];
+            m[h + a][b] = ok ? v : 0.;
+            amax = fmax(amax, fabs(m[h + a][b]));
+          }
       }
+    }
     for (int off = 32; off > 0; off >>= 1) amax = fmax(amax, __shfl_xor(amax, off, 64));
-    __syncthreads();                   // the previous patch's readers of s_red / s_prow / s_rowstep are done
+    __syncthreads();                   // the previous patch's readers of s_piv / s_step / s_flag are done
     if (lane == 0) s_red[wid] = amax;
+    for (int t = tid; t < NP; t += NT) s_step[t] = -1;
+    if (tid == 0) s_flag = 0;
     __syncthreads();
     double mx = 0.;
+#pragma unroll
     for (int w = 0; w < NW; w++) mx = fmax(mx, s_red[w]);
     const double tiny = 1e-14 * mx;
-    unsigned used = 0;                 // bit a: row i0 + a already has a pivot
-    bool singular = false;
-    for (int k = 0; k < NL; k++) {
-      const int buf = k & 1, kt = k / TR, kb = k - kt * TR;
-      if (tx == kt) {                  // candidates of column k among this tile's free rows
-        double bv = -1.;
-        int bi = NP;
+    for (int kt = 0; kt < NB; kt++) {
+      const int k0 = kt * TR, nk = min(TR, NL - k0), buf = kt & 1;
+      // the wavefront holding tile column kt runs the block's steps (none once the patch is singular: the
+      // remaining block updates then work on stale data and the patch is written as zeros)
+      if (wid == kt / GW && !s_flag) {
+        if (tx == kt)
 #pragma unroll
-        for (int a = 0; a < TR; a++) {
-          double v = 0.;
+          for (int b = 0; b < TR; b++)
 #pragma unroll
-          for (int b = 0; b < TR; b++) v = (b == kb) ? m[a][b] : v;
-          v = fabs(v);
-          if (i0 + a < NL && !((used >> a) & 1u) && v > bv) { bv = v; bi = i0 + a; }
+            for (int a = 0; a < TR; a++) s_pan[buf][b][i0 + a] = m[a][b];
+        wave_lds_sync();
+        double p[NR][TR];
+        bool fr[NR];                   // row lane + 64 s free (inside the patch, not a pivot row yet)
+#pragma unroll
+        for (int s = 0; s < NR; s++) {
+          const int row = lane + 64 * s;
+          const bool in = row < NP;
+#pragma unroll
+          for (int b = 0; b < TR; b++) p[s][b] = in ? s_pan[buf][b][row] : 0.;
+          fr[s] = row < NL && s_step[row] < 0;
         }
-        s_kv[buf][ty] = bv;
-        s_ki[buf][ty] = bi;
+        bool sing = false;
+#pragma unroll
+        for (int kb = 0; kb < TR; kb++) {
+          if (kb < nk && !sing) {
+            double bv = -1.;
+            int bi = NP;
+#pragma unroll
+            for (int s = 0; s < NR; s++) {
+              const double v = fr[s] ? fabs(p[s][kb]) : -1.;
+              if (v > bv) { bv = v; bi = lane + 64 * s; }
+            }
+            wave_argmax(bv, bi);
+            if (!(bv > tiny)) {
+              sing = true;
+            } else {
+              const int r = __builtin_amdgcn_readfirstlane(bi), rl = r & 63, rsl = r >> 6;
+              double rr[TR];
+#pragma unroll
+              for (int b = 0; b < TR; b++) {
+                double v = p[0][b];
+#pragma unroll
+                for (int s = 1; s < NR; s++) v = rsl == s ? p[s][b] : v;
+                rr[b] = readlane_d(v, rl);
+              }
+              double ra = __builtin_amdgcn_rcp(rr[kb]);   // two Newton steps on the hardware reciprocal
+              ra = fma(ra, fma(-rr[kb], ra, 1.0), ra);
+              ra = fma(ra, fma(-rr[kb], ra, 1.0), ra);
+#pragma unroll
+              for (int s = 0; s < NR; s++) {
+                const bool isp = lane + 64 * s == r;
+                const double c = p[s][kb] * ra;
+#pragma unroll
+                for (int b = 0; b < TR; b++)
+                  if (b != kb) p[s][b] = isp ? -rr[b] * ra : fma(-c, rr[b], p[s][b]);
+                p[s][kb] = isp ? ra : c;
+                fr[s] = fr[s] && !isp;
+              }
+              if (lane == 0) { s_piv[k0 + kb] = r; s_step[r] = k0 + kb; }
+            }
+          }
+        }
+#pragma unroll
+        for (int s = 0; s < NR; s++) {
+          const int row = lane + 64 * s;
+          if (row < NP)
+#pragma unroll
+            for (int b = 0; b < TR; b++) s_pan[buf][b][row] = p[s][b];
+        }
+        if (sing && lane == 0) s_flag = 1;
       }
-      __syncthreads();
-      double bv = -1.;
-      int r = NP;
-      for (int t = 0; t < G; t++) {
-        const double v = s_kv[buf][t];
-        const int ii = s_ki[buf][t];
-        if (v > bv || (v == bv && ii < r)) { bv = v; r = ii; }
+      __syncthreads();                 // factored panel, pivots and flag published
+      // the pivot rows outside the panel's columns; -delta at the panel's columns
+      int q[TR];
+#pragma unroll
+      for (int a = 0; a < TR; a++) {
+        q[a] = s_step[i0 + a] - k0;    // in [0, nk) for a pivot row of this block
+        if (tx != kt && q[a] >= 0 && q[a] < nk)
+#pragma unroll
+          for (int b = 0; b < TR; b++) s_a12[buf][q[a]][j0 + b] = m[a][b];
       }
-      if (!(bv > tiny)) { singular = true; break; }   // the same decision in every thread
-      const int ra_ = r - i0;          // row of the pivot inside this tile (valid when 0 <= ra_ < TR)
+      if (tid < TR * TR && tid / TR < nk) s_a12[buf][tid / TR][k0 + tid % TR] = tid / TR == tid % TR ? -1. : 0.;
+      for (int t = nk * NP + tid; t < TR * NP; t += NT) s_a12[buf][t / NP][t % NP] = 0.;
+      __syncthreads();                 // the pivot rows published
 #pragma unroll
-      for (int a = 0; a < TR; a++)
-        if (a == ra_)
+      for (int a = 0; a < TR; a++) {
+        const double keep = (tx == kt || (q[a] >= 0 && q[a] < nk)) ? 0. : 1.;
 #pragma unroll
-          for (int b = 0; b < TR; b++) s_row[buf][j0 + b] = m[a][b];
-      if (tx == kt)
+        for (int b = 0; b < TR; b++) m[a][b] *= keep;
+      }
 #pragma unroll
-        for (int a = 0; a < TR; a++) {
-          double v = 0.;
+      for (int c = 0; c < TR; c++) {
+        double lc[TR], ar[TR];
 #pragma unroll
-          for (int b = 0; b < TR; b++) v = (b == kb) ? m[a][b] : v;
-          s_col[buf][i0 + a] = v;
-        }
-      if (tid == 0) { s_prow[k] = r; s_rowstep[r] = k; }
-      __syncthreads();
-      const double ra = 1.0 / s_row[buf][k];
-      double cr[TR], rr[TR];
+        for (int a = 0; a < TR; a++) lc[a] = s_pan[buf][c][i0 + a];
 #pragma unroll
-      for (int a = 0; a < TR; a++) cr[a] = s_col[buf][i0 + a] * ra;
+        for (int b = 0; b < TR; b++) ar[b] = s_a12[buf][c][j0 + b];
 #pragma unroll
-      for (int b = 0; b < TR; b++) rr[b] = s_row[buf][j0 + b];
+        for (int a = 0; a < TR; a++)
 #pragma unroll
-      for (int a = 0; a < TR; a++)
-#pragma unroll
-        for (int b = 0; b < TR; b++) {
-          const bool jc = (j0 + b == k);
-          if (a == ra_) m[a][b] = jc ? ra : -rr[b] * ra;
-          else m[a][b] = jc ? cr[a] : fma(-cr[a], rr[b], m[a][b]);
-        }
-      if (ra_ >= 0 && ra_ < TR) used |= 1u << ra_;
+          for (int b = 0; b < TR; b++) m[a][b] = fma(-lc[a], ar[b], m[a][b]);
+      }
     }
-    if (singular) {
+    if (s_flag) {
       if (tid == 0) atomicAdd(status, 1);
       for (int t = tid; t < NL * NL; t += NT) {
         if (vinv) vinv[base + t] = 0.0;
@@ -137,15 +261,14 @@ __global__ __launch_bounds__(G * G) void k_vanka_invert(int nel, const unsigned 
       }
       continue;
     }
-    __syncthreads();
-    // transposed store: (K_e^-1)[rowstep[i]][prow[j]] = M[i][j] at prow[j] * NL + rowstep[i]
+    // transposed store: (K_e^-1)[step[i]][piv[j]] = M[i][j] at piv[j] * NL + step[i]
 #pragma unroll
     for (int a = 0; a < TR; a++)
 #pragma unroll
       for (int b = 0; b < TR; b++) {
         const int i = i0 + a, j = j0 + b;
         if (i < NL && j < NL) {
-          const int64_t o = base + (int64_t)s_prow[j] * NL + s_rowstep[i];
+          const int64_t o = base + (int64_t)s_piv[j] * NL + s_step[i];
           if (vinv) vinv[o] = m[a][b];
           if (vinvf) vinvf[o] = (float)m[a][b];
         }
