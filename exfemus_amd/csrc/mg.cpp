@@ -19,9 +19,9 @@
 
 int fm_vanka_setup(fm_system *s, int L, std::vector<struct LOps> *ops);
 int fm_vanka_sweep64(fm_system *s, int L, const struct LOps &o, double *x, const double *b, int its, double omega,
-                     int colored, bool xz);
+                     int colored, bool xz, bool post);
 int fm_vanka_sweep32(fm_system *s, int L, const struct LOps &o, float *x, const float *b, int its, double omega,
-                     int colored, bool xz);
+                     int colored, bool xz, bool post);
 
 // Per-level operator set used by the V-cycle (forward or transposed).
 struct LOps {
@@ -161,7 +161,7 @@ static bool smoother_writes_x(const fm_mg_opts *opt, int its) {
 }
 
 template <typename T>
-static int smooth(fm_system *s, const fm_mg_opts *opt, int L, const LOps &o, int its, bool xz) {
+static int smooth(fm_system *s, const fm_mg_opts *opt, int L, const LOps &o, int its, bool xz, bool post) {
   hipStream_t st = s->ctx->stream;
   Level *l = o.v;
   T *x = Vec<T>::x(l), *b = Vec<T>::b(l), *r = Vec<T>::r(l), *d = Vec<T>::d(l), *d2 = Vec<T>::d2(l);
@@ -199,8 +199,8 @@ static int smooth(fm_system *s, const fm_mg_opts *opt, int L, const LOps &o, int
   }
   if (opt->smoother == FM_SMOOTH_VANKA) {
     if (sizeof(T) == 8)
-      return fm_vanka_sweep64(s, L, o, (double *)x, (const double *)b, its, opt->omega, opt->vanka_colored, xz);
-    return fm_vanka_sweep32(s, L, o, (float *)x, (const float *)b, its, opt->omega, opt->vanka_colored, xz);
+      return fm_vanka_sweep64(s, L, o, (double *)x, (const double *)b, its, opt->omega, opt->vanka_colored, xz, post);
+    return fm_vanka_sweep32(s, L, o, (float *)x, (const float *)b, its, opt->omega, opt->vanka_colored, xz, post);
   }
   return fm_einval(__func__);
 }
@@ -218,7 +218,7 @@ static int vcycle(fm_system *s, const fm_mg_opts *opt, std::vector<LOps> &ops, i
   }
   int top = (int)ops.size() - 1;
   const int mult = L < top ? (opt->coarse_mult > 0 ? opt->coarse_mult : 2) : 1;
-  FM_CHECK(smooth<T>(s, opt, L, o, opt->nu_pre * mult, xz));
+  FM_CHECK(smooth<T>(s, opt, L, o, opt->nu_pre * mult, xz, false));
   CsrView<T> A = cview<T>(*o.A);
   T *x = Vec<T>::x(l), *b = Vec<T>::b(l), *r = Vec<T>::r(l);
   FM_CHECK(fmk::resid(st, A, b, x, r));
@@ -250,7 +250,7 @@ static int vcycle(fm_system *s, const fm_mg_opts *opt, std::vector<LOps> &ops, i
     alpha = 0.95;
   }
   FM_CHECK(fmk::addmul(st, cview<T>(*o.P), Vec<T>::x(lc), x, (T)alpha, cond));
-  FM_CHECK(smooth<T>(s, opt, L, o, opt->nu_post * mult, false));
+  FM_CHECK(smooth<T>(s, opt, L, o, opt->nu_post * mult, false, true));
   return 0;
 }
 
@@ -754,9 +754,9 @@ extern "C" int fm_mg_solve(fm_system *s, const fm_mg_opts *o, fm_mg_report *rep)
 // ---- Vanka (vanka.hip) -----------------------------------------------------
 int fm_vanka_invert(fm_system *s, int L, const double *Av, double **store, float **storef);
 int fm_vanka_apply64(fm_system *s, int L, const CsrView<double> &A, const double *vinv, double *x, const double *b,
-                     double *r, int its, double omega, int colored, bool xz);
+                     double *r, int its, double omega, int colored, bool xz, bool post);
 int fm_vanka_apply32(fm_system *s, int L, const CsrView<float> &A, const float *vinv, float *x, const float *b,
-                     float *r, int its, double omega, int colored, bool xz);
+                     float *r, int its, double omega, int colored, bool xz, bool post);
 int fm_vanka_setup(fm_system *s, int L, std::vector<LOps> *ops) {
   LOps &o = (*ops)[L];
   int rc = fm_vanka_invert(s, L, o.A->sell.v, &o.v->vinv, &o.v->vinvf);
@@ -766,12 +766,12 @@ int fm_vanka_setup(fm_system *s, int L, std::vector<LOps> *ops) {
   return 0;
 }
 int fm_vanka_sweep64(fm_system *s, int L, const LOps &o, double *x, const double *b, int its, double omega, int colored,
-                     bool xz) {
-  return fm_vanka_apply64(s, L, cview<double>(*o.A), o.vinv, x, b, o.v->r, its, omega, colored, xz);
+                     bool xz, bool post) {
+  return fm_vanka_apply64(s, L, cview<double>(*o.A), o.vinv, x, b, o.v->r, its, omega, colored, xz, post);
 }
 int fm_vanka_sweep32(fm_system *s, int L, const LOps &o, float *x, const float *b, int its, double omega, int colored,
-                     bool xz) {
-  return fm_vanka_apply32(s, L, cview<float>(*o.A), o.vinvf, x, b, o.v->rf, its, omega, colored, xz);
+                     bool xz, bool post) {
+  return fm_vanka_apply32(s, L, cview<float>(*o.A), o.vinvf, x, b, o.v->rf, its, omega, colored, xz, post);
 }
 
 // Transposed multigrid: A^T lambda = g on the adjoint hierarchy (A^T from the
