@@ -106,7 +106,8 @@ __global__ __launch_bounds__(G * G) __attribute__((amdgpu_waves_per_eu(3))) void
   __shared__ __align__(16) double s_pan[2][TR][NP];   // the block's columns after its exchange steps
   __shared__ __align__(16) double s_a12[2][TR][NP];   // the block's pivot rows before them
   __shared__ double s_red[NW];
-  __shared__ int s_piv[NP], s_step[NP], s_flag;        // pivot row of step k, step of pivot row r, singular
+  __shared__ __align__(16) int s_step[NP];              // step of pivot row r
+  __shared__ int s_piv[NP], s_flag;                    // pivot row of step k, singular
   const int tid = threadIdx.x, tx = tid / G, ty = tid % G, lane = tid & 63, wid = tid >> 6;
   const int i0 = ty * TR, j0 = tx * TR;
   for (int e = xcd_block(); e < nel; e += gridDim.x) {
@@ -167,9 +168,14 @@ __global__ __launch_bounds__(G * G) __attribute__((amdgpu_waves_per_eu(3))) void
         for (int s = 0; s < NR; s++) {
           const int row = lane + 64 * s;
           const bool in = row < NP;
+          const int rc = in ? row : NP - 1;   // every read issued unconditionally, before any use
+          double v[TR];
 #pragma unroll
-          for (int b = 0; b < TR; b++) p[s][b] = in ? s_pan[buf][b][row] : 0.;
-          fr[s] = row < NL && s_step[row] < 0;
+          for (int b = 0; b < TR; b++) v[b] = s_pan[buf][b][rc];
+          const int st = s_step[rc];
+#pragma unroll
+          for (int b = 0; b < TR; b++) p[s][b] = in ? v[b] : 0.;
+          fr[s] = row < NL && st < 0;
         }
         bool sing = false;
 #pragma unroll
@@ -223,10 +229,21 @@ __global__ __launch_bounds__(G * G) __attribute__((amdgpu_waves_per_eu(3))) void
       }
       __syncthreads();                 // factored panel, pivots and flag published
       // the pivot rows outside the panel's columns; -delta at the panel's columns
-      int q[TR];
+      int q[TR];                       // q[a] in [0, nk): row i0 + a is the block's pivot row q
+      if constexpr (TR % 2 == 0) {     // all reads before the writes (8-byte pairs: i0 = ty TR is even)
+        const int2 *sp2 = reinterpret_cast<const int2 *>(s_step + i0);
+#pragma unroll
+        for (int a = 0; a < TR; a += 2) {
+          const int2 v = sp2[a / 2];
+          q[a] = v.x - k0;
+          q[a + 1] = v.y - k0;
+        }
+      } else {
+#pragma unroll
+        for (int a = 0; a < TR; a++) q[a] = s_step[i0 + a] - k0;
+      }
 #pragma unroll
       for (int a = 0; a < TR; a++) {
-        q[a] = s_step[i0 + a] - k0;    // in [0, nk) for a pivot row of this block
         if (tx != kt && q[a] >= 0 && q[a] < nk)
 #pragma unroll
           for (int b = 0; b < TR; b++) s_a12[buf][q[a]][j0 + b] = m[a][b];
