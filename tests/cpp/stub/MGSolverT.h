@@ -1,4 +1,4 @@
-// Minimal stand-in for the reference's MGSolT and the members the binding in INTEGRATION.md touches, with the
+// Minimal stand-in for the reference's MGSolT and the members the integration guide's binding touches, with the
 // reference's names and types (include/MGSolverBase.h:37-98, include/MGMesh.h:65-80,
 // template_appl/nse/SRC/MGSolverT.h:23-110, contrib/matrix/numeric_vectorM.h:160,239).  Test infrastructure: it
 // only lets tests/test_integration_binding.py compile and link the binding; it holds no FEMuS code.

@@ -1,7 +1,7 @@
-"""The reference-side binding of INTEGRATION.md (the MGSolT subclass a FEMuS maintainer would add) compiles and
-links against include/femus_hip.h and the built library, with a stand-in MGSolverT.h declaring the reference
-members it uses (tests/cpp/stub).  The block is taken from INTEGRATION.md itself, so the document cannot drift
-from what compiles."""
+"""The reference-side binding in the repository's integration guide (the MGSolT subclass a FEMuS maintainer would
+add) compiles and links against include/femus_hip.h and the built library, with a stand-in MGSolverT.h declaring
+the reference members it uses (tests/cpp/stub).  The code block is taken from the guide itself, so the document
+cannot drift from what compiles."""
 import os
 import re
 import subprocess
