@@ -125,3 +125,32 @@ def test_caller_matrix_device_pointer_and_errors(femus, ctx):
     with pytest.raises(femus.FemusError) as e:
         femus._ck(femus.lib().fm_system_set_matrix(s.h, 7, v.ctypes.data_as(C.c_void_p)))
     s.close()
+
+
+@pytest.mark.parametrize("case,dim,nx,nlev", [("channel3d", 3, [4, 2, 2], 2), ("cavity", 2, [2, 2], 3)])
+def test_singular_vanka_patch_reported(femus, oracle, ctx, case, dim, nx, nlev):
+    """A caller matrix with the row and the column of one DoF zeroed makes every element patch holding that DoF
+    singular: the patch inversion stops pivoting in such a patch, writes it as zeros and the set-up reports
+    FM_ESINGULAR with the level; once the assembled matrix is set again the same solve sets up and converges."""
+    pm, om, ns, bc, x0 = ns_setup(femus, oracle, ctx, case, dim, nx, nlev)
+    top = nlev - 1
+    for L in range(nlev):
+        ns.GenMatRhs(0.0, L, 1 if L == top else 0)
+    rp, ci, v = ns.csr(0, top)
+    n = len(rp) - 1
+    d = n // 2
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    vs = v.copy()
+    vs[(rows == d) | (ci == d)] = 0.0
+    ns.set_matrix(top, vs)
+    ns.set_x(np.zeros(n))
+    o = ns.opts(precision=femus.FM_PREC_FP64_FGMRES, smoother=femus.FM_SMOOTH_VANKA, tol=1e-9, max_iter=80)
+    with pytest.raises(femus.FemusError) as e:
+        ns.MGSolve(opts=o)
+    assert e.value.code == -4
+    assert "singular element patch" in str(e.value) and f"on level {top}" in str(e.value)
+    ns.set_matrix(top, v)
+    ns.set_x(np.zeros(n))
+    rep = ns.MGSolve(opts=o)
+    assert rep.converged == 1, (rep.iters, rep.rel_resid)
+    ns.close()
